@@ -1,0 +1,216 @@
+/*
+ * frehg_gpu.h -- C ABI of the MI355X (gfx950) backend for the Frehg2 coupled step.
+ *
+ * One context = one HIP device + one HIP stream + the device-resident state of one Richards
+ * (groundwater, "GW") solver, one semi-implicit shallow-water (surface, "SW") solver and their
+ * exchange. Fields use the reference's halo-padded layout (src/core/Grid.hpp:42-60):
+ *   3D index (i+1) + (j+1)(nx+2) + (k+1)(nx+2)(ny+2), k = 0 is the TOP layer, k = -1 / nz ghosts;
+ *   2D index (i+1) + (j+1)(nx+2).
+ * All arrays crossing this ABI are host pointers; the library copies them. Every call is
+ * synchronous with respect to the host pointers it receives. Calls on one context are not
+ * thread-safe; distinct contexts may be driven from distinct host threads.
+ *
+ * Return codes: 0 = ok, > 0 = informational (FREHG_INFO_*), < 0 = error (FREHG_ERR_*); the text
+ * of the last error is frehg_last_error(ctx). The reference raises std::runtime_error for the
+ * same conditions (src/linear/backends/PetscLinearSolver.cpp:206-211, src/re/ReSolver.cpp:71-74).
+ */
+#ifndef FREHG_GPU_H
+#define FREHG_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FREHG_ABI_VERSION 1
+
+enum {
+  FREHG_OK = 0,
+  FREHG_INFO_GUESS_FALLBACK = 1, /* learned initial guess rejected, zero guess used */
+  FREHG_ERR_INVALID_ARG = -1,
+  FREHG_ERR_OOM = -2,
+  FREHG_ERR_HIP = -3,
+  FREHG_ERR_NOT_CONVERGED = -4, /* ~ KSPConvergedReason < 0 */
+  FREHG_ERR_NONFINITE = -5,
+  FREHG_ERR_STATE = -6, /* call order violated (e.g. step before init) */
+  FREHG_ERR_NO_DEVICE = -7
+};
+
+/* Krylov method / preconditioner / precision selectors. */
+enum { FREHG_KSP_CG = 0 };
+enum { FREHG_PC_JACOBI = 0 };
+enum {
+  FREHG_PREC_FP64 = 0,      /* FP64 Jacobi-PCG (reference arithmetic) */
+  FREHG_PREC_MIXED = 1      /* FP32 Jacobi-PCG inside FP64 iterative refinement */
+};
+enum {
+  FREHG_GUESS_ZERO = 0,     /* x0 = 0 (PETSc default, the reference) */
+  FREHG_GUESS_PREVIOUS = 1, /* x0 = h^n */
+  FREHG_GUESS_LEARNED = 2   /* x0 = h^n + MLP(local stencil features), zero-guess fallback */
+};
+
+/* Soil class: include/frehg2/re/SoilParams.hpp:13-25. */
+typedef struct {
+  double alpha, n, theta_s, theta_r;
+  double Ks_x, Ks_y, Ks_z, Ss;
+  int32_t use_vg, use_mvg;
+  double air_entry;
+} frehg_soil_params;
+
+/* Krylov solve configuration: include/frehg2/linear/SolverConfig.hpp:14-20, plus precision. */
+typedef struct {
+  int32_t ksp_type;  /* FREHG_KSP_* */
+  int32_t pc_type;   /* FREHG_PC_* */
+  double rtol;       /* preconditioned residual 2-norm <= rtol * ||D^-1 b|| (PETSc CG default) */
+  double atol;
+  int32_t max_it;
+  int32_t precision;  /* FREHG_PREC_* */
+  double inner_rtol;  /* FP32 inner solve relative tolerance (mixed only) */
+  int32_t max_refine; /* FP64 refinement sweeps before falling back to FP64 PCG (mixed only) */
+} frehg_solver_config;
+
+/* Richards (PCA) step parameters: src/re/ReSolver.hpp:24-46 (+ solver, Picard and guess). */
+typedef struct {
+  frehg_soil_params soil;
+  double dx, dy, dz, botz;
+  double dt, dt_min, dt_max, co_max;
+  int32_t adaptive_dt, use_corrector, use_full3d, sim_shallowwater;
+  int32_t bc_type[6]; /* x-, x+, y-, y+, z+ (bottom), z- (top); bottom 3 = free drainage */
+  double htop, hbot, qtop, qbot;
+  frehg_solver_config solver;
+  int32_t picard_max; /* 1 = reference PCA predictor; > 1 = Picard outer iteration */
+  double picard_tol;  /* max |h^{m+1} - h^m| stopping test [m] */
+  int32_t initial_guess; /* FREHG_GUESS_* */
+} frehg_re_params;
+
+/* Shallow-water parameters: src/swe/SweSolver.hpp:28-39 (+ solver). */
+typedef struct {
+  double gravity, manning, min_depth, visc_x, visc_y, hD, hE, wtfh, dt, offset;
+  frehg_solver_config solver;
+} frehg_swe_params;
+
+/* SW<->GW exchange parameters: src/coupling/Coupling.hpp:26-31. */
+typedef struct {
+  double dx, dy, visc, min_depth;
+} frehg_coupling_params;
+
+typedef struct {
+  int32_t iterations;   /* Krylov iterations (inner FP32 + FP64 for mixed) */
+  int32_t refinements;  /* FP64 refinement sweeps (mixed) */
+  double residual_norm; /* final ||D^-1 r|| */
+  double rhs_norm;      /* ||D^-1 b|| */
+  int32_t converged;
+  int32_t fallback;     /* mixed solve fell back to FP64 PCG */
+} frehg_solve_stats;
+
+typedef struct {
+  frehg_solve_stats solve; /* accumulated over Picard iterations */
+  double dt_used, dt_next;
+  int32_t picard_iterations;
+  int32_t guess_fallback;
+  double max_dh;           /* max |h^{n+1} - h^n| over interior cells */
+} frehg_re_stats;
+
+typedef struct {
+  frehg_solve_stats sw;
+  frehg_re_stats gw;       /* last GW substep; gw.solve.iterations summed over substeps */
+  int32_t gw_substeps;
+  double exchange_volume;  /* sum(q*dt) of this step (+ = net GW->SW seepage) */
+  double t, gw_time;
+} frehg_coupled_stats;
+
+typedef struct frehg_ctx_s* frehg_ctx_t;
+
+/* ---- context -------------------------------------------------------------------------------- */
+int frehg_abi_version(void);
+/* Create a context on HIP device `device`. Fails with FREHG_ERR_NO_DEVICE without a GPU. */
+int frehg_create(frehg_ctx_t* out, int device);
+void frehg_destroy(frehg_ctx_t ctx);
+const char* frehg_last_error(frehg_ctx_t ctx);
+int frehg_synchronize(frehg_ctx_t ctx);
+
+/* ---- Richards solver: replaces ReSolver (src/re/ReSolver.hpp:48-105) ------------------------ */
+/* ReSolver(Grid) + setParams + initializeGeometry (uniform dz, all cells active). */
+int frehg_re_init(frehg_ctx_t ctx, int nx, int ny, int nz, const frehg_re_params* p);
+/* ReSolver::setParams (ReSolver.hpp:53). Grid size is fixed at init. */
+int frehg_re_set_params(frehg_ctx_t ctx, const frehg_re_params* p);
+/* ReSolver::setSoilMap (ReSolver.hpp:58) with SoilMap::setClasses/setClassIndex{,3D}
+ * (src/soil/SoilMap.hpp:33-46). class_idx: nx*ny (idx3d = 0) or nx*ny*nz (idx3d = 1), may be
+ * NULL (all class 0); ks_cell: nx*ny*nz isotropic per-cell Ks replacing the class Ks (one class
+ * per cell), may be NULL. Order i + j*nx (+ k*nx*ny). */
+int frehg_re_set_soil(frehg_ctx_t ctx, const frehg_soil_params* classes, int ncls,
+                      const int32_t* class_idx, int idx3d, const double* ks_cell);
+/* ReSolver::setTopFluxField (ReSolver.hpp:105), nx*ny values, i + j*nx. */
+int frehg_re_set_top_flux(frehg_ctx_t ctx, const double* q_interior);
+/* ReSolver::setInitialHead / setInitialWaterContent + finalizeInitialState (:155-178):
+ * interior values i + j*nx + k*nx*ny; exactly one of h / wc non-NULL. */
+int frehg_re_set_initial(frehg_ctx_t ctx, const double* h_interior, const double* wc_interior);
+/* Direct access to any GwFields array by name (src/re/GwFields.hpp:62-74), halo padded.
+ * Names: h hn wc wcn wch hwc ch Kx Ky Kz qx qy qz Gct Grhs (Gct/Grhs after the last assembly). */
+int frehg_re_set_field(frehg_ctx_t ctx, const char* name, const double* src);
+int frehg_re_get_field(frehg_ctx_t ctx, const char* name, double* dst);
+/* ReSolver::advanceStep (ReSolver.cpp:931-969). */
+int frehg_re_step(frehg_ctx_t ctx, frehg_re_stats* stats);
+/* ReSolver::dt / primeAdaptiveDt (ReSolver.hpp:86-98). */
+double frehg_re_dt(frehg_ctx_t ctx);
+int frehg_re_prime_dt(frehg_ctx_t ctx, double dt);
+
+/* ---- Shallow-water solver: replaces SweSolver (src/swe/SweSolver.hpp:41-94) ----------------- */
+int frehg_swe_init(frehg_ctx_t ctx, int nx, int ny, double dx, double dy, const frehg_swe_params* p);
+int frehg_swe_set_params(frehg_ctx_t ctx, const frehg_swe_params* p);
+/* SweSolver::setBathymetry (physical, i + j*nx) -> updateBottomFaces (:144-205). */
+int frehg_swe_set_bathymetry(frehg_ctx_t ctx, const double* bed_interior);
+/* SweSolver::initializeState (:926-934): eta = init + offset clamped to bed, u = v = 0. */
+int frehg_swe_initialize_state(frehg_ctx_t ctx, double init_eta);
+/* Any SweFields array by name (src/swe/SweFields.hpp:131-146), halo padded. */
+int frehg_swe_set_field(frehg_ctx_t ctx, const char* name, const double* src);
+int frehg_swe_get_field(frehg_ctx_t ctx, const char* name, double* dst);
+/* SweSolver::advanceStep(rain, evap) (:803-853). */
+int frehg_swe_step(frehg_ctx_t ctx, double rain, double evap, frehg_solve_stats* stats);
+/* SweSolver::maxCfl (:954-975). */
+int frehg_swe_max_cfl(frehg_ctx_t ctx, double* out);
+
+/* ---- Coupling: replaces Coupling::exchange (src/coupling/Coupling.cpp:145-164) -------------- */
+/* q_out (nx*ny, i + j*nx, m^3/s, + = seepage) may be NULL. */
+int frehg_couple_exchange(frehg_ctx_t ctx, const frehg_coupling_params* cp, double dt,
+                          double* volume_out, double* q_out);
+/* Orchestrator::step, coupled branch (src/driver/Orchestrator.cpp:1322-1336, 1243-1252):
+ * SW advance at `rain`, exchange over dt, GW substeps until gw_time >= t + dt. */
+int frehg_coupled_step(frehg_ctx_t ctx, const frehg_coupling_params* cp, double dt, double rain,
+                       double evap, double t_end, frehg_coupled_stats* stats);
+/* Water volume sum(max(0, eta-b)) dx dy + sum(theta) dx dy dz (Orchestrator.cpp:898-936),
+ * FP64, fixed-order reduction. */
+int frehg_water_volume(frehg_ctx_t ctx, double* surface, double* subsurface);
+
+/* ---- LinearSolver seam: replaces LinearSolver::solve (include/frehg2/linear/LinearSolver.hpp:18-33)
+ * for operators assembled by Decomp3D / Decomp2D (STAR stencil). The operator is given per
+ * owned row as a diagonal plus one coefficient per neighbour direction (x-, x+, y-, y+, z-, z+;
+ * 2D: the first four), exactly the values addRow would insert for those columns
+ * (src/re/ReSolver.cpp:576-610, src/swe/SweSolver.cpp:542-569). b_owned / x_owned use the
+ * reference's owned ordering: 3D z-fastest ((j*nx + i)*nz + k), 2D x-fastest (j*nx + i)
+ * (src/linear/DomainDecomposition.cpp:23-33,109-119). */
+int frehg_ls_solve(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, const double* diag,
+                   const double* const* offdiag, const double* b_owned, double* x_owned,
+                   const frehg_solver_config* cfg, frehg_solve_stats* stats);
+
+/* ---- measurement --------------------------------------------------------------------------- */
+/* Kernel-duration accounting with HIP events on the context's stream. kernel ids below. */
+enum {
+  FREHG_KT_GW_SPMV = 0,   /* FP64 PCG SpMV+dot (7-point) */
+  FREHG_KT_GW_SPMV32 = 1, /* FP32 PCG SpMV+dot (7-point, mixed) */
+  FREHG_KT_GW_UPDATE = 2, /* PCG vector update + reductions */
+  FREHG_KT_GW_ASSEMBLE = 3,
+  FREHG_KT_GW_CORRECT = 4,
+  FREHG_KT_COUNT = 5
+};
+int frehg_profile_enable(frehg_ctx_t ctx, int enable);
+/* Total milliseconds and launch count recorded for kernel `kt` since the last reset. */
+int frehg_profile_read(frehg_ctx_t ctx, int kt, double* total_ms, int64_t* launches);
+int frehg_profile_reset(frehg_ctx_t ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FREHG_GPU_H */

@@ -1,0 +1,71 @@
+"""Shared problem set-ups for the tests (parameters follow the reference's own test files)."""
+import numpy as np
+
+from frehg_amd import abi
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)), "golden")
+
+
+def box3d_soil(ks=(1.0e-5, 1.0e-5, 1.0e-5), ss=0.0):
+    """tests/re/test_re_3d.cpp:39-51."""
+    return abi.soil_params(alpha=2.0, n=1.6, theta_s=0.45, theta_r=0.05, Ks_x=ks[0], Ks_y=ks[1],
+                           Ks_z=ks[2], Ss=ss)
+
+
+def box3d_params(soil, full3d=True, bc=(0, 0, 0, 0, 0, 0), rtol=1e-13, **kw):
+    """tests/re/test_re_3d.cpp:58-70: 4x4x8, dx=2, dy=1, dz=0.05, dt=5, fixed dt, closed box."""
+    return abi.re_params(soil=soil, solver=abi.solver_config(rtol), bc_type=bc, dx=2.0, dy=1.0,
+                         dz=0.05, botz=-0.4, dt=5.0, adaptive_dt=0, use_full3d=int(full3d), **kw)
+
+
+def box3d_wc(nx=4, ny=4, nz=8):
+    """tests/re/test_re_3d.cpp:55-57 (i + j*nx + k*nx*ny order)."""
+    i = np.arange(nx)[None, None, :]
+    j = np.arange(ny)[None, :, None]
+    wc = 0.20 + 0.03 * np.sin(0.9 * i) + 0.02 * np.cos(0.7 * j)
+    return np.broadcast_to(wc, (nz, ny, nx)).copy()
+
+
+def interior3(a, nx, ny, nz):
+    return a.reshape(nz + 2, ny + 2, nx + 2)[1:-1, 1:-1, 1:-1]
+
+
+def interior2(a, nx, ny):
+    return a.reshape(ny + 2, nx + 2)[1:-1, 1:-1]
+
+
+def b1_inputs():
+    """benchmarks/b1-sw inputs and tests/swe/b1_sw_runner.hpp:52-131 parameters."""
+    import os
+    bath = np.loadtxt(os.path.join(GOLDEN, "b1_sw", "bath"))
+    rain = np.loadtxt(os.path.join(GOLDEN, "b1_sw", "rain")).reshape(-1, 2)
+    ref = np.loadtxt(os.path.join(GOLDEN, "b1_sw", "frehg2_np1_depth_reference.txt"))
+    offset = -bath.min()
+    params = abi.swe_params(manning=0.019, visc_x=1e-6, visc_y=1e-6, hD=0.1, wtfh=1e-8,
+                            min_depth=1e-8, dt=5.0, offset=offset)
+    return bath, rain, ref, offset, params
+
+
+def interp_series(t, v, tc):
+    """Orchestrator.cpp:57-71 / b1_sw_runner.hpp:28-39."""
+    n = len(t)
+    if n == 0:
+        return 0.0
+    if tc == 0.0:
+        return v[0]
+    ind = 1
+    if ind < n and tc > t[ind]:
+        ind += 1
+        while ind < n and tc > t[ind]:
+            ind += 1
+    if ind >= n:
+        return v[n - 1]
+    return v[ind - 1] + (v[ind] - v[ind - 1]) * (tc - t[ind - 1]) / (t[ind] - t[ind - 1])
+
+
+def gaussian_bump_bed(nx=20, ny=20, dx=1.0, dy=1.0):
+    """tests/swe/test_swe_b0.cpp:43-51."""
+    x = (np.arange(nx) + 0.5) * dx
+    y = (np.arange(ny) + 0.5) * dy
+    X, Y = np.meshgrid(x, y)
+    return 0.5 * np.exp(-((X - 10.0) ** 2 + (Y - 10.0) ** 2) / 10.0)  # [j, i]
