@@ -1,0 +1,450 @@
+// C ABI of the gfx950 backend (include/frehg_gpu.h). Every entry point maps exceptions to the
+// return codes of the header; the reference raises std::runtime_error for the same conditions.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/frehg_gpu.h"
+#include "coupling.hpp"
+#include "mlp_impl.hpp"
+#include "re.hpp"
+#include "swe.hpp"
+
+struct frehg_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  frehg::Profiler prof;
+  std::unique_ptr<frehg::ReState> re;
+  std::unique_ptr<frehg::SweState> swe;
+  std::unique_ptr<frehg::Coupler> cpl;
+  std::unique_ptr<frehg::MlpGuess> mlp;
+  double t = 0.0, gw_time = 0.0;
+};
+
+namespace {
+
+template <class Fn>
+int guard(frehg_ctx_t ctx, Fn&& fn) {
+  if (!ctx) return FREHG_ERR_INVALID_ARG;
+  try {
+    FREHG_HIP(hipSetDevice(ctx->device));
+    return fn();
+  } catch (const frehg::ArgError& e) {
+    ctx->err = e.what();
+    return FREHG_ERR_INVALID_ARG;
+  } catch (const frehg::ConvergenceError& e) {
+    ctx->err = e.what();
+    return FREHG_ERR_NOT_CONVERGED;
+  } catch (const frehg::HipError& e) {
+    ctx->err = e.what();
+    return FREHG_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    ctx->err = "out of host memory";
+    return FREHG_ERR_OOM;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return FREHG_ERR_STATE;
+  }
+}
+
+frehg::ReState& need_re(frehg_ctx_t c) {
+  if (!c->re) throw std::runtime_error("groundwater solver not initialised (call frehg_re_init)");
+  return *c->re;
+}
+frehg::SweState& need_swe(frehg_ctx_t c) {
+  if (!c->swe) throw std::runtime_error("surface solver not initialised (call frehg_swe_init)");
+  return *c->swe;
+}
+frehg::Coupler& need_cpl(frehg_ctx_t c) {
+  if (!c->cpl) c->cpl.reset(new frehg::Coupler(c->stream));
+  return *c->cpl;
+}
+
+}  // namespace
+
+extern "C" {
+
+int frehg_abi_version(void) { return FREHG_ABI_VERSION; }
+
+int frehg_create(frehg_ctx_t* out, int device) {
+  if (!out) return FREHG_ERR_INVALID_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return FREHG_ERR_NO_DEVICE;
+  if (device < 0 || device >= n) return FREHG_ERR_INVALID_ARG;
+  frehg_ctx_t c = new (std::nothrow) frehg_ctx_s();
+  if (!c) return FREHG_ERR_OOM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return FREHG_ERR_HIP;
+  }
+  c->prof.stream = c->stream;
+  *out = c;
+  return FREHG_OK;
+}
+
+void frehg_destroy(frehg_ctx_t ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->mlp.reset();
+  ctx->cpl.reset();
+  ctx->re.reset();
+  ctx->swe.reset();
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* frehg_last_error(frehg_ctx_t ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int frehg_synchronize(frehg_ctx_t ctx) {
+  return guard(ctx, [&] {
+    FREHG_HIP(hipStreamSynchronize(ctx->stream));
+    return FREHG_OK;
+  });
+}
+
+int frehg_re_init(frehg_ctx_t ctx, int nx, int ny, int nz, const frehg_re_params* p) {
+  return guard(ctx, [&] {
+    if (!p) throw frehg::ArgError("frehg_re_init: null params");
+    ctx->re.reset();
+    ctx->re.reset(new frehg::ReState(nx, ny, nz, *p, ctx->stream, &ctx->prof));
+    if (ctx->mlp) ctx->re->set_mlp(ctx->mlp.get());
+    return FREHG_OK;
+  });
+}
+
+int frehg_re_set_params(frehg_ctx_t ctx, const frehg_re_params* p) {
+  return guard(ctx, [&] {
+    if (!p) throw frehg::ArgError("null params");
+    need_re(ctx).set_params(*p);
+    return FREHG_OK;
+  });
+}
+
+int frehg_re_set_soil(frehg_ctx_t ctx, const frehg_soil_params* classes, int ncls, const int32_t* class_idx,
+                      int idx3d, const double* ks_cell) {
+  return guard(ctx, [&] {
+    if (!classes) throw frehg::ArgError("null soil classes");
+    need_re(ctx).set_soil(classes, ncls, class_idx, idx3d, ks_cell);
+    return FREHG_OK;
+  });
+}
+
+int frehg_re_set_top_flux(frehg_ctx_t ctx, const double* q) {
+  return guard(ctx, [&] {
+    if (!q) throw frehg::ArgError("null top flux");
+    need_re(ctx).set_top_flux(q);
+    return FREHG_OK;
+  });
+}
+
+int frehg_re_set_initial(frehg_ctx_t ctx, const double* h, const double* wc) {
+  return guard(ctx, [&] {
+    need_re(ctx).set_initial(h, wc);
+    return FREHG_OK;
+  });
+}
+
+int frehg_re_set_field(frehg_ctx_t ctx, const char* name, const double* src) {
+  return guard(ctx, [&] {
+    if (!name || !src) throw frehg::ArgError("null argument");
+    need_re(ctx).set_field(name, src);
+    return FREHG_OK;
+  });
+}
+
+int frehg_re_get_field(frehg_ctx_t ctx, const char* name, double* dst) {
+  return guard(ctx, [&] {
+    if (!name || !dst) throw frehg::ArgError("null argument");
+    need_re(ctx).get_field(name, dst);
+    return FREHG_OK;
+  });
+}
+
+int frehg_re_step(frehg_ctx_t ctx, frehg_re_stats* st) {
+  return guard(ctx, [&] {
+    frehg_re_stats local;
+    need_re(ctx).step(&local);
+    if (st) *st = local;
+    return local.guess_fallback ? FREHG_INFO_GUESS_FALLBACK : FREHG_OK;
+  });
+}
+
+double frehg_re_dt(frehg_ctx_t ctx) { return (ctx && ctx->re) ? ctx->re->dt() : 0.0; }
+
+int frehg_re_prime_dt(frehg_ctx_t ctx, double dt) {
+  return guard(ctx, [&] {
+    need_re(ctx).prime_dt(dt);
+    return FREHG_OK;
+  });
+}
+
+int frehg_swe_init(frehg_ctx_t ctx, int nx, int ny, double dx, double dy, const frehg_swe_params* p) {
+  return guard(ctx, [&] {
+    if (!p) throw frehg::ArgError("frehg_swe_init: null params");
+    ctx->swe.reset();
+    ctx->swe.reset(new frehg::SweState(nx, ny, dx, dy, *p, ctx->stream, &ctx->prof));
+    return FREHG_OK;
+  });
+}
+
+int frehg_swe_set_params(frehg_ctx_t ctx, const frehg_swe_params* p) {
+  return guard(ctx, [&] {
+    if (!p) throw frehg::ArgError("null params");
+    need_swe(ctx).set_params(*p);
+    return FREHG_OK;
+  });
+}
+
+int frehg_swe_set_bathymetry(frehg_ctx_t ctx, const double* bed) {
+  return guard(ctx, [&] {
+    if (!bed) throw frehg::ArgError("null bathymetry");
+    need_swe(ctx).set_bathymetry(bed);
+    return FREHG_OK;
+  });
+}
+
+int frehg_swe_initialize_state(frehg_ctx_t ctx, double init_eta) {
+  return guard(ctx, [&] {
+    need_swe(ctx).initialize_state(init_eta);
+    return FREHG_OK;
+  });
+}
+
+int frehg_swe_set_field(frehg_ctx_t ctx, const char* name, const double* src) {
+  return guard(ctx, [&] {
+    if (!name || !src) throw frehg::ArgError("null argument");
+    need_swe(ctx).set_field(name, src);
+    return FREHG_OK;
+  });
+}
+
+int frehg_swe_get_field(frehg_ctx_t ctx, const char* name, double* dst) {
+  return guard(ctx, [&] {
+    if (!name || !dst) throw frehg::ArgError("null argument");
+    need_swe(ctx).get_field(name, dst);
+    return FREHG_OK;
+  });
+}
+
+int frehg_swe_step(frehg_ctx_t ctx, double rain, double evap, frehg_solve_stats* st) {
+  return guard(ctx, [&] {
+    frehg_solve_stats local;
+    need_swe(ctx).step(rain, evap, &local);
+    if (st) *st = local;
+    return FREHG_OK;
+  });
+}
+
+int frehg_swe_max_cfl(frehg_ctx_t ctx, double* out) {
+  return guard(ctx, [&] {
+    if (!out) throw frehg::ArgError("null output");
+    *out = need_swe(ctx).max_cfl();
+    return FREHG_OK;
+  });
+}
+
+int frehg_couple_exchange(frehg_ctx_t ctx, const frehg_coupling_params* cp, double dt, double* volume_out,
+                          double* q_out) {
+  return guard(ctx, [&] {
+    if (!cp) throw frehg::ArgError("null coupling params");
+    const double v = need_cpl(ctx).exchange(need_swe(ctx), need_re(ctx), *cp, dt, q_out);
+    if (volume_out) *volume_out = v;
+    return FREHG_OK;
+  });
+}
+
+int frehg_coupled_step(frehg_ctx_t ctx, const frehg_coupling_params* cp, double dt, double rain, double evap,
+                       double t_end, frehg_coupled_stats* st) {
+  return guard(ctx, [&] {
+    if (!cp) throw frehg::ArgError("null coupling params");
+    frehg_coupled_stats local;
+    std::memset(&local, 0, sizeof(local));
+    auto& swe = need_swe(ctx);
+    auto& re = need_re(ctx);
+    const double t_new = ctx->t + dt;  // Orchestrator::step, coupled branch (:1322-1336)
+    swe.step(rain, evap, &local.sw);
+    local.exchange_volume = need_cpl(ctx).exchange(swe, re, *cp, dt, nullptr);
+    const double eps = 1.0e-9 * std::max(1.0, t_end);  // gwCatchUpTo (:1243-1252)
+    int fallback = 0, iters = 0;
+    while (ctx->gw_time < t_new - eps) {
+      re.step(&local.gw);
+      iters += local.gw.solve.iterations;
+      fallback |= local.gw.guess_fallback;
+      ctx->gw_time += re.dt_used();
+      ++local.gw_substeps;
+    }
+    local.gw.solve.iterations = iters;
+    ctx->t = t_new;
+    local.t = ctx->t;
+    local.gw_time = ctx->gw_time;
+    if (st) *st = local;
+    return fallback ? FREHG_INFO_GUESS_FALLBACK : FREHG_OK;
+  });
+}
+
+int frehg_water_volume(frehg_ctx_t ctx, double* surface, double* subsurface) {
+  return guard(ctx, [&] {
+    if (!ctx->swe && !ctx->re) throw std::runtime_error("no solver initialised");
+    need_cpl(ctx).volume(ctx->swe.get(), ctx->re.get(), surface, subsurface);
+    return FREHG_OK;
+  });
+}
+
+int frehg_ls_solve(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, const double* diag,
+                   const double* const* offdiag, const double* b_owned, double* x_owned,
+                   const frehg_solver_config* cfg, frehg_solve_stats* st) {
+  return guard(ctx, [&] {
+    if (ndim != 2 && ndim != 3) throw frehg::ArgError("frehg_ls_solve: ndim must be 2 or 3");
+    if (nx < 1 || ny < 1 || (ndim == 3 && nz < 1)) throw frehg::ArgError("frehg_ls_solve: bad sizes");
+    if (!diag || !offdiag || !b_owned || !x_owned || !cfg) throw frehg::ArgError("frehg_ls_solve: null argument");
+    const bool three = ndim == 3;
+    const int NZ = three ? nz : 1;
+    const frehg::Geo g = frehg::make_geo(nx, ny, NZ);
+    const long long n = three ? g.ncell() : g.nsurf();
+    const int ndir = three ? 6 : 4;
+    for (int d = 0; d < ndir; ++d)
+      if (!offdiag[d]) throw frehg::ArgError("frehg_ls_solve: null off-diagonal");
+    std::vector<double> D((size_t)n, 0.0), ax((size_t)n, 0.0), ay((size_t)n, 0.0), az((size_t)n, 0.0),
+        b((size_t)n, 0.0), bn((size_t)n, 0.0), x((size_t)n, 0.0);
+    std::vector<unsigned char> fixed((size_t)n, 0);
+    std::vector<double> off[6];
+    for (int d = 0; d < 6; ++d) off[d].assign((size_t)n, 0.0);
+    auto row = [&](int i, int j, int k) -> size_t {
+      return three ? ((size_t)j * nx + i) * nz + k : (size_t)j * nx + i;
+    };
+    auto pidx = [&](int i, int j, int k) -> long long { return three ? g.idx(i, j, k) : g.sidx(i, j); };
+    const long long step[6] = {-1, 1, -g.sy, g.sy, -g.sz, g.sz};
+    for (int k = 0; k < NZ; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const size_t r = row(i, j, k);
+          const long long c = pidx(i, j, k);
+          D[(size_t)c] = diag[r];
+          b[(size_t)c] = bn[(size_t)c] = b_owned[r];
+          const bool inside[6] = {i > 0, i < nx - 1, j > 0, j < ny - 1, three && k > 0, three && k < nz - 1};
+          bool all_zero = true;
+          for (int d = 0; d < ndir; ++d) {
+            off[d][(size_t)c] = inside[d] ? offdiag[d][r] : 0.0;
+            if (off[d][(size_t)c] != 0.0) all_zero = false;
+          }
+          if (!(D[(size_t)c] != 0.0)) throw frehg::ArgError("frehg_ls_solve: zero diagonal (Jacobi needs D != 0)");
+          if (!three && all_zero) {
+            fixed[(size_t)c] = 1;
+            x[(size_t)c] = b[(size_t)c] / D[(size_t)c];
+          }
+        }
+    // Fold decoupled rows into their neighbours; the remaining couplings must be symmetric.
+    for (int k = 0; k < NZ; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const long long c = pidx(i, j, k);
+          if (fixed[(size_t)c]) continue;
+          for (int d = 0; d < ndir; ++d) {
+            const double a = off[d][(size_t)c];
+            if (a == 0.0) continue;
+            const long long nb = c + step[d];
+            if (fixed[(size_t)nb]) b[(size_t)c] -= a * x[(size_t)nb];
+          }
+          const double* pairs[3] = {&ax[(size_t)c], &ay[(size_t)c], &az[(size_t)c]};
+          (void)pairs;
+          for (int d = 1; d < ndir; d += 2) {
+            const long long nb = c + step[d];
+            const double a = off[d][(size_t)c];
+            if (a == 0.0 || fixed[(size_t)nb]) continue;
+            const double back = off[d - 1][(size_t)nb];
+            if (std::fabs(a - back) > 1e-12 * (std::fabs(a) + std::fabs(back)))
+              throw frehg::ArgError("frehg_ls_solve: operator is not symmetric (CG requires A = A^T)");
+            (d == 1 ? ax : d == 3 ? ay : az)[(size_t)c] = a;
+          }
+        }
+    auto& s = ctx->stream;
+    double *dD, *dax, *day, *daz = nullptr, *db, *dbn, *dx;
+    unsigned char* dfixed = nullptr;
+    const size_t bytes = (size_t)n * sizeof(double);
+    std::vector<void*> allocs;
+    auto alloc = [&](double** p) {
+      FREHG_HIP(hipMalloc(p, bytes));
+      allocs.push_back(*p);
+    };
+    struct Free {
+      std::vector<void*>& a;
+      ~Free() {
+        for (void* p : a) (void)hipFree(p);
+      }
+    } freer{allocs};
+    alloc(&dD);
+    alloc(&dax);
+    alloc(&day);
+    if (three) alloc(&daz);
+    alloc(&db);
+    alloc(&dbn);
+    alloc(&dx);
+    FREHG_HIP(hipMemcpy(dD, D.data(), bytes, hipMemcpyHostToDevice));
+    FREHG_HIP(hipMemcpy(dax, ax.data(), bytes, hipMemcpyHostToDevice));
+    FREHG_HIP(hipMemcpy(day, ay.data(), bytes, hipMemcpyHostToDevice));
+    if (three) FREHG_HIP(hipMemcpy(daz, az.data(), bytes, hipMemcpyHostToDevice));
+    FREHG_HIP(hipMemcpy(db, b.data(), bytes, hipMemcpyHostToDevice));
+    FREHG_HIP(hipMemcpy(dbn, bn.data(), bytes, hipMemcpyHostToDevice));
+    FREHG_HIP(hipMemcpy(dx, x.data(), bytes, hipMemcpyHostToDevice));
+    if (!three) {
+      FREHG_HIP(hipMalloc(&dfixed, (size_t)n));
+      allocs.push_back(dfixed);
+      FREHG_HIP(hipMemcpy(dfixed, fixed.data(), (size_t)n, hipMemcpyHostToDevice));
+    }
+    frehg::Krylov kry(g, three, s, &ctx->prof);
+    frehg::KrylovConfig kc;
+    kc.rtol = cfg->rtol;
+    kc.atol = cfg->atol;
+    kc.max_it = cfg->max_it;
+    kc.precision = 0;
+    frehg::StencilView<double> A{dD, dax, day, daz};
+    const frehg::SolveResult sr = kry.solve(A, nullptr, db, dbn, dx, dfixed, true, false, kc);
+    FREHG_HIP(hipMemcpy(x.data(), dx, bytes, hipMemcpyDeviceToHost));
+    for (int k = 0; k < NZ; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) x_owned[row(i, j, k)] = x[(size_t)pidx(i, j, k)];
+    if (st) {
+      std::memset(st, 0, sizeof(*st));
+      st->iterations = sr.iterations;
+      st->residual_norm = sr.residual_norm;
+      st->rhs_norm = sr.rhs_norm;
+      st->converged = sr.converged ? 1 : 0;
+    }
+    return FREHG_OK;
+  });
+}
+
+int frehg_profile_enable(frehg_ctx_t ctx, int enable) {
+  return guard(ctx, [&] {
+    ctx->prof.enabled = enable != 0;
+    return FREHG_OK;
+  });
+}
+
+int frehg_profile_read(frehg_ctx_t ctx, int kt, double* total_ms, int64_t* launches) {
+  return guard(ctx, [&] {
+    if (kt < 0 || kt >= FREHG_KT_COUNT) throw frehg::ArgError("bad kernel type");
+    long long n = 0;
+    double ms = 0.0;
+    ctx->prof.read(kt, &ms, &n);
+    if (total_ms) *total_ms = ms;
+    if (launches) *launches = n;
+    return FREHG_OK;
+  });
+}
+
+int frehg_profile_reset(frehg_ctx_t ctx) {
+  return guard(ctx, [&] {
+    ctx->prof.reset();
+    return FREHG_OK;
+  });
+}
+
+}  // extern "C"

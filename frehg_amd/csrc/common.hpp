@@ -1,0 +1,284 @@
+// Shared device/host helpers of the gfx950 backend: error handling, wavefront/block reductions,
+// Van Genuchten constitutive relations and the per-cell soil view.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace frehg {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ConvergenceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define FREHG_HIP(call)                                                                        \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      throw ::frehg::HipError(std::string(#call) + " failed: " + hipGetErrorString(e_) + " (" + \
+                              __FILE__ + ":" + std::to_string(__LINE__) + ")");                \
+  } while (0)
+
+#define FREHG_LAUNCH_CHECK() FREHG_HIP(hipGetLastError())
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------------------------
+// Reductions. Block sizes are multiples of 64; every partial sum is formed in a fixed order so
+// repeated runs are bit-identical.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+__device__ inline double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+// Sum over the block; result valid in every thread. `lds` holds >= blockDim/64 doubles.
+template <int NT>
+__device__ inline double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < NT / kWave; ++i) s += lds[i];
+  return s;
+}
+
+// Deterministic sum of n partials by one block of NT threads (strided, then tree).
+template <int NT>
+__device__ inline double block_sum_partials(const double* part, int n, double* lds) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) s += part[i];
+  return block_sum<NT>(s, lds);
+}
+
+// "Last block done" detection: every block stores its partials, fences, then bumps a counter;
+// the block that observes count == nblocks-1 owns the final reduction and resets the counter.
+__device__ inline bool last_block(unsigned int* counter, unsigned int nblocks, int* lds_flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int prev = atomicAdd(counter, 1u);
+    *lds_flag = (prev == nblocks - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  const bool last = *lds_flag != 0;
+  if (last) {
+    __threadfence();
+    if (threadIdx.x == 0) atomicExch(counter, 0u);
+  }
+  return last;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Soil class with derived constants (include/frehg2/re/SoilParams.hpp:13-25).
+struct SoilClass {
+  double alpha, n, theta_s, theta_r, Ks_x, Ks_y, Ks_z, Ss, air_entry;
+  int use_vg, use_mvg;
+  double m;      // 1 - 1/n
+  double wcm;    // VanGenuchten::wcm (VanGenuchten.hpp:21-28)
+  double capc;   // alpha * n * m * (wcm - theta_r)
+};
+
+inline SoilClass make_soil_class(double alpha, double n, double theta_s, double theta_r, double Ks_x,
+                                 double Ks_y, double Ks_z, double Ss, int use_vg, int use_mvg,
+                                 double air_entry);
+
+// Per-cell soil lookup on the halo-padded grid. cls / ks are padded arrays whose ghost entries
+// hold the clamped interior value (ReSolver::soilAt clamps ghost indices, ReSolver.cpp:87-96).
+struct SoilView {
+  const SoilClass* classes;
+  const int32_t* cls;  // nullptr -> every cell is class 0
+  const double* ks;    // nullptr -> Ks from the class; else isotropic per-cell Ks
+  __device__ inline const SoilClass& at(long long c) const { return classes[cls ? cls[c] : 0]; }
+  __device__ inline double Ksx(long long c) const { return ks ? ks[c] : at(c).Ks_x; }
+  __device__ inline double Ksy(long long c) const { return ks ? ks[c] : at(c).Ks_y; }
+  __device__ inline double Ksz(long long c) const { return ks ? ks[c] : at(c).Ks_z; }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Van Genuchten relations (src/re/VanGenuchten.hpp:18-137). The plain-VG path evaluates
+// p = |alpha h|^n and Se = (1+p)^-m once and derives theta, C and the relative conductivity
+// from them, using the identities |alpha h|^(n-1) = p/|alpha h|, (1+p)^-(m+1) = Se/(1+p) and
+// Se^(1/m) = 1/(1+p) (so 1 - Se^(1/m) = p/(1+p)): 3 pow per cell instead of the ~10 that the
+// separate functions take. Modified-VG and non-VG soils use the reference expressions.
+// kr is K/Ks: K(h, Ks) = Ks * kr(h) with kr capped at 1 and = 1 above the cut-off.
+struct VGState {
+  double theta, cap, kr;
+};
+
+__device__ inline double vg_theta_ref(const SoilClass& s, double h) {  // VanGenuchten.hpp:31-47
+  double wc;
+  if (s.use_vg) {
+    const double se = pow(1.0 + pow(fabs(s.alpha * h), s.n), -s.m);
+    wc = (h > s.air_entry) ? s.theta_s : s.theta_r + (s.wcm - s.theta_r) * se;
+  } else {
+    wc = s.theta_r + (s.theta_s - s.theta_r) * exp(0.1634 * h);
+  }
+  if (wc > s.theta_s) wc = s.theta_s;
+  if (wc < s.theta_r) wc = s.theta_r;
+  return wc;
+}
+
+__device__ inline double vg_kr_ref(const SoilClass& s, double h) {  // VanGenuchten.hpp:84-111 / Ks
+  if (!s.use_vg) return exp(0.1634 * h);
+  const double se = pow(1.0 + pow(fabs(s.alpha * h), s.n), -s.m);
+  double k;
+  if (s.use_mvg) {
+    const double nume =
+        1.0 - pow(1.0 - pow(se * (s.theta_s - s.theta_r) / (s.wcm - s.theta_r), 1.0 / s.m), s.m);
+    const double deno =
+        1.0 - pow(1.0 - pow((s.theta_s - s.theta_r) / (s.wcm - s.theta_r), 1.0 / s.m), s.m);
+    k = (deno == 0.0) ? 1.0 : sqrt(se) * (nume / deno) * (nume / deno);
+  } else {
+    const double t = 1.0 - pow(1.0 - pow(se, 1.0 / s.m), s.m);
+    k = sqrt(se) * t * t;
+  }
+  if (k > 1.0) k = 1.0;
+  if (s.use_mvg) {
+    if (h > s.air_entry) k = 1.0;
+  } else if (h > 0.0) {
+    k = 1.0;
+  }
+  return k;
+}
+
+__device__ inline double vg_cap_ref(const SoilClass& s, double h) {  // VanGenuchten.hpp:68-81
+  const double a = fabs(s.alpha * h);
+  double c = s.capc * pow(a, s.n - 1.0) / pow(1.0 + pow(a, s.n), s.m + 1.0);
+  if (s.use_mvg) {
+    if (h > s.air_entry) c = 0.0;
+  } else if (h > 0.0) {
+    c = 0.0;
+  }
+  return c;
+}
+
+__device__ inline VGState vg_state(const SoilClass& s, double h) {
+  VGState o;
+  if (!s.use_vg || s.use_mvg) {
+    o.theta = vg_theta_ref(s, h);
+    o.cap = vg_cap_ref(s, h);
+    o.kr = vg_kr_ref(s, h);
+    return o;
+  }
+  const double a = fabs(s.alpha * h);
+  const double p = (a == 0.0) ? 0.0 : pow(a, s.n);
+  const double an1 = (a == 0.0) ? 0.0 : p / a;
+  const double onep = 1.0 + p;
+  const double se = pow(onep, -s.m);
+  double th = (h > s.air_entry) ? s.theta_s : s.theta_r + (s.wcm - s.theta_r) * se;
+  if (th > s.theta_s) th = s.theta_s;
+  if (th < s.theta_r) th = s.theta_r;
+  o.theta = th;
+  o.cap = (h > 0.0) ? 0.0 : s.capc * an1 * se / onep;
+  const double t = 1.0 - pow(p / onep, s.m);
+  double kr = sqrt(se) * t * t;
+  if (kr > 1.0) kr = 1.0;
+  if (h > 0.0) kr = 1.0;
+  o.kr = kr;
+  return o;
+}
+
+__device__ inline double vg_head(const SoilClass& s, double wc) {  // VanGenuchten.hpp:50-65
+  if (s.use_vg) {
+    double th = wc;
+    if (th - s.theta_r < 1.0e-7) th = s.theta_r + 1.0e-7;
+    if (th < s.theta_s)
+      return -(1.0 / s.alpha) * pow(pow((s.wcm - s.theta_r) / (th - s.theta_r), 1.0 / s.m) - 1.0, 1.0 / s.n);
+    return 0.0;
+  }
+  return log((wc - s.theta_r) / (s.theta_s - s.theta_r)) / 0.1634;
+}
+
+__device__ inline double vg_dKdwc(const SoilClass& s, double wc, double Ks) {  // :114-136
+  double th = wc;
+  if (th > 0.9999 * s.theta_s && th < s.theta_s) th = 0.9999 * s.theta_s;
+  const double mm = s.m;
+  const double se = (th - s.theta_r) / (s.theta_s - s.theta_r);
+  if (s.use_mvg) {
+    const double c2 = (s.theta_s - s.theta_r) / (s.wcm - s.theta_r);
+    const double c1 = 1.0 / pow(1.0 - pow(1.0 - pow(c2, 1.0 / mm), mm), 2.0);
+    const double term0 = pow(1.0 - pow(c2 * se, 1.0 / mm), mm);
+    const double term1 = 0.5 * c1 * Ks * pow(se, -0.5) * (1.0 - term0) * (1.0 - term0);
+    const double term2 = 2.0 * c1 * Ks * c2 * pow(se, 0.5) * pow(c2 * se, 1.0 / mm - 1.0) * (1.0 - term0) *
+                         pow(1.0 - pow(c2 * se, 1.0 / mm), mm - 1.0);
+    return (term1 + term2) / (s.theta_s - s.theta_r);
+  }
+  const double term0 = pow(1.0 - pow(se, 1.0 / mm), mm);
+  const double term1 = 0.5 * Ks * pow(se, -0.5) * (1.0 - term0) * (1.0 - term0);
+  const double term2 = 2.0 * Ks * pow(se, (2.0 - mm) / (2.0 * mm)) * (1.0 - term0) *
+                       pow(1.0 - pow(se, 1.0 / mm), mm - 1.0);
+  return (term1 + term2) / (s.theta_s - s.theta_r);
+}
+
+inline SoilClass make_soil_class(double alpha, double n, double theta_s, double theta_r, double Ks_x,
+                                 double Ks_y, double Ks_z, double Ss, int use_vg, int use_mvg,
+                                 double air_entry) {
+  SoilClass s{};
+  s.alpha = alpha;
+  s.n = n;
+  s.theta_s = theta_s;
+  s.theta_r = theta_r;
+  s.Ks_x = Ks_x;
+  s.Ks_y = Ks_y;
+  s.Ks_z = Ks_z;
+  s.Ss = Ss;
+  s.use_vg = use_vg;
+  s.use_mvg = use_mvg;
+  s.air_entry = air_entry;
+  s.m = 1.0 - 1.0 / n;
+  s.wcm = use_mvg ? theta_r + (theta_s - theta_r) * std::pow(1.0 + std::pow(std::fabs(air_entry) * alpha, n), s.m)
+                  : theta_s;
+  s.capc = alpha * n * s.m * (s.wcm - theta_r);
+  return s;
+}
+
+// Halo-padded grid geometry (src/core/Grid.hpp:42-60).
+struct Geo {
+  int nx, ny, nz;
+  long long sy, sz;  // +1 in j, +1 in k
+  __host__ __device__ inline long long idx(int i, int j, int k) const {
+    return (long long)(i + 1) + (long long)(j + 1) * sy + (long long)(k + 1) * sz;
+  }
+  __host__ __device__ inline long long sidx(int i, int j) const {
+    return (long long)(i + 1) + (long long)(j + 1) * sy;
+  }
+  __host__ __device__ inline long long ncell() const { return sz * (nz + 2); }
+  __host__ __device__ inline long long nsurf() const { return sz; }
+};
+
+inline Geo make_geo(int nx, int ny, int nz) {
+  Geo g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.sy = nx + 2;
+  g.sz = (long long)(nx + 2) * (ny + 2);
+  return g;
+}
+
+inline unsigned int ceil_div(long long a, long long b) { return (unsigned int)((a + b - 1) / b); }
+
+}  // namespace frehg

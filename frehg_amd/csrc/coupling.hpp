@@ -1,0 +1,30 @@
+// SW <-> GW exchange and water-volume accounting on the GPU.
+#pragma once
+
+#include "re.hpp"
+#include "swe.hpp"
+
+namespace frehg {
+
+class Coupler {
+ public:
+  explicit Coupler(hipStream_t s);
+  ~Coupler();
+  // Coupling::exchange (src/coupling/Coupling.cpp:145-164); returns sum(q*dt).
+  double exchange(SweState& swe, ReState& re, const frehg_coupling_params& cp, double dt, double* q_out);
+  // Orchestrator::computeWaterVolume (src/driver/Orchestrator.cpp:898-936).
+  void volume(SweState* swe, ReState* re, double* surface, double* subsurface);
+
+ private:
+  void ensure(unsigned int nblocks, long long ncol);
+  hipStream_t s_;
+  double* part_ = nullptr;
+  unsigned int nblk_ = 0;
+  double* qdev_ = nullptr;
+  long long nq_ = 0;
+  double* dres_ = nullptr;
+  double* hres_ = nullptr;
+  unsigned int* counter_ = nullptr;
+};
+
+}  // namespace frehg

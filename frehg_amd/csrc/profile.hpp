@@ -1,0 +1,88 @@
+// HIP-event kernel timing on the context's stream (enabled on request; off by default).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace frehg {
+
+constexpr int kKernelTypes = 5;  // FREHG_KT_COUNT
+
+class Profiler {
+ public:
+  bool enabled = false;
+  hipStream_t stream = nullptr;
+
+  ~Profiler() {
+    for (auto& e : pool_) (void)hipEventDestroy(e);
+  }
+
+  // Bracket one launch of kernel type `kt` on `stream`.
+  void begin(int kt) {
+    if (!enabled) return;
+    pending_kt_ = kt;
+    pending_begin_ = take();
+    FREHG_HIP(hipEventRecord(pending_begin_, stream));
+  }
+  void end() {
+    if (!enabled || pending_kt_ < 0) return;
+    hipEvent_t e = take();
+    FREHG_HIP(hipEventRecord(e, stream));
+    recs_.push_back({pending_kt_, pending_begin_, e});
+    pending_kt_ = -1;
+    if (recs_.size() > 4096) fold();
+  }
+  // Sum recorded durations into the per-type totals (synchronises on the recorded events).
+  void fold() {
+    for (auto& r : recs_) {
+      FREHG_HIP(hipEventSynchronize(r.e1));
+      float ms = 0.f;
+      FREHG_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+      total_ms_[r.kt] += ms;
+      launches_[r.kt] += 1;
+      free_.push_back(r.e0);
+      free_.push_back(r.e1);
+    }
+    recs_.clear();
+  }
+  void read(int kt, double* ms, long long* n) {
+    fold();
+    *ms = total_ms_[kt];
+    *n = launches_[kt];
+  }
+  void reset() {
+    fold();
+    for (int i = 0; i < kKernelTypes; ++i) {
+      total_ms_[i] = 0.0;
+      launches_[i] = 0;
+    }
+  }
+
+ private:
+  struct Rec {
+    int kt;
+    hipEvent_t e0, e1;
+  };
+  hipEvent_t take() {
+    if (free_.empty()) {
+      hipEvent_t e;
+      FREHG_HIP(hipEventCreate(&e));
+      pool_.push_back(e);
+      return e;
+    }
+    hipEvent_t e = free_.back();
+    free_.pop_back();
+    return e;
+  }
+  std::vector<hipEvent_t> pool_, free_;
+  std::vector<Rec> recs_;
+  int pending_kt_ = -1;
+  hipEvent_t pending_begin_ = nullptr;
+  double total_ms_[kKernelTypes] = {0, 0, 0, 0, 0};
+  long long launches_[kKernelTypes] = {0, 0, 0, 0, 0};
+};
+
+}  // namespace frehg

@@ -1,0 +1,747 @@
+// Richards-equation PCA step on the GPU. Sequence and arithmetic follow
+// ReSolver::advanceStep (src/re/ReSolver.cpp:931-969); each kernel names the reference
+// functions it fuses. Lateral boundaries are closed (the capability gate rejects anything
+// else, src/driver/Orchestrator.cpp:155-163).
+//
+// Kernels per step (one thread per interior cell unless stated):
+//   pre        kr = K/Ks and C at h^n, incl. the face-adjacent ghost cells          (:199-215)
+//   assemble   face conductivities -> 7-point coefficients, Gct and Grhs          (:217-549)
+//   [Krylov]   Jacobi-PCG, x0 = 0 / h^n / learned                                   (:560-636)
+//   post       ghost heads, kr and C at h^{n+1}                                     (:638-669)
+//   flux       faces at h^{n+1}, Darcy fluxes qx qy qz                               (:671-743)
+//   correct    mass-conservative theta, wch/hwc, saturation flags                   (:762-786)
+//   realloc    monotone fixed point of the sequential saturation sweep              (:788-837)
+//   finalize   reallocation, clamp, moisture ghosts                                 (:788-878)
+//   dtadapt    flux-imbalance max and Courant min (adaptive dt only)                (:880-929)
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "mlp.hpp"
+#include "re.hpp"
+
+namespace frehg {
+
+namespace {
+
+constexpr int NT = 256;
+
+struct ReK {
+  Geo g;
+  SoilView soil;
+  double dx, dy, dz, dt, htop, hbot, qtop, qbot, co_max;
+  int full3d, sim_sw, bc4, bc5, use_corrector;
+  const double* qtop_field;  // surface padded or null
+};
+
+struct Cell {
+  int i, j, k;
+  long long c;
+  bool valid;
+};
+
+__device__ inline Cell cell_of(const Geo& g) {
+  Cell t;
+  t.i = blockIdx.x * 64 + (threadIdx.x & 63);
+  t.j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  t.k = blockIdx.z;
+  t.valid = t.i < g.nx && t.j < g.ny;
+  t.c = t.valid ? g.idx(t.i, t.j, t.k) : 0;
+  return t;
+}
+
+// Conductivity on the +x face of cell (i,j,k) at flat index c (i may be -1: the x- wall).
+// ReSolver::computeKFace, ReSolver.cpp:243-250 (main loop), :287-299 (vertical-only x- quirk),
+// :329-330 (3D walls), :365-379 (vertical-only lateral zeroing in unsaturated cells).
+__device__ inline double face_x(const ReK& P, const double* kr, const double* wref, long long c, int i) {
+  const Geo& g = P.g;
+  if (i < 0) return 0.0;
+  if (P.full3d) {
+    if (i >= g.nx - 1) return 0.0;
+    return 0.5 * (P.soil.Ksx(c + 1) * kr[c + 1] + P.soil.Ksx(c) * kr[c]);
+  }
+  double K;
+  if (i == 0) {
+    const double Kp = P.soil.Ksx(c) * kr[c - 1];
+    K = 0.5 * (Kp + Kp);
+  } else {
+    K = 0.5 * (P.soil.Ksx(c + 1) * kr[c + 1] + P.soil.Ksx(c) * kr[c]);
+  }
+  if (wref[c] < P.soil.at(c).theta_s) K = 0.0;
+  if (i + 1 <= g.nx - 1 && wref[c + 1] < P.soil.at(c + 1).theta_s) K = 0.0;
+  return K;
+}
+
+// +y face of cell (i,j,k): ReSolver.cpp:251-265, :301-310, :333-334, :365-379.
+__device__ inline double face_y(const ReK& P, const double* kr, const double* wref, long long c, int j) {
+  const Geo& g = P.g;
+  if (j < 0) return 0.0;
+  if (P.full3d) {
+    if (j >= g.ny - 1) return 0.0;
+    return 0.5 * (P.soil.Ksy(c + g.sy) * kr[c + g.sy] + P.soil.Ksy(c) * kr[c]);
+  }
+  double K;
+  if (j == 0) {
+    const double Kp = P.soil.Ksy(c) * kr[c - g.sy];
+    K = 0.5 * (Kp + Kp);
+  } else if (j == g.ny - 1) {
+    K = 0.0;
+  } else {
+    K = 0.5 * (P.soil.Ksy(c + g.sy) * kr[c + g.sy] + P.soil.Ksy(c) * kr[c]);
+  }
+  if (wref[c] < P.soil.at(c).theta_s) K = 0.0;
+  if (j + 1 <= g.ny - 1 && wref[c + g.sy] < P.soil.at(c + g.sy).theta_s) K = 0.0;
+  return K;
+}
+
+// +z (downward) face of cell k (k = -1: the top boundary face). ReSolver.cpp:272-284 (main),
+// :338-344 (bottom), :345-363 (top: Ks_z for Dirichlet even when htop < 0).
+__device__ inline double face_z(const ReK& P, const double* kr, long long c, int k) {
+  const Geo& g = P.g;
+  if (k < 0) {
+    if (P.sim_sw) return 0.0;
+    const long long ct = c + g.sz;
+    if (P.bc5 == 1) return P.soil.Ksz(ct);
+    if (P.bc5 == 0) return 0.0;
+    return P.soil.Ksz(ct) * kr[ct];
+  }
+  if (k < g.nz - 1) return 0.5 * (P.soil.Ksz(c + g.sz) * kr[c + g.sz] + P.soil.Ksz(c) * kr[c]);
+  return P.bc4 == 0 ? 0.0 : P.soil.Ksz(c) * kr[c];
+}
+
+// K/Ks and C at the interior cell and kr at its face-adjacent ghost cells (whose soil is the
+// cell's own: ghost indices are clamped). ReSolver::updateDiagnostics (ReSolver.cpp:199-215).
+__device__ inline void constitutive_with_ghosts(const ReK& P, const Cell& t, const double* h,
+                                                 double* kr, double* ch) {
+  const Geo& g = P.g;
+  const SoilClass& s = P.soil.at(t.c);
+  const VGState v = vg_state(s, h[t.c]);
+  kr[t.c] = v.kr;
+  ch[t.c] = v.cap;
+  const long long gh[6] = {t.i == 0 ? t.c - 1 : -1,         t.i == g.nx - 1 ? t.c + 1 : -1,
+                           t.j == 0 ? t.c - g.sy : -1,      t.j == g.ny - 1 ? t.c + g.sy : -1,
+                           t.k == 0 ? t.c - g.sz : -1,      t.k == g.nz - 1 ? t.c + g.sz : -1};
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+    if (gh[q] >= 0) kr[gh[q]] = vg_state(s, h[gh[q]]).kr;
+}
+
+__global__ __launch_bounds__(NT) void k_re_pre(ReK P, const double* hn, double* kr, double* ch) {
+  const Cell t = cell_of(P.g);
+  if (!t.valid) return;
+  constitutive_with_ghosts(P, t, hn, kr, ch);
+}
+
+// groundwaterMatCoeff + groundwaterRhs (ReSolver.cpp:397-549) as a symmetric stencil:
+// D = Gct, ax/ay/az = Gxp/Gyp/Gzp where the +neighbour is interior (Gxm(c) == Gxp(c-1) etc.).
+__global__ __launch_bounds__(NT) void k_re_assemble(ReK P, const double* kr, const double* ch,
+                                                     const double* wcn, const double* hn, double* D,
+                                                     double* ax, double* ay, double* az, double* rhs,
+                                                     float* D32, float* ax32, float* ay32, float* az32) {
+  const Cell t = cell_of(P.g);
+  if (!t.valid) return;
+  const Geo& g = P.g;
+  const long long c = t.c;
+  const double Kxp = face_x(P, kr, wcn, c, t.i), Kxm = face_x(P, kr, wcn, c - 1, t.i - 1);
+  const double Kyp = face_y(P, kr, wcn, c, t.j), Kym = face_y(P, kr, wcn, c - g.sy, t.j - 1);
+  const double Kzp = face_z(P, kr, c, t.k), Kzm = face_z(P, kr, c - g.sz, t.k - 1);
+  const SoilClass& s = P.soil.at(c);
+  const double dt = P.dt, dzk = P.dz;
+  const double Ax = P.dy * dzk, Ay = P.dx * dzk, V = P.dx * P.dy * dzk;
+  const double Gxp = -Kxp * dt * Ax / P.dx;
+  const double Gxm = -Kxm * dt * Ax / P.dx;
+  const double Gyp = -Kyp * dt * Ay / P.dy;
+  const double Gym = -Kym * dt * Ay / P.dy;
+  const double dzf_p = 0.5 * (dzk + dzk);
+  double Gzp = -Kzp * dt / (dzk * dzf_p);
+  Gzp *= V;
+  const double dzf_m = (t.k == 0) ? 0.5 * dzk : 0.5 * (dzk + dzk);
+  double Gzm = -Kzm * dt / (dzk * dzf_m);
+  Gzm *= V;
+  const double st = ch[c] + s.Ss * wcn[c] / s.theta_s;
+  double Gct = st;
+  Gct *= V;
+  Gct -= (Gxp + Gxm + Gyp + Gym);
+  if (!(t.k == g.nz - 1 && P.bc4 != 1)) Gct -= Gzp;
+  if (t.k == 0) {
+    if (!P.sim_sw && P.bc5 == 1) Gct -= Gzm;
+  } else {
+    Gct -= Gzm;
+  }
+  double r = st * hn[c] * V;
+  r -= dt * V * Kzp / dzk;
+  r += dt * V * Kzm / dzk;
+  if (t.i == g.nx - 1) r -= Gxp * hn[c + 1];
+  if (t.i == 0) r -= Gxm * hn[c - 1];
+  if (t.j == g.ny - 1) r -= Gyp * hn[c + g.sy];
+  if (t.j == 0) r -= Gym * hn[c - g.sy];
+  if (t.k == g.nz - 1) {
+    if (P.bc4 == 1) r += Gzp * hn[c + g.sz];
+    if (P.bc4 == 2) r += dt * V * (P.qbot + Kzp) / dzk;
+  }
+  if (t.k == 0 && !P.sim_sw && P.bc5 == 1) r -= Gzm * P.htop;
+  if (t.k == 0 && !P.sim_sw && P.bc5 == 2) {
+    const double qcol = P.qtop_field ? P.qtop_field[g.sidx(t.i, t.j)] : P.qtop;
+    r += -dt * V * (qcol + Kzm) / dzk;
+  }
+  const double axv = (t.i < g.nx - 1) ? Gxp : 0.0;
+  const double ayv = (t.j < g.ny - 1) ? Gyp : 0.0;
+  const double azv = (t.k < g.nz - 1) ? Gzp : 0.0;
+  D[c] = Gct;
+  ax[c] = axv;
+  ay[c] = ayv;
+  az[c] = azv;
+  rhs[c] = r;
+  if (D32) {
+    D32[c] = (float)Gct;
+    ax32[c] = (float)axv;
+    ay32[c] = (float)ayv;
+    az32[c] = (float)azv;
+  }
+}
+
+// enforceHeadBc (ReSolver.cpp:638-669) + kr and C at the solved head.
+__global__ __launch_bounds__(NT) void k_re_post(ReK P, double* h, double* kr, double* ch) {
+  const Cell t = cell_of(P.g);
+  if (!t.valid) return;
+  const Geo& g = P.g;
+  const double hc = h[t.c];
+  if (t.i == 0) h[t.c - 1] = hc;
+  if (t.i == g.nx - 1) h[t.c + 1] = hc;
+  if (t.j == 0) h[t.c - g.sy] = hc;
+  if (t.j == g.ny - 1) h[t.c + g.sy] = hc;
+  if (t.k == g.nz - 1) h[t.c + g.sz] = (P.bc4 == 1) ? P.hbot : hc;
+  if (t.k == 0) h[t.c - g.sz] = (!P.sim_sw && P.bc5 == 1) ? P.htop : hc;
+  // ghost values written above are re-read by this same thread only
+  constitutive_with_ghosts(P, t, h, kr, ch);
+}
+
+// Second computeKFace + groundwaterFlux with darcyFluxZ (ReSolver.cpp:947-948, 24-58, 671-743).
+__global__ __launch_bounds__(NT) void k_re_flux(ReK P, const double* h, const double* kr,
+                                                 const double* wcn, double* Kx, double* Ky, double* Kz,
+                                                 double* qx, double* qy, double* qz) {
+  const Cell t = cell_of(P.g);
+  if (!t.valid) return;
+  const Geo& g = P.g;
+  const long long c = t.c;
+  const double Kxp = face_x(P, kr, wcn, c, t.i);
+  const double Kyp = face_y(P, kr, wcn, c, t.j);
+  const double Kzp = face_z(P, kr, c, t.k);
+  Kx[c] = Kxp;
+  Ky[c] = Kyp;
+  Kz[c] = Kzp;
+  if (t.i == 0) Kx[c - 1] = 0.0;
+  if (t.i == g.nx - 1 && !P.full3d) Kx[c + 1] = 0.0;
+  if (t.j == 0) Ky[c - g.sy] = 0.0;
+  if (t.k == g.nz - 1) Kz[c + g.sz] = 0.0;
+  const double dzk = P.dz;
+  const double faceA = P.dx * P.dy;
+  if (P.full3d) {
+    qx[c] = (P.dy * dzk) * Kxp * (h[c + 1] - h[c]) / P.dx;
+    qy[c] = (P.dx * dzk) * Kyp * (h[c + g.sy] - h[c]) / P.dy;
+  } else {
+    qx[c] = 0.0;
+    qy[c] = 0.0;
+  }
+  double q;
+  if (t.k == g.nz - 1 && P.bc4 == 0) {
+    q = 0.0;
+  } else {
+    const double delta = (t.k == g.nz - 1) ? 0.5 * dzk : 0.5 * (dzk + dzk);
+    const double kc = P.soil.Ksz(c) * kr[c];
+    const double ks = P.soil.Ksz(c + g.sz) * kr[c + g.sz];
+    const double kface = 0.5 * (kc + ks);
+    q = faceA * kface * ((h[c + g.sz] - h[c]) / delta - 1.0);
+  }
+  if (t.k == g.nz - 1 && P.bc4 == 2) q = P.qbot * faceA;
+  qz[c] = q;
+  if (t.k == 0) {
+    Kz[c - g.sz] = face_z(P, kr, c - g.sz, -1);
+    double qt;
+    if (P.bc5 == 2) {
+      const double qcol = P.qtop_field ? P.qtop_field[g.sidx(t.i, t.j)] : P.qtop;
+      qt = qcol * faceA;
+    } else {
+      const double kface = (P.bc5 == 0) ? 0.0 : P.soil.Ksz(c);
+      qt = faceA * kface * ((h[c] - h[c - g.sz]) / (0.5 * dzk) - 1.0);
+    }
+    qz[c - g.sz] = qt;
+  }
+}
+
+// updateWaterContent (ReSolver.cpp:762-786) or theta(h) without the corrector (:953-961);
+// the diagnostics wch = theta(h), hwc = h(theta) of the second updateDiagnostics / the
+// reallocation (:825-826); saturation flags for the reallocation fixed point.
+__global__ __launch_bounds__(NT) void k_re_correct(ReK P, const double* h, const double* hn,
+                                                    const double* wcn, const double* qx, const double* qy,
+                                                    const double* qz, double* wc, double* wch, double* hwc,
+                                                    unsigned char* sat0, unsigned char* cand) {
+  const Cell t = cell_of(P.g);
+  if (!t.valid) return;
+  const Geo& g = P.g;
+  const long long c = t.c;
+  const SoilClass& s = P.soil.at(c);
+  double w;
+  const VGState v = vg_state(s, h[c]);
+  if (P.use_corrector) {
+    const double V = P.dx * P.dy * P.dz;
+    const double coeff = V * (1.0 + s.Ss * (h[c] - hn[c]) / s.theta_s);
+    const double dqx = P.dt * (qx[c] - qx[c - 1]);
+    const double dqy = P.dt * (qy[c] - qy[c - g.sy]);
+    const double dqz = P.dt * (qz[c] - qz[c - g.sz]);
+    w = (wcn[c] * V + dqx + dqy + dqz) / coeff;
+  } else {
+    w = v.theta;
+  }
+  wc[c] = w;
+  wch[c] = v.theta;
+  hwc[c] = vg_head(s, w);
+  const unsigned char s0 = (w >= s.theta_s) ? 1 : 0;
+  sat0[c] = s0;
+  cand[c] = (!s0 && v.theta >= s.theta_s) ? 1 : 0;
+  // Ghost neighbours keep the moisture ghosts of the previous step (ReSolver.cpp:855-878).
+  const long long gh[6] = {t.i == 0 ? c - 1 : -1,         t.i == g.nx - 1 ? c + 1 : -1,
+                           t.j == 0 ? c - g.sy : -1,      t.j == g.ny - 1 ? c + g.sy : -1,
+                           t.k == 0 ? c - g.sz : -1,      t.k == g.nz - 1 ? c + g.sz : -1};
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+    if (gh[q] >= 0) {
+      sat0[gh[q]] = (wcn[gh[q]] >= s.theta_s) ? 1 : 0;
+      cand[gh[q]] = 0;
+    }
+}
+
+// adjacentToSat (ReSolver.cpp:802-817) with `prev` holding the post-visit saturation state of
+// the cells the i->j->k sweep has already visited and `sat0` the pre-visit state of the rest.
+__device__ inline bool adjacent_sat(const ReK& P, const Cell& t, const double* Kx, const double* Ky,
+                                    const unsigned char* sat0, const unsigned char* prev) {
+  const Geo& g = P.g;
+  const long long c = t.c;
+  if (sat0[c + g.sz]) return true;
+  if (Kx[c] != 0.0 && sat0[c + 1]) return true;
+  if (Kx[c - 1] != 0.0 && prev[c - 1]) return true;
+  if (Ky[c] != 0.0 && sat0[c + g.sy]) return true;
+  if (Ky[c - g.sy] != 0.0 && prev[c - g.sy]) return true;
+  if (t.k == 0) return (!P.sim_sw && P.bc5 == 1 && P.htop >= 0.0);
+  return prev[c - g.sz] != 0;
+}
+
+// One Jacobi sweep of the monotone map whose unique fixed point is the sequential sweep's
+// saturation state: a candidate (unsaturated theta, saturated theta(h)) becomes saturated iff
+// it is adjacent to saturation, reading visited neighbours from `prev`.
+__global__ __launch_bounds__(NT) void k_re_realloc_iter(ReK P, const double* Kx, const double* Ky,
+                                                         const unsigned char* sat0, const unsigned char* cand,
+                                                         const unsigned char* prev, unsigned char* next,
+                                                         int* changed) {
+  const Cell t = cell_of(P.g);
+  if (!t.valid) return;
+  unsigned char v = sat0[t.c];
+  if (cand[t.c]) v = adjacent_sat(P, t, Kx, Ky, sat0, prev) ? 1 : 0;
+  next[t.c] = v;
+  if (v != prev[t.c]) atomicOr(changed, 1);
+}
+
+// reallocateWaterContent branches with the converged state + clampWaterContent +
+// enforceMoistureBc (ReSolver.cpp:819-878).
+__global__ __launch_bounds__(NT) void k_re_finalize(ReK P, const double* Kx, const double* Ky,
+                                                     const unsigned char* sat0, const unsigned char* fin,
+                                                     const double* wch, const double* hwc, double* wc,
+                                                     double* h) {
+  const Cell t = cell_of(P.g);
+  if (!t.valid) return;
+  const Geo& g = P.g;
+  const long long c = t.c;
+  const SoilClass& s = P.soil.at(c);
+  double w = wc[c];
+  if (sat0[c]) {
+    w = s.theta_s;
+  } else if (!adjacent_sat(P, t, Kx, Ky, sat0, fin)) {
+    if (w < 0.9999 * s.theta_s) h[c] = hwc[c];
+  } else {
+    w = wch[c];
+  }
+  if (w > s.theta_s) w = s.theta_s;
+  else if (w < s.theta_r) w = s.theta_r;
+  wc[c] = w;
+  if (t.i == 0) wc[c - 1] = w;
+  if (t.i == g.nx - 1) wc[c + 1] = w;
+  if (t.j == 0) wc[c - g.sy] = w;
+  if (t.j == g.ny - 1) wc[c + g.sy] = w;
+  if (t.k == g.nz - 1) wc[c + g.sz] = w;
+  if (t.k == 0) wc[c - g.sz] = w;
+}
+
+// adaptiveTimeStep reductions (ReSolver.cpp:891-913): max flux imbalance and min Courant dt
+// over non-top cells. max/min are exact, so the result is order-independent.
+__global__ __launch_bounds__(NT) void k_re_dtadapt(ReK P, const double* qx, const double* qy,
+                                                    const double* qz, const double* wc, double* part,
+                                                    unsigned int* counter, unsigned int nblocks,
+                                                    double* res) {
+  __shared__ double lds[NT / kWave];
+  __shared__ int flag;
+  const Cell t = cell_of(P.g);
+  double dqm = 0.0, dtc = 1.0e8;
+  if (t.valid && t.k > 0) {
+    const Geo& g = P.g;
+    const long long c = t.c;
+    const SoilClass& s = P.soil.at(c);
+    const double Ax = P.dy * P.dz, Ay = P.dx * P.dz, Az = P.dx * P.dy;
+    const double qin = qx[c] / Ax + qy[c] / Ay + qz[c] / Az;
+    const double qou = qx[c - 1] / Ax + qy[c - g.sy] / Ay + qz[c - g.sz] / Az;
+    dqm = fabs(qin - qou) * P.dt / P.dz;
+    if (wc[c] < s.theta_s) dtc = P.co_max * P.dz / vg_dKdwc(s, wc[c], P.soil.Ksz(c));
+  }
+  dqm = wave_max(dqm);
+  dtc = wave_min(dtc);
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  __shared__ double lmax[NT / kWave], lmin[NT / kWave];
+  if (lane == 0) {
+    lmax[w] = dqm;
+    lmin[w] = dtc;
+  }
+  __syncthreads();
+  const unsigned int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (threadIdx.x == 0) {
+    double a = lmax[0], b = lmin[0];
+    for (int q = 1; q < NT / kWave; ++q) {
+      a = fmax(a, lmax[q]);
+      b = fmin(b, lmin[q]);
+    }
+    part[bid] = a;
+    part[nblocks + bid] = b;
+  }
+  (void)lds;
+  if (last_block(counter, nblocks, &flag)) {
+    if (threadIdx.x == 0) {
+      double a = 0.0, b = 1.0e8;
+      for (unsigned int q = 0; q < nblocks; ++q) {
+        a = fmax(a, part[q]);
+        b = fmin(b, part[nblocks + q]);
+      }
+      res[0] = a;
+      res[1] = b;
+    }
+  }
+}
+
+// setInitialHead / setInitialWaterContent (ReSolver.cpp:135-173) + finalizeInitialState
+// diagnostics (:175-178) on interior cells.
+__global__ __launch_bounds__(NT) void k_re_initial(ReK P, int from_head, double* h, double* wc, double* hn,
+                                                    double* wcn, double* wch, double* hwc, double* ch) {
+  const Cell t = cell_of(P.g);
+  if (!t.valid) return;
+  const long long c = t.c;
+  const SoilClass& s = P.soil.at(c);
+  if (from_head) {
+    wc[c] = vg_theta_ref(s, h[c]);
+  } else {
+    h[c] = vg_head(s, wc[c]);
+  }
+  hn[c] = h[c];
+  wcn[c] = wc[c];
+  const VGState v = vg_state(s, h[c]);
+  hwc[c] = vg_head(s, wc[c]);
+  wch[c] = v.theta;
+  ch[c] = v.cap;
+}
+
+// x0 = h^n on interior cells (previous-step initial guess).
+__global__ __launch_bounds__(NT) void k_copy_interior(Geo g, const double* src, double* dst) {
+  const Cell t = cell_of(g);
+  if (!t.valid) return;
+  dst[t.c] = src[t.c];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+ReState::ReState(int nx, int ny, int nz, const frehg_re_params& p, hipStream_t s, Profiler* prof)
+    : g_(make_geo(nx, ny, nz)), p_(p), s_(s), prof_(prof) {
+  if (nx < 1 || ny < 1 || nz < 1) throw ArgError("frehg_re_init: grid sizes must be >= 1");
+  n_ = g_.ncell();
+  const size_t bytes = (size_t)n_ * sizeof(double);
+  for (double** f : {&h_, &hn_, &wc_, &wcn_, &wch_, &hwc_, &ch_, &Kx_, &Ky_, &Kz_, &qx_, &qy_, &qz_, &kr_,
+                     &D_, &ax_, &ay_, &az_, &rhs_}) {
+    FREHG_HIP(hipMalloc(f, bytes));
+    FREHG_HIP(hipMemsetAsync(*f, 0, bytes, s_));
+  }
+  for (unsigned char** f : {&sat0_, &cand_, &satA_, &satB_}) {
+    FREHG_HIP(hipMalloc(f, (size_t)n_));
+    FREHG_HIP(hipMemsetAsync(*f, 0, (size_t)n_, s_));
+  }
+  FREHG_HIP(hipMalloc(&dflag_, 16));
+  FREHG_HIP(hipHostMalloc(&hflag_, 16, hipHostMallocDefault));
+  const unsigned int nb = ceil_div(nx, 64) * ceil_div(ny, 4) * nz;
+  FREHG_HIP(hipMalloc(&red_, 2 * (size_t)nb * sizeof(double)));
+  FREHG_HIP(hipMalloc(&dres_, 8 * sizeof(double)));
+  FREHG_HIP(hipHostMalloc(&hres_, 8 * sizeof(double), hipHostMallocDefault));
+  FREHG_HIP(hipMalloc(&dcount_, 16));
+  FREHG_HIP(hipMemsetAsync(dcount_, 0, 16, s_));
+  frehg_soil_params sp = p.soil;
+  set_soil(&sp, 1, nullptr, 0, nullptr);
+  kry_.reset(new Krylov(g_, true, s_, prof_));
+  set_params(p);
+  FREHG_HIP(hipStreamSynchronize(s_));
+}
+
+ReState::~ReState() {
+  for (void* f : {(void*)h_, (void*)hn_, (void*)wc_, (void*)wcn_, (void*)wch_, (void*)hwc_, (void*)ch_,
+                  (void*)Kx_, (void*)Ky_, (void*)Kz_, (void*)qx_, (void*)qy_, (void*)qz_, (void*)kr_,
+                  (void*)D_, (void*)ax_, (void*)ay_, (void*)az_, (void*)rhs_, (void*)D32_, (void*)ax32_,
+                  (void*)ay32_, (void*)az32_, (void*)sat0_, (void*)cand_, (void*)satA_, (void*)satB_,
+                  (void*)dflag_, (void*)red_, (void*)dres_, (void*)dcount_, (void*)classes_, (void*)cls_,
+                  (void*)ks_, (void*)qtop_})
+    if (f) (void)hipFree(f);
+  if (hflag_) (void)hipHostFree(hflag_);
+  if (hres_) (void)hipHostFree(hres_);
+}
+
+void ReState::set_params(const frehg_re_params& p) {
+  for (int b = 0; b < 4; ++b)
+    if (p.bc_type[b] != 0)
+      throw ArgError("frehg_re: lateral GW boundaries must be no-flux (0); Dirichlet/fixed-flux "
+                     "lateral faces are not implemented (src/driver/Orchestrator.cpp:155-163)");
+  if (p.bc_type[4] < 0 || p.bc_type[4] > 3) throw ArgError("frehg_re: bottom bc must be 0, 1, 2 or 3");
+  if (p.bc_type[5] < 0 || p.bc_type[5] > 2) throw ArgError("frehg_re: top bc must be 0, 1 or 2");
+  if (!(p.dx > 0 && p.dy > 0 && p.dz > 0 && p.dt > 0)) throw ArgError("frehg_re: dx, dy, dz, dt must be > 0");
+  if (p.picard_max < 1) throw ArgError("frehg_re: picard_max must be >= 1");
+  const bool mixed = p.solver.precision == FREHG_PREC_MIXED;
+  if (mixed && !D32_) {
+    const size_t b32 = (size_t)n_ * sizeof(float);
+    for (float** f : {&D32_, &ax32_, &ay32_, &az32_}) {
+      FREHG_HIP(hipMalloc(f, b32));
+      FREHG_HIP(hipMemsetAsync(*f, 0, b32, s_));
+    }
+  }
+  const bool first = (dtn_ == 0.0);
+  p_ = p;
+  if (first) dtn_ = p.dt;
+}
+
+void ReState::set_soil(const frehg_soil_params* classes, int ncls, const int32_t* class_idx, int idx3d,
+                       const double* ks_cell) {
+  if (ncls < 1) throw ArgError("frehg_re_set_soil: need at least one soil class");
+  host_classes_.clear();
+  for (int c = 0; c < ncls; ++c) {
+    const frehg_soil_params& s = classes[c];
+    if (!(s.n > 1.0) || !(s.theta_s > s.theta_r) || !(s.alpha > 0))
+      throw ArgError("frehg_re_set_soil: invalid class (need n > 1, theta_s > theta_r, alpha > 0)");
+    host_classes_.push_back(make_soil_class(s.alpha, s.n, s.theta_s, s.theta_r, s.Ks_x, s.Ks_y, s.Ks_z, s.Ss,
+                                            s.use_vg, s.use_mvg, s.air_entry));
+  }
+  if (classes_) FREHG_HIP(hipFree(classes_));
+  FREHG_HIP(hipMalloc(&classes_, host_classes_.size() * sizeof(SoilClass)));
+  FREHG_HIP(hipMemcpy(classes_, host_classes_.data(), host_classes_.size() * sizeof(SoilClass),
+                      hipMemcpyHostToDevice));
+  const int nx = g_.nx, ny = g_.ny, nz = g_.nz;
+  auto clamp = [](int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); };
+  if (cls_) {
+    FREHG_HIP(hipFree(cls_));
+    cls_ = nullptr;
+  }
+  if (class_idx) {
+    std::vector<int32_t> pad((size_t)n_, 0);
+    for (int k = -1; k <= nz; ++k)
+      for (int j = -1; j <= ny; ++j)
+        for (int i = -1; i <= nx; ++i) {
+          const int ci = clamp(i, nx), cj = clamp(j, ny), ck = clamp(k, nz);
+          const size_t o = idx3d ? (size_t)ci + (size_t)cj * nx + (size_t)ck * nx * ny : (size_t)ci + (size_t)cj * nx;
+          const int32_t v = class_idx[o];
+          if (v < 0 || v >= ncls) throw ArgError("frehg_re_set_soil: class index out of range");
+          pad[(size_t)g_.idx(i, j, k)] = v;
+        }
+    FREHG_HIP(hipMalloc(&cls_, (size_t)n_ * sizeof(int32_t)));
+    FREHG_HIP(hipMemcpy(cls_, pad.data(), (size_t)n_ * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  if (ks_) {
+    FREHG_HIP(hipFree(ks_));
+    ks_ = nullptr;
+  }
+  if (ks_cell) {
+    std::vector<double> pad((size_t)n_, 0.0);
+    for (int k = -1; k <= nz; ++k)
+      for (int j = -1; j <= ny; ++j)
+        for (int i = -1; i <= nx; ++i)
+          pad[(size_t)g_.idx(i, j, k)] =
+              ks_cell[(size_t)clamp(i, nx) + (size_t)clamp(j, ny) * nx + (size_t)clamp(k, nz) * nx * ny];
+    FREHG_HIP(hipMalloc(&ks_, (size_t)n_ * sizeof(double)));
+    FREHG_HIP(hipMemcpy(ks_, pad.data(), (size_t)n_ * sizeof(double), hipMemcpyHostToDevice));
+  }
+}
+
+void ReState::set_top_flux(const double* q) {
+  std::vector<double> pad((size_t)g_.nsurf(), 0.0);
+  for (int j = 0; j < g_.ny; ++j)
+    for (int i = 0; i < g_.nx; ++i) pad[(size_t)g_.sidx(i, j)] = q[(size_t)i + (size_t)j * g_.nx];
+  if (!qtop_) FREHG_HIP(hipMalloc(&qtop_, pad.size() * sizeof(double)));
+  FREHG_HIP(hipMemcpy(qtop_, pad.data(), pad.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
+double* ReState::field_ptr(const std::string& n) {
+  if (n == "h") return h_;
+  if (n == "hn") return hn_;
+  if (n == "wc") return wc_;
+  if (n == "wcn") return wcn_;
+  if (n == "wch") return wch_;
+  if (n == "hwc") return hwc_;
+  if (n == "ch") return ch_;
+  if (n == "Kx") return Kx_;
+  if (n == "Ky") return Ky_;
+  if (n == "Kz") return Kz_;
+  if (n == "qx") return qx_;
+  if (n == "qy") return qy_;
+  if (n == "qz") return qz_;
+  if (n == "Gct") return D_;
+  if (n == "Grhs") return rhs_;
+  throw ArgError("unknown GW field '" + n + "'");
+}
+
+void ReState::set_field(const std::string& name, const double* src) {
+  FREHG_HIP(hipMemcpyAsync(field_ptr(name), src, (size_t)n_ * sizeof(double), hipMemcpyHostToDevice, s_));
+  FREHG_HIP(hipStreamSynchronize(s_));
+}
+
+void ReState::get_field(const std::string& name, double* dst) {
+  FREHG_HIP(hipMemcpyAsync(dst, field_ptr(name), (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipStreamSynchronize(s_));
+}
+
+static ReK make_rek(const Geo& g, const SoilView& sv, const frehg_re_params& p, const double* qtop) {
+  ReK P;
+  P.g = g;
+  P.soil = sv;
+  P.dx = p.dx;
+  P.dy = p.dy;
+  P.dz = p.dz;
+  P.dt = p.dt;
+  P.htop = p.htop;
+  P.hbot = p.hbot;
+  P.qtop = p.qtop;
+  P.qbot = p.qbot;
+  P.co_max = p.co_max;
+  P.full3d = p.use_full3d ? 1 : 0;
+  P.sim_sw = p.sim_shallowwater ? 1 : 0;
+  P.bc4 = p.bc_type[4];
+  P.bc5 = p.bc_type[5];
+  P.use_corrector = p.use_corrector ? 1 : 0;
+  P.qtop_field = qtop;
+  return P;
+}
+
+void ReState::set_initial(const double* h_int, const double* wc_int) {
+  if ((h_int == nullptr) == (wc_int == nullptr))
+    throw ArgError("frehg_re_set_initial: pass exactly one of h / wc");
+  const double* src = h_int ? h_int : wc_int;
+  std::vector<double> pad((size_t)n_, 0.0);
+  FREHG_HIP(hipMemcpy(pad.data(), h_int ? h_ : wc_, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < g_.nz; ++k)
+    for (int j = 0; j < g_.ny; ++j)
+      for (int i = 0; i < g_.nx; ++i)
+        pad[(size_t)g_.idx(i, j, k)] = src[(size_t)i + (size_t)j * g_.nx + (size_t)k * g_.nx * g_.ny];
+  FREHG_HIP(hipMemcpy(h_int ? h_ : wc_, pad.data(), (size_t)n_ * sizeof(double), hipMemcpyHostToDevice));
+  const ReK P = make_rek(g_, soil_view(), p_, qtop_);
+  const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4), g_.nz);
+  hipLaunchKernelGGL(k_re_initial, grid, dim3(NT), 0, s_, P, h_int ? 1 : 0, h_, wc_, hn_, wcn_, wch_, hwc_, ch_);
+  FREHG_LAUNCH_CHECK();
+  FREHG_HIP(hipStreamSynchronize(s_));
+  dtn_ = p_.dt;
+}
+
+void ReState::step(frehg_re_stats* st) {
+  frehg_re_stats local;
+  std::memset(&local, 0, sizeof(local));
+  dt_used_ = p_.dt;
+  const bool mixed = p_.solver.precision == FREHG_PREC_MIXED;
+  const ReK P = make_rek(g_, soil_view(), p_, qtop_);
+  const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4), g_.nz);
+  const dim3 blk(NT);
+  // savePreviousStepFields (ReSolver.cpp:190-197): hn <- h, wcn <- wc. The arrays are swapped
+  // instead of copied; the new h and wc buffers are fully rewritten below (corner ghosts, which
+  // no kernel reads or writes, hold the same values in both buffers).
+  std::swap(h_, hn_);
+  std::swap(wc_, wcn_);
+  prof_->begin(3);
+  hipLaunchKernelGGL(k_re_pre, grid, blk, 0, s_, P, hn_, kr_, ch_);
+  FREHG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_re_assemble, grid, blk, 0, s_, P, kr_, ch_, wcn_, hn_, D_, ax_, ay_, az_, rhs_,
+                     mixed ? D32_ : nullptr, ax32_, ay32_, az32_);
+  FREHG_LAUNCH_CHECK();
+  prof_->end();
+  // Initial guess in h (the solve's x): zero (PETSc default), h^n, or h^n + MLP increment.
+  bool x0_zero = true, guard = false;
+  if (p_.initial_guess == FREHG_GUESS_PREVIOUS) {
+    hipLaunchKernelGGL(k_copy_interior, grid, blk, 0, s_, g_, hn_, h_);
+    FREHG_LAUNCH_CHECK();
+    x0_zero = false;
+  } else if (p_.initial_guess == FREHG_GUESS_LEARNED) {
+    if (!mlp_ || !mlp_->ready()) throw ArgError("frehg_re_step: learned guess requested but no trained model loaded");
+    mlp_->predict_guess(*this, h_);
+    x0_zero = false;
+    guard = true;
+  }
+  KrylovConfig kc;
+  kc.rtol = p_.solver.rtol;
+  kc.atol = p_.solver.atol;
+  kc.max_it = p_.solver.max_it;
+  kc.precision = mixed ? 1 : 0;
+  kc.inner_rtol = p_.solver.inner_rtol > 0 ? p_.solver.inner_rtol : 1e-5;
+  kc.max_refine = p_.solver.max_refine > 0 ? p_.solver.max_refine : 8;
+  StencilView<double> A{D_, ax_, ay_, az_};
+  StencilView<float> A32{D32_, ax32_, ay32_, az32_};
+  const SolveResult sr = kry_->solve(A, mixed ? &A32 : nullptr, rhs_, nullptr, h_, nullptr, x0_zero, guard, kc);
+  local.solve.iterations = sr.iterations;
+  local.solve.refinements = sr.refinements;
+  local.solve.residual_norm = sr.residual_norm;
+  local.solve.rhs_norm = sr.rhs_norm;
+  local.solve.converged = sr.converged ? 1 : 0;
+  local.solve.fallback = sr.fallback_fp64 ? 1 : 0;
+  local.guess_fallback = sr.guess_rejected ? 1 : 0;
+  local.picard_iterations = 1;
+  if (mlp_ && mlp_->logging()) mlp_->log_pair(*this, hn_, h_);
+  prof_->begin(4);
+  hipLaunchKernelGGL(k_re_post, grid, blk, 0, s_, P, h_, kr_, ch_);
+  FREHG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_re_flux, grid, blk, 0, s_, P, h_, kr_, wcn_, Kx_, Ky_, Kz_, qx_, qy_, qz_);
+  FREHG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_re_correct, grid, blk, 0, s_, P, h_, hn_, wcn_, qx_, qy_, qz_, wc_, wch_, hwc_, sat0_,
+                     cand_);
+  FREHG_LAUNCH_CHECK();
+  prof_->end();
+  // Reallocation fixed point: iterate from sat0 until the saturation state stops changing.
+  FREHG_HIP(hipMemcpyAsync(satA_, sat0_, (size_t)n_, hipMemcpyDeviceToDevice, s_));
+  FREHG_HIP(hipMemcpyAsync(satB_, sat0_, (size_t)n_, hipMemcpyDeviceToDevice, s_));
+  unsigned char* prev = satA_;
+  unsigned char* next = satB_;
+  for (int it = 0; it < g_.nx * g_.ny * g_.nz + 1; ++it) {
+    FREHG_HIP(hipMemsetAsync(dflag_, 0, sizeof(int), s_));
+    hipLaunchKernelGGL(k_re_realloc_iter, grid, blk, 0, s_, P, Kx_, Ky_, sat0_, cand_, prev, next, dflag_);
+    FREHG_LAUNCH_CHECK();
+    FREHG_HIP(hipMemcpyAsync(hflag_, dflag_, sizeof(int), hipMemcpyDeviceToHost, s_));
+    FREHG_HIP(hipStreamSynchronize(s_));
+    std::swap(prev, next);
+    if (hflag_[0] == 0) break;
+  }
+  hipLaunchKernelGGL(k_re_finalize, grid, blk, 0, s_, P, Kx_, Ky_, sat0_, prev, wch_, hwc_, wc_, h_);
+  FREHG_LAUNCH_CHECK();
+  if (p_.adaptive_dt) {
+    const unsigned int nb = grid.x * grid.y * grid.z;
+    hipLaunchKernelGGL(k_re_dtadapt, grid, blk, 0, s_, P, qx_, qy_, qz_, wc_, red_, dcount_, nb, dres_);
+    FREHG_LAUNCH_CHECK();
+    FREHG_HIP(hipMemcpyAsync(hres_, dres_, 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
+    FREHG_HIP(hipStreamSynchronize(s_));
+    // ReSolver::adaptiveTimeStep (ReSolver.cpp:890-928)
+    const double dq_max = hres_[0], dt_Comin = hres_[1];
+    p_.dt = dtn_;
+    if (dq_max > 0.02) p_.dt *= 0.75;
+    else if (dq_max < 0.01) p_.dt *= 1.25;
+    if (p_.dt > dt_Comin) p_.dt = dt_Comin;
+    if (p_.dt > p_.dt_max) p_.dt = p_.dt_max;
+    if (p_.dt < p_.dt_min) p_.dt = p_.dt_min;
+    dtn_ = p_.dt;
+  }
+  local.dt_used = dt_used_;
+  local.dt_next = p_.dt;
+  if (st) *st = local;
+}
+
+}  // namespace frehg

@@ -1,0 +1,89 @@
+// Device-resident Richards-equation PCA step (replaces src/re/ReSolver.{hpp,cpp}).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/frehg_gpu.h"
+#include "common.hpp"
+#include "krylov.hpp"
+
+namespace frehg {
+
+class Mlp;  // learned initial guess (mlp.hpp)
+
+class ReState {
+ public:
+  ReState(int nx, int ny, int nz, const frehg_re_params& p, hipStream_t s, Profiler* prof);
+  ~ReState();
+
+  void set_params(const frehg_re_params& p);
+  void set_soil(const frehg_soil_params* classes, int ncls, const int32_t* class_idx, int idx3d,
+                const double* ks_cell);
+  void set_top_flux(const double* q_interior);
+  void set_initial(const double* h_interior, const double* wc_interior);
+  void set_field(const std::string& name, const double* src);
+  void get_field(const std::string& name, double* dst);
+  void step(frehg_re_stats* st);
+  double dt() const { return p_.dt; }
+  void prime_dt(double dt) {
+    p_.dt = dt;
+    dtn_ = dt;
+  }
+
+  const Geo& geo() const { return g_; }
+  const frehg_re_params& params() const { return p_; }
+  const SoilClass& class0() const { return host_classes_[0]; }
+  double dt_used() const { return dt_used_; }
+
+  // Device views used by the coupling / learned-guess kernels.
+  double* d_h() { return h_; }
+  double* d_wc() { return wc_; }
+  double* d_hn() { return hn_; }
+  double* d_kr() { return kr_; }
+  double* d_ch() { return ch_; }
+  SoilView soil_view() const { return SoilView{classes_, cls_, ks_}; }
+  Krylov& krylov() { return *kry_; }
+  void set_mlp(Mlp* m) { mlp_ = m; }
+
+ private:
+  double* field_ptr(const std::string& name);
+  void launch_pre(const double* hsrc);
+  void launch_assemble(bool mixed);
+  void apply_guess(bool* guarded);
+
+  Geo g_;
+  frehg_re_params p_;
+  hipStream_t s_;
+  Profiler* prof_;
+  long long n_ = 0;
+  // GwFields (src/re/GwFields.hpp) that cross the ABI, halo padded, FP64.
+  double *h_ = nullptr, *hn_ = nullptr, *wc_ = nullptr, *wcn_ = nullptr, *wch_ = nullptr,
+         *hwc_ = nullptr, *ch_ = nullptr, *Kx_ = nullptr, *Ky_ = nullptr, *Kz_ = nullptr,
+         *qx_ = nullptr, *qy_ = nullptr, *qz_ = nullptr;
+  // Relative conductivity K/Ks per cell (incl. face-adjacent ghosts).
+  double* kr_ = nullptr;
+  // Assembled system: D = Gct, ax/ay/az = +x/+y/+z couplings, rhs = Grhs; FP32 copies.
+  double *D_ = nullptr, *ax_ = nullptr, *ay_ = nullptr, *az_ = nullptr, *rhs_ = nullptr;
+  float *D32_ = nullptr, *ax32_ = nullptr, *ay32_ = nullptr, *az32_ = nullptr;
+  // Saturation-reallocation flags (uint8, padded).
+  unsigned char *sat0_ = nullptr, *cand_ = nullptr, *satA_ = nullptr, *satB_ = nullptr;
+  int* dflag_ = nullptr;   // device: [0] changed flag
+  int* hflag_ = nullptr;   // pinned host mirror
+  double* red_ = nullptr;  // reduction partials (adaptive dt / stats)
+  double* dres_ = nullptr; // device results [dq_max, dt_Comin, max_dh]
+  double* hres_ = nullptr;
+  unsigned int* dcount_ = nullptr;
+  // Soil.
+  SoilClass* classes_ = nullptr;
+  std::vector<SoilClass> host_classes_;
+  int32_t* cls_ = nullptr;
+  double* ks_ = nullptr;
+  double* qtop_ = nullptr;  // per-column top flux (surface padded) or null
+  std::unique_ptr<Krylov> kry_;
+  Mlp* mlp_ = nullptr;
+  double dtn_ = 0.0, dt_used_ = 0.0;
+};
+
+}  // namespace frehg

@@ -1,0 +1,52 @@
+// Device-resident semi-implicit shallow-water step (replaces src/swe/SweSolver.{hpp,cpp}).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+
+#include "../../include/frehg_gpu.h"
+#include "common.hpp"
+#include "krylov.hpp"
+
+namespace frehg {
+
+class SweState {
+ public:
+  SweState(int nx, int ny, double dx, double dy, const frehg_swe_params& p, hipStream_t s, Profiler* prof);
+  ~SweState();
+
+  void set_params(const frehg_swe_params& p);
+  void set_bathymetry(const double* bed_interior);
+  void initialize_state(double init_eta);
+  void set_field(const std::string& name, const double* src);
+  void get_field(const std::string& name, double* dst);
+  void step(double rain, double evap, frehg_solve_stats* st);
+  double max_cfl();
+
+  const Geo& geo() const { return g_; }
+  double dx() const { return dx_; }
+  double dy() const { return dy_; }
+  const frehg_swe_params& params() const { return p_; }
+  double* d(const std::string& name) { return field(name); }
+
+ private:
+  double* field(const std::string& name);
+
+  Geo g_;  // nz = 1; 2D index = sidx
+  double dx_, dy_;
+  frehg_swe_params p_;
+  hipStream_t s_;
+  Profiler* prof_;
+  long long n_ = 0;
+  std::map<std::string, double*> f_;
+  double *D_ = nullptr, *ax_ = nullptr, *ay_ = nullptr, *b_ = nullptr;
+  unsigned char* fixed_ = nullptr;
+  double* red_ = nullptr;
+  double* dres_ = nullptr;
+  double* hres_ = nullptr;
+  unsigned int* dcount_ = nullptr;
+  std::unique_ptr<Krylov> kry_;
+};
+
+}  // namespace frehg

@@ -1,0 +1,249 @@
+"""Python binding of the gfx950 backend's C ABI (include/frehg_gpu.h) via ctypes.
+
+Classes mirror the reference's method sets: ReSolver (src/re/ReSolver.hpp:48-105),
+SweSolver (src/swe/SweSolver.hpp:41-94), Coupling (src/coupling/Coupling.hpp:84) and
+LinearSolver (include/frehg2/linear/LinearSolver.hpp:18-33). Errors raise RuntimeError with the
+library's message, as the reference raises std::runtime_error. There is no CPU fallback: if the
+HIP library is missing or no GPU is present, construction fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfrehg_gpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "frehg_gpu.h")
+
+P = C.POINTER
+dp = P(C.c_double)
+_lib = None
+
+
+class FrehgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"frehg error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load libfrehg_gpu.so (built by frehg_amd.build); raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP backend not built: {LIB_PATH} missing (run python -m frehg_amd.build)")
+    L = C.CDLL(LIB_PATH)
+    ctx = C.c_void_p
+    sig = {
+        "frehg_abi_version": (C.c_int, []),
+        "frehg_create": (C.c_int, [P(ctx), C.c_int]),
+        "frehg_destroy": (None, [ctx]),
+        "frehg_last_error": (C.c_char_p, [ctx]),
+        "frehg_synchronize": (C.c_int, [ctx]),
+        "frehg_re_init": (C.c_int, [ctx, C.c_int, C.c_int, C.c_int, P(abi.ReParams)]),
+        "frehg_re_set_params": (C.c_int, [ctx, P(abi.ReParams)]),
+        "frehg_re_set_soil": (C.c_int, [ctx, P(abi.SoilParams), C.c_int, P(C.c_int32), C.c_int, dp]),
+        "frehg_re_set_top_flux": (C.c_int, [ctx, dp]),
+        "frehg_re_set_initial": (C.c_int, [ctx, dp, dp]),
+        "frehg_re_set_field": (C.c_int, [ctx, C.c_char_p, dp]),
+        "frehg_re_get_field": (C.c_int, [ctx, C.c_char_p, dp]),
+        "frehg_re_step": (C.c_int, [ctx, P(abi.ReStats)]),
+        "frehg_re_dt": (C.c_double, [ctx]),
+        "frehg_re_prime_dt": (C.c_int, [ctx, C.c_double]),
+        "frehg_swe_init": (C.c_int, [ctx, C.c_int, C.c_int, C.c_double, C.c_double, P(abi.SweParams)]),
+        "frehg_swe_set_params": (C.c_int, [ctx, P(abi.SweParams)]),
+        "frehg_swe_set_bathymetry": (C.c_int, [ctx, dp]),
+        "frehg_swe_initialize_state": (C.c_int, [ctx, C.c_double]),
+        "frehg_swe_set_field": (C.c_int, [ctx, C.c_char_p, dp]),
+        "frehg_swe_get_field": (C.c_int, [ctx, C.c_char_p, dp]),
+        "frehg_swe_step": (C.c_int, [ctx, C.c_double, C.c_double, P(abi.SolveStats)]),
+        "frehg_swe_max_cfl": (C.c_int, [ctx, dp]),
+        "frehg_couple_exchange": (C.c_int, [ctx, P(abi.CouplingParams), C.c_double, dp, dp]),
+        "frehg_coupled_step": (C.c_int, [ctx, P(abi.CouplingParams), C.c_double, C.c_double, C.c_double,
+                                         C.c_double, P(abi.CoupledStats)]),
+        "frehg_water_volume": (C.c_int, [ctx, dp, dp]),
+        "frehg_ls_solve": (C.c_int, [ctx, C.c_int, C.c_int, C.c_int, C.c_int, dp, P(dp), dp, dp,
+                                     P(abi.SolverConfig), P(abi.SolveStats)]),
+        "frehg_profile_enable": (C.c_int, [ctx, C.c_int]),
+        "frehg_profile_read": (C.c_int, [ctx, C.c_int, dp, P(C.c_int64)]),
+        "frehg_profile_reset": (C.c_int, [ctx]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    if L.frehg_abi_version() != abi.ABI_VERSION:
+        raise RuntimeError("libfrehg_gpu.so ABI version mismatch")
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(dp)
+
+
+class Context:
+    """One HIP device + stream holding a GW solver, a SW solver and their coupling."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        self.h = C.c_void_p()
+        rc = self.lib.frehg_create(C.byref(self.h), device)
+        if rc != 0:
+            raise FrehgError(rc, "frehg_create failed (no GPU?)")
+        self.re_dims = None
+        self.swe_dims = None
+
+    def close(self):
+        if self.h:
+            self.lib.frehg_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc, allow_info=True):
+        if rc < 0 or (rc > 0 and not allow_info):
+            raise FrehgError(rc, self.lib.frehg_last_error(self.h).decode())
+        return rc
+
+    # ---- ReSolver ------------------------------------------------------------------------------
+    def re_init(self, nx, ny, nz, params):
+        self.check(self.lib.frehg_re_init(self.h, nx, ny, nz, C.byref(params)))
+        self.re_dims = (nx, ny, nz)
+
+    def re_set_params(self, params):
+        self.check(self.lib.frehg_re_set_params(self.h, C.byref(params)))
+
+    def re_set_soil(self, classes, class_idx=None, idx3d=False, ks_cell=None):
+        arr = (abi.SoilParams * len(classes))(*classes)
+        ci = None
+        if class_idx is not None:
+            class_idx = np.ascontiguousarray(class_idx, dtype=np.int32).ravel()
+            ci = class_idx.ctypes.data_as(P(C.c_int32))
+        ks = None if ks_cell is None else np.ascontiguousarray(ks_cell, dtype=np.float64).ravel()
+        self.check(self.lib.frehg_re_set_soil(self.h, arr, len(classes), ci, int(idx3d), _ptr(ks)))
+
+    def re_set_top_flux(self, q):
+        self.check(self.lib.frehg_re_set_top_flux(self.h, _ptr(np.ascontiguousarray(q, dtype=np.float64).ravel())))
+
+    def re_set_initial(self, h=None, wc=None):
+        hh = None if h is None else np.ascontiguousarray(h, dtype=np.float64).ravel()
+        ww = None if wc is None else np.ascontiguousarray(wc, dtype=np.float64).ravel()
+        self.check(self.lib.frehg_re_set_initial(self.h, _ptr(hh), _ptr(ww)))
+
+    def re_ncell(self):
+        nx, ny, nz = self.re_dims
+        return (nx + 2) * (ny + 2) * (nz + 2)
+
+    def re_get(self, name):
+        out = np.empty(self.re_ncell(), dtype=np.float64)
+        self.check(self.lib.frehg_re_get_field(self.h, name.encode(), _ptr(out)))
+        return out
+
+    def re_set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        assert a.size == self.re_ncell()
+        self.check(self.lib.frehg_re_set_field(self.h, name.encode(), _ptr(a)))
+
+    def re_step(self):
+        st = abi.ReStats()
+        self.check(self.lib.frehg_re_step(self.h, C.byref(st)))
+        return st
+
+    def re_dt(self):
+        return self.lib.frehg_re_dt(self.h)
+
+    # ---- SweSolver -----------------------------------------------------------------------------
+    def swe_init(self, nx, ny, dx, dy, params):
+        self.check(self.lib.frehg_swe_init(self.h, nx, ny, dx, dy, C.byref(params)))
+        self.swe_dims = (nx, ny)
+
+    def swe_set_bathymetry(self, bed):
+        self.check(self.lib.frehg_swe_set_bathymetry(self.h, _ptr(np.ascontiguousarray(bed, dtype=np.float64).ravel())))
+
+    def swe_initialize_state(self, eta0):
+        self.check(self.lib.frehg_swe_initialize_state(self.h, eta0))
+
+    def swe_nsurf(self):
+        nx, ny = self.swe_dims
+        return (nx + 2) * (ny + 2)
+
+    def swe_get(self, name):
+        out = np.empty(self.swe_nsurf(), dtype=np.float64)
+        self.check(self.lib.frehg_swe_get_field(self.h, name.encode(), _ptr(out)))
+        return out
+
+    def swe_set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        assert a.size == self.swe_nsurf()
+        self.check(self.lib.frehg_swe_set_field(self.h, name.encode(), _ptr(a)))
+
+    def swe_step(self, rain, evap=0.0):
+        st = abi.SolveStats()
+        self.check(self.lib.frehg_swe_step(self.h, rain, evap, C.byref(st)))
+        return st
+
+    def swe_max_cfl(self):
+        v = C.c_double()
+        self.check(self.lib.frehg_swe_max_cfl(self.h, C.byref(v)))
+        return v.value
+
+    # ---- Coupling / Orchestrator::step -------------------------------------------------------------
+    def couple_exchange(self, cp, dt, want_q=False):
+        vol = C.c_double()
+        q = np.empty(self.swe_dims[0] * self.swe_dims[1]) if want_q else None
+        self.check(self.lib.frehg_couple_exchange(self.h, C.byref(cp), dt, C.byref(vol), _ptr(q)))
+        return (vol.value, q) if want_q else vol.value
+
+    def coupled_step(self, cp, dt, rain, evap=0.0, t_end=0.0):
+        st = abi.CoupledStats()
+        self.check(self.lib.frehg_coupled_step(self.h, C.byref(cp), dt, rain, evap, t_end, C.byref(st)))
+        return st
+
+    def water_volume(self):
+        a, b = C.c_double(), C.c_double()
+        self.check(self.lib.frehg_water_volume(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def synchronize(self):
+        self.check(self.lib.frehg_synchronize(self.h))
+
+    # ---- LinearSolver seam ---------------------------------------------------------------------------
+    def ls_solve(self, ndim, nx, ny, nz, diag, offdiag, b, cfg):
+        x = np.zeros_like(np.ascontiguousarray(b, dtype=np.float64))
+        offs = [np.ascontiguousarray(o, dtype=np.float64) for o in offdiag]
+        arr = (dp * len(offs))(*[_ptr(o) for o in offs])
+        st = abi.SolveStats()
+        self.check(self.lib.frehg_ls_solve(self.h, ndim, nx, ny, nz, _ptr(np.ascontiguousarray(diag, dtype=np.float64)),
+                                           arr, _ptr(np.ascontiguousarray(b, dtype=np.float64)), _ptr(x),
+                                           C.byref(cfg), C.byref(st)))
+        return x, st
+
+    # ---- measurement -----------------------------------------------------------------------------------
+    def profile_enable(self, on=True):
+        self.check(self.lib.frehg_profile_enable(self.h, int(on)))
+
+    def profile_read(self, kt):
+        ms, n = C.c_double(), C.c_int64()
+        self.check(self.lib.frehg_profile_read(self.h, kt, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def profile_reset(self):
+        self.check(self.lib.frehg_profile_reset(self.h))
+
+
+def declared_symbols(header=HEADER):
+    """Function names declared in include/frehg_gpu.h."""
+    import re
+    text = open(header).read()
+    return sorted(set(re.findall(r"\b(frehg_[a-z0-9_]+)\s*\(", text)))

@@ -1,0 +1,58 @@
+"""LinearSolver seam (frehg_ls_solve): manufactured-solution KATs of the reference
+(tests/linear/test_petsc_linear_solver.cpp:25-101, test_assembly_backend_independent.cpp:21-80),
+their 7-point analogue, an oracle comparison and the error paths."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from frehg_amd import abi, gpu
+from test_oracle_linear import laplacian, matvec
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("ndim,dims", [(2, (6, 5, 1)), (3, (5, 4, 6)), (3, (70, 33, 9))])
+def test_manufactured(ndim, dims):
+    nx, ny, nz = dims
+    diag, offs, cols = laplacian(ndim, nx, ny, nz)
+    n = len(diag)
+    ctx = gpu.Context(0)
+    for xs in (1.5 + np.sin(0.1 * np.arange(n)), 0.5 + 0.001 * np.arange(n)):
+        b = matvec(diag, offs, cols, xs)
+        x, st = ctx.ls_solve(ndim, nx, ny, nz, diag, offs, b, abi.solver_config(1e-13))
+        assert st.converged and np.linalg.norm(x - xs) < 1e-9 * max(1.0, np.sqrt(n / 30))
+        xo, _ = ol.ls_solve(ndim, nx, ny, nz, diag, offs, b, abi.solver_config(1e-13))
+        assert np.abs(x - xo).max() < 1e-10
+
+
+def test_decoupled_rows_2d():
+    """Rows with no off-diagonals (dry cells at rest) are solved exactly and folded out."""
+    nx, ny = 7, 6
+    diag, offs, cols = laplacian(2, nx, ny)
+    for d in range(4):
+        offs[d] = offs[d].copy()
+    row = 2 * nx + 3
+    for d in range(4):
+        offs[d][row] = 0.0
+    diag[row] = 6400.0
+    b = np.ones(nx * ny)
+    b[row] = 6400.0 * 0.37
+    ctx = gpu.Context(0)
+    x, st = ctx.ls_solve(2, nx, ny, 1, diag, offs, b, abi.solver_config(1e-13))
+    xo, _ = ol.ls_solve(2, nx, ny, 1, diag, offs, b, abi.solver_config(1e-13))
+    assert x[row] == pytest.approx(0.37, rel=1e-15)
+    assert np.abs(x - xo).max() < 1e-10
+
+
+def test_errors_raise():
+    ctx = gpu.Context(0)
+    nx, ny, nz = 6, 6, 6
+    diag, offs, cols = laplacian(3, nx, ny, nz)
+    b = np.random.default_rng(0).standard_normal(nx * ny * nz)
+    with pytest.raises(gpu.FrehgError) as e:
+        ctx.ls_solve(3, nx, ny, nz, diag, offs, b, abi.solver_config(1e-14, max_it=2))
+    assert e.value.code == abi.ERR_NOT_CONVERGED
+    offs[1] = offs[1] * 2.0  # x+ couplings no longer mirror the x- ones
+    with pytest.raises(gpu.FrehgError) as e:
+        ctx.ls_solve(3, nx, ny, nz, diag, offs, b, abi.solver_config(1e-10))
+    assert e.value.code == abi.ERR_INVALID_ARG

@@ -1,0 +1,98 @@
+"""Shallow-water step: HIP path (through the C ABI) vs the oracle and the reference's golden
+vector benchmarks/b1-sw/frehg2_np1_depth_reference.txt."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from cases import b1_inputs, gaussian_bump_bed, interior2, interp_series
+from frehg_amd import abi, gpu
+from gpu_common import compare_swe, say, swe_copy_state
+
+pytestmark = pytest.mark.gpu
+
+
+def test_b1_sw_golden_vector_on_gpu():
+    say("SWE b1-sw golden on GPU")
+    bath, rain, ref, offset, p = b1_inputs()
+    ctx = gpu.Context(0)
+    ctx.swe_init(1, 10, 80.0, 80.0, p)
+    ctx.swe_set_bathymetry(bath + offset)
+    ctx.swe_initialize_state(-2.0)
+    t, worst = 0.0, 0.0
+    for step in range(1, 3601):
+        t += 5.0
+        ctx.swe_step(interp_series(rain[:, 0], rain[:, 1], t), 0.0)
+        if step % 360 == 0:
+            row = ref[step // 360 - 1]
+            d = interior2(ctx.swe_get("dept"), 1, 10)[:, 0]
+            worst = max(worst, np.linalg.norm(d - row[1:]) / np.linalg.norm(row[1:]))
+    say(f"  worst rel-L2 {worst:.3e}")
+    assert worst < 1e-10
+
+
+def test_init_matches_oracle():
+    nx, ny = 13, 9
+    rng = np.random.default_rng(2)
+    bed = 0.3 * rng.random((ny, nx)) + 0.05 * np.arange(nx)[None, :]
+    p = abi.swe_params(manning=0.03, dt=0.5, offset=0.1)
+    orc = ol.OracleSwe(nx, ny, 1.0, 1.0, p)
+    orc.set_bathymetry(bed.ravel())
+    orc.initialize_state(0.25)
+    ctx = gpu.Context(0)
+    ctx.swe_init(nx, ny, 1.0, 1.0, p)
+    ctx.swe_set_bathymetry(bed.ravel())
+    ctx.swe_initialize_state(0.25)
+    compare_swe(ctx, orc, rtol=0, atol=0, label="init")
+
+
+@pytest.mark.parametrize("visc", [0.0, 1e-3])
+def test_wet_dry_rain_2d(visc):
+    """2D wetting/drying with rain, viscosity and every limiter branch active."""
+    nx, ny = 24, 17
+    say(f"SWE 2D wet/dry visc={visc}")
+    rng = np.random.default_rng(4)
+    x = np.arange(nx)[None, :]
+    bed = 0.02 * x + 0.05 * rng.standard_normal((ny, nx))
+    p = abi.swe_params(manning=0.03, dt=0.5, visc_x=visc, visc_y=visc, hE=0.001)
+    orc = ol.OracleSwe(nx, ny, 2.0, 1.5, p)
+    orc.set_bathymetry(bed.ravel())
+    orc.initialize_state(0.2)
+    ctx = gpu.Context(0)
+    ctx.swe_init(nx, ny, 2.0, 1.5, p)
+    ctx.swe_set_bathymetry(bed.ravel())
+    ctx.swe_initialize_state(0.2)
+    for s in range(150):
+        rain = 2e-4 if s < 80 else 0.0
+        orc.step(rain, 1e-6)
+        ctx.swe_step(rain, 1e-6)
+        if s % 50 == 0 or s == 149:
+            compare_swe(ctx, orc, rtol=1e-9, atol=1e-11, label=f"step {s}")
+    d = interior2(ctx.swe_get("dept"), nx, ny)
+    assert (d == 0).any() and (d > 0).any(), "case must contain wet and dry cells"
+
+
+def test_b0_lake_at_rest():
+    p = abi.swe_params(manning=0.02, dt=1.0, solver=abi.solver_config(1e-13))
+    ctx = gpu.Context(0)
+    ctx.swe_init(20, 20, 1.0, 1.0, p)
+    ctx.swe_set_bathymetry(gaussian_bump_bed().ravel())
+    ctx.swe_initialize_state(1.0)
+    for _ in range(200):
+        ctx.swe_step(0.0, 0.0)
+    assert np.abs(interior2(ctx.swe_get("eta"), 20, 20) - 1.0).max() < 1e-8
+    assert np.abs(interior2(ctx.swe_get("uu"), 20, 20)).max() < 1e-8
+
+
+def test_max_cfl():
+    nx, ny = 10, 8
+    p = abi.swe_params(dt=0.5)
+    orc = ol.OracleSwe(nx, ny, 1.0, 1.0, p)
+    orc.set_bathymetry(np.linspace(0, 0.3, nx * ny))
+    orc.initialize_state(0.2)
+    ctx = gpu.Context(0)
+    ctx.swe_init(nx, ny, 1.0, 1.0, p)
+    swe_copy_state(orc, ctx)
+    for _ in range(5):
+        orc.step(1e-4)
+        ctx.swe_step(1e-4)
+    assert ctx.swe_max_cfl() == pytest.approx(orc.max_cfl(), rel=1e-10)
