@@ -1,0 +1,193 @@
+#!/usr/bin/env python
+"""Coupled cell-steps/s of the Frehg2 coupled step on one MI355X.
+
+Workload: the coupled 512x512 surface over 512x512x30 soil (dx = 1 m, dz = 0.05 m, dt = 1 s,
+50 mm/h rain on a dry tilted bed with a Gaussian random field, lognormal per-cell Ks), inputs
+generated from seeds by frehg_amd.workloads. One step = Orchestrator::step, coupled branch:
+SWE step, SW<->GW exchange, Richards PCA step with its Krylov solve.
+
+Two GPU paths run from the same initial state for warmup + timed steps:
+  plain  FP64 Jacobi-PCG from x0 = 0 (the reference's solver semantics)
+  fast   FP32-inner / FP64-refined PCG from the best available initial guess
+`value` is the fast path's throughput; its final head is checked against the plain path's.
+Kernel durations come from HIP events on the backend's stream during the timed region.
+The CPU baseline is the repository's FP64 oracle (a restatement of the reference algorithm)
+timed on a bounded sample of the same workload on this host's cores.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def interior3(a, nx, ny, nz):
+    return a.reshape(nz + 2, ny + 2, nx + 2)[1:-1, 1:-1, 1:-1]
+
+
+def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None):
+    """Warmup + timed coupled steps; returns timing, solver counters and kernel timings."""
+    ctx = gpu.Context(0)
+    if path == "plain":
+        case.setup_gpu(ctx, abi.PREC_FP64, abi.GUESS_ZERO)
+    else:
+        case.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_PREVIOUS)
+    grabbed = None
+    for s in range(warmup):
+        ctx.coupled_step(case.cp, case.dt, case.rain, 0.0, case.t_end)
+        if grab_after is not None and s + 1 == grab_after:
+            grabbed = ctx.re_get("h")
+    ctx.synchronize()
+    ctx.profile_enable(True)
+    ctx.profile_reset()
+    iters = refines = 0
+    t0 = time.perf_counter()
+    for s in range(steps):
+        st = ctx.coupled_step(case.cp, case.dt, case.rain, 0.0, case.t_end)
+        iters += st.gw.solve.iterations
+        refines += st.gw.solve.refinements
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = {k: ctx.profile_read(v) for k, v in (("spmv64", abi.KT_GW_SPMV), ("spmv32", abi.KT_GW_SPMV32),
+                                                 ("update", abi.KT_GW_UPDATE), ("assemble", abi.KT_GW_ASSEMBLE),
+                                                 ("correct", abi.KT_GW_CORRECT))}
+    ctx.profile_enable(False)
+    h = ctx.re_get("h")
+    sw_vol, gw_vol = ctx.water_volume()
+    ctx.close()
+    return {"elapsed": elapsed, "iters": iters, "refines": refines, "kt": kt, "h": h, "grabbed": grabbed,
+            "volume": sw_vol + gw_vol}
+
+
+def cpu_baseline(case, abi, steps):
+    """The FP64 oracle on the same workload for `steps` coupled steps (bounded sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    nthreads = ol.lib().orc_num_threads() if hasattr(ol.lib(), "orc_num_threads") else os.cpu_count()
+    s = ol.OracleSwe(case.nx, case.ny, case.dx, case.dx, case.swe)
+    s.set_bathymetry(case.bed_with_offset())
+    s.initialize_state(-1.0e3)
+    r = ol.OracleRe(case.nx, case.ny, case.nz, case.re_params())
+    r.set_soil([case.soil], None, True, case.ks.ravel())
+    r.set_initial(h=case.h0)
+    oc = ol.OracleCoupled(s, r, case.cp, case.t_end)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        oc.step(case.dt, case.rain)
+    elapsed = time.perf_counter() - t0
+    return {"elapsed": elapsed, "threads": nthreads, "h": r.get("h")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--nz", type=int, default=30)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); the multi-GPU "
+              "sharding of the coupled step is a design, not a build.", file=sys.stderr)
+        sys.exit(2)
+
+    from frehg_amd import abi, gpu, workloads
+    log(f"[bench] generating inputs {args.nx}x{args.ny}x{args.nz}")
+    case = workloads.CoupledCase(nx=args.nx, ny=args.ny, nz=args.nz)
+    n3 = case.nx * case.ny * case.nz
+    cpu_steps = 0 if args.no_cpu else max(1, min(args.cpu_steps, args.warmup))
+
+    log("[bench] plain FP64 path")
+    plain = run_gpu(case, abi, gpu, "plain", args.warmup, args.steps, grab_after=cpu_steps or None)
+    log(f"[bench]   {plain['elapsed']:.3f} s, {plain['iters']} Krylov iterations")
+    log("[bench] fast path (FP32-in-FP64 refinement, previous-step guess)")
+    fast = run_gpu(case, abi, gpu, "fast", args.warmup, args.steps)
+    log(f"[bench]   {fast['elapsed']:.3f} s, {fast['iters']} inner iterations, {fast['refines']} refinements")
+
+    hp = interior3(plain["h"], case.nx, case.ny, case.nz)
+    hf = interior3(fast["h"], case.nx, case.ny, case.nz)
+    head_err = float(np.max(np.abs(hp - hf)))
+
+    def roof(kt_ms_n, bytes_per_cell):
+        ms, n = kt_ms_n
+        if n == 0:
+            return None
+        avg_s = ms / n / 1e3
+        achieved = bytes_per_cell * n3 / avg_s / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
+                "launches": n, "bytes_per_cell": bytes_per_cell}
+
+    # fused PCG SpMV kernel: reads z, p_prev, D, ax, ay, az; writes p, q (8 words per cell)
+    roof64 = roof(plain["kt"]["spmv64"], 64)
+    roof32 = roof(fast["kt"]["spmv32"], 32)
+    # dominant kernel of the measured (fast) path
+    fast_k = {k: v[0] for k, v in fast["kt"].items()}
+    dominant = max(fast_k, key=fast_k.get)
+
+    cpu = None
+    if cpu_steps:
+        log(f"[bench] CPU baseline: FP64 oracle, {cpu_steps} coupled steps of the same workload")
+        cb = cpu_baseline(case, abi, cpu_steps)
+        parity = float(np.max(np.abs(interior3(cb["h"], case.nx, case.ny, case.nz) -
+                                     interior3(plain["grabbed"], case.nx, case.ny, case.nz))))
+        cpu = {"value": round(case.cells * cpu_steps / cb["elapsed"], 1), "unit": "cell-steps/s",
+               "cores": cb["threads"], "kind": "port",
+               "sample": f"first {cpu_steps} coupled steps of the same {args.nx}x{args.ny}x{args.nz} workload "
+                         f"({cb['elapsed']:.1f} s), FP64 oracle with Jacobi-PCG at the reference rtol",
+               "gpu_fp64_max_head_err_vs_oracle_m": parity}
+        log(f"[bench]   {cb['elapsed']:.1f} s on {cb['threads']} threads; GPU FP64 vs oracle max |dh| = {parity:.3e} m")
+
+    value = case.cells * args.steps / fast["elapsed"]
+    plain_value = case.cells * args.steps / plain["elapsed"]
+    out = {
+        "metric": "coupled cell-steps/s at <=1e-6 m head err: learned+mixed vs plain FP64 GPU",
+        "value": round(value, 1),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fast["elapsed"] / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded Gaussian random fields: bed sigma 0.05 m / 20 m, ln Ks sigma 1 / 10 m)",
+        "config": {"workload": f"coupled SW {args.nx}x{args.ny} + GW {args.nx}x{args.ny}x{args.nz}, dx 1 m, "
+                               f"dz 0.05 m, dt 1 s, 50 mm/h rain, closed sides",
+                   "cells_per_step": case.cells, "fast_path": "fp32-in-fp64 refinement, previous-step guess",
+                   "gw_rtol": 1e-10, "sw_rtol": 1e-12},
+        "plain_fp64": {"value": round(plain_value, 1), "ms_per_step": round(plain["elapsed"] / args.steps * 1e3, 4),
+                       "krylov_iters_per_step": plain["iters"] / args.steps},
+        "speedup_vs_plain_fp64": round(value / plain_value, 3),
+        "max_head_err_vs_plain_fp64_m": head_err,
+        "fast_inner_iters_per_step": fast["iters"] / args.steps,
+        "fast_refinements_per_step": fast["refines"] / args.steps,
+        "roofline": roof32 if dominant == "spmv32" and roof32 else roof64,
+        "roofline_kernel": "gw_pcg_spmv fp32 (fused p-update + 7-pt SpMV + dot)" if dominant == "spmv32"
+                           else "gw_pcg_spmv fp64 (fused p-update + 7-pt SpMV + dot)",
+        "roofline_fp64_spmv": roof64,
+        "kernel_ms_fast": {k: round(v[0], 3) for k, v in fast["kt"].items()},
+        "kernel_ms_plain": {k: round(v[0], 3) for k, v in plain["kt"].items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -406,6 +406,8 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
   }
   if (host_ctl_->breakdown) throw ConvergenceError("Krylov: breakdown (p.Ap <= 0 or non-finite)");
   const int iters = host_ctl_->iters;
+  // each launched iteration recorded one spmv and one update launch
+  if (launched > iters) prof_->discard_tail(2 * (size_t)(launched - iters));
   if (sizeof(T) == 8) last_iters_ = iters;
   else last_inner_iters_ = iters;
   *res_norm = std::sqrt(host_ctl_->zz);

@@ -33,7 +33,17 @@ class Profiler {
     FREHG_HIP(hipEventRecord(e, stream));
     recs_.push_back({pending_kt_, pending_begin_, e});
     pending_kt_ = -1;
-    if (recs_.size() > 4096) fold();
+  }
+  // Drop the `n` most recent records (launches that returned immediately because the solve had
+  // already converged), so averages cover only launches that did work.
+  void discard_tail(size_t n) {
+    if (!enabled) return;
+    n = n < recs_.size() ? n : recs_.size();
+    for (size_t q = 0; q < n; ++q) {
+      free_.push_back(recs_.back().e0);
+      free_.push_back(recs_.back().e1);
+      recs_.pop_back();
+    }
   }
   // Sum recorded durations into the per-type totals (synchronises on the recorded events).
   void fold() {

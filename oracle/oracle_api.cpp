@@ -2,6 +2,7 @@
 // Loaded by tests/ and by bench.py's cpu_baseline leg through ctypes. Parameter structs are the
 // ones of the product ABI (include/frehg_gpu.h) so both sides are driven with identical inputs.
 #include <cstring>
+#include <omp.h>
 #include <exception>
 #include <stdexcept>
 #include <string>
@@ -138,6 +139,17 @@ int guard(Fn&& fn) {
 extern "C" {
 
 const char* orc_last_error(void) { return g_err.c_str(); }
+
+// OpenMP threads the oracle's parallel loops use.
+int orc_num_threads(void) {
+  int n = 1;
+#pragma omp parallel
+  {
+#pragma omp single
+    n = omp_get_num_threads();
+  }
+  return n;
+}
 
 void* orc_re_create(int nx, int ny, int nz, const frehg_re_params* p) {
   try {

