@@ -21,6 +21,7 @@ ERR_STATE = -6
 ERR_NO_DEVICE = -7
 
 KSP_CG = 0
+KSP_BICGSTAB = 1
 PC_JACOBI = 0
 PREC_FP64 = 0
 PREC_MIXED = 1

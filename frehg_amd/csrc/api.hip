@@ -340,7 +340,17 @@ int frehg_ls_solve(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, const doub
             x[(size_t)c] = b[(size_t)c] / D[(size_t)c];
           }
         }
-    // Fold decoupled rows into their neighbours; the remaining couplings must be symmetric.
+    // Fold decoupled rows into their neighbours. CG needs the remaining couplings symmetric;
+    // BiCGSTAB keeps each row's own -x/-y/-z couplings.
+    const frehg::KrylovConfig kc = frehg::krylov_config(*cfg, "frehg_ls_solve");
+    const bool nonsym = kc.ksp == FREHG_KSP_BICGSTAB;
+    std::vector<double> axm, aym, azm;
+    if (nonsym) {
+      axm.assign((size_t)n, 0.0);
+      aym.assign((size_t)n, 0.0);
+      azm.assign((size_t)n, 0.0);
+    }
+    std::vector<double>* dst[6] = {&axm, &ax, &aym, &ay, &azm, &az};
     for (int k = 0; k < NZ; ++k)
       for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
@@ -350,22 +360,22 @@ int frehg_ls_solve(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, const doub
             const double a = off[d][(size_t)c];
             if (a == 0.0) continue;
             const long long nb = c + step[d];
-            if (fixed[(size_t)nb]) b[(size_t)c] -= a * x[(size_t)nb];
-          }
-          const double* pairs[3] = {&ax[(size_t)c], &ay[(size_t)c], &az[(size_t)c]};
-          (void)pairs;
-          for (int d = 1; d < ndir; d += 2) {
-            const long long nb = c + step[d];
-            const double a = off[d][(size_t)c];
-            if (a == 0.0 || fixed[(size_t)nb]) continue;
-            const double back = off[d - 1][(size_t)nb];
-            if (std::fabs(a - back) > 1e-12 * (std::fabs(a) + std::fabs(back)))
-              throw frehg::ArgError("frehg_ls_solve: operator is not symmetric (CG requires A = A^T)");
-            (d == 1 ? ax : d == 3 ? ay : az)[(size_t)c] = a;
+            if (fixed[(size_t)nb]) {
+              b[(size_t)c] -= a * x[(size_t)nb];
+              continue;
+            }
+            if (nonsym) {
+              (*dst[d])[(size_t)c] = a;
+            } else if (d & 1) {
+              const double back = off[d - 1][(size_t)nb];
+              if (std::fabs(a - back) > 1e-12 * (std::fabs(a) + std::fabs(back)))
+                throw frehg::ArgError("frehg_ls_solve: operator is not symmetric (CG requires A = A^T)");
+              (*dst[d])[(size_t)c] = a;
+            }
           }
         }
     auto& s = ctx->stream;
-    double *dD, *dax, *day, *daz = nullptr, *db, *dbn, *dx;
+    double *dD, *dax, *day, *daz = nullptr, *db, *dbn, *dx, *daxm = nullptr, *daym = nullptr, *dazm = nullptr;
     unsigned char* dfixed = nullptr;
     const size_t bytes = (size_t)n * sizeof(double);
     std::vector<void*> allocs;
@@ -393,18 +403,23 @@ int frehg_ls_solve(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, const doub
     FREHG_HIP(hipMemcpy(db, b.data(), bytes, hipMemcpyHostToDevice));
     FREHG_HIP(hipMemcpy(dbn, bn.data(), bytes, hipMemcpyHostToDevice));
     FREHG_HIP(hipMemcpy(dx, x.data(), bytes, hipMemcpyHostToDevice));
+    if (nonsym) {
+      alloc(&daxm);
+      alloc(&daym);
+      FREHG_HIP(hipMemcpy(daxm, axm.data(), bytes, hipMemcpyHostToDevice));
+      FREHG_HIP(hipMemcpy(daym, aym.data(), bytes, hipMemcpyHostToDevice));
+      if (three) {
+        alloc(&dazm);
+        FREHG_HIP(hipMemcpy(dazm, azm.data(), bytes, hipMemcpyHostToDevice));
+      }
+    }
     if (!three) {
       FREHG_HIP(hipMalloc(&dfixed, (size_t)n));
       allocs.push_back(dfixed);
       FREHG_HIP(hipMemcpy(dfixed, fixed.data(), (size_t)n, hipMemcpyHostToDevice));
     }
     frehg::Krylov kry(g, three, s, &ctx->prof);
-    frehg::KrylovConfig kc;
-    kc.rtol = cfg->rtol;
-    kc.atol = cfg->atol;
-    kc.max_it = cfg->max_it;
-    kc.precision = 0;
-    frehg::StencilView<double> A{dD, dax, day, daz};
+    frehg::StencilView<double> A{dD, dax, day, daz, daxm, daym, dazm};
     const frehg::SolveResult sr = kry.solve(A, nullptr, db, dbn, dx, dfixed, true, false, kc);
     FREHG_HIP(hipMemcpy(x.data(), dx, bytes, hipMemcpyDeviceToHost));
     for (int k = 0; k < NZ; ++k)

@@ -73,22 +73,47 @@ __device__ inline double block_sum_partials(const double* part, int n, double* l
   return block_sum<NT>(s, lds);
 }
 
-// "Last block done" detection: every block stores its partials, fences, then bumps a counter;
-// the block that observes count == nblocks-1 owns the final reduction and resets the counter.
-__device__ inline bool last_block(unsigned int* counter, unsigned int nblocks, int* lds_flag) {
-  __threadfence();
+// One-block reductions of per-block partials (launched as <<<1, kRedThreads>>> after the
+// kernel that wrote the partials). Cross-block completion is ordered by the stream, not by
+// device-scope atomics, whose serialisation across the XCDs would cost ~0.1 us per block.
+constexpr int kRedThreads = 1024;
+
+__global__ __launch_bounds__(kRedThreads) inline void k_reduce_sum(const double* part, unsigned int n, int nsums,
+                                                                     double* out) {
+  __shared__ double lds[kRedThreads / kWave];
+  for (int q = 0; q < nsums; ++q) {
+    const double v = block_sum_partials<kRedThreads>(part + (size_t)q * n, (int)n, lds);
+    if (threadIdx.x == 0) out[q] = v;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) inline void k_reduce_maxmin(const double* part, unsigned int n,
+                                                                        double* out) {
+  __shared__ double lds[kRedThreads / kWave];
+  double a = 0.0, b = 1.0e300;
+  for (unsigned int i = threadIdx.x; i < n; i += kRedThreads) {
+    a = fmax(a, part[i]);
+    b = fmin(b, part[n + i]);
+  }
+  a = wave_max(a);
+  b = wave_min(b);
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  if (lane == 0) lds[w] = a;
   __syncthreads();
   if (threadIdx.x == 0) {
-    const unsigned int prev = atomicAdd(counter, 1u);
-    *lds_flag = (prev == nblocks - 1) ? 1 : 0;
+    double m = lds[0];
+    for (int i = 1; i < kRedThreads / kWave; ++i) m = fmax(m, lds[i]);
+    out[0] = m;
   }
   __syncthreads();
-  const bool last = *lds_flag != 0;
-  if (last) {
-    __threadfence();
-    if (threadIdx.x == 0) atomicExch(counter, 0u);
+  if (lane == 0) lds[w] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = lds[0];
+    for (int i = 1; i < kRedThreads / kWave; ++i) m = fmin(m, lds[i]);
+    out[1] = m;
   }
-  return last;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -99,6 +124,7 @@ struct SoilClass {
   double m;      // 1 - 1/n
   double wcm;    // VanGenuchten::wcm (VanGenuchten.hpp:21-28)
   double capc;   // alpha * n * m * (wcm - theta_r)
+  double inv_m, inv_n, inv_alpha;
 };
 
 inline SoilClass make_soil_class(double alpha, double n, double theta_s, double theta_r, double Ks_x,
@@ -119,11 +145,13 @@ struct SoilView {
 
 // ---------------------------------------------------------------------------------------------
 // Van Genuchten relations (src/re/VanGenuchten.hpp:18-137). The plain-VG path evaluates
-// p = |alpha h|^n and Se = (1+p)^-m once and derives theta, C and the relative conductivity
-// from them, using the identities |alpha h|^(n-1) = p/|alpha h|, (1+p)^-(m+1) = Se/(1+p) and
-// Se^(1/m) = 1/(1+p) (so 1 - Se^(1/m) = p/(1+p)): 3 pow per cell instead of the ~10 that the
-// separate functions take. Modified-VG and non-VG soils use the reference expressions.
-// kr is K/Ks: K(h, Ks) = Ks * kr(h) with kr capped at 1 and = 1 above the cut-off.
+// L = log|alpha h|, lp = n L = log p (p = |alpha h|^n) and l1p = log1p(p) once and derives
+//   sqrt(Se) = exp(-m l1p / 2), Se = sqrt(Se)^2           (Se = (1+p)^-m),
+//   C = alpha n m (wcm - theta_r) p Se / (|alpha h| (1+p))  (|alpha h|^(n-1) = p/|alpha h|),
+//   kr = sqrt(Se) t^2, t = 1 - (p/(1+p))^m = -expm1(m (lp - l1p))   (Se^(1/m) = 1/(1+p)),
+// i.e. log, exp, log1p, exp, expm1 and one division instead of ~10 pow, and without the
+// cancellation of the pow form near saturation (p << 1). Modified-VG and non-VG soils use the
+// reference expressions. kr is K/Ks: K(h, Ks) = Ks kr(h), kr <= 1 and kr = 1 above the cut-off.
 struct VGState {
   double theta, cap, kr;
 };
@@ -175,38 +203,53 @@ __device__ inline double vg_cap_ref(const SoilClass& s, double h) {  // VanGenuc
   return c;
 }
 
-__device__ inline VGState vg_state(const SoilClass& s, double h) {
+__device__ __noinline__ VGState vg_state_ref(const SoilClass& s, double h) {
   VGState o;
-  if (!s.use_vg || s.use_mvg) {
-    o.theta = vg_theta_ref(s, h);
-    o.cap = vg_cap_ref(s, h);
-    o.kr = vg_kr_ref(s, h);
-    return o;
-  }
+  o.theta = vg_theta_ref(s, h);
+  o.cap = vg_cap_ref(s, h);
+  o.kr = vg_kr_ref(s, h);
+  return o;
+}
+
+__device__ __forceinline__ VGState vg_state(const SoilClass& s, double h) {
+  if (!s.use_vg || s.use_mvg) return vg_state_ref(s, h);
+  VGState o;
   const double a = fabs(s.alpha * h);
-  const double p = (a == 0.0) ? 0.0 : pow(a, s.n);
-  const double an1 = (a == 0.0) ? 0.0 : p / a;
-  const double onep = 1.0 + p;
-  const double se = pow(onep, -s.m);
+  double se, cap, kr;
+  if (a == 0.0) {
+    se = 1.0;
+    cap = 0.0;
+    kr = 1.0;
+  } else {
+    const double lp = s.n * log(a);
+    const double p = exp(lp);
+    const double l1p = log1p(p);
+    const double sq = exp(-0.5 * s.m * l1p);
+    se = sq * sq;
+    cap = s.capc * p * se / (a * (1.0 + p));
+    const double t = -expm1(s.m * (lp - l1p));
+    kr = sq * t * t;
+    if (kr > 1.0) kr = 1.0;
+  }
   double th = (h > s.air_entry) ? s.theta_s : s.theta_r + (s.wcm - s.theta_r) * se;
   if (th > s.theta_s) th = s.theta_s;
   if (th < s.theta_r) th = s.theta_r;
   o.theta = th;
-  o.cap = (h > 0.0) ? 0.0 : s.capc * an1 * se / onep;
-  const double t = 1.0 - pow(p / onep, s.m);
-  double kr = sqrt(se) * t * t;
-  if (kr > 1.0) kr = 1.0;
-  if (h > 0.0) kr = 1.0;
-  o.kr = kr;
+  o.cap = (h > 0.0) ? 0.0 : cap;
+  o.kr = (h > 0.0) ? 1.0 : kr;
   return o;
 }
 
-__device__ inline double vg_head(const SoilClass& s, double wc) {  // VanGenuchten.hpp:50-65
+// h(theta) (VanGenuchten.hpp:50-65): x = (wcm - theta_r)/(theta - theta_r) >= 1,
+// h = -(1/alpha) (x^(1/m) - 1)^(1/n) with x^(1/m) - 1 = expm1(log(x)/m).
+__device__ __forceinline__ double vg_head(const SoilClass& s, double wc) {
   if (s.use_vg) {
     double th = wc;
     if (th - s.theta_r < 1.0e-7) th = s.theta_r + 1.0e-7;
-    if (th < s.theta_s)
-      return -(1.0 / s.alpha) * pow(pow((s.wcm - s.theta_r) / (th - s.theta_r), 1.0 / s.m) - 1.0, 1.0 / s.n);
+    if (th < s.theta_s) {
+      const double y = expm1(log((s.wcm - s.theta_r) / (th - s.theta_r)) * s.inv_m);
+      return -s.inv_alpha * exp(log(y) * s.inv_n);
+    }
     return 0.0;
   }
   return log((wc - s.theta_r) / (s.theta_s - s.theta_r)) / 0.1634;
@@ -252,6 +295,9 @@ inline SoilClass make_soil_class(double alpha, double n, double theta_s, double 
   s.wcm = use_mvg ? theta_r + (theta_s - theta_r) * std::pow(1.0 + std::pow(std::fabs(air_entry) * alpha, n), s.m)
                   : theta_s;
   s.capc = alpha * n * s.m * (s.wcm - theta_r);
+  s.inv_m = 1.0 / s.m;
+  s.inv_n = 1.0 / n;
+  s.inv_alpha = 1.0 / alpha;
   return s;
 }
 

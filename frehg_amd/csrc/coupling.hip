@@ -18,10 +18,8 @@ struct CpK {
 };
 
 __global__ __launch_bounds__(NT) void k_exchange(CpK P, double* eta, const double* bottom, double* dept, double* wc,
-                                                 double* h, double* q_out, double* part, unsigned int* counter,
-                                                 unsigned int nblocks, double* vol) {
+                                                 double* h, double* q_out, double* part) {
   __shared__ double lds[NT / kWave];
-  __shared__ int flag;
   const int i = blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
   double qdt = 0.0;
@@ -60,18 +58,13 @@ __global__ __launch_bounds__(NT) void k_exchange(CpK P, double* eta, const doubl
   const double sblk = block_sum<NT>(qdt, lds);
   const unsigned int bid = blockIdx.x + gridDim.x * blockIdx.y;
   if (threadIdx.x == 0) part[bid] = sblk;
-  if (last_block(counter, nblocks, &flag)) {
-    const double tot = block_sum_partials<NT>(part, nblocks, lds);
-    if (threadIdx.x == 0) vol[0] = tot;
-  }
 }
 
 // Orchestrator::computeWaterVolume (src/driver/Orchestrator.cpp:898-936).
 __global__ __launch_bounds__(NT) void k_volume(Geo gs, Geo gg, const double* eta, const double* bottom,
                                                const double* wc, double area, double vol, double* part,
-                                               unsigned int* counter, unsigned int nblocks, double* out) {
+                                               unsigned int nblocks) {
   __shared__ double lds[NT / kWave];
-  __shared__ int flag;
   const int i = blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
   double sw = 0.0, gw = 0.0;
@@ -90,14 +83,6 @@ __global__ __launch_bounds__(NT) void k_volume(Geo gs, Geo gg, const double* eta
   if (threadIdx.x == 0) {
     part[bid] = a;
     part[nblocks + bid] = b;
-  }
-  if (last_block(counter, nblocks, &flag)) {
-    const double ta = block_sum_partials<NT>(part, nblocks, lds);
-    const double tb = block_sum_partials<NT>(part + nblocks, nblocks, lds);
-    if (threadIdx.x == 0) {
-      out[0] = ta;
-      out[1] = tb;
-    }
   }
 }
 
@@ -149,7 +134,9 @@ double Coupler::exchange(SweState& swe, ReState& re, const frehg_coupling_params
   const unsigned int nb = grid.x * grid.y;
   ensure(nb, (long long)gs.nx * gs.ny);
   hipLaunchKernelGGL(k_exchange, grid, blk, 0, s_, P, swe.d("eta"), swe.d("bottom"), swe.d("dept"), re.d_wc(),
-                     re.d_h(), q_out ? qdev_ : nullptr, part_, counter_, nb, dres_);
+                     re.d_h(), q_out ? qdev_ : nullptr, part_);
+  FREHG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kRedThreads), 0, s_, (const double*)part_, nb, 1, dres_);
   FREHG_LAUNCH_CHECK();
   FREHG_HIP(hipMemcpyAsync(hres_, dres_, sizeof(double), hipMemcpyDeviceToHost, s_));
   if (q_out)
@@ -168,7 +155,9 @@ void Coupler::volume(SweState* swe, ReState* re, double* surface, double* subsur
   ensure(nb, 1);
   hipLaunchKernelGGL(k_volume, grid, blk, 0, s_, g, gg, swe ? swe->d("eta") : nullptr,
                      swe ? swe->d("bottom") : nullptr, re ? re->d_wc() : nullptr, dx * dy,
-                     re ? dx * dy * re->params().dz : 0.0, part_, counter_ + 1, nb, dres_);
+                     re ? dx * dy * re->params().dz : 0.0, part_, nb);
+  FREHG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kRedThreads), 0, s_, (const double*)part_, nb, 2, dres_);
   FREHG_LAUNCH_CHECK();
   FREHG_HIP(hipMemcpyAsync(hres_, dres_, 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
   FREHG_HIP(hipStreamSynchronize(s_));

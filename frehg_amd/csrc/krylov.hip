@@ -1,5 +1,5 @@
-// Jacobi-PCG kernels for the 7-/5-point stencil systems of the Richards and shallow-water
-// steps. One iteration is two kernels:
+// Jacobi-PCG and Jacobi-BiCGSTAB kernels for the 7-/5-point stencil systems of the Richards and
+// shallow-water steps. One CG iteration is two kernels:
 //   spmv:   p = z + beta p_prev (recomputed at each stencil point), q = A p, partial p.q;
 //           last block: alpha = rz / p.q
 //   update: x += alpha p, r -= alpha q, z = r / D, partials r.z and z.z;
@@ -7,8 +7,16 @@
 // Each thread owns one (i, j) column of a 64 x 4 tile and marches down k over a chunk of
 // layers, keeping the k-1 / k / k+1 values of p and the az coupling of the layer above in
 // registers, so every array is streamed from HBM once per iteration.
+// One BiCGSTAB iteration (right Jacobi preconditioning, y = D^-1 p, z = D^-1 s) is three kernels:
+//   spmv1:  p = r + beta (p_prev - omega v_prev) and y = p / D at each stencil point, v = A y,
+//           partial rh.v;                        fin: alpha = rho / rh.v
+//   spmv2:  s = r - alpha v and z = s / D at each stencil point, t = A z, partials t.s, t.t,
+//           ||D^-1 s||^2;                        fin: omega = t.s / t.t (0 once s has converged)
+//   update: x += alpha y + omega z, r = s - omega t, partials rh.r, ||D^-1 r||^2;
+//                                                fin: convergence test, beta, rho
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "krylov.hpp"
 
@@ -53,15 +61,44 @@ __device__ inline double apply_at(const Geo& g, const StencilView<T>& A, const T
   return y;
 }
 
+// -x/-y/-z couplings of row c: the row's own (nonsymmetric operators) or the neighbour's +x/+y/+z.
+template <typename T>
+__device__ inline double cxm(const StencilView<T>& A, long long c) {
+  return A.axm ? (double)A.axm[c] : (double)A.ax[c - 1];
+}
+template <typename T>
+__device__ inline double cym(const StencilView<T>& A, long long c, long long sy) {
+  return A.aym ? (double)A.aym[c] : (double)A.ay[c - sy];
+}
+template <typename T>
+__device__ inline double czm(const StencilView<T>& A, long long c, long long sz) {
+  return A.azm ? (double)A.azm[c] : (double)A.az[c - sz];
+}
+
+// (A v)(c) for cell (i, j, k) of the domain; V(cc) is read only at in-domain stencil points, so
+// it may divide by D without touching ghost cells.
+template <typename T, bool THREE_D, typename F>
+__device__ inline double apply_in_domain(const Geo& g, const StencilView<T>& A, long long c, int i, int j, int k,
+                                         double vc, F&& V) {
+  double y = (double)A.D[c] * vc;
+  if (i + 1 < g.nx) y += (double)A.ax[c] * V(c + 1);
+  if (i > 0) y += cxm(A, c) * V(c - 1);
+  if (j + 1 < g.ny) y += (double)A.ay[c] * V(c + g.sy);
+  if (j > 0) y += cym(A, c, g.sy) * V(c - g.sy);
+  if (THREE_D) {
+    if (k + 1 < g.nz) y += (double)A.az[c] * V(c + g.sz);
+    if (k > 0) y += czm(A, c, g.sz) * V(c - g.sz);
+  }
+  return y;
+}
+
 // r = b - A x (or b when x0 = 0), z = r / D, p_prev = 0; partials r.z, z.z, (bn/D)^2.
 template <typename T, bool THREE_D>
 __global__ __launch_bounds__(NT) void k_init(Geo g, int kchunk, StencilView<T> A, const T* b,
                                               const T* bn, T* x, T* r, T* z, T* pprev,
                                               const unsigned char* fixed, int x0_zero,
-                                              double* part, KrylovCtl* ctl, unsigned int nblocks,
-                                              double rtol, double atol, int max_it, int guard) {
+                                              double* part, unsigned int nblocks) {
   __shared__ double lds[NT / kWave];
-  __shared__ int flag;
   const Tile t = tile_of(g, kchunk);
   double srz = 0.0, szz = 0.0, sbb = 0.0;
   if (t.valid) {
@@ -82,8 +119,7 @@ __global__ __launch_bounds__(NT) void k_init(Geo g, int kchunk, StencilView<T> A
       sbb += bz * bz;
     }
   }
-  // x0 = 0 is applied after every thread has read its neighbours' x (only when x0_zero, in
-  // which case x is not read at all).
+  // x0 = 0 (only when x0_zero, in which case x is not read at all)
   if (x0_zero && t.valid) {
     long long c = base_index<THREE_D>(g, t);
     for (int k = t.k0; k < t.k1; ++k, c += g.sz)
@@ -98,35 +134,38 @@ __global__ __launch_bounds__(NT) void k_init(Geo g, int kchunk, StencilView<T> A
     part[nblocks + bid] = bsum;
     part[2 * nblocks + bid] = cs;
   }
-  if (last_block(&ctl->counter[0], nblocks, &flag)) {
-    const double rz = block_sum_partials<NT>(part, nblocks, lds);
-    const double zz = block_sum_partials<NT>(part + nblocks, nblocks, lds);
-    const double bb = block_sum_partials<NT>(part + 2 * nblocks, nblocks, lds);
-    if (threadIdx.x == 0) {
-      const double ttol = fmax(rtol * sqrt(bb), atol);
-      ctl->rz = rz;
-      ctl->zz = zz;
-      ctl->bb = bb;
-      ctl->ttol2 = ttol * ttol;
-      ctl->iters = 0;
-      ctl->beta = 0.0;
-      ctl->alpha = 0.0;
-      ctl->breakdown = 0;
-      ctl->max_it = max_it;
-      ctl->rr_guess = zz;
-      ctl->guess_rejected = (guard && !x0_zero && !(zz < bb)) ? 1 : 0;
-      ctl->converged = (zz <= ctl->ttol2) ? 1 : 0;
-      ctl->done = (ctl->converged || ctl->guess_rejected) ? 1 : 0;
-    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_fin_init(const double* part, unsigned int nblocks, KrylovCtl* ctl,
+                                                          double rtol, double atol, int max_it, int guard,
+                                                          int x0_zero) {
+  __shared__ double lds[kRedThreads / kWave];
+  const double rz = block_sum_partials<kRedThreads>(part, nblocks, lds);
+  const double zz = block_sum_partials<kRedThreads>(part + nblocks, nblocks, lds);
+  const double bb = block_sum_partials<kRedThreads>(part + 2 * nblocks, nblocks, lds);
+  if (threadIdx.x == 0) {
+    const double ttol = fmax(rtol * sqrt(bb), atol);
+    ctl->rz = rz;
+    ctl->zz = zz;
+    ctl->bb = bb;
+    ctl->ttol2 = ttol * ttol;
+    ctl->iters = 0;
+    ctl->beta = 0.0;
+    ctl->alpha = 0.0;
+    ctl->breakdown = 0;
+    ctl->max_it = max_it;
+    ctl->rr_guess = zz;
+    ctl->guess_rejected = (guard && !x0_zero && !(zz < bb)) ? 1 : 0;
+    ctl->converged = (zz <= ctl->ttol2) ? 1 : 0;
+    ctl->done = (ctl->converged || ctl->guess_rejected) ? 1 : 0;
   }
 }
 
 template <typename T, bool THREE_D>
 __global__ __launch_bounds__(NT) void k_spmv(Geo g, int kchunk, StencilView<T> A, const T* z,
                                               const T* pold, T* pnew, T* q, double* part,
-                                              KrylovCtl* ctl, unsigned int nblocks) {
+                                              const KrylovCtl* ctl) {
   __shared__ double lds[NT / kWave];
-  __shared__ int flag;
   if (ctl->done) return;
   const double beta = ctl->beta;
   const Tile t = tile_of(g, kchunk);
@@ -161,26 +200,28 @@ __global__ __launch_bounds__(NT) void k_spmv(Geo g, int kchunk, StencilView<T> A
   }
   const double s = block_sum<NT>(acc, lds);
   if (threadIdx.x == 0) part[block_id()] = s;
-  if (last_block(&ctl->counter[1], nblocks, &flag)) {
-    const double pq = block_sum_partials<NT>(part, nblocks, lds);
-    if (threadIdx.x == 0) {
-      ctl->pq = pq;
-      if (!(pq > 0.0) || !isfinite(pq)) {
-        ctl->breakdown = 1;
-        ctl->done = 1;
-      } else {
-        ctl->alpha = ctl->rz / pq;
-      }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_fin_spmv(const double* part, unsigned int nblocks, KrylovCtl* ctl) {
+  __shared__ double lds[kRedThreads / kWave];
+  if (ctl->done) return;
+  const double pq = block_sum_partials<kRedThreads>(part, nblocks, lds);
+  if (threadIdx.x == 0) {
+    ctl->pq = pq;
+    if (!(pq > 0.0) || !isfinite(pq)) {
+      ctl->breakdown = 1;
+      ctl->done = 1;
+    } else {
+      ctl->alpha = ctl->rz / pq;
     }
   }
 }
 
 template <typename T, bool THREE_D>
 __global__ __launch_bounds__(NT) void k_update(Geo g, int kchunk, const T* D, const T* p, const T* q,
-                                                T* x, T* r, T* z, double* part, KrylovCtl* ctl,
+                                                T* x, T* r, T* z, double* part, const KrylovCtl* ctl,
                                                 unsigned int nblocks) {
   __shared__ double lds[NT / kWave];
-  __shared__ int flag;
   if (ctl->done) return;
   const double alpha = ctl->alpha;
   const Tile t = tile_of(g, kchunk);
@@ -206,37 +247,39 @@ __global__ __launch_bounds__(NT) void k_update(Geo g, int kchunk, const T* D, co
     part[bid] = a;
     part[nblocks + bid] = b;
   }
-  if (last_block(&ctl->counter[2], nblocks, &flag)) {
-    const double rz = block_sum_partials<NT>(part, nblocks, lds);
-    const double zz = block_sum_partials<NT>(part + nblocks, nblocks, lds);
-    if (threadIdx.x == 0) {
-      const int it = ctl->iters + 1;
-      ctl->iters = it;
-      ctl->zz = zz;
-      if (zz <= ctl->ttol2) {
-        ctl->converged = 1;
-        ctl->done = 1;
-      } else if (it >= ctl->max_it) {
-        ctl->done = 1;
-      } else {
-        ctl->beta = rz / ctl->rz;
-      }
-      ctl->rz = rz;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_fin_update(const double* part, unsigned int nblocks,
+                                                            KrylovCtl* ctl) {
+  __shared__ double lds[kRedThreads / kWave];
+  if (ctl->done) return;
+  const double rz = block_sum_partials<kRedThreads>(part, nblocks, lds);
+  const double zz = block_sum_partials<kRedThreads>(part + nblocks, nblocks, lds);
+  if (threadIdx.x == 0) {
+    const int it = ctl->iters + 1;
+    ctl->iters = it;
+    ctl->zz = zz;
+    if (zz <= ctl->ttol2) {
+      ctl->converged = 1;
+      ctl->done = 1;
+    } else if (it >= ctl->max_it) {
+      ctl->done = 1;
+    } else {
+      ctl->beta = rz / ctl->rz;
     }
+    ctl->rz = rz;
   }
 }
 
 // Iterative-refinement residual: xout = xin + d (d may be null), r32 = b - A xout in FP64,
 // partial ||D^-1 r||^2 (and ||D^-1 bn||^2 on the first sweep, which also sets the test).
-template <bool THREE_D>
+template <bool THREE_D, bool SYM>
 __global__ __launch_bounds__(NT) void k_refine(Geo g, int kchunk, StencilView<double> A,
                                                 const double* b, const double* bn, const double* xin,
                                                 const float* d, double* xout, float* r32,
                                                 const unsigned char* fixed, double* part,
-                                                KrylovCtl* ctl, unsigned int nblocks, int first,
-                                                double rtol, double atol, int guard) {
+                                                unsigned int nblocks, int first) {
   __shared__ double lds[NT / kWave];
-  __shared__ int flag;
   const Tile t = tile_of(g, kchunk);
   double srr = 0.0, sbb = 0.0;
   if (t.valid) {
@@ -245,9 +288,14 @@ __global__ __launch_bounds__(NT) void k_refine(Geo g, int kchunk, StencilView<do
     for (int k = t.k0; k < t.k1; ++k, c += g.sz) {
       const bool fx = fixed != nullptr && fixed[c] != 0;
       const double xc = X(c);
-      double y = A.D[c] * xc + A.ax[c] * X(c + 1) + A.ax[c - 1] * X(c - 1) + A.ay[c] * X(c + g.sy) +
-                 A.ay[c - g.sy] * X(c - g.sy);
-      if (THREE_D) y += A.az[c] * X(c + g.sz) + A.az[c - g.sz] * X(c - g.sz);
+      double y;
+      if (SYM) {
+        y = A.D[c] * xc + A.ax[c] * X(c + 1) + A.ax[c - 1] * X(c - 1) + A.ay[c] * X(c + g.sy) +
+            A.ay[c - g.sy] * X(c - g.sy);
+        if (THREE_D) y += A.az[c] * X(c + g.sz) + A.az[c - g.sz] * X(c - g.sz);
+      } else {
+        y = apply_in_domain<double, THREE_D>(g, A, c, t.i, t.j, k, xc, X);
+      }
       const double rv = fx ? 0.0 : b[c] - y;
       if (d) xout[c] = xc;
       r32[c] = (float)rv;
@@ -266,20 +314,256 @@ __global__ __launch_bounds__(NT) void k_refine(Geo g, int kchunk, StencilView<do
     part[bid] = a;
     part[nblocks + bid] = bsum;
   }
-  if (last_block(&ctl->counter[3], nblocks, &flag)) {
-    const double rr = block_sum_partials<NT>(part, nblocks, lds);
-    const double bb = first ? block_sum_partials<NT>(part + nblocks, nblocks, lds) : 0.0;
-    if (threadIdx.x == 0) {
-      if (first) {
-        const double ttol = fmax(rtol * sqrt(bb), atol);
-        ctl->bb = bb;
-        ctl->ttol2 = ttol * ttol;
-        ctl->rr_guess = rr;
-        ctl->guess_rejected = (guard && !(rr < bb)) ? 1 : 0;
-      }
-      ctl->zz = rr;
-      ctl->converged = (rr <= ctl->ttol2) ? 1 : 0;
-      ctl->done = ctl->converged;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_fin_refine(const double* part, unsigned int nblocks, KrylovCtl* ctl,
+                                                            int first, double rtol, double atol, int guard) {
+  __shared__ double lds[kRedThreads / kWave];
+  const double rr = block_sum_partials<kRedThreads>(part, nblocks, lds);
+  const double bb = first ? block_sum_partials<kRedThreads>(part + nblocks, nblocks, lds) : 0.0;
+  if (threadIdx.x == 0) {
+    if (first) {
+      const double ttol = fmax(rtol * sqrt(bb), atol);
+      ctl->bb = bb;
+      ctl->ttol2 = ttol * ttol;
+      ctl->rr_guess = rr;
+      ctl->guess_rejected = (guard && !(rr < bb)) ? 1 : 0;
+    }
+    ctl->zz = rr;
+    ctl->converged = (rr <= ctl->ttol2) ? 1 : 0;
+    ctl->done = ctl->converged;
+  }
+}
+
+// ---- BiCGSTAB ---------------------------------------------------------------------------
+
+// r = b - A x (or b when x0 = 0), rh = r, p_prev = v_prev = 0; partials rh.r, ||D^-1 r||^2,
+// ||D^-1 bn||^2.
+template <typename T, bool THREE_D>
+__global__ __launch_bounds__(NT) void k_bcg_init(Geo g, int kchunk, StencilView<T> A, const T* b, const T* bn,
+                                                  T* x, T* r, T* rh, T* p, T* v, const unsigned char* fixed,
+                                                  int x0_zero, double* part, unsigned int nblocks) {
+  __shared__ double lds[NT / kWave];
+  const Tile t = tile_of(g, kchunk);
+  double srr = 0.0, szz = 0.0, sbb = 0.0;
+  if (t.valid) {
+    long long c = base_index<THREE_D>(g, t);
+    for (int k = t.k0; k < t.k1; ++k, c += g.sz) {
+      const bool fx = fixed != nullptr && fixed[c] != 0;
+      double ax = 0.0;
+      if (!x0_zero)
+        ax = apply_in_domain<T, THREE_D>(g, A, c, t.i, t.j, k, (double)x[c],
+                                         [&](long long cc) { return (double)x[cc]; });
+      const T rT = (T)(fx ? 0.0 : (double)b[c] - ax);
+      r[c] = rT;
+      rh[c] = rT;
+      p[c] = (T)0;
+      v[c] = (T)0;
+      const double zr = (double)rT / (double)A.D[c];
+      srr += (double)rT * (double)rT;
+      szz += zr * zr;
+      const double bz = (double)(bn ? bn[c] : b[c]) / (double)A.D[c];
+      sbb += bz * bz;
+    }
+  }
+  if (x0_zero && t.valid) {
+    long long c = base_index<THREE_D>(g, t);
+    for (int k = t.k0; k < t.k1; ++k, c += g.sz)
+      if (!(fixed != nullptr && fixed[c] != 0)) x[c] = (T)0;
+  }
+  const double a = block_sum<NT>(srr, lds);
+  const double bsum = block_sum<NT>(szz, lds);
+  const double cs = block_sum<NT>(sbb, lds);
+  const unsigned int bid = block_id();
+  if (threadIdx.x == 0) {
+    part[bid] = a;
+    part[nblocks + bid] = bsum;
+    part[2 * nblocks + bid] = cs;
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_fin_bcg_init(const double* part, unsigned int nblocks,
+                                                              KrylovCtl* ctl, double rtol, double atol, int max_it,
+                                                              int guard, int x0_zero) {
+  __shared__ double lds[kRedThreads / kWave];
+  const double rho = block_sum_partials<kRedThreads>(part, nblocks, lds);
+  const double zz = block_sum_partials<kRedThreads>(part + nblocks, nblocks, lds);
+  const double bb = block_sum_partials<kRedThreads>(part + 2 * nblocks, nblocks, lds);
+  if (threadIdx.x == 0) {
+    const double ttol = fmax(rtol * sqrt(bb), atol);
+    ctl->rz = rho;
+    ctl->zz = zz;
+    ctl->bb = bb;
+    ctl->ttol2 = ttol * ttol;
+    ctl->iters = 0;
+    ctl->beta = 0.0;  // p = r on the first iteration (p_prev = v_prev = 0)
+    ctl->alpha = 1.0;
+    ctl->omega = 1.0;
+    ctl->pq = 0.0;
+    ctl->half = 0;
+    ctl->breakdown = 0;
+    ctl->max_it = max_it;
+    ctl->rr_guess = zz;
+    ctl->guess_rejected = (guard && !x0_zero && !(zz < bb)) ? 1 : 0;
+    ctl->converged = (zz <= ctl->ttol2) ? 1 : 0;
+    ctl->done = (ctl->converged || ctl->guess_rejected) ? 1 : 0;
+    if (!ctl->done && (rho == 0.0 || !isfinite(rho))) {
+      ctl->breakdown = 3;
+      ctl->done = 1;
+    }
+  }
+}
+
+template <typename T, bool THREE_D>
+__global__ __launch_bounds__(NT) void k_bcg_spmv1(Geo g, int kchunk, StencilView<T> A, const T* r, const T* rh,
+                                                   const T* pold, const T* vold, T* pnew, T* vnew, double* part,
+                                                   const KrylovCtl* ctl) {
+  __shared__ double lds[NT / kWave];
+  if (ctl->done) return;
+  const double beta = ctl->beta, omega = ctl->omega;
+  const Tile t = tile_of(g, kchunk);
+  double acc = 0.0;
+  if (t.valid) {
+    auto P = [&](long long cc) -> T {
+      return (T)((double)r[cc] + beta * ((double)pold[cc] - omega * (double)vold[cc]));
+    };
+    auto Y = [&](long long cc) -> double { return (double)(T)((double)P(cc) / (double)A.D[cc]); };
+    long long c = base_index<THREE_D>(g, t);
+    for (int k = t.k0; k < t.k1; ++k, c += g.sz) {
+      const T pc = P(c);
+      const double yc = (double)(T)((double)pc / (double)A.D[c]);
+      const T vT = (T)apply_in_domain<T, THREE_D>(g, A, c, t.i, t.j, k, yc, Y);
+      pnew[c] = pc;
+      vnew[c] = vT;
+      acc += (double)rh[c] * (double)vT;
+    }
+  }
+  const double s = block_sum<NT>(acc, lds);
+  if (threadIdx.x == 0) part[block_id()] = s;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_fin_bcg1(const double* part, unsigned int nblocks, KrylovCtl* ctl) {
+  __shared__ double lds[kRedThreads / kWave];
+  if (ctl->done) return;
+  const double rv = block_sum_partials<kRedThreads>(part, nblocks, lds);
+  if (threadIdx.x == 0) {
+    ctl->pq = rv;
+    if (rv == 0.0 || !isfinite(rv)) {
+      ctl->breakdown = 1;
+      ctl->done = 1;
+    } else {
+      ctl->alpha = ctl->rz / rv;
+    }
+  }
+}
+
+template <typename T, bool THREE_D>
+__global__ __launch_bounds__(NT) void k_bcg_spmv2(Geo g, int kchunk, StencilView<T> A, const T* r, const T* v,
+                                                   T* s, T* tv, double* part, const KrylovCtl* ctl,
+                                                   unsigned int nblocks) {
+  __shared__ double lds[NT / kWave];
+  if (ctl->done) return;
+  const double alpha = ctl->alpha;
+  const Tile t = tile_of(g, kchunk);
+  double sts = 0.0, stt = 0.0, sss = 0.0;
+  if (t.valid) {
+    auto S = [&](long long cc) -> T { return (T)((double)r[cc] - alpha * (double)v[cc]); };
+    auto Z = [&](long long cc) -> double { return (double)(T)((double)S(cc) / (double)A.D[cc]); };
+    long long c = base_index<THREE_D>(g, t);
+    for (int k = t.k0; k < t.k1; ++k, c += g.sz) {
+      const T sc = S(c);
+      const double zc = (double)(T)((double)sc / (double)A.D[c]);
+      const T tT = (T)apply_in_domain<T, THREE_D>(g, A, c, t.i, t.j, k, zc, Z);
+      s[c] = sc;
+      tv[c] = tT;
+      sts += (double)tT * (double)sc;
+      stt += (double)tT * (double)tT;
+      const double zn = (double)sc / (double)A.D[c];
+      sss += zn * zn;
+    }
+  }
+  const double a = block_sum<NT>(sts, lds);
+  const double b = block_sum<NT>(stt, lds);
+  const double cc = block_sum<NT>(sss, lds);
+  const unsigned int bid = block_id();
+  if (threadIdx.x == 0) {
+    part[bid] = a;
+    part[nblocks + bid] = b;
+    part[2 * nblocks + bid] = cc;
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_fin_bcg2(const double* part, unsigned int nblocks, KrylovCtl* ctl) {
+  __shared__ double lds[kRedThreads / kWave];
+  if (ctl->done) return;
+  const double ts = block_sum_partials<kRedThreads>(part, nblocks, lds);
+  const double tt = block_sum_partials<kRedThreads>(part + nblocks, nblocks, lds);
+  const double ss = block_sum_partials<kRedThreads>(part + 2 * nblocks, nblocks, lds);
+  if (threadIdx.x == 0) {
+    ctl->iters = ctl->iters + 1;
+    if (ss <= ctl->ttol2) {
+      ctl->half = 1;  // x += alpha y, r = s
+      ctl->omega = 0.0;
+    } else {
+      ctl->omega = tt > 0.0 ? ts / tt : 0.0;
+    }
+  }
+}
+
+template <typename T, bool THREE_D>
+__global__ __launch_bounds__(NT) void k_bcg_update(Geo g, int kchunk, const T* D, const T* p, const T* s,
+                                                    const T* tv, const T* rh, T* x, T* r, double* part,
+                                                    const KrylovCtl* ctl, unsigned int nblocks) {
+  __shared__ double lds[NT / kWave];
+  if (ctl->done) return;
+  const double alpha = ctl->alpha, omega = ctl->omega;
+  const Tile t = tile_of(g, kchunk);
+  double srr = 0.0, szz = 0.0;
+  if (t.valid) {
+    long long c = base_index<THREE_D>(g, t);
+    for (int k = t.k0; k < t.k1; ++k, c += g.sz) {
+      const double d = (double)D[c];
+      const double yc = (double)(T)((double)p[c] / d);
+      const double zc = (double)(T)((double)s[c] / d);
+      const T xT = (T)((double)x[c] + (alpha * yc + omega * zc));
+      const T rT = (T)((double)s[c] - omega * (double)tv[c]);
+      x[c] = xT;
+      r[c] = rT;
+      srr += (double)rh[c] * (double)rT;
+      const double zr = (double)rT / d;
+      szz += zr * zr;
+    }
+  }
+  const double a = block_sum<NT>(srr, lds);
+  const double b = block_sum<NT>(szz, lds);
+  const unsigned int bid = block_id();
+  if (threadIdx.x == 0) {
+    part[bid] = a;
+    part[nblocks + bid] = b;
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_fin_bcg3(const double* part, unsigned int nblocks, KrylovCtl* ctl) {
+  __shared__ double lds[kRedThreads / kWave];
+  if (ctl->done) return;
+  const double rho = block_sum_partials<kRedThreads>(part, nblocks, lds);
+  const double rr = block_sum_partials<kRedThreads>(part + nblocks, nblocks, lds);
+  if (threadIdx.x == 0) {
+    ctl->zz = rr;
+    if (ctl->half || rr <= ctl->ttol2) {
+      ctl->converged = 1;
+      ctl->done = 1;
+    } else if (ctl->omega == 0.0) {
+      ctl->breakdown = 2;
+      ctl->done = 1;
+    } else if (ctl->iters >= ctl->max_it) {
+      ctl->done = 1;
+    } else if (rho == 0.0 || !isfinite(rho)) {
+      ctl->breakdown = 3;
+      ctl->done = 1;
+    } else {
+      ctl->beta = (rho / ctl->rz) * (ctl->alpha / ctl->omega);
+      ctl->rz = rho;
     }
   }
 }
@@ -301,14 +585,14 @@ Krylov::Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof)
     : g_(g), three_d_(three_d), s_(stream), prof_(prof) {
   if (!three_d_) g_.nz = 1;
   n_ = three_d_ ? g_.ncell() : g_.nsurf();
-  set_kchunk(three_d_ ? g_.nz : 1);
+  set_kchunks(three_d_ ? std::min(g_.nz, 4) : 1, 1);
   const size_t bytes = (size_t)n_ * sizeof(double);
   for (double** p : {&r_, &z_, &p0_, &p1_, &q_, &xw_}) {
     FREHG_HIP(hipMalloc(p, bytes));
     FREHG_HIP(hipMemsetAsync(*p, 0, bytes, s_));
   }
-  FREHG_HIP(hipMalloc(&ctl_, 2 * sizeof(KrylovCtl)));
-  FREHG_HIP(hipMemsetAsync(ctl_, 0, 2 * sizeof(KrylovCtl), s_));
+  FREHG_HIP(hipMalloc(&ctl_, 3 * sizeof(KrylovCtl)));
+  FREHG_HIP(hipMemsetAsync(ctl_, 0, 3 * sizeof(KrylovCtl), s_));
   FREHG_HIP(hipHostMalloc(&host_ctl_, sizeof(KrylovCtl), hipHostMallocDefault));
   FREHG_HIP(hipStreamSynchronize(s_));
 }
@@ -316,22 +600,65 @@ Krylov::Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof)
 Krylov::~Krylov() {
   for (void* p : {(void*)r_, (void*)z_, (void*)p0_, (void*)p1_, (void*)q_, (void*)xw_, (void*)b32_,
                   (void*)r32_, (void*)z32_, (void*)p032_, (void*)p132_, (void*)q32_, (void*)d32_,
-                  (void*)part_, (void*)ctl_})
+                  (void*)rh_, (void*)v0_, (void*)v1_, (void*)sv_, (void*)tv_, (void*)rh32_, (void*)v032_,
+                  (void*)v132_, (void*)sv32_, (void*)tv32_, (void*)part_, (void*)ctl_})
     if (p) (void)hipFree(p);
   if (host_ctl_) (void)hipHostFree(host_ctl_);
 }
 
-void Krylov::set_kchunk(int kc) {
-  kc = std::max(1, std::min(kc, three_d_ ? g_.nz : 1));
-  if (kc == kchunk_ && part_) return;
-  kchunk_ = kc;
-  grid_ = dim3(ceil_div(g_.nx, TX), ceil_div(g_.ny, TY), ceil_div(three_d_ ? g_.nz : 1, kchunk_));
-  const unsigned int nb = grid_.x * grid_.y * grid_.z;
-  if (nb > nblocks_) {
+void Krylov::set_kchunks(int kc_spmv, int kc_flat) {
+  const int nz = three_d_ ? g_.nz : 1;
+  kc_s_ = std::max(1, std::min(kc_spmv, nz));
+  kc_f_ = std::max(1, std::min(kc_flat, nz));
+  grid_s_ = dim3(ceil_div(g_.nx, TX), ceil_div(g_.ny, TY), ceil_div(nz, kc_s_));
+  grid_f_ = dim3(ceil_div(g_.nx, TX), ceil_div(g_.ny, TY), ceil_div(nz, kc_f_));
+  nb_s_ = grid_s_.x * grid_s_.y * grid_s_.z;
+  nb_f_ = grid_f_.x * grid_f_.y * grid_f_.z;
+  // partials sized for the largest grid any kernel (or candidate) can use: one layer per block
+  const unsigned int cap = ceil_div(g_.nx, TX) * ceil_div(g_.ny, TY) * (unsigned int)nz;
+  if (cap > part_cap_) {
     if (part_) FREHG_HIP(hipFree(part_));
-    FREHG_HIP(hipMalloc(&part_, 3 * (size_t)nb * sizeof(double)));
+    FREHG_HIP(hipMalloc(&part_, 3 * (size_t)cap * sizeof(double)));
+    part_cap_ = cap;
   }
-  nblocks_ = nb;
+}
+
+void Krylov::autotune(const StencilView<double>& A) {
+  tuned_ = true;
+  if (!three_d_) return;
+  KrylovCtl* tctl = ctl_ + 2;
+  FREHG_HIP(hipMemsetAsync(tctl, 0, sizeof(KrylovCtl), s_));
+  hipEvent_t e0, e1;
+  FREHG_HIP(hipEventCreate(&e0));
+  FREHG_HIP(hipEventCreate(&e1));
+  const int nz = g_.nz;
+  std::vector<int> cand;
+  for (int kc : {1, 2, 3, 4, 5, 6, 8, 10, 15, nz})
+    if (kc >= 1 && kc <= nz && std::find(cand.begin(), cand.end(), kc) == cand.end()) cand.push_back(kc);
+  int best = kc_s_;
+  float best_ms = 1e30f;
+  const int flat = kc_f_;
+  for (int kc : cand) {
+    set_kchunks(kc, flat);
+    for (int rep = 0; rep < 2; ++rep) {  // first launch warms caches / clocks
+      FREHG_HIP(hipEventRecord(e0, s_));
+      for (int it = 0; it < 3; ++it)
+        hipLaunchKernelGGL((k_spmv<double, true>), grid_s_, dim3(NT), 0, s_, g_, kc_s_, A, A.D,
+                           (const double*)p0_, p1_, q_, part_, tctl);
+      FREHG_LAUNCH_CHECK();
+      FREHG_HIP(hipEventRecord(e1, s_));
+      FREHG_HIP(hipEventSynchronize(e1));
+    }
+    float ms = 0.f;
+    FREHG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = kc;
+    }
+  }
+  set_kchunks(best, flat);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
 }
 
 void Krylov::ensure_mixed() {
@@ -340,6 +667,24 @@ void Krylov::ensure_mixed() {
   for (float** p : {&b32_, &r32_, &z32_, &p032_, &p132_, &q32_, &d32_}) {
     FREHG_HIP(hipMalloc(p, bytes));
     FREHG_HIP(hipMemsetAsync(*p, 0, bytes, s_));
+  }
+}
+
+void Krylov::ensure_bicgstab(bool fp32) {
+  if (fp32) {
+    if (rh32_) return;
+    const size_t bytes = (size_t)n_ * sizeof(float);
+    for (float** p : {&rh32_, &v032_, &v132_, &sv32_, &tv32_}) {
+      FREHG_HIP(hipMalloc(p, bytes));
+      FREHG_HIP(hipMemsetAsync(*p, 0, bytes, s_));
+    }
+  } else {
+    if (rh_) return;
+    const size_t bytes = (size_t)n_ * sizeof(double);
+    for (double** p : {&rh_, &v0_, &v1_, &sv_, &tv_}) {
+      FREHG_HIP(hipMalloc(p, bytes));
+      FREHG_HIP(hipMemsetAsync(*p, 0, bytes, s_));
+    }
   }
 }
 
@@ -354,14 +699,17 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
                     T* r, T* z, T* p0, T* p1, T* q, bool* rejected, double* res_norm,
                     double* rhs_norm, bool* converged, int kt_spmv) {
   const dim3 blk(NT);
-  const unsigned int nb = nblocks_;
+  const unsigned int nb = nb_f_;
   auto init = [&](bool zero) {
     if (three_d_)
-      hipLaunchKernelGGL((k_init<T, true>), grid_, blk, 0, s_, g_, kchunk_, A, b, bn, x, r, z, p0, fixed,
-                         zero ? 1 : 0, part_, ctl, nb, rtol, atol, max_it, guard ? 1 : 0);
+      hipLaunchKernelGGL((k_init<T, true>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, z, p0, fixed,
+                         zero ? 1 : 0, part_, nb);
     else
-      hipLaunchKernelGGL((k_init<T, false>), grid_, blk, 0, s_, g_, kchunk_, A, b, bn, x, r, z, p0, fixed,
-                         zero ? 1 : 0, part_, ctl, nb, rtol, atol, max_it, guard ? 1 : 0);
+      hipLaunchKernelGGL((k_init<T, false>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, z, p0, fixed,
+                         zero ? 1 : 0, part_, nb);
+    FREHG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(kRedThreads), 0, s_, part_, nb, ctl, rtol, atol, max_it,
+                       guard ? 1 : 0, zero ? 1 : 0);
     FREHG_LAUNCH_CHECK();
   };
   init(x0_zero);
@@ -384,18 +732,22 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
       T* pnew = pbuf[parity ^ 1];
       prof_->begin(kt_spmv);
       if (three_d_)
-        hipLaunchKernelGGL((k_spmv<T, true>), grid_, blk, 0, s_, g_, kchunk_, A, z, pold, pnew, q, part_, ctl, nb);
+        hipLaunchKernelGGL((k_spmv<T, true>), grid_s_, blk, 0, s_, g_, kc_s_, A, z, pold, pnew, q, part_, ctl);
       else
-        hipLaunchKernelGGL((k_spmv<T, false>), grid_, blk, 0, s_, g_, kchunk_, A, z, pold, pnew, q, part_, ctl, nb);
+        hipLaunchKernelGGL((k_spmv<T, false>), grid_s_, blk, 0, s_, g_, kc_s_, A, z, pold, pnew, q, part_, ctl);
       FREHG_LAUNCH_CHECK();
       prof_->end();
+      hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_s_, ctl);
+      FREHG_LAUNCH_CHECK();
       prof_->begin(2);
       if (three_d_)
-        hipLaunchKernelGGL((k_update<T, true>), grid_, blk, 0, s_, g_, kchunk_, A.D, pnew, q, x, r, z, part_, ctl, nb);
+        hipLaunchKernelGGL((k_update<T, true>), grid_f_, blk, 0, s_, g_, kc_f_, A.D, pnew, q, x, r, z, part_, ctl, nb_f_);
       else
-        hipLaunchKernelGGL((k_update<T, false>), grid_, blk, 0, s_, g_, kchunk_, A.D, pnew, q, x, r, z, part_, ctl, nb);
+        hipLaunchKernelGGL((k_update<T, false>), grid_f_, blk, 0, s_, g_, kc_f_, A.D, pnew, q, x, r, z, part_, ctl, nb_f_);
       FREHG_LAUNCH_CHECK();
       prof_->end();
+      hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_f_, ctl);
+      FREHG_LAUNCH_CHECK();
       parity ^= 1;
     }
     launched += batch;
@@ -404,7 +756,11 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
     if (launched >= max_it + 2) break;
     batch = std::max(2, std::min(16, batch / 2));
   }
-  if (host_ctl_->breakdown) throw ConvergenceError("Krylov: breakdown (p.Ap <= 0 or non-finite)");
+  if (host_ctl_->breakdown)
+    throw ConvergenceError(std::string("Krylov ") + (three_d_ ? "3D" : "2D") + (sizeof(T) == 4 ? " fp32" : " fp64") +
+                           ": breakdown (p.Ap = " + std::to_string(host_ctl_->pq) + " at iteration " +
+                           std::to_string(host_ctl_->iters + 1) + ", ||D^-1 r||^2 = " + std::to_string(host_ctl_->zz) +
+                           ")");
   const int iters = host_ctl_->iters;
   // each launched iteration recorded one spmv and one update launch
   if (launched > iters) prof_->discard_tail(2 * (size_t)(launched - iters));
@@ -416,49 +772,178 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
   return iters;
 }
 
+template <typename T>
+int Krylov::run_bicgstab(const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed,
+                         bool x0_zero, bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl,
+                         bool* rejected, double* res_norm, double* rhs_norm, bool* converged, int kt_spmv) {
+  const bool fp32 = sizeof(T) == 4;
+  ensure_bicgstab(fp32);
+  T* r = fp32 ? (T*)r32_ : (T*)r_;
+  T* rh = fp32 ? (T*)rh32_ : (T*)rh_;
+  T* pbuf[2] = {fp32 ? (T*)p032_ : (T*)p0_, fp32 ? (T*)p132_ : (T*)p1_};
+  T* vbuf[2] = {fp32 ? (T*)v032_ : (T*)v0_, fp32 ? (T*)v132_ : (T*)v1_};
+  T* sv = fp32 ? (T*)sv32_ : (T*)sv_;
+  T* tv = fp32 ? (T*)tv32_ : (T*)tv_;
+  const dim3 blk(NT);
+  auto init = [&](bool zero) {
+    if (three_d_)
+      hipLaunchKernelGGL((k_bcg_init<T, true>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, rh, pbuf[0],
+                         vbuf[0], fixed, zero ? 1 : 0, part_, nb_f_);
+    else
+      hipLaunchKernelGGL((k_bcg_init<T, false>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, rh, pbuf[0],
+                         vbuf[0], fixed, zero ? 1 : 0, part_, nb_f_);
+    FREHG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fin_bcg_init, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_f_, ctl, rtol, atol, max_it,
+                       guard ? 1 : 0, zero ? 1 : 0);
+    FREHG_LAUNCH_CHECK();
+  };
+  init(x0_zero);
+  *rejected = false;
+  if (guard && !x0_zero) {
+    read_ctl(ctl, host_ctl_);
+    if (host_ctl_->guess_rejected) {
+      *rejected = true;
+      init(true);
+    }
+  }
+  int parity = 0;
+  int& last = fp32 ? last_bcg_inner_iters_ : last_bcg_iters_;
+  int batch = std::max(2, last + 1);
+  int launched = 0;
+  for (;;) {
+    for (int it = 0; it < batch; ++it) {
+      const T* pold = pbuf[parity];
+      T* pnew = pbuf[parity ^ 1];
+      const T* vold = vbuf[parity];
+      T* vnew = vbuf[parity ^ 1];
+      prof_->begin(kt_spmv);
+      if (three_d_)
+        hipLaunchKernelGGL((k_bcg_spmv1<T, true>), grid_s_, blk, 0, s_, g_, kc_s_, A, (const T*)r, (const T*)rh, pold,
+                           vold, pnew, vnew, part_, ctl);
+      else
+        hipLaunchKernelGGL((k_bcg_spmv1<T, false>), grid_s_, blk, 0, s_, g_, kc_s_, A, (const T*)r, (const T*)rh,
+                           pold, vold, pnew, vnew, part_, ctl);
+      FREHG_LAUNCH_CHECK();
+      prof_->end();
+      hipLaunchKernelGGL(k_fin_bcg1, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_s_, ctl);
+      FREHG_LAUNCH_CHECK();
+      prof_->begin(kt_spmv);
+      if (three_d_)
+        hipLaunchKernelGGL((k_bcg_spmv2<T, true>), grid_s_, blk, 0, s_, g_, kc_s_, A, (const T*)r, (const T*)vnew, sv,
+                           tv, part_, ctl, nb_s_);
+      else
+        hipLaunchKernelGGL((k_bcg_spmv2<T, false>), grid_s_, blk, 0, s_, g_, kc_s_, A, (const T*)r, (const T*)vnew,
+                           sv, tv, part_, ctl, nb_s_);
+      FREHG_LAUNCH_CHECK();
+      prof_->end();
+      hipLaunchKernelGGL(k_fin_bcg2, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_s_, ctl);
+      FREHG_LAUNCH_CHECK();
+      prof_->begin(2);
+      if (three_d_)
+        hipLaunchKernelGGL((k_bcg_update<T, true>), grid_f_, blk, 0, s_, g_, kc_f_, A.D, (const T*)pnew, (const T*)sv,
+                           (const T*)tv, (const T*)rh, x, r, part_, ctl, nb_f_);
+      else
+        hipLaunchKernelGGL((k_bcg_update<T, false>), grid_f_, blk, 0, s_, g_, kc_f_, A.D, (const T*)pnew,
+                           (const T*)sv, (const T*)tv, (const T*)rh, x, r, part_, ctl, nb_f_);
+      FREHG_LAUNCH_CHECK();
+      prof_->end();
+      hipLaunchKernelGGL(k_fin_bcg3, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_f_, ctl);
+      FREHG_LAUNCH_CHECK();
+      parity ^= 1;
+    }
+    launched += batch;
+    read_ctl(ctl, host_ctl_);
+    if (host_ctl_->done) break;
+    if (launched >= max_it + 2) break;
+    batch = std::max(2, std::min(16, batch / 2));
+  }
+  if (host_ctl_->breakdown) {
+    static const char* what[] = {"", "rh.v = 0", "omega = 0", "rho = 0"};
+    throw ConvergenceError(std::string("BiCGSTAB ") + (three_d_ ? "3D" : "2D") + (fp32 ? " fp32" : " fp64") +
+                           ": breakdown (" + what[host_ctl_->breakdown & 3] + ") at iteration " +
+                           std::to_string(host_ctl_->iters + 1));
+  }
+  const int iters = host_ctl_->iters;
+  // each launched iteration recorded two spmv launches and one update launch
+  if (launched > iters) prof_->discard_tail(3 * (size_t)(launched - iters));
+  last = iters;
+  *res_norm = std::sqrt(host_ctl_->zz);
+  *rhs_norm = std::sqrt(host_ctl_->bb);
+  *converged = host_ctl_->converged != 0;
+  return iters;
+}
+
+template <typename T>
+int Krylov::run(int ksp, const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed,
+                bool x0_zero, bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl, bool* rejected,
+                double* res_norm, double* rhs_norm, bool* converged, int kt_spmv) {
+  if (ksp == FREHG_KSP_BICGSTAB)
+    return run_bicgstab<T>(A, b, bn, x, fixed, x0_zero, guard, rtol, atol, max_it, ctl, rejected, res_norm,
+                           rhs_norm, converged, kt_spmv);
+  if (!A.symmetric()) throw ArgError("Krylov: CG needs a symmetric operator (use BiCGSTAB)");
+  if (sizeof(T) == 4)
+    return run_pcg<T>(A, b, bn, x, fixed, x0_zero, guard, rtol, atol, max_it, ctl, (T*)r32_, (T*)z32_, (T*)p032_,
+                      (T*)p132_, (T*)q32_, rejected, res_norm, rhs_norm, converged, kt_spmv);
+  return run_pcg<T>(A, b, bn, x, fixed, x0_zero, guard, rtol, atol, max_it, ctl, (T*)r_, (T*)z_, (T*)p0_, (T*)p1_,
+                    (T*)q_, rejected, res_norm, rhs_norm, converged, kt_spmv);
+}
+
 SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>* A32, const double* b,
                           const double* bn, double* x, const unsigned char* fixed, bool x0_zero,
                           bool guard_guess, const KrylovConfig& cfg) {
   SolveResult res;
+  if (!tuned_) autotune(A);
   if (cfg.precision == 0 || A32 == nullptr) {
     bool rej = false, conv = false;
-    res.iterations = run_pcg<double>(A, b, bn, x, fixed, x0_zero, guard_guess, cfg.rtol, cfg.atol, cfg.max_it,
-                                     ctl_, r_, z_, p0_, p1_, q_, &rej, &res.residual_norm, &res.rhs_norm,
-                                     &conv, 0);
+    res.iterations = run<double>(cfg.ksp, A, b, bn, x, fixed, x0_zero, guard_guess, cfg.rtol, cfg.atol,
+                                 cfg.max_it, ctl_, &rej, &res.residual_norm, &res.rhs_norm, &conv, 0);
     res.guess_rejected = rej;
     res.converged = conv;
-    if (!conv) throw ConvergenceError("Krylov: not converged within max_it");
+    if (!conv)
+      throw ConvergenceError(std::string("Krylov ") + (three_d_ ? "3D" : "2D") + ": not converged within max_it (" +
+                             std::to_string(res.iterations) + " iterations, ||D^-1 r|| = " +
+                             std::to_string(res.residual_norm) + ", ||D^-1 b|| = " + std::to_string(res.rhs_norm) + ")");
     return res;
   }
   // FP32 inner / FP64 outer iterative refinement.
   ensure_mixed();
   const dim3 blk(NT);
-  const unsigned int nb = nblocks_;
+  const unsigned int nb = nb_f_;
   KrylovCtl* outer = ctl_;
   KrylovCtl* inner = ctl_ + 1;
   if (x0_zero) {
-    if (three_d_) hipLaunchKernelGGL((k_zero_interior<true>), grid_, blk, 0, s_, g_, kchunk_, x, fixed);
-    else hipLaunchKernelGGL((k_zero_interior<false>), grid_, blk, 0, s_, g_, kchunk_, x, fixed);
+    if (three_d_) hipLaunchKernelGGL((k_zero_interior<true>), grid_f_, blk, 0, s_, g_, kc_f_, x, fixed);
+    else hipLaunchKernelGGL((k_zero_interior<false>), grid_f_, blk, 0, s_, g_, kc_f_, x, fixed);
     FREHG_LAUNCH_CHECK();
   }
   FREHG_HIP(hipMemcpyAsync(xw_, x, (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
   double* xcur = x;
   double* xalt = xw_;
   auto refine = [&](const double* xin, const float* d, double* xout, bool first) {
-    if (three_d_)
-      hipLaunchKernelGGL((k_refine<true>), grid_, blk, 0, s_, g_, kchunk_, A, b, bn, xin, d, xout, b32_, fixed,
-                         part_, outer, nb, first ? 1 : 0, cfg.rtol, cfg.atol, (guard_guess && !x0_zero) ? 1 : 0);
+    const bool sym = A.symmetric();
+    if (three_d_ && sym)
+      hipLaunchKernelGGL((k_refine<true, true>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, xin, d, xout, b32_, fixed,
+                         part_, nb, first ? 1 : 0);
+    else if (three_d_)
+      hipLaunchKernelGGL((k_refine<true, false>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, xin, d, xout, b32_,
+                         fixed, part_, nb, first ? 1 : 0);
+    else if (sym)
+      hipLaunchKernelGGL((k_refine<false, true>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, xin, d, xout, b32_,
+                         fixed, part_, nb, first ? 1 : 0);
     else
-      hipLaunchKernelGGL((k_refine<false>), grid_, blk, 0, s_, g_, kchunk_, A, b, bn, xin, d, xout, b32_, fixed,
-                         part_, outer, nb, first ? 1 : 0, cfg.rtol, cfg.atol, (guard_guess && !x0_zero) ? 1 : 0);
+      hipLaunchKernelGGL((k_refine<false, false>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, xin, d, xout, b32_,
+                         fixed, part_, nb, first ? 1 : 0);
+    FREHG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fin_refine, dim3(1), dim3(kRedThreads), 0, s_, part_, nb, outer, first ? 1 : 0, cfg.rtol,
+                       cfg.atol, (guard_guess && !x0_zero) ? 1 : 0);
     FREHG_LAUNCH_CHECK();
   };
   refine(xcur, nullptr, xcur, true);
   read_ctl(outer, host_ctl_);
   if (host_ctl_->guess_rejected) {
     res.guess_rejected = true;
-    if (three_d_) hipLaunchKernelGGL((k_zero_interior<true>), grid_, blk, 0, s_, g_, kchunk_, xcur, fixed);
-    else hipLaunchKernelGGL((k_zero_interior<false>), grid_, blk, 0, s_, g_, kchunk_, xcur, fixed);
+    if (three_d_) hipLaunchKernelGGL((k_zero_interior<true>), grid_f_, blk, 0, s_, g_, kc_f_, xcur, fixed);
+    else hipLaunchKernelGGL((k_zero_interior<false>), grid_f_, blk, 0, s_, g_, kc_f_, xcur, fixed);
     FREHG_LAUNCH_CHECK();
     FREHG_HIP(hipMemcpyAsync(xw_, x, (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
     refine(xcur, nullptr, xcur, true);
@@ -471,9 +956,8 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
   while (!conv && res.refinements < cfg.max_refine) {
     bool rej = false, iconv = false;
     double rn = 0, bnrm = 0;
-    res.iterations += run_pcg<float>(A32v, b32_, nullptr, d32_, fixed, true, false, cfg.inner_rtol, 0.0,
-                                     cfg.max_it, inner, r32_, z32_, p032_, p132_, q32_, &rej, &rn, &bnrm,
-                                     &iconv, 1);
+    res.iterations += run<float>(cfg.ksp, A32v, b32_, nullptr, d32_, fixed, true, false, cfg.inner_rtol, 0.0,
+                                 cfg.max_it, inner, &rej, &rn, &bnrm, &iconv, 1);
     refine(xcur, d32_, xalt, false);
     std::swap(xcur, xalt);
     res.refinements += 1;
@@ -486,8 +970,8 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
     // Refinement stalled: finish in FP64 from the current iterate.
     bool rej = false, c64 = false;
     res.fallback_fp64 = true;
-    res.iterations += run_pcg<double>(A, b, bn, x, fixed, false, false, cfg.rtol, cfg.atol, cfg.max_it, ctl_,
-                                      r_, z_, p0_, p1_, q_, &rej, &res.residual_norm, &res.rhs_norm, &c64, 0);
+    res.iterations += run<double>(cfg.ksp, A, b, bn, x, fixed, false, false, cfg.rtol, cfg.atol, cfg.max_it, ctl_,
+                                  &rej, &res.residual_norm, &res.rhs_norm, &c64, 0);
     conv = c64;
     // the FP64 test above is relative to ||D^-1 b||, the same normalisation as the outer test
   }

@@ -1,34 +1,42 @@
-// Matrix-free Jacobi-preconditioned CG on 7-point (3D) / 5-point (2D) stencil operators in the
-// halo-padded layout, FP64 or FP32-inside-FP64-refinement. Replaces PETSc KSP "cg" as driven by
-// src/linear/backends/PetscLinearSolver.cpp:186-220.
+// Matrix-free Jacobi-preconditioned CG and BiCGSTAB on 7-point (3D) / 5-point (2D) stencil
+// operators in the halo-padded layout, FP64 or FP32-inside-FP64-refinement. Replaces PETSc KSP
+// "cg" / "bcgs" as driven by src/linear/backends/PetscLinearSolver.cpp:186-220.
 #pragma once
 
 #include "common.hpp"
+#include "frehg_gpu.h"
 #include "profile.hpp"
 
 namespace frehg {
 
-// Device-resident control block of one Krylov solve. Each kernel's last block finalises the
-// reductions it produced and writes the scalars the next kernel reads, so a whole iteration
-// runs without a host round trip; kernels return immediately once `done` is set.
+// Device-resident control block of one Krylov solve. After each vector kernel a one-block
+// kernel reduces its per-block partials and writes the scalars the next kernel reads, so a
+// whole iteration runs without a host round trip; kernels return immediately once `done` is set.
 struct KrylovCtl {
   double rz, alpha, beta, zz, ttol2, bb, pq, rr_guess;
+  double omega;  // BiCGSTAB: rz holds rho = rh.r, pq holds rh.v
   int iters, done, converged, breakdown, max_it, guess_rejected;
-  unsigned int counter[4];
+  int half;  // BiCGSTAB: converged on the intermediate residual s
 };
 
-// Symmetric stencil operator: diagonal plus the coupling to the +x/+y/+z neighbour of each
-// row; the -x/-y/-z couplings are read from the neighbour (ax(c-1) etc.). Off-domain entries
-// and every ghost entry are zero. az is null for 2D operators.
+// Stencil operator: diagonal plus the coupling to the +x/+y/+z neighbour of each row. When
+// axm/aym/azm are null the operator is symmetric and the -x/-y/-z couplings are read from the
+// neighbour (ax(c-1) etc.); otherwise they are the row's own -x/-y/-z couplings (BiCGSTAB
+// only). Off-domain entries and every ghost entry are zero. az/azm are null for 2D operators.
 template <typename T>
 struct StencilView {
   const T* D;
   const T* ax;
   const T* ay;
   const T* az;
+  const T* axm = nullptr;
+  const T* aym = nullptr;
+  const T* azm = nullptr;
+  bool symmetric() const { return axm == nullptr; }
 };
 
 struct KrylovConfig {
+  int ksp = 0;              // FREHG_KSP_CG or FREHG_KSP_BICGSTAB
   double rtol = 1e-10, atol = 1e-50;
   int max_it = 10000;
   int precision = 0;        // 0 FP64, 1 FP32 inner + FP64 refinement
@@ -45,6 +53,26 @@ struct SolveResult {
   bool guess_rejected = false;
 };
 
+// Validated KrylovConfig from the ABI solver configuration (include/frehg2/linear/SolverConfig.hpp:14-20).
+inline KrylovConfig krylov_config(const frehg_solver_config& c, const char* who) {
+  if (c.ksp_type != FREHG_KSP_CG && c.ksp_type != FREHG_KSP_BICGSTAB)
+    throw ArgError(std::string(who) + ": ksp_type must be FREHG_KSP_CG or FREHG_KSP_BICGSTAB");
+  if (c.pc_type != FREHG_PC_JACOBI) throw ArgError(std::string(who) + ": pc_type must be FREHG_PC_JACOBI");
+  if (c.precision != FREHG_PREC_FP64 && c.precision != FREHG_PREC_MIXED)
+    throw ArgError(std::string(who) + ": precision must be FREHG_PREC_FP64 or FREHG_PREC_MIXED");
+  if (!(c.rtol >= 0.0) || !(c.atol >= 0.0) || c.max_it < 1)
+    throw ArgError(std::string(who) + ": need rtol >= 0, atol >= 0, max_it >= 1");
+  KrylovConfig k;
+  k.ksp = c.ksp_type;
+  k.rtol = c.rtol;
+  k.atol = c.atol;
+  k.max_it = c.max_it;
+  k.precision = c.precision == FREHG_PREC_MIXED ? 1 : 0;
+  k.inner_rtol = c.inner_rtol > 0 ? c.inner_rtol : 1e-5;
+  k.max_refine = c.max_refine > 0 ? c.max_refine : 8;
+  return k;
+}
+
 class Krylov {
  public:
   Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof);
@@ -58,12 +86,19 @@ class Krylov {
   // with x0 = 0). fixed (2D, may be null) marks rows whose x is final and whose couplings are
   // already eliminated from b. guard_guess: fall back to x0 = 0 when the guess's residual is
   // not below ||D^-1 b||. A32 (may be null) is the FP32 copy of A for precision == 1.
+  // cfg.ksp selects CG (symmetric A only) or BiCGSTAB.
   SolveResult solve(const StencilView<double>& A, const StencilView<float>* A32, const double* b,
                     const double* bn, double* x, const unsigned char* fixed, bool x0_zero,
                     bool guard_guess, const KrylovConfig& cfg);
 
-  int kchunk() const { return kchunk_; }
-  void set_kchunk(int kc);
+  // Layers per thread of the SpMV's z-march (kc_spmv) and of the streaming kernels (kc_flat).
+  int kchunk_spmv() const { return kc_s_; }
+  int kchunk_flat() const { return kc_f_; }
+  void set_kchunks(int kc_spmv, int kc_flat);
+  // Times the SpMV for each candidate z-chunk on the operator at hand and keeps the fastest
+  // (run automatically at the first 3D solve; the result is cached per instance).
+  void autotune(const StencilView<double>& A);
+  bool tuned() const { return tuned_; }
 
  private:
   template <typename T>
@@ -71,25 +106,40 @@ class Krylov {
               bool x0_zero, bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl,
               T* r, T* z, T* p0, T* p1, T* q, bool* rejected, double* res_norm, double* rhs_norm,
               bool* converged, int kt_spmv);
+  template <typename T>
+  int run_bicgstab(const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed,
+                   bool x0_zero, bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl, bool* rejected,
+                   double* res_norm, double* rhs_norm, bool* converged, int kt_spmv);
+  template <typename T>
+  int run(int ksp, const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed,
+          bool x0_zero, bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl, bool* rejected,
+          double* res_norm, double* rhs_norm, bool* converged, int kt_spmv);
   void read_ctl(KrylovCtl* dev, KrylovCtl* host);
   void ensure_mixed();
+  void ensure_bicgstab(bool fp32);
 
   Geo g_;
   bool three_d_;
   hipStream_t s_;
   Profiler* prof_;
-  int kchunk_ = 0;
-  dim3 grid_;
-  unsigned int nblocks_ = 0;
+  int kc_s_ = 0, kc_f_ = 0;
+  dim3 grid_s_, grid_f_;
+  unsigned int nb_s_ = 0, nb_f_ = 0, part_cap_ = 0;
+  bool tuned_ = false;
   long long n_ = 0;  // padded length
   double *r_ = nullptr, *z_ = nullptr, *p0_ = nullptr, *p1_ = nullptr, *q_ = nullptr, *xw_ = nullptr;
   float *b32_ = nullptr, *r32_ = nullptr, *z32_ = nullptr, *p032_ = nullptr, *p132_ = nullptr,
         *q32_ = nullptr, *d32_ = nullptr;
+  // BiCGSTAB work vectors (shadow residual, ping-pong v, s, t), allocated on first use
+  double *rh_ = nullptr, *v0_ = nullptr, *v1_ = nullptr, *sv_ = nullptr, *tv_ = nullptr;
+  float *rh32_ = nullptr, *v032_ = nullptr, *v132_ = nullptr, *sv32_ = nullptr, *tv32_ = nullptr;
   double* part_ = nullptr;
   KrylovCtl* ctl_ = nullptr;  // [0] main / outer, [1] inner
   KrylovCtl* host_ctl_ = nullptr;
   int last_iters_ = 8;
   int last_inner_iters_ = 8;
+  int last_bcg_iters_ = 8;
+  int last_bcg_inner_iters_ = 8;
 };
 
 }  // namespace frehg

@@ -112,12 +112,13 @@ __device__ inline double face_z(const ReK& P, const double* kr, long long c, int
 // K/Ks and C at the interior cell and kr at its face-adjacent ghost cells (whose soil is the
 // cell's own: ghost indices are clamped). ReSolver::updateDiagnostics (ReSolver.cpp:199-215).
 __device__ inline void constitutive_with_ghosts(const ReK& P, const Cell& t, const double* h,
-                                                 double* kr, double* ch) {
+                                                 double* kr, double* ch, double* wch) {
   const Geo& g = P.g;
   const SoilClass& s = P.soil.at(t.c);
   const VGState v = vg_state(s, h[t.c]);
   kr[t.c] = v.kr;
   ch[t.c] = v.cap;
+  if (wch) wch[t.c] = v.theta;
   const long long gh[6] = {t.i == 0 ? t.c - 1 : -1,         t.i == g.nx - 1 ? t.c + 1 : -1,
                            t.j == 0 ? t.c - g.sy : -1,      t.j == g.ny - 1 ? t.c + g.sy : -1,
                            t.k == 0 ? t.c - g.sz : -1,      t.k == g.nz - 1 ? t.c + g.sz : -1};
@@ -129,7 +130,7 @@ __device__ inline void constitutive_with_ghosts(const ReK& P, const Cell& t, con
 __global__ __launch_bounds__(NT) void k_re_pre(ReK P, const double* hn, double* kr, double* ch) {
   const Cell t = cell_of(P.g);
   if (!t.valid) return;
-  constitutive_with_ghosts(P, t, hn, kr, ch);
+  constitutive_with_ghosts(P, t, hn, kr, ch, nullptr);
 }
 
 // groundwaterMatCoeff + groundwaterRhs (ReSolver.cpp:397-549) as a symmetric stencil:
@@ -200,8 +201,9 @@ __global__ __launch_bounds__(NT) void k_re_assemble(ReK P, const double* kr, con
   }
 }
 
-// enforceHeadBc (ReSolver.cpp:638-669) + kr and C at the solved head.
-__global__ __launch_bounds__(NT) void k_re_post(ReK P, double* h, double* kr, double* ch) {
+// enforceHeadBc (ReSolver.cpp:638-669) + kr, C and theta at the solved head (the second
+// updateDiagnostics' ch and wch, ReSolver.cpp:963, :825).
+__global__ __launch_bounds__(NT) void k_re_post(ReK P, double* h, double* kr, double* ch, double* wch) {
   const Cell t = cell_of(P.g);
   if (!t.valid) return;
   const Geo& g = P.g;
@@ -213,7 +215,7 @@ __global__ __launch_bounds__(NT) void k_re_post(ReK P, double* h, double* kr, do
   if (t.k == g.nz - 1) h[t.c + g.sz] = (P.bc4 == 1) ? P.hbot : hc;
   if (t.k == 0) h[t.c - g.sz] = (!P.sim_sw && P.bc5 == 1) ? P.htop : hc;
   // ghost values written above are re-read by this same thread only
-  constitutive_with_ghosts(P, t, h, kr, ch);
+  constitutive_with_ghosts(P, t, h, kr, ch, wch);
 }
 
 // Second computeKFace + groundwaterFlux with darcyFluxZ (ReSolver.cpp:947-948, 24-58, 671-743).
@@ -274,7 +276,7 @@ __global__ __launch_bounds__(NT) void k_re_flux(ReK P, const double* h, const do
 // reallocation (:825-826); saturation flags for the reallocation fixed point.
 __global__ __launch_bounds__(NT) void k_re_correct(ReK P, const double* h, const double* hn,
                                                     const double* wcn, const double* qx, const double* qy,
-                                                    const double* qz, double* wc, double* wch, double* hwc,
+                                                    const double* qz, const double* wch, double* wc, double* hwc,
                                                     unsigned char* sat0, unsigned char* cand) {
   const Cell t = cell_of(P.g);
   if (!t.valid) return;
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(NT) void k_re_correct(ReK P, const double* h, const
   const long long c = t.c;
   const SoilClass& s = P.soil.at(c);
   double w;
-  const VGState v = vg_state(s, h[c]);
+  const double th = wch[c];
   if (P.use_corrector) {
     const double V = P.dx * P.dy * P.dz;
     const double coeff = V * (1.0 + s.Ss * (h[c] - hn[c]) / s.theta_s);
@@ -291,14 +293,13 @@ __global__ __launch_bounds__(NT) void k_re_correct(ReK P, const double* h, const
     const double dqz = P.dt * (qz[c] - qz[c - g.sz]);
     w = (wcn[c] * V + dqx + dqy + dqz) / coeff;
   } else {
-    w = v.theta;
+    w = th;
   }
   wc[c] = w;
-  wch[c] = v.theta;
   hwc[c] = vg_head(s, w);
   const unsigned char s0 = (w >= s.theta_s) ? 1 : 0;
   sat0[c] = s0;
-  cand[c] = (!s0 && v.theta >= s.theta_s) ? 1 : 0;
+  cand[c] = (!s0 && th >= s.theta_s) ? 1 : 0;
   // Ghost neighbours keep the moisture ghosts of the previous step (ReSolver.cpp:855-878).
   const long long gh[6] = {t.i == 0 ? c - 1 : -1,         t.i == g.nx - 1 ? c + 1 : -1,
                            t.j == 0 ? c - g.sy : -1,      t.j == g.ny - 1 ? c + g.sy : -1,
@@ -375,10 +376,7 @@ __global__ __launch_bounds__(NT) void k_re_finalize(ReK P, const double* Kx, con
 // over non-top cells. max/min are exact, so the result is order-independent.
 __global__ __launch_bounds__(NT) void k_re_dtadapt(ReK P, const double* qx, const double* qy,
                                                     const double* qz, const double* wc, double* part,
-                                                    unsigned int* counter, unsigned int nblocks,
-                                                    double* res) {
-  __shared__ double lds[NT / kWave];
-  __shared__ int flag;
+                                                    unsigned int nblocks) {
   const Cell t = cell_of(P.g);
   double dqm = 0.0, dtc = 1.0e8;
   if (t.valid && t.k > 0) {
@@ -409,18 +407,6 @@ __global__ __launch_bounds__(NT) void k_re_dtadapt(ReK P, const double* qx, cons
     }
     part[bid] = a;
     part[nblocks + bid] = b;
-  }
-  (void)lds;
-  if (last_block(counter, nblocks, &flag)) {
-    if (threadIdx.x == 0) {
-      double a = 0.0, b = 1.0e8;
-      for (unsigned int q = 0; q < nblocks; ++q) {
-        a = fmax(a, part[q]);
-        b = fmin(b, part[nblocks + q]);
-      }
-      res[0] = a;
-      res[1] = b;
-    }
   }
 }
 
@@ -505,6 +491,7 @@ void ReState::set_params(const frehg_re_params& p) {
   if (p.bc_type[5] < 0 || p.bc_type[5] > 2) throw ArgError("frehg_re: top bc must be 0, 1 or 2");
   if (!(p.dx > 0 && p.dy > 0 && p.dz > 0 && p.dt > 0)) throw ArgError("frehg_re: dx, dy, dz, dt must be > 0");
   if (p.picard_max < 1) throw ArgError("frehg_re: picard_max must be >= 1");
+  (void)krylov_config(p.solver, "frehg_re");
   const bool mixed = p.solver.precision == FREHG_PREC_MIXED;
   if (mixed && !D32_) {
     const size_t b32 = (size_t)n_ * sizeof(float);
@@ -679,13 +666,7 @@ void ReState::step(frehg_re_stats* st) {
     x0_zero = false;
     guard = true;
   }
-  KrylovConfig kc;
-  kc.rtol = p_.solver.rtol;
-  kc.atol = p_.solver.atol;
-  kc.max_it = p_.solver.max_it;
-  kc.precision = mixed ? 1 : 0;
-  kc.inner_rtol = p_.solver.inner_rtol > 0 ? p_.solver.inner_rtol : 1e-5;
-  kc.max_refine = p_.solver.max_refine > 0 ? p_.solver.max_refine : 8;
+  const KrylovConfig kc = krylov_config(p_.solver, "frehg_re_step");
   StencilView<double> A{D_, ax_, ay_, az_};
   StencilView<float> A32{D32_, ax32_, ay32_, az32_};
   const SolveResult sr = kry_->solve(A, mixed ? &A32 : nullptr, rhs_, nullptr, h_, nullptr, x0_zero, guard, kc);
@@ -699,11 +680,11 @@ void ReState::step(frehg_re_stats* st) {
   local.picard_iterations = 1;
   if (mlp_ && mlp_->logging()) mlp_->log_pair(*this, hn_, h_);
   prof_->begin(4);
-  hipLaunchKernelGGL(k_re_post, grid, blk, 0, s_, P, h_, kr_, ch_);
+  hipLaunchKernelGGL(k_re_post, grid, blk, 0, s_, P, h_, kr_, ch_, wch_);
   FREHG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_re_flux, grid, blk, 0, s_, P, h_, kr_, wcn_, Kx_, Ky_, Kz_, qx_, qy_, qz_);
   FREHG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_re_correct, grid, blk, 0, s_, P, h_, hn_, wcn_, qx_, qy_, qz_, wc_, wch_, hwc_, sat0_,
+  hipLaunchKernelGGL(k_re_correct, grid, blk, 0, s_, P, h_, hn_, wcn_, qx_, qy_, qz_, wch_, wc_, hwc_, sat0_,
                      cand_);
   FREHG_LAUNCH_CHECK();
   prof_->end();
@@ -725,7 +706,9 @@ void ReState::step(frehg_re_stats* st) {
   FREHG_LAUNCH_CHECK();
   if (p_.adaptive_dt) {
     const unsigned int nb = grid.x * grid.y * grid.z;
-    hipLaunchKernelGGL(k_re_dtadapt, grid, blk, 0, s_, P, qx_, qy_, qz_, wc_, red_, dcount_, nb, dres_);
+    hipLaunchKernelGGL(k_re_dtadapt, grid, blk, 0, s_, P, qx_, qy_, qz_, wc_, red_, nb);
+    FREHG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce_maxmin, dim3(1), dim3(kRedThreads), 0, s_, (const double*)red_, nb, dres_);
     FREHG_LAUNCH_CHECK();
     FREHG_HIP(hipMemcpyAsync(hres_, dres_, 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
     FREHG_HIP(hipStreamSynchronize(s_));

@@ -379,9 +379,7 @@ __global__ __launch_bounds__(NT) void k_swe_geom(SwK P, const double* dept, cons
 
 // maxCfl (SweSolver.cpp:954-975).
 __global__ __launch_bounds__(NT) void k_swe_maxcfl(SwK P, const double* dept, const double* uu, const double* vv,
-                                                    double* part, unsigned int* counter, unsigned int nblocks,
-                                                    double* res) {
-  __shared__ int flag;
+                                                    double* part, unsigned int nblocks) {
   __shared__ double lm[NT / kWave];
   const C2 t = cell2(P.g);
   double v = 0.0;
@@ -400,11 +398,7 @@ __global__ __launch_bounds__(NT) void k_swe_maxcfl(SwK P, const double* dept, co
     double a = 0.0;
     for (int q = 0; q < NT / kWave; ++q) a = fmax(a, lm[q]);
     part[bid] = a;
-  }
-  if (last_block(counter, nblocks, &flag) && threadIdx.x == 0) {
-    double a = 0.0;
-    for (unsigned int q = 0; q < nblocks; ++q) a = fmax(a, part[q]);
-    res[0] = a;
+    part[nblocks + bid] = 0.0;
   }
 }
 
@@ -435,7 +429,7 @@ SweState::SweState(int nx, int ny, double dx, double dy, const frehg_swe_params&
   FREHG_HIP(hipMalloc(&fixed_, (size_t)n_));
   FREHG_HIP(hipMemsetAsync(fixed_, 0, (size_t)n_, s_));
   const unsigned int nb = ceil_div(nx, 64) * ceil_div(ny, 4);
-  FREHG_HIP(hipMalloc(&red_, (size_t)nb * sizeof(double)));
+  FREHG_HIP(hipMalloc(&red_, 2 * (size_t)nb * sizeof(double)));
   FREHG_HIP(hipMalloc(&dres_, 4 * sizeof(double)));
   FREHG_HIP(hipHostMalloc(&hres_, 4 * sizeof(double), hipHostMallocDefault));
   FREHG_HIP(hipMalloc(&dcount_, 16));
@@ -455,6 +449,7 @@ SweState::~SweState() {
 
 void SweState::set_params(const frehg_swe_params& p) {
   if (!(p.dt > 0)) throw ArgError("frehg_swe: dt must be > 0");
+  (void)krylov_config(p.solver, "frehg_swe");
   p_ = p;
 }
 
@@ -605,10 +600,7 @@ void SweState::step(double rain, double evap, frehg_solve_stats* st) {
   FREHG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_swe_symm, grid, blk, 0, s_, P, o, fixed_, F("eta"), ax_, ay_, b_);
   FREHG_LAUNCH_CHECK();
-  KrylovConfig kc;
-  kc.rtol = p_.solver.rtol;
-  kc.atol = p_.solver.atol;
-  kc.max_it = p_.solver.max_it;
+  KrylovConfig kc = krylov_config(p_.solver, "frehg_swe_step");
   kc.precision = 0;  // the surface system is small; it stays FP64
   StencilView<double> A{F("Sct"), ax_, ay_, nullptr};
   const SolveResult sr = kry_->solve(A, nullptr, b_, F("Srhs"), F("eta"), fixed_, true, false, kc);
@@ -644,8 +636,9 @@ void SweState::step(double rain, double evap, frehg_solve_stats* st) {
 double SweState::max_cfl() {
   const SwK P = make_swk(g_, dx_, dy_, p_);
   const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4)), blk(NT);
-  hipLaunchKernelGGL(k_swe_maxcfl, grid, blk, 0, s_, P, f_["dept"], f_["uu"], f_["vv"], red_, dcount_,
-                     grid.x * grid.y, dres_);
+  hipLaunchKernelGGL(k_swe_maxcfl, grid, blk, 0, s_, P, f_["dept"], f_["uu"], f_["vv"], red_, grid.x * grid.y);
+  FREHG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_maxmin, dim3(1), dim3(kRedThreads), 0, s_, (const double*)red_, grid.x * grid.y, dres_);
   FREHG_LAUNCH_CHECK();
   FREHG_HIP(hipMemcpyAsync(hres_, dres_, sizeof(double), hipMemcpyDeviceToHost, s_));
   FREHG_HIP(hipStreamSynchronize(s_));

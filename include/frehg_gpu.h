@@ -38,7 +38,10 @@ enum {
 };
 
 /* Krylov method / preconditioner / precision selectors. */
-enum { FREHG_KSP_CG = 0 };
+/* KSP types: "cg" and "bcgs" of SolverConfig::ksp_type (include/frehg2/linear/SolverConfig.hpp:15),
+ * set per module by surface_water.solver.ksp_type / groundwater.solver.ksp_type
+ * (src/driver/Orchestrator.cpp:435,488). */
+enum { FREHG_KSP_CG = 0, FREHG_KSP_BICGSTAB = 1 };
 enum { FREHG_PC_JACOBI = 0 };
 enum {
   FREHG_PREC_FP64 = 0,      /* FP64 Jacobi-PCG (reference arithmetic) */

@@ -77,8 +77,23 @@ real dot_interior(const StencilOp& A, const std::vector<real>& a, const std::vec
 }
 }  // namespace
 
+namespace {
+SolveStats pcg_only(const StencilOp& A, const std::vector<real>& b_in, std::vector<real>& x, const SolverCfg& cfg);
+SolveStats bicgstab(const StencilOp& A, const std::vector<real>& b_in, std::vector<real>& x, const SolverCfg& cfg);
+}  // namespace
+
+// KSP type dispatch (SolverConfig::ksp_type, include/frehg2/linear/SolverConfig.hpp:15): "cg"
+// (default) or "bcgs". A CG breakdown is an error, as a diverged KSP is in the reference
+// (PetscLinearSolver.cpp:210-218).
 SolveStats pcg_solve(const StencilOp& A, const std::vector<real>& b_in, std::vector<real>& x,
                      const SolverCfg& cfg) {
+  if (cfg.ksp == 1) return bicgstab(A, b_in, x, cfg);
+  return pcg_only(A, b_in, x, cfg);
+}
+
+namespace {
+SolveStats pcg_only(const StencilOp& A, const std::vector<real>& b_in, std::vector<real>& x,
+                    const SolverCfg& cfg) {
   const int nx = A.g.nx, ny = A.g.ny, nz = A.three_d ? A.g.nz : 1;
   const size_t n = b_in.size();
   const long long sy = nx + 2, sz = A.g.plane();
@@ -189,5 +204,132 @@ SolveStats pcg_solve(const StencilOp& A, const std::vector<real>& b_in, std::vec
   if (!st.converged) throw std::runtime_error("oracle pcg: not converged within max_it");
   return st;
 }
+
+// Right-Jacobi-preconditioned BiCGSTAB (van der Vorst 1992), x0 = 0, on the same (possibly
+// nonsymmetric) stencil rows, decoupled 2D rows eliminated exactly as in the CG path.
+SolveStats bicgstab(const StencilOp& A, const std::vector<real>& b_in, std::vector<real>& x,
+                    const SolverCfg& cfg) {
+  const int nx = A.g.nx, ny = A.g.ny, nz = A.three_d ? A.g.nz : 1;
+  const size_t n = b_in.size();
+  const long long sy = nx + 2;
+  x.assign(n, 0.0);
+  std::vector<real> b = b_in;
+  const bool has_fixed = !A.three_d && std::any_of(A.fixed.begin(), A.fixed.end(), [](unsigned char f) { return f != 0; });
+  const std::vector<unsigned char>* mask = has_fixed ? &A.fixed : nullptr;
+  std::vector<real> zb(n, 0.0);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const long long c = A.three_d ? A.g.idx(i, j, k) : A.g.sidx(i, j);
+        zb[c] = b[c] / A.diag[c];
+      }
+  const real bnorm = std::sqrt(dot_interior(A, zb, zb, nullptr));
+  if (has_fixed) {
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const long long c = A.g.sidx(i, j);
+        if (A.fixed[c]) x[c] = b[c] / A.diag[c];
+      }
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const long long c = A.g.sidx(i, j);
+        if (A.fixed[c]) continue;
+        real acc = b[c];
+        if (i > 0 && A.fixed[c - 1]) acc -= A.xm[c] * x[c - 1];
+        if (i < nx - 1 && A.fixed[c + 1]) acc -= A.xp[c] * x[c + 1];
+        if (j > 0 && A.fixed[c - sy]) acc -= A.ym[c] * x[c - sy];
+        if (j < ny - 1 && A.fixed[c + sy]) acc -= A.yp[c] * x[c + sy];
+        b[c] = acc;
+      }
+  }
+  auto applyFree = [&](const std::vector<real>& v, std::vector<real>& out) {
+    if (!has_fixed) {
+      A.apply(v, out);
+      return;
+    }
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const long long c = A.g.sidx(i, j);
+        if (A.fixed[c]) {
+          out[c] = 0.0;
+          continue;
+        }
+        real acc = A.diag[c] * v[c];
+        if (i > 0 && !A.fixed[c - 1]) acc += A.xm[c] * v[c - 1];
+        if (i < nx - 1 && !A.fixed[c + 1]) acc += A.xp[c] * v[c + 1];
+        if (j > 0 && !A.fixed[c - sy]) acc += A.ym[c] * v[c - sy];
+        if (j < ny - 1 && !A.fixed[c + sy]) acc += A.yp[c] * v[c + sy];
+        out[c] = acc;
+      }
+  };
+  auto forInterior = [&](auto&& fn) {
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const long long c = A.three_d ? A.g.idx(i, j, k) : A.g.sidx(i, j);
+          if (mask && (*mask)[c]) continue;
+          fn(c);
+        }
+  };
+  std::vector<real> r(n, 0.0), rh(n, 0.0), p(n, 0.0), v(n, 0.0), y(n, 0.0), s(n, 0.0), zz(n, 0.0), t(n, 0.0),
+      w(n, 0.0);
+  forInterior([&](long long c) {
+    r[c] = b[c];
+    rh[c] = b[c];
+  });
+  SolveStats st;
+  st.rhs_norm = bnorm;
+  const real ttol = std::max(cfg.rtol * bnorm, cfg.atol);
+  auto pnorm = [&](const std::vector<real>& q) {
+    forInterior([&](long long c) { w[c] = q[c] / A.diag[c]; });
+    return std::sqrt(dot_interior(A, w, w, mask));
+  };
+  real rn = pnorm(r);
+  if (rn <= ttol) {
+    st.converged = 1;
+    st.residual_norm = rn;
+    return st;
+  }
+  real rho = 1.0, alpha = 1.0, omega = 1.0;
+  for (int it = 1; it <= cfg.max_it; ++it) {
+    const real rho_new = dot_interior(A, rh, r, mask);
+    if (rho_new == 0.0 || !std::isfinite(rho_new)) throw std::runtime_error("oracle bicgstab: breakdown (rho = 0)");
+    const real beta = (rho_new / rho) * (alpha / omega);
+    rho = rho_new;
+    forInterior([&](long long c) {
+      p[c] = r[c] + beta * (p[c] - omega * v[c]);
+      y[c] = p[c] / A.diag[c];
+    });
+    applyFree(y, v);
+    const real rv = dot_interior(A, rh, v, mask);
+    if (rv == 0.0 || !std::isfinite(rv)) throw std::runtime_error("oracle bicgstab: breakdown (rh.v = 0)");
+    alpha = rho / rv;
+    forInterior([&](long long c) { s[c] = r[c] - alpha * v[c]; });
+    st.iterations = it;
+    if (pnorm(s) <= ttol) {
+      forInterior([&](long long c) { x[c] += alpha * y[c]; });
+      st.converged = 1;
+      st.residual_norm = pnorm(s);
+      return st;
+    }
+    forInterior([&](long long c) { zz[c] = s[c] / A.diag[c]; });
+    applyFree(zz, t);
+    const real tt = dot_interior(A, t, t, mask);
+    omega = tt > 0.0 ? dot_interior(A, t, s, mask) / tt : 0.0;
+    forInterior([&](long long c) {
+      x[c] += alpha * y[c] + omega * zz[c];
+      r[c] = s[c] - omega * t[c];
+    });
+    rn = pnorm(r);
+    if (rn <= ttol) {
+      st.converged = 1;
+      st.residual_norm = rn;
+      return st;
+    }
+    if (omega == 0.0) throw std::runtime_error("oracle bicgstab: breakdown (omega = 0)");
+  }
+  throw std::runtime_error("oracle bicgstab: not converged within max_it");
+}
+}  // namespace
 
 }  // namespace frehg_oracle

@@ -179,6 +179,7 @@ struct SolverCfg {
   real rtol = 1e-8;
   real atol = 1e-50;
   int max_it = 10000;
+  int ksp = 0;  // 0 "cg", 1 "bcgs"
 };
 
 struct SolveStats {
@@ -208,6 +209,7 @@ struct StencilOp {
 // Jacobi-PCG, x0 = 0, PETSc default convergence test (see header comment).
 // For 2D operators with `fixed` rows (decoupled dry rows), those unknowns are solved exactly
 // and eliminated into the right-hand side first, so CG runs on the symmetric remainder.
+// cfg.ksp == 1 selects Jacobi-preconditioned BiCGSTAB ("bcgs") instead of CG.
 SolveStats pcg_solve(const StencilOp& A, const std::vector<real>& b, std::vector<real>& x,
                      const SolverCfg& cfg);
 

@@ -36,6 +36,7 @@ SolverCfg toCfg(const frehg_solver_config& c) {
   o.rtol = c.rtol;
   o.atol = c.atol;
   o.max_it = c.max_it;
+  o.ksp = c.ksp_type == FREHG_KSP_BICGSTAB ? 1 : 0;
   return o;
 }
 

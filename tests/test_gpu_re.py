@@ -126,14 +126,15 @@ def test_baseline_dt_collapse_regime():
     assert np.max(np.abs(a - b) / (1.0 + np.abs(b))) < 1e-6
 
 
-def test_mixed_precision_matches_fp64_oracle():
+@pytest.mark.parametrize("ksp", [abi.KSP_CG, abi.KSP_BICGSTAB])
+def test_mixed_precision_matches_fp64_oracle(ksp):
     nx, ny, nz = 16, 12, 20
-    say("RE mixed precision vs FP64 oracle")
+    say(f"RE mixed precision vs FP64 oracle, ksp={ksp}")
     p = c1_params(nx, ny, nz, bc4=3)
     p.solver.rtol = 1e-12
     orc, _ = re_pair(nx, ny, nz, p, h0=np.full(nx * ny * nz, -10.0))
     pm = c1_params(nx, ny, nz, bc4=3)
-    pm.solver = abi.solver_config(1e-12, precision=abi.PREC_MIXED, inner_rtol=1e-5, max_refine=10)
+    pm.solver = abi.solver_config(1e-12, precision=abi.PREC_MIXED, inner_rtol=1e-5, max_refine=10, ksp_type=ksp)
     _, ctx = re_pair(nx, ny, nz, pm, h0=np.full(nx * ny * nz, -10.0))
     for s in range(10):
         orc.step()
@@ -153,4 +154,21 @@ def test_previous_step_guess_converges_to_same_solution():
     for s in range(8):
         orc.step()
         ctx.re_step()
+    assert np.abs(ctx.re_get("h") - orc.get("h")).max() <= 1e-8
+
+
+def test_bicgstab_fp64_matches_cg_oracle():
+    """The GW system solved by BiCGSTAB ("bcgs") instead of CG reaches the same discrete solution."""
+    nx, ny, nz = 12, 10, 15
+    say("RE BiCGSTAB fp64 vs CG oracle")
+    p = c1_params(nx, ny, nz, bc4=3)
+    p.solver.rtol = 1e-12
+    orc, _ = re_pair(nx, ny, nz, p, h0=np.full(nx * ny * nz, -10.0))
+    pb = c1_params(nx, ny, nz, bc4=3)
+    pb.solver = abi.solver_config(1e-12, ksp_type=abi.KSP_BICGSTAB)
+    _, ctx = re_pair(nx, ny, nz, pb, h0=np.full(nx * ny * nz, -10.0))
+    for s in range(8):
+        orc.step()
+        st = ctx.re_step()
+        assert st.solve.converged
     assert np.abs(ctx.re_get("h") - orc.get("h")).max() <= 1e-8

@@ -45,15 +45,16 @@ def test_init_matches_oracle():
     compare_swe(ctx, orc, rtol=0, atol=0, label="init")
 
 
-@pytest.mark.parametrize("visc", [0.0, 1e-3])
-def test_wet_dry_rain_2d(visc):
+@pytest.mark.parametrize("visc,ksp", [(0.0, abi.KSP_CG), (1e-3, abi.KSP_CG), (1e-3, abi.KSP_BICGSTAB)])
+def test_wet_dry_rain_2d(visc, ksp):
     """2D wetting/drying with rain, viscosity and every limiter branch active."""
     nx, ny = 24, 17
-    say(f"SWE 2D wet/dry visc={visc}")
+    say(f"SWE 2D wet/dry visc={visc} ksp={ksp}")
     rng = np.random.default_rng(4)
     x = np.arange(nx)[None, :]
     bed = 0.02 * x + 0.05 * rng.standard_normal((ny, nx))
     p = abi.swe_params(manning=0.03, dt=0.5, visc_x=visc, visc_y=visc, hE=0.001)
+    p.solver.ksp_type = ksp
     orc = ol.OracleSwe(nx, ny, 2.0, 1.5, p)
     orc.set_bathymetry(bed.ravel())
     orc.initialize_state(0.2)

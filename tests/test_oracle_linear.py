@@ -83,3 +83,100 @@ def test_max_it_exhaustion_raises():
     b = np.random.default_rng(0).standard_normal(nx * ny * nz)
     with pytest.raises(RuntimeError):
         ol.ls_solve(3, nx, ny, nz, diag, offs, b, abi.solver_config(1e-14, max_it=2))
+
+
+DIRS = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+
+
+def owned_index(ndim, nx, ny, nz, i, j, k):
+    return ((j * nx + i) * nz + k) if ndim == 3 else j * nx + i
+
+
+def random_stencil(ndim, nx, ny, nz=1, seed=0, symmetric=False, dominance=1.05):
+    """Random 5-/7-point operator in the owned ordering: off-diagonals in [-1, 0) (mirrored when
+    symmetric), diagonal = dominance * sum |off| + 0.1."""
+    rng = np.random.default_rng(seed)
+    nzz = nz if ndim == 3 else 1
+    n = nx * ny * nzz
+    nd = 6 if ndim == 3 else 4
+    offs = [np.zeros(n) for _ in range(nd)]
+    for j in range(ny):
+        for i in range(nx):
+            for k in range(nzz):
+                r = owned_index(ndim, nx, ny, nzz, i, j, k)
+                for d in range(1, nd, 2):
+                    di, dj, dk = DIRS[d]
+                    ii, jj, kk = i + di, j + dj, k + dk
+                    if ii < nx and jj < ny and kk < nzz:
+                        c = owned_index(ndim, nx, ny, nzz, ii, jj, kk)
+                        a = -rng.uniform(0.05, 1.0)
+                        offs[d][r] = a
+                        offs[d - 1][c] = a if symmetric else -rng.uniform(0.05, 1.0)
+    diag = dominance * sum(np.abs(o) for o in offs) + 0.1
+    return diag, offs
+
+
+def dense(ndim, nx, ny, nz, diag, offs):
+    nzz = nz if ndim == 3 else 1
+    A = np.diag(diag.astype(float))
+    for j in range(ny):
+        for i in range(nx):
+            for k in range(nzz):
+                r = owned_index(ndim, nx, ny, nzz, i, j, k)
+                for d in range(len(offs)):
+                    di, dj, dk = DIRS[d]
+                    ii, jj, kk = i + di, j + dj, k + dk
+                    if 0 <= ii < nx and 0 <= jj < ny and 0 <= kk < nzz:
+                        A[r, owned_index(ndim, nx, ny, nzz, ii, jj, kk)] = offs[d][r]
+    return A
+
+
+@pytest.mark.parametrize("ndim,dims", [(2, (7, 5, 1)), (3, (4, 3, 5))])
+def test_bicgstab_nonsymmetric_dense(ndim, dims):
+    """BiCGSTAB ("bcgs") on a nonsymmetric operator agrees with a dense LU solve."""
+    nx, ny, nz = dims
+    diag, offs = random_stencil(ndim, nx, ny, nz, seed=3)
+    A = dense(ndim, nx, ny, nz, diag, offs)
+    assert np.abs(A - A.T).max() > 0.1
+    b = np.random.default_rng(4).standard_normal(len(diag))
+    x, st = ol.ls_solve(ndim, nx, ny, nz, diag, offs, b, abi.solver_config(1e-13, ksp_type=abi.KSP_BICGSTAB))
+    assert st.converged
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), atol=1e-10)
+
+
+def indefinite_surface_operator(nx=9, ny=7, seed=5):
+    """A 5-point surface system (diag = dx*dy + sum of couplings) with one dry row whose diagonal is
+    dx*dy = 1 while it keeps a coupling of 2.14 to a deep neighbour, as shallowwaterMatCoeff
+    (src/swe/SweSolver.cpp:494-505) produces at a wetting front."""
+    diag, offs = random_stencil(2, nx, ny, seed=seed, symmetric=True, dominance=1.0)
+    diag = diag - 0.1 + 1.0
+    r, c = 3 * nx + 4, 4 * nx + 4  # dry row and its y+ neighbour
+    step = [-1, 1, -nx, nx]
+    for row in (r, c):  # weak couplings elsewhere, in both directions
+        for d in range(4):
+            if offs[d][row] != 0.0:
+                offs[d][row] *= 1e-3
+                offs[d ^ 1][row + step[d]] = offs[d][row]
+    offs[3][r] = offs[2][c] = -2.14
+    diag = 1.0 + sum(np.abs(o) for o in offs)
+    diag[r] = 1.0
+    return diag, offs
+
+
+def test_bicgstab_indefinite_surface_row():
+    nx, ny = 9, 7
+    diag, offs = indefinite_surface_operator(nx, ny)
+    A = dense(2, nx, ny, 1, diag, offs)
+    assert np.allclose(A, A.T) and np.linalg.eigvalsh(A).min() < 0
+    b = np.random.default_rng(6).uniform(0.5, 1.5, nx * ny)
+    x, st = ol.ls_solve(2, nx, ny, 1, diag, offs, b, abi.solver_config(1e-13, ksp_type=abi.KSP_BICGSTAB))
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), atol=1e-9)
+
+
+def test_bicgstab_equals_cg_on_spd():
+    nx, ny, nz = 5, 4, 6
+    diag, offs = random_stencil(3, nx, ny, nz, seed=8, symmetric=True)
+    b = np.random.default_rng(9).standard_normal(len(diag))
+    xc, _ = ol.ls_solve(3, nx, ny, nz, diag, offs, b, abi.solver_config(1e-13))
+    xb, _ = ol.ls_solve(3, nx, ny, nz, diag, offs, b, abi.solver_config(1e-13, ksp_type=abi.KSP_BICGSTAB))
+    np.testing.assert_allclose(xb, xc, atol=1e-11)
