@@ -62,13 +62,15 @@ def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None):
     elapsed = time.perf_counter() - t0
     kt = {k: ctx.profile_read(v) for k, v in (("spmv64", abi.KT_GW_SPMV), ("spmv32", abi.KT_GW_SPMV32),
                                                  ("update", abi.KT_GW_UPDATE), ("assemble", abi.KT_GW_ASSEMBLE),
-                                                 ("correct", abi.KT_GW_CORRECT))}
+                                                 ("correct", abi.KT_GW_CORRECT), ("refine", abi.KT_GW_REFINE),
+                                                 ("sw_spmv", abi.KT_SW_SPMV), ("sw_update", abi.KT_SW_UPDATE))}
     ctx.profile_enable(False)
+    shape = ctx.spmv_shape(abi.PREC_FP64 if path == "plain" else abi.PREC_MIXED)
     h = ctx.re_get("h")
     sw_vol, gw_vol = ctx.water_volume()
     ctx.close()
     return {"elapsed": elapsed, "iters": iters, "refines": refines, "kt": kt, "h": h, "grabbed": grabbed,
-            "volume": sw_vol + gw_vol}
+            "volume": sw_vol + gw_vol, "spmv_shape": shape}
 
 
 def cpu_baseline(case, abi, steps):
@@ -100,11 +102,26 @@ def main():
     ap.add_argument("--nz", type=int, default=30)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run an elementwise 1 GiB -> 1 GiB kernel (torch), the reference point that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); the multi-GPU "
               "sharding of the coupled step is a design, not a build.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+        n = (1 << 30) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.add(a, 1.0, out=b)  # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        del a, b
+        log("[bench] copy probe: 3 elementwise kernels, 1 GiB read + 1 GiB written each")
 
     from frehg_amd import abi, gpu, workloads
     log(f"[bench] generating inputs {args.nx}x{args.ny}x{args.nz}")
@@ -182,6 +199,7 @@ def main():
         "roofline_kernel": "gw_pcg_spmv fp32 (fused p-update + 7-pt SpMV + dot)" if dominant == "spmv32"
                            else "gw_pcg_spmv fp64 (fused p-update + 7-pt SpMV + dot)",
         "roofline_fp64_spmv": roof64,
+        "spmv_shape_tile_y_kchunk": {"plain_fp64": plain["spmv_shape"], "fast_fp32": fast["spmv_shape"]},
         "kernel_ms_fast": {k: round(v[0], 3) for k, v in fast["kt"].items()},
         "kernel_ms_plain": {k: round(v[0], 3) for k, v in plain["kt"].items()},
         "cpu_baseline": cpu,

@@ -34,7 +34,10 @@ KT_GW_SPMV32 = 1
 KT_GW_UPDATE = 2
 KT_GW_ASSEMBLE = 3
 KT_GW_CORRECT = 4
-KT_COUNT = 5
+KT_SW_SPMV = 5
+KT_SW_UPDATE = 6
+KT_GW_REFINE = 7
+KT_COUNT = 8
 
 BC_FREE_DRAINAGE = 3
 

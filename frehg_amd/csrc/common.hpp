@@ -117,6 +117,33 @@ __global__ __launch_bounds__(kRedThreads) inline void k_reduce_maxmin(const doub
 }
 
 // ---------------------------------------------------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs of the device (linear workgroup id mod 8),
+// and each XCD has its own L2. xcd_remap is a bijection of [0, n) that hands every XCD one
+// contiguous run of tile indices, so stencil neighbours (adjacent tiles, and the cache lines that
+// straddle tile edges) are read through the same L2 at about the same time.
+__device__ __forceinline__ unsigned int xcd_remap(unsigned int id, unsigned int n) {
+  constexpr unsigned int kXcd = 8;
+  const unsigned int q = n / kXcd, r = n % kXcd, x = id % kXcd;
+  return x * q + (x < r ? x : r) + id / kXcd;
+}
+
+struct TileId {
+  unsigned int x, y, z, linear;  // tile coordinates (x fastest) and the launch's linear block id
+};
+
+// Tile coordinates of this workgroup under xcd_remap, for a grid launched as gridDim.
+__device__ __forceinline__ TileId xcd_tile() {
+  const unsigned int gx = gridDim.x, gy = gridDim.y;
+  const unsigned int id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned int t = xcd_remap(id, gx * gy * gridDim.z);
+  TileId r;
+  r.x = t % gx;
+  r.y = (t / gx) % gy;
+  r.z = t / (gx * gy);
+  r.linear = id;
+  return r;
+}
+
 // Soil class with derived constants (include/frehg2/re/SoilParams.hpp:13-25).
 struct SoilClass {
   double alpha, n, theta_s, theta_r, Ks_x, Ks_y, Ks_z, Ss, air_entry;

@@ -15,6 +15,8 @@
 //   update: x += alpha y + omega z, r = s - omega t, partials rh.r, ||D^-1 r||^2;
 //                                                fin: convergence test, beta, rho
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -34,10 +36,11 @@ struct Tile {
 };
 
 __device__ inline Tile tile_of(const Geo& g, int kchunk) {
+  const TileId b = xcd_tile();
   Tile t;
-  t.i = blockIdx.x * TX + (threadIdx.x & (TX - 1));
-  t.j = blockIdx.y * TY + (threadIdx.x / TX);
-  t.k0 = blockIdx.z * kchunk;
+  t.i = b.x * TX + (threadIdx.x & (TX - 1));
+  t.j = b.y * TY + (threadIdx.x / TX);
+  t.k0 = b.z * kchunk;
   t.k1 = min(g.nz, t.k0 + kchunk);
   t.valid = t.i < g.nx && t.j < g.ny;
   return t;
@@ -161,10 +164,41 @@ __global__ __launch_bounds__(kRedThreads) void k_fin_init(const double* part, un
   }
 }
 
+// Operands of one layer of the z-march: z and p_prev at the four lateral neighbours and the
+// cell below, and the row's coefficients. The next layer's set is loaded before the current
+// layer is computed, so each wave keeps a full layer of loads in flight.
+template <typename T>
+struct SpmvLayer {
+  T zxm, pxm, zxp, pxp, zym, pym, zyp, pyp, zzp, pzp, d, axc, axm, ayc, aym, azc;
+};
+
 template <typename T, bool THREE_D>
-__global__ __launch_bounds__(NT) void k_spmv(Geo g, int kchunk, StencilView<T> A, const T* z,
-                                              const T* pold, T* pnew, T* q, double* part,
-                                              const KrylovCtl* ctl) {
+__device__ __forceinline__ void spmv_load(const Geo& g, const StencilView<T>& A, const T* __restrict__ z,
+                                          const T* __restrict__ pold, long long c, SpmvLayer<T>& v) {
+  v.zxm = z[c - 1];
+  v.pxm = pold[c - 1];
+  v.zxp = z[c + 1];
+  v.pxp = pold[c + 1];
+  v.zym = z[c - g.sy];
+  v.pym = pold[c - g.sy];
+  v.zyp = z[c + g.sy];
+  v.pyp = pold[c + g.sy];
+  v.d = A.D[c];
+  v.axc = A.ax[c];
+  v.axm = A.ax[c - 1];
+  v.ayc = A.ay[c];
+  v.aym = A.ay[c - g.sy];
+  if (THREE_D) {
+    v.zzp = z[c + g.sz];
+    v.pzp = pold[c + g.sz];
+    v.azc = A.az[c];
+  }
+}
+
+template <typename T, bool THREE_D>
+__global__ __launch_bounds__(NT) void k_spmv(Geo g, int kchunk, StencilView<T> A, const T* __restrict__ z,
+                                              const T* __restrict__ pold, T* __restrict__ pnew, T* __restrict__ q,
+                                              double* part, const KrylovCtl* ctl) {
   __shared__ double lds[NT / kWave];
   if (ctl->done) return;
   const double beta = ctl->beta;
@@ -172,33 +206,141 @@ __global__ __launch_bounds__(NT) void k_spmv(Geo g, int kchunk, StencilView<T> A
   double acc = 0.0;
   if (t.valid) {
     long long c = base_index<THREE_D>(g, t);
-    auto P = [&](long long cc) -> double { return (double)(T)((double)z[cc] + beta * (double)pold[cc]); };
+    auto P = [&](T zz, T pp) -> double { return (double)(T)((double)zz + beta * (double)pp); };
     double pm = 0.0, azm = 0.0;
     if (THREE_D) {
-      pm = P(c - g.sz);
+      pm = P(z[c - g.sz], pold[c - g.sz]);
       azm = (double)A.az[c - g.sz];
     }
-    double pc = P(c);
+    double pc = P(z[c], pold[c]);
+    SpmvLayer<T> cur, nxt;
+    spmv_load<T, THREE_D>(g, A, z, pold, c, cur);
     for (int k = t.k0; k < t.k1; ++k) {
-      double pp = 0.0, azc = 0.0;
-      if (THREE_D) {
-        pp = P(c + g.sz);
-        azc = (double)A.az[c];
-      }
-      double y = (double)A.D[c] * pc + (double)A.ax[c] * P(c + 1) + (double)A.ax[c - 1] * P(c - 1) +
-                 (double)A.ay[c] * P(c + g.sy) + (double)A.ay[c - g.sy] * P(c - g.sy);
-      if (THREE_D) y += azc * pp + azm * pm;
+      const bool more = THREE_D && k + 1 < t.k1;
+      if (more) spmv_load<T, THREE_D>(g, A, z, pold, c + g.sz, nxt);
+      const double pp = THREE_D ? P(cur.zzp, cur.pzp) : 0.0;
+      double y = (double)cur.d * pc + (double)cur.axc * P(cur.zxp, cur.pxp) + (double)cur.axm * P(cur.zxm, cur.pxm) +
+                 (double)cur.ayc * P(cur.zyp, cur.pyp) + (double)cur.aym * P(cur.zym, cur.pym);
+      if (THREE_D) y += (double)cur.azc * pp + azm * pm;
       const T yT = (T)y;
       pnew[c] = (T)pc;
       q[c] = yT;
       acc += pc * (double)yT;
       pm = pc;
       pc = pp;
-      azm = azc;
+      azm = (double)cur.azc;
       c += g.sz;
+      if (more) cur = nxt;
     }
   }
   const double s = block_sum<NT>(acc, lds);
+  if (threadIdx.x == 0) part[block_id()] = s;
+}
+
+// CG SpMV on 64 x TYL tiles, 3D. Same arithmetic as k_spmv (identical results), but each
+// p = z + beta p_prev is formed once per cell: x-neighbours come from the adjacent lanes of the
+// wave (the two edge lanes load theirs), y-neighbours from the adjacent waves through a
+// double-buffered LDS tile whose halo rows the first and last wave load, and the z-neighbours
+// from the registers of the z-march. That leaves six full-wave loads per cell and layer
+// (z and p_prev of the layer below, D, ax, ay, az), prefetched one layer ahead.
+template <typename T>
+struct SpmvTileLayer {
+  T zzp, pzp, d, axc, ayc, azc;  // next-layer centre; this layer's coefficients
+  T ze, pe, axe;                 // x-edge lanes: z, p_prev at c -/+ 1 and ax(c - 1)
+  T zh, ph, ayh;                 // halo waves: z, p_prev at c -/+ sy and ay(c - sy)
+};
+
+template <typename T, int TYL>
+__global__ __launch_bounds__(64 * TYL) void k_spmv_tile(Geo g, int kchunk, StencilView<T> A, const T* __restrict__ z,
+                                                         const T* __restrict__ pold, T* __restrict__ pnew,
+                                                         T* __restrict__ q, double* part, const KrylovCtl* ctl) {
+  constexpr int NTL = 64 * TYL;
+  __shared__ double ptile[2][TYL + 2][64];
+  __shared__ double aytile[2][TYL + 1][64];
+  __shared__ double lds[NTL / kWave];
+  if (ctl->done) return;  // uniform: every thread reads the same flag
+  const double beta = ctl->beta;
+  const TileId b = xcd_tile();
+  const int lane = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int i = b.x * 64 + lane, j = b.y * TYL + ty;
+  const int k0 = b.z * kchunk, k1 = min(g.nz, k0 + kchunk);
+  const bool valid = i < g.nx && j < g.ny;
+  const bool first_row = ty == 0, last_row = ty == TYL - 1;
+  auto P = [&](T zz, T pp) -> double { return (double)(T)((double)zz + beta * (double)pp); };
+  long long c = valid ? g.idx(i, j, k0) : 0;
+  auto load = [&](long long cc, SpmvTileLayer<T>& v) {
+    if (!valid) return;
+    v.zzp = z[cc + g.sz];
+    v.pzp = pold[cc + g.sz];
+    v.d = A.D[cc];
+    v.axc = A.ax[cc];
+    v.ayc = A.ay[cc];
+    v.azc = A.az[cc];
+    if (lane == 0 || lane == 63) {
+      const long long ce = lane == 0 ? cc - 1 : cc + 1;
+      v.ze = z[ce];
+      v.pe = pold[ce];
+    }
+    if (lane == 0) v.axe = A.ax[cc - 1];
+    if (first_row) {
+      v.zh = z[cc - g.sy];
+      v.ph = pold[cc - g.sy];
+      v.ayh = A.ay[cc - g.sy];
+    } else if (last_row) {
+      v.zh = z[cc + g.sy];
+      v.ph = pold[cc + g.sy];
+    }
+  };
+  double pm = 0.0, azm = 0.0, pc = 0.0, acc = 0.0;
+  if (valid) {
+    pm = P(z[c - g.sz], pold[c - g.sz]);
+    azm = (double)A.az[c - g.sz];
+    pc = P(z[c], pold[c]);
+  }
+  SpmvTileLayer<T> cur{}, nxt{};
+  load(c, cur);
+  int buf = 0;
+  for (int k = k0; k < k1; ++k) {
+    const bool more = k + 1 < k1;
+    if (more) load(c + g.sz, nxt);
+    ptile[buf][ty + 1][lane] = pc;
+    aytile[buf][ty + 1][lane] = (double)cur.ayc;
+    if (first_row) {
+      ptile[buf][0][lane] = valid ? P(cur.zh, cur.ph) : 0.0;
+      aytile[buf][0][lane] = (double)cur.ayh;
+    } else if (last_row) {
+      ptile[buf][TYL + 1][lane] = valid ? P(cur.zh, cur.ph) : 0.0;
+    }
+    double pl = __shfl_up(pc, 1, 64);
+    double pr = __shfl_down(pc, 1, 64);
+    double axl = __shfl_up((double)cur.axc, 1, 64);
+    if (lane == 0) {
+      pl = valid ? P(cur.ze, cur.pe) : 0.0;
+      axl = (double)cur.axe;
+    } else if (lane == 63) {
+      pr = valid ? P(cur.ze, cur.pe) : 0.0;
+    }
+    // LDS-only barrier: waits for this wave's LDS writes, not for its global loads in flight
+    // (__syncthreads would also drain the prefetched next layer).
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    const double pym = ptile[buf][ty][lane], pyp = ptile[buf][ty + 2][lane], aym = aytile[buf][ty][lane];
+    if (valid) {
+      const double pp = P(cur.zzp, cur.pzp);
+      double y = (double)cur.d * pc + (double)cur.axc * pr + axl * pl + (double)cur.ayc * pyp + aym * pym;
+      y += (double)cur.azc * pp + azm * pm;
+      const T yT = (T)y;
+      pnew[c] = (T)pc;
+      q[c] = yT;
+      acc += pc * (double)yT;
+      pm = pc;
+      pc = pp;
+      azm = (double)cur.azc;
+    }
+    c += g.sz;
+    buf ^= 1;
+    if (more) cur = nxt;
+  }
+  const double s = block_sum<NTL>(acc, lds);
   if (threadIdx.x == 0) part[block_id()] = s;
 }
 
@@ -585,7 +727,24 @@ Krylov::Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof)
     : g_(g), three_d_(three_d), s_(stream), prof_(prof) {
   if (!three_d_) g_.nz = 1;
   n_ = three_d_ ? g_.ncell() : g_.nsurf();
-  set_kchunks(three_d_ ? std::min(g_.nz, 4) : 1, 1);
+  kc_f_ = 1;
+  grid_f_ = dim3(ceil_div(g_.nx, TX), ceil_div(g_.ny, TY), ceil_div(three_d_ ? g_.nz : 1, kc_f_));
+  nb_f_ = grid_f_.x * grid_f_.y * grid_f_.z;
+  // partials sized for the largest grid any kernel (or candidate) uses: 64 x 4 tiles, one layer each
+  part_cap_ = ceil_div(g_.nx, TX) * ceil_div(g_.ny, TY) * (unsigned int)(three_d_ ? g_.nz : 1);
+  FREHG_HIP(hipMalloc(&part_, 3 * (size_t)part_cap_ * sizeof(double)));
+  for (int prec = 0; prec < 2; ++prec) set_spmv_shape(prec, 0, three_d_ ? std::min(g_.nz, 4) : 1);
+  // FREHG_SPMV_SHAPE="ty:kc" pins the SpMV shape (ty in {0, 4, 8, 16}) and skips the autotuner.
+  if (const char* env = std::getenv("FREHG_SPMV_SHAPE")) {
+    int ty = -1, kc = -1;
+    if (std::sscanf(env, "%d:%d", &ty, &kc) != 2 || !(ty == 0 || ty == 4 || ty == 8 || ty == 16) || kc < 1)
+      throw ArgError(std::string("FREHG_SPMV_SHAPE must be \"ty:kc\" with ty in {0,4,8,16} and kc >= 1, got \"") +
+                     env + "\"");
+    for (int prec = 0; prec < 2; ++prec) {
+      set_spmv_shape(prec, ty, kc);
+      tuned_[prec] = true;
+    }
+  }
   const size_t bytes = (size_t)n_ * sizeof(double);
   for (double** p : {&r_, &z_, &p0_, &p1_, &q_, &xw_}) {
     FREHG_HIP(hipMalloc(p, bytes));
@@ -606,57 +765,80 @@ Krylov::~Krylov() {
   if (host_ctl_) (void)hipHostFree(host_ctl_);
 }
 
-void Krylov::set_kchunks(int kc_spmv, int kc_flat) {
+void Krylov::set_spmv_shape(int prec, int ty, int kc) {
   const int nz = three_d_ ? g_.nz : 1;
-  kc_s_ = std::max(1, std::min(kc_spmv, nz));
-  kc_f_ = std::max(1, std::min(kc_flat, nz));
-  grid_s_ = dim3(ceil_div(g_.nx, TX), ceil_div(g_.ny, TY), ceil_div(nz, kc_s_));
-  grid_f_ = dim3(ceil_div(g_.nx, TX), ceil_div(g_.ny, TY), ceil_div(nz, kc_f_));
-  nb_s_ = grid_s_.x * grid_s_.y * grid_s_.z;
-  nb_f_ = grid_f_.x * grid_f_.y * grid_f_.z;
-  // partials sized for the largest grid any kernel (or candidate) can use: one layer per block
-  const unsigned int cap = ceil_div(g_.nx, TX) * ceil_div(g_.ny, TY) * (unsigned int)nz;
-  if (cap > part_cap_) {
-    if (part_) FREHG_HIP(hipFree(part_));
-    FREHG_HIP(hipMalloc(&part_, 3 * (size_t)cap * sizeof(double)));
-    part_cap_ = cap;
-  }
+  if (!three_d_) ty = 0;
+  SpmvShape& sh = shape_[prec];
+  sh.ty = ty;
+  sh.kc = std::max(1, std::min(kc, nz));
+  const int tyy = ty == 0 ? TY : ty;
+  sh.grid = dim3(ceil_div(g_.nx, TX), ceil_div(g_.ny, tyy), ceil_div(nz, sh.kc));
+  sh.nb = sh.grid.x * sh.grid.y * sh.grid.z;
+  if (sh.nb > part_cap_) throw HipError("Krylov: SpMV shape needs more partials than allocated");
 }
 
-void Krylov::autotune(const StencilView<double>& A) {
-  tuned_ = true;
+template <typename T>
+void Krylov::launch_spmv(const StencilView<T>& A, const T* z, const T* pold, T* pnew, T* q, const KrylovCtl* ctl,
+                         hipEvent_t ev0, hipEvent_t ev1) {
+  const SpmvShape& sh = shape_[sizeof(T) == 4 ? 1 : 0];
+  if (!three_d_) {
+    hipExtLaunchKernelGGL((k_spmv<T, false>), sh.grid, dim3(NT), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold, pnew, q,
+                          part_, ctl);
+  } else if (sh.ty == 0) {
+    hipExtLaunchKernelGGL((k_spmv<T, true>), sh.grid, dim3(NT), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold, pnew, q,
+                          part_, ctl);
+  } else if (sh.ty == 4) {
+    hipExtLaunchKernelGGL((k_spmv_tile<T, 4>), sh.grid, dim3(64 * 4), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold,
+                          pnew, q, part_, ctl);
+  } else if (sh.ty == 8) {
+    hipExtLaunchKernelGGL((k_spmv_tile<T, 8>), sh.grid, dim3(64 * 8), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold,
+                          pnew, q, part_, ctl);
+  } else {
+    hipExtLaunchKernelGGL((k_spmv_tile<T, 16>), sh.grid, dim3(64 * 16), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold,
+                          pnew, q, part_, ctl);
+  }
+  FREHG_LAUNCH_CHECK();
+}
+
+template <typename T>
+void Krylov::autotune(const StencilView<T>& A) {
+  const int prec = sizeof(T) == 4 ? 1 : 0;
+  tuned_[prec] = true;
   if (!three_d_) return;
+  if (prec == 1) ensure_mixed();
   KrylovCtl* tctl = ctl_ + 2;
   FREHG_HIP(hipMemsetAsync(tctl, 0, sizeof(KrylovCtl), s_));
+  T* p0 = prec ? (T*)p032_ : (T*)p0_;
+  T* p1 = prec ? (T*)p132_ : (T*)p1_;
+  T* qq = prec ? (T*)q32_ : (T*)q_;
   hipEvent_t e0, e1;
   FREHG_HIP(hipEventCreate(&e0));
   FREHG_HIP(hipEventCreate(&e1));
   const int nz = g_.nz;
-  std::vector<int> cand;
-  for (int kc : {1, 2, 3, 4, 5, 6, 8, 10, 15, nz})
-    if (kc >= 1 && kc <= nz && std::find(cand.begin(), cand.end(), kc) == cand.end()) cand.push_back(kc);
-  int best = kc_s_;
+  std::vector<int> kcs;
+  for (int kc : {1, 2, 4, 8, 15, nz})
+    if (kc >= 1 && kc <= nz && std::find(kcs.begin(), kcs.end(), kc) == kcs.end()) kcs.push_back(kc);
+  int best_ty = shape_[prec].ty, best_kc = shape_[prec].kc;
   float best_ms = 1e30f;
-  const int flat = kc_f_;
-  for (int kc : cand) {
-    set_kchunks(kc, flat);
-    for (int rep = 0; rep < 2; ++rep) {  // first launch warms caches / clocks
-      FREHG_HIP(hipEventRecord(e0, s_));
-      for (int it = 0; it < 3; ++it)
-        hipLaunchKernelGGL((k_spmv<double, true>), grid_s_, dim3(NT), 0, s_, g_, kc_s_, A, A.D,
-                           (const double*)p0_, p1_, q_, part_, tctl);
-      FREHG_LAUNCH_CHECK();
-      FREHG_HIP(hipEventRecord(e1, s_));
-      FREHG_HIP(hipEventSynchronize(e1));
-    }
-    float ms = 0.f;
-    FREHG_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = kc;
+  for (int ty : {0, 4, 8, 16}) {
+    for (int kc : kcs) {
+      set_spmv_shape(prec, ty, kc);
+      float ms = 0.f;
+      for (int rep = 0; rep < 2; ++rep) {  // the first round warms caches and clocks
+        FREHG_HIP(hipEventRecord(e0, s_));
+        for (int it = 0; it < 3; ++it) launch_spmv<T>(A, A.D, p0, p1, qq, tctl, nullptr, nullptr);
+        FREHG_HIP(hipEventRecord(e1, s_));
+        FREHG_HIP(hipEventSynchronize(e1));
+        FREHG_HIP(hipEventElapsedTime(&ms, e0, e1));
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_ty = ty;
+        best_kc = kc;
+      }
     }
   }
-  set_kchunks(best, flat);
+  set_spmv_shape(prec, best_ty, best_kc);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
 }
@@ -730,22 +912,22 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
     for (int it = 0; it < batch; ++it) {
       T* pold = pbuf[parity];
       T* pnew = pbuf[parity ^ 1];
-      prof_->begin(kt_spmv);
-      if (three_d_)
-        hipLaunchKernelGGL((k_spmv<T, true>), grid_s_, blk, 0, s_, g_, kc_s_, A, z, pold, pnew, q, part_, ctl);
-      else
-        hipLaunchKernelGGL((k_spmv<T, false>), grid_s_, blk, 0, s_, g_, kc_s_, A, z, pold, pnew, q, part_, ctl);
+      {
+        hipEvent_t ev0 = prof_->start(kt_spmv);
+        hipEvent_t ev1 = prof_->stop();
+        launch_spmv<T>(A, z, pold, pnew, q, ctl, ev0, ev1);
+      }
+      hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(kRedThreads), 0, s_, part_, shape_[sizeof(T) == 4 ? 1 : 0].nb, ctl);
       FREHG_LAUNCH_CHECK();
-      prof_->end();
-      hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_s_, ctl);
-      FREHG_LAUNCH_CHECK();
-      prof_->begin(2);
-      if (three_d_)
-        hipLaunchKernelGGL((k_update<T, true>), grid_f_, blk, 0, s_, g_, kc_f_, A.D, pnew, q, x, r, z, part_, ctl, nb_f_);
-      else
-        hipLaunchKernelGGL((k_update<T, false>), grid_f_, blk, 0, s_, g_, kc_f_, A.D, pnew, q, x, r, z, part_, ctl, nb_f_);
-      FREHG_LAUNCH_CHECK();
-      prof_->end();
+      {
+        hipEvent_t ev0 = prof_->start(three_d_ ? FREHG_KT_GW_UPDATE : FREHG_KT_SW_UPDATE);
+        hipEvent_t ev1 = prof_->stop();
+        if (three_d_)
+          hipExtLaunchKernelGGL((k_update<T, true>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A.D, pnew, q, x, r, z, part_, ctl, nb_f_);
+        else
+          hipExtLaunchKernelGGL((k_update<T, false>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A.D, pnew, q, x, r, z, part_, ctl, nb_f_);
+        FREHG_LAUNCH_CHECK();
+      }
       hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_f_, ctl);
       FREHG_LAUNCH_CHECK();
       parity ^= 1;
@@ -816,37 +998,43 @@ int Krylov::run_bicgstab(const StencilView<T>& A, const T* b, const T* bn, T* x,
       T* pnew = pbuf[parity ^ 1];
       const T* vold = vbuf[parity];
       T* vnew = vbuf[parity ^ 1];
-      prof_->begin(kt_spmv);
-      if (three_d_)
-        hipLaunchKernelGGL((k_bcg_spmv1<T, true>), grid_s_, blk, 0, s_, g_, kc_s_, A, (const T*)r, (const T*)rh, pold,
-                           vold, pnew, vnew, part_, ctl);
-      else
-        hipLaunchKernelGGL((k_bcg_spmv1<T, false>), grid_s_, blk, 0, s_, g_, kc_s_, A, (const T*)r, (const T*)rh,
-                           pold, vold, pnew, vnew, part_, ctl);
+      {
+        hipEvent_t ev0 = prof_->start(kt_spmv);
+        hipEvent_t ev1 = prof_->stop();
+        if (three_d_)
+          hipExtLaunchKernelGGL((k_bcg_spmv1<T, true>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A, (const T*)r, (const T*)rh, pold,
+                             vold, pnew, vnew, part_, ctl);
+        else
+          hipExtLaunchKernelGGL((k_bcg_spmv1<T, false>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A, (const T*)r, (const T*)rh,
+                             pold, vold, pnew, vnew, part_, ctl);
+        FREHG_LAUNCH_CHECK();
+      }
+      hipLaunchKernelGGL(k_fin_bcg1, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_f_, ctl);
       FREHG_LAUNCH_CHECK();
-      prof_->end();
-      hipLaunchKernelGGL(k_fin_bcg1, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_s_, ctl);
+      {
+        hipEvent_t ev0 = prof_->start(kt_spmv);
+        hipEvent_t ev1 = prof_->stop();
+        if (three_d_)
+          hipExtLaunchKernelGGL((k_bcg_spmv2<T, true>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A, (const T*)r, (const T*)vnew, sv,
+                             tv, part_, ctl, nb_f_);
+        else
+          hipExtLaunchKernelGGL((k_bcg_spmv2<T, false>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A, (const T*)r, (const T*)vnew,
+                             sv, tv, part_, ctl, nb_f_);
+        FREHG_LAUNCH_CHECK();
+      }
+      hipLaunchKernelGGL(k_fin_bcg2, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_f_, ctl);
       FREHG_LAUNCH_CHECK();
-      prof_->begin(kt_spmv);
-      if (three_d_)
-        hipLaunchKernelGGL((k_bcg_spmv2<T, true>), grid_s_, blk, 0, s_, g_, kc_s_, A, (const T*)r, (const T*)vnew, sv,
-                           tv, part_, ctl, nb_s_);
-      else
-        hipLaunchKernelGGL((k_bcg_spmv2<T, false>), grid_s_, blk, 0, s_, g_, kc_s_, A, (const T*)r, (const T*)vnew,
-                           sv, tv, part_, ctl, nb_s_);
-      FREHG_LAUNCH_CHECK();
-      prof_->end();
-      hipLaunchKernelGGL(k_fin_bcg2, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_s_, ctl);
-      FREHG_LAUNCH_CHECK();
-      prof_->begin(2);
-      if (three_d_)
-        hipLaunchKernelGGL((k_bcg_update<T, true>), grid_f_, blk, 0, s_, g_, kc_f_, A.D, (const T*)pnew, (const T*)sv,
-                           (const T*)tv, (const T*)rh, x, r, part_, ctl, nb_f_);
-      else
-        hipLaunchKernelGGL((k_bcg_update<T, false>), grid_f_, blk, 0, s_, g_, kc_f_, A.D, (const T*)pnew,
-                           (const T*)sv, (const T*)tv, (const T*)rh, x, r, part_, ctl, nb_f_);
-      FREHG_LAUNCH_CHECK();
-      prof_->end();
+      {
+        hipEvent_t ev0 = prof_->start(three_d_ ? FREHG_KT_GW_UPDATE : FREHG_KT_SW_UPDATE);
+        hipEvent_t ev1 = prof_->stop();
+        if (three_d_)
+          hipExtLaunchKernelGGL((k_bcg_update<T, true>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A.D, (const T*)pnew, (const T*)sv,
+                             (const T*)tv, (const T*)rh, x, r, part_, ctl, nb_f_);
+        else
+          hipExtLaunchKernelGGL((k_bcg_update<T, false>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A.D, (const T*)pnew,
+                             (const T*)sv, (const T*)tv, (const T*)rh, x, r, part_, ctl, nb_f_);
+        FREHG_LAUNCH_CHECK();
+      }
       hipLaunchKernelGGL(k_fin_bcg3, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_f_, ctl);
       FREHG_LAUNCH_CHECK();
       parity ^= 1;
@@ -892,11 +1080,12 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
                           const double* bn, double* x, const unsigned char* fixed, bool x0_zero,
                           bool guard_guess, const KrylovConfig& cfg) {
   SolveResult res;
-  if (!tuned_) autotune(A);
+  if (!tuned_[0]) autotune<double>(A);
   if (cfg.precision == 0 || A32 == nullptr) {
     bool rej = false, conv = false;
     res.iterations = run<double>(cfg.ksp, A, b, bn, x, fixed, x0_zero, guard_guess, cfg.rtol, cfg.atol,
-                                 cfg.max_it, ctl_, &rej, &res.residual_norm, &res.rhs_norm, &conv, 0);
+                                 cfg.max_it, ctl_, &rej, &res.residual_norm, &res.rhs_norm, &conv,
+                                 three_d_ ? FREHG_KT_GW_SPMV : FREHG_KT_SW_SPMV);
     res.guess_rejected = rej;
     res.converged = conv;
     if (!conv)
@@ -921,12 +1110,14 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
   double* xalt = xw_;
   auto refine = [&](const double* xin, const float* d, double* xout, bool first) {
     const bool sym = A.symmetric();
+    hipEvent_t ev0 = three_d_ ? prof_->start(FREHG_KT_GW_REFINE) : nullptr;
+    hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
     if (three_d_ && sym)
-      hipLaunchKernelGGL((k_refine<true, true>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, xin, d, xout, b32_, fixed,
-                         part_, nb, first ? 1 : 0);
+      hipExtLaunchKernelGGL((k_refine<true, true>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A, b, bn, xin, d,
+                            xout, b32_, fixed, part_, nb, first ? 1 : 0);
     else if (three_d_)
-      hipLaunchKernelGGL((k_refine<true, false>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, xin, d, xout, b32_,
-                         fixed, part_, nb, first ? 1 : 0);
+      hipExtLaunchKernelGGL((k_refine<true, false>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A, b, bn, xin, d,
+                            xout, b32_, fixed, part_, nb, first ? 1 : 0);
     else if (sym)
       hipLaunchKernelGGL((k_refine<false, true>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, xin, d, xout, b32_,
                          fixed, part_, nb, first ? 1 : 0);
@@ -953,11 +1144,13 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
   res.residual_norm = std::sqrt(host_ctl_->zz);
   bool conv = host_ctl_->converged != 0;
   const StencilView<float> A32v = *A32;
+  if (!tuned_[1]) autotune<float>(A32v);
   while (!conv && res.refinements < cfg.max_refine) {
     bool rej = false, iconv = false;
     double rn = 0, bnrm = 0;
     res.iterations += run<float>(cfg.ksp, A32v, b32_, nullptr, d32_, fixed, true, false, cfg.inner_rtol, 0.0,
-                                 cfg.max_it, inner, &rej, &rn, &bnrm, &iconv, 1);
+                                 cfg.max_it, inner, &rej, &rn, &bnrm, &iconv,
+                                 three_d_ ? FREHG_KT_GW_SPMV32 : FREHG_KT_SW_SPMV);
     refine(xcur, d32_, xalt, false);
     std::swap(xcur, xalt);
     res.refinements += 1;
@@ -971,7 +1164,8 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
     bool rej = false, c64 = false;
     res.fallback_fp64 = true;
     res.iterations += run<double>(cfg.ksp, A, b, bn, x, fixed, false, false, cfg.rtol, cfg.atol, cfg.max_it, ctl_,
-                                  &rej, &res.residual_norm, &res.rhs_norm, &c64, 0);
+                                  &rej, &res.residual_norm, &res.rhs_norm, &c64,
+                                  three_d_ ? FREHG_KT_GW_SPMV : FREHG_KT_SW_SPMV);
     conv = c64;
     // the FP64 test above is relative to ||D^-1 b||, the same normalisation as the outer test
   }

@@ -91,14 +91,21 @@ class Krylov {
                     const double* bn, double* x, const unsigned char* fixed, bool x0_zero,
                     bool guard_guess, const KrylovConfig& cfg);
 
-  // Layers per thread of the SpMV's z-march (kc_spmv) and of the streaming kernels (kc_flat).
-  int kchunk_spmv() const { return kc_s_; }
-  int kchunk_flat() const { return kc_f_; }
-  void set_kchunks(int kc_spmv, int kc_flat);
-  // Times the SpMV for each candidate z-chunk on the operator at hand and keeps the fastest
-  // (run automatically at the first 3D solve; the result is cached per instance).
-  void autotune(const StencilView<double>& A);
-  bool tuned() const { return tuned_; }
+  // CG SpMV launch shape per precision (0 FP64, 1 FP32): tile height ty (0 = one column per
+  // thread, neighbours from L1/L2; 4/8/16 = 64 x ty tiles whose y-halo is staged in LDS and
+  // whose x-neighbours come from wave shuffles) and layers per block of the z-march (kc).
+  struct SpmvShape {
+    int ty = 0, kc = 1;
+    dim3 grid;
+    unsigned int nb = 0;
+  };
+  const SpmvShape& spmv_shape(int prec) const { return shape_[prec]; }
+  void set_spmv_shape(int prec, int ty, int kc);
+  // Times the CG SpMV for every candidate shape on the operator at hand and keeps the fastest
+  // (run automatically at the first 3D solve of each precision; cached per instance).
+  template <typename T>
+  void autotune(const StencilView<T>& A);
+  bool tuned(int prec) const { return tuned_[prec]; }
 
  private:
   template <typename T>
@@ -106,6 +113,9 @@ class Krylov {
               bool x0_zero, bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl,
               T* r, T* z, T* p0, T* p1, T* q, bool* rejected, double* res_norm, double* rhs_norm,
               bool* converged, int kt_spmv);
+  template <typename T>
+  void launch_spmv(const StencilView<T>& A, const T* z, const T* pold, T* pnew, T* q, const KrylovCtl* ctl,
+                   hipEvent_t ev0, hipEvent_t ev1);
   template <typename T>
   int run_bicgstab(const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed,
                    bool x0_zero, bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl, bool* rejected,
@@ -122,10 +132,11 @@ class Krylov {
   bool three_d_;
   hipStream_t s_;
   Profiler* prof_;
-  int kc_s_ = 0, kc_f_ = 0;
-  dim3 grid_s_, grid_f_;
-  unsigned int nb_s_ = 0, nb_f_ = 0, part_cap_ = 0;
-  bool tuned_ = false;
+  SpmvShape shape_[2];
+  int kc_f_ = 1;
+  dim3 grid_f_;
+  unsigned int nb_f_ = 0, part_cap_ = 0;
+  bool tuned_[2] = {false, false};
   long long n_ = 0;  // padded length
   double *r_ = nullptr, *z_ = nullptr, *p0_ = nullptr, *p1_ = nullptr, *q_ = nullptr, *xw_ = nullptr;
   float *b32_ = nullptr, *r32_ = nullptr, *z32_ = nullptr, *p032_ = nullptr, *p132_ = nullptr,

@@ -1,6 +1,9 @@
-// HIP-event kernel timing on the context's stream (enabled on request; off by default).
+// Kernel timing on the context's stream (enabled on request; off by default). The events are
+// passed to hipExtLaunchKernelGGL as its start/stop events, so they carry the kernel's own start
+// and end times rather than the times at which markers around the launch were processed.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -9,7 +12,7 @@
 
 namespace frehg {
 
-constexpr int kKernelTypes = 5;  // FREHG_KT_COUNT
+constexpr int kKernelTypes = 8;  // FREHG_KT_COUNT
 
 class Profiler {
  public:
@@ -20,19 +23,21 @@ class Profiler {
     for (auto& e : pool_) (void)hipEventDestroy(e);
   }
 
-  // Bracket one launch of kernel type `kt` on `stream`.
-  void begin(int kt) {
-    if (!enabled) return;
+  // Time one launch, or a run of consecutive launches, of kernel type `kt`: pass start(kt) as the
+  // start event of the first launch and stop() as the stop event of the last one (both are null,
+  // i.e. an ordinary launch, while profiling is off). Call start before stop.
+  hipEvent_t start(int kt) {
+    if (!enabled) return nullptr;
     pending_kt_ = kt;
     pending_begin_ = take();
-    FREHG_HIP(hipEventRecord(pending_begin_, stream));
+    return pending_begin_;
   }
-  void end() {
-    if (!enabled || pending_kt_ < 0) return;
+  hipEvent_t stop() {
+    if (!enabled || pending_kt_ < 0) return nullptr;
     hipEvent_t e = take();
-    FREHG_HIP(hipEventRecord(e, stream));
     recs_.push_back({pending_kt_, pending_begin_, e});
     pending_kt_ = -1;
+    return e;
   }
   // Drop the `n` most recent records (launches that returned immediately because the solve had
   // already converged), so averages cover only launches that did work.
@@ -91,8 +96,8 @@ class Profiler {
   std::vector<Rec> recs_;
   int pending_kt_ = -1;
   hipEvent_t pending_begin_ = nullptr;
-  double total_ms_[kKernelTypes] = {0, 0, 0, 0, 0};
-  long long launches_[kKernelTypes] = {0, 0, 0, 0, 0};
+  double total_ms_[kKernelTypes] = {};
+  long long launches_[kKernelTypes] = {};
 };
 
 }  // namespace frehg

@@ -41,10 +41,11 @@ struct Cell {
 };
 
 __device__ inline Cell cell_of(const Geo& g) {
+  const TileId b = xcd_tile();
   Cell t;
-  t.i = blockIdx.x * 64 + (threadIdx.x & 63);
-  t.j = blockIdx.y * 4 + (threadIdx.x >> 6);
-  t.k = blockIdx.z;
+  t.i = b.x * 64 + (threadIdx.x & 63);
+  t.j = b.y * 4 + (threadIdx.x >> 6);
+  t.k = b.z;
   t.valid = t.i < g.nx && t.j < g.ny;
   t.c = t.valid ? g.idx(t.i, t.j, t.k) : 0;
   return t;
@@ -647,13 +648,15 @@ void ReState::step(frehg_re_stats* st) {
   // no kernel reads or writes, hold the same values in both buffers).
   std::swap(h_, hn_);
   std::swap(wc_, wcn_);
-  prof_->begin(3);
-  hipLaunchKernelGGL(k_re_pre, grid, blk, 0, s_, P, hn_, kr_, ch_);
-  FREHG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_re_assemble, grid, blk, 0, s_, P, kr_, ch_, wcn_, hn_, D_, ax_, ay_, az_, rhs_,
-                     mixed ? D32_ : nullptr, ax32_, ay32_, az32_);
-  FREHG_LAUNCH_CHECK();
-  prof_->end();
+  {
+    hipEvent_t ev0 = prof_->start(FREHG_KT_GW_ASSEMBLE);
+    hipEvent_t ev1 = prof_->stop();
+    hipExtLaunchKernelGGL(k_re_pre, grid, blk, 0, s_, ev0, nullptr, 0, P, hn_, kr_, ch_);
+    FREHG_LAUNCH_CHECK();
+    hipExtLaunchKernelGGL(k_re_assemble, grid, blk, 0, s_, nullptr, ev1, 0, P, kr_, ch_, wcn_, hn_, D_, ax_, ay_,
+                          az_, rhs_, mixed ? D32_ : nullptr, ax32_, ay32_, az32_);
+    FREHG_LAUNCH_CHECK();
+  }
   // Initial guess in h (the solve's x): zero (PETSc default), h^n, or h^n + MLP increment.
   bool x0_zero = true, guard = false;
   if (p_.initial_guess == FREHG_GUESS_PREVIOUS) {
@@ -679,15 +682,18 @@ void ReState::step(frehg_re_stats* st) {
   local.guess_fallback = sr.guess_rejected ? 1 : 0;
   local.picard_iterations = 1;
   if (mlp_ && mlp_->logging()) mlp_->log_pair(*this, hn_, h_);
-  prof_->begin(4);
-  hipLaunchKernelGGL(k_re_post, grid, blk, 0, s_, P, h_, kr_, ch_, wch_);
-  FREHG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_re_flux, grid, blk, 0, s_, P, h_, kr_, wcn_, Kx_, Ky_, Kz_, qx_, qy_, qz_);
-  FREHG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_re_correct, grid, blk, 0, s_, P, h_, hn_, wcn_, qx_, qy_, qz_, wch_, wc_, hwc_, sat0_,
-                     cand_);
-  FREHG_LAUNCH_CHECK();
-  prof_->end();
+  {
+    hipEvent_t ev0 = prof_->start(FREHG_KT_GW_CORRECT);
+    hipEvent_t ev1 = prof_->stop();
+    hipExtLaunchKernelGGL(k_re_post, grid, blk, 0, s_, ev0, nullptr, 0, P, h_, kr_, ch_, wch_);
+    FREHG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_re_flux, grid, blk, 0, s_, P, h_, kr_, wcn_, Kx_, Ky_, Kz_, qx_, qy_, qz_);
+    FREHG_LAUNCH_CHECK();
+    hipExtLaunchKernelGGL(k_re_correct, grid, blk, 0, s_, nullptr, ev1, 0, P, (const double*)h_, (const double*)hn_,
+                          (const double*)wcn_, (const double*)qx_, (const double*)qy_, (const double*)qz_,
+                          (const double*)wch_, wc_, hwc_, sat0_, cand_);
+    FREHG_LAUNCH_CHECK();
+  }
   // Reallocation fixed point: iterate from sat0 until the saturation state stops changing.
   FREHG_HIP(hipMemcpyAsync(satA_, sat0_, (size_t)n_, hipMemcpyDeviceToDevice, s_));
   FREHG_HIP(hipMemcpyAsync(satB_, sat0_, (size_t)n_, hipMemcpyDeviceToDevice, s_));

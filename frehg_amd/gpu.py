@@ -70,6 +70,7 @@ def load():
         "frehg_profile_enable": (C.c_int, [ctx, C.c_int]),
         "frehg_profile_read": (C.c_int, [ctx, C.c_int, dp, P(C.c_int64)]),
         "frehg_profile_reset": (C.c_int, [ctx]),
+        "frehg_spmv_shape": (C.c_int, [ctx, C.c_int, P(C.c_int), P(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -240,6 +241,11 @@ class Context:
 
     def profile_reset(self):
         self.check(self.lib.frehg_profile_reset(self.h))
+
+    def spmv_shape(self, precision):
+        ty, kc = C.c_int(), C.c_int()
+        self.check(self.lib.frehg_spmv_shape(self.h, precision, C.byref(ty), C.byref(kc)))
+        return ty.value, kc.value
 
 
 def declared_symbols(header=HEADER):

@@ -200,17 +200,26 @@ int frehg_ls_solve(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, const doub
 /* ---- measurement --------------------------------------------------------------------------- */
 /* Kernel-duration accounting with HIP events on the context's stream. kernel ids below. */
 enum {
-  FREHG_KT_GW_SPMV = 0,   /* FP64 PCG SpMV+dot (7-point) */
-  FREHG_KT_GW_SPMV32 = 1, /* FP32 PCG SpMV+dot (7-point, mixed) */
-  FREHG_KT_GW_UPDATE = 2, /* PCG vector update + reductions */
-  FREHG_KT_GW_ASSEMBLE = 3,
-  FREHG_KT_GW_CORRECT = 4,
-  FREHG_KT_COUNT = 5
+  FREHG_KT_GW_SPMV = 0,     /* groundwater FP64 Krylov SpMV+dot launches (7-point) */
+  FREHG_KT_GW_SPMV32 = 1,   /* groundwater FP32 inner Krylov SpMV+dot launches (mixed) */
+  FREHG_KT_GW_UPDATE = 2,   /* groundwater Krylov vector update + reductions (FP64 and FP32) */
+  FREHG_KT_GW_ASSEMBLE = 3, /* K, C at h^n + 7-point assembly (one span per step) */
+  FREHG_KT_GW_CORRECT = 4,  /* ghosts, K/C at h^{n+1}, fluxes, corrector (one span per step) */
+  FREHG_KT_SW_SPMV = 5,     /* surface Krylov SpMV+dot launches (5-point) */
+  FREHG_KT_SW_UPDATE = 6,   /* surface Krylov vector update + reductions */
+  FREHG_KT_GW_REFINE = 7,   /* groundwater FP64 residual sweep of iterative refinement */
+  FREHG_KT_COUNT = 8
 };
 int frehg_profile_enable(frehg_ctx_t ctx, int enable);
 /* Total milliseconds and launch count recorded for kernel `kt` since the last reset. */
 int frehg_profile_read(frehg_ctx_t ctx, int kt, double* total_ms, int64_t* launches);
 int frehg_profile_reset(frehg_ctx_t ctx);
+/* Groundwater CG SpMV launch shape in use for `precision` (FREHG_PREC_FP64: the FP64 solve,
+ * FREHG_PREC_MIXED: the FP32 inner solve): tile height (0 = one column per thread, else 64 x
+ * tile_y LDS tiles) and layers per block. Set by the autotuner at the first solve, or pinned by the
+ * environment variable FREHG_SPMV_SHAPE="tile_y:kchunk" read at frehg_re_init. No counterpart in
+ * the reference (PETSc's MatMult has no launch shape). */
+int frehg_spmv_shape(frehg_ctx_t ctx, int precision, int* tile_y, int* kchunk);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,61 @@
+"""Every SpMV launch shape the autotuner can pick (column-per-thread, or 64 x {4, 8, 16} LDS
+tiles, with several z-chunks) gives the same Krylov results, in FP64 and in FP32-in-FP64
+refinement, on a grid whose sizes are not multiples of any tile (partial tiles in x, y and z).
+The per-cell SpMV values are identical for all shapes; the dot product p.q is summed over the
+shape's blocks, so the iterates may differ in the last bits."""
+import os
+
+import numpy as np
+import pytest
+
+from frehg_amd import abi, gpu
+from gpu_common import re_pair, say
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = ["0:4", "0:1", "4:1", "4:7", "8:3", "16:2", "16:9"]
+
+
+def run(shape, precision):
+    os.environ["FREHG_SPMV_SHAPE"] = shape
+    try:
+        nx, ny, nz = 70, 33, 9
+        rng = np.random.default_rng(21)
+        soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5,
+                               Ss=1e-5)
+        p = abi.re_params(soil=soil, solver=abi.solver_config(1e-12, precision=precision, inner_rtol=1e-5,
+                                                                max_refine=10),
+                          bc_type=(0, 0, 0, 0, 3, 1), dx=1.0, dy=1.0, dz=0.05, dt=5.0, adaptive_dt=0,
+                          use_full3d=1, htop=-0.1)
+        ks = np.exp(rng.normal(np.log(1e-5), 1.0, nx * ny * nz))
+        h0 = rng.uniform(-3.0, -0.5, nx * ny * nz)
+        _, ctx = re_pair(nx, ny, nz, p, soil_map=([soil], None, True, ks), h0=h0)
+        iters = 0
+        for _ in range(4):
+            st = ctx.re_step()
+            assert st.solve.converged
+            iters += st.solve.iterations
+        return ctx.re_get("h"), iters
+    finally:
+        del os.environ["FREHG_SPMV_SHAPE"]
+
+
+@pytest.mark.parametrize("precision", [abi.PREC_FP64, abi.PREC_MIXED])
+def test_spmv_shapes_agree(precision):
+    ref, it0 = run(SHAPES[0], precision)
+    for shape in SHAPES[1:]:
+        say(f"SpMV shape {shape} precision {precision}")
+        h, it = run(shape, precision)
+        assert it == it0, (shape, it, it0)
+        assert np.abs(h - ref).max() <= 1e-11, f"shape {shape}: max diff {np.abs(h - ref).max():.3e}"
+
+
+def test_bad_shape_rejected():
+    os.environ["FREHG_SPMV_SHAPE"] = "5:2"
+    try:
+        ctx = gpu.Context(0)
+        with pytest.raises(gpu.FrehgError) as e:
+            ctx.re_init(8, 8, 4, abi.re_params())
+        assert e.value.code == abi.ERR_INVALID_ARG
+    finally:
+        del os.environ["FREHG_SPMV_SHAPE"]
