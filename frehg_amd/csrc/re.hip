@@ -491,7 +491,8 @@ void ReState::set_params(const frehg_re_params& p) {
   if (p.bc_type[4] < 0 || p.bc_type[4] > 3) throw ArgError("frehg_re: bottom bc must be 0, 1, 2 or 3");
   if (p.bc_type[5] < 0 || p.bc_type[5] > 2) throw ArgError("frehg_re: top bc must be 0, 1 or 2");
   if (!(p.dx > 0 && p.dy > 0 && p.dz > 0 && p.dt > 0)) throw ArgError("frehg_re: dx, dy, dz, dt must be > 0");
-  if (p.picard_max < 1) throw ArgError("frehg_re: picard_max must be >= 1");
+  if (p.picard_max != 1)
+    throw ArgError("frehg_re: picard_max must be 1 (the reference's predictor-corrector step; no Picard loop)");
   (void)krylov_config(p.solver, "frehg_re");
   const bool mixed = p.solver.precision == FREHG_PREC_MIXED;
   if (mixed && !D32_) {
