@@ -34,6 +34,9 @@ KT_GW_SPMV32 = 1
 KT_GW_UPDATE = 2
 KT_GW_ASSEMBLE = 3
 KT_GW_CORRECT = 4
+MLP_INPUTS = 11
+MLP_PARAMS = 4993
+MLP_NORM = 24
 KT_SW_SPMV = 5
 KT_SW_UPDATE = 6
 KT_GW_REFINE = 7

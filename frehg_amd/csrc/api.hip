@@ -9,7 +9,7 @@
 
 #include "../../include/frehg_gpu.h"
 #include "coupling.hpp"
-#include "mlp_impl.hpp"
+#include "mlp.hpp"
 #include "re.hpp"
 #include "swe.hpp"
 
@@ -21,7 +21,7 @@ struct frehg_ctx_s {
   std::unique_ptr<frehg::ReState> re;
   std::unique_ptr<frehg::SweState> swe;
   std::unique_ptr<frehg::Coupler> cpl;
-  std::unique_ptr<frehg::MlpGuess> mlp;
+  std::unique_ptr<frehg::Mlp> mlp;
   double t = 0.0, gw_time = 0.0;
 };
 
@@ -58,6 +58,24 @@ frehg::ReState& need_re(frehg_ctx_t c) {
 frehg::SweState& need_swe(frehg_ctx_t c) {
   if (!c->swe) throw std::runtime_error("surface solver not initialised (call frehg_swe_init)");
   return *c->swe;
+}
+
+// Hand the learned-guess model and the surface depth (when the surface grid is the subsurface's
+// top) to the Richards solver.
+void bind_mlp(frehg_ctx_t c) {
+  if (!c->re) return;
+  const double* depth = nullptr;
+  if (c->swe && c->swe->geo().nx == c->re->geo().nx && c->swe->geo().ny == c->re->geo().ny)
+    depth = c->swe->d("dept");
+  c->re->set_mlp(c->mlp.get(), depth);
+}
+
+frehg::Mlp& need_mlp(frehg_ctx_t c) {
+  if (!c->mlp) {
+    c->mlp.reset(new frehg::Mlp(c->stream));
+    bind_mlp(c);
+  }
+  return *c->mlp;
 }
 frehg::Coupler& need_cpl(frehg_ctx_t c) {
   if (!c->cpl) c->cpl.reset(new frehg::Coupler(c->stream));
@@ -114,7 +132,7 @@ int frehg_re_init(frehg_ctx_t ctx, int nx, int ny, int nz, const frehg_re_params
     if (!p) throw frehg::ArgError("frehg_re_init: null params");
     ctx->re.reset();
     ctx->re.reset(new frehg::ReState(nx, ny, nz, *p, ctx->stream, &ctx->prof));
-    if (ctx->mlp) ctx->re->set_mlp(ctx->mlp.get());
+    bind_mlp(ctx);
     return FREHG_OK;
   });
 }
@@ -189,7 +207,9 @@ int frehg_swe_init(frehg_ctx_t ctx, int nx, int ny, double dx, double dy, const 
   return guard(ctx, [&] {
     if (!p) throw frehg::ArgError("frehg_swe_init: null params");
     ctx->swe.reset();
+    bind_mlp(ctx);
     ctx->swe.reset(new frehg::SweState(nx, ny, dx, dy, *p, ctx->stream, &ctx->prof));
+    bind_mlp(ctx);
     return FREHG_OK;
   });
 }
@@ -470,6 +490,66 @@ int frehg_spmv_shape(frehg_ctx_t ctx, int precision, int* tile_y, int* kchunk) {
     const frehg::Krylov::SpmvShape& sh = re.krylov().spmv_shape(precision == FREHG_PREC_MIXED ? 1 : 0);
     if (tile_y) *tile_y = sh.ty;
     if (kchunk) *kchunk = sh.kc;
+    return FREHG_OK;
+  });
+}
+
+int frehg_mlp_init(frehg_ctx_t ctx, uint64_t seed) {
+  return guard(ctx, [&] {
+    need_mlp(ctx).init(seed);
+    return FREHG_OK;
+  });
+}
+
+int frehg_mlp_log(frehg_ctx_t ctx, int stride, int64_t capacity) {
+  return guard(ctx, [&] {
+    need_mlp(ctx).set_logging(stride, capacity);
+    return FREHG_OK;
+  });
+}
+
+int frehg_mlp_logged(frehg_ctx_t ctx, int64_t* n) {
+  return guard(ctx, [&] {
+    if (!n) throw frehg::ArgError("frehg_mlp_logged: null output");
+    *n = need_mlp(ctx).logged();
+    return FREHG_OK;
+  });
+}
+
+int frehg_mlp_train(frehg_ctx_t ctx, int steps, int batch, double lr, uint64_t seed, double* loss_first,
+                    double* loss_last) {
+  return guard(ctx, [&] {
+    const double l = need_mlp(ctx).train(steps, batch, lr, seed, loss_first);
+    if (loss_last) *loss_last = l;
+    return FREHG_OK;
+  });
+}
+
+int frehg_mlp_get_params(frehg_ctx_t ctx, float* params, float* norm) {
+  return guard(ctx, [&] {
+    need_mlp(ctx).get_params(params, norm);
+    return FREHG_OK;
+  });
+}
+
+int frehg_mlp_set_params(frehg_ctx_t ctx, const float* params, const float* norm) {
+  return guard(ctx, [&] {
+    need_mlp(ctx).set_params(params, norm);
+    return FREHG_OK;
+  });
+}
+
+int frehg_mlp_loss_grad(frehg_ctx_t ctx, const float* x, const float* y, int n, float* grad, double* loss) {
+  return guard(ctx, [&] {
+    const double l = need_mlp(ctx).loss_grad(x, y, n, grad);
+    if (loss) *loss = l;
+    return FREHG_OK;
+  });
+}
+
+int frehg_mlp_predict(frehg_ctx_t ctx, const float* x, int n, float* out) {
+  return guard(ctx, [&] {
+    need_mlp(ctx).predict_batch(x, n, out);
     return FREHG_OK;
   });
 }

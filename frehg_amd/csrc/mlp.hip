@@ -1,0 +1,641 @@
+// Learned initial guess: feature extraction, pair logging, BF16 MFMA forward / backward, Adam.
+//
+// Every dense layer runs on v_mfma_f32_16x16x32_bf16 (D[16x16] += A[16x32] B[32x16], FP32
+// accumulate). Lane l of a wave supplies A[l%16][8(l/16) .. 8(l/16)+7] and
+// B[8(l/16) .. 8(l/16)+7][l%16], and receives D[4(l/16)+r][l%16], r = 0..3. Operands are staged
+// in LDS as FP32 and rounded to BF16 (round to nearest even) when a lane packs them, so a
+// transposed operand is just a different LDS stride.
+//
+// A block of 256 threads (4 waves) handles 64 rows at a time; wave w owns rows 16w..16w+15 in
+// the forward pass and the 16-row block w of the weight gradients.
+//   forward:  Z1 = X W1^T + b1, A1 = tanh Z1;  Z2 = A1 W2^T + b2, A2 = tanh Z2;  y = A2 w3 + b3
+//   backward: dy = 2 (y - t) / B;  dZ2 = dy w3 (1 - A2^2);  dZ1 = (dZ2 W2) (1 - A1^2)
+//             dW2 = dZ2^T A1, dW1 = dZ1^T X, dw3 = dy^T A2, db = column sums
+// Gradients are written as per-block partials and summed in a fixed order by one kernel, which
+// also applies the Adam update, so training is deterministic for a given seed.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mlp.hpp"
+#include "re.hpp"
+
+namespace frehg {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int MT = 256;   // threads per block
+constexpr int RB = 64;    // rows per block iteration
+constexpr int XP = 33;    // LDS row pitch of X (32 padded inputs + 1)
+constexpr int HP = 65;    // LDS row pitch of 64-wide activations
+constexpr int TRAIN_BLOCKS = 256;
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// 8 consecutive FP32 values (stride `st` floats) packed as a BF16 MFMA operand.
+__device__ __forceinline__ bf16x8 pack8(const float* p, int st) {
+  bf16x8 v;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) v[t] = (__bf16)p[t * st];
+  return v;
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Inputs of the feature function: the Richards state before the solve.
+struct FeatSrc {
+  Geo g;
+  const double* hn;
+  const double* kr;
+  const double* ch;
+  const double* wcn;
+  SoilView soil;
+  const double* depth;  // surface padded, or null
+};
+
+__device__ inline void cell_features(const FeatSrc& F, int i, int j, int k, float* f) {
+  const Geo& g = F.g;
+  const long long c = g.idx(i, j, k);
+  const double hc = F.hn[c];
+  f[0] = (float)hc;
+  f[1] = (float)(F.hn[c - 1] - hc);
+  f[2] = (float)(F.hn[c + 1] - hc);
+  f[3] = (float)(F.hn[c - g.sy] - hc);
+  f[4] = (float)(F.hn[c + g.sy] - hc);
+  f[5] = (float)(F.hn[c - g.sz] - hc);
+  f[6] = (float)(F.hn[c + g.sz] - hc);
+  f[7] = (float)log(fmax(F.soil.Ksz(c) * F.kr[c], 1e-30));
+  f[8] = (float)F.ch[c];
+  f[9] = (float)F.wcn[c];
+  f[10] = F.depth ? (float)F.depth[g.sidx(i, j)] : 0.0f;
+}
+
+// Shared-memory image of the network and one block of 64 rows.
+struct MlpLds {
+  float w1[kMlpH][XP];   // W1 padded to 32 inputs
+  float w2[kMlpH][HP];
+  float w3[kMlpH];
+  float b1[kMlpH], b2[kMlpH];
+  float x[RB][XP];       // standardised inputs (zero beyond 11)
+  float a1[RB][HP];
+  float a2[RB][HP];
+  float t[RB];           // standardised targets
+  float dy[RB];
+};
+
+struct MlpBwdLds {
+  float dz1[RB][HP];
+  float dz2[RB][HP];
+};
+
+__device__ void load_weights(MlpLds& L, const float* P) {
+  for (int q = threadIdx.x; q < kMlpH * 32; q += MT) {
+    const int n = q / 32, f = q % 32;
+    L.w1[n][f] = f < kMlpIn ? P[kOffW1 + n * kMlpIn + f] : 0.0f;
+  }
+  for (int q = threadIdx.x; q < kMlpH * kMlpH; q += MT) L.w2[q / kMlpH][q % kMlpH] = P[kOffW2 + q];
+  if (threadIdx.x < kMlpH) {
+    L.w3[threadIdx.x] = P[kOffW3 + threadIdx.x];
+    L.b1[threadIdx.x] = P[kOffB1 + threadIdx.x];
+    L.b2[threadIdx.x] = P[kOffB2 + threadIdx.x];
+  }
+}
+
+// Forward pass of wave w's 16 rows; returns y for rows 16w + 4(l/16) + r in lanes l%16 == 0.
+// Called by every thread of the block (it contains block barriers).
+__device__ f32x4 forward16(MlpLds& L, float b3) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row0 = 16 * w, ra = lane & 15, kg = 8 * (lane >> 4), rd = 4 * (lane >> 4);
+  // layer 1: K = 32 (11 inputs, zero padded)
+  const bf16x8 xa = pack8(&L.x[row0 + ra][kg], 1);
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma(xa, pack8(&L.w1[16 * nb + ra][kg], 1), acc);
+    const int col = 16 * nb + ra;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) L.a1[row0 + rd + r][col] = tanhf(acc[r] + L.b1[col]);
+  }
+  __syncthreads();
+  // layer 2: K = 64
+  bf16x8 aa[2];
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) aa[ks] = pack8(&L.a1[row0 + ra][32 * ks + kg], 1);
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) acc = mfma(aa[ks], pack8(&L.w2[16 * nb + ra][32 * ks + kg], 1), acc);
+    const int col = 16 * nb + ra;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) L.a2[row0 + rd + r][col] = tanhf(acc[r] + L.b2[col]);
+  }
+  __syncthreads();
+  // layer 3: B = w3 in column 0, zero elsewhere
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    bf16x8 b;
+    if (ra == 0) {
+      b = pack8(&L.w3[32 * ks + kg], 1);
+    } else {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) b[t] = (__bf16)0.0f;
+    }
+    acc = mfma(pack8(&L.a2[row0 + ra][32 * ks + kg], 1), b, acc);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc[r] += b3;
+  return acc;
+}
+
+// Stage 64 rows (11 raw features + raw target each) into LDS, standardised. Rows past `valid`
+// are zero and carry no weight.
+__device__ void stage_rows(MlpLds& L, const float* rows, const long long* idx, int valid, const float* norm) {
+  for (int q = threadIdx.x; q < RB * 32; q += MT) {
+    const int s = q / 32, f = q % 32;
+    float v = 0.0f;
+    if (s < valid && f < kMlpIn) v = (rows[idx[s] * kMlpRow + f] - norm[f]) / norm[kMlpRow + f];
+    L.x[s][f] = v;
+  }
+  if (threadIdx.x < RB) {
+    const int s = threadIdx.x;
+    L.t[s] = s < valid ? (rows[idx[s] * kMlpRow + kMlpIn] - norm[kMlpIn]) / norm[kMlpRow + kMlpIn] : 0.0f;
+  }
+}
+
+// Loss and gradient partials over `nrows` rows; row q of the batch is log row
+// (sequential ? q : hash(seed, q) mod nlog). part[block][kMlpParams + 1] (last = sum of squares).
+__global__ __launch_bounds__(MT) void k_mlp_grad(const float* P, const float* norm, const float* rows,
+                                                  long long nlog, long long nrows, int sequential, uint64_t seed,
+                                                  float inv_batch, float* part) {
+  __shared__ MlpLds L;
+  __shared__ MlpBwdLds B;
+  __shared__ long long idx[RB];
+  __shared__ float red[MT / 64];
+  load_weights(L, P);
+  const float b3 = P[kOffB3];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ra = lane & 15, kg = 8 * (lane >> 4), rd = 4 * (lane >> 4);
+  f32x4 gW2[4], gW1[2], gW3;
+  for (int q = 0; q < 4; ++q) gW2[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+  gW1[0] = gW1[1] = gW3 = f32x4{0.f, 0.f, 0.f, 0.f};
+  float gb = 0.0f;  // thread n < 64: db1[n]; 64 <= thread < 128: db2[n-64]; thread 128: db3
+  float sq = 0.0f;
+  const long long nchunks = (nrows + RB - 1) / RB;
+  for (long long ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const long long q0 = ch * RB;
+    const int valid = (int)(nrows - q0 < RB ? nrows - q0 : RB);
+    if (threadIdx.x < RB) {
+      const long long q = q0 + threadIdx.x;
+      idx[threadIdx.x] = threadIdx.x < valid ? (sequential ? q : (long long)(splitmix64(seed ^ (uint64_t)q) % (uint64_t)nlog)) : 0;
+    }
+    __syncthreads();
+    stage_rows(L, rows, idx, valid, norm);
+    __syncthreads();
+    const f32x4 y = forward16(L, b3);
+    if (ra == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int s = 16 * w + rd + r;
+        const float e = s < valid ? y[r] - L.t[s] : 0.0f;
+        sq += e * e;
+        L.dy[s] = 2.0f * e * inv_batch;
+      }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < RB * kMlpH; q += MT) {
+      const int s = q / kMlpH, n = q % kMlpH;
+      const float a = L.a2[s][n];
+      B.dz2[s][n] = L.dy[s] * L.w3[n] * (1.0f - a * a);
+    }
+    __syncthreads();
+    // dA1 = dZ2 W2 for wave w's rows, then dZ1 = dA1 (1 - A1^2)
+    {
+      const int row0 = 16 * w;
+      bf16x8 za[2];
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) za[ks] = pack8(&B.dz2[row0 + ra][32 * ks + kg], 1);
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) acc = mfma(za[ks], pack8(&L.w2[32 * ks + kg][16 * nb + ra], HP), acc);
+        const int col = 16 * nb + ra;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float a = L.a1[row0 + rd + r][col];
+          B.dz1[row0 + rd + r][col] = acc[r] * (1.0f - a * a);
+        }
+      }
+    }
+    __syncthreads();
+    // weight gradients: K = the block's 64 rows
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int s0 = 32 * ks + kg;
+      const bf16x8 dz2a = pack8(&B.dz2[s0][16 * w + ra], HP);  // A[i][s] = dZ2[s][i]
+      const bf16x8 dz1a = pack8(&B.dz1[s0][16 * w + ra], HP);
+#pragma unroll
+      for (int nj = 0; nj < 4; ++nj) gW2[nj] = mfma(dz2a, pack8(&L.a1[s0][16 * nj + ra], HP), gW2[nj]);
+#pragma unroll
+      for (int nj = 0; nj < 2; ++nj) gW1[nj] = mfma(dz1a, pack8(&L.x[s0][16 * nj + ra], XP), gW1[nj]);
+      bf16x8 dya;
+      if (ra == 0) {
+        dya = pack8(&L.dy[s0], 1);
+      } else {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) dya[t] = (__bf16)0.0f;
+      }
+      gW3 = mfma(dya, pack8(&L.a2[s0][16 * w + ra], HP), gW3);
+    }
+    if (threadIdx.x < kMlpH) {
+      for (int s = 0; s < RB; ++s) gb += B.dz1[s][threadIdx.x];
+    } else if (threadIdx.x < 2 * kMlpH) {
+      for (int s = 0; s < RB; ++s) gb += B.dz2[s][threadIdx.x - kMlpH];
+    } else if (threadIdx.x == 2 * kMlpH) {
+      for (int s = 0; s < RB; ++s) gb += L.dy[s];
+    }
+    __syncthreads();
+  }
+  // partials of this block
+  float* out = part + (size_t)blockIdx.x * (kMlpParams + 1);
+#pragma unroll
+  for (int nj = 0; nj < 4; ++nj)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[kOffW2 + (16 * w + rd + r) * kMlpH + 16 * nj + ra] = gW2[nj][r];
+#pragma unroll
+  for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int f = 16 * nj + ra;
+      if (f < kMlpIn) out[kOffW1 + (16 * w + rd + r) * kMlpIn + f] = gW1[nj][r];
+    }
+  if (lane < 16) out[kOffW3 + 16 * w + ra] = gW3[0];  // row 0 of the tile: lanes 0..15, r = 0
+  if (threadIdx.x < kMlpH) out[kOffB1 + threadIdx.x] = gb;
+  else if (threadIdx.x < 2 * kMlpH) out[kOffB2 + threadIdx.x - kMlpH] = gb;
+  else if (threadIdx.x == 2 * kMlpH) out[kOffB3] = gb;
+  // sum of squared errors
+  float v = sq;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[kMlpParams] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Sum the partials in block order; optionally apply one Adam step.
+__global__ __launch_bounds__(MT) void k_mlp_reduce_adam(const float* part, int nblocks, float* grad, float* P,
+                                                         float* m, float* v, int adam, float lr, float b1t,
+                                                         float b2t) {
+  const int p = blockIdx.x * MT + threadIdx.x;
+  if (p > kMlpParams) return;
+  float g = 0.0f;
+  for (int b = 0; b < nblocks; ++b) g += part[(size_t)b * (kMlpParams + 1) + p];
+  grad[p] = g;
+  if (!adam || p == kMlpParams) return;
+  const float mm = 0.9f * m[p] + 0.1f * g;
+  const float vv = 0.999f * v[p] + 0.001f * g * g;
+  m[p] = mm;
+  v[p] = vv;
+  const float mh = mm / (1.0f - b1t), vh = vv / (1.0f - b2t);
+  P[p] -= lr * mh / (sqrtf(vh) + 1e-8f);
+}
+
+// Forward predictions (raw increments) for rows of a host batch.
+__global__ __launch_bounds__(MT) void k_mlp_predict_rows(const float* P, const float* norm, const float* rows,
+                                                          long long nrows, float* out) {
+  __shared__ MlpLds L;
+  __shared__ long long idx[RB];
+  load_weights(L, P);
+  const float b3 = P[kOffB3];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, ra = lane & 15, rd = 4 * (lane >> 4);
+  const long long nchunks = (nrows + RB - 1) / RB;
+  for (long long ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const long long q0 = ch * RB;
+    const int valid = (int)(nrows - q0 < RB ? nrows - q0 : RB);
+    if (threadIdx.x < RB) idx[threadIdx.x] = threadIdx.x < valid ? q0 + threadIdx.x : 0;
+    __syncthreads();
+    stage_rows(L, rows, idx, valid, norm);
+    __syncthreads();
+    const f32x4 y = forward16(L, b3);
+    if (ra == 0)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int s = 16 * w + rd + r;
+        if (s < valid) out[q0 + s] = y[r] * norm[kMlpRow + kMlpIn] + norm[kMlpIn];
+      }
+    __syncthreads();
+  }
+}
+
+// Learned guess on every interior cell: h_out = h^n + increment.
+__global__ __launch_bounds__(MT) void k_mlp_guess(const float* P, const float* norm, FeatSrc F, double* h_out) {
+  __shared__ MlpLds L;
+  load_weights(L, P);
+  const float b3 = P[kOffB3];
+  const Geo& g = F.g;
+  const long long N = (long long)g.nx * g.ny * g.nz;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, ra = lane & 15, rd = 4 * (lane >> 4);
+  const long long nchunks = (N + RB - 1) / RB;
+  for (long long ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const long long q0 = ch * RB;
+    __syncthreads();
+    if (threadIdx.x < RB) {
+      const long long q = q0 + threadIdx.x;
+      float f[kMlpIn];
+      if (q < N) {
+        const int i = (int)(q % g.nx), j = (int)((q / g.nx) % g.ny), k = (int)(q / ((long long)g.nx * g.ny));
+        cell_features(F, i, j, k, f);
+      }
+#pragma unroll
+      for (int t = 0; t < kMlpIn; ++t) L.x[threadIdx.x][t] = q < N ? (f[t] - norm[t]) / norm[kMlpRow + t] : 0.0f;
+      for (int t = kMlpIn; t < 32; ++t) L.x[threadIdx.x][t] = 0.0f;
+    }
+    __syncthreads();
+    const f32x4 y = forward16(L, b3);
+    if (ra == 0)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long q = q0 + 16 * w + rd + r;
+        if (q < N) {
+          const int i = (int)(q % g.nx), j = (int)((q / g.nx) % g.ny), k = (int)(q / ((long long)g.nx * g.ny));
+          const long long c = g.idx(i, j, k);
+          h_out[c] = F.hn[c] + (double)(y[r] * norm[kMlpRow + kMlpIn] + norm[kMlpIn]);
+        }
+      }
+  }
+}
+
+// Log (features, h - h^n) of interior cells q with q % stride == phase into the ring buffer.
+__global__ __launch_bounds__(MT) void k_mlp_log(FeatSrc F, const double* h, int stride, int phase, long long base,
+                                                 long long cap, float* log) {
+  const Geo& g = F.g;
+  const long long N = (long long)g.nx * g.ny * g.nz;
+  const long long qq = (long long)blockIdx.x * MT + threadIdx.x;  // index among sampled cells
+  const long long q = qq * stride + phase;
+  if (q >= N) return;
+  const int i = (int)(q % g.nx), j = (int)((q / g.nx) % g.ny), k = (int)(q / ((long long)g.nx * g.ny));
+  float f[kMlpIn];
+  cell_features(F, i, j, k, f);
+  float* row = log + ((base + qq) % cap) * kMlpRow;
+#pragma unroll
+  for (int t = 0; t < kMlpIn; ++t) row[t] = f[t];
+  const long long c = g.idx(i, j, k);
+  row[kMlpIn] = (float)(h[c] - F.hn[c]);
+}
+
+// Per-block sums and sums of squares of the 12 logged columns (FP64).
+__global__ __launch_bounds__(MT) void k_mlp_stats(const float* log, long long n, double* part) {
+  double s[kMlpRow], s2[kMlpRow];
+  for (int t = 0; t < kMlpRow; ++t) s[t] = s2[t] = 0.0;
+  for (long long r = (long long)blockIdx.x * MT + threadIdx.x; r < n; r += (long long)gridDim.x * MT)
+    for (int t = 0; t < kMlpRow; ++t) {
+      const double v = log[r * kMlpRow + t];
+      s[t] += v;
+      s2[t] += v * v;
+    }
+  __shared__ double lds[MT / 64];
+  for (int t = 0; t < kMlpRow; ++t) {
+    const double a = block_sum<MT>(s[t], lds);
+    const double b = block_sum<MT>(s2[t], lds);
+    if (threadIdx.x == 0) {
+      part[(size_t)blockIdx.x * 2 * kMlpRow + t] = a;
+      part[(size_t)blockIdx.x * 2 * kMlpRow + kMlpRow + t] = b;
+    }
+  }
+}
+
+FeatSrc feat_src(ReState& re, const double* depth) {
+  FeatSrc F;
+  F.g = re.geo();
+  F.hn = re.d_hn();
+  F.kr = re.d_kr();
+  F.ch = re.d_ch();
+  F.wcn = re.d_wcn();
+  F.soil = re.soil_view();
+  F.depth = depth;
+  return F;
+}
+
+constexpr int kStatBlocks = 256;
+
+}  // namespace
+
+Mlp::Mlp(hipStream_t s) : s_(s) {
+  FREHG_HIP(hipMalloc(&params_, kMlpParams * sizeof(float)));
+  FREHG_HIP(hipMalloc(&m_, kMlpParams * sizeof(float)));
+  FREHG_HIP(hipMalloc(&v_, kMlpParams * sizeof(float)));
+  FREHG_HIP(hipMalloc(&norm_, kMlpNorm * sizeof(float)));
+  FREHG_HIP(hipMalloc(&grad_, (kMlpParams + 1) * sizeof(float)));
+  FREHG_HIP(hipMalloc(&part_, (size_t)TRAIN_BLOCKS * (kMlpParams + 1) * sizeof(float)));
+  FREHG_HIP(hipMalloc(&stat_, (size_t)kStatBlocks * 2 * kMlpRow * sizeof(double)));
+  FREHG_HIP(hipHostMalloc(&hbuf_, (kMlpParams + 1) * sizeof(float), hipHostMallocDefault));
+  init(1);
+}
+
+Mlp::~Mlp() {
+  for (void* p : {(void*)params_, (void*)m_, (void*)v_, (void*)norm_, (void*)grad_, (void*)part_, (void*)log_,
+                  (void*)scratch_, (void*)stat_})
+    if (p) (void)hipFree(p);
+  if (hbuf_) (void)hipHostFree(hbuf_);
+}
+
+void Mlp::init(uint64_t seed) {
+  std::vector<float> P(kMlpParams, 0.0f), N(kMlpNorm, 0.0f);
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 12345;
+  auto uni = [&]() {  // splitmix64 -> [0, 1)
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  auto fill = [&](int off, int fan_out, int fan_in) {
+    const double a = std::sqrt(6.0 / (fan_in + fan_out));
+    for (int q = 0; q < fan_out * fan_in; ++q) P[off + q] = (float)((2.0 * uni() - 1.0) * a);
+  };
+  fill(kOffW1, kMlpH, kMlpIn);
+  fill(kOffW2, kMlpH, kMlpH);
+  fill(kOffW3, 1, kMlpH);
+  for (int t = 0; t < kMlpRow; ++t) N[kMlpRow + t] = 1.0f;
+  FREHG_HIP(hipMemcpy(params_, P.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+  FREHG_HIP(hipMemset(m_, 0, kMlpParams * sizeof(float)));
+  FREHG_HIP(hipMemset(v_, 0, kMlpParams * sizeof(float)));
+  FREHG_HIP(hipMemcpy(norm_, N.data(), kMlpNorm * sizeof(float), hipMemcpyHostToDevice));
+  adam_t_ = 0;
+  have_norm_ = false;
+  ready_ = false;
+}
+
+void Mlp::set_logging(int stride, long long capacity) {
+  if (stride < 0) throw ArgError("frehg_mlp_log: stride must be >= 0");
+  if (stride > 0 && capacity < 1) throw ArgError("frehg_mlp_log: capacity must be >= 1");
+  if (stride > 0 && capacity != cap_) {
+    if (log_) FREHG_HIP(hipFree(log_));
+    log_ = nullptr;
+    FREHG_HIP(hipMalloc(&log_, (size_t)capacity * kMlpRow * sizeof(float)));
+    cap_ = capacity;
+    count_ = 0;
+  }
+  stride_ = stride;
+}
+
+void Mlp::log_pair(ReState& re, const double* hn, const double* h, const double* depth) {
+  if (!logging()) return;
+  FeatSrc F = feat_src(re, depth);
+  F.hn = hn;
+  const Geo& g = F.g;
+  const long long N = (long long)g.nx * g.ny * g.nz;
+  const int phase = (int)(log_steps_ % stride_);
+  const long long nsamp = phase < N ? (N - phase + stride_ - 1) / stride_ : 0;
+  if (nsamp > 0) {
+    hipLaunchKernelGGL(k_mlp_log, dim3((unsigned)ceil_div(nsamp, (long long)MT)), dim3(MT), 0, s_, F, h, stride_,
+                       phase, count_ % cap_, cap_, log_);
+    FREHG_LAUNCH_CHECK();
+  }
+  count_ += nsamp;
+  ++log_steps_;
+}
+
+double Mlp::run_grad(const float* rows, long long nrows, int batch, uint64_t seed, int sequential) {
+  const long long n = sequential ? nrows : batch;
+  const int nb = (int)std::min<long long>(TRAIN_BLOCKS, (n + RB - 1) / RB);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(MT), 0, s_, (const float*)params_, (const float*)norm_, rows, nrows,
+                     n, sequential, seed, 1.0f / (float)n, part_);
+  FREHG_LAUNCH_CHECK();
+  return (double)nb;
+}
+
+double Mlp::train(int steps, int batch, double lr, uint64_t seed, double* first) {
+  const long long nlog = logged();
+  if (nlog < 1) throw ArgError("frehg_mlp_train: no logged pairs (enable logging and run plain steps first)");
+  if (steps < 1 || batch < 1) throw ArgError("frehg_mlp_train: steps and batch must be >= 1");
+  if (!have_norm_) {
+    hipLaunchKernelGGL(k_mlp_stats, dim3(kStatBlocks), dim3(MT), 0, s_, (const float*)log_, nlog, stat_);
+    FREHG_LAUNCH_CHECK();
+    std::vector<double> part((size_t)kStatBlocks * 2 * kMlpRow);
+    FREHG_HIP(hipMemcpyAsync(part.data(), stat_, part.size() * sizeof(double), hipMemcpyDeviceToHost, s_));
+    FREHG_HIP(hipStreamSynchronize(s_));
+    std::vector<float> N(kMlpNorm);
+    for (int t = 0; t < kMlpRow; ++t) {
+      double s = 0, s2 = 0;
+      for (int b = 0; b < kStatBlocks; ++b) {
+        s += part[(size_t)b * 2 * kMlpRow + t];
+        s2 += part[(size_t)b * 2 * kMlpRow + kMlpRow + t];
+      }
+      const double mu = s / (double)nlog;
+      const double var = std::max(0.0, s2 / (double)nlog - mu * mu);
+      const double sd = std::sqrt(var);
+      N[t] = (float)mu;
+      N[kMlpRow + t] = sd > 1e-12 * std::max(1.0, std::fabs(mu)) ? (float)sd : 1.0f;
+    }
+    FREHG_HIP(hipMemcpyAsync(norm_, N.data(), kMlpNorm * sizeof(float), hipMemcpyHostToDevice, s_));
+    have_norm_ = true;
+  }
+  double loss_first = 0.0, loss = 0.0;
+  for (int it = 0; it < steps; ++it) {
+    const int nb = (int)run_grad(log_, nlog, batch, seed + 0x100000001ull * (uint64_t)(adam_t_ + 1), 0);
+    ++adam_t_;
+    const float b1t = (float)std::pow(0.9, adam_t_), b2t = (float)std::pow(0.999, adam_t_);
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(ceil_div(kMlpParams + 1, MT)), dim3(MT), 0, s_, (const float*)part_, nb,
+                       grad_, params_, m_, v_, 1, (float)lr, b1t, b2t);
+    FREHG_LAUNCH_CHECK();
+    if (it == 0 || it == steps - 1) {
+      FREHG_HIP(hipMemcpyAsync(hbuf_, grad_ + kMlpParams, sizeof(float), hipMemcpyDeviceToHost, s_));
+      FREHG_HIP(hipStreamSynchronize(s_));
+      const double l = hbuf_[0] / (double)batch;
+      if (it == 0) loss_first = l;
+      loss = l;
+    }
+  }
+  if (first) *first = loss_first;
+  ready_ = true;
+  return loss;
+}
+
+void Mlp::predict_guess(ReState& re, double* h_out, const double* depth) {
+  if (!ready_) throw ArgError("learned guess requested but the model is not trained or loaded");
+  const FeatSrc F = feat_src(re, depth);
+  const long long N = (long long)F.g.nx * F.g.ny * F.g.nz;
+  const unsigned nb = (unsigned)std::min<long long>(2048, (N + RB - 1) / RB);
+  hipLaunchKernelGGL(k_mlp_guess, dim3(nb), dim3(MT), 0, s_, (const float*)params_, (const float*)norm_, F, h_out);
+  FREHG_LAUNCH_CHECK();
+}
+
+void Mlp::get_params(float* params, float* norm) {
+  if (params) FREHG_HIP(hipMemcpyAsync(params, params_, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, s_));
+  if (norm) FREHG_HIP(hipMemcpyAsync(norm, norm_, kMlpNorm * sizeof(float), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipStreamSynchronize(s_));
+}
+
+void Mlp::set_params(const float* params, const float* norm) {
+  if (!params || !norm) throw ArgError("frehg_mlp_set_params: null argument");
+  for (int t = 0; t < kMlpRow; ++t)
+    if (!(norm[kMlpRow + t] > 0.0f)) throw ArgError("frehg_mlp_set_params: standard deviations must be > 0");
+  FREHG_HIP(hipMemcpyAsync(params_, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, s_));
+  FREHG_HIP(hipMemcpyAsync(norm_, norm, kMlpNorm * sizeof(float), hipMemcpyHostToDevice, s_));
+  FREHG_HIP(hipMemsetAsync(m_, 0, kMlpParams * sizeof(float), s_));
+  FREHG_HIP(hipMemsetAsync(v_, 0, kMlpParams * sizeof(float), s_));
+  FREHG_HIP(hipStreamSynchronize(s_));
+  adam_t_ = 0;
+  have_norm_ = true;
+  ready_ = true;
+}
+
+void Mlp::ensure_scratch(int nrows) {
+  if (nrows <= scratch_rows_) return;
+  if (scratch_) FREHG_HIP(hipFree(scratch_));
+  scratch_ = nullptr;
+  FREHG_HIP(hipMalloc(&scratch_, (size_t)nrows * kMlpRow * sizeof(float)));
+  scratch_rows_ = nrows;
+}
+
+double Mlp::loss_grad(const float* X, const float* Y, int n, float* grad) {
+  if (n < 1 || !X || !Y) throw ArgError("frehg_mlp_loss_grad: need n >= 1 rows");
+  ensure_scratch(n);
+  std::vector<float> rows((size_t)n * kMlpRow);
+  for (int r = 0; r < n; ++r) {
+    for (int t = 0; t < kMlpIn; ++t) rows[(size_t)r * kMlpRow + t] = X[(size_t)r * kMlpIn + t];
+    rows[(size_t)r * kMlpRow + kMlpIn] = Y[r];
+  }
+  FREHG_HIP(hipMemcpyAsync(scratch_, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice, s_));
+  const int nb = (int)run_grad(scratch_, n, n, 0, 1);
+  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(ceil_div(kMlpParams + 1, MT)), dim3(MT), 0, s_, (const float*)part_, nb,
+                     grad_, params_, m_, v_, 0, 0.0f, 0.0f, 0.0f);
+  FREHG_LAUNCH_CHECK();
+  FREHG_HIP(hipMemcpyAsync(hbuf_, grad_, (kMlpParams + 1) * sizeof(float), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipStreamSynchronize(s_));
+  if (grad) std::memcpy(grad, hbuf_, kMlpParams * sizeof(float));
+  return hbuf_[kMlpParams] / (double)n;
+}
+
+void Mlp::predict_batch(const float* X, int n, float* out) {
+  if (n < 1 || !X || !out) throw ArgError("frehg_mlp_predict: need n >= 1 rows");
+  ensure_scratch(n);
+  std::vector<float> rows((size_t)n * kMlpRow, 0.0f);
+  for (int r = 0; r < n; ++r)
+    for (int t = 0; t < kMlpIn; ++t) rows[(size_t)r * kMlpRow + t] = X[(size_t)r * kMlpIn + t];
+  FREHG_HIP(hipMemcpyAsync(scratch_, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice, s_));
+  float* tmp = nullptr;
+  FREHG_HIP(hipMalloc(&tmp, (size_t)n * sizeof(float)));
+  const unsigned nb = (unsigned)std::min<long long>(1024, ((long long)n + RB - 1) / RB);
+  hipLaunchKernelGGL(k_mlp_predict_rows, dim3(nb), dim3(MT), 0, s_, (const float*)params_, (const float*)norm_,
+                     (const float*)scratch_, (long long)n, tmp);
+  FREHG_LAUNCH_CHECK();
+  FREHG_HIP(hipMemcpyAsync(out, tmp, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipStreamSynchronize(s_));
+  FREHG_HIP(hipFree(tmp));
+}
+
+}  // namespace frehg

@@ -1,6 +1,18 @@
-// Learned Picard/Krylov initial guess: a 11 -> 64 -> 64 -> 1 tanh MLP over local stencil
-// features, trained on (state, converged head increment) pairs logged from the plain solver.
+// Learned Krylov initial guess for the Richards step: an 11 -> 64 -> 64 -> 1 MLP with tanh hidden
+// layers that predicts each cell's head increment h^{n+1} - h^n from local stencil features, so
+// that the solve starts from x0 = h^n + prediction instead of x0 = 0. Trained on the GPU by this
+// file's own kernels (BF16 MFMA forward and backward with FP32 accumulation, Adam on FP32 master
+// weights) from (features, converged increment) pairs logged while the plain solver runs. The
+// solve itself is unchanged: it converges to the same tolerance and falls back to x0 = 0 when the
+// guess's residual is not below the zero guess's.
+//
+// Features of cell c (all from the state before the solve): h^n(c); h^n of the six face
+// neighbours minus h^n(c); ln K(h^n) (Ks_z * kr); C(h^n); theta^n; surface water depth of the
+// column (0 without a surface model). Features and target are standardised with the mean and
+// standard deviation of the logged data.
 #pragma once
+
+#include <cstdint>
 
 #include "common.hpp"
 
@@ -8,15 +20,75 @@ namespace frehg {
 
 class ReState;
 
+constexpr int kMlpIn = 11;
+constexpr int kMlpH = 64;
+// Parameter vector (FP32): W1[64][11], b1[64], W2[64][64], b2[64], w3[64], b3.
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kMlpH * kMlpIn;
+constexpr int kOffW2 = kOffB1 + kMlpH;
+constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH;
+constexpr int kOffW3 = kOffB2 + kMlpH;
+constexpr int kOffB3 = kOffW3 + kMlpH;
+constexpr int kMlpParams = kOffB3 + 1;  // 4993
+constexpr int kMlpRow = kMlpIn + 1;     // logged row: 11 features + target
+constexpr int kMlpNorm = 2 * kMlpRow;   // mean[12], sd[12]
+
 class Mlp {
  public:
-  virtual ~Mlp() = default;
-  virtual bool ready() const = 0;
-  virtual bool logging() const = 0;
+  explicit Mlp(hipStream_t s);
+  ~Mlp();
+  Mlp(const Mlp&) = delete;
+  Mlp& operator=(const Mlp&) = delete;
+
+  // Glorot-uniform weights, zero biases, identity normalisation (deterministic in `seed`).
+  void init(uint64_t seed);
+  bool ready() const { return ready_; }
+
+  // Pair logging: every `stride`-th interior cell (the phase rotates each logged step) into a
+  // ring buffer of `capacity` rows. stride 0 switches logging off; the buffer is kept.
+  void set_logging(int stride, long long capacity);
+  bool logging() const { return stride_ > 0; }
+  long long logged() const { return count_ < cap_ ? count_ : cap_; }
+  void clear_log() { count_ = 0; }
+  void log_pair(ReState& re, const double* hn, const double* h, const double* depth);
+
+  // `steps` Adam steps (lr, beta1 0.9, beta2 0.999, eps 1e-8) on minibatches of `batch` rows
+  // drawn with replacement from the log. The first call computes the normalisation from the log.
+  // Returns the mean standardised squared error of the last minibatch (first one in *first).
+  double train(int steps, int batch, double lr, uint64_t seed, double* first);
+
   // h_out (interior) = h^n + predicted increment.
-  virtual void predict_guess(ReState& re, double* h_out) = 0;
-  // Record (features(h^n), h^{n+1} - h^n) pairs after a converged solve.
-  virtual void log_pair(ReState& re, const double* hn, const double* h) = 0;
+  void predict_guess(ReState& re, double* h_out, const double* depth);
+
+  void get_params(float* params, float* norm);
+  void set_params(const float* params, const float* norm);
+
+  // Test hooks on a host batch of raw rows (n x 11 features, n targets): loss and its gradient
+  // with respect to the parameter vector, and forward predictions (raw increments).
+  double loss_grad(const float* X, const float* Y, int n, float* grad);
+  void predict_batch(const float* X, int n, float* out);
+
+ private:
+  void ensure_scratch(int nrows);
+  double run_grad(const float* rows, long long nrows, int batch, uint64_t seed, int sequential);
+
+  hipStream_t s_;
+  bool ready_ = false;
+  int stride_ = 0;
+  long long cap_ = 0, count_ = 0, log_steps_ = 0;
+  int adam_t_ = 0;
+  bool have_norm_ = false;
+  float* params_ = nullptr;  // FP32 master weights
+  float* m_ = nullptr;       // Adam moments
+  float* v_ = nullptr;
+  float* norm_ = nullptr;    // mean[12], sd[12]
+  float* grad_ = nullptr;    // reduced gradient + loss
+  float* part_ = nullptr;    // per-block partial gradients
+  float* log_ = nullptr;     // ring buffer, cap_ x 12
+  float* scratch_ = nullptr; // host-batch rows for the test hooks
+  long long scratch_rows_ = 0;
+  double* stat_ = nullptr;   // normalisation partials
+  float* hbuf_ = nullptr;    // pinned host mirror (loss)
 };
 
 }  // namespace frehg

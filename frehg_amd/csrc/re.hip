@@ -637,6 +637,9 @@ void ReState::set_initial(const double* h_int, const double* wc_int) {
 }
 
 void ReState::step(frehg_re_stats* st) {
+  // Reject a request that cannot be served before any state changes.
+  if (p_.initial_guess == FREHG_GUESS_LEARNED && (!mlp_ || !mlp_->ready()))
+    throw ArgError("frehg_re_step: learned guess requested but no trained model loaded");
   frehg_re_stats local;
   std::memset(&local, 0, sizeof(local));
   dt_used_ = p_.dt;
@@ -666,7 +669,7 @@ void ReState::step(frehg_re_stats* st) {
     x0_zero = false;
   } else if (p_.initial_guess == FREHG_GUESS_LEARNED) {
     if (!mlp_ || !mlp_->ready()) throw ArgError("frehg_re_step: learned guess requested but no trained model loaded");
-    mlp_->predict_guess(*this, h_);
+    mlp_->predict_guess(*this, h_, depth_);
     x0_zero = false;
     guard = true;
   }
@@ -682,7 +685,7 @@ void ReState::step(frehg_re_stats* st) {
   local.solve.fallback = sr.fallback_fp64 ? 1 : 0;
   local.guess_fallback = sr.guess_rejected ? 1 : 0;
   local.picard_iterations = 1;
-  if (mlp_ && mlp_->logging()) mlp_->log_pair(*this, hn_, h_);
+  if (mlp_ && mlp_->logging()) mlp_->log_pair(*this, hn_, h_, depth_);
   {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_CORRECT);
     hipEvent_t ev1 = prof_->stop();

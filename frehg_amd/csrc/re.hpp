@@ -41,11 +41,16 @@ class ReState {
   double* d_h() { return h_; }
   double* d_wc() { return wc_; }
   double* d_hn() { return hn_; }
+  double* d_wcn() { return wcn_; }
   double* d_kr() { return kr_; }
   double* d_ch() { return ch_; }
   SoilView soil_view() const { return SoilView{classes_, cls_, ks_}; }
   Krylov& krylov() { return *kry_; }
-  void set_mlp(Mlp* m) { mlp_ = m; }
+  // Learned-guess model and the surface depth it reads (either may be null).
+  void set_mlp(Mlp* m, const double* surface_depth) {
+    mlp_ = m;
+    depth_ = surface_depth;
+  }
 
  private:
   double* field_ptr(const std::string& name);
@@ -83,6 +88,7 @@ class ReState {
   double* qtop_ = nullptr;  // per-column top flux (surface padded) or null
   std::unique_ptr<Krylov> kry_;
   Mlp* mlp_ = nullptr;
+  const double* depth_ = nullptr;
   double dtn_ = 0.0, dt_used_ = 0.0;
 };
 

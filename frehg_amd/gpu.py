@@ -71,6 +71,14 @@ def load():
         "frehg_profile_read": (C.c_int, [ctx, C.c_int, dp, P(C.c_int64)]),
         "frehg_profile_reset": (C.c_int, [ctx]),
         "frehg_spmv_shape": (C.c_int, [ctx, C.c_int, P(C.c_int), P(C.c_int)]),
+        "frehg_mlp_init": (C.c_int, [ctx, C.c_uint64]),
+        "frehg_mlp_log": (C.c_int, [ctx, C.c_int, C.c_int64]),
+        "frehg_mlp_logged": (C.c_int, [ctx, P(C.c_int64)]),
+        "frehg_mlp_train": (C.c_int, [ctx, C.c_int, C.c_int, C.c_double, C.c_uint64, dp, dp]),
+        "frehg_mlp_get_params": (C.c_int, [ctx, P(C.c_float), P(C.c_float)]),
+        "frehg_mlp_set_params": (C.c_int, [ctx, P(C.c_float), P(C.c_float)]),
+        "frehg_mlp_loss_grad": (C.c_int, [ctx, P(C.c_float), P(C.c_float), C.c_int, P(C.c_float), dp]),
+        "frehg_mlp_predict": (C.c_int, [ctx, P(C.c_float), C.c_int, P(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -242,10 +250,56 @@ class Context:
     def profile_reset(self):
         self.check(self.lib.frehg_profile_reset(self.h))
 
+    # ---- learned initial guess ----
+    def mlp_init(self, seed=1):
+        self.check(self.lib.frehg_mlp_init(self.h, seed))
+
+    def mlp_log(self, stride, capacity):
+        self.check(self.lib.frehg_mlp_log(self.h, stride, capacity))
+
+    def mlp_logged(self):
+        n = C.c_int64()
+        self.check(self.lib.frehg_mlp_logged(self.h, C.byref(n)))
+        return n.value
+
+    def mlp_train(self, steps, batch=65536, lr=1e-3, seed=7):
+        a, b = C.c_double(), C.c_double()
+        self.check(self.lib.frehg_mlp_train(self.h, steps, batch, lr, seed, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def mlp_get_params(self):
+        p = np.zeros(abi.MLP_PARAMS, np.float32)
+        n = np.zeros(abi.MLP_NORM, np.float32)
+        self.check(self.lib.frehg_mlp_get_params(self.h, _fp(p), _fp(n)))
+        return p, n
+
+    def mlp_set_params(self, params, norm):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        n = np.ascontiguousarray(norm, dtype=np.float32)
+        self.check(self.lib.frehg_mlp_set_params(self.h, _fp(p), _fp(n)))
+
+    def mlp_loss_grad(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(abi.MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        self.check(self.lib.frehg_mlp_loss_grad(self.h, _fp(x), _fp(y), len(y), _fp(g), C.byref(loss)))
+        return loss.value, g
+
+    def mlp_predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.zeros(len(x), np.float32)
+        self.check(self.lib.frehg_mlp_predict(self.h, _fp(x), len(x), _fp(out)))
+        return out
+
     def spmv_shape(self, precision):
         ty, kc = C.c_int(), C.c_int()
         self.check(self.lib.frehg_spmv_shape(self.h, precision, C.byref(ty), C.byref(kc)))
         return ty.value, kc.value
+
+
+def _fp(a):
+    return a.ctypes.data_as(P(C.c_float))
 
 
 def declared_symbols(header=HEADER):

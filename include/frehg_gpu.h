@@ -197,6 +197,35 @@ int frehg_ls_solve(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, const doub
                    const double* const* offdiag, const double* b_owned, double* x_owned,
                    const frehg_solver_config* cfg, frehg_solve_stats* stats);
 
+/* ---- Learned initial guess (no counterpart in the reference: it changes only the Krylov
+ * x0 of the Richards solve, src/re/ReSolver.cpp:560-636, whose PETSc KSP always starts from 0).
+ * An 11 -> 64 -> 64 -> 1 tanh MLP over local features of the state before the solve (h^n(c), the
+ * six h^n(neighbour) - h^n(c), ln K(h^n), C(h^n), theta^n, surface depth of the column) predicts
+ * h^{n+1}(c) - h^n(c). BF16 MFMA forward and backward with FP32 accumulation, Adam on FP32
+ * master weights, all on the GPU. With frehg_re_params.initial_guess = FREHG_GUESS_LEARNED the
+ * solve starts from h^n + prediction and falls back to x0 = 0 (stats.guess_fallback = 1) when
+ * the guess's residual is not below the zero guess's. ----------------------------------------- */
+#define FREHG_MLP_INPUTS 11
+#define FREHG_MLP_PARAMS 4993 /* W1[64][11], b1[64], W2[64][64], b2[64], w3[64], b3 */
+#define FREHG_MLP_NORM 24     /* mean[12], sd[12] of the 11 features and the target */
+/* Glorot-uniform weights, zero biases, identity normalisation. */
+int frehg_mlp_init(frehg_ctx_t ctx, uint64_t seed);
+/* Log (features, converged increment) of every `stride`-th interior cell after each Richards
+ * solve into a ring buffer of `capacity` rows; stride 0 stops logging. */
+int frehg_mlp_log(frehg_ctx_t ctx, int stride, int64_t capacity);
+int frehg_mlp_logged(frehg_ctx_t ctx, int64_t* n);
+/* `steps` Adam steps (lr; beta1 0.9, beta2 0.999, eps 1e-8) on minibatches of `batch` logged
+ * rows drawn with replacement (seeded). The first call fixes the normalisation from the log.
+ * Loss = mean squared error of the standardised target, first and last minibatch. */
+int frehg_mlp_train(frehg_ctx_t ctx, int steps, int batch, double lr, uint64_t seed, double* loss_first,
+                    double* loss_last);
+int frehg_mlp_get_params(frehg_ctx_t ctx, float* params, float* norm);
+int frehg_mlp_set_params(frehg_ctx_t ctx, const float* params, const float* norm);
+/* Loss and d loss / d params on a host batch: x[n][11] raw features, y[n] raw increments. */
+int frehg_mlp_loss_grad(frehg_ctx_t ctx, const float* x, const float* y, int n, float* grad, double* loss);
+/* Predicted raw increments for x[n][11]. */
+int frehg_mlp_predict(frehg_ctx_t ctx, const float* x, int n, float* out);
+
 /* ---- measurement --------------------------------------------------------------------------- */
 /* Kernel-duration accounting with HIP events on the context's stream. kernel ids below. */
 enum {

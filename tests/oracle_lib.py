@@ -24,6 +24,9 @@ def lib():
             raise RuntimeError("oracle library missing: run `make -C oracle`")
         L = C.CDLL(LIB_PATH)
         L.orc_last_error.restype = C.c_char_p
+        fp = P(C.c_float)
+        L.orc_mlp_predict.argtypes = [fp, fp, fp, C.c_int, C.c_int, dp]
+        L.orc_mlp_loss_grad.argtypes = [fp, fp, fp, fp, C.c_int, C.c_int, dp, dp]
         L.orc_re_create.restype = C.c_void_p
         L.orc_re_create.argtypes = [C.c_int, C.c_int, C.c_int, P(abi.ReParams)]
         L.orc_re_destroy.argtypes = [C.c_void_p]
@@ -202,3 +205,28 @@ def vg_eval(soil, h=None, wc=None):
     th, cap, K, hd, dk = (np.empty(n) for _ in range(5))
     check(lib().orc_vg_eval(C.byref(soil), n, ptr(hh), ptr(ww), ptr(th), ptr(cap), ptr(K), ptr(hd), ptr(dk)))
     return {"theta": th, "C": cap, "K": K, "head": hd, "dKdwc": dk}
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def mlp_predict(params, norm, x, bf16):
+    """CPU reference forward pass of the learned-guess MLP (raw increments)."""
+    params, norm, x = _f32(params), _f32(norm), _f32(x)
+    out = np.zeros(len(x))
+    fp = C.POINTER(C.c_float)
+    lib().orc_mlp_predict(params.ctypes.data_as(fp), norm.ctypes.data_as(fp), x.ctypes.data_as(fp), len(x),
+                          int(bf16), ptr(out))
+    return out
+
+
+def mlp_loss_grad(params, norm, x, y, bf16):
+    """CPU reference loss and gradient of the learned-guess MLP."""
+    params, norm, x, y = _f32(params), _f32(norm), _f32(x), _f32(y)
+    grad = np.zeros(len(params))
+    loss = C.c_double()
+    fp = C.POINTER(C.c_float)
+    lib().orc_mlp_loss_grad(params.ctypes.data_as(fp), norm.ctypes.data_as(fp), x.ctypes.data_as(fp),
+                            y.ctypes.data_as(fp), len(x), int(bf16), ptr(grad), C.byref(loss))
+    return loss.value, grad
