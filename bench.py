@@ -6,10 +6,13 @@ Workload: the coupled 512x512 surface over 512x512x30 soil (dx = 1 m, dz = 0.05 
 generated from seeds by frehg_amd.workloads. One step = Orchestrator::step, coupled branch:
 SWE step, SW<->GW exchange, Richards PCA step with its Krylov solve.
 
-Two GPU paths run from the same initial state for warmup + timed steps:
-  plain  FP64 Jacobi-PCG from x0 = 0 (the reference's solver semantics)
-  fast   FP32-inner / FP64-refined PCG from the best available initial guess
-`value` is the fast path's throughput; its final head is checked against the plain path's.
+Three GPU paths run from the same initial state for warmup + timed steps:
+  plain    FP64 Jacobi-PCG from x0 = 0 (the reference's solver semantics)
+  learned  FP32-inner / FP64-refined PCG from x0 = h^n + MLP increment (zero-guess fallback)
+  previous FP32-inner / FP64-refined PCG from x0 = h^n
+The MLP is trained beforehand (not timed) on pairs logged from plain runs of two other seeded
+bed/Ks realisations of the same workload. `value` is the learned path's throughput; every
+path's final head is checked against the plain path's.
 Kernel durations come from HIP events on the backend's stream during the timed region.
 The CPU baseline is the repository's FP64 oracle (a restatement of the reference algorithm)
 timed on a bounded sample of the same workload on this host's cores.
@@ -27,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
 
 
 def log(msg):
@@ -37,11 +41,17 @@ def interior3(a, nx, ny, nz):
     return a.reshape(nz + 2, ny + 2, nx + 2)[1:-1, 1:-1, 1:-1]
 
 
-def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None):
+KERNEL_TYPES = ("spmv64", "spmv32", "update", "assemble", "correct", "refine", "sw_spmv", "sw_update", "guess")
+
+
+def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None):
     """Warmup + timed coupled steps; returns timing, solver counters and kernel timings."""
     ctx = gpu.Context(0)
     if path == "plain":
         case.setup_gpu(ctx, abi.PREC_FP64, abi.GUESS_ZERO)
+    elif path == "learned":
+        ctx.mlp_set_params(*model)
+        case.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_LEARNED)
     else:
         case.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_PREVIOUS)
     grabbed = None
@@ -52,25 +62,53 @@ def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None):
     ctx.synchronize()
     ctx.profile_enable(True)
     ctx.profile_reset()
-    iters = refines = 0
+    iters = refines = fallbacks = 0
     t0 = time.perf_counter()
     for s in range(steps):
         st = ctx.coupled_step(case.cp, case.dt, case.rain, 0.0, case.t_end)
         iters += st.gw.solve.iterations
         refines += st.gw.solve.refinements
+        fallbacks += st.gw.guess_fallback
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
-    kt = {k: ctx.profile_read(v) for k, v in (("spmv64", abi.KT_GW_SPMV), ("spmv32", abi.KT_GW_SPMV32),
-                                                 ("update", abi.KT_GW_UPDATE), ("assemble", abi.KT_GW_ASSEMBLE),
-                                                 ("correct", abi.KT_GW_CORRECT), ("refine", abi.KT_GW_REFINE),
-                                                 ("sw_spmv", abi.KT_SW_SPMV), ("sw_update", abi.KT_SW_UPDATE))}
+    kt = {k: ctx.profile_read(v) for k, v in zip(KERNEL_TYPES, (abi.KT_GW_SPMV, abi.KT_GW_SPMV32, abi.KT_GW_UPDATE,
+                                                               abi.KT_GW_ASSEMBLE, abi.KT_GW_CORRECT, abi.KT_GW_REFINE,
+                                                               abi.KT_SW_SPMV, abi.KT_SW_UPDATE, abi.KT_GW_GUESS))}
     ctx.profile_enable(False)
     shape = ctx.spmv_shape(abi.PREC_FP64 if path == "plain" else abi.PREC_MIXED)
     h = ctx.re_get("h")
     sw_vol, gw_vol = ctx.water_volume()
     ctx.close()
-    return {"elapsed": elapsed, "iters": iters, "refines": refines, "kt": kt, "h": h, "grabbed": grabbed,
-            "volume": sw_vol + gw_vol, "spmv_shape": shape}
+    return {"elapsed": elapsed, "iters": iters, "refines": refines, "fallbacks": fallbacks, "kt": kt, "h": h,
+            "grabbed": grabbed, "volume": sw_vol + gw_vol, "spmv_shape": shape}
+
+
+def train_model(abi, gpu, workloads, args, seeds, log_steps, stride, train_steps, batch):
+    """Log (features, converged increment) pairs from plain FP64 runs of other seeded realisations
+    of the workload, then train the MLP on the GPU. Returns (params, norm) and a summary."""
+    ctx = gpu.Context(0)
+    ctx.mlp_init(17)
+    ncell = args.nx * args.ny * args.nz
+    ctx.mlp_log(stride, capacity=len(seeds) * log_steps * ((ncell + stride - 1) // stride))
+    t0 = time.perf_counter()
+    for sb, sk in seeds:
+        case = workloads.CoupledCase(nx=args.nx, ny=args.ny, nz=args.nz, seed_bed=sb, seed_ks=sk)
+        case.setup_gpu(ctx, abi.PREC_FP64, abi.GUESS_ZERO)
+        for _ in range(log_steps):
+            ctx.coupled_step(case.cp, case.dt, case.rain, 0.0, case.t_end)
+    ctx.synchronize()
+    t_log = time.perf_counter() - t0
+    npairs = ctx.mlp_logged()
+    t0 = time.perf_counter()
+    first, last = ctx.mlp_train(train_steps, batch=batch, lr=1e-3, seed=29)
+    ctx.synchronize()
+    t_train = time.perf_counter() - t0
+    model = ctx.mlp_get_params()
+    ctx.close()
+    return model, {"pairs": npairs, "realisations": [list(x) for x in seeds], "steps_logged_each": log_steps,
+                   "cell_stride": stride, "adam_steps": train_steps, "batch": batch, "lr": 1e-3,
+                   "loss_first": round(first, 5), "loss_last": round(last, 5),
+                   "log_seconds": round(t_log, 2), "train_seconds": round(t_train, 2)}
 
 
 def cpu_baseline(case, abi, steps):
@@ -102,6 +140,8 @@ def main():
     ap.add_argument("--nz", type=int, default=30)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--log-steps", type=int, default=20)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run an elementwise 1 GiB -> 1 GiB kernel (torch), the reference point that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -132,13 +172,22 @@ def main():
     log("[bench] plain FP64 path")
     plain = run_gpu(case, abi, gpu, "plain", args.warmup, args.steps, grab_after=cpu_steps or None)
     log(f"[bench]   {plain['elapsed']:.3f} s, {plain['iters']} Krylov iterations")
-    log("[bench] fast path (FP32-in-FP64 refinement, previous-step guess)")
-    fast = run_gpu(case, abi, gpu, "fast", args.warmup, args.steps)
-    log(f"[bench]   {fast['elapsed']:.3f} s, {fast['iters']} inner iterations, {fast['refines']} refinements")
+    log("[bench] training the learned guess on two other seeded realisations (not timed)")
+    model, training = train_model(abi, gpu, workloads, args, seeds=((101, 102), (103, 104)),
+                                  log_steps=args.log_steps, stride=4, train_steps=args.train_steps, batch=65536)
+    log(f"[bench]   {training['pairs']} pairs, loss {training['loss_first']} -> {training['loss_last']}")
+    log("[bench] learned path (FP32-in-FP64 refinement, MLP guess)")
+    fast = run_gpu(case, abi, gpu, "learned", args.warmup, args.steps, model=model)
+    log(f"[bench]   {fast['elapsed']:.3f} s, {fast['iters']} inner iterations, {fast['refines']} refinements, "
+        f"{fast['fallbacks']} fallbacks")
+    log("[bench] previous-step path (FP32-in-FP64 refinement, x0 = h^n)")
+    prev = run_gpu(case, abi, gpu, "previous", args.warmup, args.steps)
+    log(f"[bench]   {prev['elapsed']:.3f} s, {prev['iters']} inner iterations, {prev['refines']} refinements")
 
     hp = interior3(plain["h"], case.nx, case.ny, case.nz)
     hf = interior3(fast["h"], case.nx, case.ny, case.nz)
     head_err = float(np.max(np.abs(hp - hf)))
+    head_err_prev = float(np.max(np.abs(hp - interior3(prev["h"], case.nx, case.ny, case.nz))))
 
     def roof(kt_ms_n, bytes_per_cell):
         ms, n = kt_ms_n
@@ -153,9 +202,14 @@ def main():
     # fused PCG SpMV kernel: reads z, p_prev, D, ax, ay, az; writes p, q (8 words per cell)
     roof64 = roof(plain["kt"]["spmv64"], 64)
     roof32 = roof(fast["kt"]["spmv32"], 32)
-    # dominant kernel of the measured (fast) path
-    fast_k = {k: v[0] for k, v in fast["kt"].items()}
-    dominant = max(fast_k, key=fast_k.get)
+    roof_guess = None
+    ms, n = fast["kt"]["guess"]
+    if n:
+        flops = 2 * (11 * 64 + 64 * 64 + 64) * case.nx * case.ny * case.nz  # algorithmic MLP forward
+        tf = flops / (ms / n / 1e3) / 1e12
+        roof_guess = {"bound": "mfma", "achieved": round(tf, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                      "frac": round(tf / BF16_PEAK_TFLOPS, 5), "avg_launch_us": round(ms / n * 1e3, 2),
+                      "launches": n, "flop_per_cell": 9728}
 
     cpu = None
     if cpu_steps:
@@ -183,11 +237,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64+f32+bf16",
         "data": "synthetic (seeded Gaussian random fields: bed sigma 0.05 m / 20 m, ln Ks sigma 1 / 10 m)",
         "config": {"workload": f"coupled SW {args.nx}x{args.ny} + GW {args.nx}x{args.ny}x{args.nz}, dx 1 m, "
                                f"dz 0.05 m, dt 1 s, 50 mm/h rain, closed sides",
-                   "cells_per_step": case.cells, "fast_path": "fp32-in-fp64 refinement, previous-step guess",
+                   "cells_per_step": case.cells, "fast_path": "fp32-in-fp64 refinement, learned (MLP) guess",
                    "gw_rtol": 1e-10, "sw_rtol": 1e-12},
         "plain_fp64": {"value": round(plain_value, 1), "ms_per_step": round(plain["elapsed"] / args.steps * 1e3, 4),
                        "krylov_iters_per_step": plain["iters"] / args.steps},
@@ -195,10 +249,18 @@ def main():
         "max_head_err_vs_plain_fp64_m": head_err,
         "fast_inner_iters_per_step": fast["iters"] / args.steps,
         "fast_refinements_per_step": fast["refines"] / args.steps,
-        "roofline": roof32 if dominant == "spmv32" and roof32 else roof64,
-        "roofline_kernel": "gw_pcg_spmv fp32 (fused p-update + 7-pt SpMV + dot)" if dominant == "spmv32"
-                           else "gw_pcg_spmv fp64 (fused p-update + 7-pt SpMV + dot)",
-        "roofline_fp64_spmv": roof64,
+        "fast_guess_fallbacks": fast["fallbacks"],
+        "previous_guess_mixed": {"value": round(case.cells * args.steps / prev["elapsed"], 1),
+                                 "ms_per_step": round(prev["elapsed"] / args.steps * 1e3, 4),
+                                 "inner_iters_per_step": prev["iters"] / args.steps,
+                                 "refinements_per_step": prev["refines"] / args.steps,
+                                 "max_head_err_vs_plain_fp64_m": head_err_prev},
+        "training": training,
+        "roofline": roof64,
+        "roofline_kernel": "k_spmv<double> of the plain path: fused p = z + beta p_prev, 7-point q = A p, "
+                           "partial p.q (64 B per cell)",
+        "roofline_fp32_spmv": roof32,
+        "roofline_mlp_guess": roof_guess,
         "spmv_shape_tile_y_kchunk": {"plain_fp64": plain["spmv_shape"], "fast_fp32": fast["spmv_shape"]},
         "kernel_ms_fast": {k: round(v[0], 3) for k, v in fast["kt"].items()},
         "kernel_ms_plain": {k: round(v[0], 3) for k, v in plain["kt"].items()},

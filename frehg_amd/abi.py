@@ -40,7 +40,8 @@ MLP_NORM = 24
 KT_SW_SPMV = 5
 KT_SW_UPDATE = 6
 KT_GW_REFINE = 7
-KT_COUNT = 8
+KT_GW_GUESS = 8
+KT_COUNT = 9
 
 BC_FREE_DRAINAGE = 3
 

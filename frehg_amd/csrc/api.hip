@@ -72,7 +72,7 @@ void bind_mlp(frehg_ctx_t c) {
 
 frehg::Mlp& need_mlp(frehg_ctx_t c) {
   if (!c->mlp) {
-    c->mlp.reset(new frehg::Mlp(c->stream));
+    c->mlp.reset(new frehg::Mlp(c->stream, &c->prof));
     bind_mlp(c);
   }
   return *c->mlp;

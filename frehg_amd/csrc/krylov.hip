@@ -1148,7 +1148,11 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
   while (!conv && res.refinements < cfg.max_refine) {
     bool rej = false, iconv = false;
     double rn = 0, bnrm = 0;
-    res.iterations += run<float>(cfg.ksp, A32v, b32_, nullptr, d32_, fixed, true, false, cfg.inner_rtol, 0.0,
+    // The inner solve only has to close the gap to the outer tolerance (with a factor-2 margin):
+    // a good initial guess leaves a small gap and needs fewer inner iterations.
+    const double gap = std::sqrt(host_ctl_->ttol2) / std::max(res.residual_norm, 1e-300);
+    const double inner_rt = std::min(0.5, std::max(cfg.inner_rtol, 0.5 * gap));
+    res.iterations += run<float>(cfg.ksp, A32v, b32_, nullptr, d32_, fixed, true, false, inner_rt, 0.0,
                                  cfg.max_it, inner, &rej, &rn, &bnrm, &iconv,
                                  three_d_ ? FREHG_KT_GW_SPMV32 : FREHG_KT_SW_SPMV);
     refine(xcur, d32_, xalt, false);

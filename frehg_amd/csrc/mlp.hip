@@ -17,6 +17,7 @@
 #include <cstring>
 #include <vector>
 
+#include "../../include/frehg_gpu.h"
 #include "mlp.hpp"
 #include "re.hpp"
 
@@ -50,6 +51,20 @@ __device__ __forceinline__ bf16x8 pack8(const float* p, int st) {
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// tanh x = 1 - 2 / (1 + e^{2x}) with the hardware exp2 and reciprocal (absolute error ~1e-7, far
+// below the BF16 rounding of the activations); saturates correctly for large |x|.
+__device__ __forceinline__ float act_tanh(float x) {
+  const float e = __expf(2.0f * x);
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
+}
+
+// Orders this wave's LDS accesses in the compiler's view (a wave executes them in order).
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Inputs of the feature function: the Richards state before the solve.
@@ -124,7 +139,7 @@ __device__ f32x4 forward16(MlpLds& L, float b3) {
     acc = mfma(xa, pack8(&L.w1[16 * nb + ra][kg], 1), acc);
     const int col = 16 * nb + ra;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) L.a1[row0 + rd + r][col] = tanhf(acc[r] + L.b1[col]);
+    for (int r = 0; r < 4; ++r) L.a1[row0 + rd + r][col] = act_tanh(acc[r] + L.b1[col]);
   }
   __syncthreads();
   // layer 2: K = 64
@@ -138,7 +153,7 @@ __device__ f32x4 forward16(MlpLds& L, float b3) {
     for (int ks = 0; ks < 2; ++ks) acc = mfma(aa[ks], pack8(&L.w2[16 * nb + ra][32 * ks + kg], 1), acc);
     const int col = 16 * nb + ra;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) L.a2[row0 + rd + r][col] = tanhf(acc[r] + L.b2[col]);
+    for (int r = 0; r < 4; ++r) L.a2[row0 + rd + r][col] = act_tanh(acc[r] + L.b2[col]);
   }
   __syncthreads();
   // layer 3: B = w3 in column 0, zero elsewhere
@@ -338,41 +353,116 @@ __global__ __launch_bounds__(MT) void k_mlp_predict_rows(const float* P, const f
   }
 }
 
-// Learned guess on every interior cell: h_out = h^n + increment.
-__global__ __launch_bounds__(MT) void k_mlp_guess(const float* P, const float* norm, FeatSrc F, double* h_out) {
-  __shared__ MlpLds L;
-  load_weights(L, P);
+// Learned guess on every interior cell: h_out = h^n + increment. Each wave works alone on
+// chunks of 64 cells (no block barriers): every lane computes one cell's features into the
+// wave's LDS rows, then four 16-cell tiles go through the three layers with the weight operands
+// held in registers for the whole kernel.
+constexpr int GW = 4;  // waves per block of the guess kernel
+struct GuessLds {
+  float x[GW][RB][XP];
+  float act[GW][16][HP];
+};
+
+__global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* P, const float* norm, FeatSrc F, double* h_out) {
+  __shared__ GuessLds L;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ra = lane & 15, kg = 8 * (lane >> 4), rd = 4 * (lane >> 4);
+  // weight operands: B[k][n] = W[n][k] for the lane's (k, n) slots
+  bf16x8 w1b[4], w2b[4][2], w3b[2];
+  float b1c[4], b2c[4];
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb) {
+    const int n = 16 * nb + ra;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int f = kg + t;
+      w1b[nb][t] = (__bf16)(f < kMlpIn ? P[kOffW1 + n * kMlpIn + f] : 0.0f);
+    }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) w2b[nb][ks][t] = (__bf16)P[kOffW2 + n * kMlpH + 32 * ks + kg + t];
+    b1c[nb] = P[kOffB1 + n];
+    b2c[nb] = P[kOffB2 + n];
+  }
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int t = 0; t < 8; ++t) w3b[ks][t] = (__bf16)(ra == 0 ? P[kOffW3 + 32 * ks + kg + t] : 0.0f);
   const float b3 = P[kOffB3];
+  float nmu[kMlpIn], nisd[kMlpIn];
+#pragma unroll
+  for (int t = 0; t < kMlpIn; ++t) {
+    nmu[t] = norm[t];
+    nisd[t] = 1.0f / norm[kMlpRow + t];
+  }
+  const float ymu = norm[kMlpIn], ysd = norm[kMlpRow + kMlpIn];
   const Geo& g = F.g;
   const long long N = (long long)g.nx * g.ny * g.nz;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, ra = lane & 15, rd = 4 * (lane >> 4);
   const long long nchunks = (N + RB - 1) / RB;
-  for (long long ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-    const long long q0 = ch * RB;
-    __syncthreads();
-    if (threadIdx.x < RB) {
-      const long long q = q0 + threadIdx.x;
-      float f[kMlpIn];
-      if (q < N) {
-        const int i = (int)(q % g.nx), j = (int)((q / g.nx) % g.ny), k = (int)(q / ((long long)g.nx * g.ny));
-        cell_features(F, i, j, k, f);
-      }
+  float(*xw)[XP] = L.x[w];
+  float(*aw)[HP] = L.act[w];
+  for (long long ch = (long long)blockIdx.x * GW + w; ch < nchunks; ch += (long long)gridDim.x * GW) {
+    const long long q = ch * RB + lane;
+    float f[kMlpIn];
 #pragma unroll
-      for (int t = 0; t < kMlpIn; ++t) L.x[threadIdx.x][t] = q < N ? (f[t] - norm[t]) / norm[kMlpRow + t] : 0.0f;
-      for (int t = kMlpIn; t < 32; ++t) L.x[threadIdx.x][t] = 0.0f;
+    for (int t = 0; t < kMlpIn; ++t) f[t] = 0.0f;
+    if (q < N) {
+      const int i = (int)(q % g.nx), j = (int)((q / g.nx) % g.ny), k = (int)(q / ((long long)g.nx * g.ny));
+      cell_features(F, i, j, k, f);
+#pragma unroll
+      for (int t = 0; t < kMlpIn; ++t) f[t] = (f[t] - nmu[t]) * nisd[t];
     }
-    __syncthreads();
-    const f32x4 y = forward16(L, b3);
-    if (ra == 0)
+    wave_lds_fence();
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const long long q = q0 + 16 * w + rd + r;
-        if (q < N) {
-          const int i = (int)(q % g.nx), j = (int)((q / g.nx) % g.ny), k = (int)(q / ((long long)g.nx * g.ny));
-          const long long c = g.idx(i, j, k);
-          h_out[c] = F.hn[c] + (double)(y[r] * norm[kMlpRow + kMlpIn] + norm[kMlpIn]);
+    for (int t = 0; t < kMlpIn; ++t) xw[lane][t] = f[t];
+#pragma unroll
+    for (int t = kMlpIn; t < 32; ++t) xw[lane][t] = 0.0f;
+    wave_lds_fence();
+#pragma unroll 1
+    for (int tile = 0; tile < 4; ++tile) {
+      const int r0 = 16 * tile;
+      const bf16x8 xa = pack8(&xw[r0 + ra][kg], 1);
+      f32x4 z1[4];
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) z1[nb] = mfma(xa, w1b[nb], f32x4{0.f, 0.f, 0.f, 0.f});
+      wave_lds_fence();
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) aw[rd + r][16 * nb + ra] = act_tanh(z1[nb][r] + b1c[nb]);
+      wave_lds_fence();
+      bf16x8 aa[2];
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) aa[ks] = pack8(&aw[ra][32 * ks + kg], 1);
+      f32x4 z2[4];
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) {
+        z2[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) z2[nb] = mfma(aa[ks], w2b[nb][ks], z2[nb]);
+      }
+      wave_lds_fence();
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) aw[rd + r][16 * nb + ra] = act_tanh(z2[nb][r] + b2c[nb]);
+      wave_lds_fence();
+      f32x4 y = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) y = mfma(pack8(&aw[ra][32 * ks + kg], 1), w3b[ks], y);
+      if (ra == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long long qq = ch * RB + r0 + rd + r;
+          if (qq < N) {
+            const int i = (int)(qq % g.nx), j = (int)((qq / g.nx) % g.ny), k = (int)(qq / ((long long)g.nx * g.ny));
+            const long long c = g.idx(i, j, k);
+            h_out[c] = F.hn[c] + (double)((y[r] + b3) * ysd + ymu);
+          }
         }
       }
+    }
   }
 }
 
@@ -431,7 +521,7 @@ constexpr int kStatBlocks = 256;
 
 }  // namespace
 
-Mlp::Mlp(hipStream_t s) : s_(s) {
+Mlp::Mlp(hipStream_t s, Profiler* prof) : s_(s), prof_(prof) {
   FREHG_HIP(hipMalloc(&params_, kMlpParams * sizeof(float)));
   FREHG_HIP(hipMalloc(&m_, kMlpParams * sizeof(float)));
   FREHG_HIP(hipMalloc(&v_, kMlpParams * sizeof(float)));
@@ -568,8 +658,12 @@ void Mlp::predict_guess(ReState& re, double* h_out, const double* depth) {
   if (!ready_) throw ArgError("learned guess requested but the model is not trained or loaded");
   const FeatSrc F = feat_src(re, depth);
   const long long N = (long long)F.g.nx * F.g.ny * F.g.nz;
-  const unsigned nb = (unsigned)std::min<long long>(2048, (N + RB - 1) / RB);
-  hipLaunchKernelGGL(k_mlp_guess, dim3(nb), dim3(MT), 0, s_, (const float*)params_, (const float*)norm_, F, h_out);
+  const long long nchunks = (N + RB - 1) / RB;
+  const unsigned nb = (unsigned)std::min<long long>(4096, (nchunks + GW - 1) / GW);
+  hipEvent_t ev0 = prof_->start(FREHG_KT_GW_GUESS);
+  hipEvent_t ev1 = prof_->stop();
+  hipExtLaunchKernelGGL(k_mlp_guess, dim3(nb), dim3(64 * GW), 0, s_, ev0, ev1, 0, (const float*)params_,
+                        (const float*)norm_, F, h_out);
   FREHG_LAUNCH_CHECK();
 }
 

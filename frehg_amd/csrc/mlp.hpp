@@ -15,6 +15,7 @@
 #include <cstdint>
 
 #include "common.hpp"
+#include "profile.hpp"
 
 namespace frehg {
 
@@ -35,7 +36,7 @@ constexpr int kMlpNorm = 2 * kMlpRow;   // mean[12], sd[12]
 
 class Mlp {
  public:
-  explicit Mlp(hipStream_t s);
+  Mlp(hipStream_t s, Profiler* prof);
   ~Mlp();
   Mlp(const Mlp&) = delete;
   Mlp& operator=(const Mlp&) = delete;
@@ -73,6 +74,7 @@ class Mlp {
   double run_grad(const float* rows, long long nrows, int batch, uint64_t seed, int sequential);
 
   hipStream_t s_;
+  Profiler* prof_;
   bool ready_ = false;
   int stride_ = 0;
   long long cap_ = 0, count_ = 0, log_steps_ = 0;

@@ -12,7 +12,7 @@
 
 namespace frehg {
 
-constexpr int kKernelTypes = 8;  // FREHG_KT_COUNT
+constexpr int kKernelTypes = 9;  // FREHG_KT_COUNT
 
 class Profiler {
  public:

@@ -237,7 +237,8 @@ enum {
   FREHG_KT_SW_SPMV = 5,     /* surface Krylov SpMV+dot launches (5-point) */
   FREHG_KT_SW_UPDATE = 6,   /* surface Krylov vector update + reductions */
   FREHG_KT_GW_REFINE = 7,   /* groundwater FP64 residual sweep of iterative refinement */
-  FREHG_KT_COUNT = 8
+  FREHG_KT_GW_GUESS = 8,    /* learned-guess inference (features + MLP forward) */
+  FREHG_KT_COUNT = 9
 };
 int frehg_profile_enable(frehg_ctx_t ctx, int enable);
 /* Total milliseconds and launch count recorded for kernel `kt` since the last reset. */
