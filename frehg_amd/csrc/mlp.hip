@@ -353,42 +353,53 @@ __global__ __launch_bounds__(MT) void k_mlp_predict_rows(const float* P, const f
   }
 }
 
-// Learned guess on every interior cell: h_out = h^n + increment. Each wave works alone on
-// chunks of 64 cells (no block barriers): every lane computes one cell's features into the
-// wave's LDS rows, then four 16-cell tiles go through the three layers with the weight operands
-// held in registers for the whole kernel.
-constexpr int GW = 4;  // waves per block of the guess kernel
-struct GuessLds {
-  float x[GW][RB][XP];
-  float act[GW][16][HP];
-};
+// Learned guess on every interior cell: h_out = h^n + increment.
+//
+// Inference runs the layers transposed (Z^T = W X^T: neurons are MFMA rows, cells are columns)
+// so that each layer's output registers already hold the next layer's B operand, with no LDS
+// round trip between layers. After layer 1, lane l (g = l/16, c = l%16) holds neurons
+// 16 nb + 4 g + r (nb, r = 0..3) of cell c. The next layer's B operand asks lane l for reduction
+// indices kappa = 32 ks + 8 g + t (ks = 0..1, t = 0..7); the bijection
+//   pi(32 ks + 8 g + t) = 16 (2 ks + t/4) + 4 g + t%4
+// maps every kappa a lane needs to a neuron the lane holds, so the weight operands are stored
+// with their columns permuted by pi (a relabelling of the hidden units, exact).
+// Each wave works alone on chunks of 64 consecutive cells of one grid row; every lane computes
+// one cell's features into the wave's LDS rows, which are the only LDS traffic.
+constexpr int GW = 4;   // waves per block of the guess kernel
+constexpr int FP = 16;  // LDS pitch of a feature row (11 used, zero padded)
+
+__device__ __forceinline__ int hidden_perm(int kappa) {
+  const int ks = kappa >> 5, g = (kappa >> 3) & 3, t = kappa & 7;
+  return 16 * (2 * ks + (t >> 2)) + 4 * g + (t & 3);
+}
 
 __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* P, const float* norm, FeatSrc F, double* h_out) {
-  __shared__ GuessLds L;
+  __shared__ __attribute__((aligned(16))) float xs[GW][RB][FP];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int ra = lane & 15, kg = 8 * (lane >> 4), rd = 4 * (lane >> 4);
-  // weight operands: B[k][n] = W[n][k] for the lane's (k, n) slots
-  bf16x8 w1b[4], w2b[4][2], w3b[2];
-  float b1c[4], b2c[4];
+  const int ra = lane & 15, g = lane >> 4, kg = 8 * g;
+  // A operands (rows = neurons): W1[16 nb + ra][kg + t]; W2[16 nb + ra][pi(32 ks + kg + t)];
+  // w3[pi(32 ks + kg + t)] in row 0 only. Biases of the neurons this lane's outputs hold.
+  bf16x8 a1w[4], a2w[4][2], a3w[2];
+  float b1r[4][4], b2r[4][4];
 #pragma unroll
   for (int nb = 0; nb < 4; ++nb) {
     const int n = 16 * nb + ra;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int f = kg + t;
-      w1b[nb][t] = (__bf16)(f < kMlpIn ? P[kOffW1 + n * kMlpIn + f] : 0.0f);
-    }
+    for (int t = 0; t < 8; ++t) a1w[nb][t] = (__bf16)(kg + t < kMlpIn ? P[kOffW1 + n * kMlpIn + kg + t] : 0.0f);
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-      for (int t = 0; t < 8; ++t) w2b[nb][ks][t] = (__bf16)P[kOffW2 + n * kMlpH + 32 * ks + kg + t];
-    b1c[nb] = P[kOffB1 + n];
-    b2c[nb] = P[kOffB2 + n];
+      for (int t = 0; t < 8; ++t) a2w[nb][ks][t] = (__bf16)P[kOffW2 + n * kMlpH + hidden_perm(32 * ks + kg + t)];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      b1r[nb][r] = P[kOffB1 + 16 * nb + 4 * g + r];
+      b2r[nb][r] = P[kOffB2 + 16 * nb + 4 * g + r];
+    }
   }
 #pragma unroll
   for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-    for (int t = 0; t < 8; ++t) w3b[ks][t] = (__bf16)(ra == 0 ? P[kOffW3 + 32 * ks + kg + t] : 0.0f);
+    for (int t = 0; t < 8; ++t) a3w[ks][t] = (__bf16)(ra == 0 ? P[kOffW3 + hidden_perm(32 * ks + kg + t)] : 0.0f);
   const float b3 = P[kOffB3];
   float nmu[kMlpIn], nisd[kMlpIn];
 #pragma unroll
@@ -397,18 +408,18 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* P, const flo
     nisd[t] = 1.0f / norm[kMlpRow + t];
   }
   const float ymu = norm[kMlpIn], ysd = norm[kMlpRow + kMlpIn];
-  const Geo& g = F.g;
-  const long long N = (long long)g.nx * g.ny * g.nz;
-  const long long nchunks = (N + RB - 1) / RB;
-  float(*xw)[XP] = L.x[w];
-  float(*aw)[HP] = L.act[w];
+  const Geo& g3 = F.g;
+  const int iblocks = (g3.nx + RB - 1) / RB;
+  const long long nchunks = (long long)iblocks * g3.ny * g3.nz;
+  float(*xw)[FP] = xs[w];
   for (long long ch = (long long)blockIdx.x * GW + w; ch < nchunks; ch += (long long)gridDim.x * GW) {
-    const long long q = ch * RB + lane;
+    const int row = (int)(ch / iblocks), ib = (int)(ch - (long long)row * iblocks);
+    const int j = row % g3.ny, k = row / g3.ny;
+    const int i = ib * RB + lane;
     float f[kMlpIn];
 #pragma unroll
     for (int t = 0; t < kMlpIn; ++t) f[t] = 0.0f;
-    if (q < N) {
-      const int i = (int)(q % g.nx), j = (int)((q / g.nx) % g.ny), k = (int)(q / ((long long)g.nx * g.ny));
+    if (i < g3.nx) {
       cell_features(F, i, j, k, f);
 #pragma unroll
       for (int t = 0; t < kMlpIn; ++t) f[t] = (f[t] - nmu[t]) * nisd[t];
@@ -417,49 +428,52 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* P, const flo
 #pragma unroll
     for (int t = 0; t < kMlpIn; ++t) xw[lane][t] = f[t];
 #pragma unroll
-    for (int t = kMlpIn; t < 32; ++t) xw[lane][t] = 0.0f;
+    for (int t = kMlpIn; t < FP; ++t) xw[lane][t] = 0.0f;
     wave_lds_fence();
 #pragma unroll 1
     for (int tile = 0; tile < 4; ++tile) {
-      const int r0 = 16 * tile;
-      const bf16x8 xa = pack8(&xw[r0 + ra][kg], 1);
-      f32x4 z1[4];
+      // layer 1: B = X^T, lane supplies features kg..kg+7 of cell 16 tile + ra (zero for kg >= 16)
+      bf16x8 xb;
+      if (g < 2) {
+        xb = pack8(&xw[16 * tile + ra][kg], 1);
+      } else {
 #pragma unroll
-      for (int nb = 0; nb < 4; ++nb) z1[nb] = mfma(xa, w1b[nb], f32x4{0.f, 0.f, 0.f, 0.f});
-      wave_lds_fence();
-#pragma unroll
-      for (int nb = 0; nb < 4; ++nb)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) aw[rd + r][16 * nb + ra] = act_tanh(z1[nb][r] + b1c[nb]);
-      wave_lds_fence();
-      bf16x8 aa[2];
-#pragma unroll
-      for (int ks = 0; ks < 2; ++ks) aa[ks] = pack8(&aw[ra][32 * ks + kg], 1);
-      f32x4 z2[4];
+        for (int t = 0; t < 8; ++t) xb[t] = (__bf16)0.0f;
+      }
+      float a[4][4];
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) {
-        z2[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const f32x4 z = mfma(a1w[nb], xb, f32x4{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) z2[nb] = mfma(aa[ks], w2b[nb][ks], z2[nb]);
+        for (int r = 0; r < 4; ++r) a[nb][r] = act_tanh(z[r] + b1r[nb][r]);
       }
-      wave_lds_fence();
+      // layer 2: B[kappa][cell] = a1[pi(kappa)] from this lane's registers
+      bf16x8 hb[2];
 #pragma unroll
-      for (int nb = 0; nb < 4; ++nb)
+      for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) aw[rd + r][16 * nb + ra] = act_tanh(z2[nb][r] + b2c[nb]);
-      wave_lds_fence();
+        for (int t = 0; t < 8; ++t) hb[ks][t] = (__bf16)a[2 * ks + (t >> 2)][t & 3];
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) {
+        f32x4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) z = mfma(a2w[nb][ks], hb[ks], z);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[nb][r] = act_tanh(z[r] + b2r[nb][r]);
+      }
+      // layer 3: row 0 of w3[pi] times A2^T
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) hb[ks][t] = (__bf16)a[2 * ks + (t >> 2)][t & 3];
       f32x4 y = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) y = mfma(pack8(&aw[ra][32 * ks + kg], 1), w3b[ks], y);
-      if (ra == 0) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const long long qq = ch * RB + r0 + rd + r;
-          if (qq < N) {
-            const int i = (int)(qq % g.nx), j = (int)((qq / g.nx) % g.ny), k = (int)(qq / ((long long)g.nx * g.ny));
-            const long long c = g.idx(i, j, k);
-            h_out[c] = F.hn[c] + (double)((y[r] + b3) * ysd + ymu);
-          }
+      for (int ks = 0; ks < 2; ++ks) y = mfma(a3w[ks], hb[ks], y);
+      if (lane < 16) {  // D row 0 (g = 0, r = 0) holds y of cell ra
+        const int ii = ib * RB + 16 * tile + ra;
+        if (ii < g3.nx) {
+          const long long c = g3.idx(ii, j, k);
+          h_out[c] = F.hn[c] + (double)((y[0] + b3) * ysd + ymu);
         }
       }
     }
@@ -654,12 +668,15 @@ double Mlp::train(int steps, int batch, double lr, uint64_t seed, double* first)
   return loss;
 }
 
-void Mlp::predict_guess(ReState& re, double* h_out, const double* depth) {
+void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const double* hn, const double* wcn) {
   if (!ready_) throw ArgError("learned guess requested but the model is not trained or loaded");
-  const FeatSrc F = feat_src(re, depth);
+  FeatSrc F = feat_src(re, depth);
+  if (hn) F.hn = hn;
+  if (wcn) F.wcn = wcn;
   const long long N = (long long)F.g.nx * F.g.ny * F.g.nz;
-  const long long nchunks = (N + RB - 1) / RB;
+  const long long nchunks = (long long)((F.g.nx + RB - 1) / RB) * F.g.ny * F.g.nz;
   const unsigned nb = (unsigned)std::min<long long>(4096, (nchunks + GW - 1) / GW);
+  (void)N;
   hipEvent_t ev0 = prof_->start(FREHG_KT_GW_GUESS);
   hipEvent_t ev1 = prof_->stop();
   hipExtLaunchKernelGGL(k_mlp_guess, dim3(nb), dim3(64 * GW), 0, s_, ev0, ev1, 0, (const float*)params_,

@@ -58,8 +58,9 @@ class Mlp {
   // Returns the mean standardised squared error of the last minibatch (first one in *first).
   double train(int steps, int batch, double lr, uint64_t seed, double* first);
 
-  // h_out (interior) = h^n + predicted increment.
-  void predict_guess(ReState& re, double* h_out, const double* depth);
+  // h_out (interior) = h^n + predicted increment; hn / wcn default to the solver's h^n / theta^n.
+  void predict_guess(ReState& re, double* h_out, const double* depth, const double* hn = nullptr,
+                     const double* wcn = nullptr);
 
   void get_params(float* params, float* norm);
   void set_params(const float* params, const float* norm);

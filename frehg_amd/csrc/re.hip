@@ -636,6 +636,34 @@ void ReState::set_initial(const double* h_int, const double* wc_int) {
   dtn_ = p_.dt;
 }
 
+void ReState::learned_increment(double* dst) {
+  if (!mlp_ || !mlp_->ready()) throw ArgError("frehg_mlp_increment: no trained model");
+  const ReK P = make_rek(g_, soil_view(), p_, qtop_);
+  const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4), g_.nz);
+  hipLaunchKernelGGL(k_re_pre, grid, dim3(NT), 0, s_, P, (const double*)h_, kr_, ch_);
+  FREHG_LAUNCH_CHECK();
+  double* tmp = nullptr;
+  FREHG_HIP(hipMalloc(&tmp, (size_t)n_ * sizeof(double)));
+  try {
+    FREHG_HIP(hipMemsetAsync(tmp, 0, (size_t)n_ * sizeof(double), s_));
+    mlp_->predict_guess(*this, tmp, depth_, h_, wc_);
+    std::vector<double> pad((size_t)n_), hh((size_t)n_);
+    FREHG_HIP(hipMemcpyAsync(pad.data(), tmp, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+    FREHG_HIP(hipMemcpyAsync(hh.data(), h_, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+    FREHG_HIP(hipStreamSynchronize(s_));
+    for (int k = 0; k < g_.nz; ++k)
+      for (int j = 0; j < g_.ny; ++j)
+        for (int i = 0; i < g_.nx; ++i) {
+          const size_t c = (size_t)g_.idx(i, j, k);
+          dst[(size_t)i + (size_t)j * g_.nx + (size_t)k * g_.nx * g_.ny] = pad[c] - hh[c];
+        }
+  } catch (...) {
+    (void)hipFree(tmp);
+    throw;
+  }
+  FREHG_HIP(hipFree(tmp));
+}
+
 void ReState::step(frehg_re_stats* st) {
   // Reject a request that cannot be served before any state changes.
   if (p_.initial_guess == FREHG_GUESS_LEARNED && (!mlp_ || !mlp_->ready()))

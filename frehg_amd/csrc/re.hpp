@@ -46,6 +46,9 @@ class ReState {
   double* d_ch() { return ch_; }
   SoilView soil_view() const { return SoilView{classes_, cls_, ks_}; }
   Krylov& krylov() { return *kry_; }
+  // Learned increment h^{n+1} - h^n the model predicts for a step taken from the current state
+  // (features from h, K/C at h, theta), interior cells in i + j nx + k nx ny order.
+  void learned_increment(double* dst);
   // Learned-guess model and the surface depth it reads (either may be null).
   void set_mlp(Mlp* m, const double* surface_depth) {
     mlp_ = m;
@@ -54,9 +57,6 @@ class ReState {
 
  private:
   double* field_ptr(const std::string& name);
-  void launch_pre(const double* hsrc);
-  void launch_assemble(bool mixed);
-  void apply_guess(bool* guarded);
 
   Geo g_;
   frehg_re_params p_;

@@ -79,6 +79,7 @@ def load():
         "frehg_mlp_set_params": (C.c_int, [ctx, P(C.c_float), P(C.c_float)]),
         "frehg_mlp_loss_grad": (C.c_int, [ctx, P(C.c_float), P(C.c_float), C.c_int, P(C.c_float), dp]),
         "frehg_mlp_predict": (C.c_int, [ctx, P(C.c_float), C.c_int, P(C.c_float)]),
+        "frehg_mlp_increment": (C.c_int, [ctx, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -290,6 +291,11 @@ class Context:
         x = np.ascontiguousarray(x, dtype=np.float32)
         out = np.zeros(len(x), np.float32)
         self.check(self.lib.frehg_mlp_predict(self.h, _fp(x), len(x), _fp(out)))
+        return out
+
+    def mlp_increment(self):
+        out = np.zeros(int(np.prod(self.re_dims)))
+        self.check(self.lib.frehg_mlp_increment(self.h, out.ctypes.data_as(dp)))
         return out
 
     def spmv_shape(self, precision):

@@ -225,6 +225,9 @@ int frehg_mlp_set_params(frehg_ctx_t ctx, const float* params, const float* norm
 int frehg_mlp_loss_grad(frehg_ctx_t ctx, const float* x, const float* y, int n, float* grad, double* loss);
 /* Predicted raw increments for x[n][11]. */
 int frehg_mlp_predict(frehg_ctx_t ctx, const float* x, int n, float* out);
+/* The increment the learned guess would add for a step taken from the current Richards state
+ * (features from h, K(h), C(h), theta), nx*ny*nz values, i + j*nx + k*nx*ny. */
+int frehg_mlp_increment(frehg_ctx_t ctx, double* dst);
 
 /* ---- measurement --------------------------------------------------------------------------- */
 /* Kernel-duration accounting with HIP events on the context's stream. kernel ids below. */

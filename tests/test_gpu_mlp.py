@@ -107,3 +107,36 @@ def test_untrained_model_rejected_or_harmless():
     ctx.mlp_set_params(params, norm)
     st = ctx.re_step()
     assert st.solve.converged
+
+
+def test_guess_kernel_matches_oracle_on_solver_state():
+    """The inference kernel's features (h, six neighbour differences, ln K, C, theta, depth) and its
+    transposed-layer network against host-side features fed to the CPU reference."""
+    nx, ny, nz = 70, 9, 6
+    ctx = gpu.Context(0)
+    p, soil, ks, h0 = infiltration(nx, ny, nz, 31, abi.GUESS_ZERO)
+    ctx.re_init(nx, ny, nz, p)
+    ctx.re_set_soil([soil], None, True, ks)
+    ctx.re_set_initial(h=h0)
+    ctx.re_step()
+    params, norm, _, _ = random_case(5)
+    norm[:11] = [-1.5, 0, 0, 0, 0, 0, 0, -12.0, 0.3, 0.25, 0.0]
+    norm[12:23] = [0.5, 0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 1.5, 0.3, 0.05, 1.0]
+    ctx.mlp_set_params(params, norm)
+    inc = ctx.mlp_increment()
+    h = ctx.re_get("h").reshape(nz + 2, ny + 2, nx + 2)
+    wc = ctx.re_get("wc").reshape(nz + 2, ny + 2, nx + 2)
+    hc = h[1:-1, 1:-1, 1:-1]
+    vg = ol.vg_eval(soil, h=hc.ravel())
+    X = np.zeros((nx * ny * nz, 11))
+    X[:, 0] = hc.ravel()
+    nbrs = [h[1:-1, 1:-1, :-2], h[1:-1, 1:-1, 2:], h[1:-1, :-2, 1:-1], h[1:-1, 2:, 1:-1], h[:-2, 1:-1, 1:-1],
+            h[2:, 1:-1, 1:-1]]
+    for q, nb in enumerate(nbrs):
+        X[:, 1 + q] = (nb - hc).ravel()
+    X[:, 7] = np.log(np.maximum(ks * vg["K"] / soil.Ks_z, 1e-30))
+    X[:, 8] = vg["C"]
+    X[:, 9] = wc[1:-1, 1:-1, 1:-1].ravel()
+    ref = ol.mlp_predict(params, norm, X.astype(np.float32), bf16=True)
+    scale = np.abs(ref).max()
+    assert np.abs(inc - ref).max() <= 3e-3 * scale, np.abs(inc - ref).max() / scale
