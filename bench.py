@@ -63,12 +63,14 @@ def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None):
     ctx.profile_enable(True)
     ctx.profile_reset()
     iters = refines = fallbacks = 0
+    mass_err = 0.0
     t0 = time.perf_counter()
     for s in range(steps):
         st = ctx.coupled_step(case.cp, case.dt, case.rain, 0.0, case.t_end)
         iters += st.gw.solve.iterations
         refines += st.gw.solve.refinements
         fallbacks += st.gw.guess_fallback
+        mass_err = max(mass_err, abs(st.gw.mass_residual) / st.gw.mass_before)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     kt = {k: ctx.profile_read(v) for k, v in zip(KERNEL_TYPES, (abi.KT_GW_SPMV, abi.KT_GW_SPMV32, abi.KT_GW_UPDATE,
@@ -80,7 +82,7 @@ def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None):
     sw_vol, gw_vol = ctx.water_volume()
     ctx.close()
     return {"elapsed": elapsed, "iters": iters, "refines": refines, "fallbacks": fallbacks, "kt": kt, "h": h,
-            "grabbed": grabbed, "volume": sw_vol + gw_vol, "spmv_shape": shape}
+            "grabbed": grabbed, "volume": sw_vol + gw_vol, "spmv_shape": shape, "mass_err": mass_err}
 
 
 def train_model(abi, gpu, workloads, args, seeds, log_steps, stride, train_steps, batch):
@@ -250,6 +252,8 @@ def main():
         "fast_inner_iters_per_step": fast["iters"] / args.steps,
         "fast_refinements_per_step": fast["refines"] / args.steps,
         "fast_guess_fallbacks": fast["fallbacks"],
+        "max_rel_mass_balance_error": {"learned": fast["mass_err"], "plain_fp64": plain["mass_err"],
+                                       "previous": prev["mass_err"]},
         "previous_guess_mixed": {"value": round(case.cells * args.steps / prev["elapsed"], 1),
                                  "ms_per_step": round(prev["elapsed"] / args.steps * 1e3, 4),
                                  "inner_iters_per_step": prev["iters"] / args.steps,

@@ -8,7 +8,7 @@ reference's parameter structs: SoilParams (include/frehg2/re/SoilParams.hpp:13-2
 """
 import ctypes as C
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK = 0
 INFO_GUESS_FALLBACK = 1
@@ -92,7 +92,9 @@ class SolveStats(C.Structure):
 class ReStats(C.Structure):
     _fields_ = [("solve", SolveStats), ("dt_used", C.c_double), ("dt_next", C.c_double),
                 ("picard_iterations", C.c_int32), ("guess_fallback", C.c_int32),
-                ("max_dh", C.c_double)]
+                ("max_dh", C.c_double), ("mass_before", C.c_double), ("mass_after", C.c_double),
+                ("mass_boundary_in", C.c_double), ("mass_storage_ss", C.c_double), ("mass_adjust", C.c_double),
+                ("mass_residual", C.c_double)]
 
 
 class CoupledStats(C.Structure):

@@ -272,36 +272,18 @@ __global__ __launch_bounds__(NT) void k_re_flux(ReK P, const double* h, const do
   }
 }
 
-// updateWaterContent (ReSolver.cpp:762-786) or theta(h) without the corrector (:953-961);
-// the diagnostics wch = theta(h), hwc = h(theta) of the second updateDiagnostics / the
-// reallocation (:825-826); saturation flags for the reallocation fixed point.
-__global__ __launch_bounds__(NT) void k_re_correct(ReK P, const double* h, const double* hn,
-                                                    const double* wcn, const double* qx, const double* qy,
-                                                    const double* qz, const double* wch, double* wc, double* hwc,
-                                                    unsigned char* sat0, unsigned char* cand) {
-  const Cell t = cell_of(P.g);
-  if (!t.valid) return;
+// Stores the corrected theta of cell t, h(theta) and the saturation flags; ghost neighbours keep
+// the moisture ghosts of the previous step (ReSolver.cpp:855-878).
+__device__ inline void correct_cell(const ReK& P, const Cell& t, double w, double th, const double* wcn, double* wc,
+                                    double* hwc, unsigned char* sat0, unsigned char* cand) {
   const Geo& g = P.g;
   const long long c = t.c;
   const SoilClass& s = P.soil.at(c);
-  double w;
-  const double th = wch[c];
-  if (P.use_corrector) {
-    const double V = P.dx * P.dy * P.dz;
-    const double coeff = V * (1.0 + s.Ss * (h[c] - hn[c]) / s.theta_s);
-    const double dqx = P.dt * (qx[c] - qx[c - 1]);
-    const double dqy = P.dt * (qy[c] - qy[c - g.sy]);
-    const double dqz = P.dt * (qz[c] - qz[c - g.sz]);
-    w = (wcn[c] * V + dqx + dqy + dqz) / coeff;
-  } else {
-    w = th;
-  }
   wc[c] = w;
   hwc[c] = vg_head(s, w);
   const unsigned char s0 = (w >= s.theta_s) ? 1 : 0;
   sat0[c] = s0;
   cand[c] = (!s0 && th >= s.theta_s) ? 1 : 0;
-  // Ghost neighbours keep the moisture ghosts of the previous step (ReSolver.cpp:855-878).
   const long long gh[6] = {t.i == 0 ? c - 1 : -1,         t.i == g.nx - 1 ? c + 1 : -1,
                            t.j == 0 ? c - g.sy : -1,      t.j == g.ny - 1 ? c + g.sy : -1,
                            t.k == 0 ? c - g.sz : -1,      t.k == g.nz - 1 ? c + g.sz : -1};
@@ -311,6 +293,52 @@ __global__ __launch_bounds__(NT) void k_re_correct(ReK P, const double* h, const
       sat0[gh[q]] = (wcn[gh[q]] >= s.theta_s) ? 1 : 0;
       cand[gh[q]] = 0;
     }
+}
+
+// updateWaterContent (ReSolver.cpp:762-786) or theta(h) without the corrector (:953-961);
+// the diagnostics wch = theta(h), hwc = h(theta) of the second updateDiagnostics / the
+// reallocation (:825-826); saturation flags for the reallocation fixed point.
+// Ledger partials of the corrector (sum theta^n V, specific-storage term, boundary inflow) are
+// written per block to part[0..2][nblocks].
+__global__ __launch_bounds__(NT) void k_re_correct(ReK P, const double* h, const double* hn,
+                                                    const double* wcn, const double* qx, const double* qy,
+                                                    const double* qz, const double* wch, double* wc, double* hwc,
+                                                    unsigned char* sat0, unsigned char* cand, double* part,
+                                                    unsigned int nblocks) {
+  __shared__ double lds[NT / kWave];
+  const Cell t = cell_of(P.g);
+  double l_before = 0.0, l_ss = 0.0, l_bnd = 0.0;
+  if (t.valid) {
+    const Geo& g = P.g;
+    const long long c = t.c;
+    const SoilClass& s = P.soil.at(c);
+    const double V = P.dx * P.dy * P.dz;
+    double w;
+    const double th = wch[c];
+    if (P.use_corrector) {
+      const double coeff = V * (1.0 + s.Ss * (h[c] - hn[c]) / s.theta_s);
+      const double dqx = P.dt * (qx[c] - qx[c - 1]);
+      const double dqy = P.dt * (qy[c] - qy[c - g.sy]);
+      const double dqz = P.dt * (qz[c] - qz[c - g.sz]);
+      w = (wcn[c] * V + dqx + dqy + dqz) / coeff;
+      l_ss = -w * V * s.Ss * (h[c] - hn[c]) / s.theta_s;
+    } else {
+      w = th;
+    }
+    l_before = wcn[c] * V;
+    if (t.k == 0) l_bnd -= P.dt * qz[c - g.sz];
+    if (t.k == g.nz - 1) l_bnd += P.dt * qz[c];
+    correct_cell(P, t, w, th, wcn, wc, hwc, sat0, cand);
+  }
+  const double a = block_sum<NT>(l_before, lds);
+  const double b = block_sum<NT>(l_ss, lds);
+  const double d = block_sum<NT>(l_bnd, lds);
+  const unsigned int bid = xcd_tile().linear;
+  if (threadIdx.x == 0) {
+    part[bid] = a;
+    part[nblocks + bid] = b;
+    part[2 * nblocks + bid] = d;
+  }
 }
 
 // adjacentToSat (ReSolver.cpp:802-817) with `prev` holding the post-visit saturation state of
@@ -344,33 +372,58 @@ __global__ __launch_bounds__(NT) void k_re_realloc_iter(ReK P, const double* Kx,
 }
 
 // reallocateWaterContent branches with the converged state + clampWaterContent +
-// enforceMoistureBc (ReSolver.cpp:819-878).
+// enforceMoistureBc (ReSolver.cpp:819-878). Ledger partials: sum theta^{n+1} V and
+// sum (theta^{n+1} - theta_c) V to part[0..1][nblocks]; max |h^{n+1} - h^n| to mx[nblocks].
 __global__ __launch_bounds__(NT) void k_re_finalize(ReK P, const double* Kx, const double* Ky,
                                                      const unsigned char* sat0, const unsigned char* fin,
-                                                     const double* wch, const double* hwc, double* wc,
-                                                     double* h) {
+                                                     const double* wch, const double* hwc, const double* hn,
+                                                     double* wc, double* h, double* part, double* mx,
+                                                     unsigned int nblocks) {
+  __shared__ double lds[NT / kWave];
   const Cell t = cell_of(P.g);
-  if (!t.valid) return;
-  const Geo& g = P.g;
-  const long long c = t.c;
-  const SoilClass& s = P.soil.at(c);
-  double w = wc[c];
-  if (sat0[c]) {
-    w = s.theta_s;
-  } else if (!adjacent_sat(P, t, Kx, Ky, sat0, fin)) {
-    if (w < 0.9999 * s.theta_s) h[c] = hwc[c];
-  } else {
-    w = wch[c];
+  double l_after = 0.0, l_adj = 0.0, dhmax = 0.0;
+  if (t.valid) {
+    const Geo& g = P.g;
+    const long long c = t.c;
+    const SoilClass& s = P.soil.at(c);
+    const double wcorr = wc[c];
+    double w = wcorr;
+    if (sat0[c]) {
+      w = s.theta_s;
+    } else if (!adjacent_sat(P, t, Kx, Ky, sat0, fin)) {
+      if (w < 0.9999 * s.theta_s) h[c] = hwc[c];
+    } else {
+      w = wch[c];
+    }
+    if (w > s.theta_s) w = s.theta_s;
+    else if (w < s.theta_r) w = s.theta_r;
+    wc[c] = w;
+    if (t.i == 0) wc[c - 1] = w;
+    if (t.i == g.nx - 1) wc[c + 1] = w;
+    if (t.j == 0) wc[c - g.sy] = w;
+    if (t.j == g.ny - 1) wc[c + g.sy] = w;
+    if (t.k == g.nz - 1) wc[c + g.sz] = w;
+    if (t.k == 0) wc[c - g.sz] = w;
+    const double V = P.dx * P.dy * P.dz;
+    l_after = w * V;
+    l_adj = (w - wcorr) * V;
+    dhmax = fabs(h[c] - hn[c]);
   }
-  if (w > s.theta_s) w = s.theta_s;
-  else if (w < s.theta_r) w = s.theta_r;
-  wc[c] = w;
-  if (t.i == 0) wc[c - 1] = w;
-  if (t.i == g.nx - 1) wc[c + 1] = w;
-  if (t.j == 0) wc[c - g.sy] = w;
-  if (t.j == g.ny - 1) wc[c + g.sy] = w;
-  if (t.k == g.nz - 1) wc[c + g.sz] = w;
-  if (t.k == 0) wc[c - g.sz] = w;
+  const double a = block_sum<NT>(l_after, lds);
+  const double b = block_sum<NT>(l_adj, lds);
+  dhmax = wave_max(dhmax);
+  __shared__ double lmax[NT / kWave];
+  if ((threadIdx.x % kWave) == 0) lmax[threadIdx.x / kWave] = dhmax;
+  __syncthreads();
+  const unsigned int bid = xcd_tile().linear;
+  if (threadIdx.x == 0) {
+    double m = lmax[0];
+    for (int q = 1; q < NT / kWave; ++q) m = fmax(m, lmax[q]);
+    part[3 * nblocks + bid] = a;
+    part[4 * nblocks + bid] = b;
+    mx[bid] = m;
+    mx[nblocks + bid] = 0.0;
+  }
 }
 
 // adaptiveTimeStep reductions (ReSolver.cpp:891-913): max flux imbalance and min Courant dt
@@ -460,8 +513,9 @@ ReState::ReState(int nx, int ny, int nz, const frehg_re_params& p, hipStream_t s
   FREHG_HIP(hipHostMalloc(&hflag_, 16, hipHostMallocDefault));
   const unsigned int nb = ceil_div(nx, 64) * ceil_div(ny, 4) * nz;
   FREHG_HIP(hipMalloc(&red_, 2 * (size_t)nb * sizeof(double)));
-  FREHG_HIP(hipMalloc(&dres_, 8 * sizeof(double)));
-  FREHG_HIP(hipHostMalloc(&hres_, 8 * sizeof(double), hipHostMallocDefault));
+  FREHG_HIP(hipMalloc(&led_, 7 * (size_t)nb * sizeof(double)));
+  FREHG_HIP(hipMalloc(&dres_, 16 * sizeof(double)));
+  FREHG_HIP(hipHostMalloc(&hres_, 16 * sizeof(double), hipHostMallocDefault));
   FREHG_HIP(hipMalloc(&dcount_, 16));
   FREHG_HIP(hipMemsetAsync(dcount_, 0, 16, s_));
   frehg_soil_params sp = p.soil;
@@ -476,7 +530,7 @@ ReState::~ReState() {
                   (void*)Kx_, (void*)Ky_, (void*)Kz_, (void*)qx_, (void*)qy_, (void*)qz_, (void*)kr_,
                   (void*)D_, (void*)ax_, (void*)ay_, (void*)az_, (void*)rhs_, (void*)D32_, (void*)ax32_,
                   (void*)ay32_, (void*)az32_, (void*)sat0_, (void*)cand_, (void*)satA_, (void*)satB_,
-                  (void*)dflag_, (void*)red_, (void*)dres_, (void*)dcount_, (void*)classes_, (void*)cls_,
+                  (void*)dflag_, (void*)red_, (void*)led_, (void*)dres_, (void*)dcount_, (void*)classes_, (void*)cls_,
                   (void*)ks_, (void*)qtop_})
     if (f) (void)hipFree(f);
   if (hflag_) (void)hipHostFree(hflag_);
@@ -675,6 +729,7 @@ void ReState::step(frehg_re_stats* st) {
   const ReK P = make_rek(g_, soil_view(), p_, qtop_);
   const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4), g_.nz);
   const dim3 blk(NT);
+  const unsigned int nbl = grid.x * grid.y * grid.z;
   // savePreviousStepFields (ReSolver.cpp:190-197): hn <- h, wcn <- wc. The arrays are swapped
   // instead of copied; the new h and wc buffers are fully rewritten below (corner ghosts, which
   // no kernel reads or writes, hold the same values in both buffers).
@@ -723,7 +778,7 @@ void ReState::step(frehg_re_stats* st) {
     FREHG_LAUNCH_CHECK();
     hipExtLaunchKernelGGL(k_re_correct, grid, blk, 0, s_, nullptr, ev1, 0, P, (const double*)h_, (const double*)hn_,
                           (const double*)wcn_, (const double*)qx_, (const double*)qy_, (const double*)qz_,
-                          (const double*)wch_, wc_, hwc_, sat0_, cand_);
+                          (const double*)wch_, wc_, hwc_, sat0_, cand_, led_, nbl);
     FREHG_LAUNCH_CHECK();
   }
   // Reallocation fixed point: iterate from sat0 until the saturation state stops changing.
@@ -740,7 +795,14 @@ void ReState::step(frehg_re_stats* st) {
     std::swap(prev, next);
     if (hflag_[0] == 0) break;
   }
-  hipLaunchKernelGGL(k_re_finalize, grid, blk, 0, s_, P, Kx_, Ky_, sat0_, prev, wch_, hwc_, wc_, h_);
+  hipLaunchKernelGGL(k_re_finalize, grid, blk, 0, s_, P, Kx_, Ky_, sat0_, prev, wch_, hwc_, (const double*)hn_, wc_,
+                     h_, led_, led_ + 5 * (size_t)nbl, nbl);
+  FREHG_LAUNCH_CHECK();
+  // ledger: [before, storage_ss, boundary_in, after, adjust] sums, then max |dh|
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kRedThreads), 0, s_, (const double*)led_, nbl, 5, dres_ + 2);
+  FREHG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_maxmin, dim3(1), dim3(kRedThreads), 0, s_, (const double*)(led_ + 5 * (size_t)nbl), nbl,
+                     dres_ + 7);
   FREHG_LAUNCH_CHECK();
   if (p_.adaptive_dt) {
     const unsigned int nb = grid.x * grid.y * grid.z;
@@ -760,6 +822,16 @@ void ReState::step(frehg_re_stats* st) {
     if (p_.dt < p_.dt_min) p_.dt = p_.dt_min;
     dtn_ = p_.dt;
   }
+  FREHG_HIP(hipMemcpyAsync(hres_ + 2, dres_ + 2, 6 * sizeof(double), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipStreamSynchronize(s_));
+  local.mass_before = hres_[2];
+  local.mass_storage_ss = hres_[3];
+  local.mass_boundary_in = hres_[4];
+  local.mass_after = hres_[5];
+  local.mass_adjust = hres_[6];
+  local.max_dh = hres_[7];
+  local.mass_residual =
+      local.mass_after - local.mass_before - local.mass_boundary_in - local.mass_storage_ss - local.mass_adjust;
   local.dt_used = dt_used_;
   local.dt_next = p_.dt;
   if (st) *st = local;

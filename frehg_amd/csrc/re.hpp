@@ -76,7 +76,8 @@ class ReState {
   unsigned char *sat0_ = nullptr, *cand_ = nullptr, *satA_ = nullptr, *satB_ = nullptr;
   int* dflag_ = nullptr;   // device: [0] changed flag
   int* hflag_ = nullptr;   // pinned host mirror
-  double* red_ = nullptr;  // reduction partials (adaptive dt / stats)
+  double* red_ = nullptr;  // reduction partials (adaptive dt)
+  double* led_ = nullptr;  // water-ledger partials: 5 sums + max|dh| (+ pad) per block
   double* dres_ = nullptr; // device results [dq_max, dt_Comin, max_dh]
   double* hres_ = nullptr;
   unsigned int* dcount_ = nullptr;

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define FREHG_ABI_VERSION 1
+#define FREHG_ABI_VERSION 2
 
 enum {
   FREHG_OK = 0,
@@ -113,6 +113,18 @@ typedef struct {
   int32_t picard_iterations;
   int32_t guess_fallback;
   double max_dh;           /* max |h^{n+1} - h^n| over interior cells */
+  /* FP64 water ledger of the step (m^3; sums over interior cells in a fixed order). With the
+   * corrector on, mass_after - mass_before = boundary_in + storage_ss + adjust up to round-off,
+   * and mass_residual is that difference. theta_c is the corrector's theta (ReSolver.cpp:762-786),
+   * before reallocation and clamp (:788-853). Lateral faces are closed, so only the top and bottom
+   * faces exchange water. Without the corrector theta_c = theta(h) and the residual is the PCA's
+   * own imbalance. */
+  double mass_before;      /* sum theta^n V (after any SW-GW exchange) */
+  double mass_after;       /* sum theta^{n+1} V */
+  double mass_boundary_in; /* dt * sum over columns of (q_bottom_face - q_top_face), + = into the soil */
+  double mass_storage_ss;  /* -sum theta_c V Ss (h - h^n) / theta_s: specific storage, outside theta V */
+  double mass_adjust;      /* sum (theta^{n+1} - theta_c) V: reallocation and clamp */
+  double mass_residual;    /* mass_after - mass_before - boundary_in - storage_ss - adjust */
 } frehg_re_stats;
 
 typedef struct {

@@ -225,10 +225,18 @@ struct ReParams {  // ReSolver.hpp:24-46 (defaults kept)
   SolverCfg solver{1e-10, 1e-50, 10000};  // Orchestrator.cpp:487-490 (GW rtol 1e-10)
 };
 
+// Water ledger of one step (m^3), as reported by frehg_re_stats: theta_c is the corrector's theta
+// before reallocation and clamp.
 struct ReStepStats {
   SolveStats solve;
   real dt_used = 0.0;
   real dt_next = 0.0;
+  real max_dh = 0.0;
+  real mass_before = 0.0;       // sum theta^n V
+  real mass_after = 0.0;        // sum theta^{n+1} V
+  real mass_boundary_in = 0.0;  // dt * sum(q_bottom_face - q_top_face)
+  real mass_storage_ss = 0.0;   // -sum theta_c V Ss (h - h^n) / theta_s
+  real mass_adjust = 0.0;       // sum (theta^{n+1} - theta_c) V
 };
 
 class ReSolver {

@@ -217,6 +217,13 @@ int orc_re_step(void* h, frehg_re_stats* st) {
       st->dt_used = s.dt_used;
       st->dt_next = s.dt_next;
       st->picard_iterations = 1;
+      st->max_dh = s.max_dh;
+      st->mass_before = s.mass_before;
+      st->mass_after = s.mass_after;
+      st->mass_boundary_in = s.mass_boundary_in;
+      st->mass_storage_ss = s.mass_storage_ss;
+      st->mass_adjust = s.mass_adjust;
+      st->mass_residual = s.mass_after - s.mass_before - s.mass_boundary_in - s.mass_storage_ss - s.mass_adjust;
     }
   });
 }

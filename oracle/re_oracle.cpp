@@ -538,15 +538,46 @@ void ReSolver::advanceStep(ReStepStats* st) {  // :931-969
       for (int j = 0; j < g.ny; ++j)
         for (int i = 0; i < g.nx; ++i) wc[g.idx(i, j, k)] = VG::theta(soilAt(i, j, k), h[g.idx(i, j, k)]);
   }
+  // Ledger terms around the corrector (not part of the reference; the checker of the GPU ledger).
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const real Vc = V();
+  real before = 0.0, ss_term = 0.0, bnd = 0.0;
+  std::vector<real> wcorr((size_t)nx * ny * nz);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const long long c = g.idx(i, j, k);
+        const Soil s = soilAt(i, j, k);
+        wcorr[(size_t)i + (size_t)j * nx + (size_t)k * nx * ny] = wc[c];
+        before += wcn[c] * Vc;
+        if (p_.use_corrector) ss_term -= wc[c] * Vc * s.Ss * (h[c] - hn[c]) / s.theta_s;
+        if (k == 0) bnd -= dt_used * qz[c - g.plane()];
+        if (k == nz - 1) bnd += dt_used * qz[c];
+      }
   updateDiagnostics();
   reallocateWaterContent();
   clampWaterContent();
   enforceMoistureBc();
+  real after = 0.0, adjust = 0.0, dhmax = 0.0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const long long c = g.idx(i, j, k);
+        after += wc[c] * Vc;
+        adjust += (wc[c] - wcorr[(size_t)i + (size_t)j * nx + (size_t)k * nx * ny]) * Vc;
+        dhmax = std::max(dhmax, std::fabs(h[c] - hn[c]));
+      }
   if (p_.adaptive_dt) adaptiveTimeStep();
   if (st) {
     st->solve = ss;
     st->dt_used = dt_used;
     st->dt_next = p_.dt;
+    st->max_dh = dhmax;
+    st->mass_before = before;
+    st->mass_after = after;
+    st->mass_boundary_in = bnd;
+    st->mass_storage_ss = ss_term;
+    st->mass_adjust = adjust;
   }
 }
 

@@ -172,3 +172,31 @@ def test_bicgstab_fp64_matches_cg_oracle():
         st = ctx.re_step()
         assert st.solve.converged
     assert np.abs(ctx.re_get("h") - orc.get("h")).max() <= 1e-8
+
+
+@pytest.mark.parametrize("precision", [abi.PREC_FP64, abi.PREC_MIXED])
+def test_water_ledger_matches_oracle_and_closes(precision):
+    """Every term of the GPU's FP64 water ledger against the oracle's, and closure to 1e-10 of the
+    stored water every step (reallocation active: saturating top, heterogeneous Ks)."""
+    nx, ny, nz = 12, 9, 16
+    say(f"RE water ledger, precision {precision}")
+    rng = np.random.default_rng(41)
+    soil = abi.soil_params(alpha=3.35, n=2.0, theta_s=0.368, theta_r=0.102, Ks_x=9.22e-5, Ks_y=9.22e-5,
+                           Ks_z=9.22e-5, Ss=1e-5)
+    ks = 9.22e-5 * np.exp(0.8 * rng.standard_normal(nx * ny * nz))
+    p = abi.re_params(soil=soil, solver=abi.solver_config(1e-12, precision=precision, max_refine=10),
+                      bc_type=(0, 0, 0, 0, 3, 1), dx=0.1, dy=0.1, dz=0.02, dt=2.0, adaptive_dt=0, use_full3d=1,
+                      htop=0.05)
+    orc, ctx = re_pair(nx, ny, nz, p, soil_map=([soil], None, True, ks), h0=np.full(nx * ny * nz, -1.0))
+    adj = 0.0
+    for s in range(12):
+        so = orc.step()
+        sg = ctx.re_step()
+        scale = so.mass_before
+        assert abs(sg.mass_residual) <= 1e-10 * scale, (s, sg.mass_residual, scale)
+        for f in ("mass_before", "mass_after", "mass_boundary_in", "mass_storage_ss", "mass_adjust"):
+            tol = (1e-12 if precision == abi.PREC_FP64 else 1e-8) * scale
+            assert abs(getattr(sg, f) - getattr(so, f)) <= tol, (s, f, getattr(sg, f), getattr(so, f))
+        assert sg.max_dh == pytest.approx(so.max_dh, rel=1e-6)
+        adj += abs(sg.mass_adjust)
+    assert adj > 0, "the case must exercise reallocation / clamp"
