@@ -30,6 +30,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# L2<->fabric bytes per cell of one k_spmv<double, true> launch on the 512x512x30 system, from two
+# rocprofv3 --pmc runs (FETCH_SIZE x2.000 and WRITE_SIZE x1.000, the scale factors of a 1 GiB
+# read + 1 GiB write elementwise kernel); see profiles/gw_spmv_fp64_512x512x30_traffic.md.
+SPMV64_TRAFFIC_BYTES_PER_CELL = 568.4e6 / 7864320
 BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
 
 
@@ -203,6 +207,9 @@ def main():
 
     # fused PCG SpMV kernel: reads z, p_prev, D, ax, ay, az; writes p, q (8 words per cell)
     roof64 = roof(plain["kt"]["spmv64"], 64)
+    if roof64 and (args.nx, args.ny, args.nz) == (512, 512, 30):
+        roof64["traffic"] = round(SPMV64_TRAFFIC_BYTES_PER_CELL * n3)
+        roof64["traffic_unit"] = "bytes per launch (rocprofv3 PMC, scaled)"
     roof32 = roof(fast["kt"]["spmv32"], 32)
     roof_guess = None
     ms, n = fast["kt"]["guess"]
