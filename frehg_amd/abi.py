@@ -97,6 +97,12 @@ class ReStats(C.Structure):
                 ("mass_residual", C.c_double)]
 
 
+class EnsStats(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("max_refinements", C.c_int32), ("converged", C.c_int32),
+                ("graph", C.c_int32), ("extra_launches", C.c_int32), ("realloc_sweeps", C.c_int32),
+                ("max_rel_mass_residual", C.c_double), ("mass_before", C.c_double), ("mass_after", C.c_double)]
+
+
 class CoupledStats(C.Structure):
     _fields_ = [("sw", SolveStats), ("gw", ReStats), ("gw_substeps", C.c_int32),
                 ("exchange_volume", C.c_double), ("t", C.c_double), ("gw_time", C.c_double)]

@@ -20,7 +20,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", os.path.join(HERE, "..", "include")]
 
-SOURCES = ["krylov.hip", "re.hip", "swe.hip", "coupling.hip", "mlp.hip", "api.hip"]
+SOURCES = ["krylov.hip", "re.hip", "swe.hip", "coupling.hip", "mlp.hip", "ensemble.hip", "api.hip"]
 
 
 def _compile(src):

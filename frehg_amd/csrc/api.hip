@@ -9,6 +9,7 @@
 
 #include "../../include/frehg_gpu.h"
 #include "coupling.hpp"
+#include "ensemble.hpp"
 #include "mlp.hpp"
 #include "re.hpp"
 #include "swe.hpp"
@@ -22,6 +23,7 @@ struct frehg_ctx_s {
   std::unique_ptr<frehg::SweState> swe;
   std::unique_ptr<frehg::Coupler> cpl;
   std::unique_ptr<frehg::Mlp> mlp;
+  std::unique_ptr<frehg::Ensemble> ens;
   double t = 0.0, gw_time = 0.0;
 };
 
@@ -110,6 +112,7 @@ void frehg_destroy(frehg_ctx_t ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  ctx->ens.reset();
   ctx->mlp.reset();
   ctx->cpl.reset();
   ctx->re.reset();
@@ -490,6 +493,49 @@ int frehg_spmv_shape(frehg_ctx_t ctx, int precision, int* tile_y, int* kchunk) {
     const frehg::Krylov::SpmvShape& sh = re.krylov().spmv_shape(precision == FREHG_PREC_MIXED ? 1 : 0);
     if (tile_y) *tile_y = sh.ty;
     if (kchunk) *kchunk = sh.kc;
+    return FREHG_OK;
+  });
+}
+
+frehg::Ensemble& need_ens(frehg_ctx_t c) {
+  if (!c->ens) throw std::runtime_error("ensemble not initialised (call frehg_ens_init)");
+  return *c->ens;
+}
+
+int frehg_ens_init(frehg_ctx_t ctx, int nboxes, const int32_t* dims, const frehg_re_params* params) {
+  return guard(ctx, [&] {
+    ctx->ens.reset();
+    ctx->ens.reset(new frehg::Ensemble(nboxes, dims, params, ctx->stream, &ctx->prof));
+    return FREHG_OK;
+  });
+}
+
+int frehg_ens_set_soil(frehg_ctx_t ctx, int box, const frehg_soil_params* classes, int ncls, const int32_t* class_idx,
+                       int idx3d, const double* ks_cell) {
+  return guard(ctx, [&] {
+    need_ens(ctx).set_soil(box, classes, ncls, class_idx, idx3d, ks_cell);
+    return FREHG_OK;
+  });
+}
+
+int frehg_ens_set_initial(frehg_ctx_t ctx, int box, const double* h, const double* wc) {
+  return guard(ctx, [&] {
+    need_ens(ctx).set_initial(box, h, wc);
+    return FREHG_OK;
+  });
+}
+
+int frehg_ens_get_field(frehg_ctx_t ctx, int box, const char* name, double* dst) {
+  return guard(ctx, [&] {
+    if (!name || !dst) throw frehg::ArgError("null argument");
+    need_ens(ctx).get_field(box, name, dst);
+    return FREHG_OK;
+  });
+}
+
+int frehg_ens_step(frehg_ctx_t ctx, int use_graph, frehg_ens_stats* stats, frehg_re_stats* per_box) {
+  return guard(ctx, [&] {
+    need_ens(ctx).step(use_graph, stats, per_box);
     return FREHG_OK;
   });
 }

@@ -71,6 +71,11 @@ def load():
         "frehg_profile_read": (C.c_int, [ctx, C.c_int, dp, P(C.c_int64)]),
         "frehg_profile_reset": (C.c_int, [ctx]),
         "frehg_spmv_shape": (C.c_int, [ctx, C.c_int, P(C.c_int), P(C.c_int)]),
+        "frehg_ens_init": (C.c_int, [ctx, C.c_int, P(C.c_int32), P(abi.ReParams)]),
+        "frehg_ens_set_soil": (C.c_int, [ctx, C.c_int, P(abi.SoilParams), C.c_int, P(C.c_int32), C.c_int, dp]),
+        "frehg_ens_set_initial": (C.c_int, [ctx, C.c_int, dp, dp]),
+        "frehg_ens_get_field": (C.c_int, [ctx, C.c_int, C.c_char_p, dp]),
+        "frehg_ens_step": (C.c_int, [ctx, C.c_int, P(abi.EnsStats), P(abi.ReStats)]),
         "frehg_mlp_init": (C.c_int, [ctx, C.c_uint64]),
         "frehg_mlp_log": (C.c_int, [ctx, C.c_int, C.c_int64]),
         "frehg_mlp_logged": (C.c_int, [ctx, P(C.c_int64)]),
@@ -250,6 +255,41 @@ class Context:
 
     def profile_reset(self):
         self.check(self.lib.frehg_profile_reset(self.h))
+
+    # ---- batched ensemble ----
+    def ens_init(self, dims, params):
+        dims = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(-1, 3))
+        arr = (abi.ReParams * len(params))(*params)
+        self.check(self.lib.frehg_ens_init(self.h, len(dims), dims.ctypes.data_as(P(C.c_int32)), arr))
+        self.ens_dims = [tuple(int(v) for v in d) for d in dims]
+
+    def ens_set_soil(self, box, classes, class_idx=None, idx3d=False, ks_cell=None):
+        arr = (abi.SoilParams * len(classes))(*classes)
+        ci = None
+        if class_idx is not None:
+            class_idx = np.ascontiguousarray(class_idx, dtype=np.int32).ravel()
+            ci = class_idx.ctypes.data_as(P(C.c_int32))
+        ks = None if ks_cell is None else np.ascontiguousarray(ks_cell, dtype=np.float64).ravel()
+        self.check(self.lib.frehg_ens_set_soil(self.h, box, arr, len(classes), ci, int(idx3d),
+                                               None if ks is None else ks.ctypes.data_as(dp)))
+
+    def ens_set_initial(self, box, h=None, wc=None):
+        hh = None if h is None else np.ascontiguousarray(h, dtype=np.float64).ravel()
+        ww = None if wc is None else np.ascontiguousarray(wc, dtype=np.float64).ravel()
+        self.check(self.lib.frehg_ens_set_initial(self.h, box, None if hh is None else hh.ctypes.data_as(dp),
+                                                  None if ww is None else ww.ctypes.data_as(dp)))
+
+    def ens_get(self, box, name):
+        nx, ny, nz = self.ens_dims[box]
+        out = np.zeros((nx + 2) * (ny + 2) * (nz + 2))
+        self.check(self.lib.frehg_ens_get_field(self.h, box, name.encode(), out.ctypes.data_as(dp)))
+        return out
+
+    def ens_step(self, use_graph=True, per_box=False):
+        st = abi.EnsStats()
+        pb = (abi.ReStats * len(self.ens_dims))() if per_box else None
+        self.check(self.lib.frehg_ens_step(self.h, int(use_graph), C.byref(st), pb))
+        return (st, list(pb)) if per_box else st
 
     # ---- learned initial guess ----
     def mlp_init(self, seed=1):

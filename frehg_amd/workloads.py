@@ -74,3 +74,43 @@ class CoupledCase:
         ctx.re_init(self.nx, self.ny, self.nz, self.re_params(precision, guess))
         ctx.re_set_soil([self.soil], None, True, self.ks.ravel())
         ctx.re_set_initial(h=self.h0)
+
+
+class EnsembleCase:
+    """Batched ensemble of independent Richards boxes of uneven size: nx, ny uniform in [32, 128],
+    nz uniform in [20, 120], dx = dy = 0.5 m, dz = 0.05 m; per box (seeded) a lognormal Ks field
+    (median 1e-5 m/s, sigma_ln = 1, Gaussian correlation 2 m), alpha uniform in [0.5, 5] 1/m,
+    n uniform in [1.3, 2.8], theta_s = 0.40, theta_r = 0.05, Ss = 1e-5; a downward top flux of
+    1e-6 m/s, a closed bottom, a uniform initial head per box drawn from [-5, -0.5] m; fixed dt."""
+
+    def __init__(self, nbox=96, seed=3, dt=30.0, nxy=(32, 128), nz=(20, 120), precision=abi.PREC_MIXED,
+                 guess=abi.GUESS_PREVIOUS, rtol=1e-10):
+        rng = np.random.Generator(np.random.Philox(seed))
+        self.dims, self.params, self.soils, self.ks, self.h0 = [], [], [], [], []
+        for b in range(nbox):
+            nx, ny = (int(v) for v in rng.integers(nxy[0], nxy[1] + 1, size=2))
+            nzb = int(rng.integers(nz[0], nz[1] + 1))
+            alpha = float(rng.uniform(0.5, 5.0))
+            n = float(rng.uniform(1.3, 2.8))
+            soil = abi.soil_params(alpha=alpha, n=n, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5,
+                                   Ss=1e-5)
+            lnk = grf((nzb, ny, nx), (0.05, 0.5, 0.5), 2.0, 1.0, seed * 1000 + b)
+            p = abi.re_params(soil=soil, solver=abi.solver_config(rtol, precision=precision, inner_rtol=1e-5,
+                                                                    max_refine=8),
+                              bc_type=(0, 0, 0, 0, 0, 2), dx=0.5, dy=0.5, dz=0.05, botz=-0.05 * nzb, dt=dt,
+                              adaptive_dt=0, use_full3d=1, qtop=-1e-6, initial_guess=guess)
+            self.dims.append((nx, ny, nzb))
+            self.params.append(p)
+            self.soils.append(soil)
+            self.ks.append(1e-5 * np.exp(lnk).ravel())
+            self.h0.append(float(rng.uniform(-5.0, -0.5)))
+
+    @property
+    def cells(self):
+        return int(sum(nx * ny * nz for nx, ny, nz in self.dims))
+
+    def setup_gpu(self, ctx):
+        ctx.ens_init(self.dims, self.params)
+        for b, (nx, ny, nz) in enumerate(self.dims):
+            ctx.ens_set_soil(b, [self.soils[b]], None, True, self.ks[b])
+            ctx.ens_set_initial(b, h=np.full(nx * ny * nz, self.h0[b]))

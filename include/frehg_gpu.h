@@ -209,6 +209,33 @@ int frehg_ls_solve(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, const doub
                    const double* const* offdiag, const double* b_owned, double* x_owned,
                    const frehg_solver_config* cfg, frehg_solve_stats* stats);
 
+/* ---- Batched ensemble of independent Richards boxes (configs of uneven box sizes or parameter
+ * ensembles). The reference advances such an ensemble as independent ReSolver instances
+ * (src/re/ReSolver.cpp:931-969); here every kernel of a step covers all boxes in one launch, each
+ * box keeps its own Krylov scalars and convergence test, and steps run from captured HIP graphs.
+ * Restrictions: fixed dt (adaptive_dt = 0), the same solver configuration in every box,
+ * initial guess zero or previous-step, closed lateral sides. ---------------------------------- */
+typedef struct {
+  int32_t max_iterations;        /* largest Krylov iteration count over the boxes (inner FP32 for mixed) */
+  int32_t max_refinements;       /* largest number of FP64 refinement sweeps (mixed) */
+  int32_t converged;             /* 1 if every box converged */
+  int32_t graph;                 /* 1 if the step ran from captured graphs */
+  int32_t extra_launches;        /* eager launches needed beyond the captured budget */
+  int32_t realloc_sweeps;        /* saturation fixed-point sweeps used */
+  double max_rel_mass_residual;  /* max over boxes of |mass_residual| / mass_before */
+  double mass_before, mass_after; /* sums over boxes */
+} frehg_ens_stats;
+/* dims: nboxes x (nx, ny, nz); params: one frehg_re_params per box. */
+int frehg_ens_init(frehg_ctx_t ctx, int nboxes, const int32_t* dims, const frehg_re_params* params);
+int frehg_ens_set_soil(frehg_ctx_t ctx, int box, const frehg_soil_params* classes, int ncls,
+                       const int32_t* class_idx, int idx3d, const double* ks_cell);
+int frehg_ens_set_initial(frehg_ctx_t ctx, int box, const double* h_interior, const double* wc_interior);
+/* Any GwFields array of one box (names as frehg_re_get_field), halo padded. */
+int frehg_ens_get_field(frehg_ctx_t ctx, int box, const char* name, double* dst);
+/* One step of every box. use_graph: run from captured graphs (captured at first use). per_box may
+ * be NULL, else nboxes entries with each box's solve and ledger. */
+int frehg_ens_step(frehg_ctx_t ctx, int use_graph, frehg_ens_stats* stats, frehg_re_stats* per_box);
+
 /* ---- Learned initial guess (no counterpart in the reference: it changes only the Krylov
  * x0 of the Richards solve, src/re/ReSolver.cpp:560-636, whose PETSc KSP always starts from 0).
  * An 11 -> 64 -> 64 -> 1 tanh MLP over local features of the state before the solve (h^n(c), the

@@ -1,0 +1,104 @@
+"""Batched ensemble of uneven Richards boxes (frehg_ens_*): every box against its own FP64 oracle
+run, graph-captured steps identical to eager ones, and the per-box water ledger."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from frehg_amd import abi, gpu, workloads
+from gpu_common import say
+
+pytestmark = pytest.mark.gpu
+
+
+def small_case(precision, guess=abi.GUESS_PREVIOUS):
+    case = workloads.EnsembleCase(nbox=5, seed=7, dt=30.0, nxy=(5, 70), nz=(4, 25), precision=precision, guess=guess,
+                                  rtol=1e-12)
+    # one box with a ponded Dirichlet top, so the reallocation fixed point has work to do
+    case.params[2].bc_type[5] = 1
+    case.params[2].htop = 0.05
+    return case
+
+
+def oracle_heads(case, b, steps):
+    nx, ny, nz = case.dims[b]
+    p = abi.re_params()
+    C = abi.ReParams
+    p = C.from_buffer_copy(case.params[b])
+    p.solver = abi.solver_config(1e-12)
+    r = ol.OracleRe(nx, ny, nz, p)
+    r.set_soil([case.soils[b]], None, True, case.ks[b])
+    r.set_initial(h=np.full(nx * ny * nz, case.h0[b]))
+    for _ in range(steps):
+        r.step()
+    return r.get("h"), r.get("wc")
+
+
+def single_box_heads(case, b, steps, precision):
+    nx, ny, nz = case.dims[b]
+    p = abi.ReParams.from_buffer_copy(case.params[b])
+    p.solver = abi.solver_config(1e-12, precision=precision, inner_rtol=1e-5, max_refine=8)
+    ctx = gpu.Context(0)
+    ctx.re_init(nx, ny, nz, p)
+    ctx.re_set_soil([case.soils[b]], None, True, case.ks[b])
+    ctx.re_set_initial(h=np.full(nx * ny * nz, case.h0[b]))
+    for _ in range(steps):
+        ctx.re_step()
+    return ctx.re_get("h"), ctx.re_get("wc")
+
+
+@pytest.mark.parametrize("precision", [abi.PREC_FP64, abi.PREC_MIXED])
+def test_ensemble_matches_oracle_per_box(precision):
+    steps = 6
+    case = small_case(precision)
+    say(f"ensemble {case.dims} precision {precision}")
+    ctx = gpu.Context(0)
+    case.setup_gpu(ctx)
+    for s in range(steps):
+        st, pb = ctx.ens_step(use_graph=True, per_box=True)
+        assert st.converged and st.max_rel_mass_residual <= 1e-10, (s, st.max_rel_mass_residual)
+        assert st.graph == (1 if s > 0 else 0)
+    for b in range(len(case.dims)):
+        ho, wo = oracle_heads(case, b, steps)
+        hs, ws = single_box_heads(case, b, steps, precision)
+        hg, wg = ctx.ens_get(b, "h"), ctx.ens_get(b, "wc")
+        say(f"box {b}: |h - oracle| {np.abs(hg - ho).max():.2e}, |h - single-box GPU| {np.abs(hg - hs).max():.2e}")
+        # the parity bar against the FP64 oracle
+        assert np.abs(hg - ho).max() <= 1e-6 and np.abs(wg - wo).max() <= 1e-6, b
+        # against the single-box GPU path with the same solver: only reduction grouping differs
+        tol = 1e-9 if precision == abi.PREC_FP64 else 1e-7
+        assert np.abs(hg - hs).max() <= tol * max(1.0, np.abs(hs).max()), b
+        assert np.abs(wg - ws).max() <= tol, b
+
+
+@pytest.mark.parametrize("precision", [abi.PREC_FP64, abi.PREC_MIXED])
+def test_graph_steps_equal_eager_steps(precision):
+    """FP64: the captured iteration budget plus eager continuation runs the same iterates as the eager
+    loop (bit-identical). Mixed: a refinement whose inner solve outgrows the captured budget is cut
+    and refined again, so the iterates differ but both meet the same tolerance."""
+    case = small_case(precision)
+    out = []
+    for use_graph in (False, True):
+        ctx = gpu.Context(0)
+        case.setup_gpu(ctx)
+        for _ in range(5):
+            ctx.ens_step(use_graph=use_graph)
+        out.append([ctx.ens_get(b, "h") for b in range(len(case.dims))])
+    for a, b in zip(*out):
+        if precision == abi.PREC_FP64:
+            assert np.array_equal(a, b)
+        else:
+            assert np.abs(a - b).max() <= 1e-8 * max(1.0, np.abs(a).max())
+
+
+def test_ensemble_rejects_bad_setup():
+    case = small_case(abi.PREC_MIXED)
+    ctx = gpu.Context(0)
+    bad = [abi.ReParams.from_buffer_copy(p) for p in case.params]
+    bad[1].adaptive_dt = 1
+    with pytest.raises(gpu.FrehgError) as e:
+        ctx.ens_init(case.dims, bad)
+    assert e.value.code == abi.ERR_INVALID_ARG
+    bad = [abi.ReParams.from_buffer_copy(p) for p in case.params]
+    bad[3].solver.rtol = 1e-6
+    with pytest.raises(gpu.FrehgError):
+        ctx.ens_init(case.dims, bad)
