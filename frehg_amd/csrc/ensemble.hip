@@ -62,6 +62,33 @@ __device__ __forceinline__ BoxCell box_cell(const EnsBox* boxes, const unsigned 
   return r;
 }
 
+// Column run of this thread in an SpMV-tile launch: cell (i, j) of the box, layers [k0, k1).
+struct BoxCol {
+  int b;
+  unsigned int tile;
+  long long off;
+  Geo g;
+  int i, j, k0, k1;
+  bool valid;
+};
+
+__device__ __forceinline__ BoxCol box_col(const EnsBox* boxes, const unsigned int* s0, int nbox) {
+  BoxCol r;
+  r.tile = xcd_remap(blockIdx.x, gridDim.x);
+  r.b = find_box(s0, nbox, r.tile);
+  const EnsBox& B = boxes[r.b];
+  r.off = B.off;
+  r.g = B.P.g;
+  const unsigned int local = r.tile - s0[r.b];
+  const unsigned int bx = local % B.ntx, rest = local / B.ntx, by = rest % B.nty, bz = rest / B.nty;
+  r.i = bx * 64 + (threadIdx.x & 63);
+  r.j = by * 4 + (threadIdx.x >> 6);
+  r.k0 = bz * B.kc;
+  r.k1 = min(r.g.nz, r.k0 + B.kc);
+  r.valid = r.i < r.g.nx && r.j < r.g.ny;
+  return r;
+}
+
 // ---- Richards kernels ------------------------------------------------------------------------
 
 __global__ __launch_bounds__(NT) void kb_save(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F) {
@@ -227,14 +254,22 @@ __global__ __launch_bounds__(NT) void kb_box_sums(const double* part, unsigned i
 }
 
 // x = xa for boxes whose refinement left the iterate in the alternate buffer.
-__global__ __launch_bounds__(NT) void kb_take_x(const EnsBox* boxes, const unsigned int* t0, int nbox, double* h,
+__global__ __launch_bounds__(NT) void kb_take_x(const EnsBox* boxes, const unsigned int* s0, int nbox, double* h,
                                                  const double* xa, const EnsCtl* ctl) {
-  const BoxCell q = box_cell(boxes, t0, nbox);
-  if (!q.t.valid || ctl[q.b].parity == 0) return;
-  h[q.off + q.t.c] = xa[q.off + q.t.c];
+  const BoxCol q = box_col(boxes, s0, nbox);
+  if (!q.valid || ctl[q.b].parity == 0) return;
+  for (int k = q.k0; k < q.k1; ++k) {
+    const long long c = q.off + q.g.idx(q.i, q.j, k);
+    h[c] = xa[c];
+  }
 }
 
 // ---- Krylov kernels (per-box control blocks; a block of a finished box returns at once) -------
+// They run over the SpMV tiles (64 x 4 columns x kc layers), so a launch has few blocks and the
+// blocks of finished boxes cost little.
+
+
+
 
 template <typename T>
 struct Sten {
@@ -245,34 +280,37 @@ __device__ __forceinline__ KrylovCtl& kctl(EnsCtl* ctl, int b, int inner) { retu
 
 // r = b - A x (or b), z = r / D, p_prev = 0; partials r.z, z.z, (b/D)^2.
 template <typename T>
-__global__ __launch_bounds__(NT) void kb_cg_init(const EnsBox* boxes, const unsigned int* t0, int nbox, Sten<T> A,
+__global__ __launch_bounds__(NT) void kb_cg_init(const EnsBox* boxes, const unsigned int* s0, int nbox, Sten<T> A,
                                                   const T* bvec, T* x, T* r, T* z, T* pprev, EnsCtl* ctl, int inner,
                                                   int x0_zero, double* part, unsigned int ntile) {
   __shared__ double lds[NT / kWave];
-  const BoxCell q = box_cell(boxes, t0, nbox);
+  const BoxCol q = box_col(boxes, s0, nbox);
   if (inner && ctl[q.b].o.done) return;
   double srz = 0.0, szz = 0.0, sbb = 0.0;
-  if (q.t.valid) {
-    const Geo& g = q.P.g;
-    const long long o = q.off, c = o + q.t.c;
-    double ax = 0.0;
-    if (!x0_zero) {
-      ax = (double)A.D[c] * (double)x[c] + (double)A.ax[c] * (double)x[c + 1] + (double)A.ax[c - 1] * (double)x[c - 1] +
-           (double)A.ay[c] * (double)x[c + g.sy] + (double)A.ay[c - g.sy] * (double)x[c - g.sy] +
-           (double)A.az[c] * (double)x[c + g.sz] + (double)A.az[c - g.sz] * (double)x[c - g.sz];
+  if (q.valid) {
+    const Geo& g = q.g;
+    for (int k = q.k0; k < q.k1; ++k) {
+      const long long c = q.off + g.idx(q.i, q.j, k);
+      double ax = 0.0;
+      if (!x0_zero) {
+        ax = (double)A.D[c] * (double)x[c] + (double)A.ax[c] * (double)x[c + 1] +
+             (double)A.ax[c - 1] * (double)x[c - 1] + (double)A.ay[c] * (double)x[c + g.sy] +
+             (double)A.ay[c - g.sy] * (double)x[c - g.sy] + (double)A.az[c] * (double)x[c + g.sz] +
+             (double)A.az[c - g.sz] * (double)x[c - g.sz];
+      }
+      const T rT = (T)((double)bvec[c] - ax);
+      const T zT = (T)((double)rT / (double)A.D[c]);
+      r[c] = rT;
+      z[c] = zT;
+      pprev[c] = (T)0;
+      srz += (double)rT * (double)zT;
+      szz += (double)zT * (double)zT;
+      const double bz = (double)bvec[c] / (double)A.D[c];
+      sbb += bz * bz;
     }
-    const T rT = (T)((double)bvec[c] - ax);
-    const T zT = (T)((double)rT / (double)A.D[c]);
-    r[c] = rT;
-    z[c] = zT;
-    pprev[c] = (T)0;
-    srz = (double)rT * (double)zT;
-    szz = (double)zT * (double)zT;
-    const double bz = (double)bvec[c] / (double)A.D[c];
-    sbb = bz * bz;
+    if (x0_zero)
+      for (int k = q.k0; k < q.k1; ++k) x[q.off + g.idx(q.i, q.j, k)] = (T)0;
   }
-  __syncthreads();
-  if (x0_zero && q.t.valid) x[q.off + q.t.c] = (T)0;
   const double a = block_sum<NT>(srz, lds);
   const double b2 = block_sum<NT>(szz, lds);
   const double c2 = block_sum<NT>(sbb, lds);
@@ -393,26 +431,28 @@ __global__ __launch_bounds__(NT) void kb_fin_spmv(const double* part, const unsi
 }
 
 template <typename T>
-__global__ __launch_bounds__(NT) void kb_update(const EnsBox* boxes, const unsigned int* t0, int nbox, const T* D,
+__global__ __launch_bounds__(NT) void kb_update(const EnsBox* boxes, const unsigned int* s0, int nbox, const T* D,
                                                  const T* p, const T* qv, T* x, T* r, T* z, EnsCtl* ctl, int inner,
                                                  double* part, unsigned int ntile) {
   __shared__ double lds[NT / kWave];
-  const BoxCell q = box_cell(boxes, t0, nbox);
+  const BoxCol q = box_col(boxes, s0, nbox);
   const KrylovCtl& kc = inner ? ctl[q.b].i : ctl[q.b].o;
   if (kc.done) return;
   const double alpha = kc.alpha;
   double srz = 0.0, szz = 0.0;
-  if (q.t.valid) {
-    const long long c = q.off + q.t.c;
-    const double xv = (double)x[c] + alpha * (double)p[c];
-    const double rv = (double)r[c] - alpha * (double)qv[c];
-    const T rT = (T)rv;
-    const T zT = (T)((double)rT / (double)D[c]);
-    x[c] = (T)xv;
-    r[c] = rT;
-    z[c] = zT;
-    srz = (double)rT * (double)zT;
-    szz = (double)zT * (double)zT;
+  if (q.valid) {
+    for (int k = q.k0; k < q.k1; ++k) {
+      const long long c = q.off + q.g.idx(q.i, q.j, k);
+      const double xv = (double)x[c] + alpha * (double)p[c];
+      const double rv = (double)r[c] - alpha * (double)qv[c];
+      const T rT = (T)rv;
+      const T zT = (T)((double)rT / (double)D[c]);
+      x[c] = (T)xv;
+      r[c] = rT;
+      z[c] = zT;
+      srz += (double)rT * (double)zT;
+      szz += (double)zT * (double)zT;
+    }
   }
   const double a = block_sum<NT>(srz, lds);
   const double b = block_sum<NT>(szz, lds);
@@ -435,7 +475,8 @@ __global__ __launch_bounds__(NT) void kb_fin_update(const double* part, unsigned
     c.iters = it;
     if (inner) {
       ctl[b].inner_total += 1;
-      ctl[b].inner_max = max(ctl[b].inner_max, it);
+      int& m = ctl[b].inner_max[ctl[b].refinements == 0 ? 0 : 1];
+      m = max(m, it);
     }
     c.zz = zz;
     if (zz <= c.ttol2) {
@@ -452,31 +493,33 @@ __global__ __launch_bounds__(NT) void kb_fin_update(const double* part, unsigned
 
 // Refinement sweep: x_new = x_cur + d (first sweep: x_cur itself), r32 = b - A x_new in FP64,
 // partials ||D^-1 r||^2 (and ||D^-1 b||^2 on the first sweep). x_cur is h or xa by the box's parity.
-__global__ __launch_bounds__(NT) void kb_refine(const EnsBox* boxes, const unsigned int* t0, int nbox, Sten<double> A,
+__global__ __launch_bounds__(NT) void kb_refine(const EnsBox* boxes, const unsigned int* s0, int nbox, Sten<double> A,
                                                  const double* bvec, double* h, double* xa, const float* d,
                                                  float* r32, EnsCtl* ctl, int first, double* part, unsigned int ntile) {
   __shared__ double lds[NT / kWave];
-  const BoxCell q = box_cell(boxes, t0, nbox);
+  const BoxCol q = box_col(boxes, s0, nbox);
   if (!first && ctl[q.b].o.done) return;
   const int par = first ? 0 : ctl[q.b].parity;
   const double* xin = par ? xa : h;
   double* xout = par ? h : xa;
   double srr = 0.0, sbb = 0.0;
-  if (q.t.valid) {
-    const Geo& g = q.P.g;
-    const long long c = q.off + q.t.c;
+  if (q.valid) {
+    const Geo& g = q.g;
     auto X = [&](long long cc) -> double { return xin[cc] + (first ? 0.0 : (double)d[cc]); };
-    const double xc = X(c);
-    const double y = A.D[c] * xc + A.ax[c] * X(c + 1) + A.ax[c - 1] * X(c - 1) + A.ay[c] * X(c + g.sy) +
-                     A.ay[c - g.sy] * X(c - g.sy) + A.az[c] * X(c + g.sz) + A.az[c - g.sz] * X(c - g.sz);
-    const double rv = bvec[c] - y;
-    if (!first) xout[c] = xc;
-    r32[c] = (float)rv;
-    const double rz = rv / A.D[c];
-    srr = rz * rz;
-    if (first) {
-      const double bz = bvec[c] / A.D[c];
-      sbb = bz * bz;
+    for (int k = q.k0; k < q.k1; ++k) {
+      const long long c = q.off + g.idx(q.i, q.j, k);
+      const double xc = X(c);
+      const double y = A.D[c] * xc + A.ax[c] * X(c + 1) + A.ax[c - 1] * X(c - 1) + A.ay[c] * X(c + g.sy) +
+                       A.ay[c - g.sy] * X(c - g.sy) + A.az[c] * X(c + g.sz) + A.az[c - g.sz] * X(c - g.sz);
+      const double rv = bvec[c] - y;
+      if (!first) xout[c] = xc;
+      r32[c] = (float)rv;
+      const double rz = rv / A.D[c];
+      srr += rz * rz;
+      if (first) {
+        const double bz = bvec[c] / A.D[c];
+        sbb += bz * bz;
+      }
     }
   }
   const double a = block_sum<NT>(srr, lds);
@@ -502,7 +545,8 @@ __global__ __launch_bounds__(NT) void kb_fin_refine(const double* part, unsigned
       E.o.ttol2 = ttol * ttol;
       E.refinements = 0;
       E.inner_total = 0;
-      E.inner_max = 0;
+      E.inner_max[0] = 0;
+      E.inner_max[1] = 0;
       E.parity = 0;
     } else {
       E.parity ^= 1;
@@ -782,22 +826,22 @@ void Ensemble::enq_prepare() {
   // Krylov start: FP64 CG init, or the first refinement residual
   if (!mixed_) {
     Sten<double> A{F_.D, F_.ax, F_.ay, F_.az};
-    hipLaunchKernelGGL((kb_cg_init<double>), grid, blk, 0, s_, bx, t0, nbox_, A, (const double*)F_.rhs, F_.h, F_.r,
-                       F_.z, F_.p0, ctl_, 0, prev_guess_ ? 0 : 1, part_, ntile_);
-    hipLaunchKernelGGL(kb_fin_init, dim3(nbox_), blk, 0, s_, (const double*)part_, ntile_, t0, ctl_, 0, kc_.rtol,
+    hipLaunchKernelGGL((kb_cg_init<double>), dim3(nspmv_), blk, 0, s_, bx, (const unsigned int*)dspmv0_, nbox_, A, (const double*)F_.rhs, F_.h, F_.r,
+                       F_.z, F_.p0, ctl_, 0, prev_guess_ ? 0 : 1, part_, nspmv_);
+    hipLaunchKernelGGL(kb_fin_init, dim3(nbox_), blk, 0, s_, (const double*)part_, nspmv_, (const unsigned int*)dspmv0_, ctl_, 0, kc_.rtol,
                        kc_.atol, kc_.max_it, kc_.inner_rtol);
   } else {
     Sten<double> A{F_.D, F_.ax, F_.ay, F_.az};
-    hipLaunchKernelGGL(kb_refine, grid, blk, 0, s_, bx, t0, nbox_, A, (const double*)F_.rhs, F_.h, F_.xa,
-                       (const float*)F_.d32, F_.b32, ctl_, 1, part_, ntile_);
-    hipLaunchKernelGGL(kb_fin_refine, dim3(nbox_), blk, 0, s_, (const double*)part_, ntile_, t0, ctl_, 1, kc_.rtol,
+    hipLaunchKernelGGL(kb_refine, dim3(nspmv_), blk, 0, s_, bx, (const unsigned int*)dspmv0_, nbox_, A, (const double*)F_.rhs,
+                       F_.h, F_.xa, (const float*)F_.d32, F_.b32, ctl_, 1, part_, nspmv_);
+    hipLaunchKernelGGL(kb_fin_refine, dim3(nbox_), blk, 0, s_, (const double*)part_, nspmv_, (const unsigned int*)dspmv0_, ctl_, 1, kc_.rtol,
                        kc_.atol);
   }
   FREHG_LAUNCH_CHECK();
 }
 
 // inner_iters: CG iterations per inner solve (mixed) or in total (FP64); refinements: mixed only.
-void Ensemble::enq_solve(int inner_iters, int refinements) {
+void Ensemble::enq_solve(int inner_iters, int refinements, int inner_first) {
   const dim3 grid(ntile_), gsp(nspmv_), blk(NT), gb(nbox_);
   const EnsBox* bx = dbox_;
   const unsigned int* t0 = dtile0_;
@@ -812,9 +856,9 @@ void Ensemble::enq_solve(int inner_iters, int refinements) {
       hipLaunchKernelGGL((kb_spmv<T>), gsp, blk, 0, s_, bx, s0, nbox_, A, (const T*)z, (const T*)pold, pnew, q, ctl_,
                          inner, part_);
       hipLaunchKernelGGL(kb_fin_spmv, gb, blk, 0, s_, (const double*)part_, s0, ctl_, inner);
-      hipLaunchKernelGGL((kb_update<T>), grid, blk, 0, s_, bx, t0, nbox_, A.D, (const T*)pnew, (const T*)q, x, r, z,
-                         ctl_, inner, part_, ntile_);
-      hipLaunchKernelGGL(kb_fin_update, gb, blk, 0, s_, (const double*)part_, ntile_, t0, ctl_, inner);
+      hipLaunchKernelGGL((kb_update<T>), gsp, blk, 0, s_, bx, s0, nbox_, A.D, (const T*)pnew, (const T*)q, x, r, z,
+                         ctl_, inner, part_, nspmv_);
+      hipLaunchKernelGGL(kb_fin_update, gb, blk, 0, s_, (const double*)part_, nspmv_, s0, ctl_, inner);
     }
     FREHG_LAUNCH_CHECK();
   };
@@ -827,15 +871,16 @@ void Ensemble::enq_solve(int inner_iters, int refinements) {
   Sten<float> A32{F_.D32, F_.ax32, F_.ay32, F_.az32};
   Sten<double> A{F_.D, F_.ax, F_.ay, F_.az};
   for (int rf = 0; rf < refinements; ++rf) {
-    hipLaunchKernelGGL((kb_cg_init<float>), grid, blk, 0, s_, bx, t0, nbox_, A32, (const float*)F_.b32, F_.d32,
-                       F_.r32, F_.z32, F_.p032, ctl_, 1, 1, part_, ntile_);
-    hipLaunchKernelGGL(kb_fin_init, gb, blk, 0, s_, (const double*)part_, ntile_, t0, ctl_, 1, kc_.rtol, kc_.atol,
+    hipLaunchKernelGGL((kb_cg_init<float>), gsp, blk, 0, s_, bx, s0, nbox_, A32, (const float*)F_.b32, F_.d32,
+                       F_.r32, F_.z32, F_.p032, ctl_, 1, 1, part_, nspmv_);
+    hipLaunchKernelGGL(kb_fin_init, gb, blk, 0, s_, (const double*)part_, nspmv_, s0, ctl_, 1, kc_.rtol, kc_.atol,
                        kc_.max_it, kc_.inner_rtol);
     FREHG_LAUNCH_CHECK();
-    iterate(0.0f, A32, F_.d32, F_.r32, F_.z32, F_.p032, F_.p132, F_.q32, 1, inner_iters);
-    hipLaunchKernelGGL(kb_refine, grid, blk, 0, s_, bx, t0, nbox_, A, (const double*)F_.rhs, F_.h, F_.xa,
-                       (const float*)F_.d32, F_.b32, ctl_, 0, part_, ntile_);
-    hipLaunchKernelGGL(kb_fin_refine, gb, blk, 0, s_, (const double*)part_, ntile_, t0, ctl_, 0, kc_.rtol, kc_.atol);
+    iterate(0.0f, A32, F_.d32, F_.r32, F_.z32, F_.p032, F_.p132, F_.q32, 1,
+            rf == 0 && inner_first >= 0 ? inner_first : inner_iters);
+    hipLaunchKernelGGL(kb_refine, gsp, blk, 0, s_, bx, s0, nbox_, A, (const double*)F_.rhs, F_.h, F_.xa,
+                       (const float*)F_.d32, F_.b32, ctl_, 0, part_, nspmv_);
+    hipLaunchKernelGGL(kb_fin_refine, gb, blk, 0, s_, (const double*)part_, nspmv_, s0, ctl_, 0, kc_.rtol, kc_.atol);
     FREHG_LAUNCH_CHECK();
   }
 }
@@ -846,7 +891,9 @@ void Ensemble::enq_update(int sweeps) {
   const dim3 grid(ntile_), blk(NT);
   const EnsBox* bx = dbox_;
   const unsigned int* t0 = dtile0_;
-  if (mixed_) hipLaunchKernelGGL(kb_take_x, grid, blk, 0, s_, bx, t0, nbox_, F_.h, (const double*)F_.xa, ctl_);
+  if (mixed_)
+    hipLaunchKernelGGL(kb_take_x, dim3(nspmv_), blk, 0, s_, bx, (const unsigned int*)dspmv0_, nbox_, F_.h,
+                       (const double*)F_.xa, ctl_);
   hipLaunchKernelGGL(kb_post, grid, blk, 0, s_, bx, t0, nbox_, P);
   hipLaunchKernelGGL(kb_flux, grid, blk, 0, s_, bx, t0, nbox_, P);
   hipLaunchKernelGGL(kb_correct, grid, blk, 0, s_, bx, t0, nbox_, P, F_.sat0, F_.cand, part_, ntile_);
@@ -921,8 +968,9 @@ void Ensemble::autotune() {
   (void)hipEventDestroy(e1);
 }
 
-void Ensemble::run_graph(int which, int budget_a, int budget_b) {
-  if (!graph_[which] || graph_budget_[which] != budget_a || graph_budget2_[which] != budget_b) {
+void Ensemble::run_graph(int which, int budget_a, int budget_b, int budget_c) {
+  if (!graph_[which] || graph_budget_[which] != budget_a || graph_budget2_[which] != budget_b ||
+      graph_budget3_[which] != budget_c) {
     if (graph_[which]) FREHG_HIP(hipGraphExecDestroy(graph_[which]));
     graph_[which] = nullptr;
     hipGraph_t g;
@@ -930,7 +978,7 @@ void Ensemble::run_graph(int which, int budget_a, int budget_b) {
     try {
       if (which == 0) {
         enq_prepare();
-        enq_solve(budget_a, budget_b);
+        enq_solve(budget_a, budget_b, budget_c);
       } else if (which == 1) {
         enq_update(budget_a);
       } else {
@@ -945,6 +993,7 @@ void Ensemble::run_graph(int which, int budget_a, int budget_b) {
     FREHG_HIP(hipGraphDestroy(g));
     graph_budget_[which] = budget_a;
     graph_budget2_[which] = budget_b;
+    graph_budget3_[which] = budget_c;
   }
   FREHG_HIP(hipGraphLaunch(graph_[which], s_));
 }
@@ -956,7 +1005,7 @@ void Ensemble::step(int use_graph, frehg_ens_stats* st, frehg_re_stats* per_box)
   // phase A: prepare + solve
   const int chunk = 8;
   if (graphs) {
-    run_graph(0, mixed_ ? budget_inner_ : budget_fp64_, mixed_ ? budget_refine_ : 0);
+    run_graph(0, mixed_ ? budget_inner_ : budget_fp64_, mixed_ ? budget_refine_ : 0, mixed_ ? budget_inner0_ : 0);
   } else {
     enq_prepare();
   }
@@ -978,9 +1027,9 @@ void Ensemble::step(int use_graph, frehg_ens_stats* st, frehg_re_stats* per_box)
       // one refinement with an inner solve run to its own tolerance
       const dim3 grid(ntile_), blk(NT), gb(nbox_);
       Sten<float> A32{F_.D32, F_.ax32, F_.ay32, F_.az32};
-      hipLaunchKernelGGL((kb_cg_init<float>), grid, blk, 0, s_, (const EnsBox*)dbox_, (const unsigned int*)dtile0_,
-                         nbox_, A32, (const float*)F_.b32, F_.d32, F_.r32, F_.z32, F_.p032, ctl_, 1, 1, part_, ntile_);
-      hipLaunchKernelGGL(kb_fin_init, gb, blk, 0, s_, (const double*)part_, ntile_, (const unsigned int*)dtile0_, ctl_,
+      hipLaunchKernelGGL((kb_cg_init<float>), dim3(nspmv_), blk, 0, s_, (const EnsBox*)dbox_, (const unsigned int*)dspmv0_,
+                         nbox_, A32, (const float*)F_.b32, F_.d32, F_.r32, F_.z32, F_.p032, ctl_, 1, 1, part_, nspmv_);
+      hipLaunchKernelGGL(kb_fin_init, gb, blk, 0, s_, (const double*)part_, nspmv_, (const unsigned int*)dspmv0_, ctl_,
                          1, kc_.rtol, kc_.atol, kc_.max_it, kc_.inner_rtol);
       FREHG_LAUNCH_CHECK();
       int inner_done = 0;
@@ -994,10 +1043,10 @@ void Ensemble::step(int use_graph, frehg_ens_stats* st, frehg_re_stats* per_box)
           hipLaunchKernelGGL((kb_spmv<float>), gsp, blk, 0, s_, (const EnsBox*)dbox_, (const unsigned int*)dspmv0_,
                              nbox_, A32, (const float*)F_.z32, (const float*)pold, pnew, F_.q32, ctl_, 1, part_);
           hipLaunchKernelGGL(kb_fin_spmv, gb, blk, 0, s_, (const double*)part_, (const unsigned int*)dspmv0_, ctl_, 1);
-          hipLaunchKernelGGL((kb_update<float>), grid, blk, 0, s_, (const EnsBox*)dbox_, (const unsigned int*)dtile0_,
+          hipLaunchKernelGGL((kb_update<float>), gsp, blk, 0, s_, (const EnsBox*)dbox_, (const unsigned int*)dspmv0_,
                              nbox_, (const float*)F_.D32, (const float*)pnew, (const float*)F_.q32, F_.d32, F_.r32,
-                             F_.z32, ctl_, 1, part_, ntile_);
-          hipLaunchKernelGGL(kb_fin_update, gb, blk, 0, s_, (const double*)part_, ntile_, (const unsigned int*)dtile0_,
+                             F_.z32, ctl_, 1, part_, nspmv_);
+          hipLaunchKernelGGL(kb_fin_update, gb, blk, 0, s_, (const double*)part_, nspmv_, (const unsigned int*)dspmv0_,
                              ctl_, 1);
         }
         FREHG_LAUNCH_CHECK();
@@ -1007,9 +1056,9 @@ void Ensemble::step(int use_graph, frehg_ens_stats* st, frehg_re_stats* per_box)
         if (hflags_[1] == 0 || hflags_[2] != 0 || inner_done >= kc_.max_it + 2) break;
       }
       Sten<double> A{F_.D, F_.ax, F_.ay, F_.az};
-      hipLaunchKernelGGL(kb_refine, grid, blk, 0, s_, (const EnsBox*)dbox_, (const unsigned int*)dtile0_, nbox_, A,
-                         (const double*)F_.rhs, F_.h, F_.xa, (const float*)F_.d32, F_.b32, ctl_, 0, part_, ntile_);
-      hipLaunchKernelGGL(kb_fin_refine, gb, blk, 0, s_, (const double*)part_, ntile_, (const unsigned int*)dtile0_,
+      hipLaunchKernelGGL(kb_refine, dim3(nspmv_), blk, 0, s_, (const EnsBox*)dbox_, (const unsigned int*)dspmv0_, nbox_,
+                         A, (const double*)F_.rhs, F_.h, F_.xa, (const float*)F_.d32, F_.b32, ctl_, 0, part_, nspmv_);
+      hipLaunchKernelGGL(kb_fin_refine, gb, blk, 0, s_, (const double*)part_, nspmv_, (const unsigned int*)dspmv0_,
                          ctl_, 0, kc_.rtol, kc_.atol);
       FREHG_LAUNCH_CHECK();
       if (graphs) ++extra;
@@ -1023,11 +1072,14 @@ void Ensemble::step(int use_graph, frehg_ens_stats* st, frehg_re_stats* per_box)
   // record the budgets (from the per-box counters) for the graphs of later steps
   FREHG_HIP(hipMemcpyAsync(hctl_, ctl_, nbox_ * sizeof(EnsCtl), hipMemcpyDeviceToHost, s_));
   FREHG_HIP(hipStreamSynchronize(s_));
-  int max_it = 0, max_ref = 0, max_inner = 0;
+  int max_it = 0, max_ref = 0, max_inner = 0, max_inner0 = 0;
   for (int b = 0; b < nbox_; ++b) {
     max_it = std::max(max_it, mixed_ ? hctl_[b].inner_total : hctl_[b].o.iters);
     max_ref = std::max(max_ref, hctl_[b].refinements);
-    if (mixed_) max_inner = std::max(max_inner, hctl_[b].inner_max);
+    if (mixed_) {
+      max_inner0 = std::max(max_inner0, hctl_[b].inner_max[0]);
+      max_inner = std::max(max_inner, hctl_[b].inner_max[1]);
+    }
   }
   // phase B1: BCs, fluxes, corrector, saturation sweeps
   if (graphs) {
@@ -1075,6 +1127,7 @@ void Ensemble::step(int use_graph, frehg_ens_stats* st, frehg_re_stats* per_box)
   // budgets for the next captured step
   if (mixed_) {
     budget_refine_ = std::max(budget_refine_, max_ref);
+    budget_inner0_ = std::max(budget_inner0_, max_inner0 + 1);
     budget_inner_ = std::max(budget_inner_, max_inner + 1);
   } else {
     budget_fp64_ = std::max(budget_fp64_, max_it);

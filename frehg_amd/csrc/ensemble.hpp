@@ -27,7 +27,9 @@ struct EnsBox {
 // Per-box Krylov state: outer (FP64 residual / FP64 CG) and inner (FP32 CG) control blocks.
 struct EnsCtl {
   KrylovCtl o, i;
-  int refinements, inner_total, parity, inner_max;  // inner_max: most inner iterations of one refinement
+  int refinements, inner_total, parity;
+  int inner_max[2];  // most inner iterations of the first refinement / of any later one
+  int pad;
 };
 
 class Ensemble {
@@ -57,12 +59,14 @@ class Ensemble {
   void autotune();
   // Kernel sequences; each can be captured into a graph.
   void enq_prepare();                  // save h^n, theta^n; K/C; assembly; initial guess
-  void enq_solve(int inner_iters, int refinements);  // Krylov budget (all boxes, early exit per box)
+  // Krylov budget (all boxes, early exit per box): FP64 iterations, or mixed refinements with
+  // inner_first iterations in the first and inner_iters in each later inner solve.
+  void enq_solve(int inner_iters, int refinements, int inner_first = -1);
   void enq_update(int sweeps);         // head BCs, fluxes, corrector, reallocation sweeps
   void enq_finish(const unsigned char* fin);  // reallocation branches, clamp, ledger
   void enq_status();
   void read_status();
-  void run_graph(int which, int budget_a, int budget_b);
+  void run_graph(int which, int budget_a, int budget_b, int budget_c = 0);
 
   int nbox_;
   hipStream_t s_;
@@ -89,10 +93,11 @@ class Ensemble {
   std::vector<int32_t*> cls_;
   std::vector<double*> ks_;
   bool tuned_ = false;
-  int budget_inner_ = 0, budget_refine_ = 0, budget_fp64_ = 0, budget_sweeps_ = 4;
+  int budget_inner_ = 0, budget_inner0_ = 0, budget_refine_ = 0, budget_fp64_ = 0, budget_sweeps_ = 4;
   hipGraphExec_t graph_[3] = {nullptr, nullptr, nullptr};
   int graph_budget_[3] = {-1, -1, -1};
   int graph_budget2_[3] = {-1, -1, -1};
+  int graph_budget3_[3] = {-1, -1, -1};
   static constexpr int kMaxSweepFlags = 64;
 };
 
