@@ -334,7 +334,7 @@ def main():
                            "partial p.q (64 B per cell)",
         "roofline_fp32_spmv": roof32,
         "roofline_mlp_guess": roof_guess,
-        "spmv_shape_tile_y_kchunk": {"plain_fp64": plain["spmv_shape"], "fast_fp32": fast["spmv_shape"]},
+        "spmv_shape_cells_per_lane_kchunk": {"plain_fp64": plain["spmv_shape"], "fast_fp32": fast["spmv_shape"]},
         "kernel_ms_fast": {k: round(v[0], 3) for k, v in fast["kt"].items()},
         "kernel_ms_plain": {k: round(v[0], 3) for k, v in plain["kt"].items()},
         "cpu_baseline": cpu,

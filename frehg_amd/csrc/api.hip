@@ -485,13 +485,13 @@ int frehg_profile_reset(frehg_ctx_t ctx) {
   });
 }
 
-int frehg_spmv_shape(frehg_ctx_t ctx, int precision, int* tile_y, int* kchunk) {
+int frehg_spmv_shape(frehg_ctx_t ctx, int precision, int* cells_per_lane, int* kchunk) {
   return guard(ctx, [&] {
     frehg::ReState& re = need_re(ctx);
     if (precision != FREHG_PREC_FP64 && precision != FREHG_PREC_MIXED)
       throw frehg::ArgError("frehg_spmv_shape: precision must be FREHG_PREC_FP64 or FREHG_PREC_MIXED");
     const frehg::Krylov::SpmvShape& sh = re.krylov().spmv_shape(precision == FREHG_PREC_MIXED ? 1 : 0);
-    if (tile_y) *tile_y = sh.ty;
+    if (cells_per_lane) *cells_per_lane = sh.cpl;
     if (kchunk) *kchunk = sh.kc;
     return FREHG_OK;
   });

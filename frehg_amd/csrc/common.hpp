@@ -88,6 +88,26 @@ __global__ __launch_bounds__(kRedThreads) inline void k_reduce_sum(const double*
   }
 }
 
+// Two-stage form for long rows: kRedStage blocks per sum each reduce one contiguous chunk, then
+// one block reduces the chunk sums; both stages use fixed orders, so the result is reproducible.
+constexpr unsigned int kRedStage = 64;
+
+__global__ __launch_bounds__(256) inline void k_reduce_sum_stage(const double* part, unsigned int n, double* out) {
+  __shared__ double lds[256 / kWave];
+  const unsigned int q = blockIdx.y, chunk = (n + gridDim.x - 1) / gridDim.x;
+  const unsigned int lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  double v = 0.0;
+  for (unsigned int i = lo + threadIdx.x; i < hi; i += 256) v += part[(size_t)q * n + i];
+  v = block_sum<256>(v, lds);
+  if (threadIdx.x == 0) out[q * gridDim.x + blockIdx.x] = v;
+}
+
+// out[q] = sum of part[q * n .. q * n + n) for q < nsums; tmp holds nsums * kRedStage doubles.
+inline void reduce_sums(const double* part, unsigned int n, int nsums, double* tmp, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_sum_stage, dim3(kRedStage, nsums), dim3(256), 0, s, part, n, tmp);
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kRedThreads), 0, s, (const double*)tmp, kRedStage, nsums, out);
+}
+
 __global__ __launch_bounds__(kRedThreads) inline void k_reduce_maxmin(const double* part, unsigned int n,
                                                                         double* out) {
   __shared__ double lds[kRedThreads / kWave];
@@ -160,15 +180,41 @@ inline SoilClass make_soil_class(double alpha, double n, double theta_s, double 
 
 // Per-cell soil lookup on the halo-padded grid. cls / ks are padded arrays whose ghost entries
 // hold the clamped interior value (ReSolver::soilAt clamps ghost indices, ReSolver.cpp:87-96).
-struct SoilView {
+// c0 repeats class 0 by value, so that with cls == nullptr (one class) the class parameters are
+// kernel arguments (scalar registers) instead of per-lane loads. The template arguments fix, per
+// kernel instantiation, whether cls / ks are absent (kAbsent), present (kPresent) or checked at
+// each access (kRuntime): fixed variants have no per-access branches, so the loads of a cell's
+// whole stencil can issue together.
+enum SoilArr { kAbsent = 0, kPresent = 1, kRuntime = 2 };
+
+template <int CLS, int KS>
+struct SoilViewT {
   const SoilClass* classes;
   const int32_t* cls;  // nullptr -> every cell is class 0
   const double* ks;    // nullptr -> Ks from the class; else isotropic per-cell Ks
-  __device__ inline const SoilClass& at(long long c) const { return classes[cls ? cls[c] : 0]; }
-  __device__ inline double Ksx(long long c) const { return ks ? ks[c] : at(c).Ks_x; }
-  __device__ inline double Ksy(long long c) const { return ks ? ks[c] : at(c).Ks_y; }
-  __device__ inline double Ksz(long long c) const { return ks ? ks[c] : at(c).Ks_z; }
+  SoilClass c0;        // classes[0]
+  __device__ inline SoilClass at(long long c) const {
+    if constexpr (CLS == kAbsent) return c0;
+    else if constexpr (CLS == kPresent) return classes[cls[c]];
+    else return cls ? classes[cls[c]] : c0;
+  }
+  __device__ inline double Ksx(long long c) const {
+    if constexpr (KS == kPresent) return ks[c];
+    else if constexpr (KS == kAbsent) return at(c).Ks_x;
+    else return ks ? ks[c] : at(c).Ks_x;
+  }
+  __device__ inline double Ksy(long long c) const {
+    if constexpr (KS == kPresent) return ks[c];
+    else if constexpr (KS == kAbsent) return at(c).Ks_y;
+    else return ks ? ks[c] : at(c).Ks_y;
+  }
+  __device__ inline double Ksz(long long c) const {
+    if constexpr (KS == kPresent) return ks[c];
+    else if constexpr (KS == kAbsent) return at(c).Ks_z;
+    else return ks ? ks[c] : at(c).Ks_z;
+  }
 };
+using SoilView = SoilViewT<kRuntime, kRuntime>;
 
 // ---------------------------------------------------------------------------------------------
 // Van Genuchten relations (src/re/VanGenuchten.hpp:18-137). The plain-VG path evaluates
@@ -230,7 +276,23 @@ __device__ inline double vg_cap_ref(const SoilClass& s, double h) {  // VanGenuc
   return c;
 }
 
-__device__ __noinline__ VGState vg_state_ref(const SoilClass& s, double h) {
+// Out of line (called only for non-VG and modified-VG classes). The class parameters are passed
+// as scalars, so that a caller's class (often a kernel argument) never has to be copied to
+// per-lane stack memory to have its address taken.
+__device__ __noinline__ VGState vg_state_ref(double alpha, double n, double m, double theta_s, double theta_r,
+                                             double wcm, double capc, double air_entry, int use_vg, int use_mvg,
+                                             double h) {
+  SoilClass s{};
+  s.alpha = alpha;
+  s.n = n;
+  s.m = m;
+  s.theta_s = theta_s;
+  s.theta_r = theta_r;
+  s.wcm = wcm;
+  s.capc = capc;
+  s.air_entry = air_entry;
+  s.use_vg = use_vg;
+  s.use_mvg = use_mvg;
   VGState o;
   o.theta = vg_theta_ref(s, h);
   o.cap = vg_cap_ref(s, h);
@@ -239,7 +301,8 @@ __device__ __noinline__ VGState vg_state_ref(const SoilClass& s, double h) {
 }
 
 __device__ __forceinline__ VGState vg_state(const SoilClass& s, double h) {
-  if (!s.use_vg || s.use_mvg) return vg_state_ref(s, h);
+  if (!s.use_vg || s.use_mvg)
+    return vg_state_ref(s.alpha, s.n, s.m, s.theta_s, s.theta_r, s.wcm, s.capc, s.air_entry, s.use_vg, s.use_mvg, h);
   VGState o;
   const double a = fabs(s.alpha * h);
   double se, cap, kr;
@@ -328,18 +391,28 @@ inline SoilClass make_soil_class(double alpha, double n, double theta_s, double 
   return s;
 }
 
-// Halo-padded grid geometry (src/core/Grid.hpp:42-60).
+// Halo-padded grid geometry. The cells are those of the reference's halo-padded arrays
+// (src/core/Grid.hpp:42-60: i in [-1, nx], j in [-1, ny], k in [-1, nz], i fastest), stored with
+// a row pitch `sy` rounded up to 16 doubles and cell (0, j, k) at a multiple of 16, so every
+// interior row starts on a 128-byte boundary and pairs of cells (i even) are 16-byte aligned.
+// Row j of layer k holds cells i = -1 .. nx at kGeoOff - 1 .. kGeoOff + nx of its pitch; the
+// rest of the pitch is padding that no kernel writes. The public field interface keeps the
+// reference's packed (nx + 2)-pitch layout (copy_field_in / copy_field_out).
+constexpr int kGeoOff = 16;
+
 struct Geo {
   int nx, ny, nz;
   long long sy, sz;  // +1 in j, +1 in k
   __host__ __device__ inline long long idx(int i, int j, int k) const {
-    return (long long)(i + 1) + (long long)(j + 1) * sy + (long long)(k + 1) * sz;
+    return (long long)(kGeoOff + i) + (long long)(j + 1) * sy + (long long)(k + 1) * sz;
   }
   __host__ __device__ inline long long sidx(int i, int j) const {
-    return (long long)(i + 1) + (long long)(j + 1) * sy;
+    return (long long)(kGeoOff + i) + (long long)(j + 1) * sy;
   }
-  __host__ __device__ inline long long ncell() const { return sz * (nz + 2); }
-  __host__ __device__ inline long long nsurf() const { return sz; }
+  // storage of a 3D field (nz + 2 layers) / a 2D field (one layer); the tail of kGeoOff doubles
+  // keeps the last row's padding (read by paired loads) inside the allocation
+  __host__ __device__ inline long long ncell() const { return sz * (nz + 2) + kGeoOff; }
+  __host__ __device__ inline long long nsurf() const { return sz + kGeoOff; }
 };
 
 inline Geo make_geo(int nx, int ny, int nz) {
@@ -347,9 +420,24 @@ inline Geo make_geo(int nx, int ny, int nz) {
   g.nx = nx;
   g.ny = ny;
   g.nz = nz;
-  g.sy = nx + 2;
-  g.sz = (long long)(nx + 2) * (ny + 2);
+  g.sy = ((long long)nx + 2 + 15) / 16 * 16;
+  g.sz = g.sy * (ny + 2);
   return g;
+}
+
+// Transfers between a device field and the reference's packed halo-padded layout
+// ((nx + 2) x (ny + 2) x planes doubles, planes = nz + 2 for 3D fields, 1 for 2D fields).
+inline void copy_field_in(const Geo& g, int planes, double* dev, const double* host, hipStream_t s) {
+  const size_t w = (size_t)(g.nx + 2) * sizeof(double);
+  const hipError_t e = hipMemcpy2DAsync(dev + kGeoOff - 1, (size_t)g.sy * sizeof(double), host, w, w,
+                                        (size_t)(g.ny + 2) * planes, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) throw HipError(std::string("copy_field_in: ") + hipGetErrorString(e));
+}
+inline void copy_field_out(const Geo& g, int planes, double* host, const double* dev, hipStream_t s) {
+  const size_t w = (size_t)(g.nx + 2) * sizeof(double);
+  const hipError_t e = hipMemcpy2DAsync(host, w, dev + kGeoOff - 1, (size_t)g.sy * sizeof(double), w,
+                                        (size_t)(g.ny + 2) * planes, hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) throw HipError(std::string("copy_field_out: ") + hipGetErrorString(e));
 }
 
 inline unsigned int ceil_div(long long a, long long b) { return (unsigned int)((a + b - 1) / b); }

@@ -17,8 +17,9 @@ struct CpK {
   double Az, dz, visc, min_depth, dt;
 };
 
+// hmod (optional) receives 1 at the top cells whose head the exchange changes.
 __global__ __launch_bounds__(NT) void k_exchange(CpK P, double* eta, const double* bottom, double* dept, double* wc,
-                                                 double* h, double* q_out, double* part) {
+                                                 double* h, unsigned char* hmod, double* q_out, double* part) {
   __shared__ double lds[NT / kWave];
   const int i = blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -52,7 +53,9 @@ __global__ __launch_bounds__(NT) void k_exchange(CpK P, double* eta, const doubl
     double wn = wc[ti] - qv * P.dt / V;
     wn = fmin(fmax(wn, s.theta_r), s.theta_s);
     wc[ti] = wn;
-    h[ti] = vg_head(s, wn);
+    const double hnew = vg_head(s, wn);
+    if (hmod && hnew != htop) hmod[ti] = 1;
+    h[ti] = hnew;
     qdt = qv * P.dt;
   }
   const double sblk = block_sum<NT>(qdt, lds);
@@ -134,7 +137,7 @@ double Coupler::exchange(SweState& swe, ReState& re, const frehg_coupling_params
   const unsigned int nb = grid.x * grid.y;
   ensure(nb, (long long)gs.nx * gs.ny);
   hipLaunchKernelGGL(k_exchange, grid, blk, 0, s_, P, swe.d("eta"), swe.d("bottom"), swe.d("dept"), re.d_wc(),
-                     re.d_h(), q_out ? qdev_ : nullptr, part_);
+                     re.d_h(), re.d_hmod(), q_out ? qdev_ : nullptr, part_);
   FREHG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kRedThreads), 0, s_, (const double*)part_, nb, 1, dres_);
   FREHG_LAUNCH_CHECK();

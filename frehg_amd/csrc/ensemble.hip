@@ -144,23 +144,17 @@ __global__ __launch_bounds__(NT) void kb_post(const EnsBox* boxes, const unsigne
   if (q.t.valid) re_post(q.P, q.t, F.h + q.off, F.kr + q.off, F.ch + q.off, F.wch + q.off);
 }
 
-__global__ __launch_bounds__(NT) void kb_flux(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F) {
-  const BoxCell q = box_cell(boxes, t0, nbox);
-  if (!q.t.valid) return;
-  const long long o = q.off;
-  re_flux(q.P, q.t, F.h + o, F.kr + o, F.wcn + o, F.Kx + o, F.Ky + o, F.Kz + o, F.qx + o, F.qy + o, F.qz + o);
-}
-
-__global__ __launch_bounds__(NT) void kb_correct(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F,
-                                                  unsigned char* sat0, unsigned char* cand, double* part,
-                                                  unsigned int ntile) {
+// Fluxes at h^{n+1} and the corrector in one pass (re_flux_correct).
+__global__ __launch_bounds__(NT) void kb_flux_correct(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F,
+                                                       unsigned char* sat0, unsigned char* cand, double* part,
+                                                       unsigned int ntile) {
   __shared__ double lds[NT / kWave];
   const BoxCell q = box_cell(boxes, t0, nbox);
   double a = 0.0, b = 0.0, d = 0.0;
   if (q.t.valid) {
     const long long o = q.off;
-    re_correct(q.P, q.t, F.h + o, F.hn + o, F.wcn + o, F.qx + o, F.qy + o, F.qz + o, F.wch + o, F.wc + o, F.hwc + o,
-               sat0 + o, cand + o, a, b, d);
+    re_flux_correct(q.P, q.t, F.h + o, F.kr + o, F.hn + o, F.wcn + o, F.wch + o, F.Kx + o, F.Ky + o, F.Kz + o,
+                    F.qx + o, F.qy + o, F.qz + o, F.wc + o, F.hwc + o, sat0 + o, cand + o, a, b, d);
   }
   a = block_sum<NT>(a, lds);
   b = block_sum<NT>(b, lds);
@@ -201,7 +195,7 @@ __global__ __launch_bounds__(NT) void kb_finalize(const EnsBox* boxes, const uns
   if (q.t.valid) {
     const long long o = q.off;
     re_finalize(q.P, q.t, F.Kx + o, F.Ky + o, sat0 + o, fin + o, F.wch + o, F.hwc + o, F.hn + o, F.wc + o, F.h + o,
-                a, b, dh);
+                nullptr, a, b, dh);
   }
   a = block_sum<NT>(a, lds);
   b = block_sum<NT>(b, lds);
@@ -622,7 +616,7 @@ Ensemble::Ensemble(int nbox, const int32_t* dims, const frehg_re_params* params,
                                    p.soil.Ks_z, p.soil.Ss, p.soil.use_vg, p.soil.use_mvg, p.soil.air_entry);
     FREHG_HIP(hipMalloc(&classes_[b], sizeof(SoilClass)));
     FREHG_HIP(hipMemcpy(classes_[b], &sc, sizeof(SoilClass), hipMemcpyHostToDevice));
-    B.P = redev::make_rek(g, SoilView{classes_[b], nullptr, nullptr}, p, nullptr);
+    B.P = redev::make_rek(g, SoilView{classes_[b], nullptr, nullptr, sc}, p, nullptr);
   }
   ntile_ = tile0_[nbox];
   // pools
@@ -754,7 +748,7 @@ void Ensemble::set_soil(int box, const frehg_soil_params* classes, int ncls, con
     FREHG_HIP(hipMalloc(&ks_[box], pk.size() * sizeof(double)));
     FREHG_HIP(hipMemcpy(ks_[box], pk.data(), pk.size() * sizeof(double), hipMemcpyHostToDevice));
   }
-  B.P.soil = SoilView{classes_[box], cls_[box], ks_[box]};
+  B.P.soil = SoilView{classes_[box], cls_[box], ks_[box], hc[0]};
   upload_boxes();
 }
 
@@ -803,8 +797,7 @@ double* Ensemble::field(const std::string& n) {
 void Ensemble::get_field(int box, const std::string& name, double* dst) {
   if (box < 0 || box >= nbox_) throw ArgError("frehg_ens_get_field: box out of range");
   const EnsBox& B = hbox_[box];
-  FREHG_HIP(hipMemcpyAsync(dst, field(name) + B.off, (size_t)B.P.g.ncell() * sizeof(double), hipMemcpyDeviceToHost,
-                           s_));
+  copy_field_out(B.P.g, B.P.g.nz + 2, dst, field(name) + B.off, s_);
   FREHG_HIP(hipStreamSynchronize(s_));
 }
 
@@ -895,8 +888,7 @@ void Ensemble::enq_update(int sweeps) {
     hipLaunchKernelGGL(kb_take_x, dim3(nspmv_), blk, 0, s_, bx, (const unsigned int*)dspmv0_, nbox_, F_.h,
                        (const double*)F_.xa, ctl_);
   hipLaunchKernelGGL(kb_post, grid, blk, 0, s_, bx, t0, nbox_, P);
-  hipLaunchKernelGGL(kb_flux, grid, blk, 0, s_, bx, t0, nbox_, P);
-  hipLaunchKernelGGL(kb_correct, grid, blk, 0, s_, bx, t0, nbox_, P, F_.sat0, F_.cand, part_, ntile_);
+  hipLaunchKernelGGL(kb_flux_correct, grid, blk, 0, s_, bx, t0, nbox_, P, F_.sat0, F_.cand, part_, ntile_);
   hipLaunchKernelGGL(kb_clear_flags, dim3(1), dim3(64), 0, s_, flags_ + 3, kMaxSweepFlags);
   for (int sw = 0; sw < sweeps; ++sw) {
     // sweep 0 reads sat0 directly; later sweeps ping-pong between satA and satB

@@ -237,110 +237,128 @@ __global__ __launch_bounds__(NT) void k_spmv(Geo g, int kchunk, StencilView<T> A
   if (threadIdx.x == 0) part[block_id()] = s;
 }
 
-// CG SpMV on 64 x TYL tiles, 3D. Same arithmetic as k_spmv (identical results), but each
-// p = z + beta p_prev is formed once per cell: x-neighbours come from the adjacent lanes of the
-// wave (the two edge lanes load theirs), y-neighbours from the adjacent waves through a
-// double-buffered LDS tile whose halo rows the first and last wave load, and the z-neighbours
-// from the registers of the z-march. That leaves six full-wave loads per cell and layer
-// (z and p_prev of the layer below, D, ax, ay, az), prefetched one layer ahead.
+// Wave-wide lane shifts (DPP wave_shr:1 / wave_shl:1): lane l receives the value of lane l - 1
+// (shr) or l + 1 (shl); the lane without a source receives 0.
+__device__ __forceinline__ double lane_from_left(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_from_right(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Two horizontally adjacent cells (i even), loaded as one aligned 2 x sizeof(T) access.
 template <typename T>
-struct SpmvTileLayer {
-  T zzp, pzp, d, axc, ayc, azc;  // next-layer centre; this layer's coefficients
-  T ze, pe, axe;                 // x-edge lanes: z, p_prev at c -/+ 1 and ax(c - 1)
-  T zh, ph, ayh;                 // halo waves: z, p_prev at c -/+ sy and ay(c - sy)
+struct alignas(2 * sizeof(T)) Pair {
+  T a, b;
+};
+template <typename T>
+__device__ __forceinline__ Pair<T> ld_pair(const T* __restrict__ v, long long c) {
+  return *reinterpret_cast<const Pair<T>*>(v + c);
+}
+
+// Operands of one layer of the paired z-march: pairs of z / p_prev at the y-neighbours and the
+// cell below, the pair's coefficients, and (edge lanes only) z, p_prev, ax at the x-edge cell.
+template <typename T>
+struct SpmvPairLayer {
+  Pair<T> zym, pym, zyp, pyp, zzp, pzp, d, ax, ay, aym, az;
+  T ze, pe, axe;
 };
 
-template <typename T, int TYL>
-__global__ __launch_bounds__(64 * TYL) void k_spmv_tile(Geo g, int kchunk, StencilView<T> A, const T* __restrict__ z,
-                                                         const T* __restrict__ pold, T* __restrict__ pnew,
-                                                         T* __restrict__ q, double* part, const KrylovCtl* ctl) {
-  constexpr int NTL = 64 * TYL;
-  __shared__ double ptile[2][TYL + 2][64];
-  __shared__ double aytile[2][TYL + 1][64];
-  __shared__ double lds[NTL / kWave];
+template <typename T>
+__device__ __forceinline__ void spmv_pair_load(const Geo& g, const StencilView<T>& A, const T* __restrict__ z,
+                                               const T* __restrict__ pold, long long c, long long ce, bool edge,
+                                               SpmvPairLayer<T>& v) {
+  v.zym = ld_pair(z, c - g.sy);
+  v.pym = ld_pair(pold, c - g.sy);
+  v.zyp = ld_pair(z, c + g.sy);
+  v.pyp = ld_pair(pold, c + g.sy);
+  v.d = ld_pair(A.D, c);
+  v.ax = ld_pair(A.ax, c);
+  v.ay = ld_pair(A.ay, c);
+  v.aym = ld_pair(A.ay, c - g.sy);
+  v.zzp = ld_pair(z, c + g.sz);
+  v.pzp = ld_pair(pold, c + g.sz);
+  v.az = ld_pair(A.az, c);
+  if (edge) {
+    v.ze = z[ce];
+    v.pe = pold[ce];
+    v.axe = A.ax[ce];
+  }
+}
+
+// CG SpMV, 3D, two cells per lane on 128 x 4 tiles (one row of 128 cells per wave). Rows start
+// on 128-byte boundaries, so every full-wave load is one aligned 16-byte (FP64) or 8-byte (FP32)
+// access per lane; the x-neighbours of a lane's pair come from the adjacent lanes by wave shifts
+// (lane 0 loads cell i0 - 1, lane 63 cell i0 + 128), the z-neighbours from the registers of the
+// z-march, the y-neighbours from L1/L2. Per cell the arithmetic is that of k_spmv. Lanes past the
+// row end load the last in-row pair and store nothing.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_spmv_pair(Geo g, int kchunk, StencilView<T> A, const T* __restrict__ z,
+                                                   const T* __restrict__ pold, T* __restrict__ pnew,
+                                                   T* __restrict__ q, double* part, const KrylovCtl* ctl) {
+  __shared__ double lds[NT / kWave];
   if (ctl->done) return;  // uniform: every thread reads the same flag
   const double beta = ctl->beta;
   const TileId b = xcd_tile();
-  const int lane = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int i = b.x * 64 + lane, j = b.y * TYL + ty;
+  const int lane = threadIdx.x & 63;
+  const int i0 = b.x * 128, i = i0 + 2 * lane, j = b.y * TY + (threadIdx.x >> 6);
   const int k0 = b.z * kchunk, k1 = min(g.nz, k0 + kchunk);
-  const bool valid = i < g.nx && j < g.ny;
-  const bool first_row = ty == 0, last_row = ty == TYL - 1;
-  auto P = [&](T zz, T pp) -> double { return (double)(T)((double)zz + beta * (double)pp); };
-  long long c = valid ? g.idx(i, j, k0) : 0;
-  auto load = [&](long long cc, SpmvTileLayer<T>& v) {
-    if (!valid) return;
-    v.zzp = z[cc + g.sz];
-    v.pzp = pold[cc + g.sz];
-    v.d = A.D[cc];
-    v.axc = A.ax[cc];
-    v.ayc = A.ay[cc];
-    v.azc = A.az[cc];
-    if (lane == 0 || lane == 63) {
-      const long long ce = lane == 0 ? cc - 1 : cc + 1;
-      v.ze = z[ce];
-      v.pe = pold[ce];
+  double acc = 0.0;
+  if (j < g.ny) {  // whole wave
+    const bool va = i < g.nx, vb = i + 1 < g.nx;
+    long long c = g.idx(min(i, g.nx & ~1), j, k0);
+    const bool edge = lane == 0 || lane == 63;
+    const long long dce = (lane == 0 ? g.idx(i0 - 1, j, k0) : g.idx(min(i0 + 128, g.nx), j, k0)) - c;
+    auto P = [&](T zz, T pp) -> double { return (double)(T)((double)zz + beta * (double)pp); };
+    const Pair<T> zm = ld_pair(z, c - g.sz), pom = ld_pair(pold, c - g.sz), azm2 = ld_pair(A.az, c - g.sz);
+    const Pair<T> z0 = ld_pair(z, c), p0 = ld_pair(pold, c);
+    double pma = P(zm.a, pom.a), pmb = P(zm.b, pom.b);
+    double azma = (double)azm2.a, azmb = (double)azm2.b;
+    double pa = P(z0.a, p0.a), pb = P(z0.b, p0.b);
+    SpmvPairLayer<T> cur, nxt;
+    spmv_pair_load<T>(g, A, z, pold, c, c + dce, edge, cur);
+    for (int k = k0; k < k1; ++k) {
+      const bool more = k + 1 < k1;
+      if (more) spmv_pair_load<T>(g, A, z, pold, c + g.sz, c + g.sz + dce, edge, nxt);
+      const double ppa = P(cur.zzp.a, cur.pzp.a), ppb = P(cur.zzp.b, cur.pzp.b);
+      const double pe = edge ? P(cur.ze, cur.pe) : 0.0;
+      const double from_l = lane_from_left(pb), from_r = lane_from_right(pa);
+      const double ax_l = lane_from_left((double)cur.ax.b);
+      const double pxm = lane == 0 ? pe : from_l;              // p(i - 1)
+      const double pxp = lane == 63 ? pe : from_r;             // p(i + 2)
+      const double axm = lane == 0 ? (double)cur.axe : ax_l;   // ax(i - 1)
+      double ya = (double)cur.d.a * pa + (double)cur.ax.a * pb + axm * pxm +
+                  (double)cur.ay.a * P(cur.zyp.a, cur.pyp.a) + (double)cur.aym.a * P(cur.zym.a, cur.pym.a);
+      ya += (double)cur.az.a * ppa + azma * pma;
+      double yb = (double)cur.d.b * pb + (double)cur.ax.b * pxp + (double)cur.ax.a * pa +
+                  (double)cur.ay.b * P(cur.zyp.b, cur.pyp.b) + (double)cur.aym.b * P(cur.zym.b, cur.pym.b);
+      yb += (double)cur.az.b * ppb + azmb * pmb;
+      const T ya_t = (T)ya, yb_t = (T)yb;
+      if (vb) {
+        *reinterpret_cast<Pair<T>*>(pnew + c) = Pair<T>{(T)pa, (T)pb};
+        *reinterpret_cast<Pair<T>*>(q + c) = Pair<T>{ya_t, yb_t};
+        acc += pa * (double)ya_t;
+        acc += pb * (double)yb_t;
+      } else if (va) {
+        pnew[c] = (T)pa;
+        q[c] = ya_t;
+        acc += pa * (double)ya_t;
+      }
+      pma = pa;
+      pmb = pb;
+      pa = ppa;
+      pb = ppb;
+      azma = (double)cur.az.a;
+      azmb = (double)cur.az.b;
+      c += g.sz;
+      if (more) cur = nxt;
     }
-    if (lane == 0) v.axe = A.ax[cc - 1];
-    if (first_row) {
-      v.zh = z[cc - g.sy];
-      v.ph = pold[cc - g.sy];
-      v.ayh = A.ay[cc - g.sy];
-    } else if (last_row) {
-      v.zh = z[cc + g.sy];
-      v.ph = pold[cc + g.sy];
-    }
-  };
-  double pm = 0.0, azm = 0.0, pc = 0.0, acc = 0.0;
-  if (valid) {
-    pm = P(z[c - g.sz], pold[c - g.sz]);
-    azm = (double)A.az[c - g.sz];
-    pc = P(z[c], pold[c]);
   }
-  SpmvTileLayer<T> cur{}, nxt{};
-  load(c, cur);
-  int buf = 0;
-  for (int k = k0; k < k1; ++k) {
-    const bool more = k + 1 < k1;
-    if (more) load(c + g.sz, nxt);
-    ptile[buf][ty + 1][lane] = pc;
-    aytile[buf][ty + 1][lane] = (double)cur.ayc;
-    if (first_row) {
-      ptile[buf][0][lane] = valid ? P(cur.zh, cur.ph) : 0.0;
-      aytile[buf][0][lane] = (double)cur.ayh;
-    } else if (last_row) {
-      ptile[buf][TYL + 1][lane] = valid ? P(cur.zh, cur.ph) : 0.0;
-    }
-    double pl = __shfl_up(pc, 1, 64);
-    double pr = __shfl_down(pc, 1, 64);
-    double axl = __shfl_up((double)cur.axc, 1, 64);
-    if (lane == 0) {
-      pl = valid ? P(cur.ze, cur.pe) : 0.0;
-      axl = (double)cur.axe;
-    } else if (lane == 63) {
-      pr = valid ? P(cur.ze, cur.pe) : 0.0;
-    }
-    // LDS-only barrier: waits for this wave's LDS writes, not for its global loads in flight
-    // (__syncthreads would also drain the prefetched next layer).
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-    const double pym = ptile[buf][ty][lane], pyp = ptile[buf][ty + 2][lane], aym = aytile[buf][ty][lane];
-    if (valid) {
-      const double pp = P(cur.zzp, cur.pzp);
-      double y = (double)cur.d * pc + (double)cur.axc * pr + axl * pl + (double)cur.ayc * pyp + aym * pym;
-      y += (double)cur.azc * pp + azm * pm;
-      const T yT = (T)y;
-      pnew[c] = (T)pc;
-      q[c] = yT;
-      acc += pc * (double)yT;
-      pm = pc;
-      pc = pp;
-      azm = (double)cur.azc;
-    }
-    c += g.sz;
-    buf ^= 1;
-    if (more) cur = nxt;
-  }
-  const double s = block_sum<NTL>(acc, lds);
+  const double s = block_sum<NT>(acc, lds);
   if (threadIdx.x == 0) part[block_id()] = s;
 }
 
@@ -733,15 +751,16 @@ Krylov::Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof)
   // partials sized for the largest grid any kernel (or candidate) uses: 64 x 4 tiles, one layer each
   part_cap_ = ceil_div(g_.nx, TX) * ceil_div(g_.ny, TY) * (unsigned int)(three_d_ ? g_.nz : 1);
   FREHG_HIP(hipMalloc(&part_, 3 * (size_t)part_cap_ * sizeof(double)));
-  for (int prec = 0; prec < 2; ++prec) set_spmv_shape(prec, 0, three_d_ ? std::min(g_.nz, 4) : 1);
-  // FREHG_SPMV_SHAPE="ty:kc" pins the SpMV shape (ty in {0, 4, 8, 16}) and skips the autotuner.
+  for (int prec = 0; prec < 2; ++prec) set_spmv_shape(prec, 1, three_d_ ? std::min(g_.nz, 4) : 1);
+  // FREHG_SPMV_SHAPE="cpl:kc" pins the SpMV shape (cells per lane 1 or 2, kc layers per block)
+  // and skips the autotuner.
   if (const char* env = std::getenv("FREHG_SPMV_SHAPE")) {
-    int ty = -1, kc = -1;
-    if (std::sscanf(env, "%d:%d", &ty, &kc) != 2 || !(ty == 0 || ty == 4 || ty == 8 || ty == 16) || kc < 1)
-      throw ArgError(std::string("FREHG_SPMV_SHAPE must be \"ty:kc\" with ty in {0,4,8,16} and kc >= 1, got \"") +
+    int cpl = -1, kc = -1;
+    if (std::sscanf(env, "%d:%d", &cpl, &kc) != 2 || !(cpl == 1 || cpl == 2) || kc < 1)
+      throw ArgError(std::string("FREHG_SPMV_SHAPE must be \"cpl:kc\" with cpl in {1,2} and kc >= 1, got \"") +
                      env + "\"");
     for (int prec = 0; prec < 2; ++prec) {
-      set_spmv_shape(prec, ty, kc);
+      set_spmv_shape(prec, cpl, kc);
       tuned_[prec] = true;
     }
   }
@@ -765,14 +784,14 @@ Krylov::~Krylov() {
   if (host_ctl_) (void)hipHostFree(host_ctl_);
 }
 
-void Krylov::set_spmv_shape(int prec, int ty, int kc) {
+void Krylov::set_spmv_shape(int prec, int cpl, int kc) {
   const int nz = three_d_ ? g_.nz : 1;
-  if (!three_d_) ty = 0;
+  if (!three_d_) cpl = 1;
+  if (cpl != 1 && cpl != 2) throw ArgError("Krylov: SpMV cells per lane must be 1 or 2");
   SpmvShape& sh = shape_[prec];
-  sh.ty = ty;
+  sh.cpl = cpl;
   sh.kc = std::max(1, std::min(kc, nz));
-  const int tyy = ty == 0 ? TY : ty;
-  sh.grid = dim3(ceil_div(g_.nx, TX), ceil_div(g_.ny, tyy), ceil_div(nz, sh.kc));
+  sh.grid = dim3(ceil_div(g_.nx, TX * cpl), ceil_div(g_.ny, TY), ceil_div(nz, sh.kc));
   sh.nb = sh.grid.x * sh.grid.y * sh.grid.z;
   if (sh.nb > part_cap_) throw HipError("Krylov: SpMV shape needs more partials than allocated");
 }
@@ -784,18 +803,12 @@ void Krylov::launch_spmv(const StencilView<T>& A, const T* z, const T* pold, T* 
   if (!three_d_) {
     hipExtLaunchKernelGGL((k_spmv<T, false>), sh.grid, dim3(NT), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold, pnew, q,
                           part_, ctl);
-  } else if (sh.ty == 0) {
+  } else if (sh.cpl == 1) {
     hipExtLaunchKernelGGL((k_spmv<T, true>), sh.grid, dim3(NT), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold, pnew, q,
                           part_, ctl);
-  } else if (sh.ty == 4) {
-    hipExtLaunchKernelGGL((k_spmv_tile<T, 4>), sh.grid, dim3(64 * 4), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold,
-                          pnew, q, part_, ctl);
-  } else if (sh.ty == 8) {
-    hipExtLaunchKernelGGL((k_spmv_tile<T, 8>), sh.grid, dim3(64 * 8), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold,
-                          pnew, q, part_, ctl);
   } else {
-    hipExtLaunchKernelGGL((k_spmv_tile<T, 16>), sh.grid, dim3(64 * 16), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold,
-                          pnew, q, part_, ctl);
+    hipExtLaunchKernelGGL((k_spmv_pair<T>), sh.grid, dim3(NT), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold, pnew, q,
+                          part_, ctl);
   }
   FREHG_LAUNCH_CHECK();
 }
@@ -818,11 +831,11 @@ void Krylov::autotune(const StencilView<T>& A) {
   std::vector<int> kcs;
   for (int kc : {1, 2, 4, 8, 15, nz})
     if (kc >= 1 && kc <= nz && std::find(kcs.begin(), kcs.end(), kc) == kcs.end()) kcs.push_back(kc);
-  int best_ty = shape_[prec].ty, best_kc = shape_[prec].kc;
+  int best_cpl = shape_[prec].cpl, best_kc = shape_[prec].kc;
   float best_ms = 1e30f;
-  for (int ty : {0, 4, 8, 16}) {
+  for (int cpl : {1, 2}) {
     for (int kc : kcs) {
-      set_spmv_shape(prec, ty, kc);
+      set_spmv_shape(prec, cpl, kc);
       float ms = 0.f;
       for (int rep = 0; rep < 2; ++rep) {  // the first round warms caches and clocks
         FREHG_HIP(hipEventRecord(e0, s_));
@@ -833,12 +846,12 @@ void Krylov::autotune(const StencilView<T>& A) {
       }
       if (ms < best_ms) {
         best_ms = ms;
-        best_ty = ty;
+        best_cpl = cpl;
         best_kc = kc;
       }
     }
   }
-  set_spmv_shape(prec, best_ty, best_kc);
+  set_spmv_shape(prec, best_cpl, best_kc);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
 }

@@ -91,16 +91,16 @@ class Krylov {
                     const double* bn, double* x, const unsigned char* fixed, bool x0_zero,
                     bool guard_guess, const KrylovConfig& cfg);
 
-  // CG SpMV launch shape per precision (0 FP64, 1 FP32): tile height ty (0 = one column per
-  // thread, neighbours from L1/L2; 4/8/16 = 64 x ty tiles whose y-halo is staged in LDS and
-  // whose x-neighbours come from wave shuffles) and layers per block of the z-march (kc).
+  // CG SpMV launch shape per precision (0 FP64, 1 FP32): cells per lane (1: one column per
+  // thread on 64 x 4 tiles; 2: two adjacent columns per lane on 128 x 4 tiles, paired aligned
+  // loads, x-neighbours by wave shifts) and layers per block of the z-march (kc).
   struct SpmvShape {
-    int ty = 0, kc = 1;
+    int cpl = 1, kc = 1;
     dim3 grid;
     unsigned int nb = 0;
   };
   const SpmvShape& spmv_shape(int prec) const { return shape_[prec]; }
-  void set_spmv_shape(int prec, int ty, int kc);
+  void set_spmv_shape(int prec, int cpl, int kc);
   // Times the CG SpMV for every candidate shape on the operator at hand and keeps the fastest
   // (run automatically at the first 3D solve of each precision; cached per instance).
   template <typename T>

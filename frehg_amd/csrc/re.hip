@@ -5,15 +5,17 @@
 //
 // Kernels per step (one thread per interior cell unless stated):
 //   pre        kr = K/Ks and C at h^n, incl. the face-adjacent ghost cells          (:199-215)
+//              (after a step: only the cells whose head changed since post, see hmod)
 //   assemble   face conductivities -> 7-point coefficients, Gct and Grhs          (:217-549)
 //   [Krylov]   Jacobi-PCG, x0 = 0 / h^n / learned                                   (:560-636)
 //   post       ghost heads, kr and C at h^{n+1}                                     (:638-669)
-//   flux       faces at h^{n+1}, Darcy fluxes qx qy qz                               (:671-743)
-//   correct    mass-conservative theta, wch/hwc, saturation flags                   (:762-786)
+//   flux+corr  faces at h^{n+1}, Darcy fluxes qx qy qz, mass-conservative theta,
+//              wch/hwc, saturation flags                                 (:671-743, :762-786)
 //   realloc    monotone fixed point of the sequential saturation sweep              (:788-837)
 //   finalize   reallocation, clamp, moisture ghosts                                 (:788-878)
 //   dtadapt    flux-imbalance max and Courant min (adaptive dt only)                (:880-929)
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -30,39 +32,46 @@ namespace {
 using namespace redev;
 constexpr int NT = kReNT;
 
-__global__ __launch_bounds__(NT) void k_re_pre(ReK P, const double* hn, double* kr, double* ch) {
+template <class K>
+__global__ __launch_bounds__(NT) void k_re_pre(K P, const double* __restrict__ hn, double* __restrict__ kr, double* __restrict__ ch) {
   const Cell t = cell_of(P.g);
   if (t.valid) re_pre(P, t, hn, kr, ch);
 }
 
-__global__ __launch_bounds__(NT) void k_re_assemble(ReK P, const double* kr, const double* ch, const double* wcn,
-                                                     const double* hn, double* D, double* ax, double* ay, double* az,
-                                                     double* rhs, float* D32, float* ax32, float* ay32, float* az32) {
+template <class K>
+__global__ __launch_bounds__(NT) void k_re_pre_stale(K P, const unsigned char* __restrict__ hmod, const double* __restrict__ hn, double* __restrict__ kr,
+                                                      double* __restrict__ ch) {
+  const Cell t = cell_of(P.g);
+  if (t.valid) re_pre_stale(P, t, hmod, hn, kr, ch);
+}
+
+template <class K>
+__global__ __launch_bounds__(NT) void k_re_assemble(K P, const double* __restrict__ kr, const double* __restrict__ ch, const double* __restrict__ wcn,
+                                                     const double* __restrict__ hn, double* __restrict__ D, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+                                                     double* __restrict__ rhs, float* __restrict__ D32, float* __restrict__ ax32, float* __restrict__ ay32, float* __restrict__ az32) {
   const Cell t = cell_of(P.g);
   if (t.valid) re_assemble(P, t, kr, ch, wcn, hn, D, ax, ay, az, rhs, D32, ax32, ay32, az32);
 }
 
-__global__ __launch_bounds__(NT) void k_re_post(ReK P, double* h, double* kr, double* ch, double* wch) {
+template <class K>
+__global__ __launch_bounds__(NT) void k_re_post(K P, double* __restrict__ h, double* __restrict__ kr, double* __restrict__ ch, double* __restrict__ wch) {
   const Cell t = cell_of(P.g);
   if (t.valid) re_post(P, t, h, kr, ch, wch);
 }
 
-__global__ __launch_bounds__(NT) void k_re_flux(ReK P, const double* h, const double* kr, const double* wcn, double* Kx,
-                                                 double* Ky, double* Kz, double* qx, double* qy, double* qz) {
-  const Cell t = cell_of(P.g);
-  if (t.valid) re_flux(P, t, h, kr, wcn, Kx, Ky, Kz, qx, qy, qz);
-}
-
-// Corrector; ledger partials (sum theta^n V, specific-storage term, boundary inflow) per block to
-// part[0..2][nblocks].
-__global__ __launch_bounds__(NT) void k_re_correct(ReK P, const double* h, const double* hn, const double* wcn,
-                                                    const double* qx, const double* qy, const double* qz,
-                                                    const double* wch, double* wc, double* hwc, unsigned char* sat0,
-                                                    unsigned char* cand, double* part, unsigned int nblocks) {
+// Fluxes + corrector (re_flux_correct); ledger partials (sum theta^n V, specific-storage term,
+// boundary inflow) per block to part[0..2][nblocks].
+template <class K>
+__global__ __launch_bounds__(NT) void k_re_flux_correct(K P, const double* __restrict__ h, const double* __restrict__ kr, const double* __restrict__ hn,
+                                                         const double* __restrict__ wcn, const double* __restrict__ wch, double* __restrict__ Kx, double* __restrict__ Ky,
+                                                         double* __restrict__ Kz, double* __restrict__ qx, double* __restrict__ qy, double* __restrict__ qz, double* __restrict__ wc,
+                                                         double* __restrict__ hwc, unsigned char* __restrict__ sat0, unsigned char* __restrict__ cand,
+                                                         double* __restrict__ part, unsigned int nblocks) {
   __shared__ double lds[NT / kWave];
   const Cell t = cell_of(P.g);
   double l_before = 0.0, l_ss = 0.0, l_bnd = 0.0;
-  if (t.valid) re_correct(P, t, h, hn, wcn, qx, qy, qz, wch, wc, hwc, sat0, cand, l_before, l_ss, l_bnd);
+  if (t.valid)
+    re_flux_correct(P, t, h, kr, hn, wcn, wch, Kx, Ky, Kz, qx, qy, qz, wc, hwc, sat0, cand, l_before, l_ss, l_bnd);
   const double a = block_sum<NT>(l_before, lds);
   const double b = block_sum<NT>(l_ss, lds);
   const double d = block_sum<NT>(l_bnd, lds);
@@ -76,25 +85,27 @@ __global__ __launch_bounds__(NT) void k_re_correct(ReK P, const double* h, const
 
 // One Jacobi sweep of the monotone map whose unique fixed point is the sequential sweep's
 // saturation state (see re_realloc).
-__global__ __launch_bounds__(NT) void k_re_realloc_iter(ReK P, const double* Kx, const double* Ky,
-                                                         const unsigned char* sat0, const unsigned char* cand,
-                                                         const unsigned char* prev, unsigned char* next, int* changed) {
+template <class K>
+__global__ __launch_bounds__(NT) void k_re_realloc_iter(K P, const double* __restrict__ Kx, const double* __restrict__ Ky,
+                                                         const unsigned char* __restrict__ sat0, const unsigned char* __restrict__ cand,
+                                                         const unsigned char* __restrict__ prev, unsigned char* __restrict__ next, int* changed) {
   const Cell t = cell_of(P.g);
   if (t.valid) re_realloc(P, t, Kx, Ky, sat0, cand, prev, next, changed);
 }
 
 // Reallocation + clamp + moisture BCs; ledger partials (sum theta^{n+1} V, sum (theta^{n+1} -
 // theta_c) V) to part[3..4][nblocks] and max |h^{n+1} - h^n| to mx[nblocks].
-__global__ __launch_bounds__(NT) void k_re_finalize(ReK P, const double* Kx, const double* Ky,
-                                                     const unsigned char* sat0, const unsigned char* fin,
-                                                     const double* wch, const double* hwc, const double* hn,
-                                                     double* wc, double* h, double* part, double* mx,
-                                                     unsigned int nblocks) {
+template <class K>
+__global__ __launch_bounds__(NT) void k_re_finalize(K P, const double* __restrict__ Kx, const double* __restrict__ Ky,
+                                                     const unsigned char* __restrict__ sat0, const unsigned char* __restrict__ fin,
+                                                     const double* __restrict__ wch, const double* __restrict__ hwc, const double* __restrict__ hn,
+                                                     double* __restrict__ wc, double* __restrict__ h, unsigned char* __restrict__ hmod, double* __restrict__ part,
+                                                     double* __restrict__ mx, unsigned int nblocks) {
   __shared__ double lds[NT / kWave];
   __shared__ double lmax[NT / kWave];
   const Cell t = cell_of(P.g);
   double l_after = 0.0, l_adj = 0.0, dh = 0.0;
-  if (t.valid) re_finalize(P, t, Kx, Ky, sat0, fin, wch, hwc, hn, wc, h, l_after, l_adj, dh);
+  if (t.valid) re_finalize(P, t, Kx, Ky, sat0, fin, wch, hwc, hn, wc, h, hmod, l_after, l_adj, dh);
   const double a = block_sum<NT>(l_after, lds);
   const double b = block_sum<NT>(l_adj, lds);
   dh = wave_max(dh);
@@ -175,7 +186,7 @@ ReState::ReState(int nx, int ny, int nz, const frehg_re_params& p, hipStream_t s
     FREHG_HIP(hipMalloc(f, bytes));
     FREHG_HIP(hipMemsetAsync(*f, 0, bytes, s_));
   }
-  for (unsigned char** f : {&sat0_, &cand_, &satA_, &satB_}) {
+  for (unsigned char** f : {&sat0_, &cand_, &satA_, &satB_, &hmod_}) {
     FREHG_HIP(hipMalloc(f, (size_t)n_));
     FREHG_HIP(hipMemsetAsync(*f, 0, (size_t)n_, s_));
   }
@@ -185,6 +196,7 @@ ReState::ReState(int nx, int ny, int nz, const frehg_re_params& p, hipStream_t s
   FREHG_HIP(hipMalloc(&red_, 2 * (size_t)nb * sizeof(double)));
   FREHG_HIP(hipMalloc(&led_, 7 * (size_t)nb * sizeof(double)));
   FREHG_HIP(hipMalloc(&dres_, 16 * sizeof(double)));
+  FREHG_HIP(hipMalloc(&red2_, 5 * kRedStage * sizeof(double)));
   FREHG_HIP(hipHostMalloc(&hres_, 16 * sizeof(double), hipHostMallocDefault));
   FREHG_HIP(hipMalloc(&dcount_, 16));
   FREHG_HIP(hipMemsetAsync(dcount_, 0, 16, s_));
@@ -199,8 +211,8 @@ ReState::~ReState() {
   for (void* f : {(void*)h_, (void*)hn_, (void*)wc_, (void*)wcn_, (void*)wch_, (void*)hwc_, (void*)ch_,
                   (void*)Kx_, (void*)Ky_, (void*)Kz_, (void*)qx_, (void*)qy_, (void*)qz_, (void*)kr_,
                   (void*)D_, (void*)ax_, (void*)ay_, (void*)az_, (void*)rhs_, (void*)D32_, (void*)ax32_,
-                  (void*)ay32_, (void*)az32_, (void*)sat0_, (void*)cand_, (void*)satA_, (void*)satB_,
-                  (void*)dflag_, (void*)red_, (void*)led_, (void*)dres_, (void*)dcount_, (void*)classes_, (void*)cls_,
+                  (void*)ay32_, (void*)az32_, (void*)sat0_, (void*)cand_, (void*)satA_, (void*)satB_, (void*)hmod_,
+                  (void*)dflag_, (void*)red_, (void*)red2_, (void*)led_, (void*)dres_, (void*)dcount_, (void*)classes_, (void*)cls_,
                   (void*)ks_, (void*)qtop_})
     if (f) (void)hipFree(f);
   if (hflag_) (void)hipHostFree(hflag_);
@@ -228,12 +240,14 @@ void ReState::set_params(const frehg_re_params& p) {
   }
   const bool first = (dtn_ == 0.0);
   p_ = p;
+  kc_fresh_ = false;
   if (first) dtn_ = p.dt;
 }
 
 void ReState::set_soil(const frehg_soil_params* classes, int ncls, const int32_t* class_idx, int idx3d,
                        const double* ks_cell) {
   if (ncls < 1) throw ArgError("frehg_re_set_soil: need at least one soil class");
+  kc_fresh_ = false;
   host_classes_.clear();
   for (int c = 0; c < ncls; ++c) {
     const frehg_soil_params& s = classes[c];
@@ -310,12 +324,13 @@ double* ReState::field_ptr(const std::string& n) {
 }
 
 void ReState::set_field(const std::string& name, const double* src) {
-  FREHG_HIP(hipMemcpyAsync(field_ptr(name), src, (size_t)n_ * sizeof(double), hipMemcpyHostToDevice, s_));
+  kc_fresh_ = false;
+  copy_field_in(g_, g_.nz + 2, field_ptr(name), src, s_);
   FREHG_HIP(hipStreamSynchronize(s_));
 }
 
 void ReState::get_field(const std::string& name, double* dst) {
-  FREHG_HIP(hipMemcpyAsync(dst, field_ptr(name), (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  copy_field_out(g_, g_.nz + 2, dst, field_ptr(name), s_);
   FREHG_HIP(hipStreamSynchronize(s_));
 }
 
@@ -324,6 +339,7 @@ void ReState::set_initial(const double* h_int, const double* wc_int) {
   if ((h_int == nullptr) == (wc_int == nullptr))
     throw ArgError("frehg_re_set_initial: pass exactly one of h / wc");
   const double* src = h_int ? h_int : wc_int;
+  kc_fresh_ = false;
   std::vector<double> pad((size_t)n_, 0.0);
   FREHG_HIP(hipMemcpy(pad.data(), h_int ? h_ : wc_, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost));
   for (int k = 0; k < g_.nz; ++k)
@@ -341,9 +357,10 @@ void ReState::set_initial(const double* h_int, const double* wc_int) {
 
 void ReState::learned_increment(double* dst) {
   if (!mlp_ || !mlp_->ready()) throw ArgError("frehg_mlp_increment: no trained model");
+  kc_fresh_ = false;  // kr and C are re-evaluated at h below
   const ReK P = make_rek(g_, soil_view(), p_, qtop_);
   const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4), g_.nz);
-  hipLaunchKernelGGL(k_re_pre, grid, dim3(NT), 0, s_, P, (const double*)h_, kr_, ch_);
+  hipLaunchKernelGGL(k_re_pre<ReK>, grid, dim3(NT), 0, s_, P, (const double*)h_, kr_, ch_);
   FREHG_LAUNCH_CHECK();
   double* tmp = nullptr;
   FREHG_HIP(hipMalloc(&tmp, (size_t)n_ * sizeof(double)));
@@ -384,14 +401,24 @@ void ReState::step(frehg_re_stats* st) {
   // no kernel reads or writes, hold the same values in both buffers).
   std::swap(h_, hn_);
   std::swap(wc_, wcn_);
+  const bool kc_fresh = kc_fresh_;
+  kc_fresh_ = false;  // set again once this step has completed
   {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_ASSEMBLE);
     hipEvent_t ev1 = prof_->stop();
-    hipExtLaunchKernelGGL(k_re_pre, grid, blk, 0, s_, ev0, nullptr, 0, P, hn_, kr_, ch_);
-    FREHG_LAUNCH_CHECK();
-    hipExtLaunchKernelGGL(k_re_assemble, grid, blk, 0, s_, nullptr, ev1, 0, P, kr_, ch_, wcn_, hn_, D_, ax_, ay_,
-                          az_, rhs_, mixed ? D32_ : nullptr, ax32_, ay32_, az32_);
-    FREHG_LAUNCH_CHECK();
+    with_soil_variant(P, [&](const auto& PK) {
+      using K = std::decay_t<decltype(PK)>;
+      if (kc_fresh)
+        hipExtLaunchKernelGGL(k_re_pre_stale<K>, grid, blk, 0, s_, ev0, nullptr, 0, PK, (const unsigned char*)hmod_,
+                              (const double*)hn_, kr_, ch_);
+      else
+        hipExtLaunchKernelGGL(k_re_pre<K>, grid, blk, 0, s_, ev0, nullptr, 0, PK, (const double*)hn_, kr_, ch_);
+      FREHG_LAUNCH_CHECK();
+      hipExtLaunchKernelGGL(k_re_assemble<K>, grid, blk, 0, s_, nullptr, ev1, 0, PK, (const double*)kr_,
+                            (const double*)ch_, (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_,
+                            mixed ? D32_ : nullptr, ax32_, ay32_, az32_);
+      FREHG_LAUNCH_CHECK();
+    });
   }
   // Initial guess in h (the solve's x): zero (PETSc default), h^n, or h^n + MLP increment.
   bool x0_zero = true, guard = false;
@@ -421,14 +448,15 @@ void ReState::step(frehg_re_stats* st) {
   {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_CORRECT);
     hipEvent_t ev1 = prof_->stop();
-    hipExtLaunchKernelGGL(k_re_post, grid, blk, 0, s_, ev0, nullptr, 0, P, h_, kr_, ch_, wch_);
-    FREHG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_re_flux, grid, blk, 0, s_, P, h_, kr_, wcn_, Kx_, Ky_, Kz_, qx_, qy_, qz_);
-    FREHG_LAUNCH_CHECK();
-    hipExtLaunchKernelGGL(k_re_correct, grid, blk, 0, s_, nullptr, ev1, 0, P, (const double*)h_, (const double*)hn_,
-                          (const double*)wcn_, (const double*)qx_, (const double*)qy_, (const double*)qz_,
-                          (const double*)wch_, wc_, hwc_, sat0_, cand_, led_, nbl);
-    FREHG_LAUNCH_CHECK();
+    with_soil_variant(P, [&](const auto& PK) {
+      using K = std::decay_t<decltype(PK)>;
+      hipExtLaunchKernelGGL(k_re_post<K>, grid, blk, 0, s_, ev0, nullptr, 0, PK, h_, kr_, ch_, wch_);
+      FREHG_LAUNCH_CHECK();
+      hipExtLaunchKernelGGL(k_re_flux_correct<K>, grid, blk, 0, s_, nullptr, ev1, 0, PK, (const double*)h_,
+                            (const double*)kr_, (const double*)hn_, (const double*)wcn_, (const double*)wch_, Kx_,
+                            Ky_, Kz_, qx_, qy_, qz_, wc_, hwc_, sat0_, cand_, led_, nbl);
+      FREHG_LAUNCH_CHECK();
+    });
   }
   // Reallocation fixed point: iterate from sat0 until the saturation state stops changing.
   FREHG_HIP(hipMemcpyAsync(satA_, sat0_, (size_t)n_, hipMemcpyDeviceToDevice, s_));
@@ -437,18 +465,27 @@ void ReState::step(frehg_re_stats* st) {
   unsigned char* next = satB_;
   for (int it = 0; it < g_.nx * g_.ny * g_.nz + 1; ++it) {
     FREHG_HIP(hipMemsetAsync(dflag_, 0, sizeof(int), s_));
-    hipLaunchKernelGGL(k_re_realloc_iter, grid, blk, 0, s_, P, Kx_, Ky_, sat0_, cand_, prev, next, dflag_);
-    FREHG_LAUNCH_CHECK();
+    with_soil_variant(P, [&](const auto& PK) {
+      using K = std::decay_t<decltype(PK)>;
+      hipLaunchKernelGGL(k_re_realloc_iter<K>, grid, blk, 0, s_, PK, (const double*)Kx_, (const double*)Ky_,
+                         (const unsigned char*)sat0_, (const unsigned char*)cand_, (const unsigned char*)prev, next,
+                         dflag_);
+      FREHG_LAUNCH_CHECK();
+    });
     FREHG_HIP(hipMemcpyAsync(hflag_, dflag_, sizeof(int), hipMemcpyDeviceToHost, s_));
     FREHG_HIP(hipStreamSynchronize(s_));
     std::swap(prev, next);
     if (hflag_[0] == 0) break;
   }
-  hipLaunchKernelGGL(k_re_finalize, grid, blk, 0, s_, P, Kx_, Ky_, sat0_, prev, wch_, hwc_, (const double*)hn_, wc_,
-                     h_, led_, led_ + 5 * (size_t)nbl, nbl);
-  FREHG_LAUNCH_CHECK();
+  with_soil_variant(P, [&](const auto& PK) {
+    using K = std::decay_t<decltype(PK)>;
+    hipLaunchKernelGGL(k_re_finalize<K>, grid, blk, 0, s_, PK, (const double*)Kx_, (const double*)Ky_,
+                       (const unsigned char*)sat0_, (const unsigned char*)prev, (const double*)wch_,
+                       (const double*)hwc_, (const double*)hn_, wc_, h_, hmod_, led_, led_ + 5 * (size_t)nbl, nbl);
+    FREHG_LAUNCH_CHECK();
+  });
   // ledger: [before, storage_ss, boundary_in, after, adjust] sums, then max |dh|
-  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kRedThreads), 0, s_, (const double*)led_, nbl, 5, dres_ + 2);
+  reduce_sums(led_, nbl, 5, red2_, dres_ + 2, s_);
   FREHG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_maxmin, dim3(1), dim3(kRedThreads), 0, s_, (const double*)(led_ + 5 * (size_t)nbl), nbl,
                      dres_ + 7);
@@ -483,6 +520,7 @@ void ReState::step(frehg_re_stats* st) {
       local.mass_after - local.mass_before - local.mass_boundary_in - local.mass_storage_ss - local.mass_adjust;
   local.dt_used = dt_used_;
   local.dt_next = p_.dt;
+  kc_fresh_ = true;
   if (st) *st = local;
 }
 

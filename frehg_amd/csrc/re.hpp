@@ -44,7 +44,10 @@ class ReState {
   double* d_wcn() { return wcn_; }
   double* d_kr() { return kr_; }
   double* d_ch() { return ch_; }
-  SoilView soil_view() const { return SoilView{classes_, cls_, ks_}; }
+  // Per-cell "head changed since kr and C were last evaluated" flags for a kernel that changes h
+  // between steps (the SW exchange), or null when the next step re-evaluates every cell anyway.
+  unsigned char* d_hmod() { return kc_fresh_ ? hmod_ : nullptr; }
+  SoilView soil_view() const { return SoilView{classes_, cls_, ks_, host_classes_[0]}; }
   Krylov& krylov() { return *kry_; }
   // Learned increment h^{n+1} - h^n the model predicts for a step taken from the current state
   // (features from h, K/C at h, theta), interior cells in i + j nx + k nx ny order.
@@ -74,9 +77,16 @@ class ReState {
   float *D32_ = nullptr, *ax32_ = nullptr, *ay32_ = nullptr, *az32_ = nullptr;
   // Saturation-reallocation flags (uint8, padded).
   unsigned char *sat0_ = nullptr, *cand_ = nullptr, *satA_ = nullptr, *satB_ = nullptr;
+  // kr and C at h after a step are the post-solve values (ReSolver.cpp:963) except at cells whose
+  // head the reallocation (or the SW exchange) replaced afterwards; those are flagged in hmod_.
+  // While kc_fresh_ holds, the next step re-evaluates only the flagged cells; any other change of
+  // the state (fields, soil, initial condition, parameters) clears it.
+  unsigned char* hmod_ = nullptr;
+  bool kc_fresh_ = false;
   int* dflag_ = nullptr;   // device: [0] changed flag
   int* hflag_ = nullptr;   // pinned host mirror
   double* red_ = nullptr;  // reduction partials (adaptive dt)
+  double* red2_ = nullptr; // first-stage sums of the ledger reduction
   double* led_ = nullptr;  // water-ledger partials: 5 sums + max|dh| (+ pad) per block
   double* dres_ = nullptr; // device results [dq_max, dt_Comin, max_dh]
   double* hres_ = nullptr;

@@ -13,13 +13,67 @@ namespace redev {
 
 constexpr int kReNT = 256;  // threads per tile: 64 (i) x 4 (j), one layer
 
-struct ReK {
+// Parameters of one box's step. The template arguments select the soil lookup variant (see
+// SoilViewT) and whether lateral flow is on (F3D: 0 off, 1 on, 2 read from full3d at run time);
+// ReK, the runtime-checked variant, is what host code builds and stores.
+template <int CLS, int KS, int F3D = 2>
+struct ReKT {
   Geo g;
-  SoilView soil;
+  SoilViewT<CLS, KS> soil;
   double dx, dy, dz, dt, htop, hbot, qtop, qbot, co_max;
   int full3d, sim_sw, bc4, bc5, use_corrector;
   const double* qtop_field;  // surface padded or null
+  __device__ inline bool full() const {
+    if constexpr (F3D == 2) return full3d != 0;
+    else return F3D == 1;
+  }
 };
+using ReK = ReKT<kRuntime, kRuntime, 2>;
+
+// The same parameters with the soil lookup fixed to the given variant.
+template <int CLS, int KS, int F3D>
+inline ReKT<CLS, KS, F3D> rek_as(const ReK& P) {
+  ReKT<CLS, KS, F3D> Q;
+  Q.g = P.g;
+  Q.soil.classes = P.soil.classes;
+  Q.soil.cls = P.soil.cls;
+  Q.soil.ks = P.soil.ks;
+  Q.soil.c0 = P.soil.c0;
+  Q.dx = P.dx;
+  Q.dy = P.dy;
+  Q.dz = P.dz;
+  Q.dt = P.dt;
+  Q.htop = P.htop;
+  Q.hbot = P.hbot;
+  Q.qtop = P.qtop;
+  Q.qbot = P.qbot;
+  Q.co_max = P.co_max;
+  Q.full3d = P.full3d;
+  Q.sim_sw = P.sim_sw;
+  Q.bc4 = P.bc4;
+  Q.bc5 = P.bc5;
+  Q.use_corrector = P.use_corrector;
+  Q.qtop_field = P.qtop_field;
+  return Q;
+}
+
+// Calls f(Q) with Q = P fixed to the variant matching which soil arrays P has (and, for one soil
+// class, whether lateral flow is on).
+template <typename F>
+inline void with_soil_variant(const ReK& P, F&& f) {
+  const bool c = P.soil.cls != nullptr, k = P.soil.ks != nullptr, f3 = P.full3d != 0;
+  if (!c && k) {
+    if (f3) f(rek_as<kAbsent, kPresent, 1>(P));
+    else f(rek_as<kAbsent, kPresent, 0>(P));
+  } else if (!c) {
+    if (f3) f(rek_as<kAbsent, kAbsent, 1>(P));
+    else f(rek_as<kAbsent, kAbsent, 0>(P));
+  } else if (k) {
+    f(rek_as<kPresent, kPresent, 2>(P));
+  } else {
+    f(rek_as<kPresent, kAbsent, 2>(P));
+  }
+}
 
 struct Cell {
   int i, j, k;
@@ -44,68 +98,69 @@ __device__ inline Cell cell_of(const Geo& g) {
   return cell_at(g, b.x, b.y, b.z);
 }
 
+// Face conductivities. Every operand is loaded whichever branch applies (the stencil points lie
+// in the halo-padded rows), so that the loads of all faces of a cell issue together instead of
+// one dependent round trip per branch; the arithmetic is that of the reference branch.
+//
 // Conductivity on the +x face of cell (i,j,k) at flat index c (i may be -1: the x- wall).
 // ReSolver::computeKFace, ReSolver.cpp:243-250 (main loop), :287-299 (vertical-only x- quirk),
 // :329-330 (3D walls), :365-379 (vertical-only lateral zeroing in unsaturated cells).
-__device__ inline double face_x(const ReK& P, const double* kr, const double* wref, long long c, int i) {
+template <class RK>
+__device__ inline double face_x(const RK& P, const double* __restrict__ kr, const double* __restrict__ wref,
+                                long long c, int i) {
+#pragma clang fp contract(off)
   const Geo& g = P.g;
-  if (i < 0) return 0.0;
-  if (P.full3d) {
-    if (i >= g.nx - 1) return 0.0;
-    return 0.5 * (P.soil.Ksx(c + 1) * kr[c + 1] + P.soil.Ksx(c) * kr[c]);
-  }
-  double K;
-  if (i == 0) {
-    const double Kp = P.soil.Ksx(c) * kr[c - 1];
-    K = 0.5 * (Kp + Kp);
-  } else {
-    K = 0.5 * (P.soil.Ksx(c + 1) * kr[c + 1] + P.soil.Ksx(c) * kr[c]);
-  }
-  if (wref[c] < P.soil.at(c).theta_s) K = 0.0;
-  if (i + 1 <= g.nx - 1 && wref[c + 1] < P.soil.at(c + 1).theta_s) K = 0.0;
-  return K;
+  const double ks0 = P.soil.Ksx(c), ks1 = P.soil.Ksx(c + 1);
+  const double kr0 = kr[c], kr1 = kr[c + 1];
+  const double Kmid = 0.5 * (ks1 * kr1 + ks0 * kr0);
+  if (P.full()) return (i < 0 || i >= g.nx - 1) ? 0.0 : Kmid;
+  const double krm = kr[i == 0 ? c - 1 : c];
+  const double w0 = wref[c], w1 = wref[c + 1];
+  const double ts0 = P.soil.at(c).theta_s, ts1 = P.soil.at(c + 1).theta_s;
+  const double Kp = ks0 * krm;
+  double K = (i == 0) ? 0.5 * (Kp + Kp) : Kmid;
+  if (w0 < ts0) K = 0.0;
+  if (i + 1 <= g.nx - 1 && w1 < ts1) K = 0.0;
+  return i < 0 ? 0.0 : K;
 }
 
 // +y face of cell (i,j,k): ReSolver.cpp:251-265, :301-310, :333-334, :365-379.
-__device__ inline double face_y(const ReK& P, const double* kr, const double* wref, long long c, int j) {
+template <class RK>
+__device__ inline double face_y(const RK& P, const double* __restrict__ kr, const double* __restrict__ wref,
+                                long long c, int j) {
+#pragma clang fp contract(off)
   const Geo& g = P.g;
-  if (j < 0) return 0.0;
-  if (P.full3d) {
-    if (j >= g.ny - 1) return 0.0;
-    return 0.5 * (P.soil.Ksy(c + g.sy) * kr[c + g.sy] + P.soil.Ksy(c) * kr[c]);
-  }
-  double K;
-  if (j == 0) {
-    const double Kp = P.soil.Ksy(c) * kr[c - g.sy];
-    K = 0.5 * (Kp + Kp);
-  } else if (j == g.ny - 1) {
-    K = 0.0;
-  } else {
-    K = 0.5 * (P.soil.Ksy(c + g.sy) * kr[c + g.sy] + P.soil.Ksy(c) * kr[c]);
-  }
-  if (wref[c] < P.soil.at(c).theta_s) K = 0.0;
-  if (j + 1 <= g.ny - 1 && wref[c + g.sy] < P.soil.at(c + g.sy).theta_s) K = 0.0;
-  return K;
+  const double ks0 = P.soil.Ksy(c), ks1 = P.soil.Ksy(c + g.sy);
+  const double kr0 = kr[c], kr1 = kr[c + g.sy];
+  const double Kmid = 0.5 * (ks1 * kr1 + ks0 * kr0);
+  if (P.full()) return (j < 0 || j >= g.ny - 1) ? 0.0 : Kmid;
+  const double krm = kr[j == 0 ? c - g.sy : c];
+  const double w0 = wref[c], w1 = wref[c + g.sy];
+  const double ts0 = P.soil.at(c).theta_s, ts1 = P.soil.at(c + g.sy).theta_s;
+  const double Kp = ks0 * krm;
+  double K = (j == 0) ? 0.5 * (Kp + Kp) : (j == g.ny - 1 ? 0.0 : Kmid);
+  if (w0 < ts0) K = 0.0;
+  if (j + 1 <= g.ny - 1 && w1 < ts1) K = 0.0;
+  return j < 0 ? 0.0 : K;
 }
 
 // +z (downward) face of cell k (k = -1: the top boundary face). ReSolver.cpp:272-284 (main),
 // :338-344 (bottom), :345-363 (top: Ks_z for Dirichlet even when htop < 0).
-__device__ inline double face_z(const ReK& P, const double* kr, long long c, int k) {
+template <class RK>
+__device__ inline double face_z(const RK& P, const double* __restrict__ kr, long long c, int k) {
+#pragma clang fp contract(off)
   const Geo& g = P.g;
-  if (k < 0) {
-    if (P.sim_sw) return 0.0;
-    const long long ct = c + g.sz;
-    if (P.bc5 == 1) return P.soil.Ksz(ct);
-    if (P.bc5 == 0) return 0.0;
-    return P.soil.Ksz(ct) * kr[ct];
-  }
-  if (k < g.nz - 1) return 0.5 * (P.soil.Ksz(c + g.sz) * kr[c + g.sz] + P.soil.Ksz(c) * kr[c]);
-  return P.bc4 == 0 ? 0.0 : P.soil.Ksz(c) * kr[c];
+  const double ks0 = P.soil.Ksz(c), ks1 = P.soil.Ksz(c + g.sz);
+  const double kr0 = kr[c], kr1 = kr[c + g.sz];
+  if (k < 0) return P.sim_sw ? 0.0 : (P.bc5 == 1 ? ks1 : (P.bc5 == 0 ? 0.0 : ks1 * kr1));
+  if (k < g.nz - 1) return 0.5 * (ks1 * kr1 + ks0 * kr0);
+  return P.bc4 == 0 ? 0.0 : ks0 * kr0;
 }
 
 // K/Ks and C at the interior cell and kr at its face-adjacent ghost cells (whose soil is the
 // cell's own: ghost indices are clamped). ReSolver::updateDiagnostics (ReSolver.cpp:199-215).
-__device__ inline void constitutive_with_ghosts(const ReK& P, const Cell& t, const double* h,
+template <class RK>
+__device__ inline void constitutive_with_ghosts(const RK& P, const Cell& t, const double* h,
                                                  double* kr, double* ch, double* wch) {
   const Geo& g = P.g;
   const SoilClass& s = P.soil.at(t.c);
@@ -121,13 +176,28 @@ __device__ inline void constitutive_with_ghosts(const ReK& P, const Cell& t, con
     if (gh[q] >= 0) kr[gh[q]] = vg_state(s, h[gh[q]]).kr;
 }
 
-__device__ __forceinline__ void re_pre(const ReK& P, const Cell& t, const double* hn, double* kr, double* ch) {
+template <class RK>
+__device__ __forceinline__ void re_pre(const RK& P, const Cell& t, const double* hn, double* kr, double* ch) {
   constitutive_with_ghosts(P, t, hn, kr, ch, nullptr);
+}
+
+// re_pre when kr and C already hold their values at the head of every cell except those flagged
+// in hmod (the post-solve evaluation of the previous step, ReSolver.cpp:963, at the same heads):
+// only flagged interior cells are re-evaluated. Ghost heads, and so ghost kr, are unchanged
+// since that evaluation.
+template <class RK>
+__device__ __forceinline__ void re_pre_stale(const RK& P, const Cell& t, const unsigned char* hmod, const double* hn,
+                                             double* kr, double* ch) {
+  if (!hmod[t.c]) return;
+  const VGState v = vg_state(P.soil.at(t.c), hn[t.c]);
+  kr[t.c] = v.kr;
+  ch[t.c] = v.cap;
 }
 
 // groundwaterMatCoeff + groundwaterRhs (ReSolver.cpp:397-549) as a symmetric stencil:
 // D = Gct, ax/ay/az = Gxp/Gyp/Gzp where the +neighbour is interior (Gxm(c) == Gxp(c-1) etc.).
-__device__ __forceinline__ void re_assemble(const ReK& P, const Cell& t, const double* kr, const double* ch, const double* wcn, const double* hn, double* D, double* ax, double* ay, double* az, double* rhs, float* D32, float* ax32, float* ay32, float* az32) {
+template <class RK>
+__device__ __forceinline__ void re_assemble(const RK& P, const Cell& t, const double* kr, const double* ch, const double* wcn, const double* hn, double* D, double* ax, double* ay, double* az, double* rhs, float* D32, float* ax32, float* ay32, float* az32) {
   const Geo& g = P.g;
   const long long c = t.c;
   const double Kxp = face_x(P, kr, wcn, c, t.i), Kxm = face_x(P, kr, wcn, c - 1, t.i - 1);
@@ -156,15 +226,16 @@ __device__ __forceinline__ void re_assemble(const ReK& P, const Cell& t, const d
   } else {
     Gct -= Gzm;
   }
+  const double hxp = hn[c + 1], hxm = hn[c - 1], hyp = hn[c + g.sy], hym = hn[c - g.sy], hzp = hn[c + g.sz];
   double r = st * hn[c] * V;
   r -= dt * V * Kzp / dzk;
   r += dt * V * Kzm / dzk;
-  if (t.i == g.nx - 1) r -= Gxp * hn[c + 1];
-  if (t.i == 0) r -= Gxm * hn[c - 1];
-  if (t.j == g.ny - 1) r -= Gyp * hn[c + g.sy];
-  if (t.j == 0) r -= Gym * hn[c - g.sy];
+  if (t.i == g.nx - 1) r -= Gxp * hxp;
+  if (t.i == 0) r -= Gxm * hxm;
+  if (t.j == g.ny - 1) r -= Gyp * hyp;
+  if (t.j == 0) r -= Gym * hym;
   if (t.k == g.nz - 1) {
-    if (P.bc4 == 1) r += Gzp * hn[c + g.sz];
+    if (P.bc4 == 1) r += Gzp * hzp;
     if (P.bc4 == 2) r += dt * V * (P.qbot + Kzp) / dzk;
   }
   if (t.k == 0 && !P.sim_sw && P.bc5 == 1) r -= Gzm * P.htop;
@@ -190,7 +261,8 @@ __device__ __forceinline__ void re_assemble(const ReK& P, const Cell& t, const d
 
 // enforceHeadBc (ReSolver.cpp:638-669) + kr, C and theta at the solved head (the second
 // updateDiagnostics' ch and wch, ReSolver.cpp:963, :825).
-__device__ __forceinline__ void re_post(const ReK& P, const Cell& t, double* h, double* kr, double* ch, double* wch) {
+template <class RK>
+__device__ __forceinline__ void re_post(const RK& P, const Cell& t, double* h, double* kr, double* ch, double* wch) {
   const Geo& g = P.g;
   const double hc = h[t.c];
   if (t.i == 0) h[t.c - 1] = hc;
@@ -203,58 +275,108 @@ __device__ __forceinline__ void re_post(const ReK& P, const Cell& t, double* h, 
   constitutive_with_ghosts(P, t, h, kr, ch, wch);
 }
 
-// Second computeKFace + groundwaterFlux with darcyFluxZ (ReSolver.cpp:947-948, 24-58, 671-743).
-__device__ __forceinline__ void re_flux(const ReK& P, const Cell& t, const double* h, const double* kr, const double* wcn, double* Kx, double* Ky, double* Kz, double* qx, double* qy, double* qz) {
+// Darcy fluxes through the +x / +y / +z face of interior cell c (groundwaterFlux with
+// darcyFluxZ, ReSolver.cpp:24-58, 671-743), given the face conductivity. Evaluated without
+// contraction so that a face's flux has the same bits whichever of its two cells computes it.
+template <class RK>
+__device__ inline double flux_x(const RK& P, const double* h, double Kxp, long long c) {
+#pragma clang fp contract(off)
+  return (P.dy * P.dz) * Kxp * (h[c + 1] - h[c]) / P.dx;
+}
+template <class RK>
+__device__ inline double flux_y(const RK& P, const double* h, double Kyp, long long c) {
+#pragma clang fp contract(off)
+  return (P.dx * P.dz) * Kyp * (h[c + P.g.sy] - h[c]) / P.dy;
+}
+template <class RK>
+__device__ inline double flux_z(const RK& P, const double* h, const double* kr, long long c, int k) {
+#pragma clang fp contract(off)
+  const Geo& g = P.g;
+  const double faceA = P.dx * P.dy;
+  if (k == g.nz - 1 && P.bc4 == 2) return P.qbot * faceA;
+  if (k == g.nz - 1 && P.bc4 == 0) return 0.0;
+  const double delta = (k == g.nz - 1) ? 0.5 * P.dz : 0.5 * (P.dz + P.dz);
+  const double kc = P.soil.Ksz(c) * kr[c];
+  const double ks = P.soil.Ksz(c + g.sz) * kr[c + g.sz];
+  const double kface = 0.5 * (kc + ks);
+  return faceA * kface * ((h[c + g.sz] - h[c]) / delta - 1.0);
+}
+// Top-boundary flux of the column of top cell c (darcyFluxZ's k = -1 face, ReSolver.cpp:24-58).
+template <class RK>
+__device__ inline double flux_top(const RK& P, const Cell& t, const double* h) {
+#pragma clang fp contract(off)
   const Geo& g = P.g;
   const long long c = t.c;
-  const double Kxp = face_x(P, kr, wcn, c, t.i);
-  const double Kyp = face_y(P, kr, wcn, c, t.j);
-  const double Kzp = face_z(P, kr, c, t.k);
-  Kx[c] = Kxp;
-  Ky[c] = Kyp;
-  Kz[c] = Kzp;
+  const double faceA = P.dx * P.dy;
+  if (P.bc5 == 2) {
+    const double qcol = P.qtop_field ? P.qtop_field[g.sidx(t.i, t.j)] : P.qtop;
+    return qcol * faceA;
+  }
+  const double kface = (P.bc5 == 0) ? 0.0 : P.soil.Ksz(c);
+  return faceA * kface * ((h[c] - h[c - g.sz]) / (0.5 * P.dz) - 1.0);
+}
+
+// Faces and fluxes of cell t at h^{n+1} (second computeKFace + groundwaterFlux,
+// ReSolver.cpp:947-948): the cell's + faces and, for a top cell, the top face of the column.
+struct CellFlux {
+  double Kxp, Kyp, Kzp, Kzt, qxp, qyp, qzp, qt;
+};
+template <class RK>
+__device__ __forceinline__ CellFlux flux_values(const RK& P, const Cell& t, const double* __restrict__ h,
+                                                const double* __restrict__ kr, const double* __restrict__ wcn) {
+  const Geo& g = P.g;
+  const long long c = t.c;
+  CellFlux f;
+  f.Kxp = face_x(P, kr, wcn, c, t.i);
+  f.Kyp = face_y(P, kr, wcn, c, t.j);
+  f.Kzp = face_z(P, kr, c, t.k);
+  f.qxp = P.full() ? flux_x(P, h, f.Kxp, c) : 0.0;
+  f.qyp = P.full() ? flux_y(P, h, f.Kyp, c) : 0.0;
+  f.qzp = flux_z(P, h, kr, c, t.k);
+  f.Kzt = 0.0;
+  f.qt = 0.0;
+  if (t.k == 0) {
+    f.Kzt = face_z(P, kr, c - g.sz, -1);
+    f.qt = flux_top(P, t, h);
+  }
+  return f;
+}
+// Stores Kx/Ky/Kz and qx/qy/qz of the cell's + faces, the top face of a top cell, and the zero
+// ghost-face conductivities of boundary cells.
+template <class RK>
+__device__ __forceinline__ void store_flux(const RK& P, const Cell& t, const CellFlux& f, double* __restrict__ Kx,
+                                           double* __restrict__ Ky, double* __restrict__ Kz, double* __restrict__ qx,
+                                           double* __restrict__ qy, double* __restrict__ qz) {
+  const Geo& g = P.g;
+  const long long c = t.c;
+  Kx[c] = f.Kxp;
+  Ky[c] = f.Kyp;
+  Kz[c] = f.Kzp;
   if (t.i == 0) Kx[c - 1] = 0.0;
-  if (t.i == g.nx - 1 && !P.full3d) Kx[c + 1] = 0.0;
+  if (t.i == g.nx - 1 && !P.full()) Kx[c + 1] = 0.0;
   if (t.j == 0) Ky[c - g.sy] = 0.0;
   if (t.k == g.nz - 1) Kz[c + g.sz] = 0.0;
-  const double dzk = P.dz;
-  const double faceA = P.dx * P.dy;
-  if (P.full3d) {
-    qx[c] = (P.dy * dzk) * Kxp * (h[c + 1] - h[c]) / P.dx;
-    qy[c] = (P.dx * dzk) * Kyp * (h[c + g.sy] - h[c]) / P.dy;
-  } else {
-    qx[c] = 0.0;
-    qy[c] = 0.0;
-  }
-  double q;
-  if (t.k == g.nz - 1 && P.bc4 == 0) {
-    q = 0.0;
-  } else {
-    const double delta = (t.k == g.nz - 1) ? 0.5 * dzk : 0.5 * (dzk + dzk);
-    const double kc = P.soil.Ksz(c) * kr[c];
-    const double ks = P.soil.Ksz(c + g.sz) * kr[c + g.sz];
-    const double kface = 0.5 * (kc + ks);
-    q = faceA * kface * ((h[c + g.sz] - h[c]) / delta - 1.0);
-  }
-  if (t.k == g.nz - 1 && P.bc4 == 2) q = P.qbot * faceA;
-  qz[c] = q;
+  qx[c] = f.qxp;
+  qy[c] = f.qyp;
+  qz[c] = f.qzp;
   if (t.k == 0) {
-    Kz[c - g.sz] = face_z(P, kr, c - g.sz, -1);
-    double qt;
-    if (P.bc5 == 2) {
-      const double qcol = P.qtop_field ? P.qtop_field[g.sidx(t.i, t.j)] : P.qtop;
-      qt = qcol * faceA;
-    } else {
-      const double kface = (P.bc5 == 0) ? 0.0 : P.soil.Ksz(c);
-      qt = faceA * kface * ((h[c] - h[c - g.sz]) / (0.5 * dzk) - 1.0);
-    }
-    qz[c - g.sz] = qt;
+    Kz[c - g.sz] = f.Kzt;
+    qz[c - g.sz] = f.qt;
   }
+}
+template <class RK>
+__device__ __forceinline__ CellFlux re_flux(const RK& P, const Cell& t, const double* h, const double* kr,
+                                            const double* wcn, double* Kx, double* Ky, double* Kz, double* qx,
+                                            double* qy, double* qz) {
+  const CellFlux f = flux_values(P, t, h, kr, wcn);
+  store_flux(P, t, f, Kx, Ky, Kz, qx, qy, qz);
+  return f;
 }
 
 // Stores the corrected theta of cell t, h(theta) and the saturation flags; ghost neighbours keep
 // the moisture ghosts of the previous step (ReSolver.cpp:855-878).
-__device__ inline void correct_cell(const ReK& P, const Cell& t, double w, double th, const double* wcn, double* wc,
+template <class RK>
+__device__ inline void correct_cell(const RK& P, const Cell& t, double w, double th, const double* wcn, double* wc,
                                     double* hwc, unsigned char* sat0, unsigned char* cand) {
   const Geo& g = P.g;
   const long long c = t.c;
@@ -276,11 +398,14 @@ __device__ inline void correct_cell(const ReK& P, const Cell& t, double w, doubl
 }
 
 // updateWaterContent (ReSolver.cpp:762-786) or theta(h) without the corrector (:953-961) for one
-// cell, plus its ledger terms: theta^n V, the specific-storage term and the top/bottom inflow.
-__device__ __forceinline__ void re_correct(const ReK& P, const Cell& t, const double* h, const double* hn,
-                                           const double* wcn, const double* qx, const double* qy, const double* qz,
-                                           const double* wch, double* wc, double* hwc, unsigned char* sat0,
-                                           unsigned char* cand, double& l_before, double& l_ss, double& l_bnd) {
+// cell from the fluxes through its + faces (qxp, qyp, qzp) and - faces (qxm, qym, qzm), plus its
+// ledger terms: theta^n V, the specific-storage term and the top/bottom inflow.
+template <class RK>
+__device__ __forceinline__ void correct_fluxes(const RK& P, const Cell& t, const double* h, const double* hn,
+                                               const double* wcn, double qxp, double qxm, double qyp, double qym,
+                                               double qzp, double qzm, const double* wch, double* wc, double* hwc,
+                                               unsigned char* sat0, unsigned char* cand, double& l_before,
+                                               double& l_ss, double& l_bnd) {
   const Geo& g = P.g;
   const long long c = t.c;
   const SoilClass& s = P.soil.at(c);
@@ -289,65 +414,120 @@ __device__ __forceinline__ void re_correct(const ReK& P, const Cell& t, const do
   const double th = wch[c];
   if (P.use_corrector) {
     const double coeff = V * (1.0 + s.Ss * (h[c] - hn[c]) / s.theta_s);
-    const double dqx = P.dt * (qx[c] - qx[c - 1]);
-    const double dqy = P.dt * (qy[c] - qy[c - g.sy]);
-    const double dqz = P.dt * (qz[c] - qz[c - g.sz]);
+    const double dqx = P.dt * (qxp - qxm);
+    const double dqy = P.dt * (qyp - qym);
+    const double dqz = P.dt * (qzp - qzm);
     w = (wcn[c] * V + dqx + dqy + dqz) / coeff;
     l_ss = -w * V * s.Ss * (h[c] - hn[c]) / s.theta_s;
   } else {
     w = th;
   }
   l_before = wcn[c] * V;
-  if (t.k == 0) l_bnd -= P.dt * qz[c - g.sz];
-  if (t.k == g.nz - 1) l_bnd += P.dt * qz[c];
+  if (t.k == 0) l_bnd -= P.dt * qzm;
+  if (t.k == g.nz - 1) l_bnd += P.dt * qzp;
   correct_cell(P, t, w, th, wcn, wc, hwc, sat0, cand);
+}
+
+// The corrector with the face fluxes read from the stored qx / qy / qz.
+template <class RK>
+__device__ __forceinline__ void re_correct(const RK& P, const Cell& t, const double* h, const double* hn,
+                                           const double* wcn, const double* qx, const double* qy, const double* qz,
+                                           const double* wch, double* wc, double* hwc, unsigned char* sat0,
+                                           unsigned char* cand, double& l_before, double& l_ss, double& l_bnd) {
+  const Geo& g = P.g;
+  const long long c = t.c;
+  correct_fluxes(P, t, h, hn, wcn, qx[c], qx[c - 1], qy[c], qy[c - g.sy], qz[c], qz[c - g.sz], wch, wc, hwc, sat0,
+                 cand, l_before, l_ss, l_bnd);
+}
+
+// Fluxes + corrector of one cell in one pass: the cell stores its + face fluxes (re_flux) and
+// evaluates the fluxes through its - faces with the same functions of the same operands as the
+// neighbour that stores them (identical bits), instead of re-reading them. Ghost-face fluxes
+// (i = 0 / j = 0 x- and y- faces), which no cell writes, are read. All loads precede the stores.
+template <class RK>
+__device__ __forceinline__ void re_flux_correct(const RK& P, const Cell& t, const double* __restrict__ h,
+                                                const double* __restrict__ kr, const double* __restrict__ hn,
+                                                const double* __restrict__ wcn, const double* __restrict__ wch,
+                                                double* __restrict__ Kx, double* __restrict__ Ky,
+                                                double* __restrict__ Kz, double* __restrict__ qx,
+                                                double* __restrict__ qy, double* __restrict__ qz,
+                                                double* __restrict__ wc, double* __restrict__ hwc,
+                                                unsigned char* __restrict__ sat0, unsigned char* __restrict__ cand,
+                                                double& l_before, double& l_ss, double& l_bnd) {
+  const Geo& g = P.g;
+  const long long c = t.c;
+  const CellFlux f = flux_values(P, t, h, kr, wcn);
+  const double qx_ghost = qx[c - 1], qy_ghost = qy[c - g.sy];
+  const double qxm = t.i > 0 ? (P.full() ? flux_x(P, h, face_x(P, kr, wcn, c - 1, t.i - 1), c - 1) : 0.0) : qx_ghost;
+  const double qym =
+      t.j > 0 ? (P.full() ? flux_y(P, h, face_y(P, kr, wcn, c - g.sy, t.j - 1), c - g.sy) : 0.0) : qy_ghost;
+  const double qzm = (t.k > 0) ? flux_z(P, h, kr, c - g.sz, t.k - 1) : f.qt;
+  correct_fluxes(P, t, h, hn, wcn, f.qxp, qxm, f.qyp, qym, f.qzp, qzm, wch, wc, hwc, sat0, cand, l_before, l_ss,
+                 l_bnd);
+  store_flux(P, t, f, Kx, Ky, Kz, qx, qy, qz);
 }
 
 // adjacentToSat (ReSolver.cpp:802-817) with `prev` holding the post-visit saturation state of
 // the cells the i->j->k sweep has already visited and `sat0` the pre-visit state of the rest.
-__device__ inline bool adjacent_sat(const ReK& P, const Cell& t, const double* Kx, const double* Ky,
-                                    const unsigned char* sat0, const unsigned char* prev) {
+// The reference's early-exit chain as one boolean expression over operands loaded together.
+template <class RK>
+__device__ inline bool adjacent_sat(const RK& P, const Cell& t, const double* __restrict__ Kx,
+                                    const double* __restrict__ Ky, const unsigned char* __restrict__ sat0,
+                                    const unsigned char* __restrict__ prev) {
   const Geo& g = P.g;
   const long long c = t.c;
-  if (sat0[c + g.sz]) return true;
-  if (Kx[c] != 0.0 && sat0[c + 1]) return true;
-  if (Kx[c - 1] != 0.0 && prev[c - 1]) return true;
-  if (Ky[c] != 0.0 && sat0[c + g.sy]) return true;
-  if (Ky[c - g.sy] != 0.0 && prev[c - g.sy]) return true;
-  if (t.k == 0) return (!P.sim_sw && P.bc5 == 1 && P.htop >= 0.0);
-  return prev[c - g.sz] != 0;
+  const unsigned char below = sat0[c + g.sz], xp = sat0[c + 1], xm = prev[c - 1], yp = sat0[c + g.sy],
+                      ym = prev[c - g.sy], above = prev[c - g.sz];
+  const double kx0 = Kx[c], kxm = Kx[c - 1], ky0 = Ky[c], kym = Ky[c - g.sy];
+  const bool top = (t.k == 0) ? (!P.sim_sw && P.bc5 == 1 && P.htop >= 0.0) : (above != 0);
+  return below || (kx0 != 0.0 && xp) || (kxm != 0.0 && xm) || (ky0 != 0.0 && yp) || (kym != 0.0 && ym) || top;
 }
 
 // One Jacobi sweep of the monotone map whose unique fixed point is the sequential sweep's
 // saturation state: a candidate (unsaturated theta, saturated theta(h)) becomes saturated iff
 // it is adjacent to saturation, reading visited neighbours from `prev`.
-__device__ __forceinline__ void re_realloc(const ReK& P, const Cell& t, const double* Kx, const double* Ky, const unsigned char* sat0, const unsigned char* cand, const unsigned char* prev, unsigned char* next, int* changed) {
+template <class RK>
+__device__ __forceinline__ void re_realloc(const RK& P, const Cell& t, const double* __restrict__ Kx,
+                                           const double* __restrict__ Ky, const unsigned char* __restrict__ sat0,
+                                           const unsigned char* __restrict__ cand,
+                                           const unsigned char* __restrict__ prev, unsigned char* __restrict__ next,
+                                           int* changed) {
   unsigned char v = sat0[t.c];
-  if (cand[t.c]) v = adjacent_sat(P, t, Kx, Ky, sat0, prev) ? 1 : 0;
+  if (cand[t.c]) v = adjacent_sat(P, t, Kx, Ky, sat0, prev) ? 1 : 0;  // candidates are few
   next[t.c] = v;
   if (v != prev[t.c]) atomicOr(changed, 1);
 }
 
 // reallocateWaterContent branches with the converged saturation state + clampWaterContent +
-// enforceMoistureBc (ReSolver.cpp:819-878) for one cell, plus its ledger terms.
-__device__ __forceinline__ void re_finalize(const ReK& P, const Cell& t, const double* Kx, const double* Ky,
-                                            const unsigned char* sat0, const unsigned char* fin, const double* wch,
-                                            const double* hwc, const double* hn, double* wc, double* h,
-                                            double& l_after, double& l_adj, double& dh) {
+// enforceMoistureBc (ReSolver.cpp:819-878) for one cell, plus its ledger terms. hmod (optional)
+// receives 1 where the head was replaced by h(theta), else 0.
+template <class RK>
+__device__ __forceinline__ void re_finalize(const RK& P, const Cell& t, const double* __restrict__ Kx,
+                                            const double* __restrict__ Ky, const unsigned char* __restrict__ sat0,
+                                            const unsigned char* __restrict__ fin, const double* __restrict__ wch,
+                                            const double* __restrict__ hwc, const double* __restrict__ hn,
+                                            double* __restrict__ wc, double* __restrict__ h,
+                                            unsigned char* __restrict__ hmod, double& l_after, double& l_adj,
+                                            double& dh) {
   const Geo& g = P.g;
   const long long c = t.c;
   const SoilClass& s = P.soil.at(c);
-  const double wcorr = wc[c];
+  const double wcorr = wc[c], hw = hwc[c], hc = h[c], whc = wch[c], hnc = hn[c];
+  const unsigned char s0 = sat0[c];
+  const bool adj = adjacent_sat(P, t, Kx, Ky, sat0, fin);
   double w = wcorr;
-  if (sat0[c]) {
+  bool replace = false;
+  if (s0) {
     w = s.theta_s;
-  } else if (!adjacent_sat(P, t, Kx, Ky, sat0, fin)) {
-    if (w < 0.9999 * s.theta_s) h[c] = hwc[c];
+  } else if (!adj) {
+    replace = w < 0.9999 * s.theta_s;
   } else {
-    w = wch[c];
+    w = whc;
   }
+  const double hnew = replace ? hw : hc;
   if (w > s.theta_s) w = s.theta_s;
   else if (w < s.theta_r) w = s.theta_r;
+  if (replace) h[c] = hnew;
   wc[c] = w;
   if (t.i == 0) wc[c - 1] = w;
   if (t.i == g.nx - 1) wc[c + 1] = w;
@@ -355,15 +535,17 @@ __device__ __forceinline__ void re_finalize(const ReK& P, const Cell& t, const d
   if (t.j == g.ny - 1) wc[c + g.sy] = w;
   if (t.k == g.nz - 1) wc[c + g.sz] = w;
   if (t.k == 0) wc[c - g.sz] = w;
+  if (hmod) hmod[c] = (replace && hnew != hc) ? 1 : 0;
   const double V = P.dx * P.dy * P.dz;
   l_after = w * V;
   l_adj = (w - wcorr) * V;
-  dh = fabs(h[c] - hn[c]);
+  dh = fabs(hnew - hnc);
 }
 
 // setInitialHead / setInitialWaterContent (ReSolver.cpp:135-173) + finalizeInitialState
 // diagnostics (:175-178) on interior cells.
-__device__ __forceinline__ void re_initial(const ReK& P, const Cell& t, int from_head, double* h, double* wc, double* hn, double* wcn, double* wch, double* hwc, double* ch) {
+template <class RK>
+__device__ __forceinline__ void re_initial(const RK& P, const Cell& t, int from_head, double* h, double* wc, double* hn, double* wcn, double* wch, double* hwc, double* ch) {
   const long long c = t.c;
   const SoilClass& s = P.soil.at(c);
   if (from_head) {

@@ -460,11 +460,21 @@ double* SweState::field(const std::string& name) {
 }
 
 void SweState::set_field(const std::string& name, const double* src) {
-  FREHG_HIP(hipMemcpyAsync(field(name), src, (size_t)n_ * sizeof(double), hipMemcpyHostToDevice, s_));
+  copy_field_in(g_, 1, field(name), src, s_);
   FREHG_HIP(hipStreamSynchronize(s_));
 }
 
 void SweState::get_field(const std::string& name, double* dst) {
+  copy_field_out(g_, 1, dst, field(name), s_);
+  FREHG_HIP(hipStreamSynchronize(s_));
+}
+
+void SweState::set_raw(const std::string& name, const double* src) {
+  FREHG_HIP(hipMemcpyAsync(field(name), src, (size_t)n_ * sizeof(double), hipMemcpyHostToDevice, s_));
+  FREHG_HIP(hipStreamSynchronize(s_));
+}
+
+void SweState::get_raw(const std::string& name, double* dst) {
   FREHG_HIP(hipMemcpyAsync(dst, field(name), (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
   FREHG_HIP(hipStreamSynchronize(s_));
 }
@@ -490,9 +500,9 @@ static SwK make_swk(const Geo& g, double dx, double dy, const frehg_swe_params& 
 void SweState::set_bathymetry(const double* bed) {
   const int nx = g_.nx, ny = g_.ny;
   std::vector<double> b((size_t)n_, 0.0), bxp((size_t)n_, 0.0), byp((size_t)n_, 0.0);
-  get_field("bottom", b.data());
-  get_field("bottomXP", bxp.data());
-  get_field("bottomYP", byp.data());
+  get_raw("bottom", b.data());
+  get_raw("bottomXP", bxp.data());
+  get_raw("bottomYP", byp.data());
   auto s = [&](int i, int j) { return (size_t)g_.sidx(i, j); };
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) b[s(i, j)] = bed[(size_t)i + (size_t)j * nx];
@@ -513,17 +523,17 @@ void SweState::set_bathymetry(const double* bed) {
   for (int i = 0; i < nx; ++i) byp[s(i, ny)] = byp[s(i, ny - 1)];
   for (int j = 0; j < ny; ++j) bxp[s(-1, j)] = b[s(-1, j)] > b[s(0, j)] ? b[s(-1, j)] : b[s(0, j)];
   for (int j = 0; j < ny; ++j) bxp[s(nx, j)] = bxp[s(nx - 1, j)];
-  set_field("bottom", b.data());
-  set_field("bottomXP", bxp.data());
-  set_field("bottomYP", byp.data());
+  set_raw("bottom", b.data());
+  set_raw("bottomXP", bxp.data());
+  set_raw("bottomYP", byp.data());
 }
 
 // initializeState (SweSolver.cpp:926-934) with finalizeInitialState (:904-924).
 void SweState::initialize_state(double init_eta) {
   const int nx = g_.nx, ny = g_.ny;
   std::vector<double> eta((size_t)n_), bot((size_t)n_);
-  get_field("eta", eta.data());
-  get_field("bottom", bot.data());
+  get_raw("eta", eta.data());
+  get_raw("bottom", bot.data());
   auto s = [&](int i, int j) { return (size_t)g_.sidx(i, j); };
   // setInitialEtaPhysical (:855-870) + enforceSurfBc (:371-384)
   for (int j = 0; j < ny; ++j)
@@ -545,11 +555,11 @@ void SweState::initialize_state(double init_eta) {
       const double diff = eta[s(i, j)] - bot[s(i, j)];
       if (diff > 0.0 && diff < p_.min_depth) eta[s(i, j)] = bot[s(i, j)];
     }
-  set_field("eta", eta.data());
+  set_raw("eta", eta.data());
   // setInitialVelocityConstant(0, 0) (:872-884) + zero-gradient velocity ghosts (:906-907)
   for (const char* nm : {"uu", "vv", "CDx", "CDy"}) {
     std::vector<double> a((size_t)n_);
-    get_field(nm, a.data());
+    get_raw(nm, a.data());
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) a[s(i, j)] = 0.0;
     if (std::string(nm) == "uu" || std::string(nm) == "vv") {
@@ -562,7 +572,7 @@ void SweState::initialize_state(double init_eta) {
         a[s(nx, j)] = a[s(nx - 1, j)];
       }
     }
-    set_field(nm, a.data());
+    set_raw(nm, a.data());
   }
   const SwK P = make_swk(g_, dx_, dy_, p_);
   const dim3 grid(ceil_div(nx, 64), ceil_div(ny, 4)), blk(NT);

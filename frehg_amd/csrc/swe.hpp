@@ -19,6 +19,7 @@ class SweState {
   void set_params(const frehg_swe_params& p);
   void set_bathymetry(const double* bed_interior);
   void initialize_state(double init_eta);
+  // fields in the reference's packed halo-padded layout ((nx + 2) x (ny + 2))
   void set_field(const std::string& name, const double* src);
   void get_field(const std::string& name, double* dst);
   void step(double rain, double evap, frehg_solve_stats* st);
@@ -32,6 +33,9 @@ class SweState {
 
  private:
   double* field(const std::string& name);
+  // whole device-layout arrays (nsurf() doubles, indexed by sidx)
+  void set_raw(const std::string& name, const double* src);
+  void get_raw(const std::string& name, double* dst);
 
   Geo g_;  // nz = 1; 2D index = sidx
   double dx_, dy_;

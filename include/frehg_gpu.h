@@ -287,11 +287,11 @@ int frehg_profile_enable(frehg_ctx_t ctx, int enable);
 int frehg_profile_read(frehg_ctx_t ctx, int kt, double* total_ms, int64_t* launches);
 int frehg_profile_reset(frehg_ctx_t ctx);
 /* Groundwater CG SpMV launch shape in use for `precision` (FREHG_PREC_FP64: the FP64 solve,
- * FREHG_PREC_MIXED: the FP32 inner solve): tile height (0 = one column per thread, else 64 x
- * tile_y LDS tiles) and layers per block. Set by the autotuner at the first solve, or pinned by the
- * environment variable FREHG_SPMV_SHAPE="tile_y:kchunk" read at frehg_re_init. No counterpart in
- * the reference (PETSc's MatMult has no launch shape). */
-int frehg_spmv_shape(frehg_ctx_t ctx, int precision, int* tile_y, int* kchunk);
+ * FREHG_PREC_MIXED: the FP32 inner solve): cells per lane (1: one column per thread; 2: two
+ * adjacent columns per lane with paired aligned loads) and layers per block. Set by the autotuner
+ * at the first solve, or pinned by the environment variable FREHG_SPMV_SHAPE="cpl:kchunk" read at
+ * frehg_re_init. No counterpart in the reference (PETSc's MatMult has no launch shape). */
+int frehg_spmv_shape(frehg_ctx_t ctx, int precision, int* cells_per_lane, int* kchunk);
 
 #ifdef __cplusplus
 }
