@@ -45,7 +45,7 @@ def interior3(a, nx, ny, nz):
     return a.reshape(nz + 2, ny + 2, nx + 2)[1:-1, 1:-1, 1:-1]
 
 
-KERNEL_TYPES = ("spmv64", "spmv32", "update", "assemble", "correct", "refine", "sw_spmv", "sw_update", "guess")
+KERNEL_TYPES = ("spmv64", "spmv32", "update", "assemble", "correct", "refine", "guess")
 
 
 def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None):
@@ -79,7 +79,7 @@ def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None):
     elapsed = time.perf_counter() - t0
     kt = {k: ctx.profile_read(v) for k, v in zip(KERNEL_TYPES, (abi.KT_GW_SPMV, abi.KT_GW_SPMV32, abi.KT_GW_UPDATE,
                                                                abi.KT_GW_ASSEMBLE, abi.KT_GW_CORRECT, abi.KT_GW_REFINE,
-                                                               abi.KT_SW_SPMV, abi.KT_SW_UPDATE, abi.KT_GW_GUESS))}
+                                                               abi.KT_GW_GUESS))}
     ctx.profile_enable(False)
     shape = ctx.spmv_shape(abi.PREC_FP64 if path == "plain" else abi.PREC_MIXED)
     h = ctx.re_get("h")

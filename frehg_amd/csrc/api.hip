@@ -293,7 +293,8 @@ int frehg_coupled_step(frehg_ctx_t ctx, const frehg_coupling_params* cp, double 
     auto& re = need_re(ctx);
     const double t_new = ctx->t + dt;  // Orchestrator::step, coupled branch (:1322-1336)
     swe.step(rain, evap, &local.sw);
-    local.exchange_volume = need_cpl(ctx).exchange(swe, re, *cp, dt, nullptr);
+    auto& cpl = need_cpl(ctx);
+    cpl.exchange_async(swe, re, *cp, dt);  // its volume is read after the groundwater step
     const double eps = 1.0e-9 * std::max(1.0, t_end);  // gwCatchUpTo (:1243-1252)
     int fallback = 0, iters = 0;
     while (ctx->gw_time < t_new - eps) {
@@ -304,6 +305,7 @@ int frehg_coupled_step(frehg_ctx_t ctx, const frehg_coupling_params* cp, double 
       ++local.gw_substeps;
     }
     local.gw.solve.iterations = iters;
+    local.exchange_volume = cpl.exchanged_volume();
     ctx->t = t_new;
     local.t = ctx->t;
     local.gw_time = ctx->gw_time;

@@ -65,11 +65,18 @@ __device__ inline double block_sum(double v, double* lds) {
   return s;
 }
 
-// Deterministic sum of n partials by one block of NT threads (strided, then tree).
+// Deterministic sum of n partials by one block of NT threads: each thread keeps eight
+// accumulators over a fixed stride pattern (so eight loads are in flight, not one), then a tree.
 template <int NT>
 __device__ inline double block_sum_partials(const double* part, int n, double* lds) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += NT) s += part[i];
+  double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int i = threadIdx.x;
+  for (; i + 7 * NT < n; i += 8 * NT) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] += part[i + u * NT];
+  }
+  for (; i < n; i += NT) a[0] += part[i];
+  const double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   return block_sum<NT>(s, lds);
 }
 
@@ -160,6 +167,21 @@ __device__ __forceinline__ TileId xcd_tile() {
   r.x = t % gx;
   r.y = (t / gx) % gy;
   r.z = t / (gx * gy);
+  r.linear = id;
+  return r;
+}
+
+// As xcd_tile, with the layer index varying fastest along each XCD's run of tiles: an XCD
+// processes the layers of one column tile one after the other, so the layer above and below a
+// tile (its z-neighbours) were read through the same L2 just before.
+__device__ __forceinline__ TileId zfast_tile() {
+  const unsigned int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+  const unsigned int id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned int t = xcd_remap(id, gx * gy * gz);
+  TileId r;
+  r.z = t % gz;
+  r.x = (t / gz) % gx;
+  r.y = t / (gz * gx);
   r.linear = id;
   return r;
 }

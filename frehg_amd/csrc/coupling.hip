@@ -117,7 +117,7 @@ void Coupler::ensure(unsigned int nblocks, long long ncol) {
   }
 }
 
-double Coupler::exchange(SweState& swe, ReState& re, const frehg_coupling_params& cp, double dt, double* q_out) {
+void Coupler::exchange_async(SweState& swe, ReState& re, const frehg_coupling_params& cp, double dt) {
   const Geo& gs = swe.geo();
   const Geo& gg = re.geo();
   if (gs.nx != gg.nx || gs.ny != gg.ny) throw ArgError("frehg_couple_exchange: surface and subsurface grids differ");
@@ -137,15 +137,32 @@ double Coupler::exchange(SweState& swe, ReState& re, const frehg_coupling_params
   const unsigned int nb = grid.x * grid.y;
   ensure(nb, (long long)gs.nx * gs.ny);
   hipLaunchKernelGGL(k_exchange, grid, blk, 0, s_, P, swe.d("eta"), swe.d("bottom"), swe.d("dept"), re.d_wc(),
-                     re.d_h(), re.d_hmod(), q_out ? qdev_ : nullptr, part_);
+                     re.d_h(), re.d_hmod(), want_q_ ? qdev_ : nullptr, part_);
   FREHG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kRedThreads), 0, s_, (const double*)part_, nb, 1, dres_);
+  // slot 2: volume() uses slots 0 and 1 and may run before this result is read
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kRedThreads), 0, s_, (const double*)part_, nb, 1, dres_ + 2);
   FREHG_LAUNCH_CHECK();
-  FREHG_HIP(hipMemcpyAsync(hres_, dres_, sizeof(double), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipMemcpyAsync(hres_ + 2, dres_ + 2, sizeof(double), hipMemcpyDeviceToHost, s_));
+}
+
+double Coupler::exchanged_volume() {
+  FREHG_HIP(hipStreamSynchronize(s_));  // returns at once when the stream has already drained
+  return hres_[2];
+}
+
+double Coupler::exchange(SweState& swe, ReState& re, const frehg_coupling_params& cp, double dt, double* q_out) {
+  want_q_ = q_out != nullptr;
+  try {
+    exchange_async(swe, re, cp, dt);
+  } catch (...) {
+    want_q_ = false;
+    throw;
+  }
+  want_q_ = false;
+  const Geo& gs = swe.geo();
   if (q_out)
     FREHG_HIP(hipMemcpyAsync(q_out, qdev_, (size_t)gs.nx * gs.ny * sizeof(double), hipMemcpyDeviceToHost, s_));
-  FREHG_HIP(hipStreamSynchronize(s_));
-  return hres_[0];
+  return exchanged_volume();
 }
 
 void Coupler::volume(SweState* swe, ReState* re, double* surface, double* subsurface) {

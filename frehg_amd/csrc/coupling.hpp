@@ -12,6 +12,10 @@ class Coupler {
   ~Coupler();
   // Coupling::exchange (src/coupling/Coupling.cpp:145-164); returns sum(q*dt).
   double exchange(SweState& swe, ReState& re, const frehg_coupling_params& cp, double dt, double* q_out);
+  // The same exchange without waiting for it: sum(q*dt) arrives in host memory with the stream's
+  // next synchronisation and is read by exchanged_volume().
+  void exchange_async(SweState& swe, ReState& re, const frehg_coupling_params& cp, double dt);
+  double exchanged_volume();
   // Orchestrator::computeWaterVolume (src/driver/Orchestrator.cpp:898-936).
   void volume(SweState* swe, ReState* re, double* surface, double* subsurface);
 
@@ -25,6 +29,7 @@ class Coupler {
   double* dres_ = nullptr;
   double* hres_ = nullptr;
   unsigned int* counter_ = nullptr;
+  bool want_q_ = false;  // exchange_async also stores the per-column rates
 };
 
 }  // namespace frehg

@@ -120,13 +120,9 @@ __global__ __launch_bounds__(NT) void k_init(Geo g, int kchunk, StencilView<T> A
       szz += (double)zT * (double)zT;
       const double bz = (double)(bn ? bn[c] : b[c]) / (double)A.D[c];
       sbb += bz * bz;
+      // x0 = 0 (only when x0_zero, in which case x is not read at all)
+      if (x0_zero && !fx) x[c] = (T)0;
     }
-  }
-  // x0 = 0 (only when x0_zero, in which case x is not read at all)
-  if (x0_zero && t.valid) {
-    long long c = base_index<THREE_D>(g, t);
-    for (int k = t.k0; k < t.k1; ++k, c += g.sz)
-      if (!(fixed != nullptr && fixed[c] != 0)) x[c] = (T)0;
   }
   const double a = block_sum<NT>(srz, lds);
   const double bsum = block_sum<NT>(szz, lds);
@@ -925,16 +921,18 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
     for (int it = 0; it < batch; ++it) {
       T* pold = pbuf[parity];
       T* pnew = pbuf[parity ^ 1];
+      // Only the 3D (groundwater) launches are timed: the 2D surface solve is launch-bound, and
+      // launches with timing events take the host several microseconds longer to issue.
       {
-        hipEvent_t ev0 = prof_->start(kt_spmv);
-        hipEvent_t ev1 = prof_->stop();
+        hipEvent_t ev0 = three_d_ ? prof_->start(kt_spmv) : nullptr;
+        hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
         launch_spmv<T>(A, z, pold, pnew, q, ctl, ev0, ev1);
       }
       hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(kRedThreads), 0, s_, part_, shape_[sizeof(T) == 4 ? 1 : 0].nb, ctl);
       FREHG_LAUNCH_CHECK();
       {
-        hipEvent_t ev0 = prof_->start(three_d_ ? FREHG_KT_GW_UPDATE : FREHG_KT_SW_UPDATE);
-        hipEvent_t ev1 = prof_->stop();
+        hipEvent_t ev0 = three_d_ ? prof_->start(FREHG_KT_GW_UPDATE) : nullptr;
+        hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
         if (three_d_)
           hipExtLaunchKernelGGL((k_update<T, true>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A.D, pnew, q, x, r, z, part_, ctl, nb_f_);
         else
@@ -957,8 +955,8 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
                            std::to_string(host_ctl_->iters + 1) + ", ||D^-1 r||^2 = " + std::to_string(host_ctl_->zz) +
                            ")");
   const int iters = host_ctl_->iters;
-  // each launched iteration recorded one spmv and one update launch
-  if (launched > iters) prof_->discard_tail(2 * (size_t)(launched - iters));
+  // each launched 3D iteration recorded one spmv and one update launch
+  if (three_d_ && launched > iters) prof_->discard_tail(2 * (size_t)(launched - iters));
   if (sizeof(T) == 8) last_iters_ = iters;
   else last_inner_iters_ = iters;
   *res_norm = std::sqrt(host_ctl_->zz);
@@ -1012,8 +1010,8 @@ int Krylov::run_bicgstab(const StencilView<T>& A, const T* b, const T* bn, T* x,
       const T* vold = vbuf[parity];
       T* vnew = vbuf[parity ^ 1];
       {
-        hipEvent_t ev0 = prof_->start(kt_spmv);
-        hipEvent_t ev1 = prof_->stop();
+        hipEvent_t ev0 = three_d_ ? prof_->start(kt_spmv) : nullptr;
+        hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
         if (three_d_)
           hipExtLaunchKernelGGL((k_bcg_spmv1<T, true>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A, (const T*)r, (const T*)rh, pold,
                              vold, pnew, vnew, part_, ctl);
@@ -1025,8 +1023,8 @@ int Krylov::run_bicgstab(const StencilView<T>& A, const T* b, const T* bn, T* x,
       hipLaunchKernelGGL(k_fin_bcg1, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_f_, ctl);
       FREHG_LAUNCH_CHECK();
       {
-        hipEvent_t ev0 = prof_->start(kt_spmv);
-        hipEvent_t ev1 = prof_->stop();
+        hipEvent_t ev0 = three_d_ ? prof_->start(kt_spmv) : nullptr;
+        hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
         if (three_d_)
           hipExtLaunchKernelGGL((k_bcg_spmv2<T, true>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A, (const T*)r, (const T*)vnew, sv,
                              tv, part_, ctl, nb_f_);
@@ -1038,8 +1036,8 @@ int Krylov::run_bicgstab(const StencilView<T>& A, const T* b, const T* bn, T* x,
       hipLaunchKernelGGL(k_fin_bcg2, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_f_, ctl);
       FREHG_LAUNCH_CHECK();
       {
-        hipEvent_t ev0 = prof_->start(three_d_ ? FREHG_KT_GW_UPDATE : FREHG_KT_SW_UPDATE);
-        hipEvent_t ev1 = prof_->stop();
+        hipEvent_t ev0 = three_d_ ? prof_->start(FREHG_KT_GW_UPDATE) : nullptr;
+        hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
         if (three_d_)
           hipExtLaunchKernelGGL((k_bcg_update<T, true>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A.D, (const T*)pnew, (const T*)sv,
                              (const T*)tv, (const T*)rh, x, r, part_, ctl, nb_f_);
@@ -1065,8 +1063,8 @@ int Krylov::run_bicgstab(const StencilView<T>& A, const T* b, const T* bn, T* x,
                            std::to_string(host_ctl_->iters + 1));
   }
   const int iters = host_ctl_->iters;
-  // each launched iteration recorded two spmv launches and one update launch
-  if (launched > iters) prof_->discard_tail(3 * (size_t)(launched - iters));
+  // each launched 3D iteration recorded two spmv launches and one update launch
+  if (three_d_ && launched > iters) prof_->discard_tail(3 * (size_t)(launched - iters));
   last = iters;
   *res_norm = std::sqrt(host_ctl_->zz);
   *rhs_norm = std::sqrt(host_ctl_->bb);

@@ -15,6 +15,7 @@
 //   finalize   reallocation, clamp, moisture ghosts                                 (:788-878)
 //   dtadapt    flux-imbalance max and Courant min (adaptive dt only)                (:880-929)
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 #include <cstring>
 #include <vector>
@@ -32,54 +33,79 @@ namespace {
 using namespace redev;
 constexpr int NT = kReNT;
 
-template <class K>
-__global__ __launch_bounds__(NT) void k_re_pre(K P, const double* __restrict__ hn, double* __restrict__ kr, double* __restrict__ ch) {
-  const Cell t = cell_of(P.g);
+// Cell of this thread; ZF selects the tile order (0: x fastest, 1: layer fastest). Layer-fastest
+// order lets the kernels with vertical stencils (assemble, flux + corrector) find the layers above
+// and below in L2; measured faster for all per-cell kernels except finalize.
+template <int ZF>
+__device__ inline Cell tile_cell(const Geo& g, TileId& b) {
+  b = ZF ? zfast_tile() : xcd_tile();
+  return cell_at(g, b.x, b.y, b.z);
+}
+
+template <class K, int ZF>
+__global__ __launch_bounds__(NT) void k_re_pre(K P, const double* __restrict__ hn, double* __restrict__ kr,
+                                                double* __restrict__ ch) {
+  TileId b;
+  const Cell t = tile_cell<ZF>(P.g, b);
   if (t.valid) re_pre(P, t, hn, kr, ch);
 }
 
-template <class K>
-__global__ __launch_bounds__(NT) void k_re_pre_stale(K P, const unsigned char* __restrict__ hmod, const double* __restrict__ hn, double* __restrict__ kr,
+template <class K, int ZF>
+__global__ __launch_bounds__(NT) void k_re_pre_stale(K P, const unsigned char* __restrict__ hmod,
+                                                      const double* __restrict__ hn, double* __restrict__ kr,
                                                       double* __restrict__ ch) {
-  const Cell t = cell_of(P.g);
+  TileId b;
+  const Cell t = tile_cell<ZF>(P.g, b);
   if (t.valid) re_pre_stale(P, t, hmod, hn, kr, ch);
 }
 
-template <class K>
-__global__ __launch_bounds__(NT) void k_re_assemble(K P, const double* __restrict__ kr, const double* __restrict__ ch, const double* __restrict__ wcn,
-                                                     const double* __restrict__ hn, double* __restrict__ D, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
-                                                     double* __restrict__ rhs, float* __restrict__ D32, float* __restrict__ ax32, float* __restrict__ ay32, float* __restrict__ az32) {
-  const Cell t = cell_of(P.g);
+template <class K, int ZF>
+__global__ __launch_bounds__(NT) void k_re_assemble(K P, const double* __restrict__ kr, const double* __restrict__ ch,
+                                                     const double* __restrict__ wcn, const double* __restrict__ hn,
+                                                     double* __restrict__ D, double* __restrict__ ax,
+                                                     double* __restrict__ ay, double* __restrict__ az,
+                                                     double* __restrict__ rhs, float* __restrict__ D32,
+                                                     float* __restrict__ ax32, float* __restrict__ ay32,
+                                                     float* __restrict__ az32) {
+  TileId b;
+  const Cell t = tile_cell<ZF>(P.g, b);
   if (t.valid) re_assemble(P, t, kr, ch, wcn, hn, D, ax, ay, az, rhs, D32, ax32, ay32, az32);
 }
 
-template <class K>
-__global__ __launch_bounds__(NT) void k_re_post(K P, double* __restrict__ h, double* __restrict__ kr, double* __restrict__ ch, double* __restrict__ wch) {
-  const Cell t = cell_of(P.g);
+template <class K, int ZF>
+__global__ __launch_bounds__(NT) void k_re_post(K P, double* __restrict__ h, double* __restrict__ kr,
+                                                 double* __restrict__ ch, double* __restrict__ wch) {
+  TileId b;
+  const Cell t = tile_cell<ZF>(P.g, b);
   if (t.valid) re_post(P, t, h, kr, ch, wch);
 }
 
 // Fluxes + corrector (re_flux_correct); ledger partials (sum theta^n V, specific-storage term,
 // boundary inflow) per block to part[0..2][nblocks].
-template <class K>
-__global__ __launch_bounds__(NT) void k_re_flux_correct(K P, const double* __restrict__ h, const double* __restrict__ kr, const double* __restrict__ hn,
-                                                         const double* __restrict__ wcn, const double* __restrict__ wch, double* __restrict__ Kx, double* __restrict__ Ky,
-                                                         double* __restrict__ Kz, double* __restrict__ qx, double* __restrict__ qy, double* __restrict__ qz, double* __restrict__ wc,
-                                                         double* __restrict__ hwc, unsigned char* __restrict__ sat0, unsigned char* __restrict__ cand,
-                                                         double* __restrict__ part, unsigned int nblocks) {
+template <class K, int ZF>
+__global__ __launch_bounds__(NT) void k_re_flux_correct(K P, const double* __restrict__ h, const double* __restrict__ kr,
+                                                         const double* __restrict__ hn, const double* __restrict__ wcn,
+                                                         const double* __restrict__ wch, double* __restrict__ Kx,
+                                                         double* __restrict__ Ky, double* __restrict__ Kz,
+                                                         double* __restrict__ qx, double* __restrict__ qy,
+                                                         double* __restrict__ qz, double* __restrict__ wc,
+                                                         double* __restrict__ hwc, unsigned char* __restrict__ sat0,
+                                                         unsigned char* __restrict__ cand, double* __restrict__ part,
+                                                         unsigned int nblocks, int* __restrict__ flags, int nflags) {
   __shared__ double lds[NT / kWave];
-  const Cell t = cell_of(P.g);
+  TileId b;
+  const Cell t = tile_cell<ZF>(P.g, b);
+  if (b.linear == 0 && (int)threadIdx.x < nflags) flags[threadIdx.x] = 0;  // reallocation sweep flags
   double l_before = 0.0, l_ss = 0.0, l_bnd = 0.0;
   if (t.valid)
     re_flux_correct(P, t, h, kr, hn, wcn, wch, Kx, Ky, Kz, qx, qy, qz, wc, hwc, sat0, cand, l_before, l_ss, l_bnd);
   const double a = block_sum<NT>(l_before, lds);
-  const double b = block_sum<NT>(l_ss, lds);
+  const double bs = block_sum<NT>(l_ss, lds);
   const double d = block_sum<NT>(l_bnd, lds);
-  const unsigned int bid = xcd_tile().linear;
   if (threadIdx.x == 0) {
-    part[bid] = a;
-    part[nblocks + bid] = b;
-    part[2 * nblocks + bid] = d;
+    part[b.linear] = a;
+    part[nblocks + b.linear] = bs;
+    part[2 * nblocks + b.linear] = d;
   }
 }
 
@@ -95,30 +121,33 @@ __global__ __launch_bounds__(NT) void k_re_realloc_iter(K P, const double* __res
 
 // Reallocation + clamp + moisture BCs; ledger partials (sum theta^{n+1} V, sum (theta^{n+1} -
 // theta_c) V) to part[3..4][nblocks] and max |h^{n+1} - h^n| to mx[nblocks].
-template <class K>
+template <class K, int ZF>
 __global__ __launch_bounds__(NT) void k_re_finalize(K P, const double* __restrict__ Kx, const double* __restrict__ Ky,
-                                                     const unsigned char* __restrict__ sat0, const unsigned char* __restrict__ fin,
-                                                     const double* __restrict__ wch, const double* __restrict__ hwc, const double* __restrict__ hn,
-                                                     double* __restrict__ wc, double* __restrict__ h, unsigned char* __restrict__ hmod, double* __restrict__ part,
-                                                     double* __restrict__ mx, unsigned int nblocks) {
+                                                     const unsigned char* __restrict__ sat0,
+                                                     const unsigned char* __restrict__ fin,
+                                                     const double* __restrict__ wch, const double* __restrict__ hwc,
+                                                     const double* __restrict__ hn, double* __restrict__ wc,
+                                                     double* __restrict__ h, unsigned char* __restrict__ hmod,
+                                                     double* __restrict__ part, double* __restrict__ mx,
+                                                     unsigned int nblocks) {
   __shared__ double lds[NT / kWave];
   __shared__ double lmax[NT / kWave];
-  const Cell t = cell_of(P.g);
+  TileId b;
+  const Cell t = tile_cell<ZF>(P.g, b);
   double l_after = 0.0, l_adj = 0.0, dh = 0.0;
   if (t.valid) re_finalize(P, t, Kx, Ky, sat0, fin, wch, hwc, hn, wc, h, hmod, l_after, l_adj, dh);
   const double a = block_sum<NT>(l_after, lds);
-  const double b = block_sum<NT>(l_adj, lds);
+  const double bs = block_sum<NT>(l_adj, lds);
   dh = wave_max(dh);
   if ((threadIdx.x % kWave) == 0) lmax[threadIdx.x / kWave] = dh;
   __syncthreads();
-  const unsigned int bid = xcd_tile().linear;
   if (threadIdx.x == 0) {
     double m = lmax[0];
     for (int q = 1; q < NT / kWave; ++q) m = fmax(m, lmax[q]);
-    part[3 * nblocks + bid] = a;
-    part[4 * nblocks + bid] = b;
-    mx[bid] = m;
-    mx[nblocks + bid] = 0.0;
+    part[3 * nblocks + b.linear] = a;
+    part[4 * nblocks + b.linear] = bs;
+    mx[b.linear] = m;
+    mx[nblocks + b.linear] = 0.0;
   }
 }
 
@@ -190,7 +219,8 @@ ReState::ReState(int nx, int ny, int nz, const frehg_re_params& p, hipStream_t s
     FREHG_HIP(hipMalloc(f, (size_t)n_));
     FREHG_HIP(hipMemsetAsync(*f, 0, (size_t)n_, s_));
   }
-  FREHG_HIP(hipMalloc(&dflag_, 16));
+  FREHG_HIP(hipMalloc(&dflag_, kSweepFlags * sizeof(int)));
+  FREHG_HIP(hipMemsetAsync(dflag_, 0, kSweepFlags * sizeof(int), s_));
   FREHG_HIP(hipHostMalloc(&hflag_, 16, hipHostMallocDefault));
   const unsigned int nb = ceil_div(nx, 64) * ceil_div(ny, 4) * nz;
   FREHG_HIP(hipMalloc(&red_, 2 * (size_t)nb * sizeof(double)));
@@ -360,7 +390,7 @@ void ReState::learned_increment(double* dst) {
   kc_fresh_ = false;  // kr and C are re-evaluated at h below
   const ReK P = make_rek(g_, soil_view(), p_, qtop_);
   const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4), g_.nz);
-  hipLaunchKernelGGL(k_re_pre<ReK>, grid, dim3(NT), 0, s_, P, (const double*)h_, kr_, ch_);
+  hipLaunchKernelGGL((k_re_pre<ReK, 1>), grid, dim3(NT), 0, s_, P, (const double*)h_, kr_, ch_);
   FREHG_LAUNCH_CHECK();
   double* tmp = nullptr;
   FREHG_HIP(hipMalloc(&tmp, (size_t)n_ * sizeof(double)));
@@ -409,12 +439,12 @@ void ReState::step(frehg_re_stats* st) {
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
       if (kc_fresh)
-        hipExtLaunchKernelGGL(k_re_pre_stale<K>, grid, blk, 0, s_, ev0, nullptr, 0, PK, (const unsigned char*)hmod_,
-                              (const double*)hn_, kr_, ch_);
+        hipExtLaunchKernelGGL((k_re_pre_stale<K, 1>), grid, blk, 0, s_, ev0, nullptr, 0, PK,
+                              (const unsigned char*)hmod_, (const double*)hn_, kr_, ch_);
       else
-        hipExtLaunchKernelGGL(k_re_pre<K>, grid, blk, 0, s_, ev0, nullptr, 0, PK, (const double*)hn_, kr_, ch_);
+        hipExtLaunchKernelGGL((k_re_pre<K, 1>), grid, blk, 0, s_, ev0, nullptr, 0, PK, (const double*)hn_, kr_, ch_);
       FREHG_LAUNCH_CHECK();
-      hipExtLaunchKernelGGL(k_re_assemble<K>, grid, blk, 0, s_, nullptr, ev1, 0, PK, (const double*)kr_,
+      hipExtLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, nullptr, ev1, 0, PK, (const double*)kr_,
                             (const double*)ch_, (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_,
                             mixed ? D32_ : nullptr, ax32_, ay32_, az32_);
       FREHG_LAUNCH_CHECK();
@@ -450,38 +480,38 @@ void ReState::step(frehg_re_stats* st) {
     hipEvent_t ev1 = prof_->stop();
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
-      hipExtLaunchKernelGGL(k_re_post<K>, grid, blk, 0, s_, ev0, nullptr, 0, PK, h_, kr_, ch_, wch_);
+      hipExtLaunchKernelGGL((k_re_post<K, 1>), grid, blk, 0, s_, ev0, nullptr, 0, PK, h_, kr_, ch_, wch_);
       FREHG_LAUNCH_CHECK();
-      hipExtLaunchKernelGGL(k_re_flux_correct<K>, grid, blk, 0, s_, nullptr, ev1, 0, PK, (const double*)h_,
-                            (const double*)kr_, (const double*)hn_, (const double*)wcn_, (const double*)wch_, Kx_,
-                            Ky_, Kz_, qx_, qy_, qz_, wc_, hwc_, sat0_, cand_, led_, nbl);
+      hipExtLaunchKernelGGL((k_re_flux_correct<K, 1>), grid, blk, 0, s_, nullptr, ev1, 0, PK, (const double*)h_,
+                            (const double*)kr_, (const double*)hn_, (const double*)wcn_, (const double*)wch_, Kx_, Ky_,
+                            Kz_, qx_, qy_, qz_, wc_, hwc_, sat0_, cand_, led_, nbl, dflag_, kSweepFlags);
       FREHG_LAUNCH_CHECK();
     });
   }
-  // Reallocation fixed point: iterate from sat0 until the saturation state stops changing.
-  FREHG_HIP(hipMemcpyAsync(satA_, sat0_, (size_t)n_, hipMemcpyDeviceToDevice, s_));
-  FREHG_HIP(hipMemcpyAsync(satB_, sat0_, (size_t)n_, hipMemcpyDeviceToDevice, s_));
-  unsigned char* prev = satA_;
-  unsigned char* next = satB_;
+  // Reallocation fixed point: sweep until the saturation state stops changing. Sweep 0 reads
+  // sat0 itself; later sweeps alternate between satA and satB. Sweep s reports a change in
+  // dflag_[s] (cleared by the flux kernel above).
+  const unsigned char* prev = sat0_;
   for (int it = 0; it < g_.nx * g_.ny * g_.nz + 1; ++it) {
-    FREHG_HIP(hipMemsetAsync(dflag_, 0, sizeof(int), s_));
+    unsigned char* next = (it & 1) ? satB_ : satA_;
+    const int f = it < kSweepFlags ? it : kSweepFlags - 1;
+    if (it >= kSweepFlags) FREHG_HIP(hipMemsetAsync(dflag_ + f, 0, sizeof(int), s_));
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
       hipLaunchKernelGGL(k_re_realloc_iter<K>, grid, blk, 0, s_, PK, (const double*)Kx_, (const double*)Ky_,
-                         (const unsigned char*)sat0_, (const unsigned char*)cand_, (const unsigned char*)prev, next,
-                         dflag_);
+                         (const unsigned char*)sat0_, (const unsigned char*)cand_, prev, next, dflag_ + f);
       FREHG_LAUNCH_CHECK();
     });
-    FREHG_HIP(hipMemcpyAsync(hflag_, dflag_, sizeof(int), hipMemcpyDeviceToHost, s_));
+    FREHG_HIP(hipMemcpyAsync(hflag_, dflag_ + f, sizeof(int), hipMemcpyDeviceToHost, s_));
     FREHG_HIP(hipStreamSynchronize(s_));
-    std::swap(prev, next);
+    prev = next;
     if (hflag_[0] == 0) break;
   }
   with_soil_variant(P, [&](const auto& PK) {
     using K = std::decay_t<decltype(PK)>;
-    hipLaunchKernelGGL(k_re_finalize<K>, grid, blk, 0, s_, PK, (const double*)Kx_, (const double*)Ky_,
-                       (const unsigned char*)sat0_, (const unsigned char*)prev, (const double*)wch_,
-                       (const double*)hwc_, (const double*)hn_, wc_, h_, hmod_, led_, led_ + 5 * (size_t)nbl, nbl);
+    hipLaunchKernelGGL((k_re_finalize<K, 0>), grid, blk, 0, s_, PK, (const double*)Kx_, (const double*)Ky_,
+                       (const unsigned char*)sat0_, prev, (const double*)wch_, (const double*)hwc_, (const double*)hn_,
+                       wc_, h_, hmod_, led_, led_ + 5 * (size_t)nbl, nbl);
     FREHG_LAUNCH_CHECK();
   });
   // ledger: [before, storage_ss, boundary_in, after, adjust] sums, then max |dh|

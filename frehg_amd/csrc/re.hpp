@@ -83,7 +83,8 @@ class ReState {
   // the state (fields, soil, initial condition, parameters) clears it.
   unsigned char* hmod_ = nullptr;
   bool kc_fresh_ = false;
-  int* dflag_ = nullptr;   // device: [0] changed flag
+  static constexpr int kSweepFlags = 64;
+  int* dflag_ = nullptr;   // device: per-sweep "saturation state changed" flags
   int* hflag_ = nullptr;   // pinned host mirror
   double* red_ = nullptr;  // reduction partials (adaptive dt)
   double* red2_ = nullptr; // first-stage sums of the ledger reduction

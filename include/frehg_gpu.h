@@ -276,8 +276,8 @@ enum {
   FREHG_KT_GW_UPDATE = 2,   /* groundwater Krylov vector update + reductions (FP64 and FP32) */
   FREHG_KT_GW_ASSEMBLE = 3, /* K, C at h^n + 7-point assembly (one span per step) */
   FREHG_KT_GW_CORRECT = 4,  /* ghosts, K/C at h^{n+1}, fluxes, corrector (one span per step) */
-  FREHG_KT_SW_SPMV = 5,     /* surface Krylov SpMV+dot launches (5-point) */
-  FREHG_KT_SW_UPDATE = 6,   /* surface Krylov vector update + reductions */
+  FREHG_KT_SW_SPMV = 5,     /* surface Krylov SpMV (reserved: the launch-bound 2D solve is not timed) */
+  FREHG_KT_SW_UPDATE = 6,   /* surface Krylov update (reserved, not timed) */
   FREHG_KT_GW_REFINE = 7,   /* groundwater FP64 residual sweep of iterative refinement */
   FREHG_KT_GW_GUESS = 8,    /* learned-guess inference (features + MLP forward) */
   FREHG_KT_COUNT = 9
