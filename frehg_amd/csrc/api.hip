@@ -468,6 +468,13 @@ int frehg_profile_enable(frehg_ctx_t ctx, int enable) {
   });
 }
 
+int frehg_profile_select(frehg_ctx_t ctx, uint32_t kernel_mask) {
+  return guard(ctx, [&] {
+    ctx->prof.mask = kernel_mask;
+    return FREHG_OK;
+  });
+}
+
 int frehg_profile_read(frehg_ctx_t ctx, int kt, double* total_ms, int64_t* launches) {
   return guard(ctx, [&] {
     if (kt < 0 || kt >= FREHG_KT_COUNT) throw frehg::ArgError("bad kernel type");

@@ -270,7 +270,6 @@ struct Sten {
   const T *D, *ax, *ay, *az;
 };
 
-__device__ __forceinline__ KrylovCtl& kctl(EnsCtl* ctl, int b, int inner) { return inner ? ctl[b].i : ctl[b].o; }
 
 // r = b - A x (or b), z = r / D, p_prev = 0; partials r.z, z.z, (b/D)^2.
 template <typename T>
@@ -837,7 +836,6 @@ void Ensemble::enq_prepare() {
 void Ensemble::enq_solve(int inner_iters, int refinements, int inner_first) {
   const dim3 grid(ntile_), gsp(nspmv_), blk(NT), gb(nbox_);
   const EnsBox* bx = dbox_;
-  const unsigned int* t0 = dtile0_;
   const unsigned int* s0 = dspmv0_;
   auto iterate = [&](auto tag, Sten<decltype(tag)> A, decltype(tag)* x, decltype(tag)* r, decltype(tag)* z,
                      decltype(tag)* p0, decltype(tag)* p1, decltype(tag)* q, int inner, int n) {

@@ -1,9 +1,10 @@
 // Jacobi-PCG and Jacobi-BiCGSTAB kernels for the 7-/5-point stencil systems of the Richards and
-// shallow-water steps. One CG iteration is two kernels:
-//   spmv:   p = z + beta p_prev (recomputed at each stencil point), q = A p, partial p.q;
-//           last block: alpha = rz / p.q
-//   update: x += alpha p, r -= alpha q, z = r / D, partials r.z and z.z;
-//           last block: convergence test, beta, rz
+// shallow-water steps. One CG iteration is two kernels, with no reduction kernel in between:
+//   spmv:   every block first reduces the previous update's partials r.z, z.z (convergence test,
+//           beta = r.z / r.z_old), then p = z + beta p_prev (recomputed at each stencil point),
+//           q = A p, partial p.q
+//   update: every block first reduces the spmv's partials p.q (alpha = r.z / p.q), then
+//           x += alpha p, r -= alpha q, z = r / D, partials r.z and z.z
 // Each thread owns one (i, j) column of a 64 x 4 tile and marches down k over a chunk of
 // layers, keeping the k-1 / k / k+1 values of p and the az coupling of the layer above in
 // registers, so every array is streamed from HBM once per iteration.
@@ -145,6 +146,8 @@ __global__ __launch_bounds__(kRedThreads) void k_fin_init(const double* part, un
   if (threadIdx.x == 0) {
     const double ttol = fmax(rtol * sqrt(bb), atol);
     ctl->rz = rz;
+    ctl->rzb[0] = rz;
+    ctl->rzb[1] = rz;
     ctl->zz = zz;
     ctl->bb = bb;
     ctl->ttol2 = ttol * ttol;
@@ -158,6 +161,61 @@ __global__ __launch_bounds__(kRedThreads) void k_fin_init(const double* part, un
     ctl->converged = (zz <= ctl->ttol2) ? 1 : 0;
     ctl->done = (ctl->converged || ctl->guess_rejected) ? 1 : 0;
   }
+}
+
+// Start of CG iteration `it` in the SpMV. For it > 0, PETSc's convergence test on the residual
+// of update it - 1 and beta = r.z / r.z_old, from that update's partials, which every block reduces
+// itself in the same order (so all blocks agree bit for bit); the lead block records the outcome
+// in ctl. False when the solve has ended (the block then does nothing). The done flag is read by
+// one thread and shared through LDS, so a block's threads agree even while the lead block of the
+// same launch is setting it (both reach the same decision).
+template <int NTB>
+__device__ inline bool cg_spmv_begin(KrylovCtl* ctl, const double* part_u, unsigned int nb_u, int it, bool lead,
+                                     double* lds, int* sflag, double* beta) {
+  if (threadIdx.x == 0) *sflag = ctl->done;
+  __syncthreads();
+  if (*sflag) return false;
+  if (it == 0) {
+    *beta = 0.0;
+    return true;
+  }
+  const double rz = block_sum_partials<NTB>(part_u, nb_u, lds);
+  const double zz = block_sum_partials<NTB>(part_u + nb_u, nb_u, lds);
+  const bool conv = zz <= ctl->ttol2;
+  const bool stop = conv || it >= ctl->max_it;
+  if (lead && threadIdx.x == 0) {
+    ctl->iters = it;
+    ctl->zz = zz;
+    ctl->rz = rz;
+    ctl->rzb[it & 1] = rz;
+    if (stop) {
+      ctl->converged = conv ? 1 : 0;
+      ctl->done = 1;
+    }
+  }
+  *beta = rz / ctl->rzb[(it - 1) & 1];
+  return !stop;
+}
+
+// Start of the CG update of iteration `it`: alpha = r.z / p.q from the SpMV's partials (reduced by
+// every block); p.q <= 0 is a breakdown (recorded by the lead block; every block stops).
+template <int NTB>
+__device__ inline bool cg_update_begin(KrylovCtl* ctl, const double* part_s, unsigned int nb_s, int it, bool lead,
+                                       double* lds, int* sflag, double* alpha) {
+  if (threadIdx.x == 0) *sflag = ctl->done;
+  __syncthreads();
+  if (*sflag) return false;
+  const double pq = block_sum_partials<NTB>(part_s, nb_s, lds);
+  const bool brk = !(pq > 0.0) || !isfinite(pq);
+  if (lead && threadIdx.x == 0) {
+    ctl->pq = pq;
+    if (brk) {
+      ctl->breakdown = 1;
+      ctl->done = 1;
+    }
+  }
+  *alpha = ctl->rzb[it & 1] / pq;
+  return !brk;
 }
 
 // Operands of one layer of the z-march: z and p_prev at the four lateral neighbours and the
@@ -194,10 +252,12 @@ __device__ __forceinline__ void spmv_load(const Geo& g, const StencilView<T>& A,
 template <typename T, bool THREE_D>
 __global__ __launch_bounds__(NT) void k_spmv(Geo g, int kchunk, StencilView<T> A, const T* __restrict__ z,
                                               const T* __restrict__ pold, T* __restrict__ pnew, T* __restrict__ q,
-                                              double* part, const KrylovCtl* ctl) {
+                                              double* part, KrylovCtl* ctl, const double* part_u, unsigned int nb_u,
+                                              int it) {
   __shared__ double lds[NT / kWave];
-  if (ctl->done) return;
-  const double beta = ctl->beta;
+  __shared__ int sflag;
+  double beta;
+  if (!cg_spmv_begin<NT>(ctl, part_u, nb_u, it, block_id() == 0, lds, &sflag, &beta)) return;
   const Tile t = tile_of(g, kchunk);
   double acc = 0.0;
   if (t.valid) {
@@ -295,11 +355,13 @@ __device__ __forceinline__ void spmv_pair_load(const Geo& g, const StencilView<T
 template <typename T>
 __global__ __launch_bounds__(NT) void k_spmv_pair(Geo g, int kchunk, StencilView<T> A, const T* __restrict__ z,
                                                    const T* __restrict__ pold, T* __restrict__ pnew,
-                                                   T* __restrict__ q, double* part, const KrylovCtl* ctl) {
+                                                   T* __restrict__ q, double* part, KrylovCtl* ctl,
+                                                   const double* part_u, unsigned int nb_u, int it) {
   __shared__ double lds[NT / kWave];
-  if (ctl->done) return;  // uniform: every thread reads the same flag
-  const double beta = ctl->beta;
+  __shared__ int sflag;
   const TileId b = xcd_tile();
+  double beta;
+  if (!cg_spmv_begin<NT>(ctl, part_u, nb_u, it, b.linear == 0, lds, &sflag, &beta)) return;
   const int lane = threadIdx.x & 63;
   const int i0 = b.x * 128, i = i0 + 2 * lane, j = b.y * TY + (threadIdx.x >> 6);
   const int k0 = b.z * kchunk, k1 = min(g.nz, k0 + kchunk);
@@ -358,73 +420,68 @@ __global__ __launch_bounds__(NT) void k_spmv_pair(Geo g, int kchunk, StencilView
   if (threadIdx.x == 0) part[block_id()] = s;
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_fin_spmv(const double* part, unsigned int nblocks, KrylovCtl* ctl) {
-  __shared__ double lds[kRedThreads / kWave];
-  if (ctl->done) return;
-  const double pq = block_sum_partials<kRedThreads>(part, nblocks, lds);
-  if (threadIdx.x == 0) {
-    ctl->pq = pq;
-    if (!(pq > 0.0) || !isfinite(pq)) {
-      ctl->breakdown = 1;
-      ctl->done = 1;
-    } else {
-      ctl->alpha = ctl->rz / pq;
-    }
-  }
-}
-
+// CG update of iteration `it` on 128 x 4 tiles of cell pairs (paired aligned loads), kchunk
+// layers per block (2D: one): x += alpha p, r -= alpha q, z = r / D, partials r.z and z.z to
+// part_u[0..nb) and part_u[nb..2nb). A pair straddling the row end updates its first cell only.
 template <typename T, bool THREE_D>
-__global__ __launch_bounds__(NT) void k_update(Geo g, int kchunk, const T* D, const T* p, const T* q,
-                                                T* x, T* r, T* z, double* part, const KrylovCtl* ctl,
-                                                unsigned int nblocks) {
+__global__ __launch_bounds__(NT) void k_cg_update(Geo g, int kchunk, const T* __restrict__ D,
+                                                   const T* __restrict__ p, const T* __restrict__ q,
+                                                   T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                   double* __restrict__ part_u, KrylovCtl* ctl,
+                                                   const double* __restrict__ part_s, unsigned int nb_s, int it) {
   __shared__ double lds[NT / kWave];
-  if (ctl->done) return;
-  const double alpha = ctl->alpha;
-  const Tile t = tile_of(g, kchunk);
+  __shared__ int sflag;
+  const TileId b = xcd_tile();
+  double alpha;
+  if (!cg_update_begin<NT>(ctl, part_s, nb_s, it, b.linear == 0, lds, &sflag, &alpha)) return;
+  const int i = b.x * 128 + 2 * (threadIdx.x & 63), j = b.y * TY + (threadIdx.x >> 6);
+  const int k0 = THREE_D ? b.z * kchunk : 0, k1 = THREE_D ? min(g.nz, k0 + kchunk) : 1;
   double srz = 0.0, szz = 0.0;
-  if (t.valid) {
-    long long c = base_index<THREE_D>(g, t);
-    for (int k = t.k0; k < t.k1; ++k, c += g.sz) {
-      const double xv = (double)x[c] + alpha * (double)p[c];
-      const double rv = (double)r[c] - alpha * (double)q[c];
-      const T rT = (T)rv;
-      const T zT = (T)((double)rT / (double)D[c]);
-      x[c] = (T)xv;
-      r[c] = rT;
-      z[c] = zT;
-      srz += (double)rT * (double)zT;
-      szz += (double)zT * (double)zT;
+  auto cell = [&](T xc, T pc, T rc, T qc, T dc, T& xo, T& ro, T& zo) {
+    const double xv = (double)xc + alpha * (double)pc;
+    const double rv = (double)rc - alpha * (double)qc;
+    const T rT = (T)rv;
+    const T zT = (T)((double)rT / (double)dc);
+    xo = (T)xv;
+    ro = rT;
+    zo = zT;
+    srz += (double)rT * (double)zT;
+    szz += (double)zT * (double)zT;
+  };
+  if (i < g.nx && j < g.ny) {
+    long long c = THREE_D ? g.idx(i, j, k0) : g.sidx(i, j);
+    if (i + 1 < g.nx) {
+#pragma unroll 2
+      for (int k = k0; k < k1; ++k, c += g.sz) {
+        const Pair<T> xv = ld_pair(x, c), pv = ld_pair(p, c), rv = ld_pair(r, c), qv = ld_pair(q, c),
+                      dv = ld_pair(D, c);
+        Pair<T> xo, ro, zo;
+        cell(xv.a, pv.a, rv.a, qv.a, dv.a, xo.a, ro.a, zo.a);
+        cell(xv.b, pv.b, rv.b, qv.b, dv.b, xo.b, ro.b, zo.b);
+        *reinterpret_cast<Pair<T>*>(x + c) = xo;
+        *reinterpret_cast<Pair<T>*>(r + c) = ro;
+        *reinterpret_cast<Pair<T>*>(z + c) = zo;
+      }
+    } else {
+      for (int k = k0; k < k1; ++k, c += g.sz) cell(x[c], p[c], r[c], q[c], D[c], x[c], r[c], z[c]);
     }
   }
   const double a = block_sum<NT>(srz, lds);
-  const double b = block_sum<NT>(szz, lds);
-  const unsigned int bid = block_id();
+  const double bs = block_sum<NT>(szz, lds);
   if (threadIdx.x == 0) {
-    part[bid] = a;
-    part[nblocks + bid] = b;
+    const unsigned int nb = gridDim.x * gridDim.y * gridDim.z;
+    part_u[b.linear] = a;
+    part_u[nb + b.linear] = bs;
   }
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_fin_update(const double* part, unsigned int nblocks,
-                                                            KrylovCtl* ctl) {
-  __shared__ double lds[kRedThreads / kWave];
-  if (ctl->done) return;
-  const double rz = block_sum_partials<kRedThreads>(part, nblocks, lds);
-  const double zz = block_sum_partials<kRedThreads>(part + nblocks, nblocks, lds);
-  if (threadIdx.x == 0) {
-    const int it = ctl->iters + 1;
-    ctl->iters = it;
-    ctl->zz = zz;
-    if (zz <= ctl->ttol2) {
-      ctl->converged = 1;
-      ctl->done = 1;
-    } else if (it >= ctl->max_it) {
-      ctl->done = 1;
-    } else {
-      ctl->beta = rz / ctl->rz;
-    }
-    ctl->rz = rz;
-  }
+// After the last update of a batch: the test the next SpMV would make (same reduction, same
+// block size, so the same bits), so that the host sees the outcome of the batch.
+__global__ __launch_bounds__(NT) void k_cg_check(KrylovCtl* ctl, const double* part_u, unsigned int nb_u, int it) {
+  __shared__ double lds[NT / kWave];
+  __shared__ int sflag;
+  double beta;
+  (void)cg_spmv_begin<NT>(ctl, part_u, nb_u, it, true, lds, &sflag, &beta);
 }
 
 // Iterative-refinement residual: xout = xin + d (d may be null), r32 = b - A xout in FP64,
@@ -747,6 +804,13 @@ Krylov::Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof)
   // partials sized for the largest grid any kernel (or candidate) uses: 64 x 4 tiles, one layer each
   part_cap_ = ceil_div(g_.nx, TX) * ceil_div(g_.ny, TY) * (unsigned int)(three_d_ ? g_.nz : 1);
   FREHG_HIP(hipMalloc(&part_, 3 * (size_t)part_cap_ * sizeof(double)));
+  FREHG_HIP(hipMalloc(&part_u_, 2 * (size_t)part_cap_ * sizeof(double)));
+  // CG update: whole columns per block in 3D (its partials are reduced by every SpMV block, so
+  // their number is kept small)
+  kc_u_ = three_d_ ? g_.nz : 1;
+  if (const char* env = std::getenv("FREHG_CG_UPDATE_KC")) kc_u_ = std::max(1, std::min(three_d_ ? g_.nz : 1, std::atoi(env)));
+  grid_u_ = dim3(ceil_div(g_.nx, 2 * TX), ceil_div(g_.ny, TY), three_d_ ? ceil_div(g_.nz, kc_u_) : 1);
+  nb_u_ = grid_u_.x * grid_u_.y * grid_u_.z;
   for (int prec = 0; prec < 2; ++prec) set_spmv_shape(prec, 1, three_d_ ? std::min(g_.nz, 4) : 1);
   // FREHG_SPMV_SHAPE="cpl:kc" pins the SpMV shape (cells per lane 1 or 2, kc layers per block)
   // and skips the autotuner.
@@ -775,7 +839,7 @@ Krylov::~Krylov() {
   for (void* p : {(void*)r_, (void*)z_, (void*)p0_, (void*)p1_, (void*)q_, (void*)xw_, (void*)b32_,
                   (void*)r32_, (void*)z32_, (void*)p032_, (void*)p132_, (void*)q32_, (void*)d32_,
                   (void*)rh_, (void*)v0_, (void*)v1_, (void*)sv_, (void*)tv_, (void*)rh32_, (void*)v032_,
-                  (void*)v132_, (void*)sv32_, (void*)tv32_, (void*)part_, (void*)ctl_})
+                  (void*)v132_, (void*)sv32_, (void*)tv32_, (void*)part_, (void*)part_u_, (void*)ctl_})
     if (p) (void)hipFree(p);
   if (host_ctl_) (void)hipHostFree(host_ctl_);
 }
@@ -793,18 +857,19 @@ void Krylov::set_spmv_shape(int prec, int cpl, int kc) {
 }
 
 template <typename T>
-void Krylov::launch_spmv(const StencilView<T>& A, const T* z, const T* pold, T* pnew, T* q, const KrylovCtl* ctl,
+void Krylov::launch_spmv(const StencilView<T>& A, const T* z, const T* pold, T* pnew, T* q, KrylovCtl* ctl, int it,
                          hipEvent_t ev0, hipEvent_t ev1) {
   const SpmvShape& sh = shape_[sizeof(T) == 4 ? 1 : 0];
+  const double* pu = part_u_;
   if (!three_d_) {
     hipExtLaunchKernelGGL((k_spmv<T, false>), sh.grid, dim3(NT), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold, pnew, q,
-                          part_, ctl);
+                          part_, ctl, pu, nb_u_, it);
   } else if (sh.cpl == 1) {
     hipExtLaunchKernelGGL((k_spmv<T, true>), sh.grid, dim3(NT), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold, pnew, q,
-                          part_, ctl);
+                          part_, ctl, pu, nb_u_, it);
   } else {
     hipExtLaunchKernelGGL((k_spmv_pair<T>), sh.grid, dim3(NT), 0, s_, ev0, ev1, 0, g_, sh.kc, A, z, pold, pnew, q,
-                          part_, ctl);
+                          part_, ctl, pu, nb_u_, it);
   }
   FREHG_LAUNCH_CHECK();
 }
@@ -835,7 +900,7 @@ void Krylov::autotune(const StencilView<T>& A) {
       float ms = 0.f;
       for (int rep = 0; rep < 2; ++rep) {  // the first round warms caches and clocks
         FREHG_HIP(hipEventRecord(e0, s_));
-        for (int it = 0; it < 3; ++it) launch_spmv<T>(A, A.D, p0, p1, qq, tctl, nullptr, nullptr);
+        for (int it = 0; it < 3; ++it) launch_spmv<T>(A, A.D, p0, p1, qq, tctl, 0, nullptr, nullptr);
         FREHG_HIP(hipEventRecord(e1, s_));
         FREHG_HIP(hipEventSynchronize(e1));
         FREHG_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -918,7 +983,8 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
   int batch = std::max(2, est + 1);
   int launched = 0;
   for (;;) {
-    for (int it = 0; it < batch; ++it) {
+    for (int b = 0; b < batch; ++b) {
+      const int it = launched + b;
       T* pold = pbuf[parity];
       T* pnew = pbuf[parity ^ 1];
       // Only the 3D (groundwater) launches are timed: the 2D surface solve is launch-bound, and
@@ -926,24 +992,25 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
       {
         hipEvent_t ev0 = three_d_ ? prof_->start(kt_spmv) : nullptr;
         hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
-        launch_spmv<T>(A, z, pold, pnew, q, ctl, ev0, ev1);
+        launch_spmv<T>(A, z, pold, pnew, q, ctl, it, ev0, ev1);
       }
-      hipLaunchKernelGGL(k_fin_spmv, dim3(1), dim3(kRedThreads), 0, s_, part_, shape_[sizeof(T) == 4 ? 1 : 0].nb, ctl);
-      FREHG_LAUNCH_CHECK();
       {
         hipEvent_t ev0 = three_d_ ? prof_->start(FREHG_KT_GW_UPDATE) : nullptr;
         hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
+        const unsigned int nbs = shape_[sizeof(T) == 4 ? 1 : 0].nb;
         if (three_d_)
-          hipExtLaunchKernelGGL((k_update<T, true>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A.D, pnew, q, x, r, z, part_, ctl, nb_f_);
+          hipExtLaunchKernelGGL((k_cg_update<T, true>), grid_u_, blk, 0, s_, ev0, ev1, 0, g_, kc_u_, A.D,
+                                (const T*)pnew, (const T*)q, x, r, z, part_u_, ctl, (const double*)part_, nbs, it);
         else
-          hipExtLaunchKernelGGL((k_update<T, false>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A.D, pnew, q, x, r, z, part_, ctl, nb_f_);
+          hipExtLaunchKernelGGL((k_cg_update<T, false>), grid_u_, blk, 0, s_, ev0, ev1, 0, g_, kc_u_, A.D,
+                                (const T*)pnew, (const T*)q, x, r, z, part_u_, ctl, (const double*)part_, nbs, it);
         FREHG_LAUNCH_CHECK();
       }
-      hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kRedThreads), 0, s_, part_, nb_f_, ctl);
-      FREHG_LAUNCH_CHECK();
       parity ^= 1;
     }
     launched += batch;
+    hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(NT), 0, s_, ctl, (const double*)part_u_, nb_u_, launched);
+    FREHG_LAUNCH_CHECK();
     read_ctl(ctl, host_ctl_);
     if (host_ctl_->done) break;
     if (launched >= max_it + 2) break;
@@ -956,7 +1023,10 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
                            ")");
   const int iters = host_ctl_->iters;
   // each launched 3D iteration recorded one spmv and one update launch
-  if (three_d_ && launched > iters) prof_->discard_tail(2 * (size_t)(launched - iters));
+  if (three_d_ && launched > iters) {
+    prof_->discard_last(kt_spmv, (size_t)(launched - iters));
+    prof_->discard_last(FREHG_KT_GW_UPDATE, (size_t)(launched - iters));
+  }
   if (sizeof(T) == 8) last_iters_ = iters;
   else last_inner_iters_ = iters;
   *res_norm = std::sqrt(host_ctl_->zz);
@@ -1064,7 +1134,10 @@ int Krylov::run_bicgstab(const StencilView<T>& A, const T* b, const T* bn, T* x,
   }
   const int iters = host_ctl_->iters;
   // each launched 3D iteration recorded two spmv launches and one update launch
-  if (three_d_ && launched > iters) prof_->discard_tail(3 * (size_t)(launched - iters));
+  if (three_d_ && launched > iters) {
+    prof_->discard_last(kt_spmv, 2 * (size_t)(launched - iters));
+    prof_->discard_last(FREHG_KT_GW_UPDATE, (size_t)(launched - iters));
+  }
   last = iters;
   *res_norm = std::sqrt(host_ctl_->zz);
   *rhs_norm = std::sqrt(host_ctl_->bb);

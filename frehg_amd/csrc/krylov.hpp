@@ -9,12 +9,14 @@
 
 namespace frehg {
 
-// Device-resident control block of one Krylov solve. After each vector kernel a one-block
-// kernel reduces its per-block partials and writes the scalars the next kernel reads, so a
-// whole iteration runs without a host round trip; kernels return immediately once `done` is set.
+// Device-resident control block of one Krylov solve, so that whole iterations run without a host
+// round trip; kernels return immediately once `done` is set. CG: each of the two kernels of an
+// iteration reduces the other's per-block partials itself (every block, in the same order), and
+// block 0 records the scalars here. BiCGSTAB: one-block kernels reduce the partials in between.
 struct KrylovCtl {
   double rz, alpha, beta, zz, ttol2, bb, pq, rr_guess;
   double omega;  // BiCGSTAB: rz holds rho = rh.r, pq holds rh.v
+  double rzb[2];  // CG: r.z of iterate `it` in rzb[it & 1]
   int iters, done, converged, breakdown, max_it, guess_rejected;
   int half;  // BiCGSTAB: converged on the intermediate residual s
 };
@@ -114,7 +116,7 @@ class Krylov {
               T* r, T* z, T* p0, T* p1, T* q, bool* rejected, double* res_norm, double* rhs_norm,
               bool* converged, int kt_spmv);
   template <typename T>
-  void launch_spmv(const StencilView<T>& A, const T* z, const T* pold, T* pnew, T* q, const KrylovCtl* ctl,
+  void launch_spmv(const StencilView<T>& A, const T* z, const T* pold, T* pnew, T* q, KrylovCtl* ctl, int it,
                    hipEvent_t ev0, hipEvent_t ev1);
   template <typename T>
   int run_bicgstab(const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed,
@@ -136,6 +138,11 @@ class Krylov {
   int kc_f_ = 1;
   dim3 grid_f_;
   unsigned int nb_f_ = 0, part_cap_ = 0;
+  // CG update: 128 x 4 tiles of cell pairs, kc_u_ layers per block; its partials (r.z, z.z)
+  int kc_u_ = 1;
+  dim3 grid_u_;
+  unsigned int nb_u_ = 0;
+  double* part_u_ = nullptr;
   bool tuned_[2] = {false, false};
   long long n_ = 0;  // padded length
   double *r_ = nullptr, *z_ = nullptr, *p0_ = nullptr, *p1_ = nullptr, *q_ = nullptr, *xw_ = nullptr;

@@ -17,6 +17,7 @@ constexpr int kKernelTypes = 9;  // FREHG_KT_COUNT
 class Profiler {
  public:
   bool enabled = false;
+  unsigned int mask = ~0u;  // kernel types timed while enabled (bit kt)
   hipStream_t stream = nullptr;
 
   ~Profiler() {
@@ -27,7 +28,7 @@ class Profiler {
   // start event of the first launch and stop() as the stop event of the last one (both are null,
   // i.e. an ordinary launch, while profiling is off). Call start before stop.
   hipEvent_t start(int kt) {
-    if (!enabled) return nullptr;
+    if (!enabled || !((mask >> kt) & 1u)) return nullptr;
     pending_kt_ = kt;
     pending_begin_ = take();
     return pending_begin_;
@@ -39,15 +40,16 @@ class Profiler {
     pending_kt_ = -1;
     return e;
   }
-  // Drop the `n` most recent records (launches that returned immediately because the solve had
-  // already converged), so averages cover only launches that did work.
-  void discard_tail(size_t n) {
+  // Drop the `n` most recent records of type `kt` (launches that returned immediately because the
+  // solve had already converged), so averages cover only launches that did work.
+  void discard_last(int kt, size_t n) {
     if (!enabled) return;
-    n = n < recs_.size() ? n : recs_.size();
-    for (size_t q = 0; q < n; ++q) {
-      free_.push_back(recs_.back().e0);
-      free_.push_back(recs_.back().e1);
-      recs_.pop_back();
+    for (size_t q = recs_.size(); q > 0 && n > 0; --q) {
+      if (recs_[q - 1].kt != kt) continue;
+      free_.push_back(recs_[q - 1].e0);
+      free_.push_back(recs_[q - 1].e1);
+      recs_.erase(recs_.begin() + (q - 1));
+      --n;
     }
   }
   // Sum recorded durations into the per-type totals (synchronises on the recorded events).

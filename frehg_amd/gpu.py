@@ -68,6 +68,7 @@ def load():
         "frehg_ls_solve": (C.c_int, [ctx, C.c_int, C.c_int, C.c_int, C.c_int, dp, P(dp), dp, dp,
                                      P(abi.SolverConfig), P(abi.SolveStats)]),
         "frehg_profile_enable": (C.c_int, [ctx, C.c_int]),
+        "frehg_profile_select": (C.c_int, [ctx, C.c_uint32]),
         "frehg_profile_read": (C.c_int, [ctx, C.c_int, dp, P(C.c_int64)]),
         "frehg_profile_reset": (C.c_int, [ctx]),
         "frehg_spmv_shape": (C.c_int, [ctx, C.c_int, P(C.c_int), P(C.c_int)]),
@@ -247,6 +248,13 @@ class Context:
     # ---- measurement -----------------------------------------------------------------------------------
     def profile_enable(self, on=True):
         self.check(self.lib.frehg_profile_enable(self.h, int(on)))
+
+    def profile_select(self, kinds):
+        """Time only the kernel types in `kinds` (an iterable of abi.KT_* ids) while enabled."""
+        mask = 0
+        for k in kinds:
+            mask |= 1 << int(k)
+        self.check(self.lib.frehg_profile_select(self.h, mask))
 
     def profile_read(self, kt):
         ms, n = C.c_double(), C.c_int64()

@@ -283,6 +283,10 @@ enum {
   FREHG_KT_COUNT = 9
 };
 int frehg_profile_enable(frehg_ctx_t ctx, int enable);
+/* Kernel types timed while profiling is enabled: bit kt of kernel_mask (default: all). A timed
+ * launch costs the host a few microseconds more to issue, so a measurement of the whole step
+ * times only the kernel types it reports. */
+int frehg_profile_select(frehg_ctx_t ctx, uint32_t kernel_mask);
 /* Total milliseconds and launch count recorded for kernel `kt` since the last reset. */
 int frehg_profile_read(frehg_ctx_t ctx, int kt, double* total_ms, int64_t* launches);
 int frehg_profile_reset(frehg_ctx_t ctx);
