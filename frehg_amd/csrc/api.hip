@@ -1,6 +1,7 @@
 // C ABI of the gfx950 backend (include/frehg_gpu.h). Every entry point maps exceptions to the
 // return codes of the header; the reference raises std::runtime_error for the same conditions.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -99,7 +100,18 @@ int frehg_create(frehg_ctx_t* out, int device) {
   frehg_ctx_t c = new (std::nothrow) frehg_ctx_s();
   if (!c) return FREHG_ERR_OOM;
   c->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return FREHG_ERR_HIP;
+  }
+  // A step waits on the stream several times (Krylov convergence, reallocation sweeps, ledger);
+  // spin-waiting wakes the host within a microsecond instead of after an interrupt. The flag can
+  // only be set before the device is first used in the process; otherwise the current setting is
+  // kept (FREHG_SYNC=yield keeps the default).
+  const char* sync = std::getenv("FREHG_SYNC");
+  if (!(sync && std::string(sync) == "yield")) (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
+  (void)hipGetLastError();
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
     return FREHG_ERR_HIP;
   }

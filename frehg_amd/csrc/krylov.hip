@@ -950,71 +950,68 @@ void Krylov::read_ctl(KrylovCtl* dev, KrylovCtl* host) {
 }
 
 template <typename T>
-int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed,
-                    bool x0_zero, bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl,
-                    T* r, T* z, T* p0, T* p1, T* q, bool* rejected, double* res_norm,
-                    double* rhs_norm, bool* converged, int kt_spmv) {
+void Krylov::cg_start(const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed, bool zero,
+                      bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl, T* r, T* z, T* p0) {
   const dim3 blk(NT);
   const unsigned int nb = nb_f_;
-  auto init = [&](bool zero) {
-    if (three_d_)
-      hipLaunchKernelGGL((k_init<T, true>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, z, p0, fixed,
-                         zero ? 1 : 0, part_, nb);
-    else
-      hipLaunchKernelGGL((k_init<T, false>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, z, p0, fixed,
-                         zero ? 1 : 0, part_, nb);
-    FREHG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(kRedThreads), 0, s_, part_, nb, ctl, rtol, atol, max_it,
-                       guard ? 1 : 0, zero ? 1 : 0);
-    FREHG_LAUNCH_CHECK();
-  };
-  init(x0_zero);
-  *rejected = false;
-  if (guard && !x0_zero) {
-    read_ctl(ctl, host_ctl_);
-    if (host_ctl_->guess_rejected) {
-      *rejected = true;
-      init(true);
+  if (three_d_)
+    hipLaunchKernelGGL((k_init<T, true>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, z, p0, fixed, zero ? 1 : 0,
+                       part_, nb);
+  else
+    hipLaunchKernelGGL((k_init<T, false>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, z, p0, fixed,
+                       zero ? 1 : 0, part_, nb);
+  FREHG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(kRedThreads), 0, s_, part_, nb, ctl, rtol, atol, max_it,
+                     guard ? 1 : 0, zero ? 1 : 0);
+  FREHG_LAUNCH_CHECK();
+}
+
+// CG iterations from .. from + count - 1 (p_prev of iteration it is p[it & 1]), then the test of
+// the last update (k_cg_check), all without host synchronisation.
+template <typename T>
+void Krylov::cg_iterations(const StencilView<T>& A, T* x, T* r, T* z, T* p0, T* p1, T* q, KrylovCtl* ctl, int from,
+                           int count, int kt_spmv) {
+  const dim3 blk(NT);
+  T* pbuf[2] = {p0, p1};
+  for (int it = from; it < from + count; ++it) {
+    T* pold = pbuf[it & 1];
+    T* pnew = pbuf[(it & 1) ^ 1];
+    // Only the 3D (groundwater) launches are timed: the 2D surface solve is launch-bound, and
+    // launches with timing events take the host several microseconds longer to issue.
+    {
+      hipEvent_t ev0 = three_d_ ? prof_->start(kt_spmv) : nullptr;
+      hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
+      launch_spmv<T>(A, z, pold, pnew, q, ctl, it, ev0, ev1);
+    }
+    {
+      hipEvent_t ev0 = three_d_ ? prof_->start(FREHG_KT_GW_UPDATE) : nullptr;
+      hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
+      const unsigned int nbs = shape_[sizeof(T) == 4 ? 1 : 0].nb;
+      if (three_d_)
+        hipExtLaunchKernelGGL((k_cg_update<T, true>), grid_u_, blk, 0, s_, ev0, ev1, 0, g_, kc_u_, A.D, (const T*)pnew,
+                              (const T*)q, x, r, z, part_u_, ctl, (const double*)part_, nbs, it);
+      else
+        hipExtLaunchKernelGGL((k_cg_update<T, false>), grid_u_, blk, 0, s_, ev0, ev1, 0, g_, kc_u_, A.D,
+                              (const T*)pnew, (const T*)q, x, r, z, part_u_, ctl, (const double*)part_, nbs, it);
+      FREHG_LAUNCH_CHECK();
     }
   }
-  T* pbuf[2] = {p0, p1};
-  int parity = 0;
-  int est = (sizeof(T) == 8) ? last_iters_ : last_inner_iters_;
-  int batch = std::max(2, est + 1);
-  int launched = 0;
+  hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(NT), 0, s_, ctl, (const double*)part_u_, nb_u_, from + count);
+  FREHG_LAUNCH_CHECK();
+}
+
+// Host loop after `launched` queued iterations: wait, and queue further batches until the solve
+// has ended; then the outcome (throws on breakdown).
+template <typename T>
+int Krylov::cg_finish(const StencilView<T>& A, T* x, T* r, T* z, T* p0, T* p1, T* q, KrylovCtl* ctl, int max_it,
+                      int launched, int batch, double* res_norm, double* rhs_norm, bool* converged, int kt_spmv) {
   for (;;) {
-    for (int b = 0; b < batch; ++b) {
-      const int it = launched + b;
-      T* pold = pbuf[parity];
-      T* pnew = pbuf[parity ^ 1];
-      // Only the 3D (groundwater) launches are timed: the 2D surface solve is launch-bound, and
-      // launches with timing events take the host several microseconds longer to issue.
-      {
-        hipEvent_t ev0 = three_d_ ? prof_->start(kt_spmv) : nullptr;
-        hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
-        launch_spmv<T>(A, z, pold, pnew, q, ctl, it, ev0, ev1);
-      }
-      {
-        hipEvent_t ev0 = three_d_ ? prof_->start(FREHG_KT_GW_UPDATE) : nullptr;
-        hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
-        const unsigned int nbs = shape_[sizeof(T) == 4 ? 1 : 0].nb;
-        if (three_d_)
-          hipExtLaunchKernelGGL((k_cg_update<T, true>), grid_u_, blk, 0, s_, ev0, ev1, 0, g_, kc_u_, A.D,
-                                (const T*)pnew, (const T*)q, x, r, z, part_u_, ctl, (const double*)part_, nbs, it);
-        else
-          hipExtLaunchKernelGGL((k_cg_update<T, false>), grid_u_, blk, 0, s_, ev0, ev1, 0, g_, kc_u_, A.D,
-                                (const T*)pnew, (const T*)q, x, r, z, part_u_, ctl, (const double*)part_, nbs, it);
-        FREHG_LAUNCH_CHECK();
-      }
-      parity ^= 1;
-    }
-    launched += batch;
-    hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(NT), 0, s_, ctl, (const double*)part_u_, nb_u_, launched);
-    FREHG_LAUNCH_CHECK();
     read_ctl(ctl, host_ctl_);
     if (host_ctl_->done) break;
     if (launched >= max_it + 2) break;
     batch = std::max(2, std::min(16, batch / 2));
+    cg_iterations<T>(A, x, r, z, p0, p1, q, ctl, launched, batch, kt_spmv);
+    launched += batch;
   }
   if (host_ctl_->breakdown)
     throw ConvergenceError(std::string("Krylov ") + (three_d_ ? "3D" : "2D") + (sizeof(T) == 4 ? " fp32" : " fp64") +
@@ -1033,6 +1030,48 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
   *rhs_norm = std::sqrt(host_ctl_->bb);
   *converged = host_ctl_->converged != 0;
   return iters;
+}
+
+template <typename T>
+int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed,
+                    bool x0_zero, bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl,
+                    T* r, T* z, T* p0, T* p1, T* q, bool* rejected, double* res_norm,
+                    double* rhs_norm, bool* converged, int kt_spmv) {
+  cg_start<T>(A, b, bn, x, fixed, x0_zero, guard, rtol, atol, max_it, ctl, r, z, p0);
+  *rejected = false;
+  if (guard && !x0_zero) {
+    read_ctl(ctl, host_ctl_);
+    if (host_ctl_->guess_rejected) {
+      *rejected = true;
+      cg_start<T>(A, b, bn, x, fixed, true, guard, rtol, atol, max_it, ctl, r, z, p0);
+    }
+  }
+  const int est = (sizeof(T) == 8) ? last_iters_ : last_inner_iters_;
+  const int batch = std::max(2, est + 1);
+  cg_iterations<T>(A, x, r, z, p0, p1, q, ctl, 0, batch, kt_spmv);
+  return cg_finish<T>(A, x, r, z, p0, p1, q, ctl, max_it, batch, batch, res_norm, rhs_norm, converged, kt_spmv);
+}
+
+void Krylov::enqueue_cg(const StencilView<double>& A, const double* b, const double* bn, double* x,
+                        const unsigned char* fixed, bool x0_zero, const KrylovConfig& cfg, int budget) {
+  if (cfg.ksp != FREHG_KSP_CG || cfg.precision != 0) throw ArgError("Krylov::enqueue_cg: FP64 CG only");
+  if (three_d_ && !tuned_[0]) throw ArgError("Krylov::enqueue_cg: 3D SpMV shape not tuned yet");
+  cg_start<double>(A, b, bn, x, fixed, x0_zero, false, cfg.rtol, cfg.atol, cfg.max_it, ctl_, r_, z_, p0_);
+  cg_iterations<double>(A, x, r_, z_, p0_, p1_, q_, ctl_, 0, budget, three_d_ ? FREHG_KT_GW_SPMV : FREHG_KT_SW_SPMV);
+}
+
+SolveResult Krylov::finish_cg(const StencilView<double>& A, double* x, const KrylovConfig& cfg, int budget) {
+  SolveResult res;
+  bool conv = false;
+  res.iterations = cg_finish<double>(A, x, r_, z_, p0_, p1_, q_, ctl_, cfg.max_it, budget, budget,
+                                     &res.residual_norm, &res.rhs_norm, &conv,
+                                     three_d_ ? FREHG_KT_GW_SPMV : FREHG_KT_SW_SPMV);
+  res.converged = conv;
+  if (!conv)
+    throw ConvergenceError(std::string("Krylov ") + (three_d_ ? "3D" : "2D") + ": not converged within max_it (" +
+                           std::to_string(res.iterations) + " iterations, ||D^-1 r|| = " +
+                           std::to_string(res.residual_norm) + ", ||D^-1 b|| = " + std::to_string(res.rhs_norm) + ")");
+  return res;
 }
 
 template <typename T>

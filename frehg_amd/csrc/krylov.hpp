@@ -93,6 +93,14 @@ class Krylov {
                     const double* bn, double* x, const unsigned char* fixed, bool x0_zero,
                     bool guard_guess, const KrylovConfig& cfg);
 
+  // FP64 CG split for graph capture: enqueue_cg queues the start and `budget` iterations without
+  // any host synchronisation; finish_cg (same budget) waits, continues past the budget if needed
+  // and returns the result as solve() would (x0 = 0 or x, no guess guard).
+  void enqueue_cg(const StencilView<double>& A, const double* b, const double* bn, double* x,
+                  const unsigned char* fixed, bool x0_zero, const KrylovConfig& cfg, int budget);
+  SolveResult finish_cg(const StencilView<double>& A, double* x, const KrylovConfig& cfg, int budget);
+  int last_iterations() const { return last_iters_; }
+
   // CG SpMV launch shape per precision (0 FP64, 1 FP32): cells per lane (1: one column per
   // thread on 64 x 4 tiles; 2: two adjacent columns per lane on 128 x 4 tiles, paired aligned
   // loads, x-neighbours by wave shifts) and layers per block of the z-march (kc).
@@ -115,6 +123,15 @@ class Krylov {
               bool x0_zero, bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl,
               T* r, T* z, T* p0, T* p1, T* q, bool* rejected, double* res_norm, double* rhs_norm,
               bool* converged, int kt_spmv);
+  template <typename T>
+  void cg_start(const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed, bool zero,
+                bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl, T* r, T* z, T* p0);
+  template <typename T>
+  void cg_iterations(const StencilView<T>& A, T* x, T* r, T* z, T* p0, T* p1, T* q, KrylovCtl* ctl, int from,
+                     int count, int kt_spmv);
+  template <typename T>
+  int cg_finish(const StencilView<T>& A, T* x, T* r, T* z, T* p0, T* p1, T* q, KrylovCtl* ctl, int max_it,
+                int launched, int batch, double* res_norm, double* rhs_norm, bool* converged, int kt_spmv);
   template <typename T>
   void launch_spmv(const StencilView<T>& A, const T* z, const T* pold, T* pnew, T* q, KrylovCtl* ctl, int it,
                    hipEvent_t ev0, hipEvent_t ev1);

@@ -18,6 +18,7 @@
 //   interp     interpVelocity (:780-792), clears cfl_active (:755)
 //   geom       updateGeometry (:306-369)
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -408,6 +409,12 @@ const char* kNames[] = {"eta",  "etan", "bottom", "bottomXP", "bottomYP", "dept"
                         "Ex",   "Ey",   "cflx",   "cfly",     "Fu",       "Fv",   "Sxp",   "Sxm",   "Syp",
                         "Sym",  "Sct",  "Srhs",   "cfl_active", "wtfx",   "wtfy"};
 
+__global__ __launch_bounds__(NT) void k_copy_field(const double* __restrict__ src, double* __restrict__ dst,
+                                                   long long n) {
+  const long long c = (long long)blockIdx.x * NT + threadIdx.x;
+  if (c < n) dst[c] = src[c];
+}
+
 }  // namespace
 
 SweState::SweState(int nx, int ny, double dx, double dy, const frehg_swe_params& p, hipStream_t s, Profiler* prof)
@@ -435,11 +442,14 @@ SweState::SweState(int nx, int ny, double dx, double dy, const frehg_swe_params&
   FREHG_HIP(hipMalloc(&dcount_, 16));
   FREHG_HIP(hipMemsetAsync(dcount_, 0, 16, s_));
   kry_.reset(new Krylov(g_, false, s_, prof_));
+  if (const char* env = std::getenv("FREHG_SW_GRAPH")) use_graph_ = std::atoi(env) != 0;
   set_params(p);
   FREHG_HIP(hipStreamSynchronize(s_));
 }
 
 SweState::~SweState() {
+  for (hipGraphExec_t ge : {graph_pre_, graph_post_})
+    if (ge) (void)hipGraphExecDestroy(ge);
   for (auto& kv : f_) (void)hipFree(kv.second);
   for (void* q : {(void*)D_, (void*)ax_, (void*)ay_, (void*)b_, (void*)fixed_, (void*)red_, (void*)dres_,
                   (void*)dcount_})
@@ -589,12 +599,13 @@ void SweState::initialize_state(double init_eta) {
   FREHG_HIP(hipStreamSynchronize(s_));
 }
 
-void SweState::step(double rain, double evap, frehg_solve_stats* st) {
+// Kernels of the step before the surface solve: savePrevious (eta), surface BCs, momentum
+// coefficients, assembly and the symmetric form of the system.
+void SweState::enq_pre(double evap) {
   const SwK P = make_swk(g_, dx_, dy_, p_);
   const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4)), blk(NT);
-  const size_t bytes = (size_t)n_ * sizeof(double);
   auto F = [&](const char* nm) { return f_[nm]; };
-  FREHG_HIP(hipMemcpyAsync(F("etan"), F("eta"), bytes, hipMemcpyDeviceToDevice, s_));
+  copy_field(F("eta"), F("etan"));
   hipLaunchKernelGGL(k_swe_surfbc, grid, blk, 0, s_, P, F("eta"), F("bottom"));
   FREHG_LAUNCH_CHECK();
   MomIn m{F("uu"), F("vv"), F("vx"), F("uy"), F("cflx"), F("cfly"), F("Vsx"), F("Vsy"),
@@ -610,17 +621,14 @@ void SweState::step(double rain, double evap, frehg_solve_stats* st) {
   FREHG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_swe_symm, grid, blk, 0, s_, P, o, fixed_, F("eta"), ax_, ay_, b_);
   FREHG_LAUNCH_CHECK();
-  KrylovConfig kc = krylov_config(p_.solver, "frehg_swe_step");
-  kc.precision = 0;  // the surface system is small; it stays FP64
-  StencilView<double> A{F("Sct"), ax_, ay_, nullptr};
-  const SolveResult sr = kry_->solve(A, nullptr, b_, F("Srhs"), F("eta"), fixed_, true, false, kc);
-  if (st) {
-    std::memset(st, 0, sizeof(*st));
-    st->iterations = sr.iterations;
-    st->residual_norm = sr.residual_norm;
-    st->rhs_norm = sr.rhs_norm;
-    st->converged = sr.converged ? 1 : 0;
-  }
+}
+
+// Kernels of the step after the surface solve: depth, drag and wet/dry faces, velocities,
+// their interpolation and the surface geometry for the next step.
+void SweState::enq_post(double rain, double evap) {
+  const SwK P = make_swk(g_, dx_, dy_, p_);
+  const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4)), blk(NT);
+  auto F = [&](const char* nm) { return f_[nm]; };
   hipLaunchKernelGGL(k_swe_post, grid, blk, 0, s_, P, F("eta"), F("bottom"), F("dept"), F("cfl_active"), rain, evap);
   FREHG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_swe_depth, grid, blk, 0, s_, P, F("eta"), F("bottom"), F("dept"), F("deptx"), F("depty"));
@@ -628,8 +636,8 @@ void SweState::step(double rain, double evap, frehg_solve_stats* st) {
   hipLaunchKernelGGL(k_swe_dragwtf, grid, blk, 0, s_, P, F("Vs"), F("eta"), F("bottomXP"), F("bottomYP"), F("CDx"),
                      F("CDy"), F("wtfx"), F("wtfy"));
   FREHG_LAUNCH_CHECK();
-  FREHG_HIP(hipMemcpyAsync(F("un"), F("uu"), bytes, hipMemcpyDeviceToDevice, s_));
-  FREHG_HIP(hipMemcpyAsync(F("vn"), F("vv"), bytes, hipMemcpyDeviceToDevice, s_));
+  copy_field(F("uu"), F("un"));
+  copy_field(F("vv"), F("vn"));
   VelIn vi{F("Ex"), F("Ey"), F("Dx"), F("Dy"), F("Vsx"), F("Vsy"), F("Asx"), F("Asy"), F("eta"), F("dept"),
            F("cfl_active")};
   VelOut vo{F("uu"), F("vv"), F("Fu"), F("Fv"), F("cflx"), F("cfly")};
@@ -637,10 +645,89 @@ void SweState::step(double rain, double evap, frehg_solve_stats* st) {
   FREHG_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_swe_interp, grid, blk, 0, s_, P, F("uu"), F("vv"), F("uy"), F("vx"), F("cfl_active"));
   FREHG_LAUNCH_CHECK();
-  FREHG_HIP(hipMemcpyAsync(F("Vsn"), F("Vs"), bytes, hipMemcpyDeviceToDevice, s_));
+  copy_field(F("Vs"), F("Vsn"));
   GeoOut go{F("Vs"), F("Asx"), F("Asy"), F("Asz"), F("Aszx"), F("Aszy"), F("Vsx"), F("Vsy")};
   hipLaunchKernelGGL(k_swe_geom, grid, blk, 0, s_, P, F("dept"), F("deptx"), F("depty"), go);
   FREHG_LAUNCH_CHECK();
+}
+
+void SweState::copy_field(const double* src, double* dst) {
+  hipLaunchKernelGGL(k_copy_field, dim3(ceil_div(n_, NT)), dim3(NT), 0, s_, src, dst, n_);
+  FREHG_LAUNCH_CHECK();
+}
+
+// With FREHG_SW_GRAPH=1 the step's kernels are replayed from two captured graphs around the
+// surface solve (its start and a budget of CG iterations are in the first): a graph launch
+// replaces a host launch per kernel of this launch-bound 2D step. The graphs are captured again
+// when the parameters, the forcing or the budget change; iterations past the budget run eagerly
+// (Krylov::finish_cg). Off by default: on the 512 x 512 coupled workload the replayed kernels
+// run back to back but the graph is dispatched in segments separated by 10-40 us, and the step
+// is no faster than with eager launches.
+void SweState::step(double rain, double evap, frehg_solve_stats* st) {
+  KrylovConfig kc = krylov_config(p_.solver, "frehg_swe_step");
+  kc.precision = 0;  // the surface system is small; it stays FP64
+  StencilView<double> A{f_["Sct"], ax_, ay_, nullptr};
+  SolveResult sr;
+  if (use_graph_ && kc.ksp == FREHG_KSP_CG && solved_) {
+    // budget: one iteration more than the last solve took; grown at once, shrunk only after it
+    // has been too large for 16 steps (each change costs a capture)
+    const int want = std::max(2, kry_->last_iterations() + 1);
+    if (budget_ == 0 || want > budget_) {
+      budget_ = want;
+      shrink_ = 0;
+    } else if (want < budget_ && ++shrink_ >= 16) {
+      budget_ = want;
+      shrink_ = 0;
+    } else if (want == budget_) {
+      shrink_ = 0;
+    }
+    const bool same = graph_pre_ && graph_rain_ == rain && graph_evap_ == evap && graph_budget_ == budget_ &&
+                      std::memcmp(&graph_params_, &p_, sizeof(p_)) == 0;
+    if (!same) {
+      for (hipGraphExec_t* ge : {&graph_pre_, &graph_post_})
+        if (*ge) {
+          FREHG_HIP(hipGraphExecDestroy(*ge));
+          *ge = nullptr;
+        }
+      auto capture = [&](hipGraphExec_t* out, auto&& enqueue) {
+        hipGraph_t g;
+        FREHG_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        try {
+          enqueue();
+        } catch (...) {
+          (void)hipStreamEndCapture(s_, &g);
+          throw;
+        }
+        FREHG_HIP(hipStreamEndCapture(s_, &g));
+        FREHG_HIP(hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+        FREHG_HIP(hipGraphDestroy(g));
+      };
+      capture(&graph_pre_, [&] {
+        enq_pre(evap);
+        kry_->enqueue_cg(A, b_, f_["Srhs"], f_["eta"], fixed_, true, kc, budget_);
+      });
+      capture(&graph_post_, [&] { enq_post(rain, evap); });
+      graph_rain_ = rain;
+      graph_evap_ = evap;
+      graph_budget_ = budget_;
+      graph_params_ = p_;
+    }
+    FREHG_HIP(hipGraphLaunch(graph_pre_, s_));
+    sr = kry_->finish_cg(A, f_["eta"], kc, budget_);
+    FREHG_HIP(hipGraphLaunch(graph_post_, s_));
+  } else {
+    enq_pre(evap);
+    sr = kry_->solve(A, nullptr, b_, f_["Srhs"], f_["eta"], fixed_, true, false, kc);
+    enq_post(rain, evap);
+    solved_ = true;
+  }
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->iterations = sr.iterations;
+    st->residual_norm = sr.residual_norm;
+    st->rhs_norm = sr.rhs_norm;
+    st->converged = sr.converged ? 1 : 0;
+  }
 }
 
 double SweState::max_cfl() {

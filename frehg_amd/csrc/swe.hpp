@@ -36,6 +36,9 @@ class SweState {
   // whole device-layout arrays (nsurf() doubles, indexed by sidx)
   void set_raw(const std::string& name, const double* src);
   void get_raw(const std::string& name, double* dst);
+  void enq_pre(double evap);
+  void enq_post(double rain, double evap);
+  void copy_field(const double* src, double* dst);
 
   Geo g_;  // nz = 1; 2D index = sidx
   double dx_, dy_;
@@ -51,6 +54,14 @@ class SweState {
   double* hres_ = nullptr;
   unsigned int* dcount_ = nullptr;
   std::unique_ptr<Krylov> kry_;
+  // captured step graphs (before / after the surface solve) and what they were captured for
+  bool use_graph_ = false;  // FREHG_SW_GRAPH=1: replay the step from captured graphs
+  bool solved_ = false;  // the first step runs eagerly and sets the iteration estimate
+  int budget_ = 0, shrink_ = 0;
+  hipGraphExec_t graph_pre_ = nullptr, graph_post_ = nullptr;
+  double graph_rain_ = 0.0, graph_evap_ = 0.0;
+  int graph_budget_ = 0;
+  frehg_swe_params graph_params_{};
 };
 
 }  // namespace frehg
