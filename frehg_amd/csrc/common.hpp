@@ -202,8 +202,9 @@ inline SoilClass make_soil_class(double alpha, double n, double theta_s, double 
 
 // Per-cell soil lookup on the halo-padded grid. cls / ks are padded arrays whose ghost entries
 // hold the clamped interior value (ReSolver::soilAt clamps ghost indices, ReSolver.cpp:87-96).
-// c0 repeats class 0 by value, so that with cls == nullptr (one class) the class parameters are
-// kernel arguments (scalar registers) instead of per-lane loads. The template arguments fix, per
+// c0 repeats class 0 by value, so that with cls == nullptr (one class) the fixed variants read the
+// class parameters as kernel arguments (scalar registers) instead of per-lane loads. The template
+// arguments fix, per
 // kernel instantiation, whether cls / ks are absent (kAbsent), present (kPresent) or checked at
 // each access (kRuntime): fixed variants have no per-access branches, so the loads of a cell's
 // whole stencil can issue together.
@@ -215,10 +216,12 @@ struct SoilViewT {
   const int32_t* cls;  // nullptr -> every cell is class 0
   const double* ks;    // nullptr -> Ks from the class; else isotropic per-cell Ks
   SoilClass c0;        // classes[0]
-  __device__ inline SoilClass at(long long c) const {
+  // (a reference either way: a struct chosen at run time between two places would be copied to
+  // per-lane stack memory)
+  __device__ inline const SoilClass& at(long long c) const {
     if constexpr (CLS == kAbsent) return c0;
     else if constexpr (CLS == kPresent) return classes[cls[c]];
-    else return cls ? classes[cls[c]] : c0;
+    else return classes[cls ? cls[c] : 0];
   }
   __device__ inline double Ksx(long long c) const {
     if constexpr (KS == kPresent) return ks[c];
