@@ -154,7 +154,7 @@ __global__ __launch_bounds__(NT) void kb_flux_correct(const EnsBox* boxes, const
   if (q.t.valid) {
     const long long o = q.off;
     re_flux_correct(q.P, q.t, F.h + o, F.kr + o, F.hn + o, F.wcn + o, F.wch + o, F.Kx + o, F.Ky + o, F.Kz + o,
-                    F.qx + o, F.qy + o, F.qz + o, F.wc + o, F.hwc + o, sat0 + o, cand + o, a, b, d);
+                    F.qx + o, F.qy + o, F.qz + o, F.wc + o, F.hwc + o, sat0 + o, cand + o, nullptr, a, b, d);
   }
   a = block_sum<NT>(a, lds);
   b = block_sum<NT>(b, lds);
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(NT) void kb_realloc(const EnsBox* boxes, const unsi
     next[o + q.t.c] = prev[o + q.t.c];
     return;
   }
-  re_realloc(q.P, q.t, F.Kx + o, F.Ky + o, sat0 + o, cand + o, prev + o, next + o, flags + s);
+  re_realloc(q.P, q.t, FacesK{F.Kx + o, F.Ky + o}, sat0 + o, cand + o, prev + o, next + o, flags + s);
 }
 
 __global__ __launch_bounds__(NT) void kb_finalize(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F,
@@ -194,8 +194,8 @@ __global__ __launch_bounds__(NT) void kb_finalize(const EnsBox* boxes, const uns
   double a = 0.0, b = 0.0, dh = 0.0;
   if (q.t.valid) {
     const long long o = q.off;
-    re_finalize(q.P, q.t, F.Kx + o, F.Ky + o, sat0 + o, fin + o, F.wch + o, F.hwc + o, F.hn + o, F.wc + o, F.h + o,
-                nullptr, a, b, dh);
+    re_finalize(q.P, q.t, FacesK{F.Kx + o, F.Ky + o}, sat0 + o, fin + o, F.wch + o, F.hwc + o, F.hn + o,
+                F.wc + o, F.h + o, nullptr, nullptr, a, b, dh);
   }
   a = block_sum<NT>(a, lds);
   b = block_sum<NT>(b, lds);

@@ -11,6 +11,8 @@
 //   post       ghost heads, kr and C at h^{n+1}                                     (:638-669)
 //   flux+corr  faces at h^{n+1}, Darcy fluxes qx qy qz, mass-conservative theta,
 //              wch/hwc, saturation flags                                 (:671-743, :762-786)
+//              (by default Kx..qz are not stored: face bits for the reallocation instead, and
+//              the fields are rebuilt from the post-solve state when read, see flux_pending_)
 //   realloc    monotone fixed point of the sequential saturation sweep              (:788-837)
 //   finalize   reallocation, clamp, moisture ghosts                                 (:788-878)
 //   dtadapt    flux-imbalance max and Courant min (adaptive dt only)                (:880-929)
@@ -90,7 +92,8 @@ __global__ __launch_bounds__(NT) void k_re_flux_correct(K P, const double* __res
                                                          double* __restrict__ qx, double* __restrict__ qy,
                                                          double* __restrict__ qz, double* __restrict__ wc,
                                                          double* __restrict__ hwc, unsigned char* __restrict__ sat0,
-                                                         unsigned char* __restrict__ cand, double* __restrict__ part,
+                                                         unsigned char* __restrict__ cand,
+                                                         unsigned char* __restrict__ kf, double* __restrict__ part,
                                                          unsigned int nblocks, int* __restrict__ flags, int nflags) {
   __shared__ double lds[NT / kWave];
   TileId b;
@@ -98,7 +101,8 @@ __global__ __launch_bounds__(NT) void k_re_flux_correct(K P, const double* __res
   if (b.linear == 0 && (int)threadIdx.x < nflags) flags[threadIdx.x] = 0;  // reallocation sweep flags
   double l_before = 0.0, l_ss = 0.0, l_bnd = 0.0;
   if (t.valid)
-    re_flux_correct(P, t, h, kr, hn, wcn, wch, Kx, Ky, Kz, qx, qy, qz, wc, hwc, sat0, cand, l_before, l_ss, l_bnd);
+    re_flux_correct(P, t, h, kr, hn, wcn, wch, Kx, Ky, Kz, qx, qy, qz, wc, hwc, sat0, cand, kf, l_before, l_ss,
+                    l_bnd);
   const double a = block_sum<NT>(l_before, lds);
   const double bs = block_sum<NT>(l_ss, lds);
   const double d = block_sum<NT>(l_bnd, lds);
@@ -109,25 +113,38 @@ __global__ __launch_bounds__(NT) void k_re_flux_correct(K P, const double* __res
   }
 }
 
+// Kx..qz at the post-solve head (the stores the flux + corrector kernel leaves out by default).
+template <class K, int ZF>
+__global__ __launch_bounds__(NT) void k_re_flux(K P, const double* __restrict__ h, const double* __restrict__ kr,
+                                                 const double* __restrict__ wcn, double* __restrict__ Kx,
+                                                 double* __restrict__ Ky, double* __restrict__ Kz,
+                                                 double* __restrict__ qx, double* __restrict__ qy,
+                                                 double* __restrict__ qz) {
+  TileId b;
+  const Cell t = tile_cell<ZF>(P.g, b);
+  if (t.valid) (void)re_flux(P, t, h, kr, wcn, Kx, Ky, Kz, qx, qy, qz);
+}
+
 // One Jacobi sweep of the monotone map whose unique fixed point is the sequential sweep's
 // saturation state (see re_realloc).
-template <class K>
-__global__ __launch_bounds__(NT) void k_re_realloc_iter(K P, const double* __restrict__ Kx, const double* __restrict__ Ky,
-                                                         const unsigned char* __restrict__ sat0, const unsigned char* __restrict__ cand,
-                                                         const unsigned char* __restrict__ prev, unsigned char* __restrict__ next, int* changed) {
+template <class K, class FA>
+__global__ __launch_bounds__(NT) void k_re_realloc_iter(K P, FA fa, const unsigned char* __restrict__ sat0,
+                                                         const unsigned char* __restrict__ cand,
+                                                         const unsigned char* __restrict__ prev,
+                                                         unsigned char* __restrict__ next, int* changed) {
   const Cell t = cell_of(P.g);
-  if (t.valid) re_realloc(P, t, Kx, Ky, sat0, cand, prev, next, changed);
+  if (t.valid) re_realloc(P, t, fa, sat0, cand, prev, next, changed);
 }
 
 // Reallocation + clamp + moisture BCs; ledger partials (sum theta^{n+1} V, sum (theta^{n+1} -
 // theta_c) V) to part[3..4][nblocks] and max |h^{n+1} - h^n| to mx[nblocks].
-template <class K, int ZF>
-__global__ __launch_bounds__(NT) void k_re_finalize(K P, const double* __restrict__ Kx, const double* __restrict__ Ky,
-                                                     const unsigned char* __restrict__ sat0,
+template <class K, int ZF, class FA>
+__global__ __launch_bounds__(NT) void k_re_finalize(K P, FA fa, const unsigned char* __restrict__ sat0,
                                                      const unsigned char* __restrict__ fin,
                                                      const double* __restrict__ wch, const double* __restrict__ hwc,
                                                      const double* __restrict__ hn, double* __restrict__ wc,
-                                                     double* __restrict__ h, unsigned char* __restrict__ hmod,
+                                                     double* __restrict__ h, double* __restrict__ hout,
+                                                     unsigned char* __restrict__ hmod,
                                                      double* __restrict__ part, double* __restrict__ mx,
                                                      unsigned int nblocks) {
   __shared__ double lds[NT / kWave];
@@ -135,7 +152,7 @@ __global__ __launch_bounds__(NT) void k_re_finalize(K P, const double* __restric
   TileId b;
   const Cell t = tile_cell<ZF>(P.g, b);
   double l_after = 0.0, l_adj = 0.0, dh = 0.0;
-  if (t.valid) re_finalize(P, t, Kx, Ky, sat0, fin, wch, hwc, hn, wc, h, hmod, l_after, l_adj, dh);
+  if (t.valid) re_finalize(P, t, fa, sat0, fin, wch, hwc, hn, wc, h, hout, hmod, l_after, l_adj, dh);
   const double a = block_sum<NT>(l_after, lds);
   const double bs = block_sum<NT>(l_adj, lds);
   dh = wave_max(dh);
@@ -210,12 +227,14 @@ ReState::ReState(int nx, int ny, int nz, const frehg_re_params& p, hipStream_t s
   if (nx < 1 || ny < 1 || nz < 1) throw ArgError("frehg_re_init: grid sizes must be >= 1");
   n_ = g_.ncell();
   const size_t bytes = (size_t)n_ * sizeof(double);
-  for (double** f : {&h_, &hn_, &wc_, &wcn_, &wch_, &hwc_, &ch_, &Kx_, &Ky_, &Kz_, &qx_, &qy_, &qz_, &kr_,
+  const char* lz = std::getenv("FREHG_LAZY_FLUX");
+  lazy_flux_ = !(lz && lz[0] == '0');
+  for (double** f : {&h_, &hn_, &hp_, &wc_, &wcn_, &wch_, &hwc_, &ch_, &Kx_, &Ky_, &Kz_, &qx_, &qy_, &qz_, &kr_,
                      &D_, &ax_, &ay_, &az_, &rhs_}) {
     FREHG_HIP(hipMalloc(f, bytes));
     FREHG_HIP(hipMemsetAsync(*f, 0, bytes, s_));
   }
-  for (unsigned char** f : {&sat0_, &cand_, &satA_, &satB_, &hmod_}) {
+  for (unsigned char** f : {&sat0_, &cand_, &satA_, &satB_, &hmod_, &kf_}) {
     FREHG_HIP(hipMalloc(f, (size_t)n_));
     FREHG_HIP(hipMemsetAsync(*f, 0, (size_t)n_, s_));
   }
@@ -238,11 +257,11 @@ ReState::ReState(int nx, int ny, int nz, const frehg_re_params& p, hipStream_t s
 }
 
 ReState::~ReState() {
-  for (void* f : {(void*)h_, (void*)hn_, (void*)wc_, (void*)wcn_, (void*)wch_, (void*)hwc_, (void*)ch_,
+  for (void* f : {(void*)h_, (void*)hn_, (void*)hp_, (void*)wc_, (void*)wcn_, (void*)wch_, (void*)hwc_, (void*)ch_,
                   (void*)Kx_, (void*)Ky_, (void*)Kz_, (void*)qx_, (void*)qy_, (void*)qz_, (void*)kr_,
                   (void*)D_, (void*)ax_, (void*)ay_, (void*)az_, (void*)rhs_, (void*)D32_, (void*)ax32_,
                   (void*)ay32_, (void*)az32_, (void*)sat0_, (void*)cand_, (void*)satA_, (void*)satB_, (void*)hmod_,
-                  (void*)dflag_, (void*)red_, (void*)red2_, (void*)led_, (void*)dres_, (void*)dcount_, (void*)classes_, (void*)cls_,
+                  (void*)kf_, (void*)dflag_, (void*)red_, (void*)red2_, (void*)led_, (void*)dres_, (void*)dcount_, (void*)classes_, (void*)cls_,
                   (void*)ks_, (void*)qtop_})
     if (f) (void)hipFree(f);
   if (hflag_) (void)hipHostFree(hflag_);
@@ -268,6 +287,7 @@ void ReState::set_params(const frehg_re_params& p) {
       FREHG_HIP(hipMemsetAsync(*f, 0, b32, s_));
     }
   }
+  materialize_fluxes();
   const bool first = (dtn_ == 0.0);
   p_ = p;
   kc_fresh_ = false;
@@ -277,6 +297,7 @@ void ReState::set_params(const frehg_re_params& p) {
 void ReState::set_soil(const frehg_soil_params* classes, int ncls, const int32_t* class_idx, int idx3d,
                        const double* ks_cell) {
   if (ncls < 1) throw ArgError("frehg_re_set_soil: need at least one soil class");
+  materialize_fluxes();
   kc_fresh_ = false;
   host_classes_.clear();
   for (int c = 0; c < ncls; ++c) {
@@ -327,6 +348,7 @@ void ReState::set_soil(const frehg_soil_params* classes, int ncls, const int32_t
 }
 
 void ReState::set_top_flux(const double* q) {
+  materialize_fluxes();
   std::vector<double> pad((size_t)g_.nsurf(), 0.0);
   for (int j = 0; j < g_.ny; ++j)
     for (int i = 0; i < g_.nx; ++i) pad[(size_t)g_.sidx(i, j)] = q[(size_t)i + (size_t)j * g_.nx];
@@ -354,13 +376,17 @@ double* ReState::field_ptr(const std::string& n) {
 }
 
 void ReState::set_field(const std::string& name, const double* src) {
+  double* dst = field_ptr(name);
+  materialize_fluxes();
   kc_fresh_ = false;
-  copy_field_in(g_, g_.nz + 2, field_ptr(name), src, s_);
+  copy_field_in(g_, g_.nz + 2, dst, src, s_);
   FREHG_HIP(hipStreamSynchronize(s_));
 }
 
 void ReState::get_field(const std::string& name, double* dst) {
-  copy_field_out(g_, g_.nz + 2, dst, field_ptr(name), s_);
+  const double* src = field_ptr(name);
+  if (src == Kx_ || src == Ky_ || src == Kz_ || src == qx_ || src == qy_ || src == qz_) materialize_fluxes();
+  copy_field_out(g_, g_.nz + 2, dst, src, s_);
   FREHG_HIP(hipStreamSynchronize(s_));
 }
 
@@ -369,6 +395,7 @@ void ReState::set_initial(const double* h_int, const double* wc_int) {
   if ((h_int == nullptr) == (wc_int == nullptr))
     throw ArgError("frehg_re_set_initial: pass exactly one of h / wc");
   const double* src = h_int ? h_int : wc_int;
+  materialize_fluxes();
   kc_fresh_ = false;
   std::vector<double> pad((size_t)n_, 0.0);
   FREHG_HIP(hipMemcpy(pad.data(), h_int ? h_ : wc_, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost));
@@ -387,7 +414,8 @@ void ReState::set_initial(const double* h_int, const double* wc_int) {
 
 void ReState::learned_increment(double* dst) {
   if (!mlp_ || !mlp_->ready()) throw ArgError("frehg_mlp_increment: no trained model");
-  kc_fresh_ = false;  // kr and C are re-evaluated at h below
+  materialize_fluxes();  // before kr changes
+  kc_fresh_ = false;    // kr and C are re-evaluated at h below
   const ReK P = make_rek(g_, soil_view(), p_, qtop_);
   const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4), g_.nz);
   hipLaunchKernelGGL((k_re_pre<ReK, 1>), grid, dim3(NT), 0, s_, P, (const double*)h_, kr_, ch_);
@@ -414,6 +442,19 @@ void ReState::learned_increment(double* dst) {
   FREHG_HIP(hipFree(tmp));
 }
 
+void ReState::materialize_fluxes() {
+  if (!flux_pending_) return;
+  const ReK P = make_rek(g_, soil_view(), p_, qtop_);
+  const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4), g_.nz);
+  with_soil_variant(P, [&](const auto& PK) {
+    using K = std::decay_t<decltype(PK)>;
+    hipLaunchKernelGGL((k_re_flux<K, 1>), grid, dim3(NT), 0, s_, PK, (const double*)hp_, (const double*)kr_,
+                       (const double*)wcn_, Kx_, Ky_, Kz_, qx_, qy_, qz_);
+    FREHG_LAUNCH_CHECK();
+  });
+  flux_pending_ = false;
+}
+
 void ReState::step(frehg_re_stats* st) {
   // Reject a request that cannot be served before any state changes.
   if (p_.initial_guess == FREHG_GUESS_LEARNED && (!mlp_ || !mlp_->ready()))
@@ -433,6 +474,7 @@ void ReState::step(frehg_re_stats* st) {
   std::swap(wc_, wcn_);
   const bool kc_fresh = kc_fresh_;
   kc_fresh_ = false;  // set again once this step has completed
+  flux_pending_ = false;  // this step's fluxes supersede the previous step's
   {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_ASSEMBLE);
     hipEvent_t ev1 = prof_->stop();
@@ -484,7 +526,8 @@ void ReState::step(frehg_re_stats* st) {
       FREHG_LAUNCH_CHECK();
       hipExtLaunchKernelGGL((k_re_flux_correct<K, 1>), grid, blk, 0, s_, nullptr, ev1, 0, PK, (const double*)h_,
                             (const double*)kr_, (const double*)hn_, (const double*)wcn_, (const double*)wch_, Kx_, Ky_,
-                            Kz_, qx_, qy_, qz_, wc_, hwc_, sat0_, cand_, led_, nbl, dflag_, kSweepFlags);
+                            Kz_, qx_, qy_, qz_, wc_, hwc_, sat0_, cand_, lazy_flux_ ? kf_ : nullptr, led_, nbl,
+                            dflag_, kSweepFlags);
       FREHG_LAUNCH_CHECK();
     });
   }
@@ -498,8 +541,12 @@ void ReState::step(frehg_re_stats* st) {
     if (it >= kSweepFlags) FREHG_HIP(hipMemsetAsync(dflag_ + f, 0, sizeof(int), s_));
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
-      hipLaunchKernelGGL(k_re_realloc_iter<K>, grid, blk, 0, s_, PK, (const double*)Kx_, (const double*)Ky_,
-                         (const unsigned char*)sat0_, (const unsigned char*)cand_, prev, next, dflag_ + f);
+      if (lazy_flux_)
+        hipLaunchKernelGGL((k_re_realloc_iter<K, FacesF>), grid, blk, 0, s_, PK, FacesF{kf_},
+                           (const unsigned char*)sat0_, (const unsigned char*)cand_, prev, next, dflag_ + f);
+      else
+        hipLaunchKernelGGL((k_re_realloc_iter<K, FacesK>), grid, blk, 0, s_, PK, FacesK{Kx_, Ky_},
+                           (const unsigned char*)sat0_, (const unsigned char*)cand_, prev, next, dflag_ + f);
       FREHG_LAUNCH_CHECK();
     });
     FREHG_HIP(hipMemcpyAsync(hflag_, dflag_ + f, sizeof(int), hipMemcpyDeviceToHost, s_));
@@ -507,13 +554,24 @@ void ReState::step(frehg_re_stats* st) {
     prev = next;
     if (hflag_[0] == 0) break;
   }
+  // Lazy fluxes: the final head goes to hp_ and h_ keeps the post-solve head; the swap below
+  // leaves the final head in h_ and the post-solve head (a flux rebuild input) in hp_.
   with_soil_variant(P, [&](const auto& PK) {
     using K = std::decay_t<decltype(PK)>;
-    hipLaunchKernelGGL((k_re_finalize<K, 0>), grid, blk, 0, s_, PK, (const double*)Kx_, (const double*)Ky_,
-                       (const unsigned char*)sat0_, prev, (const double*)wch_, (const double*)hwc_, (const double*)hn_,
-                       wc_, h_, hmod_, led_, led_ + 5 * (size_t)nbl, nbl);
+    if (lazy_flux_)
+      hipLaunchKernelGGL((k_re_finalize<K, 0, FacesF>), grid, blk, 0, s_, PK, FacesF{kf_},
+                         (const unsigned char*)sat0_, prev, (const double*)wch_, (const double*)hwc_,
+                         (const double*)hn_, wc_, h_, hp_, hmod_, led_, led_ + 5 * (size_t)nbl, nbl);
+    else
+      hipLaunchKernelGGL((k_re_finalize<K, 0, FacesK>), grid, blk, 0, s_, PK, FacesK{Kx_, Ky_},
+                         (const unsigned char*)sat0_, prev, (const double*)wch_, (const double*)hwc_,
+                         (const double*)hn_, wc_, h_, nullptr, hmod_, led_, led_ + 5 * (size_t)nbl, nbl);
     FREHG_LAUNCH_CHECK();
   });
+  if (lazy_flux_) {
+    std::swap(h_, hp_);
+    flux_pending_ = true;
+  }
   // ledger: [before, storage_ss, boundary_in, after, adjust] sums, then max |dh|
   reduce_sums(led_, nbl, 5, red2_, dres_ + 2, s_);
   FREHG_LAUNCH_CHECK();
@@ -521,6 +579,7 @@ void ReState::step(frehg_re_stats* st) {
                      dres_ + 7);
   FREHG_LAUNCH_CHECK();
   if (p_.adaptive_dt) {
+    materialize_fluxes();
     const unsigned int nb = grid.x * grid.y * grid.z;
     hipLaunchKernelGGL(k_re_dtadapt, grid, blk, 0, s_, P, qx_, qy_, qz_, wc_, red_, nb);
     FREHG_LAUNCH_CHECK();

@@ -49,6 +49,8 @@ class ReState {
   unsigned char* d_hmod() { return kc_fresh_ ? hmod_ : nullptr; }
   SoilView soil_view() const { return SoilView{classes_, cls_, ks_, host_classes_[0]}; }
   Krylov& krylov() { return *kry_; }
+  // Stores Kx..qz of the last step if they are pending (see flux_pending_).
+  void materialize_fluxes();
   // Learned increment h^{n+1} - h^n the model predicts for a step taken from the current state
   // (features from h, K/C at h, theta), interior cells in i + j nx + k nx ny order.
   void learned_increment(double* dst);
@@ -70,6 +72,16 @@ class ReState {
   double *h_ = nullptr, *hn_ = nullptr, *wc_ = nullptr, *wcn_ = nullptr, *wch_ = nullptr,
          *hwc_ = nullptr, *ch_ = nullptr, *Kx_ = nullptr, *Ky_ = nullptr, *Kz_ = nullptr,
          *qx_ = nullptr, *qy_ = nullptr, *qz_ = nullptr;
+  // Lazy flux fields (default; FREHG_LAZY_FLUX=0 stores them in the step). The step's flux +
+  // corrector kernel evaluates the face conductivities and fluxes but stores only one byte of
+  // nonzero-face bits per cell (kf_), which is all the reallocation reads; finalize writes the
+  // final head to a second buffer, so the post-solve head survives in hp_. Until the next step,
+  // Kx..qz follow bit for bit from hp_, kr_ (post-solve) and wcn_, and are stored from them when
+  // read (get_field) or before anything changes those inputs (flux_pending_).
+  double* hp_ = nullptr;
+  unsigned char* kf_ = nullptr;
+  bool lazy_flux_ = true;
+  bool flux_pending_ = false;
   // Relative conductivity K/Ks per cell (incl. face-adjacent ghosts).
   double* kr_ = nullptr;
   // Assembled system: D = Gct, ax/ay/az = +x/+y/+z couplings, rhs = Grhs; FP32 copies.

@@ -444,6 +444,10 @@ __device__ __forceinline__ void re_correct(const RK& P, const Cell& t, const dou
 // evaluates the fluxes through its - faces with the same functions of the same operands as the
 // neighbour that stores them (identical bits), instead of re-reading them. Ghost-face fluxes
 // (i = 0 / j = 0 x- and y- faces), which no cell writes, are read. All loads precede the stores.
+// With kf != nullptr the face conductivities and fluxes are not stored (re_flux rebuilds them from
+// the post-solve head, kr and theta^n; the ghost-face entries read here are never written by
+// either); kf receives the nonzero-face bits that the reallocation needs (bit 0: Kx != 0, bit 1:
+// Ky != 0), zero at the x- and y- ghost faces.
 template <class RK>
 __device__ __forceinline__ void re_flux_correct(const RK& P, const Cell& t, const double* __restrict__ h,
                                                 const double* __restrict__ kr, const double* __restrict__ hn,
@@ -453,7 +457,8 @@ __device__ __forceinline__ void re_flux_correct(const RK& P, const Cell& t, cons
                                                 double* __restrict__ qy, double* __restrict__ qz,
                                                 double* __restrict__ wc, double* __restrict__ hwc,
                                                 unsigned char* __restrict__ sat0, unsigned char* __restrict__ cand,
-                                                double& l_before, double& l_ss, double& l_bnd) {
+                                                unsigned char* __restrict__ kf, double& l_before, double& l_ss,
+                                                double& l_bnd) {
   const Geo& g = P.g;
   const long long c = t.c;
   const CellFlux f = flux_values(P, t, h, kr, wcn);
@@ -464,36 +469,55 @@ __device__ __forceinline__ void re_flux_correct(const RK& P, const Cell& t, cons
   const double qzm = (t.k > 0) ? flux_z(P, h, kr, c - g.sz, t.k - 1) : f.qt;
   correct_fluxes(P, t, h, hn, wcn, f.qxp, qxm, f.qyp, qym, f.qzp, qzm, wch, wc, hwc, sat0, cand, l_before, l_ss,
                  l_bnd);
-  store_flux(P, t, f, Kx, Ky, Kz, qx, qy, qz);
+  if (kf) {
+    kf[c] = (unsigned char)((f.Kxp != 0.0 ? 1 : 0) | (f.Kyp != 0.0 ? 2 : 0));
+    if (t.i == 0) kf[c - 1] = 0;
+    if (t.j == 0) kf[c - g.sy] = 0;
+  } else {
+    store_flux(P, t, f, Kx, Ky, Kz, qx, qy, qz);
+  }
 }
+
+// Nonzero-face tests of the reallocation, from the stored face conductivities or from the face
+// bits of re_flux_correct.
+struct FacesK {
+  const double* __restrict__ Kx;
+  const double* __restrict__ Ky;
+  __device__ bool x(long long c) const { return Kx[c] != 0.0; }
+  __device__ bool y(long long c) const { return Ky[c] != 0.0; }
+};
+struct FacesF {
+  const unsigned char* __restrict__ f;
+  __device__ bool x(long long c) const { return (f[c] & 1) != 0; }
+  __device__ bool y(long long c) const { return (f[c] & 2) != 0; }
+};
 
 // adjacentToSat (ReSolver.cpp:802-817) with `prev` holding the post-visit saturation state of
 // the cells the i->j->k sweep has already visited and `sat0` the pre-visit state of the rest.
 // The reference's early-exit chain as one boolean expression over operands loaded together.
-template <class RK>
-__device__ inline bool adjacent_sat(const RK& P, const Cell& t, const double* __restrict__ Kx,
-                                    const double* __restrict__ Ky, const unsigned char* __restrict__ sat0,
+template <class RK, class FA>
+__device__ inline bool adjacent_sat(const RK& P, const Cell& t, const FA& fa, const unsigned char* __restrict__ sat0,
                                     const unsigned char* __restrict__ prev) {
   const Geo& g = P.g;
   const long long c = t.c;
   const unsigned char below = sat0[c + g.sz], xp = sat0[c + 1], xm = prev[c - 1], yp = sat0[c + g.sy],
                       ym = prev[c - g.sy], above = prev[c - g.sz];
-  const double kx0 = Kx[c], kxm = Kx[c - 1], ky0 = Ky[c], kym = Ky[c - g.sy];
+  const bool kx0 = fa.x(c), kxm = fa.x(c - 1), ky0 = fa.y(c), kym = fa.y(c - g.sy);
   const bool top = (t.k == 0) ? (!P.sim_sw && P.bc5 == 1 && P.htop >= 0.0) : (above != 0);
-  return below || (kx0 != 0.0 && xp) || (kxm != 0.0 && xm) || (ky0 != 0.0 && yp) || (kym != 0.0 && ym) || top;
+  return below || (kx0 && xp) || (kxm && xm) || (ky0 && yp) || (kym && ym) || top;
 }
 
 // One Jacobi sweep of the monotone map whose unique fixed point is the sequential sweep's
 // saturation state: a candidate (unsaturated theta, saturated theta(h)) becomes saturated iff
 // it is adjacent to saturation, reading visited neighbours from `prev`.
-template <class RK>
-__device__ __forceinline__ void re_realloc(const RK& P, const Cell& t, const double* __restrict__ Kx,
-                                           const double* __restrict__ Ky, const unsigned char* __restrict__ sat0,
+template <class RK, class FA>
+__device__ __forceinline__ void re_realloc(const RK& P, const Cell& t, const FA& fa,
+                                           const unsigned char* __restrict__ sat0,
                                            const unsigned char* __restrict__ cand,
                                            const unsigned char* __restrict__ prev, unsigned char* __restrict__ next,
                                            int* changed) {
   unsigned char v = sat0[t.c];
-  if (cand[t.c]) v = adjacent_sat(P, t, Kx, Ky, sat0, prev) ? 1 : 0;  // candidates are few
+  if (cand[t.c]) v = adjacent_sat(P, t, fa, sat0, prev) ? 1 : 0;  // candidates are few
   next[t.c] = v;
   if (v != prev[t.c]) atomicOr(changed, 1);
 }
@@ -501,20 +525,22 @@ __device__ __forceinline__ void re_realloc(const RK& P, const Cell& t, const dou
 // reallocateWaterContent branches with the converged saturation state + clampWaterContent +
 // enforceMoistureBc (ReSolver.cpp:819-878) for one cell, plus its ledger terms. hmod (optional)
 // receives 1 where the head was replaced by h(theta), else 0.
-template <class RK>
-__device__ __forceinline__ void re_finalize(const RK& P, const Cell& t, const double* __restrict__ Kx,
-                                            const double* __restrict__ Ky, const unsigned char* __restrict__ sat0,
+// hout (optional): the final head goes to hout (every interior cell, and the ghost heads of
+// boundary cells copied from h) and h keeps the post-solve head; without it h is updated in place.
+template <class RK, class FA>
+__device__ __forceinline__ void re_finalize(const RK& P, const Cell& t, const FA& fa,
+                                            const unsigned char* __restrict__ sat0,
                                             const unsigned char* __restrict__ fin, const double* __restrict__ wch,
                                             const double* __restrict__ hwc, const double* __restrict__ hn,
                                             double* __restrict__ wc, double* __restrict__ h,
-                                            unsigned char* __restrict__ hmod, double& l_after, double& l_adj,
-                                            double& dh) {
+                                            double* __restrict__ hout, unsigned char* __restrict__ hmod,
+                                            double& l_after, double& l_adj, double& dh) {
   const Geo& g = P.g;
   const long long c = t.c;
   const SoilClass& s = P.soil.at(c);
   const double wcorr = wc[c], hw = hwc[c], hc = h[c], whc = wch[c], hnc = hn[c];
   const unsigned char s0 = sat0[c];
-  const bool adj = adjacent_sat(P, t, Kx, Ky, sat0, fin);
+  const bool adj = adjacent_sat(P, t, fa, sat0, fin);
   double w = wcorr;
   bool replace = false;
   if (s0) {
@@ -527,7 +553,17 @@ __device__ __forceinline__ void re_finalize(const RK& P, const Cell& t, const do
   const double hnew = replace ? hw : hc;
   if (w > s.theta_s) w = s.theta_s;
   else if (w < s.theta_r) w = s.theta_r;
-  if (replace) h[c] = hnew;
+  if (hout) {
+    hout[c] = hnew;
+    const long long gh[6] = {t.i == 0 ? c - 1 : -1,         t.i == g.nx - 1 ? c + 1 : -1,
+                             t.j == 0 ? c - g.sy : -1,      t.j == g.ny - 1 ? c + g.sy : -1,
+                             t.k == 0 ? c - g.sz : -1,      t.k == g.nz - 1 ? c + g.sz : -1};
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      if (gh[q] >= 0) hout[gh[q]] = h[gh[q]];
+  } else if (replace) {
+    h[c] = hnew;
+  }
   wc[c] = w;
   if (t.i == 0) wc[c - 1] = w;
   if (t.i == g.nx - 1) wc[c + 1] = w;

@@ -13,7 +13,7 @@ import pytest
 
 from cases import box3d_params, box3d_soil, box3d_wc, interior3
 from frehg_amd import abi
-from gpu_common import compare_re, re_pair, say
+from gpu_common import RE_COMPARE, compare_re, re_pair, say
 
 pytestmark = pytest.mark.gpu
 
@@ -107,6 +107,41 @@ def test_incremental_kc_matches_full_reevaluation(full3d):
         b.re_set("h", b.re_get("h"))  # invalidates the incremental kr / C state
         b.re_step()
         for f in ("h", "wc", "ch", "wch", "hwc", "qz", "Gct", "Grhs"):
+            assert np.array_equal(a.re_get(f), b.re_get(f)), f"step {s} field {f}"
+
+
+@pytest.mark.parametrize("full3d,adaptive", [(1, 0), (0, 0), (1, 1)])
+def test_lazy_flux_fields_match_stored(full3d, adaptive):
+    """By default the step stores no Kx..qz (only face bits for the reallocation) and rebuilds
+    them from the post-solve state when they are read or before the state changes; a context
+    that stores them in the step (FREHG_LAZY_FLUX=0) must give every field bit for bit, whether
+    the fields are read after every step, skipped for a step, or forced out by set_params."""
+    nx, ny, nz = 9, 7, 12
+    say(f"RE lazy flux fields full3d={full3d} adaptive={adaptive}")
+    soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5, Ss=1e-5)
+    p = abi.re_params(soil=soil, solver=abi.solver_config(1e-13), bc_type=(0, 0, 0, 0, 0, 1), dx=0.5, dy=0.5,
+                      dz=0.05, dt=60.0, adaptive_dt=adaptive, dt_max=120.0, dt_min=1.0, use_full3d=full3d,
+                      htop=0.05)
+    k = np.arange(nz)[:, None, None]
+    i = np.arange(nx)[None, None, :]
+    h0 = np.broadcast_to(-0.05 - 0.08 * k - 0.01 * i, (nz, ny, nx)).ravel()
+    os.environ["FREHG_SPMV_SHAPE"] = "2:12"
+    try:
+        _, a = re_pair(nx, ny, nz, p, h0=h0)
+        os.environ["FREHG_LAZY_FLUX"] = "0"
+        _, b = re_pair(nx, ny, nz, p, h0=h0)
+    finally:
+        del os.environ["FREHG_SPMV_SHAPE"]
+        os.environ.pop("FREHG_LAZY_FLUX", None)
+    for s in range(9):
+        a.re_step()
+        b.re_step()
+        if s % 3 == 1:
+            continue  # the pending fields of this step are superseded by the next one
+        if s == 5:
+            a.re_set_params(p)  # stores the pending fields before the parameters change
+            b.re_set_params(p)
+        for f in RE_COMPARE:
             assert np.array_equal(a.re_get(f), b.re_get(f)), f"step {s} field {f}"
 
 
