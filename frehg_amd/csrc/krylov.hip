@@ -484,8 +484,9 @@ __global__ __launch_bounds__(NT) void k_cg_check(KrylovCtl* ctl, const double* p
   (void)cg_spmv_begin<NT>(ctl, part_u, nb_u, it, true, lds, &sflag, &beta);
 }
 
-// Iterative-refinement residual: xout = xin + d (d may be null), r32 = b - A xout in FP64,
-// partial ||D^-1 r||^2 (and ||D^-1 bn||^2 on the first sweep, which also sets the test).
+// Iterative-refinement residual: xout = xin + d (d may be null; xout may be xin only then),
+// r32 = b - A xout in FP64, partial ||D^-1 r||^2 (and ||D^-1 bn||^2 on the first sweep, which
+// also sets the test).
 template <bool THREE_D, bool SYM>
 __global__ __launch_bounds__(NT) void k_refine(Geo g, int kchunk, StencilView<double> A,
                                                 const double* b, const double* bn, const double* xin,
@@ -510,7 +511,7 @@ __global__ __launch_bounds__(NT) void k_refine(Geo g, int kchunk, StencilView<do
         y = apply_in_domain<double, THREE_D>(g, A, c, t.i, t.j, k, xc, X);
       }
       const double rv = fx ? 0.0 : b[c] - y;
-      if (d) xout[c] = xc;
+      if (xout != xin) xout[c] = xc;
       r32[c] = (float)rv;
       const double rz = rv / A.D[c];
       srr += rz * rz;
@@ -526,6 +527,129 @@ __global__ __launch_bounds__(NT) void k_refine(Geo g, int kchunk, StencilView<do
   if (threadIdx.x == 0) {
     part[bid] = a;
     part[nblocks + bid] = bsum;
+  }
+}
+
+// Operands of one layer of the paired refinement march (see k_refine_pair).
+struct RefinePairLayer {
+  Pair<double> xzp, xym, xyp, d, ax, ay, aym, az, b;
+  Pair<float> dzp, dym, dyp;
+  double xe, axe;
+  float de;
+};
+
+template <bool HAS_D>
+__device__ __forceinline__ void refine_pair_load(const Geo& g, const StencilView<double>& A,
+                                                 const double* __restrict__ b, const double* __restrict__ xin,
+                                                 const float* __restrict__ d, long long c, long long ce, bool edge,
+                                                 RefinePairLayer& v) {
+  v.xzp = ld_pair(xin, c + g.sz);
+  v.xym = ld_pair(xin, c - g.sy);
+  v.xyp = ld_pair(xin, c + g.sy);
+  v.d = ld_pair(A.D, c);
+  v.ax = ld_pair(A.ax, c);
+  v.ay = ld_pair(A.ay, c);
+  v.aym = ld_pair(A.ay, c - g.sy);
+  v.az = ld_pair(A.az, c);
+  v.b = ld_pair(b, c);
+  v.dzp = v.dym = v.dyp = Pair<float>{0.f, 0.f};
+  if (HAS_D) {
+    v.dzp = ld_pair(d, c + g.sz);
+    v.dym = ld_pair(d, c - g.sy);
+    v.dyp = ld_pair(d, c + g.sy);
+  }
+  v.xe = v.axe = 0.0;
+  v.de = 0.f;
+  if (edge) {
+    v.xe = xin[ce];
+    v.axe = A.ax[ce];
+    if (HAS_D) v.de = d[ce];
+  }
+}
+
+// k_refine for the 3D symmetric operator without fixed rows or a separate bn, on the 128 x 4 cell
+// pair tiles and kchunk-layer z-march of k_spmv_pair (aligned 16-byte loads, x-neighbours by wave
+// shifts, z-neighbours from registers). Per cell the arithmetic is that of k_refine. On the first
+// sweep (no d) xout receives a copy of xin, so that the sweep after the first inner solve can write
+// its iterate back into xin's buffer.
+template <bool HAS_D>
+__global__ __launch_bounds__(NT) void k_refine_pair(Geo g, int kchunk, StencilView<double> A,
+                                                     const double* __restrict__ b, const double* __restrict__ xin,
+                                                     const float* __restrict__ d, double* __restrict__ xout,
+                                                     float* __restrict__ r32, double* part, int first) {
+  __shared__ double lds[NT / kWave];
+  const TileId tb = xcd_tile();
+  const int lane = threadIdx.x & 63;
+  const int i0 = tb.x * 128, i = i0 + 2 * lane, j = tb.y * TY + (threadIdx.x >> 6);
+  const int k0 = tb.z * kchunk, k1 = min(g.nz, k0 + kchunk);
+  double srr = 0.0, sbb = 0.0;
+  if (j < g.ny) {  // whole wave
+    const bool va = i < g.nx, vb = i + 1 < g.nx;
+    long long c = g.idx(min(i, g.nx & ~1), j, k0);
+    const bool edge = lane == 0 || lane == 63;
+    const long long dce = (lane == 0 ? g.idx(i0 - 1, j, k0) : g.idx(min(i0 + 128, g.nx), j, k0)) - c;
+    auto add = [](Pair<double> x, Pair<float> e) -> Pair<double> {
+      return HAS_D ? Pair<double>{x.a + (double)e.a, x.b + (double)e.b} : x;
+    };
+    Pair<float> dz0{0.f, 0.f}, dzm{0.f, 0.f};
+    if (HAS_D) {
+      dz0 = ld_pair(d, c);
+      dzm = ld_pair(d, c - g.sz);
+    }
+    const Pair<double> xm = add(ld_pair(xin, c - g.sz), dzm);
+    const Pair<double> azm2 = ld_pair(A.az, c - g.sz);
+    const Pair<double> x0 = add(ld_pair(xin, c), dz0);
+    double xma = xm.a, xmb = xm.b, azma = azm2.a, azmb = azm2.b, xa = x0.a, xb = x0.b;
+    RefinePairLayer cur, nxt;
+    refine_pair_load<HAS_D>(g, A, b, xin, d, c, c + dce, edge, cur);
+    for (int k = k0; k < k1; ++k) {
+      const bool more = k + 1 < k1;
+      if (more) refine_pair_load<HAS_D>(g, A, b, xin, d, c + g.sz, c + g.sz + dce, edge, nxt);
+      const Pair<double> xp = add(cur.xzp, cur.dzp), xym = add(cur.xym, cur.dym), xyp = add(cur.xyp, cur.dyp);
+      const double xe = edge ? cur.xe + (HAS_D ? (double)cur.de : 0.0) : 0.0;
+      const double from_l = lane_from_left(xb), from_r = lane_from_right(xa);
+      const double ax_l = lane_from_left(cur.ax.b);
+      const double xxm = lane == 0 ? xe : from_l;         // X(i - 1)
+      const double xxp = lane == 63 ? xe : from_r;        // X(i + 2)
+      const double axm = lane == 0 ? cur.axe : ax_l;      // ax(i - 1)
+      double ya = cur.d.a * xa + cur.ax.a * xb + axm * xxm + cur.ay.a * xyp.a + cur.aym.a * xym.a;
+      ya += cur.az.a * xp.a + azma * xma;
+      double yb = cur.d.b * xb + cur.ax.b * xxp + cur.ax.a * xa + cur.ay.b * xyp.b + cur.aym.b * xym.b;
+      yb += cur.az.b * xp.b + azmb * xmb;
+      const double ra = cur.b.a - ya, rb = cur.b.b - yb;
+      const double za = ra / cur.d.a, zb = rb / cur.d.b;
+      const double ba = cur.b.a / cur.d.a, bbz = cur.b.b / cur.d.b;
+      if (vb) {
+        *reinterpret_cast<Pair<double>*>(xout + c) = Pair<double>{xa, xb};
+        *reinterpret_cast<Pair<float>*>(r32 + c) = Pair<float>{(float)ra, (float)rb};
+        srr += za * za;
+        srr += zb * zb;
+        if (first) {
+          sbb += ba * ba;
+          sbb += bbz * bbz;
+        }
+      } else if (va) {
+        xout[c] = xa;
+        r32[c] = (float)ra;
+        srr += za * za;
+        if (first) sbb += ba * ba;
+      }
+      xma = xa;
+      xmb = xb;
+      xa = xp.a;
+      xb = xp.b;
+      azma = cur.az.a;
+      azmb = cur.az.b;
+      c += g.sz;
+      if (more) cur = nxt;
+    }
+  }
+  const double a = block_sum<NT>(srr, lds);
+  const double bs = block_sum<NT>(sbb, lds);
+  const unsigned int bid = block_id();
+  if (threadIdx.x == 0) {
+    part[bid] = a;
+    part[gridDim.x * gridDim.y * gridDim.z + bid] = bs;
   }
 }
 
@@ -1228,14 +1352,26 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
     else hipLaunchKernelGGL((k_zero_interior<false>), grid_f_, blk, 0, s_, g_, kc_f_, x, fixed);
     FREHG_LAUNCH_CHECK();
   }
-  FREHG_HIP(hipMemcpyAsync(xw_, x, (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
-  double* xcur = x;
-  double* xalt = xw_;
+  // The iterate alternates between x and xw_: the first sweep copies x to xw_, and each later
+  // sweep reads one and writes the other, so that after one refinement it is back in x. xw_'s
+  // ghost entries are never written: they stay finite (zero or earlier heads) and meet zero
+  // coefficients only.
   auto refine = [&](const double* xin, const float* d, double* xout, bool first) {
     const bool sym = A.symmetric();
     hipEvent_t ev0 = three_d_ ? prof_->start(FREHG_KT_GW_REFINE) : nullptr;
     hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
-    if (three_d_ && sym)
+    unsigned int nbr = nb;
+    if (three_d_ && sym && fixed == nullptr && bn == nullptr) {
+      const int kc = shape_[0].kc;
+      const dim3 grid(ceil_div(g_.nx, 2 * TX), ceil_div(g_.ny, TY), ceil_div(g_.nz, kc));
+      nbr = grid.x * grid.y * grid.z;
+      if (d)
+        hipExtLaunchKernelGGL((k_refine_pair<true>), grid, blk, 0, s_, ev0, ev1, 0, g_, kc, A, b, xin, d, xout, b32_,
+                              part_, first ? 1 : 0);
+      else
+        hipExtLaunchKernelGGL((k_refine_pair<false>), grid, blk, 0, s_, ev0, ev1, 0, g_, kc, A, b, xin, d, xout,
+                              b32_, part_, first ? 1 : 0);
+    } else if (three_d_ && sym)
       hipExtLaunchKernelGGL((k_refine<true, true>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A, b, bn, xin, d,
                             xout, b32_, fixed, part_, nb, first ? 1 : 0);
     else if (three_d_)
@@ -1248,19 +1384,20 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
       hipLaunchKernelGGL((k_refine<false, false>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, xin, d, xout, b32_,
                          fixed, part_, nb, first ? 1 : 0);
     FREHG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fin_refine, dim3(1), dim3(kRedThreads), 0, s_, part_, nb, outer, first ? 1 : 0, cfg.rtol,
+    hipLaunchKernelGGL(k_fin_refine, dim3(1), dim3(kRedThreads), 0, s_, part_, nbr, outer, first ? 1 : 0, cfg.rtol,
                        cfg.atol, (guard_guess && !x0_zero) ? 1 : 0);
     FREHG_LAUNCH_CHECK();
   };
-  refine(xcur, nullptr, xcur, true);
+  double* xcur = xw_;
+  double* xalt = x;
+  refine(x, nullptr, xw_, true);
   read_ctl(outer, host_ctl_);
   if (host_ctl_->guess_rejected) {
     res.guess_rejected = true;
-    if (three_d_) hipLaunchKernelGGL((k_zero_interior<true>), grid_f_, blk, 0, s_, g_, kc_f_, xcur, fixed);
-    else hipLaunchKernelGGL((k_zero_interior<false>), grid_f_, blk, 0, s_, g_, kc_f_, xcur, fixed);
+    if (three_d_) hipLaunchKernelGGL((k_zero_interior<true>), grid_f_, blk, 0, s_, g_, kc_f_, x, fixed);
+    else hipLaunchKernelGGL((k_zero_interior<false>), grid_f_, blk, 0, s_, g_, kc_f_, x, fixed);
     FREHG_LAUNCH_CHECK();
-    FREHG_HIP(hipMemcpyAsync(xw_, x, (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
-    refine(xcur, nullptr, xcur, true);
+    refine(x, nullptr, xw_, true);
     read_ctl(outer, host_ctl_);
   }
   res.rhs_norm = std::sqrt(host_ctl_->bb);
@@ -1285,7 +1422,9 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
     res.residual_norm = std::sqrt(host_ctl_->zz);
     conv = host_ctl_->converged != 0;
   }
-  if (xcur != x) FREHG_HIP(hipMemcpyAsync(x, xcur, (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+  // without a refinement x itself holds the iterate (xw_ holds a copy)
+  if (res.refinements > 0 && xcur != x)
+    FREHG_HIP(hipMemcpyAsync(x, xcur, (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
   if (!conv) {
     // Refinement stalled: finish in FP64 from the current iterate.
     bool rej = false, c64 = false;
