@@ -1,4 +1,5 @@
-// Learned initial guess: feature extraction, pair logging, BF16 MFMA forward / backward, Adam.
+// Learned initial guess: feature extraction, pair logging, BF16 MFMA forward / backward, Adam;
+// FP16 MFMA inference (k_mlp_guess).
 //
 // Every dense layer runs on v_mfma_f32_16x16x32_bf16 (D[16x16] += A[16x32] B[32x16], FP32
 // accumulate). Lane l of a wave supplies A[l%16][8(l/16) .. 8(l/16)+7] and
@@ -26,6 +27,8 @@ namespace frehg {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int MT = 256;   // threads per block
@@ -89,7 +92,7 @@ __device__ inline void cell_features(const FeatSrc& F, int i, int j, int k, floa
   f[4] = (float)(F.hn[c + g.sy] - hc);
   f[5] = (float)(F.hn[c - g.sz] - hc);
   f[6] = (float)(F.hn[c + g.sz] - hc);
-  f[7] = (float)log(fmax(F.soil.Ksz(c) * F.kr[c], 1e-30));
+  f[7] = logf((float)fmax(F.soil.Ksz(c) * F.kr[c], 1e-30));
   f[8] = (float)F.ch[c];
   f[9] = (float)F.wcn[c];
   f[10] = F.depth ? (float)F.depth[g.sidx(i, j)] : 0.0f;
@@ -355,11 +358,19 @@ __global__ __launch_bounds__(MT) void k_mlp_predict_rows(const float* P, const f
 
 // Learned guess on every interior cell: h_out = h^n + increment.
 //
-// Inference runs the layers transposed (Z^T = W X^T: neurons are MFMA rows, cells are columns)
-// so that each layer's output registers already hold the next layer's B operand, with no LDS
-// round trip between layers. After layer 1, lane l (g = l/16, c = l%16) holds neurons
-// 16 nb + 4 g + r (nb, r = 0..3) of cell c. The next layer's B operand asks lane l for reduction
-// indices kappa = 32 ks + 8 g + t (ks = 0..1, t = 0..7); the bijection
+// Inference form of the network (k_mlp_prep_infer). With s = 2 log2(e) and r = 1 / (1 + 2^(s z)),
+// tanh z = 1 - 2 r: a hidden layer outputs r (one exp2, one add, one reciprocal per neuron) and
+// the next layer absorbs the affine map, W (1 - 2 r) + b = (-2 W) r + (b + W 1); the scale s of
+// the pre-activations is carried by the weights and biases. The layers run on
+// v_mfma_f32_16x16x32_f16 with the biases as the accumulator's initial value; weights are rounded
+// to FP16 (nearest even), the standardised features and the r in [0, 1] to FP16 (toward zero,
+// two values per conversion).
+//
+// The layers run transposed (Z^T = W X^T: neurons are MFMA rows, cells are columns) so that each
+// layer's output registers already hold the next layer's B operand, with no LDS round trip
+// between layers. After layer 1, lane l (g = l/16, c = l%16) holds neurons 16 nb + 4 g + r
+// (nb, r = 0..3) of cell c. The next layer's B operand asks lane l for reduction indices
+// kappa = 32 ks + 8 g + t (ks = 0..1, t = 0..7); the bijection
 //   pi(32 ks + 8 g + t) = 16 (2 ks + t/4) + 4 g + t%4
 // maps every kappa a lane needs to a neuron the lane holds, so the weight operands are stored
 // with their columns permuted by pi (a relabelling of the hidden units, exact).
@@ -368,39 +379,81 @@ __global__ __launch_bounds__(MT) void k_mlp_predict_rows(const float* P, const f
 constexpr int GW = 4;   // waves per block of the guess kernel
 constexpr int FP = 16;  // LDS pitch of a feature row (11 used, zero padded)
 
+// Inference parameter vector (FP32): W1s[64][16] (inputs zero padded), B1s[64], W2s[64][64],
+// B2s[64], W3s[64], B3s.
+constexpr int kIW1 = 0, kIB1 = kIW1 + kMlpH * FP, kIW2 = kIB1 + kMlpH, kIB2 = kIW2 + kMlpH * kMlpH,
+              kIW3 = kIB2 + kMlpH, kIB3 = kIW3 + kMlpH, kInfer = kIB3 + 1;
+
+__global__ __launch_bounds__(MT) void k_mlp_prep_infer(const float* P, float* Q) {
+  const float s = 2.0f * 1.44269504088896341f;  // 2 log2(e)
+  for (int q = threadIdx.x; q < kMlpH * FP; q += MT) {
+    const int n = q / FP, f = q % FP;
+    Q[kIW1 + q] = f < kMlpIn ? s * P[kOffW1 + n * kMlpIn + f] : 0.0f;
+  }
+  for (int q = threadIdx.x; q < kMlpH * kMlpH; q += MT) Q[kIW2 + q] = (-2.0f * s) * P[kOffW2 + q];
+  if (threadIdx.x < kMlpH) {
+    const int n = threadIdx.x;
+    float sum = 0.0f;
+    for (int k = 0; k < kMlpH; ++k) sum += P[kOffW2 + n * kMlpH + k];
+    Q[kIB1 + n] = s * P[kOffB1 + n];
+    Q[kIB2 + n] = s * (P[kOffB2 + n] + sum);
+    Q[kIW3 + n] = -2.0f * P[kOffW3 + n];
+  }
+  if (threadIdx.x == 0) {
+    float sum = 0.0f;
+    for (int k = 0; k < kMlpH; ++k) sum += P[kOffW3 + k];
+    Q[kIB3] = P[kOffB3] + sum;
+  }
+}
+
 __device__ __forceinline__ int hidden_perm(int kappa) {
   const int ks = kappa >> 5, g = (kappa >> 3) & 3, t = kappa & 7;
   return 16 * (2 * ks + (t >> 2)) + 4 * g + (t & 3);
 }
 
-__global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* P, const float* norm, FeatSrc F, double* h_out) {
+// Two FP32 values as one FP16 pair (round toward zero).
+__device__ __forceinline__ unsigned pk16(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(a, b));
+}
+__device__ __forceinline__ f16x8 pk16x8(float v0, float v1, float v2, float v3, float v4, float v5, float v6,
+                                        float v7) {
+  return __builtin_bit_cast(f16x8, u32x4{pk16(v0, v1), pk16(v2, v3), pk16(v4, v5), pk16(v6, v7)});
+}
+__device__ __forceinline__ f32x4 mfma16(f16x8 a, f16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+// r = 1 / (1 + 2^z) (z = s times the pre-activation); 0 for large z, 1 for very negative z.
+__device__ __forceinline__ float act_r(float z) { return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z)); }
+
+__global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const float* norm, FeatSrc F, double* h_out) {
   __shared__ __attribute__((aligned(16))) float xs[GW][RB][FP];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int ra = lane & 15, g = lane >> 4, kg = 8 * g;
-  // A operands (rows = neurons): W1[16 nb + ra][kg + t]; W2[16 nb + ra][pi(32 ks + kg + t)];
-  // w3[pi(32 ks + kg + t)] in row 0 only. Biases of the neurons this lane's outputs hold.
-  bf16x8 a1w[4], a2w[4][2], a3w[2];
-  float b1r[4][4], b2r[4][4];
+  // A operands (rows = neurons): W1s[16 nb + ra][kg + t] (kg < 16); W2s[16 nb + ra][pi(32 ks + kg + t)];
+  // W3s[pi(32 ks + kg + t)] in row 0 only. Accumulator initial values: the biases of the neurons
+  // this lane's outputs hold.
+  f16x8 a1w[4], a2w[4][2], a3w[2];
+  f32x4 b1r[4], b2r[4];
 #pragma unroll
   for (int nb = 0; nb < 4; ++nb) {
     const int n = 16 * nb + ra;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) a1w[nb][t] = (__bf16)(kg + t < kMlpIn ? P[kOffW1 + n * kMlpIn + kg + t] : 0.0f);
+    for (int t = 0; t < 8; ++t) a1w[nb][t] = (_Float16)(kg < FP ? Q[kIW1 + n * FP + kg + t] : 0.0f);
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-      for (int t = 0; t < 8; ++t) a2w[nb][ks][t] = (__bf16)P[kOffW2 + n * kMlpH + hidden_perm(32 * ks + kg + t)];
+      for (int t = 0; t < 8; ++t) a2w[nb][ks][t] = (_Float16)Q[kIW2 + n * kMlpH + hidden_perm(32 * ks + kg + t)];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      b1r[nb][r] = P[kOffB1 + 16 * nb + 4 * g + r];
-      b2r[nb][r] = P[kOffB2 + 16 * nb + 4 * g + r];
+      b1r[nb][r] = Q[kIB1 + 16 * nb + 4 * g + r];
+      b2r[nb][r] = Q[kIB2 + 16 * nb + 4 * g + r];
     }
   }
 #pragma unroll
   for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-    for (int t = 0; t < 8; ++t) a3w[ks][t] = (__bf16)(ra == 0 ? P[kOffW3 + hidden_perm(32 * ks + kg + t)] : 0.0f);
-  const float b3 = P[kOffB3];
+    for (int t = 0; t < 8; ++t) a3w[ks][t] = (_Float16)(ra == 0 ? Q[kIW3 + hidden_perm(32 * ks + kg + t)] : 0.0f);
+  const f32x4 b3r = {Q[kIB3], 0.f, 0.f, 0.f};
   float nmu[kMlpIn], nisd[kMlpIn];
 #pragma unroll
   for (int t = 0; t < kMlpIn; ++t) {
@@ -433,47 +486,45 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* P, const flo
 #pragma unroll 1
     for (int tile = 0; tile < 4; ++tile) {
       // layer 1: B = X^T, lane supplies features kg..kg+7 of cell 16 tile + ra (zero for kg >= 16)
-      bf16x8 xb;
+      f16x8 xb = pk16x8(0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f);
       if (g < 2) {
-        xb = pack8(&xw[16 * tile + ra][kg], 1);
-      } else {
-#pragma unroll
-        for (int t = 0; t < 8; ++t) xb[t] = (__bf16)0.0f;
+        const float* xr = &xw[16 * tile + ra][kg];
+        xb = pk16x8(xr[0], xr[1], xr[2], xr[3], xr[4], xr[5], xr[6], xr[7]);
       }
       float a[4][4];
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) {
-        const f32x4 z = mfma(a1w[nb], xb, f32x4{0.f, 0.f, 0.f, 0.f});
+        const f32x4 z = mfma16(a1w[nb], xb, b1r[nb]);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) a[nb][r] = act_tanh(z[r] + b1r[nb][r]);
+        for (int r = 0; r < 4; ++r) a[nb][r] = act_r(z[r]);
       }
-      // layer 2: B[kappa][cell] = a1[pi(kappa)] from this lane's registers
-      bf16x8 hb[2];
+      // layer 2: B[kappa][cell] = r1[pi(kappa)] from this lane's registers
+      f16x8 hb[2];
 #pragma unroll
       for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-        for (int t = 0; t < 8; ++t) hb[ks][t] = (__bf16)a[2 * ks + (t >> 2)][t & 3];
+        hb[ks] = pk16x8(a[2 * ks][0], a[2 * ks][1], a[2 * ks][2], a[2 * ks][3], a[2 * ks + 1][0],
+                        a[2 * ks + 1][1], a[2 * ks + 1][2], a[2 * ks + 1][3]);
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) {
-        f32x4 z = {0.f, 0.f, 0.f, 0.f};
+        f32x4 z = b2r[nb];
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) z = mfma(a2w[nb][ks], hb[ks], z);
+        for (int ks = 0; ks < 2; ++ks) z = mfma16(a2w[nb][ks], hb[ks], z);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) a[nb][r] = act_tanh(z[r] + b2r[nb][r]);
+        for (int r = 0; r < 4; ++r) a[nb][r] = act_r(z[r]);
       }
-      // layer 3: row 0 of w3[pi] times A2^T
+      // layer 3: row 0 of W3s[pi] times R2^T
 #pragma unroll
       for (int ks = 0; ks < 2; ++ks)
+        hb[ks] = pk16x8(a[2 * ks][0], a[2 * ks][1], a[2 * ks][2], a[2 * ks][3], a[2 * ks + 1][0],
+                        a[2 * ks + 1][1], a[2 * ks + 1][2], a[2 * ks + 1][3]);
+      f32x4 y = b3r;
 #pragma unroll
-        for (int t = 0; t < 8; ++t) hb[ks][t] = (__bf16)a[2 * ks + (t >> 2)][t & 3];
-      f32x4 y = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < 2; ++ks) y = mfma(a3w[ks], hb[ks], y);
+      for (int ks = 0; ks < 2; ++ks) y = mfma16(a3w[ks], hb[ks], y);
       if (lane < 16) {  // D row 0 (g = 0, r = 0) holds y of cell ra
         const int ii = ib * RB + 16 * tile + ra;
         if (ii < g3.nx) {
           const long long c = g3.idx(ii, j, k);
-          h_out[c] = F.hn[c] + (double)((y[0] + b3) * ysd + ymu);
+          h_out[c] = F.hn[c] + (double)(y[0] * ysd + ymu);
         }
       }
     }
@@ -537,6 +588,7 @@ constexpr int kStatBlocks = 256;
 
 Mlp::Mlp(hipStream_t s, Profiler* prof) : s_(s), prof_(prof) {
   FREHG_HIP(hipMalloc(&params_, kMlpParams * sizeof(float)));
+  FREHG_HIP(hipMalloc(&infer_, kInfer * sizeof(float)));
   FREHG_HIP(hipMalloc(&m_, kMlpParams * sizeof(float)));
   FREHG_HIP(hipMalloc(&v_, kMlpParams * sizeof(float)));
   FREHG_HIP(hipMalloc(&norm_, kMlpNorm * sizeof(float)));
@@ -548,7 +600,7 @@ Mlp::Mlp(hipStream_t s, Profiler* prof) : s_(s), prof_(prof) {
 }
 
 Mlp::~Mlp() {
-  for (void* p : {(void*)params_, (void*)m_, (void*)v_, (void*)norm_, (void*)grad_, (void*)part_, (void*)log_,
+  for (void* p : {(void*)params_, (void*)infer_, (void*)m_, (void*)v_, (void*)norm_, (void*)grad_, (void*)part_, (void*)log_,
                   (void*)scratch_, (void*)stat_})
     if (p) (void)hipFree(p);
   if (hbuf_) (void)hipHostFree(hbuf_);
@@ -579,6 +631,7 @@ void Mlp::init(uint64_t seed) {
   FREHG_HIP(hipMemcpy(norm_, N.data(), kMlpNorm * sizeof(float), hipMemcpyHostToDevice));
   adam_t_ = 0;
   have_norm_ = false;
+  infer_stale_ = true;
   ready_ = false;
 }
 
@@ -665,6 +718,7 @@ double Mlp::train(int steps, int batch, double lr, uint64_t seed, double* first)
   }
   if (first) *first = loss_first;
   ready_ = true;
+  infer_stale_ = true;
   return loss;
 }
 
@@ -677,9 +731,14 @@ void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const d
   const long long nchunks = (long long)((F.g.nx + RB - 1) / RB) * F.g.ny * F.g.nz;
   const unsigned nb = (unsigned)std::min<long long>(4096, (nchunks + GW - 1) / GW);
   (void)N;
+  if (infer_stale_) {
+    hipLaunchKernelGGL(k_mlp_prep_infer, dim3(1), dim3(MT), 0, s_, (const float*)params_, infer_);
+    FREHG_LAUNCH_CHECK();
+    infer_stale_ = false;
+  }
   hipEvent_t ev0 = prof_->start(FREHG_KT_GW_GUESS);
   hipEvent_t ev1 = prof_->stop();
-  hipExtLaunchKernelGGL(k_mlp_guess, dim3(nb), dim3(64 * GW), 0, s_, ev0, ev1, 0, (const float*)params_,
+  hipExtLaunchKernelGGL(k_mlp_guess, dim3(nb), dim3(64 * GW), 0, s_, ev0, ev1, 0, (const float*)infer_,
                         (const float*)norm_, F, h_out);
   FREHG_LAUNCH_CHECK();
 }
@@ -702,6 +761,7 @@ void Mlp::set_params(const float* params, const float* norm) {
   adam_t_ = 0;
   have_norm_ = true;
   ready_ = true;
+  infer_stale_ = true;
 }
 
 void Mlp::ensure_scratch(int nrows) {

@@ -77,6 +77,10 @@ class Mlp {
   hipStream_t s_;
   Profiler* prof_;
   bool ready_ = false;
+  // Inference form of the parameters (k_mlp_prep_infer), rebuilt before the next guess whenever
+  // the parameters or the normalisation change.
+  float* infer_ = nullptr;
+  bool infer_stale_ = true;
   int stride_ = 0;
   long long cap_ = 0, count_ = 0, log_steps_ = 0;
   int adam_t_ = 0;

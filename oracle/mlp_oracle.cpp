@@ -8,7 +8,13 @@
 //   bf16 = 1: the GPU's arithmetic -- standardisation, biases, tanh and elementwise products in
 //             FP32, every matrix-product operand rounded to BF16 (round to nearest even), the
 //             products accumulated exactly in FP64 (the GPU accumulates in FP32).
+// Predictions have a third mode, 2, the arithmetic of the GPU's inference kernel: with
+// s = 2 log2(e) and r = 1 / (1 + 2^(s z)) = (1 - tanh z) / 2, the hidden layers output r and the
+// next layer takes weights -2 W and biases b + sum_k W[.][k]; the scaled weights and biases are
+// formed in FP32, weights rounded to FP16 (nearest even), standardised inputs and every r to FP16
+// (toward zero), products accumulated in FP64 (the GPU accumulates in FP32).
 // Parameter layout: W1[64][11], b1[64], W2[64][64], b2[64], w3[64], b3.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -30,6 +36,18 @@ double bf(double v) {
   u &= 0xFFFF0000u;
   std::memcpy(&f, &u, 4);
   return (double)f;
+}
+
+// double -> FP16 value (11-bit significand, normal exponents down to -14, subnormals below),
+// round to nearest even or toward zero; no overflow expected here.
+double f16(double v, bool rtz) {
+  if (v == 0.0) return 0.0;
+  int e;
+  (void)std::frexp(std::fabs(v), &e);  // |v| = m 2^e, m in [0.5, 1)
+  const double ulp = std::ldexp(1.0, std::max(e - 1, -14) - 10);
+  const double q = std::fabs(v) / ulp;
+  const double r = rtz ? std::floor(q) : std::nearbyint(q);
+  return std::copysign(r * ulp, v);
 }
 
 struct Net {
@@ -68,16 +86,46 @@ void forward(const Net& n, const float* row, Fwd& f, bool with_target) {
   f.y = n.fl(n.fl(y) + n.P[B3]);
 }
 
+// Prediction of the inference kernel's arithmetic (mode 2 above), standardised output.
+double forward_infer(const float* P, const float* N, const float* row) {
+  const float s = 2.0f * 1.44269504088896341f;
+  double x[NI], r1[NH], r2[NH];
+  for (int q = 0; q < NI; ++q) x[q] = f16((double)(((float)row[q] - N[q]) * (1.0f / N[ROW + q])), true);
+  for (int h = 0; h < NH; ++h) {
+    double z = (double)(s * P[B1 + h]);
+    for (int q = 0; q < NI; ++q) z += f16((double)(s * P[W1 + h * NI + q]), false) * x[q];
+    z = (double)(float)z;
+    r1[h] = f16((double)(float)(1.0 / (1.0 + std::exp2(z))), true);
+  }
+  for (int h = 0; h < NH; ++h) {
+    float sum = 0.0f;
+    for (int k = 0; k < NH; ++k) sum += P[W2 + h * NH + k];
+    double z = (double)(s * (P[B2 + h] + sum));
+    for (int k = 0; k < NH; ++k) z += f16((double)((-2.0f * s) * P[W2 + h * NH + k]), false) * r1[k];
+    z = (double)(float)z;
+    r2[h] = f16((double)(float)(1.0 / (1.0 + std::exp2(z))), true);
+  }
+  float sum = 0.0f;
+  for (int k = 0; k < NH; ++k) sum += P[W3 + k];
+  double y = (double)(P[B3] + sum);
+  for (int k = 0; k < NH; ++k) y += f16((double)(-2.0f * P[W3 + k]), false) * r2[k];
+  return (double)(float)y;
+}
+
 }  // namespace
 
 extern "C" {
 
-// Raw predictions (increments) for x[n][11].
+// Raw predictions (increments) for x[n][11]; bf16 is the mode (0, 1 or 2, see above).
 int orc_mlp_predict(const float* params, const float* norm, const float* x, int n, int bf16, double* out) {
   const Net net{params, norm, bf16 != 0};
   std::vector<float> row(ROW, 0.0f);
   for (int r = 0; r < n; ++r) {
     for (int q = 0; q < NI; ++q) row[q] = x[(size_t)r * NI + q];
+    if (bf16 == 2) {
+      out[r] = (double)((float)forward_infer(params, norm, row.data()) * norm[ROW + NI] + norm[NI]);
+      continue;
+    }
     Fwd f;
     forward(net, row.data(), f, false);
     out[r] = net.b16 ? (double)((float)f.y * norm[ROW + NI] + norm[NI]) : f.y * norm[ROW + NI] + norm[NI];

@@ -212,7 +212,8 @@ def _f32(a):
 
 
 def mlp_predict(params, norm, x, bf16):
-    """CPU reference forward pass of the learned-guess MLP (raw increments)."""
+    """CPU reference forward pass of the learned-guess MLP (raw increments). bf16: 0 FP64, 1 the
+    training kernels' BF16 arithmetic, 2 the inference kernel's FP16 arithmetic."""
     params, norm, x = _f32(params), _f32(norm), _f32(x)
     out = np.zeros(len(x))
     fp = C.POINTER(C.c_float)

@@ -111,7 +111,8 @@ def test_untrained_model_rejected_or_harmless():
 
 def test_guess_kernel_matches_oracle_on_solver_state():
     """The inference kernel's features (h, six neighbour differences, ln K, C, theta, depth) and its
-    transposed-layer network against host-side features fed to the CPU reference."""
+    transposed-layer FP16 network against host-side features fed to the CPU reference, in the
+    reference's emulation of the kernel's arithmetic and in FP64."""
     nx, ny, nz = 70, 9, 6
     ctx = gpu.Context(0)
     p, soil, ks, h0 = infiltration(nx, ny, nz, 31, abi.GUESS_ZERO)
@@ -137,6 +138,9 @@ def test_guess_kernel_matches_oracle_on_solver_state():
     X[:, 7] = np.log(np.maximum(ks * vg["K"] / soil.Ks_z, 1e-30))
     X[:, 8] = vg["C"]
     X[:, 9] = wc[1:-1, 1:-1, 1:-1].ravel()
-    ref = ol.mlp_predict(params, norm, X.astype(np.float32), bf16=True)
-    scale = np.abs(ref).max()
-    assert np.abs(inc - ref).max() <= 3e-3 * scale, np.abs(inc - ref).max() / scale
+    # the kernel's arithmetic (FP16 operands, folded tanh), and the FP64 network
+    ref = ol.mlp_predict(params, norm, X.astype(np.float32), bf16=2)
+    ref64 = ol.mlp_predict(params, norm, X.astype(np.float32), bf16=0)
+    scale = np.abs(ref64).max()
+    assert np.abs(inc - ref).max() <= 1e-3 * scale, np.abs(inc - ref).max() / scale
+    assert np.abs(inc - ref64).max() <= 3e-3 * scale, np.abs(inc - ref64).max() / scale
