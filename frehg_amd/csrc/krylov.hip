@@ -99,7 +99,7 @@ __device__ inline double apply_in_domain(const Geo& g, const StencilView<T>& A, 
 // r = b - A x (or b when x0 = 0), z = r / D, p_prev = 0; partials r.z, z.z, (bn/D)^2.
 template <typename T, bool THREE_D>
 __global__ __launch_bounds__(NT) void k_init(Geo g, int kchunk, StencilView<T> A, const T* b,
-                                              const T* bn, T* x, T* r, T* z, T* pprev,
+                                              const T* bn, T* x, T* r, T* z,
                                               const unsigned char* fixed, int x0_zero,
                                               double* part, unsigned int nblocks) {
   __shared__ double lds[NT / kWave];
@@ -116,13 +116,13 @@ __global__ __launch_bounds__(NT) void k_init(Geo g, int kchunk, StencilView<T> A
       const T zT = (T)((double)rT / (double)A.D[c]);
       r[c] = rT;
       z[c] = zT;
-      pprev[c] = (T)0;
       srz += (double)rT * (double)zT;
       szz += (double)zT * (double)zT;
       const double bz = (double)(bn ? bn[c] : b[c]) / (double)A.D[c];
       sbb += bz * bz;
-      // x0 = 0 (only when x0_zero, in which case x is not read at all)
-      if (x0_zero && !fx) x[c] = (T)0;
+      // x0 = 0 (only when x0_zero, in which case x is not read at all). Without fixed rows x is
+      // left as it is: the first update writes x = alpha p (x_fresh).
+      if (x0_zero && fixed != nullptr && !fx) x[c] = (T)0;
     }
   }
   const double a = block_sum<NT>(srz, lds);
@@ -423,12 +423,14 @@ __global__ __launch_bounds__(NT) void k_spmv_pair(Geo g, int kchunk, StencilView
 // CG update of iteration `it` on 128 x 4 tiles of cell pairs (paired aligned loads), kchunk
 // layers per block (2D: one): x += alpha p, r -= alpha q, z = r / D, partials r.z and z.z to
 // part_u[0..nb) and part_u[nb..2nb). A pair straddling the row end updates its first cell only.
+// xfresh: x is taken as zero and not read (first update of a solve from x0 = 0).
 template <typename T, bool THREE_D>
 __global__ __launch_bounds__(NT) void k_cg_update(Geo g, int kchunk, const T* __restrict__ D,
                                                    const T* __restrict__ p, const T* __restrict__ q,
                                                    T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
                                                    double* __restrict__ part_u, KrylovCtl* ctl,
-                                                   const double* __restrict__ part_s, unsigned int nb_s, int it) {
+                                                   const double* __restrict__ part_s, unsigned int nb_s, int it,
+                                                   int xfresh) {
   __shared__ double lds[NT / kWave];
   __shared__ int sflag;
   const TileId b = xcd_tile();
@@ -453,8 +455,8 @@ __global__ __launch_bounds__(NT) void k_cg_update(Geo g, int kchunk, const T* __
     if (i + 1 < g.nx) {
 #pragma unroll 2
       for (int k = k0; k < k1; ++k, c += g.sz) {
-        const Pair<T> xv = ld_pair(x, c), pv = ld_pair(p, c), rv = ld_pair(r, c), qv = ld_pair(q, c),
-                      dv = ld_pair(D, c);
+        const Pair<T> xv = xfresh ? Pair<T>{(T)0, (T)0} : ld_pair(x, c);
+        const Pair<T> pv = ld_pair(p, c), rv = ld_pair(r, c), qv = ld_pair(q, c), dv = ld_pair(D, c);
         Pair<T> xo, ro, zo;
         cell(xv.a, pv.a, rv.a, qv.a, dv.a, xo.a, ro.a, zo.a);
         cell(xv.b, pv.b, rv.b, qv.b, dv.b, xo.b, ro.b, zo.b);
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(NT) void k_cg_update(Geo g, int kchunk, const T* __
         *reinterpret_cast<Pair<T>*>(z + c) = zo;
       }
     } else {
-      for (int k = k0; k < k1; ++k, c += g.sz) cell(x[c], p[c], r[c], q[c], D[c], x[c], r[c], z[c]);
+      for (int k = k0; k < k1; ++k, c += g.sz) cell(xfresh ? (T)0 : x[c], p[c], r[c], q[c], D[c], x[c], r[c], z[c]);
     }
   }
   const double a = block_sum<NT>(srz, lds);
@@ -572,22 +574,36 @@ __device__ __forceinline__ void refine_pair_load(const Geo& g, const StencilView
 // shifts, z-neighbours from registers). Per cell the arithmetic is that of k_refine. On the first
 // sweep (no d) xout receives a copy of xin, so that the sweep after the first inner solve can write
 // its iterate back into xin's buffer.
+//
+// With z32 non-null the sweep also starts the inner FP32 CG solve (k_init's work for b = r32,
+// x0 = 0): z32 = r32 / D32 and the partials r.z, z.z and ||D32^-1 r32||^2 to part[2nb..5nb).
 template <bool HAS_D>
 __global__ __launch_bounds__(NT) void k_refine_pair(Geo g, int kchunk, StencilView<double> A,
                                                      const double* __restrict__ b, const double* __restrict__ xin,
                                                      const float* __restrict__ d, double* __restrict__ xout,
-                                                     float* __restrict__ r32, double* part, int first) {
+                                                     float* __restrict__ r32, const float* __restrict__ D32,
+                                                     float* __restrict__ z32, double* part, int first) {
   __shared__ double lds[NT / kWave];
   const TileId tb = xcd_tile();
   const int lane = threadIdx.x & 63;
   const int i0 = tb.x * 128, i = i0 + 2 * lane, j = tb.y * TY + (threadIdx.x >> 6);
   const int k0 = tb.z * kchunk, k1 = min(g.nz, k0 + kchunk);
-  double srr = 0.0, sbb = 0.0;
+  double srr = 0.0, sbb = 0.0, irz = 0.0, izz = 0.0, ibb = 0.0;
+  const bool inner = z32 != nullptr;
   if (j < g.ny) {  // whole wave
     const bool va = i < g.nx, vb = i + 1 < g.nx;
     long long c = g.idx(min(i, g.nx & ~1), j, k0);
     const bool edge = lane == 0 || lane == 63;
     const long long dce = (lane == 0 ? g.idx(i0 - 1, j, k0) : g.idx(min(i0 + 128, g.nx), j, k0)) - c;
+    // k_init's arithmetic for one cell of the inner solve (b = r32, x0 = 0)
+    auto start = [&](float rT, float dd, float& zo) {
+      const float zT = (float)((double)rT / (double)dd);
+      zo = zT;
+      irz += (double)rT * (double)zT;
+      izz += (double)zT * (double)zT;
+      const double bz = (double)rT / (double)dd;
+      ibb += bz * bz;
+    };
     auto add = [](Pair<double> x, Pair<float> e) -> Pair<double> {
       return HAS_D ? Pair<double>{x.a + (double)e.a, x.b + (double)e.b} : x;
     };
@@ -620,19 +636,28 @@ __global__ __launch_bounds__(NT) void k_refine_pair(Geo g, int kchunk, StencilVi
       const double za = ra / cur.d.a, zb = rb / cur.d.b;
       const double ba = cur.b.a / cur.d.a, bbz = cur.b.b / cur.d.b;
       if (vb) {
+        const Pair<float> rT{(float)ra, (float)rb};
         *reinterpret_cast<Pair<double>*>(xout + c) = Pair<double>{xa, xb};
-        *reinterpret_cast<Pair<float>*>(r32 + c) = Pair<float>{(float)ra, (float)rb};
+        *reinterpret_cast<Pair<float>*>(r32 + c) = rT;
         srr += za * za;
         srr += zb * zb;
         if (first) {
           sbb += ba * ba;
           sbb += bbz * bbz;
         }
+        if (inner) {
+          const Pair<float> dd = ld_pair(D32, c);
+          Pair<float> zo;
+          start(rT.a, dd.a, zo.a);
+          start(rT.b, dd.b, zo.b);
+          *reinterpret_cast<Pair<float>*>(z32 + c) = zo;
+        }
       } else if (va) {
         xout[c] = xa;
         r32[c] = (float)ra;
         srr += za * za;
         if (first) sbb += ba * ba;
+        if (inner) start((float)ra, D32[c], z32[c]);
       }
       xma = xa;
       xmb = xb;
@@ -646,18 +671,38 @@ __global__ __launch_bounds__(NT) void k_refine_pair(Geo g, int kchunk, StencilVi
   }
   const double a = block_sum<NT>(srr, lds);
   const double bs = block_sum<NT>(sbb, lds);
-  const unsigned int bid = block_id();
+  const unsigned int bid = block_id(), nbl = gridDim.x * gridDim.y * gridDim.z;
   if (threadIdx.x == 0) {
     part[bid] = a;
-    part[gridDim.x * gridDim.y * gridDim.z + bid] = bs;
+    part[nbl + bid] = bs;
+  }
+  if (inner) {
+    const double x1 = block_sum<NT>(irz, lds);
+    const double x2 = block_sum<NT>(izz, lds);
+    const double x3 = block_sum<NT>(ibb, lds);
+    if (threadIdx.x == 0) {
+      part[2 * nbl + bid] = x1;
+      part[3 * nbl + bid] = x2;
+      part[4 * nbl + bid] = x3;
+    }
   }
 }
 
+// Outer test of a refinement sweep; with `inner` non-null also k_fin_init's start of the inner CG
+// solve from the sweep's partials, its tolerance relative to ||D32^-1 r32|| being the one the
+// refinement loop asks for: min(0.5, max(inner_rtol, gap / 2)), gap = outer target / residual.
 __global__ __launch_bounds__(kRedThreads) void k_fin_refine(const double* part, unsigned int nblocks, KrylovCtl* ctl,
-                                                            int first, double rtol, double atol, int guard) {
+                                                            int first, double rtol, double atol, int guard,
+                                                            KrylovCtl* inner, double inner_rtol, int inner_max_it) {
   __shared__ double lds[kRedThreads / kWave];
   const double rr = block_sum_partials<kRedThreads>(part, nblocks, lds);
   const double bb = first ? block_sum_partials<kRedThreads>(part + nblocks, nblocks, lds) : 0.0;
+  double irz = 0.0, izz = 0.0, ibb = 0.0;
+  if (inner) {
+    irz = block_sum_partials<kRedThreads>(part + 2 * nblocks, nblocks, lds);
+    izz = block_sum_partials<kRedThreads>(part + 3 * nblocks, nblocks, lds);
+    ibb = block_sum_partials<kRedThreads>(part + 4 * nblocks, nblocks, lds);
+  }
   if (threadIdx.x == 0) {
     if (first) {
       const double ttol = fmax(rtol * sqrt(bb), atol);
@@ -669,6 +714,26 @@ __global__ __launch_bounds__(kRedThreads) void k_fin_refine(const double* part, 
     ctl->zz = rr;
     ctl->converged = (rr <= ctl->ttol2) ? 1 : 0;
     ctl->done = ctl->converged;
+    if (inner) {
+      const double gap = sqrt(ctl->ttol2) / fmax(sqrt(rr), 1e-300);
+      const double irt = fmin(0.5, fmax(inner_rtol, 0.5 * gap));
+      const double ttol = fmax(irt * sqrt(ibb), 0.0);
+      inner->rz = irz;
+      inner->rzb[0] = irz;
+      inner->rzb[1] = irz;
+      inner->zz = izz;
+      inner->bb = ibb;
+      inner->ttol2 = ttol * ttol;
+      inner->iters = 0;
+      inner->beta = 0.0;
+      inner->alpha = 0.0;
+      inner->breakdown = 0;
+      inner->max_it = inner_max_it;
+      inner->rr_guess = izz;
+      inner->guess_rejected = 0;
+      inner->converged = (izz <= inner->ttol2) ? 1 : 0;
+      inner->done = inner->converged;
+    }
   }
 }
 
@@ -906,14 +971,14 @@ __global__ __launch_bounds__(kRedThreads) void k_fin_bcg3(const double* part, un
 }
 
 // x = 0 on non-fixed interior rows (rejected initial guess).
-template <bool THREE_D>
-__global__ __launch_bounds__(NT) void k_zero_interior(Geo g, int kchunk, double* x,
+template <bool THREE_D, typename T>
+__global__ __launch_bounds__(NT) void k_zero_interior(Geo g, int kchunk, T* x,
                                                        const unsigned char* fixed) {
   const Tile t = tile_of(g, kchunk);
   if (!t.valid) return;
   long long c = base_index<THREE_D>(g, t);
   for (int k = t.k0; k < t.k1; ++k, c += g.sz)
-    if (!(fixed != nullptr && fixed[c] != 0)) x[c] = 0.0;
+    if (!(fixed != nullptr && fixed[c] != 0)) x[c] = (T)0;
 }
 
 }  // namespace
@@ -927,7 +992,7 @@ Krylov::Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof)
   nb_f_ = grid_f_.x * grid_f_.y * grid_f_.z;
   // partials sized for the largest grid any kernel (or candidate) uses: 64 x 4 tiles, one layer each
   part_cap_ = ceil_div(g_.nx, TX) * ceil_div(g_.ny, TY) * (unsigned int)(three_d_ ? g_.nz : 1);
-  FREHG_HIP(hipMalloc(&part_, 3 * (size_t)part_cap_ * sizeof(double)));
+  FREHG_HIP(hipMalloc(&part_, 5 * (size_t)part_cap_ * sizeof(double)));  // up to 5 sums per block
   FREHG_HIP(hipMalloc(&part_u_, 2 * (size_t)part_cap_ * sizeof(double)));
   // CG update: whole columns per block in 3D (its partials are reduced by every SpMV block, so
   // their number is kept small)
@@ -1079,14 +1144,22 @@ void Krylov::cg_start(const StencilView<T>& A, const T* b, const T* bn, T* x, co
   const dim3 blk(NT);
   const unsigned int nb = nb_f_;
   if (three_d_)
-    hipLaunchKernelGGL((k_init<T, true>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, z, p0, fixed, zero ? 1 : 0,
+    hipLaunchKernelGGL((k_init<T, true>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, z, fixed, zero ? 1 : 0,
                        part_, nb);
   else
-    hipLaunchKernelGGL((k_init<T, false>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, z, p0, fixed,
-                       zero ? 1 : 0, part_, nb);
+    hipLaunchKernelGGL((k_init<T, false>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, z, fixed, zero ? 1 : 0,
+                       part_, nb);
   FREHG_LAUNCH_CHECK();
+  (void)p0;
   hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(kRedThreads), 0, s_, part_, nb, ctl, rtol, atol, max_it,
                      guard ? 1 : 0, zero ? 1 : 0);
+  FREHG_LAUNCH_CHECK();
+}
+
+template <typename T>
+void Krylov::zero_interior(T* x, const unsigned char* fixed) {
+  if (three_d_) hipLaunchKernelGGL((k_zero_interior<true, T>), grid_f_, dim3(NT), 0, s_, g_, kc_f_, x, fixed);
+  else hipLaunchKernelGGL((k_zero_interior<false, T>), grid_f_, dim3(NT), 0, s_, g_, kc_f_, x, fixed);
   FREHG_LAUNCH_CHECK();
 }
 
@@ -1094,11 +1167,12 @@ void Krylov::cg_start(const StencilView<T>& A, const T* b, const T* bn, T* x, co
 // the last update (k_cg_check), all without host synchronisation.
 template <typename T>
 void Krylov::cg_iterations(const StencilView<T>& A, T* x, T* r, T* z, T* p0, T* p1, T* q, KrylovCtl* ctl, int from,
-                           int count, int kt_spmv) {
+                           int count, int kt_spmv, bool x_fresh) {
   const dim3 blk(NT);
   T* pbuf[2] = {p0, p1};
   for (int it = from; it < from + count; ++it) {
-    T* pold = pbuf[it & 1];
+    // iteration 0 has beta = 0: p = z + 0 p_prev, with z itself as p_prev (no cleared buffer read)
+    T* pold = it == 0 ? z : pbuf[it & 1];
     T* pnew = pbuf[(it & 1) ^ 1];
     // Only the 3D (groundwater) launches are timed: the 2D surface solve is launch-bound, and
     // launches with timing events take the host several microseconds longer to issue.
@@ -1111,12 +1185,13 @@ void Krylov::cg_iterations(const StencilView<T>& A, T* x, T* r, T* z, T* p0, T* 
       hipEvent_t ev0 = three_d_ ? prof_->start(FREHG_KT_GW_UPDATE) : nullptr;
       hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
       const unsigned int nbs = shape_[sizeof(T) == 4 ? 1 : 0].nb;
+      const int xf = (it == 0 && x_fresh) ? 1 : 0;
       if (three_d_)
         hipExtLaunchKernelGGL((k_cg_update<T, true>), grid_u_, blk, 0, s_, ev0, ev1, 0, g_, kc_u_, A.D, (const T*)pnew,
-                              (const T*)q, x, r, z, part_u_, ctl, (const double*)part_, nbs, it);
+                              (const T*)q, x, r, z, part_u_, ctl, (const double*)part_, nbs, it, xf);
       else
         hipExtLaunchKernelGGL((k_cg_update<T, false>), grid_u_, blk, 0, s_, ev0, ev1, 0, g_, kc_u_, A.D,
-                              (const T*)pnew, (const T*)q, x, r, z, part_u_, ctl, (const double*)part_, nbs, it);
+                              (const T*)pnew, (const T*)q, x, r, z, part_u_, ctl, (const double*)part_, nbs, it, xf);
       FREHG_LAUNCH_CHECK();
     }
   }
@@ -1128,15 +1203,18 @@ void Krylov::cg_iterations(const StencilView<T>& A, T* x, T* r, T* z, T* p0, T* 
 // has ended; then the outcome (throws on breakdown).
 template <typename T>
 int Krylov::cg_finish(const StencilView<T>& A, T* x, T* r, T* z, T* p0, T* p1, T* q, KrylovCtl* ctl, int max_it,
-                      int launched, int batch, double* res_norm, double* rhs_norm, bool* converged, int kt_spmv) {
+                      int launched, int batch, double* res_norm, double* rhs_norm, bool* converged, int kt_spmv,
+                      bool x_fresh) {
   for (;;) {
     read_ctl(ctl, host_ctl_);
     if (host_ctl_->done) break;
     if (launched >= max_it + 2) break;
     batch = std::max(2, std::min(16, batch / 2));
-    cg_iterations<T>(A, x, r, z, p0, p1, q, ctl, launched, batch, kt_spmv);
+    cg_iterations<T>(A, x, r, z, p0, p1, q, ctl, launched, batch, kt_spmv, x_fresh);
     launched += batch;
   }
+  // a solve from an uncleared x0 = 0 that ended before its first update
+  if (x_fresh && host_ctl_->iters == 0) zero_interior<T>(x, nullptr);
   if (host_ctl_->breakdown)
     throw ConvergenceError(std::string("Krylov ") + (three_d_ ? "3D" : "2D") + (sizeof(T) == 4 ? " fp32" : " fp64") +
                            ": breakdown (p.Ap = " + std::to_string(host_ctl_->pq) + " at iteration " +
@@ -1163,17 +1241,21 @@ int Krylov::run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, cons
                     double* rhs_norm, bool* converged, int kt_spmv) {
   cg_start<T>(A, b, bn, x, fixed, x0_zero, guard, rtol, atol, max_it, ctl, r, z, p0);
   *rejected = false;
+  bool zero = x0_zero;
   if (guard && !x0_zero) {
     read_ctl(ctl, host_ctl_);
     if (host_ctl_->guess_rejected) {
       *rejected = true;
+      zero = true;
       cg_start<T>(A, b, bn, x, fixed, true, guard, rtol, atol, max_it, ctl, r, z, p0);
     }
   }
+  const bool fresh = zero && fixed == nullptr;
   const int est = (sizeof(T) == 8) ? last_iters_ : last_inner_iters_;
   const int batch = std::max(2, est + 1);
-  cg_iterations<T>(A, x, r, z, p0, p1, q, ctl, 0, batch, kt_spmv);
-  return cg_finish<T>(A, x, r, z, p0, p1, q, ctl, max_it, batch, batch, res_norm, rhs_norm, converged, kt_spmv);
+  cg_iterations<T>(A, x, r, z, p0, p1, q, ctl, 0, batch, kt_spmv, fresh);
+  return cg_finish<T>(A, x, r, z, p0, p1, q, ctl, max_it, batch, batch, res_norm, rhs_norm, converged, kt_spmv,
+                      fresh);
 }
 
 void Krylov::enqueue_cg(const StencilView<double>& A, const double* b, const double* bn, double* x,
@@ -1181,7 +1263,9 @@ void Krylov::enqueue_cg(const StencilView<double>& A, const double* b, const dou
   if (cfg.ksp != FREHG_KSP_CG || cfg.precision != 0) throw ArgError("Krylov::enqueue_cg: FP64 CG only");
   if (three_d_ && !tuned_[0]) throw ArgError("Krylov::enqueue_cg: 3D SpMV shape not tuned yet");
   cg_start<double>(A, b, bn, x, fixed, x0_zero, false, cfg.rtol, cfg.atol, cfg.max_it, ctl_, r_, z_, p0_);
-  cg_iterations<double>(A, x, r_, z_, p0_, p1_, q_, ctl_, 0, budget, three_d_ ? FREHG_KT_GW_SPMV : FREHG_KT_SW_SPMV);
+  enq_fresh_ = x0_zero && fixed == nullptr;
+  cg_iterations<double>(A, x, r_, z_, p0_, p1_, q_, ctl_, 0, budget, three_d_ ? FREHG_KT_GW_SPMV : FREHG_KT_SW_SPMV,
+                        enq_fresh_);
 }
 
 SolveResult Krylov::finish_cg(const StencilView<double>& A, double* x, const KrylovConfig& cfg, int budget) {
@@ -1189,7 +1273,7 @@ SolveResult Krylov::finish_cg(const StencilView<double>& A, double* x, const Kry
   bool conv = false;
   res.iterations = cg_finish<double>(A, x, r_, z_, p0_, p1_, q_, ctl_, cfg.max_it, budget, budget,
                                      &res.residual_norm, &res.rhs_norm, &conv,
-                                     three_d_ ? FREHG_KT_GW_SPMV : FREHG_KT_SW_SPMV);
+                                     three_d_ ? FREHG_KT_GW_SPMV : FREHG_KT_SW_SPMV, enq_fresh_);
   res.converged = conv;
   if (!conv)
     throw ConvergenceError(std::string("Krylov ") + (three_d_ ? "3D" : "2D") + ": not converged within max_it (" +
@@ -1348,29 +1432,35 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
   KrylovCtl* outer = ctl_;
   KrylovCtl* inner = ctl_ + 1;
   if (x0_zero) {
-    if (three_d_) hipLaunchKernelGGL((k_zero_interior<true>), grid_f_, blk, 0, s_, g_, kc_f_, x, fixed);
-    else hipLaunchKernelGGL((k_zero_interior<false>), grid_f_, blk, 0, s_, g_, kc_f_, x, fixed);
-    FREHG_LAUNCH_CHECK();
+    zero_interior<double>(x, fixed);
   }
   // The iterate alternates between x and xw_: the first sweep copies x to xw_, and each later
   // sweep reads one and writes the other, so that after one refinement it is back in x. xw_'s
   // ghost entries are never written: they stay finite (zero or earlier heads) and meet zero
   // coefficients only.
+  // Paired sweep (3D, symmetric, no fixed rows, bn = b); with CG as the inner solver it also
+  // starts the inner solve (residual straight into r32_, z32_, inner control block), so the
+  // inner CG begins with its first iteration.
+  const bool paired = three_d_ && A.symmetric() && fixed == nullptr && bn == nullptr;
+  const bool fused = paired && cfg.ksp == FREHG_KSP_CG;
   auto refine = [&](const double* xin, const float* d, double* xout, bool first) {
     const bool sym = A.symmetric();
     hipEvent_t ev0 = three_d_ ? prof_->start(FREHG_KT_GW_REFINE) : nullptr;
     hipEvent_t ev1 = three_d_ ? prof_->stop() : nullptr;
     unsigned int nbr = nb;
-    if (three_d_ && sym && fixed == nullptr && bn == nullptr) {
+    if (paired) {
       const int kc = shape_[0].kc;
       const dim3 grid(ceil_div(g_.nx, 2 * TX), ceil_div(g_.ny, TY), ceil_div(g_.nz, kc));
       nbr = grid.x * grid.y * grid.z;
+      float* r32 = fused ? r32_ : b32_;
+      const float* D32 = fused ? A32->D : nullptr;
+      float* z32 = fused ? z32_ : nullptr;
       if (d)
-        hipExtLaunchKernelGGL((k_refine_pair<true>), grid, blk, 0, s_, ev0, ev1, 0, g_, kc, A, b, xin, d, xout, b32_,
-                              part_, first ? 1 : 0);
+        hipExtLaunchKernelGGL((k_refine_pair<true>), grid, blk, 0, s_, ev0, ev1, 0, g_, kc, A, b, xin, d, xout, r32,
+                              D32, z32, part_, first ? 1 : 0);
       else
         hipExtLaunchKernelGGL((k_refine_pair<false>), grid, blk, 0, s_, ev0, ev1, 0, g_, kc, A, b, xin, d, xout,
-                              b32_, part_, first ? 1 : 0);
+                              r32, D32, z32, part_, first ? 1 : 0);
     } else if (three_d_ && sym)
       hipExtLaunchKernelGGL((k_refine<true, true>), grid_f_, blk, 0, s_, ev0, ev1, 0, g_, kc_f_, A, b, bn, xin, d,
                             xout, b32_, fixed, part_, nb, first ? 1 : 0);
@@ -1385,7 +1475,8 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
                          fixed, part_, nb, first ? 1 : 0);
     FREHG_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_fin_refine, dim3(1), dim3(kRedThreads), 0, s_, part_, nbr, outer, first ? 1 : 0, cfg.rtol,
-                       cfg.atol, (guard_guess && !x0_zero) ? 1 : 0);
+                       cfg.atol, (guard_guess && !x0_zero) ? 1 : 0, fused ? inner : nullptr, cfg.inner_rtol,
+                       cfg.max_it);
     FREHG_LAUNCH_CHECK();
   };
   double* xcur = xw_;
@@ -1394,9 +1485,7 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
   read_ctl(outer, host_ctl_);
   if (host_ctl_->guess_rejected) {
     res.guess_rejected = true;
-    if (three_d_) hipLaunchKernelGGL((k_zero_interior<true>), grid_f_, blk, 0, s_, g_, kc_f_, x, fixed);
-    else hipLaunchKernelGGL((k_zero_interior<false>), grid_f_, blk, 0, s_, g_, kc_f_, x, fixed);
-    FREHG_LAUNCH_CHECK();
+    zero_interior<double>(x, fixed);
     refine(x, nullptr, xw_, true);
     read_ctl(outer, host_ctl_);
   }
@@ -1412,9 +1501,17 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
     // a good initial guess leaves a small gap and needs fewer inner iterations.
     const double gap = std::sqrt(host_ctl_->ttol2) / std::max(res.residual_norm, 1e-300);
     const double inner_rt = std::min(0.5, std::max(cfg.inner_rtol, 0.5 * gap));
-    res.iterations += run<float>(cfg.ksp, A32v, b32_, nullptr, d32_, fixed, true, false, inner_rt, 0.0,
-                                 cfg.max_it, inner, &rej, &rn, &bnrm, &iconv,
-                                 three_d_ ? FREHG_KT_GW_SPMV32 : FREHG_KT_SW_SPMV);
+    if (fused) {
+      // started by the sweep (same tolerance, computed on the device)
+      const int batch = std::max(2, last_inner_iters_ + 1);
+      cg_iterations<float>(A32v, d32_, r32_, z32_, p032_, p132_, q32_, inner, 0, batch, FREHG_KT_GW_SPMV32, true);
+      res.iterations += cg_finish<float>(A32v, d32_, r32_, z32_, p032_, p132_, q32_, inner, cfg.max_it, batch, batch,
+                                         &rn, &bnrm, &iconv, FREHG_KT_GW_SPMV32, true);
+    } else {
+      res.iterations += run<float>(cfg.ksp, A32v, b32_, nullptr, d32_, fixed, true, false, inner_rt, 0.0,
+                                   cfg.max_it, inner, &rej, &rn, &bnrm, &iconv,
+                                   three_d_ ? FREHG_KT_GW_SPMV32 : FREHG_KT_SW_SPMV);
+    }
     refine(xcur, d32_, xalt, false);
     std::swap(xcur, xalt);
     res.refinements += 1;

@@ -126,12 +126,17 @@ class Krylov {
   template <typename T>
   void cg_start(const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed, bool zero,
                 bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl, T* r, T* z, T* p0);
+  // x_fresh: x0 = 0 without x having been cleared (the first update writes x = alpha p; a solve
+  // that ends before its first update clears x in cg_finish). Only without fixed rows.
   template <typename T>
   void cg_iterations(const StencilView<T>& A, T* x, T* r, T* z, T* p0, T* p1, T* q, KrylovCtl* ctl, int from,
-                     int count, int kt_spmv);
+                     int count, int kt_spmv, bool x_fresh = false);
   template <typename T>
   int cg_finish(const StencilView<T>& A, T* x, T* r, T* z, T* p0, T* p1, T* q, KrylovCtl* ctl, int max_it,
-                int launched, int batch, double* res_norm, double* rhs_norm, bool* converged, int kt_spmv);
+                int launched, int batch, double* res_norm, double* rhs_norm, bool* converged, int kt_spmv,
+                bool x_fresh = false);
+  template <typename T>
+  void zero_interior(T* x, const unsigned char* fixed);
   template <typename T>
   void launch_spmv(const StencilView<T>& A, const T* z, const T* pold, T* pnew, T* q, KrylovCtl* ctl, int it,
                    hipEvent_t ev0, hipEvent_t ev1);
@@ -155,6 +160,7 @@ class Krylov {
   int kc_f_ = 1;
   dim3 grid_f_;
   unsigned int nb_f_ = 0, part_cap_ = 0;
+  bool enq_fresh_ = false;  // enqueue_cg started from an uncleared x0 = 0
   // CG update: 128 x 4 tiles of cell pairs, kc_u_ layers per block; its partials (r.z, z.z)
   int kc_u_ = 1;
   dim3 grid_u_;
