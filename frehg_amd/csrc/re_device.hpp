@@ -159,12 +159,17 @@ __device__ inline double face_z(const RK& P, const double* __restrict__ kr, long
 
 // K/Ks and C at the interior cell and kr at its face-adjacent ghost cells (whose soil is the
 // cell's own: ghost indices are clamped). ReSolver::updateDiagnostics (ReSolver.cpp:199-215).
+// A ghost whose head equals the cell's (every no-flux face) takes the cell's kr, the same
+// function of the same operands; only a ghost with a head of its own (Dirichlet faces, or a
+// boundary cell whose head the reallocation replaced) is evaluated, so that a wave holding one
+// boundary cell no longer runs the constitutive law again for the whole wave.
 template <class RK>
 __device__ inline void constitutive_with_ghosts(const RK& P, const Cell& t, const double* h,
                                                  double* kr, double* ch, double* wch) {
   const Geo& g = P.g;
   const SoilClass& s = P.soil.at(t.c);
-  const VGState v = vg_state(s, h[t.c]);
+  const double hc = h[t.c];
+  const VGState v = vg_state(s, hc);
   kr[t.c] = v.kr;
   ch[t.c] = v.cap;
   if (wch) wch[t.c] = v.theta;
@@ -172,8 +177,11 @@ __device__ inline void constitutive_with_ghosts(const RK& P, const Cell& t, cons
                            t.j == 0 ? t.c - g.sy : -1,      t.j == g.ny - 1 ? t.c + g.sy : -1,
                            t.k == 0 ? t.c - g.sz : -1,      t.k == g.nz - 1 ? t.c + g.sz : -1};
 #pragma unroll
-  for (int q = 0; q < 6; ++q)
-    if (gh[q] >= 0) kr[gh[q]] = vg_state(s, h[gh[q]]).kr;
+  for (int q = 0; q < 6; ++q) {
+    if (gh[q] < 0) continue;
+    const double hg = h[gh[q]];
+    kr[gh[q]] = (hg == hc) ? v.kr : vg_state(s, hg).kr;
+  }
 }
 
 template <class RK>
