@@ -316,6 +316,55 @@ __device__ __forceinline__ Pair<T> ld_pair(const T* __restrict__ v, long long c)
   return *reinterpret_cast<const Pair<T>*>(v + c);
 }
 
+// k_init for x0 = 0 on a 3D operator without fixed rows (r = b, z = b / D, the same partials),
+// on 128 x 4 tiles of cell pairs with kchunk layers per block: an elementwise pass with aligned
+// paired loads and few partials. A pair straddling the row end handles its first cell only.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_init_zero_pair(Geo g, int kchunk, const T* __restrict__ D,
+                                                        const T* __restrict__ b, const T* __restrict__ bn,
+                                                        T* __restrict__ r, T* __restrict__ z, double* part) {
+  __shared__ double lds[NT / kWave];
+  const TileId tb = xcd_tile();
+  const int i = tb.x * 128 + 2 * (threadIdx.x & 63), j = tb.y * TY + (threadIdx.x >> 6);
+  const int k0 = tb.z * kchunk, k1 = min(g.nz, k0 + kchunk);
+  double srz = 0.0, szz = 0.0, sbb = 0.0;
+  auto cell = [&](T bc, T bnc, T dc, T& ro, T& zo) {
+    const T rT = (T)(double)bc;
+    const T zT = (T)((double)rT / (double)dc);
+    ro = rT;
+    zo = zT;
+    srz += (double)rT * (double)zT;
+    szz += (double)zT * (double)zT;
+    const double bz = (double)bnc / (double)dc;
+    sbb += bz * bz;
+  };
+  if (i < g.nx && j < g.ny) {
+    long long c = g.idx(i, j, k0);
+    if (i + 1 < g.nx) {
+      for (int k = k0; k < k1; ++k, c += g.sz) {
+        const Pair<T> bv = ld_pair(b, c), dv = ld_pair(D, c);
+        const Pair<T> bnv = bn ? ld_pair(bn, c) : bv;
+        Pair<T> ro, zo;
+        cell(bv.a, bnv.a, dv.a, ro.a, zo.a);
+        cell(bv.b, bnv.b, dv.b, ro.b, zo.b);
+        *reinterpret_cast<Pair<T>*>(r + c) = ro;
+        *reinterpret_cast<Pair<T>*>(z + c) = zo;
+      }
+    } else {
+      for (int k = k0; k < k1; ++k, c += g.sz) cell(b[c], bn ? bn[c] : b[c], D[c], r[c], z[c]);
+    }
+  }
+  const double a = block_sum<NT>(srz, lds);
+  const double bs = block_sum<NT>(szz, lds);
+  const double cs = block_sum<NT>(sbb, lds);
+  if (threadIdx.x == 0) {
+    const unsigned int nbl = gridDim.x * gridDim.y * gridDim.z, bid = block_id();
+    part[bid] = a;
+    part[nbl + bid] = bs;
+    part[2 * nbl + bid] = cs;
+  }
+}
+
 // Operands of one layer of the paired z-march: pairs of z / p_prev at the y-neighbours and the
 // cell below, the pair's coefficients, and (edge lanes only) z, p_prev, ax at the x-edge cell.
 template <typename T>
@@ -1142,8 +1191,14 @@ template <typename T>
 void Krylov::cg_start(const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed, bool zero,
                       bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl, T* r, T* z, T* p0) {
   const dim3 blk(NT);
-  const unsigned int nb = nb_f_;
-  if (three_d_)
+  unsigned int nb = nb_f_;
+  if (three_d_ && zero && fixed == nullptr) {
+    // x0 = 0 without fixed rows: elementwise, on the SpMV's tiles and layers per block
+    const int kc = shape_[sizeof(T) == 4 ? 1 : 0].kc;
+    const dim3 grid(ceil_div(g_.nx, 2 * TX), ceil_div(g_.ny, TY), ceil_div(g_.nz, kc));
+    nb = grid.x * grid.y * grid.z;
+    hipLaunchKernelGGL((k_init_zero_pair<T>), grid, blk, 0, s_, g_, kc, A.D, b, bn, r, z, part_);
+  } else if (three_d_)
     hipLaunchKernelGGL((k_init<T, true>), grid_f_, blk, 0, s_, g_, kc_f_, A, b, bn, x, r, z, fixed, zero ? 1 : 0,
                        part_, nb);
   else
