@@ -205,6 +205,60 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_learned_eval(args):
+    """Held-out evaluation of the learned guess on the coupled 512x512 + 512x512x30 workload: train on
+    pairs logged from plain FP64 runs of 20 seeded bed/Ks realisations (seed r for the bed, r + 100
+    for Ks, r = 101..120; about 2e7 pairs), then run held-out realisations r = 201..205 with FP64
+    Jacobi-PCG at the same tolerance from x0 = 0, from x0 = h^n + MLP increment and from x0 = h^n,
+    reporting Krylov (and Picard) iterations per step and the head difference to the zero guess."""
+    from frehg_amd import abi, gpu, workloads
+    train_seeds = [(r, r + 100) for r in range(101, 121)]
+    eval_seeds = [(r, r + 100) for r in range(201, 206)]
+    ncell = args.nx * args.ny * args.nz
+    log_steps = 10
+    stride = max(1, (len(train_seeds) * log_steps * ncell) // 20_000_000)
+    log(f"[bench] training on {len(train_seeds)} realisations, {log_steps} logged steps each, cell stride {stride}")
+    model, training = train_model(abi, gpu, workloads, args, seeds=train_seeds, log_steps=log_steps, stride=stride,
+                                  train_steps=args.train_steps, batch=65536)
+    log(f"[bench]   {training['pairs']} pairs, loss {training['loss_first']} -> {training['loss_last']}")
+    per_seed = []
+    for sb, sk in eval_seeds:
+        case = workloads.CoupledCase(nx=args.nx, ny=args.ny, nz=args.nz, seed_bed=sb, seed_ks=sk)
+        row = {"seeds": [sb, sk]}
+        heads = {}
+        for name, guess in (("zero", abi.GUESS_ZERO), ("learned", abi.GUESS_LEARNED), ("previous", abi.GUESS_PREVIOUS)):
+            ctx = gpu.Context(0)
+            if guess == abi.GUESS_LEARNED:
+                ctx.mlp_set_params(*model)
+            case.setup_gpu(ctx, abi.PREC_FP64, guess)
+            its = pic = fb = 0
+            for _ in range(args.steps):
+                st = ctx.coupled_step(case.cp, case.dt, case.rain, 0.0, case.t_end)
+                its += st.gw.solve.iterations
+                pic += st.gw.picard_iterations
+                fb += st.gw.guess_fallback
+            heads[name] = ctx.re_get("h")
+            ctx.close()
+            row[name] = {"krylov_iters_per_step": round(its / args.steps, 3),
+                         "picard_iters_per_step": round(pic / args.steps, 3), "guess_fallbacks": fb}
+        for name in ("learned", "previous"):
+            row[name]["max_head_diff_vs_zero_guess_m"] = float(np.max(np.abs(heads[name] - heads["zero"])))
+        log(f"[bench]   held-out {sb}/{sk}: {row}")
+        per_seed.append(row)
+    mean = {name: round(float(np.mean([r[name]["krylov_iters_per_step"] for r in per_seed])), 3)
+            for name in ("zero", "learned", "previous")}
+    out = {"metric": "Krylov iterations per coupled step on held-out realisations, learned vs zero initial guess",
+           "value": mean["learned"], "unit": "iterations/step", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f16",
+           "data": "synthetic (seeded Gaussian random fields: bed sigma 0.05 m / 20 m, ln Ks sigma 1 / 10 m)",
+           "config": {"workload": f"coupled SW {args.nx}x{args.ny} + GW {args.nx}x{args.ny}x{args.nz}, dt 1 s, "
+                                  "FP64 Jacobi-PCG at rtol 1e-10 for every guess",
+                      "train_realisations": [list(x) for x in train_seeds],
+                      "held_out_realisations": [list(x) for x in eval_seeds]},
+           "mean_krylov_iters_per_step": mean, "per_realisation": per_seed, "training": training}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,8 +274,10 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run an elementwise 1 GiB -> 1 GiB kernel (torch), the reference point that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
-    ap.add_argument("--workload", choices=("coupled", "ensemble"), default="coupled",
-                    help="coupled: the 512x512 + 512x512x30 coupled step (default); ensemble: 96 uneven boxes")
+    ap.add_argument("--workload", choices=("coupled", "ensemble", "learned-eval"), default="coupled",
+                    help="coupled: the 512x512 + 512x512x30 coupled step (default); ensemble: 96 uneven boxes; "
+                         "learned-eval: the learned guess trained on 20 realisations and evaluated on 5 "
+                         "held-out ones (iterations per step against the zero guess)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); the multi-GPU "
@@ -229,6 +285,9 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble":
         bench_ensemble(args)
+        return
+    if args.workload == "learned-eval":
+        bench_learned_eval(args)
         return
 
     if args.copy_probe:
