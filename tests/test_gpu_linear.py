@@ -25,6 +25,23 @@ def test_manufactured(ndim, dims):
         assert np.abs(x - xo).max() < 1e-10
 
 
+@pytest.mark.parametrize("ndim,dims", [(2, (6, 5, 1)), (3, (70, 33, 9))])
+def test_zero_rhs_ends_before_the_first_update(ndim, dims):
+    """b = 0: the solve ends at its start (0 iterations) and x is exactly zero. In 3D without fixed
+    rows the start kernel does not clear x (the first update writes it), so this is the path where
+    the solver clears x after a solve that had no update."""
+    nx, ny, nz = dims
+    diag, offs, cols = laplacian(ndim, nx, ny, nz)
+    n = len(diag)
+    ctx = gpu.Context(0)
+    xs = 1.5 + np.sin(0.1 * np.arange(n))
+    x, st = ctx.ls_solve(ndim, nx, ny, nz, diag, offs, matvec(diag, offs, cols, xs), abi.solver_config(1e-13))
+    assert st.converged
+    x, st = ctx.ls_solve(ndim, nx, ny, nz, diag, offs, np.zeros(n), abi.solver_config(1e-13))
+    assert st.converged and st.iterations == 0
+    assert np.array_equal(x, np.zeros(n))
+
+
 def test_decoupled_rows_2d():
     """Rows with no off-diagonals (dry cells at rest) are solved exactly and folded out."""
     nx, ny = 7, 6
