@@ -96,7 +96,8 @@ __device__ inline double apply_in_domain(const Geo& g, const StencilView<T>& A, 
   return y;
 }
 
-// r = b - A x (or b when x0 = 0), z = r / D, p_prev = 0; partials r.z, z.z, (bn/D)^2.
+// r = b - A x (or b when x0 = 0), z = r / D; partials r.z, z.z, (bn/D)^2. Iteration 0 reads z as
+// p_prev (beta = 0), so no p_prev is cleared.
 template <typename T, bool THREE_D>
 __global__ __launch_bounds__(NT) void k_init(Geo g, int kchunk, StencilView<T> A, const T* b,
                                               const T* bn, T* x, T* r, T* z,
