@@ -62,6 +62,43 @@ __device__ __forceinline__ BoxCell box_cell(const EnsBox* boxes, const unsigned 
   return r;
 }
 
+// box_cell with the box parameters in soil variant K (see with_soil_variant_code) and the tile
+// order of the single-box kernels: ZF = 1 takes the layer fastest within the box's tiles (the
+// layers above and below a tile were just read through the same L2), ZF = 0 x fastest.
+template <class K>
+struct BoxCellT {
+  int b;
+  unsigned int tile;
+  long long off;
+  K P;
+  Cell t;
+};
+
+template <class K, int ZF>
+__device__ __forceinline__ BoxCellT<K> box_cell_as(const EnsBox* boxes, const unsigned int* t0, int nbox) {
+  BoxCellT<K> r;
+  r.tile = xcd_remap(blockIdx.x, gridDim.x);
+  r.b = find_box(t0, nbox, r.tile);
+  const EnsBox& B = boxes[r.b];
+  r.P = rek_to<K>(B.P);
+  r.off = B.off;
+  const unsigned int local = r.tile - t0[r.b];
+  unsigned int bx, by, bz;
+  if (ZF) {
+    const unsigned int nz = (unsigned int)r.P.g.nz, rest = local / nz;
+    bz = local - rest * nz;
+    bx = rest % B.ntx;
+    by = rest / B.ntx;
+  } else {
+    const unsigned int rest = local / B.ntx;
+    bx = local - rest * B.ntx;
+    by = rest % B.nty;
+    bz = rest / B.nty;
+  }
+  r.t = cell_at(r.P.g, bx, by, bz);
+  return r;
+}
+
 // Column run of this thread in an SpMV-tile launch: cell (i, j) of the box, layers [k0, k1).
 struct BoxCol {
   int b;
@@ -116,14 +153,16 @@ __global__ __launch_bounds__(NT) void kb_save_ghosts(const EnsBox* boxes, const 
     }
 }
 
+template <class K, int ZF>
 __global__ __launch_bounds__(NT) void kb_pre(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F) {
-  const BoxCell q = box_cell(boxes, t0, nbox);
+  const BoxCellT<K> q = box_cell_as<K, ZF>(boxes, t0, nbox);
   if (q.t.valid) re_pre(q.P, q.t, F.hn + q.off, F.kr + q.off, F.ch + q.off);
 }
 
+template <class K, int ZF>
 __global__ __launch_bounds__(NT) void kb_assemble(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F,
                                                    float* D32, float* ax32, float* ay32, float* az32) {
-  const BoxCell q = box_cell(boxes, t0, nbox);
+  const BoxCellT<K> q = box_cell_as<K, ZF>(boxes, t0, nbox);
   if (!q.t.valid) return;
   const long long o = q.off;
   re_assemble(q.P, q.t, F.kr + o, F.ch + o, F.wcn + o, F.hn + o, F.D + o, F.ax + o, F.ay + o, F.az + o, F.rhs + o,
@@ -139,17 +178,19 @@ __global__ __launch_bounds__(NT) void kb_guess(const EnsBox* boxes, const unsign
   F.h[c] = previous ? F.hn[c] : 0.0;
 }
 
+template <class K, int ZF>
 __global__ __launch_bounds__(NT) void kb_post(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F) {
-  const BoxCell q = box_cell(boxes, t0, nbox);
+  const BoxCellT<K> q = box_cell_as<K, ZF>(boxes, t0, nbox);
   if (q.t.valid) re_post(q.P, q.t, F.h + q.off, F.kr + q.off, F.ch + q.off, F.wch + q.off);
 }
 
 // Fluxes at h^{n+1} and the corrector in one pass (re_flux_correct).
+template <class K, int ZF>
 __global__ __launch_bounds__(NT) void kb_flux_correct(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F,
                                                        unsigned char* sat0, unsigned char* cand, double* part,
                                                        unsigned int ntile) {
   __shared__ double lds[NT / kWave];
-  const BoxCell q = box_cell(boxes, t0, nbox);
+  const BoxCellT<K> q = box_cell_as<K, ZF>(boxes, t0, nbox);
   double a = 0.0, b = 0.0, d = 0.0;
   if (q.t.valid) {
     const long long o = q.off;
@@ -185,12 +226,13 @@ __global__ __launch_bounds__(NT) void kb_realloc(const EnsBox* boxes, const unsi
   re_realloc(q.P, q.t, FacesK{F.Kx + o, F.Ky + o}, sat0 + o, cand + o, prev + o, next + o, flags + s);
 }
 
+template <class K, int ZF>
 __global__ __launch_bounds__(NT) void kb_finalize(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F,
                                                    const unsigned char* sat0, const unsigned char* fin, double* part,
                                                    unsigned int ntile) {
   __shared__ double lds[NT / kWave];
   __shared__ double lmax[NT / kWave];
-  const BoxCell q = box_cell(boxes, t0, nbox);
+  const BoxCellT<K> q = box_cell_as<K, ZF>(boxes, t0, nbox);
   double a = 0.0, b = 0.0, dh = 0.0;
   if (q.t.valid) {
     const long long o = q.off;
@@ -678,6 +720,10 @@ Ensemble::~Ensemble() {
 }
 
 void Ensemble::upload_boxes() {
+  // one soil variant for the per-cell kernels when every box has the same (else run-time checks)
+  variant_ = nbox_ > 0 ? soil_variant_code(hbox_[0].P) : 0;
+  for (int b = 1; b < nbox_; ++b)
+    if (soil_variant_code(hbox_[b].P) != variant_) variant_ = 0;
   for (int b = 0; b < nbox_; ++b) {
     const EnsBox& B = hbox_[b];
     spmv0_[b + 1] = spmv0_[b] + (unsigned int)(B.ntx * B.nty * ceil_div(B.P.g.nz, B.kc));
@@ -810,9 +856,12 @@ void Ensemble::enq_prepare() {
   const unsigned int* t0 = dtile0_;
   hipLaunchKernelGGL(kb_save, grid, blk, 0, s_, bx, t0, nbox_, P);
   hipLaunchKernelGGL(kb_save_ghosts, grid, blk, 0, s_, bx, t0, nbox_, P);
-  hipLaunchKernelGGL(kb_pre, grid, blk, 0, s_, bx, t0, nbox_, P);
-  hipLaunchKernelGGL(kb_assemble, grid, blk, 0, s_, bx, t0, nbox_, P, mixed_ ? F_.D32 : nullptr, F_.ax32, F_.ay32,
-                     F_.az32);
+  with_soil_variant_code(variant_, [&](auto tag) {
+    using K = decltype(tag);
+    hipLaunchKernelGGL((kb_pre<K, 1>), grid, blk, 0, s_, bx, t0, nbox_, P);
+    hipLaunchKernelGGL((kb_assemble<K, 1>), grid, blk, 0, s_, bx, t0, nbox_, P, mixed_ ? F_.D32 : nullptr, F_.ax32,
+                       F_.ay32, F_.az32);
+  });
   hipLaunchKernelGGL(kb_guess, grid, blk, 0, s_, bx, t0, nbox_, P, prev_guess_ ? 1 : 0);
   FREHG_LAUNCH_CHECK();
   // Krylov start: FP64 CG init, or the first refinement residual
@@ -885,8 +934,11 @@ void Ensemble::enq_update(int sweeps) {
   if (mixed_)
     hipLaunchKernelGGL(kb_take_x, dim3(nspmv_), blk, 0, s_, bx, (const unsigned int*)dspmv0_, nbox_, F_.h,
                        (const double*)F_.xa, ctl_);
-  hipLaunchKernelGGL(kb_post, grid, blk, 0, s_, bx, t0, nbox_, P);
-  hipLaunchKernelGGL(kb_flux_correct, grid, blk, 0, s_, bx, t0, nbox_, P, F_.sat0, F_.cand, part_, ntile_);
+  with_soil_variant_code(variant_, [&](auto tag) {
+    using K = decltype(tag);
+    hipLaunchKernelGGL((kb_post<K, 1>), grid, blk, 0, s_, bx, t0, nbox_, P);
+    hipLaunchKernelGGL((kb_flux_correct<K, 1>), grid, blk, 0, s_, bx, t0, nbox_, P, F_.sat0, F_.cand, part_, ntile_);
+  });
   hipLaunchKernelGGL(kb_clear_flags, dim3(1), dim3(64), 0, s_, flags_ + 3, kMaxSweepFlags);
   for (int sw = 0; sw < sweeps; ++sw) {
     // sweep 0 reads sat0 directly; later sweeps ping-pong between satA and satB
@@ -903,8 +955,11 @@ void Ensemble::enq_finish(const unsigned char* fin) {
                 F_.D, F_.ax, F_.ay, F_.az, F_.rhs};
   const dim3 grid(ntile_), blk(NT);
   // after an odd number of sweeps the state is in satA, after an even number in satB
-  hipLaunchKernelGGL(kb_finalize, grid, blk, 0, s_, (const EnsBox*)dbox_, (const unsigned int*)dtile0_, nbox_, P,
-                     (const unsigned char*)F_.sat0, fin, part_, ntile_);
+  with_soil_variant_code(variant_, [&](auto tag) {
+    using K = decltype(tag);
+    hipLaunchKernelGGL((kb_finalize<K, 0>), grid, blk, 0, s_, (const EnsBox*)dbox_, (const unsigned int*)dtile0_,
+                       nbox_, P, (const unsigned char*)F_.sat0, fin, part_, ntile_);
+  });
   hipLaunchKernelGGL(kb_box_sums, dim3(nbox_), blk, 0, s_, (const double*)part_, ntile_, (const unsigned int*)dtile0_,
                      boxsum_);
   FREHG_LAUNCH_CHECK();

@@ -99,6 +99,7 @@ class Ensemble {
   int graph_budget2_[3] = {-1, -1, -1};
   int graph_budget3_[3] = {-1, -1, -1};
   static constexpr int kMaxSweepFlags = 64;
+  int variant_ = 0;  // soil variant code of the per-cell kernels (with_soil_variant_code)
 };
 
 }  // namespace frehg

@@ -32,7 +32,7 @@ using ReK = ReKT<kRuntime, kRuntime, 2>;
 
 // The same parameters with the soil lookup fixed to the given variant.
 template <int CLS, int KS, int F3D>
-inline ReKT<CLS, KS, F3D> rek_as(const ReK& P) {
+__host__ __device__ inline ReKT<CLS, KS, F3D> rek_as(const ReK& P) {
   ReKT<CLS, KS, F3D> Q;
   Q.g = P.g;
   Q.soil.classes = P.soil.classes;
@@ -55,6 +55,39 @@ inline ReKT<CLS, KS, F3D> rek_as(const ReK& P) {
   Q.use_corrector = P.use_corrector;
   Q.qtop_field = P.qtop_field;
   return Q;
+}
+
+// rek_as with the variant given as the target type.
+template <class K>
+struct RekTraits;
+template <int CLS, int KS, int F3D>
+struct RekTraits<ReKT<CLS, KS, F3D>> {
+  static constexpr int cls = CLS, ks = KS, f3d = F3D;
+};
+template <class K>
+__host__ __device__ inline K rek_to(const ReK& P) {
+  return rek_as<RekTraits<K>::cls, RekTraits<K>::ks, RekTraits<K>::f3d>(P);
+}
+
+// Variant code of P for with_soil_variant_code: 1..6 as in with_soil_variant, 0 = run time.
+inline int soil_variant_code(const ReK& P) {
+  const bool c = P.soil.cls != nullptr, k = P.soil.ks != nullptr, f3 = P.full3d != 0;
+  if (!c) return k ? (f3 ? 1 : 2) : (f3 ? 3 : 4);
+  return k ? 5 : 6;
+}
+
+// Calls f(Q) with Q of the variant `code` (0: the runtime-checked ReK) built from P.
+template <typename F>
+inline void with_soil_variant_code(int code, F&& f) {
+  switch (code) {
+    case 1: f(ReKT<kAbsent, kPresent, 1>{}); break;
+    case 2: f(ReKT<kAbsent, kPresent, 0>{}); break;
+    case 3: f(ReKT<kAbsent, kAbsent, 1>{}); break;
+    case 4: f(ReKT<kAbsent, kAbsent, 0>{}); break;
+    case 5: f(ReKT<kPresent, kPresent, 2>{}); break;
+    case 6: f(ReKT<kPresent, kAbsent, 2>{}); break;
+    default: f(ReK{});
+  }
 }
 
 // Calls f(Q) with Q = P fixed to the variant matching which soil arrays P has (and, for one soil
