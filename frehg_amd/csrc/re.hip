@@ -206,6 +206,31 @@ __global__ __launch_bounds__(NT) void k_re_dtadapt(ReK P, const double* qx, cons
   }
 }
 
+// Picard test: per-block max |h - ref| over interior cells to part[0..nblocks) (part[nblocks..)
+// = 0 for k_reduce_maxmin), and hm = h (the iterate the next test compares against; hm may be
+// ref itself).
+__global__ __launch_bounds__(NT) void k_picard_delta(Geo g, const double* ref, const double* h, double* hm,
+                                                      double* part, unsigned int nblocks) {
+  const Cell t = cell_of(g);
+  double d = 0.0;
+  if (t.valid) {
+    const double hv = h[t.c];
+    d = fabs(hv - ref[t.c]);
+    hm[t.c] = hv;
+  }
+  d = wave_max(d);
+  __shared__ double lmax[NT / kWave];
+  if ((threadIdx.x % kWave) == 0) lmax[threadIdx.x / kWave] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = lmax[0];
+    for (int q = 1; q < NT / kWave; ++q) m = fmax(m, lmax[q]);
+    const unsigned int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    part[bid] = m;
+    part[nblocks + bid] = 0.0;
+  }
+}
+
 __global__ __launch_bounds__(NT) void k_re_initial(ReK P, int from_head, double* h, double* wc, double* hn,
                                                     double* wcn, double* wch, double* hwc, double* ch) {
   const Cell t = cell_of(P.g);
@@ -276,8 +301,8 @@ void ReState::set_params(const frehg_re_params& p) {
   if (p.bc_type[4] < 0 || p.bc_type[4] > 3) throw ArgError("frehg_re: bottom bc must be 0, 1, 2 or 3");
   if (p.bc_type[5] < 0 || p.bc_type[5] > 2) throw ArgError("frehg_re: top bc must be 0, 1 or 2");
   if (!(p.dx > 0 && p.dy > 0 && p.dz > 0 && p.dt > 0)) throw ArgError("frehg_re: dx, dy, dz, dt must be > 0");
-  if (p.picard_max != 1)
-    throw ArgError("frehg_re: picard_max must be 1 (the reference's predictor-corrector step; no Picard loop)");
+  if (p.picard_max < 1) throw ArgError("frehg_re: picard_max must be >= 1 (1 = the reference's predictor)");
+  if (p.picard_max > 1 && !(p.picard_tol > 0.0)) throw ArgError("frehg_re: picard_tol must be > 0");
   (void)krylov_config(p.solver, "frehg_re");
   const bool mixed = p.solver.precision == FREHG_PREC_MIXED;
   if (mixed && !D32_) {
@@ -516,6 +541,38 @@ void ReState::step(frehg_re_stats* st) {
   local.solve.fallback = sr.fallback_fp64 ? 1 : 0;
   local.guess_fallback = sr.guess_rejected ? 1 : 0;
   local.picard_iterations = 1;
+  // Picard outer loop (picard_max > 1; a new method, the reference's step is its first iterate):
+  // K and C at the iterate h^m with its head-BC ghosts, the storage and rhs terms at time level n
+  // as in the predictor, solve for h^{m+1} (from x0 = 0 with the zero guess, else from h^m); stop
+  // when max |h^{m+1} - h^m| <= picard_tol (h^0 = h^n).
+  for (int m = 1; m < std::max(1, p_.picard_max); ++m) {
+    hipLaunchKernelGGL(k_picard_delta, grid, blk, 0, s_, g_, (const double*)(m == 1 ? hn_ : hp_), (const double*)h_,
+                       hp_, red_, nbl);
+    FREHG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce_maxmin, dim3(1), dim3(kRedThreads), 0, s_, (const double*)red_, nbl, dres_);
+    FREHG_LAUNCH_CHECK();
+    FREHG_HIP(hipMemcpyAsync(hres_, dres_, sizeof(double), hipMemcpyDeviceToHost, s_));
+    FREHG_HIP(hipStreamSynchronize(s_));
+    if (hres_[0] <= p_.picard_tol) break;
+    with_soil_variant(P, [&](const auto& PK) {
+      using K = std::decay_t<decltype(PK)>;
+      hipLaunchKernelGGL((k_re_post<K, 1>), grid, blk, 0, s_, PK, h_, kr_, ch_, wch_);
+      FREHG_LAUNCH_CHECK();
+      hipLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
+                         (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, mixed ? D32_ : nullptr,
+                         ax32_, ay32_, az32_);
+      FREHG_LAUNCH_CHECK();
+    });
+    const bool zero = p_.initial_guess == FREHG_GUESS_ZERO;
+    const SolveResult sm = kry_->solve(A, mixed ? &A32 : nullptr, rhs_, nullptr, h_, nullptr, zero, false, kc);
+    local.solve.iterations += sm.iterations;
+    local.solve.refinements += sm.refinements;
+    local.solve.residual_norm = sm.residual_norm;
+    local.solve.rhs_norm = sm.rhs_norm;
+    local.solve.converged = (local.solve.converged && sm.converged) ? 1 : 0;
+    local.solve.fallback |= sm.fallback_fp64 ? 1 : 0;
+    local.picard_iterations = m + 1;
+  }
   if (mlp_ && mlp_->logging()) mlp_->log_pair(*this, hn_, h_, depth_);
   {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_CORRECT);

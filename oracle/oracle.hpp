@@ -223,12 +223,16 @@ struct ReParams {  // ReSolver.hpp:24-46 (defaults kept)
   int bc_type[6] = {0, 0, 0, 0, 0, 1};
   real htop = 0.0, hbot = 0.0, qtop = 0.0, qbot = 0.0;
   SolverCfg solver{1e-10, 1e-50, 10000};  // Orchestrator.cpp:487-490 (GW rtol 1e-10)
+  // Picard outer loop (no counterpart in the reference: picard_max = 1 is its predictor).
+  int picard_max = 1;
+  real picard_tol = 1e-8;
 };
 
 // Water ledger of one step (m^3), as reported by frehg_re_stats: theta_c is the corrector's theta
 // before reallocation and clamp.
 struct ReStepStats {
-  SolveStats solve;
+  SolveStats solve;         // the last Picard iteration's solve (iterations: all of them)
+  int picard_iterations = 1;
   real dt_used = 0.0;
   real dt_next = 0.0;
   real max_dh = 0.0;

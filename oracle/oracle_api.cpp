@@ -61,6 +61,8 @@ ReParams toRe(const frehg_re_params& p) {
   o.qtop = p.qtop;
   o.qbot = p.qbot;
   o.solver = toCfg(p.solver);
+  o.picard_max = p.picard_max;
+  o.picard_tol = p.picard_tol;
   return o;
 }
 
@@ -216,7 +218,7 @@ int orc_re_step(void* h, frehg_re_stats* st) {
       fillStats(s.solve, &st->solve);
       st->dt_used = s.dt_used;
       st->dt_next = s.dt_next;
-      st->picard_iterations = 1;
+      st->picard_iterations = s.picard_iterations;
       st->max_dh = s.max_dh;
       st->mass_before = s.mass_before;
       st->mass_after = s.mass_after;

@@ -521,13 +521,34 @@ void ReSolver::advanceStep(ReStepStats* st) {  // :931-969
     hn[c] = h[c];
     wcn[c] = wc[c];
   }
-  updateDiagnostics();
-  computeKFace();
-  matCoeff();
-  rhs();
+  // Picard outer loop (a new method; picard_max = 1 is the reference's predictor): K and C at the
+  // iterate h^m (h^0 = h^n, ghost heads from the head BCs), the storage and rhs terms at time level
+  // n as in the predictor, solve from x0 = 0 for h^{m+1}; stop when max |h^{m+1} - h^m| <= tol.
   SolveStats ss;
-  assembleAndSolve(&ss);
-  enforceHeadBc();
+  int picard = 0, iters = 0;
+  std::vector<real> hm;
+  const int pmax = std::max(1, p_.picard_max);
+  for (;;) {
+    updateDiagnostics();
+    computeKFace();
+    matCoeff();
+    rhs();
+    if (pmax > 1) hm = h;
+    assembleAndSolve(&ss);
+    iters += ss.iterations;
+    ++picard;
+    enforceHeadBc();
+    if (picard >= pmax) break;
+    real delta = 0.0;
+    for (int k = 0; k < g.nz; ++k)
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) {
+          const long long c = g.idx(i, j, k);
+          delta = std::max(delta, std::fabs(h[c] - hm[c]));
+        }
+    if (delta <= p_.picard_tol) break;
+  }
+  ss.iterations = iters;
   computeKFace();
   groundwaterFlux();
   if (p_.use_corrector) {
@@ -570,6 +591,7 @@ void ReSolver::advanceStep(ReStepStats* st) {  // :931-969
   if (p_.adaptive_dt) adaptiveTimeStep();
   if (st) {
     st->solve = ss;
+    st->picard_iterations = picard;
     st->dt_used = dt_used;
     st->dt_next = p_.dt;
     st->max_dh = dhmax;

@@ -145,6 +145,41 @@ def test_lazy_flux_fields_match_stored(full3d, adaptive):
             assert np.array_equal(a.re_get(f), b.re_get(f)), f"step {s} field {f}"
 
 
+@pytest.mark.parametrize("mode", ["fp64_zero", "mixed_previous"])
+def test_picard_outer_loop_matches_oracle(mode):
+    """Picard outer loop (picard_max > 1) on an infiltration box at a large step: K and C at each
+    iterate, storage terms at h^n, stop at max |dh| <= tol. FP64 from x0 = 0 is the oracle's
+    algorithm (every field, same Picard count); FP32-in-FP64 from the previous iterate reaches
+    the same heads to 1e-6 m."""
+    nx, ny, nz = 9, 7, 12
+    say(f"RE Picard loop {mode}")
+    soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5, Ss=1e-5)
+    kw = dict(soil=soil, bc_type=(0, 0, 0, 0, 0, 1), dx=0.5, dy=0.5, dz=0.05, dt=60.0, adaptive_dt=0,
+              use_full3d=1, htop=-0.3, picard_max=40, picard_tol=1e-9)
+    p = abi.re_params(solver=abi.solver_config(1e-13), **kw)
+    k = np.arange(nz)[:, None, None]
+    i = np.arange(nx)[None, None, :]
+    h0 = np.broadcast_to(-1.5 - 0.05 * k - 0.02 * i, (nz, ny, nx)).ravel()
+    orc, _ = re_pair(nx, ny, nz, p, h0=h0)
+    if mode == "fp64_zero":
+        _, ctx = re_pair(nx, ny, nz, p, h0=h0)
+    else:
+        pm = abi.re_params(solver=abi.solver_config(1e-13, precision=abi.PREC_MIXED, inner_rtol=1e-5, max_refine=10),
+                           initial_guess=abi.GUESS_PREVIOUS, **kw)
+        _, ctx = re_pair(nx, ny, nz, pm, h0=h0)
+    for s in range(4):
+        so = orc.step()
+        sg = ctx.re_step()
+        assert so.picard_iterations > 1
+        if mode == "fp64_zero":
+            assert sg.picard_iterations == so.picard_iterations, (s, sg.picard_iterations, so.picard_iterations)
+        assert abs(sg.mass_residual) <= 1e-10 * sg.mass_before
+    if mode == "fp64_zero":
+        compare_re(ctx, orc, label="picard")
+    else:
+        assert np.abs(ctx.re_get("h") - orc.get("h")).max() <= 1e-6
+
+
 def test_heterogeneous_soil_and_top_flux():
     nx, ny, nz = 9, 7, 10
     say("RE heterogeneous soil + per-column top flux")

@@ -135,6 +135,73 @@ def test_column_predictor_matches_dense_solve():
     assert qz[cells[0]] == pytest.approx(q_dense_mid, rel=1e-9)
 
 
+def _column_system(soil, p, h_kc, h_time, dz, dt):
+    """3x3 system of the 1x1x3 top-Dirichlet column written from the discretisation by hand
+    (ReSolver.cpp:397-549): K and C evaluated at h_kc, the storage and time terms at h_time."""
+    ev = ol.vg_eval(soil, h=h_kc)
+    K, Cap = ev["K"], ev["C"]
+    th = ol.vg_eval(soil, h=h_time)["theta"]
+    Vc = dz
+    kf = [0.5 * (K[0] + K[1]), 0.5 * (K[1] + K[2]), 0.0]  # +z faces; closed bottom
+    ktop = soil.Ks_z  # top Dirichlet face uses Ks_z (ReSolver.cpp:355-356)
+    A = np.zeros((3, 3))
+    b = np.zeros(3)
+    for k in range(3):
+        st = (Cap[k] + soil.Ss * th[k] / soil.theta_s) * Vc
+        A[k, k] = st
+        b[k] = st * h_time[k]
+        kup = ktop if k == 0 else kf[k - 1]
+        gup = kup * dt * Vc / (dz * (0.5 * dz if k == 0 else dz))
+        gdn = kf[k] * dt * Vc / (dz * dz)
+        A[k, k] += gup + gdn
+        if k > 0:
+            A[k, k - 1] -= gup
+        else:
+            b[k] += gup * p.htop
+        if k < 2:
+            A[k, k + 1] -= gdn
+        b[k] += -dt * Vc * kf[k] / dz + dt * Vc * kup / dz
+    return A, b, ktop
+
+
+def test_column_picard_matches_dense_fixed_point():
+    """Picard outer loop (picard_max > 1): the converged head of a 1x1x3 column is the fixed point
+    h = A(h)^-1 b(h; h^n) of the hand-written system with K and C at the iterate and the storage
+    terms at h^n, found here by dense solves; it differs from the predictor (one iterate)."""
+    soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.4, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5, Ss=1e-5)
+    dz, dt = 0.05, 50.0
+    h0 = np.array([-2.0, -2.5, -3.0])
+
+    def run(picard_max):
+        p = abi.re_params(soil=soil, solver=abi.solver_config(1e-14), bc_type=(0, 0, 0, 0, 0, 1), dx=1.0, dy=1.0,
+                          dz=dz, dt=dt, adaptive_dt=0, htop=-0.5, use_full3d=1, picard_max=picard_max,
+                          picard_tol=1e-12)
+        r = ol.OracleRe(1, 1, 3, p)
+        r.set_initial(h=h0)
+        st = r.step()
+        return p, st, r.get("qz")
+
+    p, st, qz = run(60)
+    assert 2 <= st.picard_iterations < 60
+    hm = h0.copy()
+    for it in range(60):
+        A, b, ktop = _column_system(soil, p, hm, h0, dz, dt)
+        hn1 = np.linalg.solve(A, b)
+        d = np.abs(hn1 - hm).max()
+        hm = hn1
+        if d <= 1e-12:
+            break
+    q_top = ktop * ((hm[0] - p.htop) / (0.5 * dz) - 1.0)
+    k01 = ol.vg_eval(soil, h=hm[:2])["K"]
+    q_mid = 0.5 * (k01[0] + k01[1]) * ((hm[1] - hm[0]) / dz - 1.0)
+    assert qz[1 + 3] == pytest.approx(q_top, rel=1e-9)   # idx(0,0,-1): top ghost face
+    assert qz[3 * 3 * 1 + 1 + 3] == pytest.approx(q_mid, rel=1e-9)  # idx(0,0,0)
+    # one iterate is the reference's predictor, which is a different head here
+    _, st1, qz1 = run(1)
+    assert st1.picard_iterations == 1
+    assert abs(qz1[1 + 3] - qz[1 + 3]) > 1e-3 * abs(qz[1 + 3])
+
+
 @pytest.mark.parametrize("case", ["closed_ss", "infiltration", "saturating", "drainage"])
 def test_water_ledger_closes(case):
     """The step's FP64 ledger: mass_after - mass_before = boundary inflow + specific-storage term +
