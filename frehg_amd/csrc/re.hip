@@ -529,6 +529,23 @@ void ReState::step(frehg_re_stats* st) {
     x0_zero = false;
     guard = true;
   }
+  // With the Picard loop the learned head is also the loop's starting iterate h^0: K and C at it
+  // (with its head-BC ghosts), the assembly with the time-level-n terms, and the first stopping
+  // test against it (kept in hp_). The features above were taken at h^n.
+  const bool learned_h0 = p_.initial_guess == FREHG_GUESS_LEARNED && p_.picard_max > 1;
+  if (learned_h0) {
+    with_soil_variant(P, [&](const auto& PK) {
+      using K = std::decay_t<decltype(PK)>;
+      hipLaunchKernelGGL((k_re_post<K, 1>), grid, blk, 0, s_, PK, h_, kr_, ch_, wch_);
+      FREHG_LAUNCH_CHECK();
+      hipLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
+                         (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, mixed ? D32_ : nullptr,
+                         ax32_, ay32_, az32_);
+      FREHG_LAUNCH_CHECK();
+    });
+    hipLaunchKernelGGL(k_copy_interior, grid, blk, 0, s_, g_, (const double*)h_, hp_);
+    FREHG_LAUNCH_CHECK();
+  }
   const KrylovConfig kc = krylov_config(p_.solver, "frehg_re_step");
   StencilView<double> A{D_, ax_, ay_, az_};
   StencilView<float> A32{D32_, ax32_, ay32_, az32_};
@@ -544,10 +561,10 @@ void ReState::step(frehg_re_stats* st) {
   // Picard outer loop (picard_max > 1; a new method, the reference's step is its first iterate):
   // K and C at the iterate h^m with its head-BC ghosts, the storage and rhs terms at time level n
   // as in the predictor, solve for h^{m+1} (from x0 = 0 with the zero guess, else from h^m); stop
-  // when max |h^{m+1} - h^m| <= picard_tol (h^0 = h^n).
+  // when max |h^{m+1} - h^m| <= picard_tol (h^0 = h^n, or the learned head).
   for (int m = 1; m < std::max(1, p_.picard_max); ++m) {
-    hipLaunchKernelGGL(k_picard_delta, grid, blk, 0, s_, g_, (const double*)(m == 1 ? hn_ : hp_), (const double*)h_,
-                       hp_, red_, nbl);
+    const double* ref = (m == 1 && !learned_h0) ? hn_ : hp_;
+    hipLaunchKernelGGL(k_picard_delta, grid, blk, 0, s_, g_, ref, (const double*)h_, hp_, red_, nbl);
     FREHG_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_reduce_maxmin, dim3(1), dim3(kRedThreads), 0, s_, (const double*)red_, nbl, dres_);
     FREHG_LAUNCH_CHECK();
@@ -573,7 +590,16 @@ void ReState::step(frehg_re_stats* st) {
     local.solve.fallback |= sm.fallback_fp64 ? 1 : 0;
     local.picard_iterations = m + 1;
   }
-  if (mlp_ && mlp_->logging()) mlp_->log_pair(*this, hn_, h_, depth_);
+  if (mlp_ && mlp_->logging()) {
+    // the features are K and C at h^n: the Picard iterates replaced them
+    if (p_.picard_max > 1 || learned_h0)
+      with_soil_variant(P, [&](const auto& PK) {
+        using K = std::decay_t<decltype(PK)>;
+        hipLaunchKernelGGL((k_re_pre<K, 1>), grid, blk, 0, s_, PK, (const double*)hn_, kr_, ch_);
+        FREHG_LAUNCH_CHECK();
+      });
+    mlp_->log_pair(*this, hn_, h_, depth_);
+  }
   {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_CORRECT);
     hipEvent_t ev1 = prof_->stop();

@@ -41,18 +41,19 @@ def test_gradient_matches_oracle(n):
         assert err <= 5e-3 * scale, f"{name}: {err:.3e} vs scale {scale:.3e}"
 
 
-def infiltration(nx, ny, nz, seed, guess):
+def infiltration(nx, ny, nz, seed, guess, picard_max=1):
     rng = np.random.default_rng(seed)
     soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5, Ss=1e-5)
     p = abi.re_params(soil=soil, solver=abi.solver_config(1e-10), bc_type=(0, 0, 0, 0, 3, 1), dx=1.0, dy=1.0,
-                      dz=0.05, dt=2.0, adaptive_dt=0, use_full3d=1, htop=-0.2, initial_guess=guess)
+                      dz=0.05, dt=2.0, adaptive_dt=0, use_full3d=1, htop=-0.2, initial_guess=guess,
+                      picard_max=picard_max, picard_tol=1e-8)
     ks = np.exp(rng.normal(np.log(1e-5), 1.0, nx * ny * nz))
     h0 = np.full(nx * ny * nz, -2.0) + 0.2 * rng.standard_normal(nx * ny * nz)
     return p, soil, ks, h0
 
 
-def run(ctx, nx, ny, nz, seed, guess, steps):
-    p, soil, ks, h0 = infiltration(nx, ny, nz, seed, guess)
+def run(ctx, nx, ny, nz, seed, guess, steps, picard_max=1, picards=None):
+    p, soil, ks, h0 = infiltration(nx, ny, nz, seed, guess, picard_max)
     ctx.re_init(nx, ny, nz, p)
     ctx.re_set_soil([soil], None, True, ks)
     ctx.re_set_initial(h=h0)
@@ -62,6 +63,8 @@ def run(ctx, nx, ny, nz, seed, guess, steps):
         assert st.solve.converged
         its.append(st.solve.iterations)
         fallbacks += st.guess_fallback
+        if picards is not None:
+            picards.append(st.picard_iterations)
     return np.array(its), fallbacks, ctx.re_get("h")
 
 
@@ -90,6 +93,16 @@ def test_training_and_learned_guess_gain():
     assert its1.sum() < its0.sum() and its1.sum() <= itsp.sum()
     assert fb == 0
     assert np.abs(h1 - h0).max() <= 1e-7 and np.abs(hp - h0).max() <= 1e-7
+    say("MLP: held-out field with the Picard loop (tol 1e-8 m): from h^n vs from the learned head")
+    # (Picard contracts by about 0.3-0.5 per iteration here, so a start that is closer by one order
+    # of magnitude saves only one or two iterations of a step)
+    pz, pl = [], []
+    _, _, hz = run(ctx, nx, ny, nz, 201, abi.GUESS_ZERO, steps, picard_max=80, picards=pz)
+    _, fb, hl = run(ctx, nx, ny, nz, 201, abi.GUESS_LEARNED, steps, picard_max=80, picards=pl)
+    say(f"Picard iterations per step: from h^n {np.mean(pz):.2f}, from the learned head {np.mean(pl):.2f}")
+    assert max(pz) > 1 and max(pz) < 80 and max(pl) < 80
+    assert sum(pl) <= sum(pz)
+    assert np.abs(hl - hz).max() <= 1e-6
 
 
 def test_untrained_model_rejected_or_harmless():
