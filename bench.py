@@ -6,12 +6,15 @@ Workload: the coupled 512x512 surface over 512x512x30 soil (dx = 1 m, dz = 0.05 
 generated from seeds by frehg_amd.workloads. One step = Orchestrator::step, coupled branch:
 SWE step, SW<->GW exchange, Richards PCA step with its Krylov solve.
 
-Three GPU paths run from the same initial state for warmup + timed steps:
-  plain    FP64 Jacobi-PCG from x0 = 0 (the reference's solver semantics)
-  learned  FP32-inner / FP64-refined PCG from x0 = h^n + MLP increment (zero-guess fallback)
-  previous FP32-inner / FP64-refined PCG from x0 = h^n
-The MLP is trained beforehand (not timed) on pairs logged from plain runs of two other seeded
-bed/Ks realisations of the same workload. `value` is the learned path's throughput; every
+Each Richards step is the Picard loop to max |h^{m+1} - h^m| <= 1e-8 m (--picard-max iterates at
+most); the reference's own predictor step (one iterate) is measured as well and reported under
+predictor_step. Three GPU paths run from the same initial state for warmup + timed steps:
+  plain    FP64 Jacobi-PCG from x0 = 0 in every iterate (the reference's solver semantics)
+  learned  FP32-inner / FP64-refined PCG; the first iterate and its Krylov guess are h^n + MLP
+           increment (zero-guess fallback), later iterates start from the previous one
+  previous FP32-inner / FP64-refined PCG from x0 = h^n, later iterates from the previous one
+The MLP is trained beforehand (not timed) on pairs (state, converged increment) logged from plain
+runs of two other seeded bed/Ks realisations of the same workload. `value` is the learned path's throughput; every
 path's final head is checked against the plain path's.
 Kernel durations come from HIP events on the backend's stream during the timed region.
 The CPU baseline is the repository's FP64 oracle (a restatement of the reference algorithm)
@@ -52,16 +55,16 @@ LIVE_TYPES = ("spmv64", "spmv32", "guess")
 BREAKDOWN_STEPS = 10
 
 
-def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None):
+def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None, picard_max=1):
     """Warmup + timed coupled steps; returns timing, solver counters and kernel timings."""
     ctx = gpu.Context(0)
     if path == "plain":
-        case.setup_gpu(ctx, abi.PREC_FP64, abi.GUESS_ZERO)
+        case.setup_gpu(ctx, abi.PREC_FP64, abi.GUESS_ZERO, picard_max=picard_max)
     elif path == "learned":
         ctx.mlp_set_params(*model)
-        case.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_LEARNED)
+        case.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_LEARNED, picard_max=picard_max)
     else:
-        case.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_PREVIOUS)
+        case.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_PREVIOUS, picard_max=picard_max)
     grabbed = None
     for s in range(warmup):
         ctx.coupled_step(case.cp, case.dt, case.rain, 0.0, case.t_end)
@@ -73,7 +76,7 @@ def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None):
     ctx.profile_select([kt_ids[k] for k in LIVE_TYPES])
     ctx.profile_enable(True)
     ctx.profile_reset()
-    iters = refines = fallbacks = 0
+    iters = refines = fallbacks = picards = 0
     mass_err = 0.0
     t0 = time.perf_counter()
     for s in range(steps):
@@ -81,6 +84,7 @@ def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None):
         iters += st.gw.solve.iterations
         refines += st.gw.solve.refinements
         fallbacks += st.gw.guess_fallback
+        picards += st.gw.picard_iterations
         mass_err = max(mass_err, abs(st.gw.mass_residual) / st.gw.mass_before)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
@@ -97,11 +101,12 @@ def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None):
     h = ctx.re_get("h")
     sw_vol, gw_vol = ctx.water_volume()
     ctx.close()
-    return {"elapsed": elapsed, "iters": iters, "refines": refines, "fallbacks": fallbacks, "kt": kt, "live": live, "h": h,
+    return {"elapsed": elapsed, "iters": iters, "refines": refines, "fallbacks": fallbacks, "picards": picards,
+            "kt": kt, "live": live, "h": h,
             "grabbed": grabbed, "volume": sw_vol + gw_vol, "spmv_shape": shape, "mass_err": mass_err}
 
 
-def train_model(abi, gpu, workloads, args, seeds, log_steps, stride, train_steps, batch):
+def train_model(abi, gpu, workloads, args, seeds, log_steps, stride, train_steps, batch, picard_max=1):
     """Log (features, converged increment) pairs from plain FP64 runs of other seeded realisations
     of the workload, then train the MLP on the GPU. Returns (params, norm) and a summary."""
     ctx = gpu.Context(0)
@@ -111,7 +116,7 @@ def train_model(abi, gpu, workloads, args, seeds, log_steps, stride, train_steps
     t0 = time.perf_counter()
     for sb, sk in seeds:
         case = workloads.CoupledCase(nx=args.nx, ny=args.ny, nz=args.nz, seed_bed=sb, seed_ks=sk)
-        case.setup_gpu(ctx, abi.PREC_FP64, abi.GUESS_ZERO)
+        case.setup_gpu(ctx, abi.PREC_FP64, abi.GUESS_ZERO, picard_max=picard_max)
         for _ in range(log_steps):
             ctx.coupled_step(case.cp, case.dt, case.rain, 0.0, case.t_end)
     ctx.synchronize()
@@ -129,7 +134,7 @@ def train_model(abi, gpu, workloads, args, seeds, log_steps, stride, train_steps
                    "log_seconds": round(t_log, 2), "train_seconds": round(t_train, 2)}
 
 
-def cpu_baseline(case, abi, steps):
+def cpu_baseline(case, abi, steps, picard_max=1):
     """The FP64 oracle on the same workload for `steps` coupled steps (bounded sample)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
@@ -137,7 +142,7 @@ def cpu_baseline(case, abi, steps):
     s = ol.OracleSwe(case.nx, case.ny, case.dx, case.dx, case.swe)
     s.set_bathymetry(case.bed_with_offset())
     s.initialize_state(-1.0e3)
-    r = ol.OracleRe(case.nx, case.ny, case.nz, case.re_params())
+    r = ol.OracleRe(case.nx, case.ny, case.nz, case.re_params(picard_max=picard_max))
     r.set_soil([case.soil], None, True, case.ks.ravel())
     r.set_initial(h=case.h0)
     oc = ol.OracleCoupled(s, r, case.cp, case.t_end)
@@ -210,7 +215,9 @@ def bench_learned_eval(args):
     pairs logged from plain FP64 runs of 20 seeded bed/Ks realisations (seed r for the bed, r + 100
     for Ks, r = 101..120; about 2e7 pairs), then run held-out realisations r = 201..205 with FP64
     Jacobi-PCG at the same tolerance from x0 = 0, from x0 = h^n + MLP increment and from x0 = h^n,
-    reporting Krylov (and Picard) iterations per step and the head difference to the zero guess."""
+    reporting Krylov (and Picard) iterations per step and the head difference to the zero guess: once
+    with the reference's predictor step (one Picard iterate) and once with the Picard loop to
+    1e-8 m, where the learned head is also the loop's starting iterate."""
     from frehg_amd import abi, gpu, workloads
     train_seeds = [(r, r + 100) for r in range(101, 121)]
     eval_seeds = [(r, r + 100) for r in range(201, 206)]
@@ -226,11 +233,13 @@ def bench_learned_eval(args):
         case = workloads.CoupledCase(nx=args.nx, ny=args.ny, nz=args.nz, seed_bed=sb, seed_ks=sk)
         row = {"seeds": [sb, sk]}
         heads = {}
-        for name, guess in (("zero", abi.GUESS_ZERO), ("learned", abi.GUESS_LEARNED), ("previous", abi.GUESS_PREVIOUS)):
+        runs = (("zero", abi.GUESS_ZERO, 1), ("learned", abi.GUESS_LEARNED, 1), ("previous", abi.GUESS_PREVIOUS, 1),
+                ("picard_zero", abi.GUESS_ZERO, 50), ("picard_learned", abi.GUESS_LEARNED, 50))
+        for name, guess, picard_max in runs:
             ctx = gpu.Context(0)
             if guess == abi.GUESS_LEARNED:
                 ctx.mlp_set_params(*model)
-            case.setup_gpu(ctx, abi.PREC_FP64, guess)
+            case.setup_gpu(ctx, abi.PREC_FP64, guess, picard_max=picard_max)
             its = pic = fb = 0
             for _ in range(args.steps):
                 st = ctx.coupled_step(case.cp, case.dt, case.rain, 0.0, case.t_end)
@@ -243,10 +252,14 @@ def bench_learned_eval(args):
                          "picard_iters_per_step": round(pic / args.steps, 3), "guess_fallbacks": fb}
         for name in ("learned", "previous"):
             row[name]["max_head_diff_vs_zero_guess_m"] = float(np.max(np.abs(heads[name] - heads["zero"])))
+        row["picard_learned"]["max_head_diff_vs_zero_guess_m"] = float(
+            np.max(np.abs(heads["picard_learned"] - heads["picard_zero"])))
         log(f"[bench]   held-out {sb}/{sk}: {row}")
         per_seed.append(row)
     mean = {name: round(float(np.mean([r[name]["krylov_iters_per_step"] for r in per_seed])), 3)
-            for name in ("zero", "learned", "previous")}
+            for name in ("zero", "learned", "previous", "picard_zero", "picard_learned")}
+    mean_picard = {name: round(float(np.mean([r[name]["picard_iters_per_step"] for r in per_seed])), 3)
+                   for name in ("picard_zero", "picard_learned")}
     out = {"metric": "Krylov iterations per coupled step on held-out realisations, learned vs zero initial guess",
            "value": mean["learned"], "unit": "iterations/step", "n_gpus": 1, "steps": args.steps, "warmup": 0,
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f16",
@@ -255,7 +268,8 @@ def bench_learned_eval(args):
                                   "FP64 Jacobi-PCG at rtol 1e-10 for every guess",
                       "train_realisations": [list(x) for x in train_seeds],
                       "held_out_realisations": [list(x) for x in eval_seeds]},
-           "mean_krylov_iters_per_step": mean, "per_realisation": per_seed, "training": training}
+           "mean_krylov_iters_per_step": mean, "mean_picard_iters_per_step": mean_picard,
+           "picard": {"max": 50, "tol_m": 1e-8}, "per_realisation": per_seed, "training": training}
     print(json.dumps(out), flush=True)
 
 
@@ -271,6 +285,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--log-steps", type=int, default=20)
+    ap.add_argument("--picard-max", type=int, default=50,
+                    help="Picard iterations per Richards step at most, to max |dh| <= 1e-8 m (1: the "
+                         "reference's predictor step, which is also measured and reported as predictor_step)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run an elementwise 1 GiB -> 1 GiB kernel (torch), the reference point that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -309,19 +326,41 @@ def main():
     cpu_steps = 0 if args.no_cpu else max(1, min(args.cpu_steps, args.warmup))
 
     log("[bench] plain FP64 path")
-    plain = run_gpu(case, abi, gpu, "plain", args.warmup, args.steps, grab_after=cpu_steps or None)
+    plain = run_gpu(case, abi, gpu, "plain", args.warmup, args.steps, grab_after=cpu_steps or None,
+                    picard_max=args.picard_max)
     log(f"[bench]   {plain['elapsed']:.3f} s, {plain['iters']} Krylov iterations")
     log("[bench] training the learned guess on two other seeded realisations (not timed)")
     model, training = train_model(abi, gpu, workloads, args, seeds=((101, 102), (103, 104)),
-                                  log_steps=args.log_steps, stride=4, train_steps=args.train_steps, batch=65536)
+                                  log_steps=args.log_steps, stride=4, train_steps=args.train_steps, batch=65536,
+                                  picard_max=args.picard_max)
     log(f"[bench]   {training['pairs']} pairs, loss {training['loss_first']} -> {training['loss_last']}")
     log("[bench] learned path (FP32-in-FP64 refinement, MLP guess)")
-    fast = run_gpu(case, abi, gpu, "learned", args.warmup, args.steps, model=model)
+    fast = run_gpu(case, abi, gpu, "learned", args.warmup, args.steps, model=model, picard_max=args.picard_max)
     log(f"[bench]   {fast['elapsed']:.3f} s, {fast['iters']} inner iterations, {fast['refines']} refinements, "
         f"{fast['fallbacks']} fallbacks")
     log("[bench] previous-step path (FP32-in-FP64 refinement, x0 = h^n)")
-    prev = run_gpu(case, abi, gpu, "previous", args.warmup, args.steps)
+    prev = run_gpu(case, abi, gpu, "previous", args.warmup, args.steps, picard_max=args.picard_max)
     log(f"[bench]   {prev['elapsed']:.3f} s, {prev['iters']} inner iterations, {prev['refines']} refinements")
+    predictor = None
+    if args.picard_max > 1:
+        log("[bench] the reference's predictor step (one Picard iterate): plain, learned, previous")
+        p1 = run_gpu(case, abi, gpu, "plain", args.warmup, args.steps, picard_max=1)
+        f1 = run_gpu(case, abi, gpu, "learned", args.warmup, args.steps, model=model, picard_max=1)
+        v1 = run_gpu(case, abi, gpu, "previous", args.warmup, args.steps, picard_max=1)
+        h1 = interior3(p1["h"], case.nx, case.ny, case.nz)
+
+        def summary(r, inner):
+            out = {"value": round(case.cells * args.steps / r["elapsed"], 1),
+                   "ms_per_step": round(r["elapsed"] / args.steps * 1e3, 4),
+                   ("inner_iters_per_step" if inner else "krylov_iters_per_step"): r["iters"] / args.steps}
+            if inner:
+                out["refinements_per_step"] = r["refines"] / args.steps
+                out["max_head_err_vs_plain_fp64_m"] = float(np.max(np.abs(h1 - interior3(r["h"], case.nx, case.ny,
+                                                                                          case.nz))))
+            return out
+        predictor = {"plain_fp64": summary(p1, False), "learned_mixed": summary(f1, True),
+                     "previous_mixed": summary(v1, True), "speedup_vs_plain_fp64": round(p1["elapsed"] / f1["elapsed"], 3)}
+        log(f"[bench]   {predictor}")
 
     hp = interior3(plain["h"], case.nx, case.ny, case.nz)
     hf = interior3(fast["h"], case.nx, case.ny, case.nz)
@@ -356,13 +395,14 @@ def main():
     cpu = None
     if cpu_steps:
         log(f"[bench] CPU baseline: FP64 oracle, {cpu_steps} coupled steps of the same workload")
-        cb = cpu_baseline(case, abi, cpu_steps)
+        cb = cpu_baseline(case, abi, cpu_steps, picard_max=args.picard_max)
         parity = float(np.max(np.abs(interior3(cb["h"], case.nx, case.ny, case.nz) -
                                      interior3(plain["grabbed"], case.nx, case.ny, case.nz))))
         cpu = {"value": round(case.cells * cpu_steps / cb["elapsed"], 1), "unit": "cell-steps/s",
                "cores": cb["threads"], "kind": "port",
                "sample": f"first {cpu_steps} coupled steps of the same {args.nx}x{args.ny}x{args.nz} workload "
-                         f"({cb['elapsed']:.1f} s), FP64 oracle with Jacobi-PCG at the reference rtol",
+                         f"({cb['elapsed']:.1f} s), FP64 oracle with Jacobi-PCG at the reference rtol, the same "
+                         f"Picard settings as the plain path",
                "gpu_fp64_max_head_err_vs_oracle_m": parity}
         log(f"[bench]   {cb['elapsed']:.1f} s on {cb['threads']} threads; GPU FP64 vs oracle max |dh| = {parity:.3e} m")
 
@@ -384,9 +424,15 @@ def main():
         "config": {"workload": f"coupled SW {args.nx}x{args.ny} + GW {args.nx}x{args.ny}x{args.nz}, dx 1 m, "
                                f"dz 0.05 m, dt 1 s, 50 mm/h rain, closed sides",
                    "cells_per_step": case.cells, "fast_path": "fp32-in-fp64 refinement, learned (MLP) guess",
+                   "richards_step": (f"Picard loop to max |dh| <= 1e-8 m (at most {args.picard_max} iterates); "
+                                     "plain path: FP64 PCG from x0 = 0 in every iterate; learned path: the MLP "
+                                     "head as the first iterate and Krylov guess, later iterates warm-started"
+                                     if args.picard_max > 1 else "the reference's predictor step (one iterate)"),
                    "gw_rtol": 1e-10, "sw_rtol": 1e-12},
         "plain_fp64": {"value": round(plain_value, 1), "ms_per_step": round(plain["elapsed"] / args.steps * 1e3, 4),
-                       "krylov_iters_per_step": plain["iters"] / args.steps},
+                       "krylov_iters_per_step": plain["iters"] / args.steps,
+                       "picard_iters_per_step": plain["picards"] / args.steps},
+        "fast_picard_iters_per_step": fast["picards"] / args.steps,
         "speedup_vs_plain_fp64": round(value / plain_value, 3),
         "max_head_err_vs_plain_fp64_m": head_err,
         "fast_inner_iters_per_step": fast["iters"] / args.steps,
@@ -400,6 +446,7 @@ def main():
                                  "refinements_per_step": prev["refines"] / args.steps,
                                  "max_head_err_vs_plain_fp64_m": head_err_prev},
         "training": training,
+        "predictor_step": predictor,
         "roofline": roof64,
         "roofline_kernel": "FP64 CG SpMV of the plain path (k_spmv_pair<double> at the autotuned shape): "
                            "fused p = z + beta p_prev, 7-point q = A p, partial p.q (64 B per cell)",

@@ -206,17 +206,22 @@ __global__ __launch_bounds__(NT) void k_re_dtadapt(ReK P, const double* qx, cons
   }
 }
 
-// Picard test: per-block max |h - ref| over interior cells to part[0..nblocks) (part[nblocks..)
-// = 0 for k_reduce_maxmin), and hm = h (the iterate the next test compares against; hm may be
-// ref itself).
-__global__ __launch_bounds__(NT) void k_picard_delta(Geo g, const double* ref, const double* h, double* hm,
-                                                      double* part, unsigned int nblocks) {
-  const Cell t = cell_of(g);
+// k_re_post at the Picard iterate h, with the Picard test: per-block max |h - ref| over interior
+// cells to part[0..nblocks) (part[nblocks..) = 0 for k_reduce_maxmin), and hm = h (the iterate
+// the next test compares against; hm may be ref itself).
+template <class K, int ZF>
+__global__ __launch_bounds__(NT) void k_re_post_delta(K P, double* __restrict__ h, double* __restrict__ kr,
+                                                       double* __restrict__ ch, double* __restrict__ wch,
+                                                       const double* ref, double* hm, double* part,
+                                                       unsigned int nblocks) {
+  TileId b;
+  const Cell t = tile_cell<ZF>(P.g, b);
   double d = 0.0;
   if (t.valid) {
     const double hv = h[t.c];
     d = fabs(hv - ref[t.c]);
     hm[t.c] = hv;
+    re_post(P, t, h, kr, ch, wch);
   }
   d = wave_max(d);
   __shared__ double lmax[NT / kWave];
@@ -225,9 +230,8 @@ __global__ __launch_bounds__(NT) void k_picard_delta(Geo g, const double* ref, c
   if (threadIdx.x == 0) {
     double m = lmax[0];
     for (int q = 1; q < NT / kWave; ++q) m = fmax(m, lmax[q]);
-    const unsigned int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    part[bid] = m;
-    part[nblocks + bid] = 0.0;
+    part[b.linear] = m;
+    part[nblocks + b.linear] = 0.0;
   }
 }
 
@@ -500,21 +504,27 @@ void ReState::step(frehg_re_stats* st) {
   const bool kc_fresh = kc_fresh_;
   kc_fresh_ = false;  // set again once this step has completed
   flux_pending_ = false;  // this step's fluxes supersede the previous step's
+  // With the Picard loop the learned head is the loop's starting iterate h^0 (below): the system
+  // is assembled at it, not at h^n, whose K and C serve the features only.
+  const bool learned_h0 = p_.initial_guess == FREHG_GUESS_LEARNED && p_.picard_max > 1;
   {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_ASSEMBLE);
     hipEvent_t ev1 = prof_->stop();
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
+      hipEvent_t e1 = learned_h0 ? ev1 : nullptr;
       if (kc_fresh)
-        hipExtLaunchKernelGGL((k_re_pre_stale<K, 1>), grid, blk, 0, s_, ev0, nullptr, 0, PK,
+        hipExtLaunchKernelGGL((k_re_pre_stale<K, 1>), grid, blk, 0, s_, ev0, e1, 0, PK,
                               (const unsigned char*)hmod_, (const double*)hn_, kr_, ch_);
       else
-        hipExtLaunchKernelGGL((k_re_pre<K, 1>), grid, blk, 0, s_, ev0, nullptr, 0, PK, (const double*)hn_, kr_, ch_);
+        hipExtLaunchKernelGGL((k_re_pre<K, 1>), grid, blk, 0, s_, ev0, e1, 0, PK, (const double*)hn_, kr_, ch_);
       FREHG_LAUNCH_CHECK();
-      hipExtLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, nullptr, ev1, 0, PK, (const double*)kr_,
-                            (const double*)ch_, (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_,
-                            mixed ? D32_ : nullptr, ax32_, ay32_, az32_);
-      FREHG_LAUNCH_CHECK();
+      if (!learned_h0) {
+        hipExtLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, nullptr, ev1, 0, PK, (const double*)kr_,
+                              (const double*)ch_, (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_,
+                              mixed ? D32_ : nullptr, ax32_, ay32_, az32_);
+        FREHG_LAUNCH_CHECK();
+      }
     });
   }
   // Initial guess in h (the solve's x): zero (PETSc default), h^n, or h^n + MLP increment.
@@ -529,10 +539,8 @@ void ReState::step(frehg_re_stats* st) {
     x0_zero = false;
     guard = true;
   }
-  // With the Picard loop the learned head is also the loop's starting iterate h^0: K and C at it
-  // (with its head-BC ghosts), the assembly with the time-level-n terms, and the first stopping
-  // test against it (kept in hp_). The features above were taken at h^n.
-  const bool learned_h0 = p_.initial_guess == FREHG_GUESS_LEARNED && p_.picard_max > 1;
+  // The learned head as the Picard loop's h^0: K and C at it (with its head-BC ghosts), the
+  // assembly with the time-level-n terms, and the first stopping test against it (kept in hp_).
   if (learned_h0) {
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
@@ -562,10 +570,18 @@ void ReState::step(frehg_re_stats* st) {
   // K and C at the iterate h^m with its head-BC ghosts, the storage and rhs terms at time level n
   // as in the predictor, solve for h^{m+1} (from x0 = 0 with the zero guess, else from h^m); stop
   // when max |h^{m+1} - h^m| <= picard_tol (h^0 = h^n, or the learned head).
-  for (int m = 1; m < std::max(1, p_.picard_max); ++m) {
+  // The post-solve kernel (head BCs, K, C and theta at the iterate) runs at every iterate with
+  // the test fused in; at the last one it is the step's own post-solve evaluation.
+  bool post_done = false;
+  for (int m = 1; p_.picard_max > 1; ++m) {
     const double* ref = (m == 1 && !learned_h0) ? hn_ : hp_;
-    hipLaunchKernelGGL(k_picard_delta, grid, blk, 0, s_, g_, ref, (const double*)h_, hp_, red_, nbl);
-    FREHG_LAUNCH_CHECK();
+    with_soil_variant(P, [&](const auto& PK) {
+      using K = std::decay_t<decltype(PK)>;
+      hipLaunchKernelGGL((k_re_post_delta<K, 1>), grid, blk, 0, s_, PK, h_, kr_, ch_, wch_, ref, hp_, red_, nbl);
+      FREHG_LAUNCH_CHECK();
+    });
+    post_done = true;
+    if (m >= p_.picard_max) break;
     hipLaunchKernelGGL(k_reduce_maxmin, dim3(1), dim3(kRedThreads), 0, s_, (const double*)red_, nbl, dres_);
     FREHG_LAUNCH_CHECK();
     FREHG_HIP(hipMemcpyAsync(hres_, dres_, sizeof(double), hipMemcpyDeviceToHost, s_));
@@ -573,13 +589,12 @@ void ReState::step(frehg_re_stats* st) {
     if (hres_[0] <= p_.picard_tol) break;
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
-      hipLaunchKernelGGL((k_re_post<K, 1>), grid, blk, 0, s_, PK, h_, kr_, ch_, wch_);
-      FREHG_LAUNCH_CHECK();
       hipLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
                          (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, mixed ? D32_ : nullptr,
                          ax32_, ay32_, az32_);
       FREHG_LAUNCH_CHECK();
     });
+    post_done = false;
     const bool zero = p_.initial_guess == FREHG_GUESS_ZERO;
     const SolveResult sm = kry_->solve(A, mixed ? &A32 : nullptr, rhs_, nullptr, h_, nullptr, zero, false, kc);
     local.solve.iterations += sm.iterations;
@@ -592,12 +607,14 @@ void ReState::step(frehg_re_stats* st) {
   }
   if (mlp_ && mlp_->logging()) {
     // the features are K and C at h^n: the Picard iterates replaced them
-    if (p_.picard_max > 1 || learned_h0)
+    if (p_.picard_max > 1) {
       with_soil_variant(P, [&](const auto& PK) {
         using K = std::decay_t<decltype(PK)>;
         hipLaunchKernelGGL((k_re_pre<K, 1>), grid, blk, 0, s_, PK, (const double*)hn_, kr_, ch_);
         FREHG_LAUNCH_CHECK();
       });
+      post_done = false;
+    }
     mlp_->log_pair(*this, hn_, h_, depth_);
   }
   {
@@ -605,9 +622,12 @@ void ReState::step(frehg_re_stats* st) {
     hipEvent_t ev1 = prof_->stop();
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
-      hipExtLaunchKernelGGL((k_re_post<K, 1>), grid, blk, 0, s_, ev0, nullptr, 0, PK, h_, kr_, ch_, wch_);
-      FREHG_LAUNCH_CHECK();
-      hipExtLaunchKernelGGL((k_re_flux_correct<K, 1>), grid, blk, 0, s_, nullptr, ev1, 0, PK, (const double*)h_,
+      if (!post_done) {
+        hipExtLaunchKernelGGL((k_re_post<K, 1>), grid, blk, 0, s_, ev0, nullptr, 0, PK, h_, kr_, ch_, wch_);
+        FREHG_LAUNCH_CHECK();
+      }
+      hipExtLaunchKernelGGL((k_re_flux_correct<K, 1>), grid, blk, 0, s_, post_done ? ev0 : nullptr, ev1, 0, PK,
+                            (const double*)h_,
                             (const double*)kr_, (const double*)hn_, (const double*)wcn_, (const double*)wch_, Kx_, Ky_,
                             Kz_, qx_, qy_, qz_, wc_, hwc_, sat0_, cand_, lazy_flux_ ? kf_ : nullptr, led_, nbl,
                             dflag_, kSweepFlags);

@@ -54,11 +54,13 @@ class CoupledCase:
         # closed sides and bottom; the exchange is the only top flux (bc z- = 0)
         self.bc = (0, 0, 0, 0, 0, 0)
 
-    def re_params(self, precision=abi.PREC_FP64, guess=abi.GUESS_ZERO, rtol=1e-10, inner_rtol=1e-5):
+    def re_params(self, precision=abi.PREC_FP64, guess=abi.GUESS_ZERO, rtol=1e-10, inner_rtol=1e-5, picard_max=1,
+                  picard_tol=1e-8):
         return abi.re_params(soil=self.soil, solver=abi.solver_config(rtol, precision=precision, inner_rtol=inner_rtol,
                                                                       max_refine=8),
                              bc_type=self.bc, dx=self.dx, dy=self.dx, dz=self.dz, botz=-self.dz * self.nz, dt=self.dt,
-                             adaptive_dt=0, use_full3d=1, initial_guess=guess)
+                             adaptive_dt=0, use_full3d=1, initial_guess=guess, picard_max=picard_max,
+                             picard_tol=picard_tol)
 
     @property
     def cells(self):
@@ -67,11 +69,11 @@ class CoupledCase:
     def bed_with_offset(self):
         return (self.bed + self.swe.offset).ravel()
 
-    def setup_gpu(self, ctx, precision=abi.PREC_FP64, guess=abi.GUESS_ZERO):
+    def setup_gpu(self, ctx, precision=abi.PREC_FP64, guess=abi.GUESS_ZERO, picard_max=1):
         ctx.swe_init(self.nx, self.ny, self.dx, self.dx, self.swe)
         ctx.swe_set_bathymetry(self.bed_with_offset())
         ctx.swe_initialize_state(-1.0e3)  # dry start: eta clamps to the bed
-        ctx.re_init(self.nx, self.ny, self.nz, self.re_params(precision, guess))
+        ctx.re_init(self.nx, self.ny, self.nz, self.re_params(precision, guess, picard_max=picard_max))
         ctx.re_set_soil([self.soil], None, True, self.ks.ravel())
         ctx.re_set_initial(h=self.h0)
 
