@@ -544,15 +544,15 @@ void ReState::step(frehg_re_stats* st) {
   if (learned_h0) {
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
-      hipLaunchKernelGGL((k_re_post<K, 1>), grid, blk, 0, s_, PK, h_, kr_, ch_, wch_);
+      // post-solve pass at h^0, which also keeps h^0 in hp_ (its test output is not used)
+      hipLaunchKernelGGL((k_re_post_delta<K, 1>), grid, blk, 0, s_, PK, h_, kr_, ch_, wch_, (const double*)hn_, hp_,
+                         red_, nbl);
       FREHG_LAUNCH_CHECK();
       hipLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
                          (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, mixed ? D32_ : nullptr,
                          ax32_, ay32_, az32_);
       FREHG_LAUNCH_CHECK();
     });
-    hipLaunchKernelGGL(k_copy_interior, grid, blk, 0, s_, g_, (const double*)h_, hp_);
-    FREHG_LAUNCH_CHECK();
   }
   const KrylovConfig kc = krylov_config(p_.solver, "frehg_re_step");
   StencilView<double> A{D_, ax_, ay_, az_};
