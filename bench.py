@@ -213,14 +213,15 @@ def bench_ensemble(args):
 def bench_learned_eval(args):
     """Held-out evaluation of the learned guess on the coupled 512x512 + 512x512x30 workload: train on
     pairs logged from plain FP64 runs of 20 seeded bed/Ks realisations (seed r for the bed, r + 100
-    for Ks, r = 101..120; about 2e7 pairs), then run held-out realisations r = 201..205 with FP64
-    Jacobi-PCG at the same tolerance from x0 = 0, from x0 = h^n + MLP increment and from x0 = h^n,
+    for Ks, r = 101..120; about 2e7 pairs), then run held-out realisations r = 301..305 (none of their
+    bed or Ks seeds is used in training) with FP64 Jacobi-PCG at the same tolerance from x0 = 0, from
+    x0 = h^n + MLP increment and from x0 = h^n,
     reporting Krylov (and Picard) iterations per step and the head difference to the zero guess: once
     with the reference's predictor step (one Picard iterate) and once with the Picard loop to
     1e-8 m, where the learned head is also the loop's starting iterate."""
     from frehg_amd import abi, gpu, workloads
     train_seeds = [(r, r + 100) for r in range(101, 121)]
-    eval_seeds = [(r, r + 100) for r in range(201, 206)]
+    eval_seeds = [(r, r + 100) for r in range(301, 306)]
     ncell = args.nx * args.ny * args.nz
     log_steps = 10
     stride = max(1, (len(train_seeds) * log_steps * ncell) // 20_000_000)
