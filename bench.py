@@ -63,6 +63,8 @@ def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None, pi
     elif path == "learned":
         ctx.mlp_set_params(*model)
         case.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_LEARNED, picard_max=picard_max)
+    elif path == "extrapolated":
+        case.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_EXTRAPOLATED, picard_max=picard_max)
     else:
         case.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_PREVIOUS, picard_max=picard_max)
     grabbed = None
@@ -215,16 +217,17 @@ def bench_learned_eval(args):
     pairs logged from plain FP64 runs of 20 seeded bed/Ks realisations (seed r for the bed, r + 100
     for Ks, r = 101..120; about 2e7 pairs), then run held-out realisations r = 301..305 (none of their
     bed or Ks seeds is used in training) with FP64 Jacobi-PCG at the same tolerance from x0 = 0, from
-    x0 = h^n + MLP increment and from x0 = h^n,
-    reporting Krylov (and Picard) iterations per step and the head difference to the zero guess: once
-    with the reference's predictor step (one Picard iterate) and once with the Picard loop to
-    1e-8 m, where the learned head is also the loop's starting iterate."""
+    x0 = h^n + the previous step's solve increment + the MLP's correction, from x0 = h^n and from
+    x0 = h^n + the previous step's solve increment, reporting Krylov (and Picard) iterations per step
+    and the head difference to the zero guess: once with the reference's predictor step (one Picard
+    iterate) and once with the Picard loop to 1e-8 m, where the starting head is also the loop's
+    first iterate."""
     from frehg_amd import abi, gpu, workloads
     train_seeds = [(r, r + 100) for r in range(101, 121)]
     eval_seeds = [(r, r + 100) for r in range(301, 306)]
     ncell = args.nx * args.ny * args.nz
-    log_steps = 10
-    stride = max(1, (len(train_seeds) * log_steps * ncell) // 20_000_000)
+    log_steps = 11  # the first step of each run logs no pairs (no previous solve increment)
+    stride = max(1, (len(train_seeds) * (log_steps - 1) * ncell) // 20_000_000)
     log(f"[bench] training on {len(train_seeds)} realisations, {log_steps} logged steps each, cell stride {stride}")
     model, training = train_model(abi, gpu, workloads, args, seeds=train_seeds, log_steps=log_steps, stride=stride,
                                   train_steps=args.train_steps, batch=65536)
@@ -235,7 +238,9 @@ def bench_learned_eval(args):
         row = {"seeds": [sb, sk]}
         heads = {}
         runs = (("zero", abi.GUESS_ZERO, 1), ("learned", abi.GUESS_LEARNED, 1), ("previous", abi.GUESS_PREVIOUS, 1),
-                ("picard_zero", abi.GUESS_ZERO, 50), ("picard_learned", abi.GUESS_LEARNED, 50))
+                ("extrapolated", abi.GUESS_EXTRAPOLATED, 1), ("picard_zero", abi.GUESS_ZERO, 50),
+                ("picard_learned", abi.GUESS_LEARNED, 50), ("picard_previous", abi.GUESS_PREVIOUS, 50),
+                ("picard_extrapolated", abi.GUESS_EXTRAPOLATED, 50))
         for name, guess, picard_max in runs:
             ctx = gpu.Context(0)
             if guess == abi.GUESS_LEARNED:
@@ -251,16 +256,16 @@ def bench_learned_eval(args):
             ctx.close()
             row[name] = {"krylov_iters_per_step": round(its / args.steps, 3),
                          "picard_iters_per_step": round(pic / args.steps, 3), "guess_fallbacks": fb}
-        for name in ("learned", "previous"):
+        for name in ("learned", "previous", "extrapolated"):
             row[name]["max_head_diff_vs_zero_guess_m"] = float(np.max(np.abs(heads[name] - heads["zero"])))
-        row["picard_learned"]["max_head_diff_vs_zero_guess_m"] = float(
-            np.max(np.abs(heads["picard_learned"] - heads["picard_zero"])))
+        for name in ("picard_learned", "picard_previous", "picard_extrapolated"):
+            row[name]["max_head_diff_vs_zero_guess_m"] = float(np.max(np.abs(heads[name] - heads["picard_zero"])))
         log(f"[bench]   held-out {sb}/{sk}: {row}")
         per_seed.append(row)
-    mean = {name: round(float(np.mean([r[name]["krylov_iters_per_step"] for r in per_seed])), 3)
-            for name in ("zero", "learned", "previous", "picard_zero", "picard_learned")}
+    names = [r[0] for r in runs]
+    mean = {name: round(float(np.mean([r[name]["krylov_iters_per_step"] for r in per_seed])), 3) for name in names}
     mean_picard = {name: round(float(np.mean([r[name]["picard_iters_per_step"] for r in per_seed])), 3)
-                   for name in ("picard_zero", "picard_learned")}
+                   for name in names if name.startswith("picard")}
     out = {"metric": "Krylov iterations per coupled step on held-out realisations, learned vs zero initial guess",
            "value": mean["learned"], "unit": "iterations/step", "n_gpus": 1, "steps": args.steps, "warmup": 0,
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f16",
@@ -342,12 +347,16 @@ def main():
     log("[bench] previous-step path (FP32-in-FP64 refinement, x0 = h^n)")
     prev = run_gpu(case, abi, gpu, "previous", args.warmup, args.steps, picard_max=args.picard_max)
     log(f"[bench]   {prev['elapsed']:.3f} s, {prev['iters']} inner iterations, {prev['refines']} refinements")
+    log("[bench] extrapolated path (FP32-in-FP64 refinement, x0 = h^n + the previous step's solve increment)")
+    ext = run_gpu(case, abi, gpu, "extrapolated", args.warmup, args.steps, picard_max=args.picard_max)
+    log(f"[bench]   {ext['elapsed']:.3f} s, {ext['iters']} inner iterations, {ext['refines']} refinements")
     predictor = None
     if args.picard_max > 1:
-        log("[bench] the reference's predictor step (one Picard iterate): plain, learned, previous")
+        log("[bench] the reference's predictor step (one Picard iterate): plain, learned, previous, extrapolated")
         p1 = run_gpu(case, abi, gpu, "plain", args.warmup, args.steps, picard_max=1)
         f1 = run_gpu(case, abi, gpu, "learned", args.warmup, args.steps, model=model, picard_max=1)
         v1 = run_gpu(case, abi, gpu, "previous", args.warmup, args.steps, picard_max=1)
+        e1 = run_gpu(case, abi, gpu, "extrapolated", args.warmup, args.steps, picard_max=1)
         h1 = interior3(p1["h"], case.nx, case.ny, case.nz)
 
         def summary(r, inner):
@@ -360,13 +369,15 @@ def main():
                                                                                           case.nz))))
             return out
         predictor = {"plain_fp64": summary(p1, False), "learned_mixed": summary(f1, True),
-                     "previous_mixed": summary(v1, True), "speedup_vs_plain_fp64": round(p1["elapsed"] / f1["elapsed"], 3)}
+                     "previous_mixed": summary(v1, True), "extrapolated_mixed": summary(e1, True),
+                     "speedup_vs_plain_fp64": round(p1["elapsed"] / f1["elapsed"], 3)}
         log(f"[bench]   {predictor}")
 
     hp = interior3(plain["h"], case.nx, case.ny, case.nz)
     hf = interior3(fast["h"], case.nx, case.ny, case.nz)
     head_err = float(np.max(np.abs(hp - hf)))
     head_err_prev = float(np.max(np.abs(hp - interior3(prev["h"], case.nx, case.ny, case.nz))))
+    head_err_ext = float(np.max(np.abs(hp - interior3(ext["h"], case.nx, case.ny, case.nz))))
 
     def roof(kt_ms_n, bytes_per_cell):
         ms, n = kt_ms_n
@@ -426,8 +437,9 @@ def main():
                                f"dz 0.05 m, dt 1 s, 50 mm/h rain, closed sides",
                    "cells_per_step": case.cells, "fast_path": "fp32-in-fp64 refinement, learned (MLP) guess",
                    "richards_step": (f"Picard loop to max |dh| <= 1e-8 m (at most {args.picard_max} iterates); "
-                                     "plain path: FP64 PCG from x0 = 0 in every iterate; learned path: the MLP "
-                                     "head as the first iterate and Krylov guess, later iterates warm-started"
+                                     "plain path: FP64 PCG from x0 = 0 in every iterate; learned path: h^n + the "
+                                     "previous step's solve increment + the MLP's correction as the first iterate "
+                                     "and Krylov guess, later iterates warm-started"
                                      if args.picard_max > 1 else "the reference's predictor step (one iterate)"),
                    "gw_rtol": 1e-10, "sw_rtol": 1e-12},
         "plain_fp64": {"value": round(plain_value, 1), "ms_per_step": round(plain["elapsed"] / args.steps * 1e3, 4),
@@ -440,12 +452,19 @@ def main():
         "fast_refinements_per_step": fast["refines"] / args.steps,
         "fast_guess_fallbacks": fast["fallbacks"],
         "max_rel_mass_balance_error": {"learned": fast["mass_err"], "plain_fp64": plain["mass_err"],
-                                       "previous": prev["mass_err"]},
+                                       "previous": prev["mass_err"], "extrapolated": ext["mass_err"]},
         "previous_guess_mixed": {"value": round(case.cells * args.steps / prev["elapsed"], 1),
                                  "ms_per_step": round(prev["elapsed"] / args.steps * 1e3, 4),
+                                 "picard_iters_per_step": prev["picards"] / args.steps,
                                  "inner_iters_per_step": prev["iters"] / args.steps,
                                  "refinements_per_step": prev["refines"] / args.steps,
                                  "max_head_err_vs_plain_fp64_m": head_err_prev},
+        "extrapolated_guess_mixed": {"value": round(case.cells * args.steps / ext["elapsed"], 1),
+                                     "ms_per_step": round(ext["elapsed"] / args.steps * 1e3, 4),
+                                     "picard_iters_per_step": ext["picards"] / args.steps,
+                                     "inner_iters_per_step": ext["iters"] / args.steps,
+                                     "refinements_per_step": ext["refines"] / args.steps,
+                                     "max_head_err_vs_plain_fp64_m": head_err_ext},
         "training": training,
         "predictor_step": predictor,
         "roofline": roof64,

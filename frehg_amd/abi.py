@@ -28,6 +28,7 @@ PREC_MIXED = 1
 GUESS_ZERO = 0
 GUESS_PREVIOUS = 1
 GUESS_LEARNED = 2
+GUESS_EXTRAPOLATED = 3
 
 KT_GW_SPMV = 0
 KT_GW_SPMV32 = 1

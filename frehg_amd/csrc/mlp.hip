@@ -79,6 +79,9 @@ struct FeatSrc {
   const double* wcn;
   SoilView soil;
   const double* depth;  // surface padded, or null
+  // Base increment per cell (the last step's solve increment): the network predicts the
+  // increment minus it. Null = zero.
+  const double* inc0;
 };
 
 __device__ inline void cell_features(const FeatSrc& F, int i, int j, int k, float* f) {
@@ -427,6 +430,7 @@ __device__ __forceinline__ float act_r(float z) { return __builtin_amdgcn_rcpf(1
 
 __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const float* norm, FeatSrc F, double* h_out) {
   __shared__ __attribute__((aligned(16))) float xs[GW][RB][FP];
+  __shared__ float ys[GW][RB];  // network outputs of the chunk's cells
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int ra = lane & 15, g = lane >> 4, kg = 8 * g;
   // A operands (rows = neurons): W1s[16 nb + ra][kg + t] (kg < 16); W2s[16 nb + ra][pi(32 ks + kg + t)];
@@ -472,7 +476,11 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const flo
     float f[kMlpIn];
 #pragma unroll
     for (int t = 0; t < kMlpIn; ++t) f[t] = 0.0f;
+    long long c = 0;
+    double base = 0.0;  // h^n + base increment of this lane's cell
     if (i < g3.nx) {
+      c = g3.idx(i, j, k);
+      base = F.hn[c] + (F.inc0 ? F.inc0[c] : 0.0);
       cell_features(F, i, j, k, f);
 #pragma unroll
       for (int t = 0; t < kMlpIn; ++t) f[t] = (f[t] - nmu[t]) * nisd[t];
@@ -520,18 +528,15 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const flo
       f32x4 y = b3r;
 #pragma unroll
       for (int ks = 0; ks < 2; ++ks) y = mfma16(a3w[ks], hb[ks], y);
-      if (lane < 16) {  // D row 0 (g = 0, r = 0) holds y of cell ra
-        const int ii = ib * RB + 16 * tile + ra;
-        if (ii < g3.nx) {
-          const long long c = g3.idx(ii, j, k);
-          h_out[c] = F.hn[c] + (double)(y[0] * ysd + ymu);
-        }
-      }
+      if (lane < 16) ys[w][16 * tile + ra] = y[0];  // D row 0 (g = 0, r = 0) holds y of cell ra
     }
+    wave_lds_fence();
+    if (i < g3.nx) h_out[c] = base + (double)(ys[w][lane] * ysd + ymu);
   }
 }
 
-// Log (features, h - h^n) of interior cells q with q % stride == phase into the ring buffer.
+// Log (features, h - h^n - base increment) of interior cells q with q % stride == phase into the
+// ring buffer.
 __global__ __launch_bounds__(MT) void k_mlp_log(FeatSrc F, const double* h, int stride, int phase, long long base,
                                                  long long cap, float* log) {
   const Geo& g = F.g;
@@ -546,7 +551,7 @@ __global__ __launch_bounds__(MT) void k_mlp_log(FeatSrc F, const double* h, int 
 #pragma unroll
   for (int t = 0; t < kMlpIn; ++t) row[t] = f[t];
   const long long c = g.idx(i, j, k);
-  row[kMlpIn] = (float)(h[c] - F.hn[c]);
+  row[kMlpIn] = (float)(h[c] - F.hn[c] - (F.inc0 ? F.inc0[c] : 0.0));
 }
 
 // Per-block sums and sums of squares of the 12 logged columns (FP64).
@@ -579,6 +584,7 @@ FeatSrc feat_src(ReState& re, const double* depth) {
   F.wcn = re.d_wcn();
   F.soil = re.soil_view();
   F.depth = depth;
+  F.inc0 = nullptr;
   return F;
 }
 
@@ -648,10 +654,11 @@ void Mlp::set_logging(int stride, long long capacity) {
   stride_ = stride;
 }
 
-void Mlp::log_pair(ReState& re, const double* hn, const double* h, const double* depth) {
+void Mlp::log_pair(ReState& re, const double* hn, const double* h, const double* depth, const double* inc0) {
   if (!logging()) return;
   FeatSrc F = feat_src(re, depth);
   F.hn = hn;
+  F.inc0 = inc0;
   const Geo& g = F.g;
   const long long N = (long long)g.nx * g.ny * g.nz;
   const int phase = (int)(log_steps_ % stride_);
@@ -722,9 +729,11 @@ double Mlp::train(int steps, int batch, double lr, uint64_t seed, double* first)
   return loss;
 }
 
-void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const double* hn, const double* wcn) {
+void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const double* inc0, const double* hn,
+                        const double* wcn) {
   if (!ready_) throw ArgError("learned guess requested but the model is not trained or loaded");
   FeatSrc F = feat_src(re, depth);
+  F.inc0 = inc0;
   if (hn) F.hn = hn;
   if (wcn) F.wcn = wcn;
   const long long N = (long long)F.g.nx * F.g.ny * F.g.nz;

@@ -51,15 +51,17 @@ class Mlp {
   bool logging() const { return stride_ > 0; }
   long long logged() const { return count_ < cap_ ? count_ : cap_; }
   void clear_log() { count_ = 0; }
-  void log_pair(ReState& re, const double* hn, const double* h, const double* depth);
+  // The logged target is h - h^n - inc0 (inc0: the base increment the guess adds, or null).
+  void log_pair(ReState& re, const double* hn, const double* h, const double* depth, const double* inc0);
 
   // `steps` Adam steps (lr, beta1 0.9, beta2 0.999, eps 1e-8) on minibatches of `batch` rows
   // drawn with replacement from the log. The first call computes the normalisation from the log.
   // Returns the mean standardised squared error of the last minibatch (first one in *first).
   double train(int steps, int batch, double lr, uint64_t seed, double* first);
 
-  // h_out (interior) = h^n + predicted increment; hn / wcn default to the solver's h^n / theta^n.
-  void predict_guess(ReState& re, double* h_out, const double* depth, const double* hn = nullptr,
+  // h_out (interior) = h^n + inc0 + predicted increment (inc0 null: zero); hn / wcn default to
+  // the solver's h^n / theta^n.
+  void predict_guess(ReState& re, double* h_out, const double* depth, const double* inc0, const double* hn = nullptr,
                      const double* wcn = nullptr);
 
   void get_params(float* params, float* norm);

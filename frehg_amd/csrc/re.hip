@@ -93,8 +93,9 @@ __global__ __launch_bounds__(NT) void k_re_flux_correct(K P, const double* __res
                                                          double* __restrict__ qz, double* __restrict__ wc,
                                                          double* __restrict__ hwc, unsigned char* __restrict__ sat0,
                                                          unsigned char* __restrict__ cand,
-                                                         unsigned char* __restrict__ kf, double* __restrict__ part,
-                                                         unsigned int nblocks, int* __restrict__ flags, int nflags) {
+                                                         unsigned char* __restrict__ kf, double* __restrict__ dh,
+                                                         double* __restrict__ part, unsigned int nblocks,
+                                                         int* __restrict__ flags, int nflags) {
   __shared__ double lds[NT / kWave];
   TileId b;
   const Cell t = tile_cell<ZF>(P.g, b);
@@ -103,6 +104,8 @@ __global__ __launch_bounds__(NT) void k_re_flux_correct(K P, const double* __res
   if (t.valid)
     re_flux_correct(P, t, h, kr, hn, wcn, wch, Kx, Ky, Kz, qx, qy, qz, wc, hwc, sat0, cand, kf, l_before, l_ss,
                     l_bnd);
+  // the step's solve increment, the extrapolated start of the next step
+  if (dh && t.valid) dh[t.c] = h[t.c] - hn[t.c];
   const double a = block_sum<NT>(l_before, lds);
   const double bs = block_sum<NT>(l_ss, lds);
   const double d = block_sum<NT>(l_bnd, lds);
@@ -248,6 +251,12 @@ __global__ __launch_bounds__(NT) void k_copy_interior(Geo g, const double* src, 
   dst[t.c] = src[t.c];
 }
 
+__global__ __launch_bounds__(NT) void k_add_interior(Geo g, const double* a, const double* b, double* dst) {
+  const Cell t = cell_of(g);
+  if (!t.valid) return;
+  dst[t.c] = a[t.c] + b[t.c];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -291,10 +300,17 @@ ReState::~ReState() {
                   (void*)D_, (void*)ax_, (void*)ay_, (void*)az_, (void*)rhs_, (void*)D32_, (void*)ax32_,
                   (void*)ay32_, (void*)az32_, (void*)sat0_, (void*)cand_, (void*)satA_, (void*)satB_, (void*)hmod_,
                   (void*)kf_, (void*)dflag_, (void*)red_, (void*)red2_, (void*)led_, (void*)dres_, (void*)dcount_, (void*)classes_, (void*)cls_,
-                  (void*)ks_, (void*)qtop_})
+                  (void*)ks_, (void*)qtop_, (void*)dhp_})
     if (f) (void)hipFree(f);
   if (hflag_) (void)hipHostFree(hflag_);
   if (hres_) (void)hipHostFree(hres_);
+}
+
+void ReState::ensure_dhp() {
+  if (dhp_) return;
+  FREHG_HIP(hipMalloc(&dhp_, (size_t)n_ * sizeof(double)));
+  FREHG_HIP(hipMemsetAsync(dhp_, 0, (size_t)n_ * sizeof(double), s_));
+  dhp_valid_ = false;
 }
 
 void ReState::set_params(const frehg_re_params& p) {
@@ -307,7 +323,10 @@ void ReState::set_params(const frehg_re_params& p) {
   if (!(p.dx > 0 && p.dy > 0 && p.dz > 0 && p.dt > 0)) throw ArgError("frehg_re: dx, dy, dz, dt must be > 0");
   if (p.picard_max < 1) throw ArgError("frehg_re: picard_max must be >= 1 (1 = the reference's predictor)");
   if (p.picard_max > 1 && !(p.picard_tol > 0.0)) throw ArgError("frehg_re: picard_tol must be > 0");
+  if (p.initial_guess < FREHG_GUESS_ZERO || p.initial_guess > FREHG_GUESS_EXTRAPOLATED)
+    throw ArgError("frehg_re: unknown initial_guess");
   (void)krylov_config(p.solver, "frehg_re");
+  if (p.initial_guess == FREHG_GUESS_EXTRAPOLATED || p.initial_guess == FREHG_GUESS_LEARNED) ensure_dhp();
   const bool mixed = p.solver.precision == FREHG_PREC_MIXED;
   if (mixed && !D32_) {
     const size_t b32 = (size_t)n_ * sizeof(float);
@@ -439,6 +458,7 @@ void ReState::set_initial(const double* h_int, const double* wc_int) {
   FREHG_LAUNCH_CHECK();
   FREHG_HIP(hipStreamSynchronize(s_));
   dtn_ = p_.dt;
+  dhp_valid_ = false;
 }
 
 void ReState::learned_increment(double* dst) {
@@ -453,7 +473,7 @@ void ReState::learned_increment(double* dst) {
   FREHG_HIP(hipMalloc(&tmp, (size_t)n_ * sizeof(double)));
   try {
     FREHG_HIP(hipMemsetAsync(tmp, 0, (size_t)n_ * sizeof(double), s_));
-    mlp_->predict_guess(*this, tmp, depth_, h_, wc_);
+    mlp_->predict_guess(*this, tmp, depth_, nullptr, h_, wc_);
     std::vector<double> pad((size_t)n_), hh((size_t)n_);
     FREHG_HIP(hipMemcpyAsync(pad.data(), tmp, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
     FREHG_HIP(hipMemcpyAsync(hh.data(), h_, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
@@ -507,7 +527,18 @@ void ReState::step(frehg_re_stats* st) {
   // With the Picard loop the learned head is the loop's starting iterate h^0 (below): the system
   // is assembled at it, not at h^n, whose K and C serve the features only.
   const bool learned_h0 = p_.initial_guess == FREHG_GUESS_LEARNED && p_.picard_max > 1;
-  {
+  // The extrapolated head as h^0 likewise; K and C at h^n are then not needed at all.
+  const bool extrap_h0 = p_.initial_guess == FREHG_GUESS_EXTRAPOLATED && p_.picard_max > 1;
+  const bool start_h0 = learned_h0 || extrap_h0;
+  // The solve increment h - h^n is kept for the next step: the extrapolated start adds it to h^n,
+  // the learned start adds it and the network's prediction, and pair logging subtracts it from
+  // the logged increment. A step without one (the first after set-up) logs no pairs.
+  const bool logging = mlp_ && mlp_->logging();
+  const bool track_inc = p_.initial_guess == FREHG_GUESS_EXTRAPOLATED ||
+                         p_.initial_guess == FREHG_GUESS_LEARNED || logging;
+  if (track_inc) ensure_dhp();
+  const double* inc0 = dhp_valid_ ? dhp_ : nullptr;
+  if (!extrap_h0) {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_ASSEMBLE);
     hipEvent_t ev1 = prof_->stop();
     with_soil_variant(P, [&](const auto& PK) {
@@ -527,21 +558,27 @@ void ReState::step(frehg_re_stats* st) {
       }
     });
   }
-  // Initial guess in h (the solve's x): zero (PETSc default), h^n, or h^n + MLP increment.
+  // Initial guess in h (the solve's x): zero (PETSc default), h^n, h^n + the previous step's
+  // solve increment, or h^n + MLP increment.
   bool x0_zero = true, guard = false;
-  if (p_.initial_guess == FREHG_GUESS_PREVIOUS) {
+  if (p_.initial_guess == FREHG_GUESS_PREVIOUS || (p_.initial_guess == FREHG_GUESS_EXTRAPOLATED && !dhp_valid_)) {
     hipLaunchKernelGGL(k_copy_interior, grid, blk, 0, s_, g_, hn_, h_);
+    FREHG_LAUNCH_CHECK();
+    x0_zero = false;
+  } else if (p_.initial_guess == FREHG_GUESS_EXTRAPOLATED) {
+    hipLaunchKernelGGL(k_add_interior, grid, blk, 0, s_, g_, (const double*)hn_, (const double*)dhp_, h_);
     FREHG_LAUNCH_CHECK();
     x0_zero = false;
   } else if (p_.initial_guess == FREHG_GUESS_LEARNED) {
     if (!mlp_ || !mlp_->ready()) throw ArgError("frehg_re_step: learned guess requested but no trained model loaded");
-    mlp_->predict_guess(*this, h_, depth_);
+    mlp_->predict_guess(*this, h_, depth_, inc0);
     x0_zero = false;
     guard = true;
   }
-  // The learned head as the Picard loop's h^0: K and C at it (with its head-BC ghosts), the
-  // assembly with the time-level-n terms, and the first stopping test against it (kept in hp_).
-  if (learned_h0) {
+  // The learned (or extrapolated) head as the Picard loop's h^0: K and C at it (with its head-BC
+  // ghosts), the assembly with the time-level-n terms, and the first stopping test against it
+  // (kept in hp_).
+  if (start_h0) {
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
       // post-solve pass at h^0, which also keeps h^0 in hp_ (its test output is not used)
@@ -574,7 +611,7 @@ void ReState::step(frehg_re_stats* st) {
   // the test fused in; at the last one it is the step's own post-solve evaluation.
   bool post_done = false;
   for (int m = 1; p_.picard_max > 1; ++m) {
-    const double* ref = (m == 1 && !learned_h0) ? hn_ : hp_;
+    const double* ref = (m == 1 && !start_h0) ? hn_ : hp_;
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
       hipLaunchKernelGGL((k_re_post_delta<K, 1>), grid, blk, 0, s_, PK, h_, kr_, ch_, wch_, ref, hp_, red_, nbl);
@@ -605,7 +642,7 @@ void ReState::step(frehg_re_stats* st) {
     local.solve.fallback |= sm.fallback_fp64 ? 1 : 0;
     local.picard_iterations = m + 1;
   }
-  if (mlp_ && mlp_->logging()) {
+  if (logging && inc0) {
     // the features are K and C at h^n: the Picard iterates replaced them
     if (p_.picard_max > 1) {
       with_soil_variant(P, [&](const auto& PK) {
@@ -615,8 +652,9 @@ void ReState::step(frehg_re_stats* st) {
       });
       post_done = false;
     }
-    mlp_->log_pair(*this, hn_, h_, depth_);
+    mlp_->log_pair(*this, hn_, h_, depth_, inc0);
   }
+  double* dh_store = track_inc ? dhp_ : nullptr;
   {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_CORRECT);
     hipEvent_t ev1 = prof_->stop();
@@ -629,8 +667,8 @@ void ReState::step(frehg_re_stats* st) {
       hipExtLaunchKernelGGL((k_re_flux_correct<K, 1>), grid, blk, 0, s_, post_done ? ev0 : nullptr, ev1, 0, PK,
                             (const double*)h_,
                             (const double*)kr_, (const double*)hn_, (const double*)wcn_, (const double*)wch_, Kx_, Ky_,
-                            Kz_, qx_, qy_, qz_, wc_, hwc_, sat0_, cand_, lazy_flux_ ? kf_ : nullptr, led_, nbl,
-                            dflag_, kSweepFlags);
+                            Kz_, qx_, qy_, qz_, wc_, hwc_, sat0_, cand_, lazy_flux_ ? kf_ : nullptr, dh_store,
+                            led_, nbl, dflag_, kSweepFlags);
       FREHG_LAUNCH_CHECK();
     });
   }
@@ -713,6 +751,7 @@ void ReState::step(frehg_re_stats* st) {
   local.dt_used = dt_used_;
   local.dt_next = p_.dt;
   kc_fresh_ = true;
+  if (dh_store) dhp_valid_ = true;
   if (st) *st = local;
 }
 

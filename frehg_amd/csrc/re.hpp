@@ -79,6 +79,11 @@ class ReState {
   // Kx..qz follow bit for bit from hp_, kr_ (post-solve) and wcn_, and are stored from them when
   // read (get_field) or before anything changes those inputs (flux_pending_).
   double* hp_ = nullptr;
+  // The last step's solve increment (post-solve head - h^n): the extrapolated and learned starts
+  // add it to h^n. Allocated on first use; invalid until a step has written it.
+  double* dhp_ = nullptr;
+  bool dhp_valid_ = false;
+  void ensure_dhp();
   unsigned char* kf_ = nullptr;
   bool lazy_flux_ = true;
   bool flux_pending_ = false;

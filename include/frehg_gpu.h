@@ -50,7 +50,10 @@ enum {
 enum {
   FREHG_GUESS_ZERO = 0,     /* x0 = 0 (PETSc default, the reference) */
   FREHG_GUESS_PREVIOUS = 1, /* x0 = h^n */
-  FREHG_GUESS_LEARNED = 2   /* x0 = h^n + MLP(local stencil features), zero-guess fallback */
+  FREHG_GUESS_LEARNED = 2,  /* x0 = h^n + the previous step's solve increment + MLP(local stencil
+                               features), zero-guess fallback; the network is trained on the
+                               increment minus the previous one */
+  FREHG_GUESS_EXTRAPOLATED = 3 /* x0 = h^n + the previous step's solve increment (h^n on the first step) */
 };
 
 /* Soil class: include/frehg2/re/SoilParams.hpp:13-25. */
@@ -240,17 +243,20 @@ int frehg_ens_step(frehg_ctx_t ctx, int use_graph, frehg_ens_stats* stats, frehg
  * x0 of the Richards solve, src/re/ReSolver.cpp:560-636, whose PETSc KSP always starts from 0).
  * An 11 -> 64 -> 64 -> 1 tanh MLP over local features of the state before the solve (h^n(c), the
  * six h^n(neighbour) - h^n(c), ln K(h^n), C(h^n), theta^n, surface depth of the column) predicts
- * h^{n+1}(c) - h^n(c). BF16 MFMA forward and backward with FP32 accumulation, Adam on FP32
- * master weights, all on the GPU. With frehg_re_params.initial_guess = FREHG_GUESS_LEARNED the
- * solve starts from h^n + prediction and falls back to x0 = 0 (stats.guess_fallback = 1) when
- * the guess's residual is not below the zero guess's. ----------------------------------------- */
+ * the cell's solve increment h^{n+1}(c) - h^n(c) minus the previous step's, d^{n-1}(c). BF16 MFMA
+ * forward and backward with FP32 accumulation, Adam on FP32 master weights, all on the GPU. With
+ * frehg_re_params.initial_guess = FREHG_GUESS_LEARNED the solve (and the Picard loop, with
+ * picard_max > 1) starts from h^n + d^{n-1} + prediction (d^{n-1} = 0 on the first step after
+ * set-up) and falls back to x0 = 0 (stats.guess_fallback = 1) when the guess's residual is not
+ * below the zero guess's. ---------------------------------------------------------------------- */
 #define FREHG_MLP_INPUTS 11
 #define FREHG_MLP_PARAMS 4993 /* W1[64][11], b1[64], W2[64][64], b2[64], w3[64], b3 */
 #define FREHG_MLP_NORM 24     /* mean[12], sd[12] of the 11 features and the target */
 /* Glorot-uniform weights, zero biases, identity normalisation. */
 int frehg_mlp_init(frehg_ctx_t ctx, uint64_t seed);
-/* Log (features, converged increment) of every `stride`-th interior cell after each Richards
- * solve into a ring buffer of `capacity` rows; stride 0 stops logging. */
+/* Log (features, converged increment minus the previous step's) of every `stride`-th interior
+ * cell after each Richards solve into a ring buffer of `capacity` rows; stride 0 stops logging.
+ * The first step after set-up (no previous increment) logs nothing. */
 int frehg_mlp_log(frehg_ctx_t ctx, int stride, int64_t capacity);
 int frehg_mlp_logged(frehg_ctx_t ctx, int64_t* n);
 /* `steps` Adam steps (lr; beta1 0.9, beta2 0.999, eps 1e-8) on minibatches of `batch` logged
@@ -264,8 +270,9 @@ int frehg_mlp_set_params(frehg_ctx_t ctx, const float* params, const float* norm
 int frehg_mlp_loss_grad(frehg_ctx_t ctx, const float* x, const float* y, int n, float* grad, double* loss);
 /* Predicted raw increments for x[n][11]. */
 int frehg_mlp_predict(frehg_ctx_t ctx, const float* x, int n, float* out);
-/* The increment the learned guess would add for a step taken from the current Richards state
- * (features from h, K(h), C(h), theta), nx*ny*nz values, i + j*nx + k*nx*ny. */
+/* The network's prediction for a step taken from the current Richards state (features from h,
+ * K(h), C(h), theta): the part of the learned guess's increment on top of the previous step's
+ * solve increment. nx*ny*nz values, i + j*nx + k*nx*ny. */
 int frehg_mlp_increment(frehg_ctx_t ctx, double* dst);
 
 /* ---- measurement --------------------------------------------------------------------------- */

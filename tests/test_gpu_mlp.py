@@ -76,7 +76,8 @@ def test_training_and_learned_guess_gain():
     say("MLP: logging plain-solver pairs on two training fields")
     for seed in (101, 102):
         run(ctx, nx, ny, nz, seed, abi.GUESS_ZERO, steps)
-    assert ctx.mlp_logged() == 2 * steps * nx * ny * nz
+    # the first step of a run has no previous solve increment and logs nothing
+    assert ctx.mlp_logged() == 2 * (steps - 1) * nx * ny * nz
     ctx.mlp_log(stride=0, capacity=0)
     say("MLP: training")
     first, last = ctx.mlp_train(steps=1500, batch=8192, lr=1e-3, seed=9)
