@@ -79,10 +79,16 @@ struct FeatSrc {
   const double* wcn;
   SoilView soil;
   const double* depth;  // surface padded, or null
-  // Base increment per cell (the last step's solve increment): the network predicts the
-  // increment minus it. Null = zero.
-  const double* inc0;
+  // Base increment per cell, inc_a - inc_b (the previous step's solve increment; inc_b null:
+  // inc_a alone; inc_a null: zero): the network predicts the increment minus it.
+  const double* inc_a;
+  const double* inc_b;
 };
+
+__device__ __forceinline__ double base_increment(const FeatSrc& F, long long c) {
+  if (!F.inc_a) return 0.0;
+  return F.inc_b ? F.inc_a[c] - F.inc_b[c] : F.inc_a[c];
+}
 
 __device__ inline void cell_features(const FeatSrc& F, int i, int j, int k, float* f) {
   const Geo& g = F.g;
@@ -480,7 +486,7 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const flo
     double base = 0.0;  // h^n + base increment of this lane's cell
     if (i < g3.nx) {
       c = g3.idx(i, j, k);
-      base = F.hn[c] + (F.inc0 ? F.inc0[c] : 0.0);
+      base = F.hn[c] + base_increment(F, c);
       cell_features(F, i, j, k, f);
 #pragma unroll
       for (int t = 0; t < kMlpIn; ++t) f[t] = (f[t] - nmu[t]) * nisd[t];
@@ -551,7 +557,7 @@ __global__ __launch_bounds__(MT) void k_mlp_log(FeatSrc F, const double* h, int 
 #pragma unroll
   for (int t = 0; t < kMlpIn; ++t) row[t] = f[t];
   const long long c = g.idx(i, j, k);
-  row[kMlpIn] = (float)(h[c] - F.hn[c] - (F.inc0 ? F.inc0[c] : 0.0));
+  row[kMlpIn] = (float)(h[c] - F.hn[c] - base_increment(F, c));
 }
 
 // Per-block sums and sums of squares of the 12 logged columns (FP64).
@@ -584,7 +590,8 @@ FeatSrc feat_src(ReState& re, const double* depth) {
   F.wcn = re.d_wcn();
   F.soil = re.soil_view();
   F.depth = depth;
-  F.inc0 = nullptr;
+  F.inc_a = nullptr;
+  F.inc_b = nullptr;
   return F;
 }
 
@@ -658,7 +665,7 @@ void Mlp::log_pair(ReState& re, const double* hn, const double* h, const double*
   if (!logging()) return;
   FeatSrc F = feat_src(re, depth);
   F.hn = hn;
-  F.inc0 = inc0;
+  F.inc_a = inc0;
   const Geo& g = F.g;
   const long long N = (long long)g.nx * g.ny * g.nz;
   const int phase = (int)(log_steps_ % stride_);
@@ -729,11 +736,12 @@ double Mlp::train(int steps, int batch, double lr, uint64_t seed, double* first)
   return loss;
 }
 
-void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const double* inc0, const double* hn,
-                        const double* wcn) {
+void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const double* inc_a, const double* inc_b,
+                        const double* hn, const double* wcn) {
   if (!ready_) throw ArgError("learned guess requested but the model is not trained or loaded");
   FeatSrc F = feat_src(re, depth);
-  F.inc0 = inc0;
+  F.inc_a = inc_a;
+  F.inc_b = inc_b;
   if (hn) F.hn = hn;
   if (wcn) F.wcn = wcn;
   const long long N = (long long)F.g.nx * F.g.ny * F.g.nz;

@@ -59,10 +59,11 @@ class Mlp {
   // Returns the mean standardised squared error of the last minibatch (first one in *first).
   double train(int steps, int batch, double lr, uint64_t seed, double* first);
 
-  // h_out (interior) = h^n + inc0 + predicted increment (inc0 null: zero); hn / wcn default to
-  // the solver's h^n / theta^n.
-  void predict_guess(ReState& re, double* h_out, const double* depth, const double* inc0, const double* hn = nullptr,
-                     const double* wcn = nullptr);
+  // h_out (interior) = h^n + (inc_a - inc_b) + predicted increment (inc_a null: no base
+  // increment; inc_b may be h_out itself, each cell's value is read before it is written);
+  // hn / wcn default to the solver's h^n / theta^n.
+  void predict_guess(ReState& re, double* h_out, const double* depth, const double* inc_a, const double* inc_b,
+                     const double* hn = nullptr, const double* wcn = nullptr);
 
   void get_params(float* params, float* norm);
   void set_params(const float* params, const float* norm);

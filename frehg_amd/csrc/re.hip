@@ -93,9 +93,8 @@ __global__ __launch_bounds__(NT) void k_re_flux_correct(K P, const double* __res
                                                          double* __restrict__ qz, double* __restrict__ wc,
                                                          double* __restrict__ hwc, unsigned char* __restrict__ sat0,
                                                          unsigned char* __restrict__ cand,
-                                                         unsigned char* __restrict__ kf, double* __restrict__ dh,
-                                                         double* __restrict__ part, unsigned int nblocks,
-                                                         int* __restrict__ flags, int nflags) {
+                                                         unsigned char* __restrict__ kf, double* __restrict__ part,
+                                                         unsigned int nblocks, int* __restrict__ flags, int nflags) {
   __shared__ double lds[NT / kWave];
   TileId b;
   const Cell t = tile_cell<ZF>(P.g, b);
@@ -104,8 +103,6 @@ __global__ __launch_bounds__(NT) void k_re_flux_correct(K P, const double* __res
   if (t.valid)
     re_flux_correct(P, t, h, kr, hn, wcn, wch, Kx, Ky, Kz, qx, qy, qz, wc, hwc, sat0, cand, kf, l_before, l_ss,
                     l_bnd);
-  // the step's solve increment, the extrapolated start of the next step
-  if (dh && t.valid) dh[t.c] = h[t.c] - hn[t.c];
   const double a = block_sum<NT>(l_before, lds);
   const double bs = block_sum<NT>(l_ss, lds);
   const double d = block_sum<NT>(l_bnd, lds);
@@ -251,10 +248,18 @@ __global__ __launch_bounds__(NT) void k_copy_interior(Geo g, const double* src, 
   dst[t.c] = src[t.c];
 }
 
-__global__ __launch_bounds__(NT) void k_add_interior(Geo g, const double* a, const double* b, double* dst) {
+// h <- h^n + (hp - h): h^n plus the previous step's solve increment, hp holding that step's
+// post-solve head and h its starting head on entry.
+__global__ __launch_bounds__(NT) void k_extrap_interior(Geo g, const double* hn, const double* hp, double* h) {
   const Cell t = cell_of(g);
   if (!t.valid) return;
-  dst[t.c] = a[t.c] + b[t.c];
+  h[t.c] = hn[t.c] + (hp[t.c] - h[t.c]);
+}
+
+__global__ __launch_bounds__(NT) void k_sub_interior(Geo g, const double* a, const double* b, double* dst) {
+  const Cell t = cell_of(g);
+  if (!t.valid) return;
+  dst[t.c] = a[t.c] - b[t.c];
 }
 
 }  // namespace
@@ -310,7 +315,6 @@ void ReState::ensure_dhp() {
   if (dhp_) return;
   FREHG_HIP(hipMalloc(&dhp_, (size_t)n_ * sizeof(double)));
   FREHG_HIP(hipMemsetAsync(dhp_, 0, (size_t)n_ * sizeof(double), s_));
-  dhp_valid_ = false;
 }
 
 void ReState::set_params(const frehg_re_params& p) {
@@ -326,7 +330,6 @@ void ReState::set_params(const frehg_re_params& p) {
   if (p.initial_guess < FREHG_GUESS_ZERO || p.initial_guess > FREHG_GUESS_EXTRAPOLATED)
     throw ArgError("frehg_re: unknown initial_guess");
   (void)krylov_config(p.solver, "frehg_re");
-  if (p.initial_guess == FREHG_GUESS_EXTRAPOLATED || p.initial_guess == FREHG_GUESS_LEARNED) ensure_dhp();
   const bool mixed = p.solver.precision == FREHG_PREC_MIXED;
   if (mixed && !D32_) {
     const size_t b32 = (size_t)n_ * sizeof(float);
@@ -427,6 +430,7 @@ void ReState::set_field(const std::string& name, const double* src) {
   double* dst = field_ptr(name);
   materialize_fluxes();
   kc_fresh_ = false;
+  inc_valid_ = false;
   copy_field_in(g_, g_.nz + 2, dst, src, s_);
   FREHG_HIP(hipStreamSynchronize(s_));
 }
@@ -458,7 +462,7 @@ void ReState::set_initial(const double* h_int, const double* wc_int) {
   FREHG_LAUNCH_CHECK();
   FREHG_HIP(hipStreamSynchronize(s_));
   dtn_ = p_.dt;
-  dhp_valid_ = false;
+  inc_valid_ = false;
 }
 
 void ReState::learned_increment(double* dst) {
@@ -473,7 +477,7 @@ void ReState::learned_increment(double* dst) {
   FREHG_HIP(hipMalloc(&tmp, (size_t)n_ * sizeof(double)));
   try {
     FREHG_HIP(hipMemsetAsync(tmp, 0, (size_t)n_ * sizeof(double), s_));
-    mlp_->predict_guess(*this, tmp, depth_, nullptr, h_, wc_);
+    mlp_->predict_guess(*this, tmp, depth_, nullptr, nullptr, h_, wc_);
     std::vector<double> pad((size_t)n_), hh((size_t)n_);
     FREHG_HIP(hipMemcpyAsync(pad.data(), tmp, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
     FREHG_HIP(hipMemcpyAsync(hh.data(), h_, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
@@ -530,14 +534,23 @@ void ReState::step(frehg_re_stats* st) {
   // The extrapolated head as h^0 likewise; K and C at h^n are then not needed at all.
   const bool extrap_h0 = p_.initial_guess == FREHG_GUESS_EXTRAPOLATED && p_.picard_max > 1;
   const bool start_h0 = learned_h0 || extrap_h0;
-  // The solve increment h - h^n is kept for the next step: the extrapolated start adds it to h^n,
-  // the learned start adds it and the network's prediction, and pair logging subtracts it from
-  // the logged increment. A step without one (the first after set-up) logs no pairs.
+  // The previous step's solve increment d^{n-1} = hp_ - h_ here (its post-solve head minus its
+  // starting head, both kept since): the extrapolated start adds it to h^n, the learned start
+  // adds it and the network's prediction, and pair logging subtracts it from the logged
+  // increment (a copy, taken before the guess overwrites h_). A step without one (the first
+  // after set-up) logs no pairs.
   const bool logging = mlp_ && mlp_->logging();
   const bool track_inc = p_.initial_guess == FREHG_GUESS_EXTRAPOLATED ||
                          p_.initial_guess == FREHG_GUESS_LEARNED || logging;
-  if (track_inc) ensure_dhp();
-  const double* inc0 = dhp_valid_ ? dhp_ : nullptr;
+  const bool have_inc = inc_valid_;
+  inc_valid_ = false;
+  const double* inc0 = nullptr;  // d^{n-1} for the logged pairs
+  if (logging && have_inc) {
+    ensure_dhp();
+    hipLaunchKernelGGL(k_sub_interior, grid, blk, 0, s_, g_, (const double*)hp_, (const double*)h_, dhp_);
+    FREHG_LAUNCH_CHECK();
+    inc0 = dhp_;
+  }
   if (!extrap_h0) {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_ASSEMBLE);
     hipEvent_t ev1 = prof_->stop();
@@ -561,17 +574,17 @@ void ReState::step(frehg_re_stats* st) {
   // Initial guess in h (the solve's x): zero (PETSc default), h^n, h^n + the previous step's
   // solve increment, or h^n + MLP increment.
   bool x0_zero = true, guard = false;
-  if (p_.initial_guess == FREHG_GUESS_PREVIOUS || (p_.initial_guess == FREHG_GUESS_EXTRAPOLATED && !dhp_valid_)) {
+  if (p_.initial_guess == FREHG_GUESS_PREVIOUS || (p_.initial_guess == FREHG_GUESS_EXTRAPOLATED && !have_inc)) {
     hipLaunchKernelGGL(k_copy_interior, grid, blk, 0, s_, g_, hn_, h_);
     FREHG_LAUNCH_CHECK();
     x0_zero = false;
   } else if (p_.initial_guess == FREHG_GUESS_EXTRAPOLATED) {
-    hipLaunchKernelGGL(k_add_interior, grid, blk, 0, s_, g_, (const double*)hn_, (const double*)dhp_, h_);
+    hipLaunchKernelGGL(k_extrap_interior, grid, blk, 0, s_, g_, (const double*)hn_, (const double*)hp_, h_);
     FREHG_LAUNCH_CHECK();
     x0_zero = false;
   } else if (p_.initial_guess == FREHG_GUESS_LEARNED) {
     if (!mlp_ || !mlp_->ready()) throw ArgError("frehg_re_step: learned guess requested but no trained model loaded");
-    mlp_->predict_guess(*this, h_, depth_, inc0);
+    mlp_->predict_guess(*this, h_, depth_, have_inc ? hp_ : nullptr, have_inc ? h_ : nullptr);
     x0_zero = false;
     guard = true;
   }
@@ -654,7 +667,12 @@ void ReState::step(frehg_re_stats* st) {
     }
     mlp_->log_pair(*this, hn_, h_, depth_, inc0);
   }
-  double* dh_store = track_inc ? dhp_ : nullptr;
+  // Without lazy fluxes finalize writes the final head over the post-solve head: keep the latter
+  // in hp_ for the next step's increment.
+  if (track_inc && !lazy_flux_) {
+    hipLaunchKernelGGL(k_copy_interior, grid, blk, 0, s_, g_, (const double*)h_, hp_);
+    FREHG_LAUNCH_CHECK();
+  }
   {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_CORRECT);
     hipEvent_t ev1 = prof_->stop();
@@ -667,8 +685,8 @@ void ReState::step(frehg_re_stats* st) {
       hipExtLaunchKernelGGL((k_re_flux_correct<K, 1>), grid, blk, 0, s_, post_done ? ev0 : nullptr, ev1, 0, PK,
                             (const double*)h_,
                             (const double*)kr_, (const double*)hn_, (const double*)wcn_, (const double*)wch_, Kx_, Ky_,
-                            Kz_, qx_, qy_, qz_, wc_, hwc_, sat0_, cand_, lazy_flux_ ? kf_ : nullptr, dh_store,
-                            led_, nbl, dflag_, kSweepFlags);
+                            Kz_, qx_, qy_, qz_, wc_, hwc_, sat0_, cand_, lazy_flux_ ? kf_ : nullptr, led_, nbl,
+                            dflag_, kSweepFlags);
       FREHG_LAUNCH_CHECK();
     });
   }
@@ -751,7 +769,7 @@ void ReState::step(frehg_re_stats* st) {
   local.dt_used = dt_used_;
   local.dt_next = p_.dt;
   kc_fresh_ = true;
-  if (dh_store) dhp_valid_ = true;
+  inc_valid_ = track_inc || lazy_flux_;
   if (st) *st = local;
 }
 

@@ -79,10 +79,12 @@ class ReState {
   // Kx..qz follow bit for bit from hp_, kr_ (post-solve) and wcn_, and are stored from them when
   // read (get_field) or before anything changes those inputs (flux_pending_).
   double* hp_ = nullptr;
-  // The last step's solve increment (post-solve head - h^n): the extrapolated and learned starts
-  // add it to h^n. Allocated on first use; invalid until a step has written it.
+  // After a step, hp_ holds its post-solve head and hn_ its starting head (h_ after the next
+  // step's swap): inc_valid_ says so, and their difference is the increment the extrapolated and
+  // learned starts add to h^n. Any change of the state clears it. dhp_ holds a copy of that
+  // increment for pair logging (allocated on first use).
+  bool inc_valid_ = false;
   double* dhp_ = nullptr;
-  bool dhp_valid_ = false;
   void ensure_dhp();
   unsigned char* kf_ = nullptr;
   bool lazy_flux_ = true;

@@ -110,18 +110,21 @@ def test_incremental_kc_matches_full_reevaluation(full3d):
             assert np.array_equal(a.re_get(f), b.re_get(f)), f"step {s} field {f}"
 
 
-@pytest.mark.parametrize("full3d,adaptive", [(1, 0), (0, 0), (1, 1)])
-def test_lazy_flux_fields_match_stored(full3d, adaptive):
+@pytest.mark.parametrize("full3d,adaptive,guess", [(1, 0, abi.GUESS_ZERO), (0, 0, abi.GUESS_ZERO),
+                                                   (1, 1, abi.GUESS_ZERO), (1, 0, abi.GUESS_EXTRAPOLATED)])
+def test_lazy_flux_fields_match_stored(full3d, adaptive, guess):
     """By default the step stores no Kx..qz (only face bits for the reallocation) and rebuilds
     them from the post-solve state when they are read or before the state changes; a context
     that stores them in the step (FREHG_LAZY_FLUX=0) must give every field bit for bit, whether
-    the fields are read after every step, skipped for a step, or forced out by set_params."""
+    the fields are read after every step, skipped for a step, or forced out by set_params. With
+    the extrapolated start, both keep the post-solve head for the next step's increment (the
+    stored-flux context copies it), so they start every solve from the same head."""
     nx, ny, nz = 9, 7, 12
-    say(f"RE lazy flux fields full3d={full3d} adaptive={adaptive}")
+    say(f"RE lazy flux fields full3d={full3d} adaptive={adaptive} guess={guess}")
     soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5, Ss=1e-5)
     p = abi.re_params(soil=soil, solver=abi.solver_config(1e-13), bc_type=(0, 0, 0, 0, 0, 1), dx=0.5, dy=0.5,
                       dz=0.05, dt=60.0, adaptive_dt=adaptive, dt_max=120.0, dt_min=1.0, use_full3d=full3d,
-                      htop=0.05)
+                      htop=0.05, initial_guess=guess)
     k = np.arange(nz)[:, None, None]
     i = np.arange(nx)[None, None, :]
     h0 = np.broadcast_to(-0.05 - 0.08 * k - 0.01 * i, (nz, ny, nx)).ravel()
