@@ -74,8 +74,6 @@ __device__ __forceinline__ void wave_lds_fence() {
 struct FeatSrc {
   Geo g;
   const double* hn;
-  const double* kr;
-  const double* ch;
   const double* wcn;
   SoilView soil;
   const double* depth;  // surface padded, or null
@@ -90,6 +88,44 @@ __device__ __forceinline__ double base_increment(const FeatSrc& F, long long c) 
   return F.inc_b ? F.inc_a[c] - F.inc_b[c] : F.inc_a[c];
 }
 
+// ln K and C at head h for the features, from the hardware base-2 exponential and logarithm
+// (the features enter the network in FP16). Van Genuchten classes: with p = (alpha |h|)^n,
+// Se = (1 + p)^-m, K = Ks Se^(1/2) t^2 with t = 1 - (p / (1 + p))^m = 1 - exp(-m ln(1 + 1/p)),
+// C = capc p Se / (alpha |h| (1 + p)); K = Ks and C = 0 for h > 0. ln(1 + x) and 1 - exp(-x) are
+// taken from their series below 1e-3. Other classes through the FP64 law.
+__device__ __forceinline__ float log2_1p(float x) {  // log2(1 + x), x >= 0
+  constexpr float kLog2e = 1.44269504f;
+  return x < 1e-3f ? x * kLog2e * (1.0f - 0.5f * x) : __builtin_amdgcn_logf(1.0f + x);
+}
+template <bool VG_ONLY>
+__device__ inline void feature_kc(const SoilClass& s, double ksz, double h, float& lnk, float& cap) {
+  constexpr float kLnFloor = -69.0775528f;  // ln(1e-30)
+  constexpr float kLn2 = 0.693147181f, kLog2e = 1.44269504f;
+  if (!VG_ONLY && (!s.use_vg || s.use_mvg)) {
+    const VGState v = vg_state(s, h);
+    lnk = logf((float)fmax(ksz * v.kr, 1e-30));
+    cap = (float)v.cap;
+    return;
+  }
+  const float lks = fmaxf(kLn2 * __builtin_amdgcn_logf((float)ksz), kLnFloor);
+  const float a = fabsf((float)s.alpha * (float)h);
+  if (h > 0.0 || a == 0.0f) {
+    lnk = lks;
+    cap = 0.0f;
+    return;
+  }
+  const float m = (float)s.m;
+  const float l2p = (float)s.n * __builtin_amdgcn_logf(a);  // log2 p
+  const float p = __builtin_amdgcn_exp2f(l2p);
+  const float l2_1p = log2_1p(p);                                       // log2(1 + p)
+  const float md = m * kLn2 * log2_1p(__builtin_amdgcn_exp2f(-l2p));  // m ln(1 + 1/p)
+  const float t = md < 1e-3f ? md * (1.0f - 0.5f * md) : 1.0f - __builtin_amdgcn_exp2f(-kLog2e * md);
+  lnk = fmaxf(lks + fminf(kLn2 * (2.0f * __builtin_amdgcn_logf(t) - 0.5f * m * l2_1p), 0.0f), kLnFloor);
+  cap = (float)s.capc * p * __builtin_amdgcn_exp2f(-m * l2_1p) / (a * (1.0f + p));
+}
+
+// VG_ONLY: every soil class is a plain van Genuchten class (no FP64 fallback code in the kernel).
+template <bool VG_ONLY>
 __device__ inline void cell_features(const FeatSrc& F, int i, int j, int k, float* f) {
   const Geo& g = F.g;
   const long long c = g.idx(i, j, k);
@@ -101,8 +137,7 @@ __device__ inline void cell_features(const FeatSrc& F, int i, int j, int k, floa
   f[4] = (float)(F.hn[c + g.sy] - hc);
   f[5] = (float)(F.hn[c - g.sz] - hc);
   f[6] = (float)(F.hn[c + g.sz] - hc);
-  f[7] = logf((float)fmax(F.soil.Ksz(c) * F.kr[c], 1e-30));
-  f[8] = (float)F.ch[c];
+  feature_kc<VG_ONLY>(F.soil.at(c), F.soil.Ksz(c), hc, f[7], f[8]);
   f[9] = (float)F.wcn[c];
   f[10] = F.depth ? (float)F.depth[g.sidx(i, j)] : 0.0f;
 }
@@ -434,6 +469,7 @@ __device__ __forceinline__ f32x4 mfma16(f16x8 a, f16x8 b, f32x4 c) {
 // r = 1 / (1 + 2^z) (z = s times the pre-activation); 0 for large z, 1 for very negative z.
 __device__ __forceinline__ float act_r(float z) { return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z)); }
 
+template <bool VG_ONLY>
 __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const float* norm, FeatSrc F, double* h_out) {
   __shared__ __attribute__((aligned(16))) float xs[GW][RB][FP];
   __shared__ float ys[GW][RB];  // network outputs of the chunk's cells
@@ -487,7 +523,7 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const flo
     if (i < g3.nx) {
       c = g3.idx(i, j, k);
       base = F.hn[c] + base_increment(F, c);
-      cell_features(F, i, j, k, f);
+      cell_features<VG_ONLY>(F, i, j, k, f);
 #pragma unroll
       for (int t = 0; t < kMlpIn; ++t) f[t] = (f[t] - nmu[t]) * nisd[t];
     }
@@ -543,6 +579,7 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const flo
 
 // Log (features, h - h^n - base increment) of interior cells q with q % stride == phase into the
 // ring buffer.
+template <bool VG_ONLY>
 __global__ __launch_bounds__(MT) void k_mlp_log(FeatSrc F, const double* h, int stride, int phase, long long base,
                                                  long long cap, float* log) {
   const Geo& g = F.g;
@@ -552,7 +589,7 @@ __global__ __launch_bounds__(MT) void k_mlp_log(FeatSrc F, const double* h, int 
   if (q >= N) return;
   const int i = (int)(q % g.nx), j = (int)((q / g.nx) % g.ny), k = (int)(q / ((long long)g.nx * g.ny));
   float f[kMlpIn];
-  cell_features(F, i, j, k, f);
+  cell_features<VG_ONLY>(F, i, j, k, f);
   float* row = log + ((base + qq) % cap) * kMlpRow;
 #pragma unroll
   for (int t = 0; t < kMlpIn; ++t) row[t] = f[t];
@@ -585,8 +622,6 @@ FeatSrc feat_src(ReState& re, const double* depth) {
   FeatSrc F;
   F.g = re.geo();
   F.hn = re.d_hn();
-  F.kr = re.d_kr();
-  F.ch = re.d_ch();
   F.wcn = re.d_wcn();
   F.soil = re.soil_view();
   F.depth = depth;
@@ -671,8 +706,11 @@ void Mlp::log_pair(ReState& re, const double* hn, const double* h, const double*
   const int phase = (int)(log_steps_ % stride_);
   const long long nsamp = phase < N ? (N - phase + stride_ - 1) / stride_ : 0;
   if (nsamp > 0) {
-    hipLaunchKernelGGL(k_mlp_log, dim3((unsigned)ceil_div(nsamp, (long long)MT)), dim3(MT), 0, s_, F, h, stride_,
-                       phase, count_ % cap_, cap_, log_);
+    const dim3 grid((unsigned)ceil_div(nsamp, (long long)MT));
+    if (re.soil_plain_vg())
+      hipLaunchKernelGGL(k_mlp_log<true>, grid, dim3(MT), 0, s_, F, h, stride_, phase, count_ % cap_, cap_, log_);
+    else
+      hipLaunchKernelGGL(k_mlp_log<false>, grid, dim3(MT), 0, s_, F, h, stride_, phase, count_ % cap_, cap_, log_);
     FREHG_LAUNCH_CHECK();
   }
   count_ += nsamp;
@@ -755,8 +793,12 @@ void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const d
   }
   hipEvent_t ev0 = prof_->start(FREHG_KT_GW_GUESS);
   hipEvent_t ev1 = prof_->stop();
-  hipExtLaunchKernelGGL(k_mlp_guess, dim3(nb), dim3(64 * GW), 0, s_, ev0, ev1, 0, (const float*)infer_,
-                        (const float*)norm_, F, h_out);
+  if (re.soil_plain_vg())
+    hipExtLaunchKernelGGL(k_mlp_guess<true>, dim3(nb), dim3(64 * GW), 0, s_, ev0, ev1, 0, (const float*)infer_,
+                          (const float*)norm_, F, h_out);
+  else
+    hipExtLaunchKernelGGL(k_mlp_guess<false>, dim3(nb), dim3(64 * GW), 0, s_, ev0, ev1, 0, (const float*)infer_,
+                          (const float*)norm_, F, h_out);
   FREHG_LAUNCH_CHECK();
 }
 

@@ -467,12 +467,6 @@ void ReState::set_initial(const double* h_int, const double* wc_int) {
 
 void ReState::learned_increment(double* dst) {
   if (!mlp_ || !mlp_->ready()) throw ArgError("frehg_mlp_increment: no trained model");
-  materialize_fluxes();  // before kr changes
-  kc_fresh_ = false;    // kr and C are re-evaluated at h below
-  const ReK P = make_rek(g_, soil_view(), p_, qtop_);
-  const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4), g_.nz);
-  hipLaunchKernelGGL((k_re_pre<ReK, 1>), grid, dim3(NT), 0, s_, P, (const double*)h_, kr_, ch_);
-  FREHG_LAUNCH_CHECK();
   double* tmp = nullptr;
   FREHG_HIP(hipMalloc(&tmp, (size_t)n_ * sizeof(double)));
   try {
@@ -529,9 +523,10 @@ void ReState::step(frehg_re_stats* st) {
   kc_fresh_ = false;  // set again once this step has completed
   flux_pending_ = false;  // this step's fluxes supersede the previous step's
   // With the Picard loop the learned head is the loop's starting iterate h^0 (below): the system
-  // is assembled at it, not at h^n, whose K and C serve the features only.
+  // is assembled at it, and K and C at h^n are not needed (the network's features evaluate their
+  // own from h^n).
   const bool learned_h0 = p_.initial_guess == FREHG_GUESS_LEARNED && p_.picard_max > 1;
-  // The extrapolated head as h^0 likewise; K and C at h^n are then not needed at all.
+  // The extrapolated head as h^0 likewise.
   const bool extrap_h0 = p_.initial_guess == FREHG_GUESS_EXTRAPOLATED && p_.picard_max > 1;
   const bool start_h0 = learned_h0 || extrap_h0;
   // The previous step's solve increment d^{n-1} = hp_ - h_ here (its post-solve head minus its
@@ -551,24 +546,21 @@ void ReState::step(frehg_re_stats* st) {
     FREHG_LAUNCH_CHECK();
     inc0 = dhp_;
   }
-  if (!extrap_h0) {
+  if (!start_h0) {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_ASSEMBLE);
     hipEvent_t ev1 = prof_->stop();
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
-      hipEvent_t e1 = learned_h0 ? ev1 : nullptr;
       if (kc_fresh)
-        hipExtLaunchKernelGGL((k_re_pre_stale<K, 1>), grid, blk, 0, s_, ev0, e1, 0, PK,
+        hipExtLaunchKernelGGL((k_re_pre_stale<K, 1>), grid, blk, 0, s_, ev0, nullptr, 0, PK,
                               (const unsigned char*)hmod_, (const double*)hn_, kr_, ch_);
       else
-        hipExtLaunchKernelGGL((k_re_pre<K, 1>), grid, blk, 0, s_, ev0, e1, 0, PK, (const double*)hn_, kr_, ch_);
+        hipExtLaunchKernelGGL((k_re_pre<K, 1>), grid, blk, 0, s_, ev0, nullptr, 0, PK, (const double*)hn_, kr_, ch_);
       FREHG_LAUNCH_CHECK();
-      if (!learned_h0) {
-        hipExtLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, nullptr, ev1, 0, PK, (const double*)kr_,
-                              (const double*)ch_, (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_,
-                              mixed ? D32_ : nullptr, ax32_, ay32_, az32_);
-        FREHG_LAUNCH_CHECK();
-      }
+      hipExtLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, nullptr, ev1, 0, PK, (const double*)kr_,
+                            (const double*)ch_, (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_,
+                            mixed ? D32_ : nullptr, ax32_, ay32_, az32_);
+      FREHG_LAUNCH_CHECK();
     });
   }
   // Initial guess in h (the solve's x): zero (PETSc default), h^n, h^n + the previous step's
@@ -655,18 +647,7 @@ void ReState::step(frehg_re_stats* st) {
     local.solve.fallback |= sm.fallback_fp64 ? 1 : 0;
     local.picard_iterations = m + 1;
   }
-  if (logging && inc0) {
-    // the features are K and C at h^n: the Picard iterates replaced them
-    if (p_.picard_max > 1) {
-      with_soil_variant(P, [&](const auto& PK) {
-        using K = std::decay_t<decltype(PK)>;
-        hipLaunchKernelGGL((k_re_pre<K, 1>), grid, blk, 0, s_, PK, (const double*)hn_, kr_, ch_);
-        FREHG_LAUNCH_CHECK();
-      });
-      post_done = false;
-    }
-    mlp_->log_pair(*this, hn_, h_, depth_, inc0);
-  }
+  if (logging && inc0) mlp_->log_pair(*this, hn_, h_, depth_, inc0);
   // Without lazy fluxes finalize writes the final head over the post-solve head: keep the latter
   // in hp_ for the next step's increment.
   if (track_inc && !lazy_flux_) {

@@ -48,6 +48,12 @@ class ReState {
   // between steps (the SW exchange), or null when the next step re-evaluates every cell anyway.
   unsigned char* d_hmod() { return kc_fresh_ ? hmod_ : nullptr; }
   SoilView soil_view() const { return SoilView{classes_, cls_, ks_, host_classes_[0]}; }
+  // Every soil class is plain van Genuchten (neither the exponential law nor modified VG).
+  bool soil_plain_vg() const {
+    for (const SoilClass& c : host_classes_)
+      if (!c.use_vg || c.use_mvg) return false;
+    return true;
+  }
   Krylov& krylov() { return *kry_; }
   // Stores Kx..qz of the last step if they are pending (see flux_pending_).
   void materialize_fluxes();
