@@ -41,9 +41,10 @@ def test_gradient_matches_oracle(n):
         assert err <= 5e-3 * scale, f"{name}: {err:.3e} vs scale {scale:.3e}"
 
 
-def infiltration(nx, ny, nz, seed, guess, picard_max=1):
+def infiltration(nx, ny, nz, seed, guess, picard_max=1, use_mvg=0):
     rng = np.random.default_rng(seed)
-    soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5, Ss=1e-5)
+    soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5, Ss=1e-5,
+                           use_mvg=use_mvg)
     p = abi.re_params(soil=soil, solver=abi.solver_config(1e-10), bc_type=(0, 0, 0, 0, 3, 1), dx=1.0, dy=1.0,
                       dz=0.05, dt=2.0, adaptive_dt=0, use_full3d=1, htop=-0.2, initial_guess=guess,
                       picard_max=picard_max, picard_tol=1e-8)
@@ -123,13 +124,16 @@ def test_untrained_model_rejected_or_harmless():
     assert st.solve.converged
 
 
-def test_guess_kernel_matches_oracle_on_solver_state():
+@pytest.mark.parametrize("use_mvg", [0, 1])
+def test_guess_kernel_matches_oracle_on_solver_state(use_mvg):
     """The inference kernel's features (h, six neighbour differences, ln K, C, theta, depth) and its
     transposed-layer FP16 network against host-side features fed to the CPU reference, in the
-    reference's emulation of the kernel's arithmetic and in FP64."""
+    reference's emulation of the kernel's arithmetic and in FP64. The kernel evaluates ln K and C
+    itself: in single precision for van Genuchten soils, through the FP64 law for modified VG."""
     nx, ny, nz = 70, 9, 6
+    say(f"MLP guess kernel vs oracle use_mvg={use_mvg}")
     ctx = gpu.Context(0)
-    p, soil, ks, h0 = infiltration(nx, ny, nz, 31, abi.GUESS_ZERO)
+    p, soil, ks, h0 = infiltration(nx, ny, nz, 31, abi.GUESS_ZERO, use_mvg=use_mvg)
     ctx.re_init(nx, ny, nz, p)
     ctx.re_set_soil([soil], None, True, ks)
     ctx.re_set_initial(h=h0)
