@@ -646,6 +646,13 @@ void ReState::step(frehg_re_stats* st) {
     local.solve.converged = (local.solve.converged && sm.converged) ? 1 : 0;
     local.solve.fallback |= sm.fallback_fp64 ? 1 : 0;
     local.picard_iterations = m + 1;
+    // A solve from h^m that returned it unchanged (converged at its first residual check, no
+    // update): h^{m+1} = h^m bit for bit, so the test passes (max |dh| = 0), and the post-solve
+    // pass at h^{m+1} would repeat the one just done at h^m, whose results are still in place.
+    if (!zero && sm.iterations == 0 && sm.refinements == 0 && !sm.fallback_fp64) {
+      post_done = true;
+      break;
+    }
   }
   if (logging && inc0) mlp_->log_pair(*this, hn_, h_, depth_, inc0);
   // Without lazy fluxes finalize writes the final head over the post-solve head: keep the latter
