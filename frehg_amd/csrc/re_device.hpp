@@ -235,10 +235,17 @@ __device__ __forceinline__ void re_pre_stale(const RK& P, const Cell& t, const u
   ch[t.c] = v.cap;
 }
 
+// One row of the assembled system: D = Gct, the couplings to the six neighbours as stored (zero
+// where the neighbour is a ghost), rhs = Grhs.
+struct ReRow {
+  double D, xp, xm, yp, ym, zp, zm, rhs;
+};
+
 // groundwaterMatCoeff + groundwaterRhs (ReSolver.cpp:397-549) as a symmetric stencil:
 // D = Gct, ax/ay/az = Gxp/Gyp/Gzp where the +neighbour is interior (Gxm(c) == Gxp(c-1) etc.).
 template <class RK>
-__device__ __forceinline__ void re_assemble(const RK& P, const Cell& t, const double* kr, const double* ch, const double* wcn, const double* hn, double* D, double* ax, double* ay, double* az, double* rhs, float* D32, float* ax32, float* ay32, float* az32) {
+__device__ __forceinline__ ReRow re_assemble_row(const RK& P, const Cell& t, const double* kr, const double* ch,
+                                                 const double* wcn, const double* hn) {
   const Geo& g = P.g;
   const long long c = t.c;
   const double Kxp = face_x(P, kr, wcn, c, t.i), Kxm = face_x(P, kr, wcn, c - 1, t.i - 1);
@@ -284,20 +291,41 @@ __device__ __forceinline__ void re_assemble(const RK& P, const Cell& t, const do
     const double qcol = P.qtop_field ? P.qtop_field[g.sidx(t.i, t.j)] : P.qtop;
     r += -dt * V * (qcol + Kzm) / dzk;
   }
-  const double axv = (t.i < g.nx - 1) ? Gxp : 0.0;
-  const double ayv = (t.j < g.ny - 1) ? Gyp : 0.0;
-  const double azv = (t.k < g.nz - 1) ? Gzp : 0.0;
-  D[c] = Gct;
-  ax[c] = axv;
-  ay[c] = ayv;
-  az[c] = azv;
-  rhs[c] = r;
+  ReRow row;
+  row.D = Gct;
+  row.xp = (t.i < g.nx - 1) ? Gxp : 0.0;
+  row.xm = (t.i > 0) ? Gxm : 0.0;
+  row.yp = (t.j < g.ny - 1) ? Gyp : 0.0;
+  row.ym = (t.j > 0) ? Gym : 0.0;
+  row.zp = (t.k < g.nz - 1) ? Gzp : 0.0;
+  row.zm = (t.k > 0) ? Gzm : 0.0;
+  row.rhs = r;
+  return row;
+}
+
+// Stores row c: D, ax/ay/az (the + couplings), rhs, and the FP32 copies when D32 is set.
+__device__ __forceinline__ void re_store_row(long long c, const ReRow& row, double* D, double* ax, double* ay,
+                                             double* az, double* rhs, float* D32, float* ax32, float* ay32,
+                                             float* az32) {
+  D[c] = row.D;
+  ax[c] = row.xp;
+  ay[c] = row.yp;
+  az[c] = row.zp;
+  rhs[c] = row.rhs;
   if (D32) {
-    D32[c] = (float)Gct;
-    ax32[c] = (float)axv;
-    ay32[c] = (float)ayv;
-    az32[c] = (float)azv;
+    D32[c] = (float)row.D;
+    ax32[c] = (float)row.xp;
+    ay32[c] = (float)row.yp;
+    az32[c] = (float)row.zp;
   }
+}
+
+template <class RK>
+__device__ __forceinline__ void re_assemble(const RK& P, const Cell& t, const double* kr, const double* ch,
+                                            const double* wcn, const double* hn, double* D, double* ax, double* ay,
+                                            double* az, double* rhs, float* D32, float* ax32, float* ay32,
+                                            float* az32) {
+  re_store_row(t.c, re_assemble_row(P, t, kr, ch, wcn, hn), D, ax, ay, az, rhs, D32, ax32, ay32, az32);
 }
 
 // enforceHeadBc (ReSolver.cpp:638-669) + kr, C and theta at the solved head (the second
