@@ -644,6 +644,13 @@ Mlp::Mlp(hipStream_t s, Profiler* prof) : s_(s), prof_(prof) {
   FREHG_HIP(hipMalloc(&part_, (size_t)TRAIN_BLOCKS * (kMlpParams + 1) * sizeof(float)));
   FREHG_HIP(hipMalloc(&stat_, (size_t)kStatBlocks * 2 * kMlpRow * sizeof(double)));
   FREHG_HIP(hipHostMalloc(&hbuf_, (kMlpParams + 1) * sizeof(float), hipHostMallocDefault));
+  // The guess kernel's blocks loop over chunks of cells: one round of resident blocks, so that
+  // no partial last round idles most of the chip and each block loads the weights once.
+  int dev = 0, cus = 0, per_cu = 0;
+  FREHG_HIP(hipGetDevice(&dev));
+  FREHG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  FREHG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mlp_guess<true>, 64 * GW, 0));
+  guess_blocks_ = (unsigned)std::max(1, cus * std::max(1, per_cu));
   init(1);
 }
 
@@ -784,7 +791,7 @@ void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const d
   if (wcn) F.wcn = wcn;
   const long long N = (long long)F.g.nx * F.g.ny * F.g.nz;
   const long long nchunks = (long long)((F.g.nx + RB - 1) / RB) * F.g.ny * F.g.nz;
-  const unsigned nb = (unsigned)std::min<long long>(4096, (nchunks + GW - 1) / GW);
+  const unsigned nb = (unsigned)std::min<long long>(guess_blocks_, (nchunks + GW - 1) / GW);
   (void)N;
   if (infer_stale_) {
     hipLaunchKernelGGL(k_mlp_prep_infer, dim3(1), dim3(MT), 0, s_, (const float*)params_, infer_);

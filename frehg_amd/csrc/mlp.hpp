@@ -84,6 +84,7 @@ class Mlp {
   // the parameters or the normalisation change.
   float* infer_ = nullptr;
   bool infer_stale_ = true;
+  unsigned guess_blocks_ = 4096;  // guess kernel grid: resident blocks of one round
   int stride_ = 0;
   long long cap_ = 0, count_ = 0, log_steps_ = 0;
   int adam_t_ = 0;
