@@ -186,6 +186,9 @@ def test_picard_outer_loop_matches_oracle(mode):
         compare_re(ctx, orc, label="picard")
     else:
         assert np.abs(ctx.re_get("h") - orc.get("h")).max() <= 1e-6
+        # every field after the step (K, C, theta and fluxes at the post-solve head included), also
+        # where a solve from h^m returned it unchanged and the loop ended without another pass
+        compare_re(ctx, orc, rtol=1e-6, atol_h=1e-6, label=f"picard {mode}")
 
 
 def test_heterogeneous_soil_and_top_flux():
