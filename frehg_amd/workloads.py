@@ -33,7 +33,7 @@ def grf(shape, spacing, corr_len, sigma, seed):
 
 
 class CoupledCase:
-    """Coupled surface/subsurface case: the BASELINE 512x512 surface over 512x512x30 soil."""
+    """Coupled surface/subsurface benchmark case: a 512x512 surface over 512x512x30 soil."""
 
     def __init__(self, nx=512, ny=512, nz=30, dx=1.0, dz=0.05, seed_bed=1, seed_ks=2, slope=0.01,
                  bed_sigma=0.05, bed_corr=20.0, ks_median=1e-5, ks_sigma_ln=1.0, ks_corr=10.0,

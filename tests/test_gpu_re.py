@@ -1,7 +1,7 @@
 """Richards PCA step: HIP path (through the C ABI) vs the FP64 oracle on identical inputs.
 
 Cases follow the reference's RE tests (tests/re/test_re_3d.cpp, test_re_b2_gw.cpp) plus the
-BASELINE 40x40x50 infiltration column at reduced lateral size, saturation/reallocation-heavy
+40x40x50 infiltration benchmark column (at reduced lateral size and at its own size), saturation/reallocation-heavy
 cases, heterogeneous soil, every bottom/top boundary mode, adaptive dt and the FP32-in-FP64
 refinement solver. Tolerance: head 1e-9 m per step for the FP64 path (the stated bar is 1e-6 m),
 1e-9 relative for the other fields; mixed precision: head 1e-6 m.
@@ -19,8 +19,8 @@ pytestmark = pytest.mark.gpu
 
 
 def c1_params(nx, ny, nz, bc4=0, dt=2.0, **kw):
-    """BASELINE infiltration column: alpha 3.35, n 2, theta_s .368, theta_r .102, Ks 9.22e-5,
-    Ss 1e-6, dx=dy=0.1, dz=0.02, top Dirichlet -0.75, fixed dt, full 3D. At the BASELINE dt of
+    """The 40x40x50 infiltration benchmark column: alpha 3.35, n 2, theta_s .368, theta_r .102, Ks 9.22e-5,
+    Ss 1e-6, dx=dy=0.1, dz=0.02, top Dirichlet -0.75, fixed dt, full 3D. At the benchmark dt of
     10 s the PCA corrector drains the top cell below theta_r in the first step (the explicit
     flux divergence exceeds the cell's water; the clamp then sets h = h(theta_r + 1e-7) ~ -8e5 m),
     exactly as the reference's algorithm does; that regime is covered by a relative-tolerance test."""
@@ -222,6 +222,23 @@ def test_adaptive_dt_b2gw_column():
         assert sg.dt_used == pytest.approx(so.dt_used, rel=1e-12), f"step {s}"
     compare_re(ctx, orc, label="b2gw", atol_h=1e-8, rtol=1e-8)
     assert ctx.re_get("wc")[1 + 3 + 9] > 0.033
+
+
+def test_benchmark_column_full_size_picard_matches_oracle():
+    """The 40x40x50 infiltration benchmark at its own size and dt (10 s) with the Picard loop to
+    1e-8 m (at most 50 iterates): the first 60 of its 360 steps, FP64 GPU path vs the oracle, with
+    the same Picard count every step (the first step stops at the 50-iterate cap in both), every
+    field at the end, and a closed water ledger every step."""
+    nx, ny, nz = 40, 40, 50
+    say("RE 40x40x50 benchmark column, Picard, 60 steps")
+    p = c1_params(nx, ny, nz, dt=10.0, picard_max=50, picard_tol=1e-8)
+    orc, ctx = re_pair(nx, ny, nz, p, h0=np.full(nx * ny * nz, -10.0))
+    for s in range(60):
+        so = orc.step()
+        sg = ctx.re_step()
+        assert sg.picard_iterations == so.picard_iterations, (s, sg.picard_iterations, so.picard_iterations)
+        assert abs(sg.mass_residual) <= 1e-10 * sg.mass_before
+    compare_re(ctx, orc, label="40x40x50 Picard")
 
 
 def test_baseline_dt_collapse_regime():
