@@ -629,11 +629,14 @@ void ReState::step(frehg_re_stats* st) {
     FREHG_HIP(hipMemcpyAsync(hres_, dres_, sizeof(double), hipMemcpyDeviceToHost, s_));
     FREHG_HIP(hipStreamSynchronize(s_));
     if (hres_[0] <= p_.picard_tol) break;
+    // The FP32 copy of the operator is the one of the step's first assembly: it serves only the
+    // inner solves of the refinement, whose outer residuals use this iterate's FP64 operator, and
+    // the loop's later solves seldom take an inner solve.
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
       hipLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
-                         (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, mixed ? D32_ : nullptr,
-                         ax32_, ay32_, az32_);
+                         (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, nullptr, ax32_, ay32_,
+                         az32_);
       FREHG_LAUNCH_CHECK();
     });
     post_done = false;
