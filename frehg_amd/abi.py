@@ -8,10 +8,11 @@ reference's parameter structs: SoilParams (include/frehg2/re/SoilParams.hpp:13-2
 """
 import ctypes as C
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 OK = 0
 INFO_GUESS_FALLBACK = 1
+INFO_PICARD_NOT_CONVERGED = 2
 ERR_INVALID_ARG = -1
 ERR_OOM = -2
 ERR_HIP = -3
@@ -95,7 +96,7 @@ class ReStats(C.Structure):
                 ("picard_iterations", C.c_int32), ("guess_fallback", C.c_int32),
                 ("max_dh", C.c_double), ("mass_before", C.c_double), ("mass_after", C.c_double),
                 ("mass_boundary_in", C.c_double), ("mass_storage_ss", C.c_double), ("mass_adjust", C.c_double),
-                ("mass_residual", C.c_double)]
+                ("mass_residual", C.c_double), ("picard_converged", C.c_int32)]
 
 
 class EnsStats(C.Structure):

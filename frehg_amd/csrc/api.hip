@@ -205,6 +205,7 @@ int frehg_re_step(frehg_ctx_t ctx, frehg_re_stats* st) {
     frehg_re_stats local;
     need_re(ctx).step(&local);
     if (st) *st = local;
+    if (!local.picard_converged) return FREHG_INFO_PICARD_NOT_CONVERGED;
     return local.guess_fallback ? FREHG_INFO_GUESS_FALLBACK : FREHG_OK;
   });
 }
@@ -248,6 +249,14 @@ int frehg_swe_set_bathymetry(frehg_ctx_t ctx, const double* bed) {
 int frehg_swe_initialize_state(frehg_ctx_t ctx, double init_eta) {
   return guard(ctx, [&] {
     need_swe(ctx).initialize_state(init_eta);
+    return FREHG_OK;
+  });
+}
+
+int frehg_swe_set_initial_eta(frehg_ctx_t ctx, const double* eta_interior) {
+  return guard(ctx, [&] {
+    if (!eta_interior) throw frehg::ArgError("null argument");
+    need_swe(ctx).initialize_eta(eta_interior, 0.0);
     return FREHG_OK;
   });
 }
@@ -308,11 +317,12 @@ int frehg_coupled_step(frehg_ctx_t ctx, const frehg_coupling_params* cp, double 
     auto& cpl = need_cpl(ctx);
     cpl.exchange_async(swe, re, *cp, dt);  // its volume is read after the groundwater step
     const double eps = 1.0e-9 * std::max(1.0, t_end);  // gwCatchUpTo (:1243-1252)
-    int fallback = 0, iters = 0;
+    int fallback = 0, iters = 0, unconverged = 0;
     while (ctx->gw_time < t_new - eps) {
       re.step(&local.gw);
       iters += local.gw.solve.iterations;
       fallback |= local.gw.guess_fallback;
+      unconverged |= !local.gw.picard_converged;
       ctx->gw_time += re.dt_used();
       ++local.gw_substeps;
     }
@@ -322,6 +332,7 @@ int frehg_coupled_step(frehg_ctx_t ctx, const frehg_coupling_params* cp, double 
     local.t = ctx->t;
     local.gw_time = ctx->gw_time;
     if (st) *st = local;
+    if (unconverged) return FREHG_INFO_PICARD_NOT_CONVERGED;
     return fallback ? FREHG_INFO_GUESS_FALLBACK : FREHG_OK;
   });
 }
@@ -610,6 +621,14 @@ int frehg_mlp_loss_grad(frehg_ctx_t ctx, const float* x, const float* y, int n, 
   return guard(ctx, [&] {
     const double l = need_mlp(ctx).loss_grad(x, y, n, grad);
     if (loss) *loss = l;
+    return FREHG_OK;
+  });
+}
+
+int frehg_mlp_gate(frehg_ctx_t ctx, double gate) {
+  return guard(ctx, [&] {
+    if (!(gate >= 0.0)) throw frehg::ArgError("frehg_mlp_gate: gate must be >= 0");
+    need_mlp(ctx).set_gate(gate);
     return FREHG_OK;
   });
 }

@@ -30,6 +30,23 @@ struct ConvergenceError : std::runtime_error {
 
 #define FREHG_LAUNCH_CHECK() FREHG_HIP(hipGetLastError())
 
+// Owning device allocation of n elements of T, freed on scope exit (also when an exception leaves
+// the scope).
+template <typename T>
+class DeviceBuffer {
+ public:
+  explicit DeviceBuffer(size_t n) { FREHG_HIP(hipMalloc(&p_, n * sizeof(T))); }
+  ~DeviceBuffer() {
+    if (p_) (void)hipFree(p_);
+  }
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  T* get() const { return p_; }
+
+ private:
+  T* p_ = nullptr;
+};
+
 constexpr int kWave = 64;
 
 // ---------------------------------------------------------------------------------------------

@@ -613,6 +613,15 @@ __global__ void kb_status(const EnsCtl* ctl, int nbox, int* flags) {
 
 Ensemble::Ensemble(int nbox, const int32_t* dims, const frehg_re_params* params, hipStream_t s, Profiler* prof)
     : nbox_(nbox), s_(s), prof_(prof) {
+  try {
+    build(nbox, dims, params);
+  } catch (...) {
+    release();  // a failed set-up frees what it allocated
+    throw;
+  }
+}
+
+void Ensemble::build(int nbox, const int32_t* dims, const frehg_re_params* params) {
   if (nbox < 1) throw ArgError("frehg_ens_init: need at least one box");
   if (!dims || !params) throw ArgError("frehg_ens_init: null argument");
   dims_.assign(dims, dims + 3 * nbox);
@@ -698,7 +707,9 @@ Ensemble::Ensemble(int nbox, const int32_t* dims, const frehg_re_params* params,
   FREHG_HIP(hipStreamSynchronize(s_));
 }
 
-Ensemble::~Ensemble() {
+Ensemble::~Ensemble() { release(); }
+
+void Ensemble::release() {
   for (auto& g : graph_)
     if (g) (void)hipGraphExecDestroy(g);
   std::vector<void*> ptrs = {F_.h,   F_.hn,   F_.wc,  F_.wcn,  F_.wch,  F_.hwc,  F_.ch,   F_.kr,   F_.Kx,   F_.Ky,
@@ -717,6 +728,18 @@ Ensemble::~Ensemble() {
   if (hflags_) (void)hipHostFree(hflags_);
   if (hboxsum_) (void)hipHostFree(hboxsum_);
   if (hctl_) (void)hipHostFree(hctl_);
+  for (auto& g : graph_) g = nullptr;
+  F_ = Pool{};
+  dbox_ = nullptr;
+  dtile0_ = dspmv0_ = nullptr;
+  ctl_ = nullptr;
+  part_ = boxsum_ = nullptr;
+  flags_ = hflags_ = nullptr;
+  hboxsum_ = nullptr;
+  hctl_ = nullptr;
+  classes_.clear();
+  cls_.clear();
+  ks_.clear();
 }
 
 void Ensemble::upload_boxes() {
@@ -1205,6 +1228,7 @@ void Ensemble::step(int use_graph, frehg_ens_stats* st, frehg_re_stats* per_box)
       r.dt_used = params_[b].dt;
       r.dt_next = params_[b].dt;
       r.picard_iterations = 1;
+      r.picard_converged = 1;
       r.max_dh = s[5];
       r.mass_before = s[0];
       r.mass_storage_ss = s[1];

@@ -54,6 +54,8 @@ class Ensemble {
     float *D32, *ax32, *ay32, *az32, *b32, *r32, *z32, *p032, *p132, *q32, *d32;
     unsigned char *sat0, *cand, *satA, *satB;
   };
+  void build(int nbox, const int32_t* dims, const frehg_re_params* params);
+  void release();
   double* field(const std::string& name);
   void upload_boxes();
   void autotune();

@@ -1035,6 +1035,15 @@ __global__ __launch_bounds__(NT) void k_zero_interior(Geo g, int kchunk, T* x,
 
 Krylov::Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof)
     : g_(g), three_d_(three_d), s_(stream), prof_(prof) {
+  try {
+    build();
+  } catch (...) {
+    release();  // a failed set-up frees what it allocated
+    throw;
+  }
+}
+
+void Krylov::setup_shapes() {
   if (!three_d_) g_.nz = 1;
   n_ = three_d_ ? g_.ncell() : g_.nsurf();
   kc_f_ = 1;
@@ -1042,8 +1051,6 @@ Krylov::Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof)
   nb_f_ = grid_f_.x * grid_f_.y * grid_f_.z;
   // partials sized for the largest grid any kernel (or candidate) uses: 64 x 4 tiles, one layer each
   part_cap_ = ceil_div(g_.nx, TX) * ceil_div(g_.ny, TY) * (unsigned int)(three_d_ ? g_.nz : 1);
-  FREHG_HIP(hipMalloc(&part_, 5 * (size_t)part_cap_ * sizeof(double)));  // up to 5 sums per block
-  FREHG_HIP(hipMalloc(&part_u_, 2 * (size_t)part_cap_ * sizeof(double)));
   // CG update: whole columns per block in 3D (its partials are reduced by every SpMV block, so
   // their number is kept small)
   kc_u_ = three_d_ ? g_.nz : 1;
@@ -1063,6 +1070,12 @@ Krylov::Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof)
       tuned_[prec] = true;
     }
   }
+}
+
+void Krylov::build() {
+  setup_shapes();
+  FREHG_HIP(hipMalloc(&part_, 5 * (size_t)part_cap_ * sizeof(double)));  // up to 5 sums per block
+  FREHG_HIP(hipMalloc(&part_u_, 2 * (size_t)part_cap_ * sizeof(double)));
   const size_t bytes = (size_t)n_ * sizeof(double);
   for (double** p : {&r_, &z_, &p0_, &p1_, &q_, &xw_}) {
     FREHG_HIP(hipMalloc(p, bytes));
@@ -1074,7 +1087,9 @@ Krylov::Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof)
   FREHG_HIP(hipStreamSynchronize(s_));
 }
 
-Krylov::~Krylov() {
+Krylov::~Krylov() { release(); }
+
+void Krylov::release() {
   for (void* p : {(void*)r_, (void*)z_, (void*)p0_, (void*)p1_, (void*)q_, (void*)xw_, (void*)b32_,
                   (void*)r32_, (void*)z32_, (void*)p032_, (void*)p132_, (void*)q32_, (void*)d32_,
                   (void*)rh_, (void*)v0_, (void*)v1_, (void*)sv_, (void*)tv_, (void*)rh32_, (void*)v032_,

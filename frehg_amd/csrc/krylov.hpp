@@ -118,6 +118,9 @@ class Krylov {
   bool tuned(int prec) const { return tuned_[prec]; }
 
  private:
+  void setup_shapes();
+  void build();
+  void release();
   template <typename T>
   int run_pcg(const StencilView<T>& A, const T* b, const T* bn, T* x, const unsigned char* fixed,
               bool x0_zero, bool guard, double rtol, double atol, int max_it, KrylovCtl* ctl,

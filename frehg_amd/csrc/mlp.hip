@@ -81,6 +81,12 @@ struct FeatSrc {
   // inc_a alone; inc_a null: zero): the network predicts the increment minus it.
   const double* inc_a;
   const double* inc_b;
+  // Inference gate: with a previous increment available and gate > 0, a cell whose |previous
+  // increment| <= gate takes the extrapolated start (no network correction).
+  double gate;
+  // Set to 1 by the inference kernel when a network output is not finite (that cell then takes
+  // the extrapolated start); may be null.
+  double* bad;
 };
 
 __device__ __forceinline__ double base_increment(const FeatSrc& F, long long c) {
@@ -520,12 +526,24 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const flo
     for (int t = 0; t < kMlpIn; ++t) f[t] = 0.0f;
     long long c = 0;
     double base = 0.0;  // h^n + base increment of this lane's cell
+    bool need = false;  // this lane's cell takes the network's correction
     if (i < g3.nx) {
       c = g3.idx(i, j, k);
-      base = F.hn[c] + base_increment(F, c);
+      const double inc = base_increment(F, c);
+      base = F.hn[c] + inc;
+      need = !(F.gate > 0.0 && F.inc_a) || fabs(inc) > F.gate;
+    }
+    if (!__any(need)) {  // the whole chunk takes the extrapolated start (wave-uniform)
+      if (i < g3.nx) h_out[c] = base;
+      continue;
+    }
+    if (i < g3.nx) {
       cell_features<VG_ONLY>(F, i, j, k, f);
+      // standardised features, clamped far outside any training distribution so that the FP16
+      // operands below stay finite (heads of -1e5 m and beyond occur where the corrector drains a
+      // cell to theta_r)
 #pragma unroll
-      for (int t = 0; t < kMlpIn; ++t) f[t] = (f[t] - nmu[t]) * nisd[t];
+      for (int t = 0; t < kMlpIn; ++t) f[t] = fminf(fmaxf((f[t] - nmu[t]) * nisd[t], -1024.0f), 1024.0f);
     }
     wave_lds_fence();
 #pragma unroll
@@ -573,7 +591,12 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const flo
       if (lane < 16) ys[w][16 * tile + ra] = y[0];  // D row 0 (g = 0, r = 0) holds y of cell ra
     }
     wave_lds_fence();
-    if (i < g3.nx) h_out[c] = base + (double)(ys[w][lane] * ysd + ymu);
+    if (i < g3.nx) {
+      const double y = (double)(ys[w][lane] * ysd + ymu);
+      const bool ok = isfinite(y);
+      if (!ok && F.bad) *F.bad = 1.0;
+      h_out[c] = base + ((need && ok) ? y : 0.0);
+    }
   }
 }
 
@@ -627,6 +650,8 @@ FeatSrc feat_src(ReState& re, const double* depth) {
   F.depth = depth;
   F.inc_a = nullptr;
   F.inc_b = nullptr;
+  F.gate = 0.0;
+  F.bad = nullptr;
   return F;
 }
 
@@ -782,11 +807,13 @@ double Mlp::train(int steps, int batch, double lr, uint64_t seed, double* first)
 }
 
 void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const double* inc_a, const double* inc_b,
-                        const double* hn, const double* wcn) {
+                        const double* hn, const double* wcn, double* bad) {
   if (!ready_) throw ArgError("learned guess requested but the model is not trained or loaded");
   FeatSrc F = feat_src(re, depth);
   F.inc_a = inc_a;
   F.inc_b = inc_b;
+  F.gate = gate_;
+  F.bad = bad;
   if (hn) F.hn = hn;
   if (wcn) F.wcn = wcn;
   const long long N = (long long)F.g.nx * F.g.ny * F.g.nz;
@@ -864,15 +891,13 @@ void Mlp::predict_batch(const float* X, int n, float* out) {
   for (int r = 0; r < n; ++r)
     for (int t = 0; t < kMlpIn; ++t) rows[(size_t)r * kMlpRow + t] = X[(size_t)r * kMlpIn + t];
   FREHG_HIP(hipMemcpyAsync(scratch_, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice, s_));
-  float* tmp = nullptr;
-  FREHG_HIP(hipMalloc(&tmp, (size_t)n * sizeof(float)));
+  DeviceBuffer<float> tmp((size_t)n);
   const unsigned nb = (unsigned)std::min<long long>(1024, ((long long)n + RB - 1) / RB);
   hipLaunchKernelGGL(k_mlp_predict_rows, dim3(nb), dim3(MT), 0, s_, (const float*)params_, (const float*)norm_,
-                     (const float*)scratch_, (long long)n, tmp);
+                     (const float*)scratch_, (long long)n, tmp.get());
   FREHG_LAUNCH_CHECK();
-  FREHG_HIP(hipMemcpyAsync(out, tmp, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipMemcpyAsync(out, tmp.get(), (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s_));
   FREHG_HIP(hipStreamSynchronize(s_));
-  FREHG_HIP(hipFree(tmp));
 }
 
 }  // namespace frehg

@@ -61,9 +61,14 @@ class Mlp {
 
   // h_out (interior) = h^n + (inc_a - inc_b) + predicted increment (inc_a null: no base
   // increment; inc_b may be h_out itself, each cell's value is read before it is written);
-  // hn / wcn default to the solver's h^n / theta^n.
+  // hn / wcn default to the solver's h^n / theta^n. A cell whose network output is not finite,
+  // or (with a base increment and gate > 0) whose |base increment| <= gate, takes no predicted
+  // increment; the first case sets *bad = 1 (bad may be null).
   void predict_guess(ReState& re, double* h_out, const double* depth, const double* inc_a, const double* inc_b,
-                     const double* hn = nullptr, const double* wcn = nullptr);
+                     const double* hn = nullptr, const double* wcn = nullptr, double* bad = nullptr);
+  // Inference gate in metres (0: the network corrects every cell).
+  void set_gate(double gate) { gate_ = gate; }
+  double gate() const { return gate_; }
 
   void get_params(float* params, float* norm);
   void set_params(const float* params, const float* norm);
@@ -85,6 +90,7 @@ class Mlp {
   float* infer_ = nullptr;
   bool infer_stale_ = true;
   unsigned guess_blocks_ = 4096;  // guess kernel grid: resident blocks of one round
+  double gate_ = 0.0;
   int stride_ = 0;
   long long cap_ = 0, count_ = 0, log_steps_ = 0;
   int adam_t_ = 0;

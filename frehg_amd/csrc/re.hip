@@ -268,6 +268,16 @@ __global__ __launch_bounds__(NT) void k_sub_interior(Geo g, const double* a, con
 ReState::ReState(int nx, int ny, int nz, const frehg_re_params& p, hipStream_t s, Profiler* prof)
     : g_(make_geo(nx, ny, nz)), p_(p), s_(s), prof_(prof) {
   if (nx < 1 || ny < 1 || nz < 1) throw ArgError("frehg_re_init: grid sizes must be >= 1");
+  try {
+    build(p);
+  } catch (...) {
+    release();  // a failed set-up frees what it allocated
+    throw;
+  }
+}
+
+void ReState::build(const frehg_re_params& p) {
+  const int nx = g_.nx, ny = g_.ny, nz = g_.nz;
   n_ = g_.ncell();
   const size_t bytes = (size_t)n_ * sizeof(double);
   const char* lz = std::getenv("FREHG_LAZY_FLUX");
@@ -299,7 +309,9 @@ ReState::ReState(int nx, int ny, int nz, const frehg_re_params& p, hipStream_t s
   FREHG_HIP(hipStreamSynchronize(s_));
 }
 
-ReState::~ReState() {
+ReState::~ReState() { release(); }
+
+void ReState::release() {
   for (void* f : {(void*)h_, (void*)hn_, (void*)hp_, (void*)wc_, (void*)wcn_, (void*)wch_, (void*)hwc_, (void*)ch_,
                   (void*)Kx_, (void*)Ky_, (void*)Kz_, (void*)qx_, (void*)qy_, (void*)qz_, (void*)kr_,
                   (void*)D_, (void*)ax_, (void*)ay_, (void*)az_, (void*)rhs_, (void*)D32_, (void*)ax32_,
@@ -467,26 +479,19 @@ void ReState::set_initial(const double* h_int, const double* wc_int) {
 
 void ReState::learned_increment(double* dst) {
   if (!mlp_ || !mlp_->ready()) throw ArgError("frehg_mlp_increment: no trained model");
-  double* tmp = nullptr;
-  FREHG_HIP(hipMalloc(&tmp, (size_t)n_ * sizeof(double)));
-  try {
-    FREHG_HIP(hipMemsetAsync(tmp, 0, (size_t)n_ * sizeof(double), s_));
-    mlp_->predict_guess(*this, tmp, depth_, nullptr, nullptr, h_, wc_);
-    std::vector<double> pad((size_t)n_), hh((size_t)n_);
-    FREHG_HIP(hipMemcpyAsync(pad.data(), tmp, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
-    FREHG_HIP(hipMemcpyAsync(hh.data(), h_, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
-    FREHG_HIP(hipStreamSynchronize(s_));
-    for (int k = 0; k < g_.nz; ++k)
-      for (int j = 0; j < g_.ny; ++j)
-        for (int i = 0; i < g_.nx; ++i) {
-          const size_t c = (size_t)g_.idx(i, j, k);
-          dst[(size_t)i + (size_t)j * g_.nx + (size_t)k * g_.nx * g_.ny] = pad[c] - hh[c];
-        }
-  } catch (...) {
-    (void)hipFree(tmp);
-    throw;
-  }
-  FREHG_HIP(hipFree(tmp));
+  DeviceBuffer<double> tmp((size_t)n_);
+  FREHG_HIP(hipMemsetAsync(tmp.get(), 0, (size_t)n_ * sizeof(double), s_));
+  mlp_->predict_guess(*this, tmp.get(), depth_, nullptr, nullptr, h_, wc_);
+  std::vector<double> pad((size_t)n_), hh((size_t)n_);
+  FREHG_HIP(hipMemcpyAsync(pad.data(), tmp.get(), (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipMemcpyAsync(hh.data(), h_, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipStreamSynchronize(s_));
+  for (int k = 0; k < g_.nz; ++k)
+    for (int j = 0; j < g_.ny; ++j)
+      for (int i = 0; i < g_.nx; ++i) {
+        const size_t c = (size_t)g_.idx(i, j, k);
+        dst[(size_t)i + (size_t)j * g_.nx + (size_t)k * g_.nx * g_.ny] = pad[c] - hh[c];
+      }
 }
 
 void ReState::materialize_fluxes() {
@@ -576,7 +581,9 @@ void ReState::step(frehg_re_stats* st) {
     x0_zero = false;
   } else if (p_.initial_guess == FREHG_GUESS_LEARNED) {
     if (!mlp_ || !mlp_->ready()) throw ArgError("frehg_re_step: learned guess requested but no trained model loaded");
-    mlp_->predict_guess(*this, h_, depth_, have_inc ? hp_ : nullptr, have_inc ? h_ : nullptr);
+    FREHG_HIP(hipMemsetAsync(dres_ + 10, 0, sizeof(double), s_));
+    mlp_->predict_guess(*this, h_, depth_, have_inc ? hp_ : nullptr, have_inc ? h_ : nullptr, nullptr, nullptr,
+                        dres_ + 10);
     x0_zero = false;
     guard = true;
   }
@@ -599,7 +606,30 @@ void ReState::step(frehg_re_stats* st) {
   const KrylovConfig kc = krylov_config(p_.solver, "frehg_re_step");
   StencilView<double> A{D_, ax_, ay_, az_};
   StencilView<float> A32{D32_, ax32_, ay32_, az32_};
-  const SolveResult sr = kry_->solve(A, mixed ? &A32 : nullptr, rhs_, nullptr, h_, nullptr, x0_zero, guard, kc);
+  SolveResult sr = kry_->solve(A, mixed ? &A32 : nullptr, rhs_, nullptr, h_, nullptr, x0_zero, guard, kc);
+  // A learned Picard start whose residual did not drop below the zero guess's: the solve above
+  // started from zero, but on the operator assembled at the rejected head. The loop restarts as
+  // the plain method: h^0 = h^n, K and C at h^n, x0 = 0.
+  bool from_hn = !start_h0;
+  if (start_h0 && sr.guess_rejected) {
+    with_soil_variant(P, [&](const auto& PK) {
+      using K = std::decay_t<decltype(PK)>;
+      hipLaunchKernelGGL((k_re_pre<K, 1>), grid, blk, 0, s_, PK, (const double*)hn_, kr_, ch_);
+      FREHG_LAUNCH_CHECK();
+      hipLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
+                         (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, mixed ? D32_ : nullptr,
+                         ax32_, ay32_, az32_);
+      FREHG_LAUNCH_CHECK();
+    });
+    const SolveResult s0 = kry_->solve(A, mixed ? &A32 : nullptr, rhs_, nullptr, h_, nullptr, true, false, kc);
+    sr.iterations += s0.iterations;
+    sr.refinements += s0.refinements;
+    sr.residual_norm = s0.residual_norm;
+    sr.rhs_norm = s0.rhs_norm;
+    sr.converged = s0.converged;
+    sr.fallback_fp64 = sr.fallback_fp64 || s0.fallback_fp64;
+    from_hn = true;
+  }
   local.solve.iterations = sr.iterations;
   local.solve.refinements = sr.refinements;
   local.solve.residual_norm = sr.residual_norm;
@@ -608,6 +638,8 @@ void ReState::step(frehg_re_stats* st) {
   local.solve.fallback = sr.fallback_fp64 ? 1 : 0;
   local.guess_fallback = sr.guess_rejected ? 1 : 0;
   local.picard_iterations = 1;
+  local.picard_converged = p_.picard_max > 1 ? 0 : 1;
+  bool refined = sr.refinements > 0;  // the last solve took a refinement (mixed precision)
   // Picard outer loop (picard_max > 1; a new method, the reference's step is its first iterate):
   // K and C at the iterate h^m with its head-BC ghosts, the storage and rhs terms at time level n
   // as in the predictor, solve for h^{m+1} (from x0 = 0 with the zero guess, else from h^m); stop
@@ -616,31 +648,36 @@ void ReState::step(frehg_re_stats* st) {
   // the test fused in; at the last one it is the step's own post-solve evaluation.
   bool post_done = false;
   for (int m = 1; p_.picard_max > 1; ++m) {
-    const double* ref = (m == 1 && !start_h0) ? hn_ : hp_;
+    const double* ref = (m == 1 && from_hn) ? hn_ : hp_;
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
       hipLaunchKernelGGL((k_re_post_delta<K, 1>), grid, blk, 0, s_, PK, h_, kr_, ch_, wch_, ref, hp_, red_, nbl);
       FREHG_LAUNCH_CHECK();
     });
     post_done = true;
-    if (m >= p_.picard_max) break;
     hipLaunchKernelGGL(k_reduce_maxmin, dim3(1), dim3(kRedThreads), 0, s_, (const double*)red_, nbl, dres_);
     FREHG_LAUNCH_CHECK();
     FREHG_HIP(hipMemcpyAsync(hres_, dres_, sizeof(double), hipMemcpyDeviceToHost, s_));
     FREHG_HIP(hipStreamSynchronize(s_));
-    if (hres_[0] <= p_.picard_tol) break;
-    // The FP32 copy of the operator is the one of the step's first assembly: it serves only the
-    // inner solves of the refinement, whose outer residuals use this iterate's FP64 operator, and
-    // the loop's later solves seldom take an inner solve.
+    if (hres_[0] <= p_.picard_tol) {
+      local.picard_converged = 1;
+      break;
+    }
+    if (m >= p_.picard_max) break;  // the cap: the step continues from the last iterate
+    const bool zero = p_.initial_guess == FREHG_GUESS_ZERO;
+    // The FP32 copy of the operator is refreshed when the coming solve will need inner solves:
+    // from x0 = 0, or after a solve that took a refinement. A solve from h^m that converged at
+    // its first residual check leaves the previous copy in place (its inner solves, if any,
+    // would only serve refinements whose outer residuals use this iterate's FP64 operator).
+    const bool fresh32 = mixed && (zero || refined);
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
       hipLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
-                         (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, nullptr, ax32_, ay32_,
-                         az32_);
+                         (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, fresh32 ? D32_ : nullptr,
+                         ax32_, ay32_, az32_);
       FREHG_LAUNCH_CHECK();
     });
     post_done = false;
-    const bool zero = p_.initial_guess == FREHG_GUESS_ZERO;
     const SolveResult sm = kry_->solve(A, mixed ? &A32 : nullptr, rhs_, nullptr, h_, nullptr, zero, false, kc);
     local.solve.iterations += sm.iterations;
     local.solve.refinements += sm.refinements;
@@ -649,11 +686,13 @@ void ReState::step(frehg_re_stats* st) {
     local.solve.converged = (local.solve.converged && sm.converged) ? 1 : 0;
     local.solve.fallback |= sm.fallback_fp64 ? 1 : 0;
     local.picard_iterations = m + 1;
+    refined = sm.refinements > 0;
     // A solve from h^m that returned it unchanged (converged at its first residual check, no
     // update): h^{m+1} = h^m bit for bit, so the test passes (max |dh| = 0), and the post-solve
     // pass at h^{m+1} would repeat the one just done at h^m, whose results are still in place.
     if (!zero && sm.iterations == 0 && sm.refinements == 0 && !sm.fallback_fp64) {
       post_done = true;
+      local.picard_converged = 1;
       break;
     }
   }
@@ -747,8 +786,10 @@ void ReState::step(frehg_re_stats* st) {
     if (p_.dt < p_.dt_min) p_.dt = p_.dt_min;
     dtn_ = p_.dt;
   }
-  FREHG_HIP(hipMemcpyAsync(hres_ + 2, dres_ + 2, 6 * sizeof(double), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipMemcpyAsync(hres_ + 2, dres_ + 2, 9 * sizeof(double), hipMemcpyDeviceToHost, s_));
   FREHG_HIP(hipStreamSynchronize(s_));
+  // a network output that was not finite: those cells started from h^n + d^{n-1} instead
+  if (p_.initial_guess == FREHG_GUESS_LEARNED && hres_[10] != 0.0) local.guess_fallback = 1;
   local.mass_before = hres_[2];
   local.mass_storage_ss = hres_[3];
   local.mass_boundary_in = hres_[4];

@@ -67,6 +67,8 @@ class ReState {
   }
 
  private:
+  void build(const frehg_re_params& p);
+  void release();
   double* field_ptr(const std::string& name);
 
   Geo g_;
@@ -114,7 +116,9 @@ class ReState {
   double* red_ = nullptr;  // reduction partials (adaptive dt)
   double* red2_ = nullptr; // first-stage sums of the ledger reduction
   double* led_ = nullptr;  // water-ledger partials: 5 sums + max|dh| (+ pad) per block
-  double* dres_ = nullptr; // device results [dq_max, dt_Comin, max_dh]
+  // device results: [0..1] Picard max |dh| or adaptive-dt max/min, [2..6] ledger sums, [7..8] max
+  // |h^{n+1} - h^n| (and its unused min), [10] learned guess had a non-finite output
+  double* dres_ = nullptr;
   double* hres_ = nullptr;
   unsigned int* dcount_ = nullptr;
   // Soil.

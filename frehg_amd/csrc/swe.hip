@@ -539,7 +539,9 @@ void SweState::set_bathymetry(const double* bed) {
 }
 
 // initializeState (SweSolver.cpp:926-934) with finalizeInitialState (:904-924).
-void SweState::initialize_state(double init_eta) {
+void SweState::initialize_state(double init_eta) { initialize_eta(nullptr, init_eta); }
+
+void SweState::initialize_eta(const double* eta_phys, double init_eta) {
   const int nx = g_.nx, ny = g_.ny;
   std::vector<double> eta((size_t)n_), bot((size_t)n_);
   get_raw("eta", eta.data());
@@ -548,7 +550,7 @@ void SweState::initialize_state(double init_eta) {
   // setInitialEtaPhysical (:855-870) + enforceSurfBc (:371-384)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
-      eta[s(i, j)] = init_eta + p_.offset;
+      eta[s(i, j)] = (eta_phys ? eta_phys[(size_t)i + (size_t)j * nx] : init_eta) + p_.offset;
       if (eta[s(i, j)] < bot[s(i, j)]) eta[s(i, j)] = bot[s(i, j)];
     }
   for (int i = 0; i < nx; ++i) {

@@ -19,6 +19,9 @@ class SweState {
   void set_params(const frehg_swe_params& p);
   void set_bathymetry(const double* bed_interior);
   void initialize_state(double init_eta);
+  // setInitialEtaPhysical + setInitialVelocityConstant(0, 0) + finalizeInitialState
+  // (src/swe/SweSolver.cpp:855-924); eta_phys: nx*ny values (i + j*nx), or null for init_eta everywhere.
+  void initialize_eta(const double* eta_phys, double init_eta);
   // fields in the reference's packed halo-padded layout ((nx + 2) x (ny + 2))
   void set_field(const std::string& name, const double* src);
   void get_field(const std::string& name, double* dst);

@@ -57,6 +57,7 @@ def load():
         "frehg_swe_set_params": (C.c_int, [ctx, P(abi.SweParams)]),
         "frehg_swe_set_bathymetry": (C.c_int, [ctx, dp]),
         "frehg_swe_initialize_state": (C.c_int, [ctx, C.c_double]),
+        "frehg_swe_set_initial_eta": (C.c_int, [ctx, dp]),
         "frehg_swe_set_field": (C.c_int, [ctx, C.c_char_p, dp]),
         "frehg_swe_get_field": (C.c_int, [ctx, C.c_char_p, dp]),
         "frehg_swe_step": (C.c_int, [ctx, C.c_double, C.c_double, P(abi.SolveStats)]),
@@ -86,6 +87,7 @@ def load():
         "frehg_mlp_loss_grad": (C.c_int, [ctx, P(C.c_float), P(C.c_float), C.c_int, P(C.c_float), dp]),
         "frehg_mlp_predict": (C.c_int, [ctx, P(C.c_float), C.c_int, P(C.c_float)]),
         "frehg_mlp_increment": (C.c_int, [ctx, dp]),
+        "frehg_mlp_gate": (C.c_int, [ctx, C.c_double]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -115,6 +117,7 @@ class Context:
             raise FrehgError(rc, "frehg_create failed (no GPU?)")
         self.re_dims = None
         self.swe_dims = None
+        self.last_rc = 0  # return code of the last checked call (> 0: FREHG_INFO_*)
 
     def close(self):
         if self.h:
@@ -128,6 +131,7 @@ class Context:
             pass
 
     def check(self, rc, allow_info=True):
+        self.last_rc = rc
         if rc < 0 or (rc > 0 and not allow_info):
             raise FrehgError(rc, self.lib.frehg_last_error(self.h).decode())
         return rc
@@ -189,6 +193,11 @@ class Context:
 
     def swe_initialize_state(self, eta0):
         self.check(self.lib.frehg_swe_initialize_state(self.h, eta0))
+
+    def swe_set_initial_eta(self, eta_phys):
+        a = np.ascontiguousarray(eta_phys, dtype=np.float64).ravel()
+        assert a.size == self.swe_dims[0] * self.swe_dims[1]
+        self.check(self.lib.frehg_swe_set_initial_eta(self.h, _ptr(a)))
 
     def swe_nsurf(self):
         nx, ny = self.swe_dims
@@ -340,6 +349,9 @@ class Context:
         out = np.zeros(len(x), np.float32)
         self.check(self.lib.frehg_mlp_predict(self.h, _fp(x), len(x), _fp(out)))
         return out
+
+    def mlp_gate(self, gate):
+        self.check(self.lib.frehg_mlp_gate(self.h, float(gate)))
 
     def mlp_increment(self):
         out = np.zeros(int(np.prod(self.re_dims)))

@@ -23,11 +23,13 @@
 extern "C" {
 #endif
 
-#define FREHG_ABI_VERSION 2
+#define FREHG_ABI_VERSION 3
 
 enum {
   FREHG_OK = 0,
   FREHG_INFO_GUESS_FALLBACK = 1, /* learned initial guess rejected, zero guess used */
+  FREHG_INFO_PICARD_NOT_CONVERGED = 2, /* Picard loop stopped at picard_max before its max |dh| test
+                                          passed; the step is taken from the last iterate */
   FREHG_ERR_INVALID_ARG = -1,
   FREHG_ERR_OOM = -2,
   FREHG_ERR_HIP = -3,
@@ -128,6 +130,7 @@ typedef struct {
   double mass_storage_ss;  /* -sum theta_c V Ss (h - h^n) / theta_s: specific storage, outside theta V */
   double mass_adjust;      /* sum (theta^{n+1} - theta_c) V: reallocation and clamp */
   double mass_residual;    /* mass_after - mass_before - boundary_in - storage_ss - adjust */
+  int32_t picard_converged; /* 1 if the Picard test passed (always 1 with picard_max = 1) */
 } frehg_re_stats;
 
 typedef struct {
@@ -181,6 +184,10 @@ int frehg_swe_set_params(frehg_ctx_t ctx, const frehg_swe_params* p);
 int frehg_swe_set_bathymetry(frehg_ctx_t ctx, const double* bed_interior);
 /* SweSolver::initializeState (:926-934): eta = init + offset clamped to bed, u = v = 0. */
 int frehg_swe_initialize_state(frehg_ctx_t ctx, double init_eta);
+/* SweSolver::setInitialEtaPhysical + setInitialVelocityConstant(0, 0) + finalizeInitialState
+ * (src/swe/SweSolver.cpp:855-924): a non-uniform initial surface, nx*ny physical values
+ * (i + j*nx, the offset is added), clamped to the bed, zero velocity. */
+int frehg_swe_set_initial_eta(frehg_ctx_t ctx, const double* eta_interior);
 /* Any SweFields array by name (src/swe/SweFields.hpp:131-146), halo padded. */
 int frehg_swe_set_field(frehg_ctx_t ctx, const char* name, const double* src);
 int frehg_swe_get_field(frehg_ctx_t ctx, const char* name, double* dst);
@@ -270,6 +277,10 @@ int frehg_mlp_set_params(frehg_ctx_t ctx, const float* params, const float* norm
 int frehg_mlp_loss_grad(frehg_ctx_t ctx, const float* x, const float* y, int n, float* grad, double* loss);
 /* Predicted raw increments for x[n][11]. */
 int frehg_mlp_predict(frehg_ctx_t ctx, const float* x, int n, float* out);
+/* Inference gate (metres, default 0): with a previous solve increment d^{n-1} available, a cell
+ * whose |d^{n-1}| <= gate starts from h^n + d^{n-1} without the network's correction, and a chunk
+ * of cells that all do so skips the network. 0 applies the network to every cell. */
+int frehg_mlp_gate(frehg_ctx_t ctx, double gate);
 /* The network's prediction for a step taken from the current Richards state (features from h,
  * K(h), C(h), theta): the part of the learned guess's increment on top of the previous step's
  * solve increment. nx*ny*nz values, i + j*nx + k*nx*ny. */

@@ -233,6 +233,7 @@ struct ReParams {  // ReSolver.hpp:24-46 (defaults kept)
 struct ReStepStats {
   SolveStats solve;         // the last Picard iteration's solve (iterations: all of them)
   int picard_iterations = 1;
+  bool picard_converged = true;  // the max |dh| test passed (picard_max = 1: no test, true)
   real dt_used = 0.0;
   real dt_next = 0.0;
   real max_dh = 0.0;
@@ -258,6 +259,13 @@ class ReSolver {
   void setInitialHead(const std::vector<real>& h);            // :155-173
   void finalizeInitialState();                                // :175-178
   void advanceStep(ReStepStats* st = nullptr);                // :931-969
+  // The step's tail on the fields as they stand (reallocateWaterContent, clampWaterContent,
+  // enforceMoistureBc: ReSolver.cpp:788-878), for hand-built reallocation cases.
+  void reallocateAndClamp() {
+    reallocateWaterContent();
+    clampWaterContent();
+    enforceMoistureBc();
+  }
 
   Grid g;
   // GwFields (src/re/GwFields.hpp:14-50), halo padded.

@@ -209,6 +209,9 @@ int orc_re_set(void* h, const char* name, const double* src) {
     std::memcpy(f->data(), src, f->size() * sizeof(double));
   });
 }
+int orc_re_reallocate(void* h) {
+  return guard([&] { static_cast<ReSolver*>(h)->reallocateAndClamp(); });
+}
 int orc_re_step(void* h, frehg_re_stats* st) {
   return guard([&] {
     ReStepStats s;
@@ -219,6 +222,7 @@ int orc_re_step(void* h, frehg_re_stats* st) {
       st->dt_used = s.dt_used;
       st->dt_next = s.dt_next;
       st->picard_iterations = s.picard_iterations;
+      st->picard_converged = s.picard_converged ? 1 : 0;
       st->max_dh = s.max_dh;
       st->mass_before = s.mass_before;
       st->mass_after = s.mass_after;
@@ -248,6 +252,16 @@ int orc_swe_set_bathymetry(void* h, const double* bed) {
 }
 int orc_swe_initialize_state(void* h, double eta0) {
   return guard([&] { static_cast<SweSolver*>(h)->initializeState(eta0); });
+}
+// SweSolver::setInitialEtaPhysical + setInitialVelocityConstant(0, 0) + finalizeInitialState
+// (src/swe/SweSolver.cpp:855-924): a non-uniform initial surface (nx*ny values, i + j*nx).
+int orc_swe_set_initial_eta(void* h, const double* eta_phys) {
+  return guard([&] {
+    auto* s = static_cast<SweSolver*>(h);
+    s->setInitialEtaPhysical(std::vector<real>(eta_phys, eta_phys + (size_t)s->g.nx * s->g.ny));
+    s->setInitialVelocityConstant(0.0, 0.0);
+    s->finalizeInitialState();
+  });
 }
 int orc_swe_get(void* h, const char* name, double* dst) {
   return guard([&] {

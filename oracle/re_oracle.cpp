@@ -513,6 +513,8 @@ void ReSolver::adaptiveTimeStep() {  // :880-929 (max/min are exact, so order-fr
   dtn = p_.dt;
 }
 
+static bool pmax_is_one(const ReParams& p) { return p.picard_max <= 1; }
+
 void ReSolver::advanceStep(ReStepStats* st) {  // :931-969
   dt_used = p_.dt;
   const size_t n = h.size();
@@ -526,6 +528,7 @@ void ReSolver::advanceStep(ReStepStats* st) {  // :931-969
   // n as in the predictor, solve from x0 = 0 for h^{m+1}; stop when max |h^{m+1} - h^m| <= tol.
   SolveStats ss;
   int picard = 0, iters = 0;
+  bool converged = pmax_is_one(p_);
   std::vector<real> hm;
   const int pmax = std::max(1, p_.picard_max);
   for (;;) {
@@ -538,7 +541,7 @@ void ReSolver::advanceStep(ReStepStats* st) {  // :931-969
     iters += ss.iterations;
     ++picard;
     enforceHeadBc();
-    if (picard >= pmax) break;
+    if (pmax == 1) break;
     real delta = 0.0;
     for (int k = 0; k < g.nz; ++k)
       for (int j = 0; j < g.ny; ++j)
@@ -546,7 +549,11 @@ void ReSolver::advanceStep(ReStepStats* st) {  // :931-969
           const long long c = g.idx(i, j, k);
           delta = std::max(delta, std::fabs(h[c] - hm[c]));
         }
-    if (delta <= p_.picard_tol) break;
+    if (delta <= p_.picard_tol) {
+      converged = true;
+      break;
+    }
+    if (picard >= pmax) break;  // the cap: the step continues from the last iterate
   }
   ss.iterations = iters;
   computeKFace();
@@ -592,6 +599,7 @@ void ReSolver::advanceStep(ReStepStats* st) {  // :931-969
   if (st) {
     st->solve = ss;
     st->picard_iterations = picard;
+    st->picard_converged = converged;
     st->dt_used = dt_used;
     st->dt_next = p_.dt;
     st->max_dh = dhmax;

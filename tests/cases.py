@@ -69,3 +69,12 @@ def gaussian_bump_bed(nx=20, ny=20, dx=1.0, dy=1.0):
     y = (np.arange(ny) + 0.5) * dy
     X, Y = np.meshgrid(x, y)
     return 0.5 * np.exp(-((X - 10.0) ** 2 + (Y - 10.0) ** 2) / 10.0)  # [j, i]
+
+
+def flow_2d_inputs(n=16, m=12):
+    """A rain-free 2D wet/dry problem with flow: an asymmetric bed (slope, ripples, a ridge) and a
+    mound of water in one corner of an n x m basin, so that water runs out over dry ground."""
+    I, J = np.meshgrid(np.arange(n), np.arange(m))  # [j, i]
+    bed = 0.02 * I + 0.035 * J + 0.05 * np.cos(0.5 * I) * np.cos(0.3 * J) + 0.08 * np.exp(-((I - 9.0) ** 2) / 3.0)
+    eta0 = np.maximum(bed, 0.35 - 0.3 * np.exp(-((I - 3.0) ** 2 + (J - 4.0) ** 2) / 6.0))
+    return bed, eta0

@@ -37,6 +37,7 @@ def lib():
         L.orc_re_get.argtypes = [C.c_void_p, C.c_char_p, dp]
         L.orc_re_set.argtypes = [C.c_void_p, C.c_char_p, dp]
         L.orc_re_step.argtypes = [C.c_void_p, P(abi.ReStats)]
+        L.orc_re_reallocate.argtypes = [C.c_void_p]
         L.orc_re_dt.restype = C.c_double
         L.orc_re_dt.argtypes = [C.c_void_p]
         L.orc_swe_create.restype = C.c_void_p
@@ -44,6 +45,7 @@ def lib():
         L.orc_swe_destroy.argtypes = [C.c_void_p]
         L.orc_swe_set_bathymetry.argtypes = [C.c_void_p, dp]
         L.orc_swe_initialize_state.argtypes = [C.c_void_p, C.c_double]
+        L.orc_swe_set_initial_eta.argtypes = [C.c_void_p, dp]
         L.orc_swe_get.argtypes = [C.c_void_p, C.c_char_p, dp]
         L.orc_swe_set.argtypes = [C.c_void_p, C.c_char_p, dp]
         L.orc_swe_step.argtypes = [C.c_void_p, C.c_double, C.c_double, P(abi.SolveStats)]
@@ -126,6 +128,10 @@ class OracleRe:
     def dt(self):
         return lib().orc_re_dt(self.h)
 
+    def reallocate(self):
+        """reallocateWaterContent + clampWaterContent + enforceMoistureBc on the current fields."""
+        check(lib().orc_re_reallocate(self.h))
+
 
 class OracleSwe:
     """Oracle SweSolver (src/swe/SweSolver.hpp method set)."""
@@ -148,6 +154,10 @@ class OracleSwe:
 
     def initialize_state(self, eta0):
         check(lib().orc_swe_initialize_state(self.h, eta0))
+
+    def set_initial_eta(self, eta_phys):
+        """setInitialEtaPhysical + setInitialVelocityConstant(0, 0) + finalizeInitialState."""
+        check(lib().orc_swe_set_initial_eta(self.h, ptr(np.ascontiguousarray(eta_phys, dtype=np.float64).ravel())))
 
     def get(self, name):
         out = np.empty(self.nsurf, dtype=np.float64)

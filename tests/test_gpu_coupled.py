@@ -11,11 +11,11 @@ from gpu_common import compare_re, compare_swe, say
 pytestmark = pytest.mark.gpu
 
 
-def make_pair(nx, ny, nz, dx, bed, eta0, h0, rtol_gw=1e-12):
+def make_pair(nx, ny, nz, dx, bed, eta0, h0, rtol_gw=1e-12, gw_dt=1.0, adaptive=0):
     sp = abi.swe_params(manning=0.03, dt=1.0)
     soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5, Ss=1e-5)
     rp = abi.re_params(soil=soil, solver=abi.solver_config(rtol_gw), bc_type=(0, 0, 0, 0, 0, 0), dx=dx, dy=dx, dz=0.05,
-                       dt=1.0, adaptive_dt=0, use_full3d=1)
+                       dt=gw_dt, adaptive_dt=adaptive, dt_min=0.05, dt_max=0.6, co_max=2.0, use_full3d=1)
     cp = abi.coupling_params(dx=dx, dy=dx)
     os_ = ol.OracleSwe(nx, ny, dx, dx, sp)
     os_.set_bathymetry(bed.ravel())
@@ -48,22 +48,36 @@ def test_exchange_matches_oracle_and_conserves():
     compare_re(ctx, orr, fields=("h", "wc"), rtol=1e-12, atol_h=1e-12)
 
 
-def test_coupled_run_matches_oracle():
+@pytest.mark.parametrize("gw", ["same_dt", "substeps", "adaptive"])
+def test_coupled_run_matches_oracle(gw):
     """A small version of the 512x512 coupled configuration: tilted bed with a random field,
-    uniform rain, dry start, 1 s steps."""
+    uniform rain, dry start, 1 s surface steps. The groundwater runs at the surface dt, at a
+    quarter of it (four substeps per coupled step: gwCatchUpTo, src/driver/Orchestrator.cpp:1243-1252),
+    or with its own adaptive dt (substep counts vary, the last substep may overshoot t + dt)."""
     nx, ny, nz = 24, 20, 8
-    say("coupled 24x20x8 vs oracle")
+    say(f"coupled 24x20x8 vs oracle, groundwater {gw}")
     rng = np.random.default_rng(5)
     bed = 0.01 * np.arange(nx)[None, :] + 0.02 * rng.standard_normal((ny, nx))
     h0 = np.full(nx * ny * nz, -0.3)
-    os_, orr, ctx, cp = make_pair(nx, ny, nz, 1.0, bed, -10.0, h0)
+    gw_dt, adaptive = {"same_dt": (1.0, 0), "substeps": (0.25, 0), "adaptive": (0.3, 1)}[gw]
+    os_, orr, ctx, cp = make_pair(nx, ny, nz, 1.0, bed, -10.0, h0, gw_dt=gw_dt, adaptive=adaptive)
     oc = ol.OracleCoupled(os_, orr, cp, 100.0)
     rain = 5e-4
+    subs = []
     for s in range(60):
+        before = int(oc.state[3])  # the oracle counts substeps cumulatively
         oc.step(1.0, rain)
         st = ctx.coupled_step(cp, 1.0, rain, 0.0, 100.0)
-        assert st.gw_substeps == 1
+        subs.append(st.gw_substeps)
+        assert st.gw_substeps == int(oc.state[3]) - before, s
+        assert abs(st.gw_time - oc.state[1]) <= 1e-9, (s, st.gw_time, oc.state[1])
     compare_swe(ctx, os_, fields=("eta", "dept", "uu", "vv"), rtol=1e-7, atol=1e-9, label="coupled")
     compare_re(ctx, orr, fields=("h", "wc"), rtol=1e-7, atol_h=1e-7, label="coupled")
     d = interior2(ctx.swe_get("dept"), nx, ny)
     assert (d > 0).any(), "rain must pond somewhere"
+    if gw == "same_dt":
+        assert set(subs) == {1}
+    elif gw == "substeps":
+        assert set(subs) == {4}
+    else:
+        assert len(set(subs)) > 1 and max(subs) > 1, subs

@@ -2,6 +2,7 @@
 src/coupling/Coupling.cpp:25-164) with the reference's conservation gate
 (tests/coupling/test_coupled_timestep.cpp:64-121) and the sign / donor-limit special cases."""
 import numpy as np
+import pytest
 
 import oracle_lib as ol
 from cases import interior2, interior3
@@ -51,3 +52,38 @@ def test_donor_limits():
     vol, q = ol.couple_exchange(s, r, cp, 1.0, want_q=True)
     np.testing.assert_allclose(q, -1e-4 * 100.0 / 1.0, rtol=1e-12)
     assert interior2(s.get("dept"), 10, 10).max() < 1e-15
+
+
+def test_exchange_rate_closed_form_and_apply():
+    """computeExchangeRates / limitExchange / applyExchangeTo* (src/coupling/Coupling.cpp:25-143)
+    written out for a case whose donor limits do not bind: q = Az Ks_z visc (h_top - depth) /
+    (dz/2) per column (class-0 soil), eta += q dt / Az, theta_top -= q dt / V and h_top = h(theta_top).
+    Heads and depths vary per column; infiltration and seepage columns are both present."""
+    nx, ny, nz, dx, dy, dz, dt, visc = 4, 3, 5, 2.0, 3.0, 0.1, 0.05, 0.7
+    soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_z=3e-5)
+    s = ol.OracleSwe(nx, ny, dx, dy, abi.swe_params(dt=dt))
+    s.set_bathymetry(np.zeros(nx * ny))
+    depth = 0.05 + 0.02 * np.arange(nx * ny) % 0.13
+    s.set_initial_eta(depth)
+    r = ol.OracleRe(nx, ny, nz, abi.re_params(soil=soil, dx=dx, dy=dy, dz=dz))
+    # infiltrating columns (unsaturated top cell) and seeping columns (top head above the depth)
+    htop = np.concatenate([np.linspace(-0.6, -0.15, 8), np.linspace(0.2, 0.4, nx * ny - 8)])
+    h0 = np.concatenate([htop, np.full(nx * ny * (nz - 1), -1.0)])
+    r.set_initial(h=h0)
+    wc_top = interior3(r.get("wc"), nx, ny, nz)[0].ravel().copy()
+    eta0 = interior2(s.get("eta"), nx, ny).ravel().copy()
+    cp = abi.coupling_params(dx=dx, dy=dy, visc=visc)
+    vol, q = ol.couple_exchange(s, r, cp, dt, want_q=True)
+    Az, V = dx * dy, dx * dy * dz
+    q_expect = Az * soil.Ks_z * visc * (htop - depth) / (0.5 * dz)
+    assert (q_expect < 0).any() and (q_expect > 0).any()
+    # the donor limits are far away for this dt (checked, so that the closed form is the rate)
+    assert np.all(q_expect > -depth * Az / dt) and np.all(q_expect > -(soil.theta_s - wc_top) * V / dt)
+    assert np.all(q_expect < (wc_top - soil.theta_r) * V / dt)
+    np.testing.assert_allclose(q, q_expect, rtol=1e-13, atol=0)
+    assert vol == pytest.approx(np.sum(q_expect * dt), rel=1e-13)
+    np.testing.assert_allclose(interior2(s.get("eta"), nx, ny).ravel(), eta0 + q_expect * dt / Az, rtol=0, atol=1e-15)
+    wc1 = wc_top - q_expect * dt / V
+    np.testing.assert_allclose(interior3(r.get("wc"), nx, ny, nz)[0].ravel(), wc1, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(interior3(r.get("h"), nx, ny, nz)[0].ravel(), ol.vg_eval(soil, wc=wc1)["head"],
+                               rtol=1e-13)

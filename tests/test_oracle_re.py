@@ -85,7 +85,153 @@ def test_b2gw_one_step_wets_top():  # test_re_b2_gw.cpp:109-153
     wc0, h0 = r.get("wc")[c], r.get("h")[c]
     r.step()
     assert r.get("h")[c] > h0 + 1e-6 and r.get("wc")[c] > wc0 + 1e-8
-    assert r.dt() == pytest.approx(1e-4 * 1.25) or r.dt() >= 1e-4  # adaptive ramp (ReSolver.cpp:921-927)
+    assert r.dt() == pytest.approx(1e-4 * 1.25, rel=1e-14)  # adaptive ramp (ReSolver.cpp:921-927)
+
+
+def test_b2gw_adaptive_dt_ramp_and_plateau():
+    """tests/re/test_re_b2_gw.cpp:338-397: the b2-gw column with adaptive dt run to t = 46800 s.
+    The legacy dt file the reference compares the first 12 steps with is absent; its content there
+    is the deterministic x1.25 ramp, so dt_k = 1e-4 * 1.25^k is asserted as a closed form. Every dt
+    stays in [dt_min, dt_max] and the run ends on the dt_max plateau."""
+    soil = abi.soil_params(alpha=1.43, n=1.56, theta_s=0.33, Ks_z=2.89e-6, Ss=1e-5)
+    p = abi.re_params(soil=soil, solver=abi.solver_config(1e-10), bc_type=(0, 0, 0, 0, 0, 1), dz=0.01,
+                      dt=1e-4, dt_min=1e-4, dt_max=2.0, co_max=2.0, adaptive_dt=1, htop=0.0)
+    r = ol.OracleRe(1, 1, 100, p)
+    r.set_initial(wc=np.full(100, 0.033))
+    t, dts = 0.0, []
+    while t < 46800.0 + 1e-9:
+        used = r.dt()
+        st = r.step()
+        assert st.dt_used == used
+        dts.append(used)
+        t += st.dt_used
+        assert 1e-4 - 1e-300 <= r.dt() <= 2.0 + 1e-12
+    ramp = 1e-4 * 1.25 ** np.arange(12)
+    np.testing.assert_allclose(dts[:12], ramp, rtol=1e-13)
+    assert r.dt() == 2.0
+    # below dt_max every step is x1.25 or a hold (0.01 <= dq_max <= 0.02 near the wetting front);
+    # the one other ratio is the clamp to dt_max
+    ratio = np.array(dts[1:]) / np.array(dts[:-1])
+    below = np.array(dts[1:]) < 2.0
+    assert np.all(np.isclose(ratio[below], 1.25, rtol=1e-12) | (ratio[below] == 1.0))
+    assert np.sum(ratio[below] == 1.0) > 0
+    k = int(np.argmax(np.array(dts) >= 2.0))
+    assert all(d == 2.0 for d in dts[k:])
+
+
+def _dt_statistics(r, p, nx, ny, nz, dt_prev):
+    """The adaptive-dt statistics of the state after a step, computed here from the oracle's face
+    fluxes and water contents: max over non-top cells of |net inflow per unit area| dt/dz, and
+    min of co_max dz / (dK/dtheta) over the non-top unsaturated cells (ReSolver.cpp:890-914)."""
+    sh = (nz + 2, ny + 2, nx + 2)
+    qx, qy, qz, wc = (r.get(f).reshape(sh) for f in ("qx", "qy", "qz", "wc"))
+    Ax, Ay, Az = p.dy * p.dz, p.dx * p.dz, p.dx * p.dy
+    k, j, i = slice(2, nz + 1), slice(1, ny + 1), slice(1, nx + 1)  # k = 1..nz-1 (not the top layer)
+    net = (qx[k, j, i] - qx[k, j, 0:nx]) / Ax + (qy[k, j, i] - qy[k, 0:ny, i]) / Ay + \
+        (qz[k, j, i] - qz[1:nz, j, i]) / Az
+    dq = float(np.abs(net).max() * dt_prev / p.dz)
+    w = wc[k, j, i]
+    w = w[w < p.soil.theta_s]
+    dtco = float((p.co_max * p.dz / ol.vg_eval(p.soil, wc=w.ravel())["dKdwc"]).min()) if w.size else 1e8
+    return dq, dtco
+
+
+@pytest.mark.parametrize("case", ["ramp_reduce_hold", "courant_dtmin"])
+def test_adaptive_dt_every_branch(case):
+    """adaptiveTimeStep (ReSolver.cpp:880-929) on an infiltration box: the next dt is 0.75 dt when
+    dq_max > 0.02, 1.25 dt when dq_max < 0.01, dt otherwise, then clamped to the Courant limit,
+    dt_max and dt_min. Each rule is checked against statistics computed here from the fields, and
+    the two cases make every branch fire (x0.75, hold, x1.25, Courant, dt_min)."""
+    soil = abi.soil_params(alpha=3.35, n=2.0, theta_s=0.368, theta_r=0.102, Ks_x=9.22e-5, Ks_y=9.22e-5,
+                           Ks_z=9.22e-5, Ss=1e-6)
+    nx, ny, nz = 3, 2, 10
+    kw = dict(dt=5.0, dt_min=1e-3, co_max=1e9) if case == "ramp_reduce_hold" else dict(dt=0.5, dt_min=0.4,
+                                                                                        co_max=1e-4)
+    p = abi.re_params(soil=soil, solver=abi.solver_config(1e-12), bc_type=(0, 0, 0, 0, 0, 1), htop=-0.1, dx=0.1,
+                      dy=0.1, dz=0.02, dt_max=100.0, adaptive_dt=1, use_full3d=1, **kw)
+    r = ol.OracleRe(nx, ny, nz, p)
+    r.set_initial(h=np.full(nx * ny * nz, -3.0))
+    fired = set()
+    for _ in range(8):
+        dt0 = r.dt()
+        r.step()
+        dq, dtco = _dt_statistics(r, p, nx, ny, nz, dt0)
+        rule = dt0 * (0.75 if dq > 0.02 else (1.25 if dq < 0.01 else 1.0))
+        fired.add("x0.75" if dq > 0.02 else ("x1.25" if dq < 0.01 else "hold"))
+        expect = rule
+        if expect > dtco:
+            expect = dtco
+            fired.add("courant")
+        expect = min(expect, p.dt_max)
+        if expect < p.dt_min:
+            expect = p.dt_min
+            fired.add("dt_min")
+        assert r.dt() == pytest.approx(expect, rel=1e-12), (dt0, dq, dtco)
+    want = {"x0.75", "hold", "x1.25"} if case == "ramp_reduce_hold" else {"x0.75", "courant", "dt_min"}
+    assert want <= fired, fired
+
+
+def _realloc_case(nx, nz, wc, h, kx_faces=(), bc5=0, htop=0.0, full3d=1):
+    """Oracle with a hand-set state (i + k*nx order, ny = 1) and nonzero Kx on the listed +x faces."""
+    soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5)
+    p = abi.re_params(soil=soil, bc_type=(0, 0, 0, 0, 0, bc5), htop=htop, use_full3d=full3d, dx=1.0, dy=1.0, dz=0.1)
+    r = ol.OracleRe(nx, 1, nz, p)
+    sh = (nz + 2, 3, nx + 2)
+    W = np.full(sh, 0.1)
+    H = np.full(sh, -5.0)
+    K = np.zeros(sh)
+    for (i, k) in [(i, k) for k in range(nz) for i in range(nx)]:
+        W[k + 1, 1, i + 1] = wc[k * nx + i]
+        H[k + 1, 1, i + 1] = h[k * nx + i]
+    for (i, k) in kx_faces:
+        K[k + 1, 1, i + 1] = 1e-6
+    r.set("wc", W.ravel())
+    r.set("h", H.ravel())
+    r.set("Kx", K.ravel())
+    r.set("Ky", np.zeros(W.size))
+    r.reallocate()
+    g = lambda f: r.get(f).reshape(sh)[1:nz + 1, 1, 1:nx + 1].ravel()  # noqa: E731
+    return soil, g("wc"), g("h")
+
+
+def test_reallocation_column_branches_and_order():
+    """reallocateWaterContent (ReSolver.cpp:788-837) on a hand-set 1x1x4 column, expected values
+    worked out by hand. Branches: theta >= theta_s -> theta_s; no saturated neighbour -> h = h(theta);
+    saturated neighbour -> theta = theta(h). The sweep runs top (k = 0) to bottom and reads the
+    cells above it after their update: with a ponded top Dirichlet face the top cell takes
+    theta(+0.01 m) = theta_s, and that saturates the neighbour test of the cell below it, which a
+    sweep over the old values would not see."""
+    ts = 0.40
+    wc = [0.36, 0.38, 0.20, 0.42]
+    h = [0.01, -1.0, -3.0, -0.3]
+    soil, w, hh = _realloc_case(1, 4, wc, h, bc5=1, htop=0.0)
+    th = lambda x: float(ol.vg_eval(soil, h=np.array([x]))["theta"][0])  # noqa: E731
+    hd = lambda x: float(ol.vg_eval(soil, wc=np.array([x]))["head"][0])  # noqa: E731
+    assert th(0.01) == ts
+    np.testing.assert_array_equal(w, [ts, th(-1.0), th(-3.0), ts])
+    np.testing.assert_array_equal(hh, h)
+    # no ponded top: the top cell has no saturated neighbour, so its head follows its water
+    # content, and so does the second cell's (its neighbours stay unsaturated)
+    _, w, hh = _realloc_case(1, 4, wc, h, bc5=0)
+    np.testing.assert_array_equal(w, [0.36, 0.38, th(-3.0), ts])
+    np.testing.assert_array_equal(hh, [hd(0.36), hd(0.38), -3.0, -0.3])
+
+
+def test_reallocation_lateral_neighbours():
+    """2x1x2 box (i + k*nx): the lateral neighbour counts only across a face with Kx != 0, and the
+    +x cell sees the -x cell after its update. Cell (0,1) sits below the saturated (0,0) and takes
+    theta(+0.01 m) = theta_s; cell (1,1) is then next to a saturated cell across the open face."""
+    wc = [0.41, 0.30, 0.30, 0.30]
+    h = [-0.1, -0.5, 0.01, -0.9]
+    soil, w, hh = _realloc_case(2, 2, wc, h, kx_faces=[(0, 1)])
+    th = lambda x: float(ol.vg_eval(soil, h=np.array([x]))["theta"][0])  # noqa: E731
+    hd = lambda x: float(ol.vg_eval(soil, wc=np.array([x]))["head"][0])  # noqa: E731
+    np.testing.assert_array_equal(w, [0.40, 0.30, 0.40, th(-0.9)])
+    np.testing.assert_array_equal(hh, [-0.1, hd(0.30), 0.01, -0.9])
+    # the same face closed: (1,1) has no saturated neighbour
+    _, w, hh = _realloc_case(2, 2, wc, h, kx_faces=[])
+    np.testing.assert_array_equal(w, [0.40, 0.30, 0.40, 0.30])
+    np.testing.assert_array_equal(hh, [-0.1, hd(0.30), 0.01, hd(0.30)])
 
 
 def test_column_predictor_matches_dense_solve():

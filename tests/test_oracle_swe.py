@@ -5,7 +5,7 @@ against the reference's golden vector benchmarks/b1-sw/frehg2_np1_depth_referenc
 import numpy as np
 
 import oracle_lib as ol
-from cases import b1_inputs, gaussian_bump_bed, interior2, interp_series
+from cases import b1_inputs, flow_2d_inputs, gaussian_bump_bed, interior2, interp_series
 from frehg_amd import abi
 
 
@@ -38,9 +38,43 @@ def test_b0_lake_at_rest():
     assert np.abs(interior2(s.get("vv"), 20, 20)).max() < 1e-8
 
 
-def test_transpose_symmetry():
-    """Transposing a rain-free problem swaps x and y: depth fields must transpose (to round-off)."""
-    # square basin whose bed is symmetric under i <-> j, initially still water above it
+def _flow_case(bed, eta0, dx, dy, visc_x, visc_y, steps=60):
+    ny, nx = bed.shape
+    p = abi.swe_params(manning=0.03, dt=0.2, visc_x=visc_x, visc_y=visc_y, solver=abi.solver_config(1e-13))
+    s = ol.OracleSwe(nx, ny, dx, dy, p)
+    s.set_bathymetry(bed.ravel())
+    s.set_initial_eta(eta0.ravel())
+    umax = 0.0
+    for _ in range(steps):
+        s.step(0.0, 0.0)
+        umax = max(umax, np.abs(interior2(s.get("uu"), nx, ny)).max(), np.abs(interior2(s.get("vv"), nx, ny)).max())
+    return s, umax
+
+
+def test_transpose_symmetry_with_flow():
+    """The scheme is the same in x and y (src/swe/SweSolver.cpp:159-792): a rain-free problem
+    solved on the transposed grid, with dx <-> dy and the x/y viscosities swapped, gives the
+    transposed depth and swaps u and v, to round-off. The y-direction terms are pinned by the
+    b1-sw golden vector (nx = 1); this carries the pin over to every x-direction term in geometry,
+    momentum, matrix, velocity and limiter, in a wet/dry run with velocities near 1 m/s. A wrong
+    index, area or spacing in an x-direction term breaks the symmetry."""
+    bed, eta0 = flow_2d_inputs()
+    m, n = bed.shape
+    a, umax = _flow_case(bed, eta0, 1.0, 1.5, 1e-2, 3e-2)
+    b, _ = _flow_case(bed.T.copy(), eta0.T.copy(), 1.5, 1.0, 3e-2, 1e-2)
+    assert umax > 0.3
+    da, db = interior2(a.get("dept"), n, m), interior2(b.get("dept"), m, n)
+    assert (da <= 0).sum() > 10 and (da > 0).sum() > 10  # wet and dry cells
+    np.testing.assert_allclose(da, db.T, rtol=0, atol=1e-12)
+    for fa, fb in (("uu", "vv"), ("vv", "uu"), ("CDx", "CDy"), ("Fu", "Fv"), ("cflx", "cfly")):
+        # relative: the drag coefficient of a film a few mm deep magnifies depth round-off
+        np.testing.assert_allclose(interior2(a.get(fa), n, m), interior2(b.get(fb), m, n).T, rtol=1e-8,
+                                   atol=1e-11, err_msg=fa)
+    np.testing.assert_allclose(interior2(a.get("eta"), n, m), interior2(b.get("eta"), m, n).T, atol=1e-12)
+
+
+def test_transpose_symmetry_still_water():
+    """Still water over an i <-> j symmetric bed stays symmetric (and at rest)."""
     n = 8
     bs = 0.1 * np.add.outer(np.arange(n), np.arange(n)) / n
     p = abi.swe_params(manning=0.03, dt=0.5)
