@@ -20,7 +20,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", os.path.join(HERE, "..", "include")]
 
-SOURCES = ["krylov.hip", "re.hip", "swe.hip", "coupling.hip", "mlp.hip", "ensemble.hip", "api.hip"]
+SOURCES = ["krylov.hip", "re.hip", "swe.hip", "coupling.hip", "mlp.hip", "ensemble.hip", "noalloc.hip", "api.hip"]
 
 # Per-file flags. The MLP kernels keep their MFMA accumulators in ordinary VGPRs (the guess kernel
 # otherwise reads every result back from accumulation registers: 36 extra moves per 16-cell tile).

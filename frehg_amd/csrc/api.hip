@@ -12,6 +12,7 @@
 #include "coupling.hpp"
 #include "ensemble.hpp"
 #include "mlp.hpp"
+#include "noalloc.hpp"
 #include "re.hpp"
 #include "swe.hpp"
 
@@ -116,6 +117,14 @@ int frehg_create(frehg_ctx_t* out, int device) {
     return FREHG_ERR_HIP;
   }
   c->prof.stream = c->stream;
+  // the allocation-free solve's scratch address on this device (once per device)
+  try {
+    frehg::noalloc_setup(device);
+  } catch (const std::exception& e) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FREHG_ERR_HIP;
+  }
   *out = c;
   return FREHG_OK;
 }
@@ -480,6 +489,16 @@ int frehg_ls_solve(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, const doub
       st->rhs_norm = sr.rhs_norm;
       st->converged = sr.converged ? 1 : 0;
     }
+    return FREHG_OK;
+  });
+}
+
+int frehg_ls_solve_noalloc(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, const double* diag,
+                           const double* const* offdiag, const double* b_owned, double* x_owned,
+                           const frehg_solver_config* cfg, frehg_solve_stats* st) {
+  return guard(ctx, [&] {
+    if (!cfg) throw frehg::ArgError("frehg_ls_solve_noalloc: null argument");
+    frehg::ls_solve_noalloc(ctx->device, ctx->stream, ndim, nx, ny, nz, diag, offdiag, b_owned, x_owned, *cfg, st);
     return FREHG_OK;
   });
 }

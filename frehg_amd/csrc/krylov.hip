@@ -1043,6 +1043,65 @@ Krylov::Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof)
   }
 }
 
+Krylov::Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof, const KrylovWorkspace& ws)
+    : g_(g), three_d_(three_d), s_(stream), prof_(prof), borrowed_(true) {
+  setup_shapes();
+  r_ = ws.r;
+  z_ = ws.z;
+  p0_ = ws.p0;
+  p1_ = ws.p1;
+  q_ = ws.q;
+  part_ = ws.part;
+  part_u_ = ws.part_u;
+  ctl_ = ws.ctl;
+  host_ctl_ = &host_ctl_own_;
+  // no autotuning (it would time launches with events): two columns per lane, whole columns per
+  // block in 3D, unless FREHG_SPMV_SHAPE pins a shape
+  if (!tuned_[0]) {
+    for (int prec = 0; prec < 2; ++prec) {
+      set_spmv_shape(prec, three_d_ ? 2 : 1, three_d_ ? g_.nz : 1);
+      tuned_[prec] = true;
+    }
+  }
+}
+
+namespace {
+inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+}  // namespace
+
+size_t Krylov::workspace_bytes(const Geo& g0, bool three_d) {
+  Geo g = g0;
+  if (!three_d) g.nz = 1;
+  const long long n = three_d ? g.ncell() : g.nsurf();
+  const size_t cap = ceil_div(g.nx, TX) * ceil_div(g.ny, TY) * (size_t)(three_d ? g.nz : 1);
+  return 5 * align256((size_t)n * sizeof(double)) + align256(5 * cap * sizeof(double)) +
+         align256(2 * cap * sizeof(double)) + align256(3 * sizeof(KrylovCtl));
+}
+
+KrylovWorkspace Krylov::carve_workspace(void* base, const Geo& g0, bool three_d) {
+  Geo g = g0;
+  if (!three_d) g.nz = 1;
+  const long long n = three_d ? g.ncell() : g.nsurf();
+  const size_t cap = ceil_div(g.nx, TX) * ceil_div(g.ny, TY) * (size_t)(three_d ? g.nz : 1);
+  char* p = static_cast<char*>(base);
+  auto take = [&](size_t bytes) {
+    char* q = p;
+    p += align256(bytes);
+    return q;
+  };
+  KrylovWorkspace w;
+  w.r = reinterpret_cast<double*>(take((size_t)n * sizeof(double)));
+  w.z = reinterpret_cast<double*>(take((size_t)n * sizeof(double)));
+  w.p0 = reinterpret_cast<double*>(take((size_t)n * sizeof(double)));
+  w.p1 = reinterpret_cast<double*>(take((size_t)n * sizeof(double)));
+  w.q = reinterpret_cast<double*>(take((size_t)n * sizeof(double)));
+  w.part = reinterpret_cast<double*>(take(5 * cap * sizeof(double)));
+  w.part_u = reinterpret_cast<double*>(take(2 * cap * sizeof(double)));
+  w.ctl = reinterpret_cast<KrylovCtl*>(take(3 * sizeof(KrylovCtl)));
+  w.n = n;
+  return w;
+}
+
 void Krylov::setup_shapes() {
   if (!three_d_) g_.nz = 1;
   n_ = three_d_ ? g_.ncell() : g_.nsurf();
@@ -1090,6 +1149,7 @@ void Krylov::build() {
 Krylov::~Krylov() { release(); }
 
 void Krylov::release() {
+  if (borrowed_) return;  // the workspace belongs to the caller
   for (void* p : {(void*)r_, (void*)z_, (void*)p0_, (void*)p1_, (void*)q_, (void*)xw_, (void*)b32_,
                   (void*)r32_, (void*)z32_, (void*)p032_, (void*)p132_, (void*)q32_, (void*)d32_,
                   (void*)rh_, (void*)v0_, (void*)v1_, (void*)sv_, (void*)tv_, (void*)rh32_, (void*)v032_,
@@ -1132,6 +1192,7 @@ template <typename T>
 void Krylov::autotune(const StencilView<T>& A) {
   const int prec = sizeof(T) == 4 ? 1 : 0;
   tuned_[prec] = true;
+  if (borrowed_) return;
   if (!three_d_) return;
   if (prec == 1) ensure_mixed();
   KrylovCtl* tctl = ctl_ + 2;
@@ -1173,6 +1234,7 @@ void Krylov::autotune(const StencilView<T>& A) {
 
 void Krylov::ensure_mixed() {
   if (b32_) return;
+  if (borrowed_) throw ArgError("Krylov: a solve on a borrowed workspace is FP64 CG only");
   const size_t bytes = (size_t)n_ * sizeof(float);
   for (float** p : {&b32_, &r32_, &z32_, &p032_, &p132_, &q32_, &d32_}) {
     FREHG_HIP(hipMalloc(p, bytes));
@@ -1181,6 +1243,7 @@ void Krylov::ensure_mixed() {
 }
 
 void Krylov::ensure_bicgstab(bool fp32) {
+  if (borrowed_) throw ArgError("Krylov: a solve on a borrowed workspace is FP64 CG only");
   if (fp32) {
     if (rh32_) return;
     const size_t bytes = (size_t)n_ * sizeof(float);

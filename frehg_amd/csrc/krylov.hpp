@@ -46,6 +46,15 @@ struct KrylovConfig {
   int max_refine = 8;
 };
 
+// Work vectors and control blocks of an FP64 CG solve carved from caller-owned device memory
+// (Krylov::carve_workspace); n = padded vector length.
+struct KrylovWorkspace {
+  double *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;
+  double *part = nullptr, *part_u = nullptr;
+  KrylovCtl* ctl = nullptr;
+  long long n = 0;
+};
+
 struct SolveResult {
   int iterations = 0;
   int refinements = 0;
@@ -78,7 +87,15 @@ inline KrylovConfig krylov_config(const frehg_solver_config& c, const char* who)
 class Krylov {
  public:
   Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof);
+  // A solver that allocates nothing: FP64 CG only, on the caller's workspace (which it does not
+  // free), with a fixed SpMV shape instead of the autotuner. Its work vectors must hold finite
+  // values (the SpMV meets their ghost entries with zero coefficients).
+  Krylov(const Geo& g, bool three_d, hipStream_t stream, Profiler* prof, const KrylovWorkspace& ws);
   ~Krylov();
+  // Device bytes of a borrowed workspace for this geometry, and its layout at `base` (256-byte
+  // aligned pieces).
+  static size_t workspace_bytes(const Geo& g, bool three_d);
+  static KrylovWorkspace carve_workspace(void* base, const Geo& g, bool three_d);
   Krylov(const Krylov&) = delete;
   Krylov& operator=(const Krylov&) = delete;
 
@@ -180,6 +197,8 @@ class Krylov {
   double* part_ = nullptr;
   KrylovCtl* ctl_ = nullptr;  // [0] main / outer, [1] inner
   KrylovCtl* host_ctl_ = nullptr;
+  bool borrowed_ = false;  // workspace owned by the caller (no allocation, no autotuning)
+  KrylovCtl host_ctl_own_{};  // host mirror of the control block in borrowed mode
   int last_iters_ = 8;
   int last_inner_iters_ = 8;
   int last_bcg_iters_ = 8;

@@ -88,6 +88,9 @@ def load():
         "frehg_mlp_predict": (C.c_int, [ctx, P(C.c_float), C.c_int, P(C.c_float)]),
         "frehg_mlp_increment": (C.c_int, [ctx, dp]),
         "frehg_mlp_gate": (C.c_int, [ctx, C.c_double]),
+        "frehg_ls_solve_noalloc": (C.c_int, [ctx, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                             P(C.c_void_p), C.c_void_p, C.c_void_p, P(abi.SolverConfig),
+                                             P(abi.SolveStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -253,6 +256,16 @@ class Context:
                                            arr, _ptr(np.ascontiguousarray(b, dtype=np.float64)), _ptr(x),
                                            C.byref(cfg), C.byref(st)))
         return x, st
+
+    def ls_solve_noalloc(self, ndim, nx, ny, nz, diag_dev, offdiag_dev, b_dev, x_dev, cfg):
+        """frehg_ls_solve_noalloc on device arrays given by address (owned ordering); x_dev
+        receives the solution."""
+        arr = (C.c_void_p * len(offdiag_dev))(*[C.c_void_p(int(p)) for p in offdiag_dev])
+        st = abi.SolveStats()
+        self.check(self.lib.frehg_ls_solve_noalloc(self.h, ndim, nx, ny, nz, C.c_void_p(int(diag_dev)), arr,
+                                                   C.c_void_p(int(b_dev)), C.c_void_p(int(x_dev)), C.byref(cfg),
+                                                   C.byref(st)))
+        return st
 
     # ---- measurement -----------------------------------------------------------------------------------
     def profile_enable(self, on=True):

@@ -219,6 +219,20 @@ int frehg_ls_solve(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, const doub
                    const double* const* offdiag, const double* b_owned, double* x_owned,
                    const frehg_solver_config* cfg, frehg_solve_stats* stats);
 
+/* Allocation-free variant of frehg_ls_solve. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory at solve time (tight serving loops, harnesses that forbid
+ * hipMalloc). diag, offdiag[0..5] (2D: [0..3]), b_owned and x_owned are DEVICE arrays in the same
+ * owned ordering and meaning as frehg_ls_solve (offdiag itself is a host array of device
+ * pointers); x_owned receives the solution. FP64 CG (cfg->ksp_type FREHG_KSP_CG, precision
+ * FREHG_PREC_FP64) on a symmetric operator, x0 = 0. The call allocates nothing: the padded
+ * operator and the CG vectors live in a 64 MiB device array per GPU, held by the library until
+ * the process ends, so an operator of more than about 760k padded cells is rejected with
+ * FREHG_ERR_INVALID_ARG. Calls that overlap on one GPU (other threads or processes using the same
+ * GPU at the same time) race on that array; this process's threads are serialised. */
+int frehg_ls_solve_noalloc(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, const double* diag,
+                           const double* const* offdiag, const double* b_owned, double* x_owned,
+                           const frehg_solver_config* cfg, frehg_solve_stats* stats);
+
 /* ---- Batched ensemble of independent Richards boxes (configs of uneven box sizes or parameter
  * ensembles). The reference advances such an ensemble as independent ReSolver instances
  * (src/re/ReSolver.cpp:931-969); here every kernel of a step covers all boxes in one launch, each
