@@ -57,13 +57,16 @@ LIVE_TYPES = ("spmv64", "spmv32", "guess")
 BREAKDOWN_STEPS = 10
 
 
-def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None, picard_max=1):
+def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None, picard_max=1, gate=0.0):
     """Warmup + timed coupled steps; returns timing, solver counters and kernel timings."""
     ctx = gpu.Context(0)
     if path == "plain":
         case.setup_gpu(ctx, abi.PREC_FP64, abi.GUESS_ZERO, picard_max=picard_max)
+    elif path == "plain_previous":
+        case.setup_gpu(ctx, abi.PREC_FP64, abi.GUESS_PREVIOUS, picard_max=picard_max)
     elif path == "learned":
         ctx.mlp_set_params(*model)
+        ctx.mlp_gate(gate)
         case.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_LEARNED, picard_max=picard_max)
     elif path == "extrapolated":
         case.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_EXTRAPOLATED, picard_max=picard_max)
@@ -101,12 +104,14 @@ def run_gpu(case, abi, gpu, path, warmup, steps, grab_after=None, model=None, pi
     ctx.synchronize()
     kt = {k: ctx.profile_read(v) for k, v in kt_ids.items()}
     ctx.profile_enable(False)
-    shape = ctx.spmv_shape(abi.PREC_FP64 if path == "plain" else abi.PREC_MIXED)
+    inferred = ctx.mlp_inferred_cells() if path == "learned" else None
+    shape = ctx.spmv_shape(abi.PREC_FP64 if path.startswith("plain") else abi.PREC_MIXED)
     h = ctx.re_get("h")
+    depth = ctx.swe_get("dept")
     sw_vol, gw_vol = ctx.water_volume()
     ctx.close()
     return {"elapsed": elapsed, "iters": iters, "refines": refines, "fallbacks": fallbacks, "picards": picards,
-            "kt": kt, "live": live, "h": h,
+            "kt": kt, "live": live, "h": h, "depth": depth, "inferred": inferred,
             "grabbed": grabbed, "volume": sw_vol + gw_vol, "spmv_shape": shape, "mass_err": mass_err}
 
 
@@ -136,6 +141,17 @@ def train_model(abi, gpu, workloads, args, seeds, log_steps, stride, train_steps
                    "cell_stride": stride, "adam_steps": train_steps, "batch": batch, "lr": 1e-3,
                    "loss_first": round(first, 5), "loss_last": round(last, 5),
                    "log_seconds": round(t_log, 2), "train_seconds": round(t_train, 2)}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(case, abi, steps, picard_max=1):
@@ -214,6 +230,110 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_infiltration(args):
+    """The 40x40x50 infiltration benchmark in full: 360 Picard steps of 10 s (free-drainage bottom),
+    the plain FP64 GPU path against the FP64 oracle over the same 360 steps on this host's cores."""
+    from frehg_amd import abi, gpu, workloads
+    case = workloads.InfiltrationCase()
+    ctx = gpu.Context(0)
+    ctx.re_init(case.nx, case.ny, case.nz, case.re_params())
+    ctx.re_set_initial(h=case.h0)
+    ctx.synchronize()
+    pic = its = 0
+    worst = 0.0
+    t0 = time.perf_counter()
+    for _ in range(case.steps):
+        st = ctx.re_step()
+        pic += st.picard_iterations
+        its += st.solve.iterations
+        worst = max(worst, abs(st.mass_residual) / st.mass_before)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    hg = ctx.re_get("h")
+    ctx.close()
+    log(f"[bench] GPU: {el:.3f} s for {case.steps} steps")
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    r = ol.OracleRe(case.nx, case.ny, case.nz, case.re_params())
+    r.set_initial(h=case.h0)
+    t0 = time.perf_counter()
+    for _ in range(case.steps):
+        r.step()
+    cel = time.perf_counter() - t0
+    log(f"[bench] CPU oracle: {cel:.1f} s for {case.steps} steps")
+    err = float(np.max(np.abs(interior3(hg, case.nx, case.ny, case.nz) - interior3(r.get("h"), case.nx, case.ny,
+                                                                                       case.nz))))
+    nthreads = ol.lib().orc_num_threads()
+    out = {"metric": "Richards cell-steps/s, 40x40x50 infiltration benchmark (360 Picard steps of 10 s)",
+           "value": round(case.cells * case.steps / el, 1), "unit": "cell-steps/s", "n_gpus": 1,
+           "steps": case.steps, "warmup": 0, "ms_per_step": round(el / case.steps * 1e3, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (the configured homogeneous column)",
+           "config": {"workload": "GW 40x40x50, dx=dy=0.1 m, dz=0.02 m, top Dirichlet -0.75 m, free-drainage bottom, "
+                                  "dt 10 s x 360, Picard to 1e-8 m (at most 50 iterates), FP64 CG rtol 1e-10"},
+           "picard_iters_per_step": pic / case.steps, "krylov_iters_per_step": its / case.steps,
+           "max_rel_mass_balance_error": worst, "max_head_err_vs_fp64_oracle_m": err,
+           "cpu_baseline": {"value": round(case.cells * case.steps / cel, 1), "unit": "cell-steps/s",
+                            "cores": nthreads, "kind": "port", "cpu_model": cpu_model(),
+                            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+                            "sample": f"all {case.steps} steps ({cel:.1f} s), FP64 oracle, Jacobi-PCG, same Picard "
+                                      "settings"}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_large(args):
+    """The large coupled configuration, flow only: 1024x1024 surface at dx = 2 m over 1024x1024x64 soil at
+    dz = 0.04 m (bed seed 11, Ks seed 12), Picard to 1e-8 m, dt 1 s: plain FP64, extrapolated + mixed
+    and learned + mixed (network trained on two other seeded realisations of this size); FP64 water
+    ledger every step; the plain path after one step against the FP64 oracle."""
+    from frehg_amd import abi, gpu, workloads
+    log("[bench] generating the 1024x1024x64 inputs")
+    case = workloads.CoupledCase.large()
+    steps, warmup = args.steps, args.warmup
+    a = argparse.Namespace(nx=case.nx, ny=case.ny, nz=case.nz)
+
+    class Large:  # train_model builds its realisations through workloads.CoupledCase(nx, ny, nz, ...)
+        CoupledCase = staticmethod(lambda nx, ny, nz, seed_bed, seed_ks: workloads.CoupledCase.large(
+            seed_bed=seed_bed, seed_ks=seed_ks))
+    log("[bench] plain FP64 path")
+    plain = run_gpu(case, abi, gpu, "plain", warmup, steps, grab_after=1, picard_max=args.picard_max)
+    log(f"[bench]   {plain['elapsed']:.3f} s")
+    ext = run_gpu(case, abi, gpu, "extrapolated", warmup, steps, picard_max=args.picard_max)
+    log(f"[bench]   extrapolated {ext['elapsed']:.3f} s")
+    model, training = train_model(abi, gpu, Large, a, seeds=((21, 22), (23, 24)), log_steps=6, stride=32,
+                                  train_steps=args.train_steps, batch=65536, picard_max=args.picard_max)
+    fast = run_gpu(case, abi, gpu, "learned", warmup, steps, model=model, picard_max=args.picard_max, gate=args.gate)
+    log(f"[bench]   learned {fast['elapsed']:.3f} s")
+    parity = None
+    if not args.no_cpu:
+        log("[bench] FP64 oracle, one coupled step of the same workload")
+        cb = cpu_baseline(case, abi, 1, picard_max=args.picard_max)
+        parity = float(np.max(np.abs(interior3(cb["h"], case.nx, case.ny, case.nz) -
+                                     interior3(plain["grabbed"], case.nx, case.ny, case.nz))))
+        log(f"[bench]   {cb['elapsed']:.1f} s; GPU FP64 vs oracle max |dh| = {parity:.3e} m")
+    hp = interior3(plain["h"], case.nx, case.ny, case.nz)
+
+    def row(r):
+        return {"value": round(case.cells * steps / r["elapsed"], 1),
+                "ms_per_step": round(r["elapsed"] / steps * 1e3, 3),
+                "picard_iters_per_step": r["picards"] / steps, "iters_per_step": r["iters"] / steps,
+                "max_rel_mass_balance_error": r["mass_err"],
+                "max_head_err_vs_plain_fp64_m": float(np.max(np.abs(hp - interior3(r["h"], case.nx, case.ny,
+                                                                                   case.nz))))}
+    out = {"metric": "coupled cell-steps/s, 1024x1024 + 1024x1024x64 (flow only)",
+           "value": round(case.cells * steps / fast["elapsed"], 1), "unit": "cell-steps/s", "n_gpus": 1,
+           "steps": steps, "warmup": warmup, "ms_per_step": round(fast["elapsed"] / steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32+f16",
+           "data": "synthetic (seeded Gaussian random fields: bed sigma 0.05 m / 20 m, ln Ks sigma 1 / 10 m)",
+           "config": {"workload": "coupled SW 1024x1024 (dx 2 m) + GW 1024x1024x64 (dz 0.04 m), dt 1 s, 50 mm/h rain, "
+                                  "closed sides, Picard to 1e-8 m; no solute tracer", "cells_per_step": case.cells},
+           "learned_mixed": row(fast), "extrapolated_mixed": row(ext), "plain_fp64": row(plain),
+           "speedup_vs_plain_fp64": round(plain["elapsed"] / fast["elapsed"], 3),
+           "gpu_fp64_max_head_err_vs_oracle_after_1_step_m": parity, "training": training,
+           "spmv_shape_cells_per_lane_kchunk": plain["spmv_shape"]}
+    print(json.dumps(out), flush=True)
+
+
 def bench_learned_eval(args):
     """Held-out evaluation of the learned guess on the coupled 512x512 + 512x512x30 workload: train on
     pairs logged from plain FP64 runs of 20 seeded bed/Ks realisations (seed r for the bed, r + 100
@@ -247,6 +367,7 @@ def bench_learned_eval(args):
             ctx = gpu.Context(0)
             if guess == abi.GUESS_LEARNED:
                 ctx.mlp_set_params(*model)
+                ctx.mlp_gate(args.gate)
             case.setup_gpu(ctx, abi.PREC_FP64, guess, picard_max=picard_max)
             its = pic = fb = 0
             for _ in range(args.steps):
@@ -292,17 +413,26 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-steps", type=int, default=1500)
-    ap.add_argument("--log-steps", type=int, default=20)
+    ap.add_argument("--log-steps", type=int, default=11,
+                    help="coupled steps logged per training realisation (the first logs no pairs)")
+    ap.add_argument("--train-realisations", type=int, default=20,
+                    help="seeded bed/Ks realisations the learned guess is trained on (bed seed r, Ks seed r + 100, "
+                         "r = 101, 102, ...; about 2e7 logged pairs in all)")
     ap.add_argument("--picard-max", type=int, default=50,
                     help="Picard iterations per Richards step at most, to max |dh| <= 1e-8 m (1: the "
                          "reference's predictor step, which is also measured and reported as predictor_step)")
+    ap.add_argument("--gate", type=float, default=1e-6,
+                    help="learned-guess inference gate [m]: cells whose previous solve increment is at most "
+                         "this take the extrapolated start without the network (0: the network everywhere)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run an elementwise 1 GiB -> 1 GiB kernel (torch), the reference point that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
-    ap.add_argument("--workload", choices=("coupled", "ensemble", "learned-eval"), default="coupled",
+    ap.add_argument("--workload", choices=("coupled", "ensemble", "learned-eval", "large", "infiltration"),
+                    default="coupled",
                     help="coupled: the 512x512 + 512x512x30 coupled step (default); ensemble: 96 uneven boxes; "
                          "learned-eval: the learned guess trained on 20 realisations and evaluated on 5 "
-                         "held-out ones (iterations per step against the zero guess)")
+                         "held-out ones (iterations per step against the zero guess); large: 1024x1024 + "
+                         "1024x1024x64 flow only; infiltration: the 40x40x50 column, 360 steps, GPU and CPU")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); the multi-GPU "
@@ -313,6 +443,12 @@ def main():
         return
     if args.workload == "learned-eval":
         bench_learned_eval(args)
+        return
+    if args.workload == "large":
+        bench_large(args)
+        return
+    if args.workload == "infiltration":
+        bench_infiltration(args)
         return
 
     if args.copy_probe:
@@ -337,13 +473,15 @@ def main():
     plain = run_gpu(case, abi, gpu, "plain", args.warmup, args.steps, grab_after=cpu_steps or None,
                     picard_max=args.picard_max)
     log(f"[bench]   {plain['elapsed']:.3f} s, {plain['iters']} Krylov iterations")
-    log("[bench] training the learned guess on two other seeded realisations (not timed)")
-    model, training = train_model(abi, gpu, workloads, args, seeds=((101, 102), (103, 104)),
-                                  log_steps=args.log_steps, stride=4, train_steps=args.train_steps, batch=65536,
-                                  picard_max=args.picard_max)
+    seeds = [(r, r + 100) for r in range(101, 101 + args.train_realisations)]
+    stride = max(1, (len(seeds) * (args.log_steps - 1) * n3) // 20_000_000)
+    log(f"[bench] training the learned guess on {len(seeds)} other seeded realisations (not timed)")
+    model, training = train_model(abi, gpu, workloads, args, seeds=seeds, log_steps=args.log_steps, stride=stride,
+                                  train_steps=args.train_steps, batch=65536, picard_max=args.picard_max)
     log(f"[bench]   {training['pairs']} pairs, loss {training['loss_first']} -> {training['loss_last']}")
-    log("[bench] learned path (FP32-in-FP64 refinement, MLP guess)")
-    fast = run_gpu(case, abi, gpu, "learned", args.warmup, args.steps, model=model, picard_max=args.picard_max)
+    log(f"[bench] learned path (FP32-in-FP64 refinement, MLP guess, gate {args.gate:g} m)")
+    fast = run_gpu(case, abi, gpu, "learned", args.warmup, args.steps, model=model, picard_max=args.picard_max,
+                   gate=args.gate)
     log(f"[bench]   {fast['elapsed']:.3f} s, {fast['iters']} inner iterations, {fast['refines']} refinements, "
         f"{fast['fallbacks']} fallbacks")
     log("[bench] previous-step path (FP32-in-FP64 refinement, x0 = h^n)")
@@ -352,11 +490,14 @@ def main():
     log("[bench] extrapolated path (FP32-in-FP64 refinement, x0 = h^n + the previous step's solve increment)")
     ext = run_gpu(case, abi, gpu, "extrapolated", args.warmup, args.steps, picard_max=args.picard_max)
     log(f"[bench]   {ext['elapsed']:.3f} s, {ext['iters']} inner iterations, {ext['refines']} refinements")
+    log("[bench] plain FP64 path with warm starts (every Krylov solve from x0 = h^n / h^m)")
+    plain_prev = run_gpu(case, abi, gpu, "plain_previous", args.warmup, args.steps, picard_max=args.picard_max)
+    log(f"[bench]   {plain_prev['elapsed']:.3f} s, {plain_prev['iters']} Krylov iterations")
     predictor = None
     if args.picard_max > 1:
         log("[bench] the reference's predictor step (one Picard iterate): plain, learned, previous, extrapolated")
         p1 = run_gpu(case, abi, gpu, "plain", args.warmup, args.steps, picard_max=1)
-        f1 = run_gpu(case, abi, gpu, "learned", args.warmup, args.steps, model=model, picard_max=1)
+        f1 = run_gpu(case, abi, gpu, "learned", args.warmup, args.steps, model=model, picard_max=1, gate=args.gate)
         v1 = run_gpu(case, abi, gpu, "previous", args.warmup, args.steps, picard_max=1)
         e1 = run_gpu(case, abi, gpu, "extrapolated", args.warmup, args.steps, picard_max=1)
         h1 = interior3(p1["h"], case.nx, case.ny, case.nz)
@@ -380,6 +521,12 @@ def main():
     head_err = float(np.max(np.abs(hp - hf)))
     head_err_prev = float(np.max(np.abs(hp - interior3(prev["h"], case.nx, case.ny, case.nz))))
     head_err_ext = float(np.max(np.abs(hp - interior3(ext["h"], case.nx, case.ny, case.nz))))
+    head_err_pp = float(np.max(np.abs(hp - interior3(plain_prev["h"], case.nx, case.ny, case.nz))))
+
+    def depth_err(r):
+        a = r["depth"].reshape(case.ny + 2, case.nx + 2)[1:-1, 1:-1]
+        b = plain["depth"].reshape(case.ny + 2, case.nx + 2)[1:-1, 1:-1]
+        return float(np.max(np.abs(a - b)))
 
     def roof(kt_ms_n, bytes_per_cell):
         ms, n = kt_ms_n
@@ -400,11 +547,14 @@ def main():
     roof_guess = None
     ms, n = fast["live"]["guess"]
     if n:
-        flops = 2 * (11 * 64 + 64 * 64 + 64) * case.nx * case.ny * case.nz  # algorithmic MLP forward
+        # algorithmic MLP forward over the cells the network ran on (the gate sends the others
+        # to the extrapolated start; the count is the last step's, steps differ little)
+        flops = 2 * (11 * 64 + 64 * 64 + 64) * fast["inferred"]
         tf = flops / (ms / n / 1e3) / 1e12
         roof_guess = {"bound": "mfma", "achieved": round(tf, 2), "peak": F16_PEAK_TFLOPS, "unit": "TFLOP/s",
                       "frac": round(tf / F16_PEAK_TFLOPS, 5), "avg_launch_us": round(ms / n * 1e3, 2),
-                      "launches": n, "flop_per_cell": 9728}
+                      "launches": n, "flop_per_cell": 9728, "cells_inferred": fast["inferred"],
+                      "cells_total": n3}
 
     cpu = None
     if cpu_steps:
@@ -413,7 +563,8 @@ def main():
         parity = float(np.max(np.abs(interior3(cb["h"], case.nx, case.ny, case.nz) -
                                      interior3(plain["grabbed"], case.nx, case.ny, case.nz))))
         cpu = {"value": round(case.cells * cpu_steps / cb["elapsed"], 1), "unit": "cell-steps/s",
-               "cores": cb["threads"], "kind": "port",
+               "cores": cb["threads"], "kind": "port", "cpu_model": cpu_model(),
+               "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
                "sample": f"first {cpu_steps} coupled steps of the same {args.nx}x{args.ny}x{args.nz} workload "
                          f"({cb['elapsed']:.1f} s), FP64 oracle with Jacobi-PCG at the reference rtol, the same "
                          f"Picard settings as the plain path",
@@ -447,9 +598,20 @@ def main():
         "plain_fp64": {"value": round(plain_value, 1), "ms_per_step": round(plain["elapsed"] / args.steps * 1e3, 4),
                        "krylov_iters_per_step": plain["iters"] / args.steps,
                        "picard_iters_per_step": plain["picards"] / args.steps},
+        "plain_fp64_warm_start": {"value": round(case.cells * args.steps / plain_prev["elapsed"], 1),
+                                  "ms_per_step": round(plain_prev["elapsed"] / args.steps * 1e3, 4),
+                                  "krylov_iters_per_step": plain_prev["iters"] / args.steps,
+                                  "picard_iters_per_step": plain_prev["picards"] / args.steps,
+                                  "max_head_err_vs_plain_fp64_m": head_err_pp,
+                                  "note": "FP64 PCG, every solve from x0 = h^n (then h^m): the plain method with "
+                                          "warm starts"},
         "fast_picard_iters_per_step": fast["picards"] / args.steps,
         "speedup_vs_plain_fp64": round(value / plain_value, 3),
+        "speedup_vs_plain_fp64_warm_start": round(plain_prev["elapsed"] / fast["elapsed"], 3),
+        "speedup_learned_vs_extrapolated": round(ext["elapsed"] / fast["elapsed"], 3),
         "max_head_err_vs_plain_fp64_m": head_err,
+        "max_depth_err_vs_plain_fp64_m": depth_err(fast),
+        "learned_gate_m": args.gate,
         "fast_inner_iters_per_step": fast["iters"] / args.steps,
         "fast_refinements_per_step": fast["refines"] / args.steps,
         "fast_guess_fallbacks": fast["fallbacks"],
@@ -460,13 +622,15 @@ def main():
                                  "picard_iters_per_step": prev["picards"] / args.steps,
                                  "inner_iters_per_step": prev["iters"] / args.steps,
                                  "refinements_per_step": prev["refines"] / args.steps,
-                                 "max_head_err_vs_plain_fp64_m": head_err_prev},
+                                 "max_head_err_vs_plain_fp64_m": head_err_prev,
+                                 "max_depth_err_vs_plain_fp64_m": depth_err(prev)},
         "extrapolated_guess_mixed": {"value": round(case.cells * args.steps / ext["elapsed"], 1),
                                      "ms_per_step": round(ext["elapsed"] / args.steps * 1e3, 4),
                                      "picard_iters_per_step": ext["picards"] / args.steps,
                                      "inner_iters_per_step": ext["iters"] / args.steps,
                                      "refinements_per_step": ext["refines"] / args.steps,
-                                     "max_head_err_vs_plain_fp64_m": head_err_ext},
+                                     "max_head_err_vs_plain_fp64_m": head_err_ext,
+                                     "max_depth_err_vs_plain_fp64_m": depth_err(ext)},
         "training": training,
         "predictor_step": predictor,
         "roofline": roof64,

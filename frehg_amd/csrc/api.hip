@@ -652,6 +652,14 @@ int frehg_mlp_gate(frehg_ctx_t ctx, double gate) {
   });
 }
 
+int frehg_mlp_inferred_cells(frehg_ctx_t ctx, int64_t* cells) {
+  return guard(ctx, [&] {
+    if (!cells) throw frehg::ArgError("frehg_mlp_inferred_cells: null output");
+    *cells = need_mlp(ctx).inferred_cells();
+    return FREHG_OK;
+  });
+}
+
 int frehg_mlp_increment(frehg_ctx_t ctx, double* dst) {
   return guard(ctx, [&] {
     if (!dst) throw frehg::ArgError("frehg_mlp_increment: null output");

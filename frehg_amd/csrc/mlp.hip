@@ -87,6 +87,8 @@ struct FeatSrc {
   // Set to 1 by the inference kernel when a network output is not finite (that cell then takes
   // the extrapolated start); may be null.
   double* bad;
+  // Cells of the chunks the network ran on (counted per chunk); may be null.
+  unsigned long long* inferred;
 };
 
 __device__ __forceinline__ double base_increment(const FeatSrc& F, long long c) {
@@ -537,6 +539,7 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const flo
       if (i < g3.nx) h_out[c] = base;
       continue;
     }
+    if (lane == 0 && F.inferred) atomicAdd(F.inferred, (unsigned long long)min(RB, g3.nx - ib * RB));
     if (i < g3.nx) {
       cell_features<VG_ONLY>(F, i, j, k, f);
       // standardised features, clamped far outside any training distribution so that the FP16
@@ -652,6 +655,7 @@ FeatSrc feat_src(ReState& re, const double* depth) {
   F.inc_b = nullptr;
   F.gate = 0.0;
   F.bad = nullptr;
+  F.inferred = nullptr;
   return F;
 }
 
@@ -669,6 +673,8 @@ Mlp::Mlp(hipStream_t s, Profiler* prof) : s_(s), prof_(prof) {
   FREHG_HIP(hipMalloc(&part_, (size_t)TRAIN_BLOCKS * (kMlpParams + 1) * sizeof(float)));
   FREHG_HIP(hipMalloc(&stat_, (size_t)kStatBlocks * 2 * kMlpRow * sizeof(double)));
   FREHG_HIP(hipHostMalloc(&hbuf_, (kMlpParams + 1) * sizeof(float), hipHostMallocDefault));
+  FREHG_HIP(hipMalloc(&inferred_, sizeof(unsigned long long)));
+  FREHG_HIP(hipMemsetAsync(inferred_, 0, sizeof(unsigned long long), s_));
   // The guess kernel's blocks loop over chunks of cells: one round of resident blocks, so that
   // no partial last round idles most of the chip and each block loads the weights once.
   int dev = 0, cus = 0, per_cu = 0;
@@ -681,9 +687,16 @@ Mlp::Mlp(hipStream_t s, Profiler* prof) : s_(s), prof_(prof) {
 
 Mlp::~Mlp() {
   for (void* p : {(void*)params_, (void*)infer_, (void*)m_, (void*)v_, (void*)norm_, (void*)grad_, (void*)part_, (void*)log_,
-                  (void*)scratch_, (void*)stat_})
+                  (void*)scratch_, (void*)stat_, (void*)inferred_})
     if (p) (void)hipFree(p);
   if (hbuf_) (void)hipHostFree(hbuf_);
+}
+
+long long Mlp::inferred_cells() {
+  unsigned long long n = 0;
+  FREHG_HIP(hipMemcpyAsync(&n, inferred_, sizeof(n), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipStreamSynchronize(s_));
+  return (long long)n;
 }
 
 void Mlp::init(uint64_t seed) {
@@ -814,6 +827,8 @@ void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const d
   F.inc_b = inc_b;
   F.gate = gate_;
   F.bad = bad;
+  F.inferred = inferred_;
+  FREHG_HIP(hipMemsetAsync(inferred_, 0, sizeof(unsigned long long), s_));
   if (hn) F.hn = hn;
   if (wcn) F.wcn = wcn;
   const long long N = (long long)F.g.nx * F.g.ny * F.g.nz;

@@ -69,6 +69,8 @@ class Mlp {
   // Inference gate in metres (0: the network corrects every cell).
   void set_gate(double gate) { gate_ = gate; }
   double gate() const { return gate_; }
+  // Cells of the chunks the network ran on in the last predict_guess (synchronises the stream).
+  long long inferred_cells();
 
   void get_params(float* params, float* norm);
   void set_params(const float* params, const float* norm);
@@ -91,6 +93,7 @@ class Mlp {
   bool infer_stale_ = true;
   unsigned guess_blocks_ = 4096;  // guess kernel grid: resident blocks of one round
   double gate_ = 0.0;
+  unsigned long long* inferred_ = nullptr;  // device counter of the last guess
   int stride_ = 0;
   long long cap_ = 0, count_ = 0, log_steps_ = 0;
   int adam_t_ = 0;

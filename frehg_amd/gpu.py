@@ -88,6 +88,7 @@ def load():
         "frehg_mlp_predict": (C.c_int, [ctx, P(C.c_float), C.c_int, P(C.c_float)]),
         "frehg_mlp_increment": (C.c_int, [ctx, dp]),
         "frehg_mlp_gate": (C.c_int, [ctx, C.c_double]),
+        "frehg_mlp_inferred_cells": (C.c_int, [ctx, P(C.c_int64)]),
         "frehg_ls_solve_noalloc": (C.c_int, [ctx, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                              P(C.c_void_p), C.c_void_p, C.c_void_p, P(abi.SolverConfig),
                                              P(abi.SolveStats)]),
@@ -362,6 +363,11 @@ class Context:
         out = np.zeros(len(x), np.float32)
         self.check(self.lib.frehg_mlp_predict(self.h, _fp(x), len(x), _fp(out)))
         return out
+
+    def mlp_inferred_cells(self):
+        n = C.c_int64(0)
+        self.check(self.lib.frehg_mlp_inferred_cells(self.h, C.byref(n)))
+        return n.value
 
     def mlp_gate(self, gate):
         self.check(self.lib.frehg_mlp_gate(self.h, float(gate)))

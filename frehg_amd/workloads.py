@@ -33,7 +33,13 @@ def grf(shape, spacing, corr_len, sigma, seed):
 
 
 class CoupledCase:
-    """Coupled surface/subsurface benchmark case: a 512x512 surface over 512x512x30 soil."""
+    """Coupled surface/subsurface benchmark case: a 512x512 surface over 512x512x30 soil (the
+    defaults), or any size of the same recipe (large() gives the 1024x1024 surface at dx = 2 m over
+    1024x1024x64 soil at dz = 0.04 m)."""
+
+    @classmethod
+    def large(cls, seed_bed=11, seed_ks=12, **kw):
+        return cls(nx=1024, ny=1024, nz=64, dx=2.0, dz=0.04, seed_bed=seed_bed, seed_ks=seed_ks, **kw)
 
     def __init__(self, nx=512, ny=512, nz=30, dx=1.0, dz=0.05, seed_bed=1, seed_ks=2, slope=0.01,
                  bed_sigma=0.05, bed_corr=20.0, ks_median=1e-5, ks_sigma_ln=1.0, ks_corr=10.0,
@@ -76,6 +82,31 @@ class CoupledCase:
         ctx.re_init(self.nx, self.ny, self.nz, self.re_params(precision, guess, picard_max=picard_max))
         ctx.re_set_soil([self.soil], None, True, self.ks.ravel())
         ctx.re_set_initial(h=self.h0)
+
+
+class InfiltrationCase:
+    """The 40x40x50 infiltration benchmark: homogeneous van Genuchten soil (alpha 3.35 1/m, n 2.0,
+    theta_s 0.368, theta_r 0.102, Ks 9.22e-5 m/s, Ss 1e-6 1/m), dx = dy = 0.1 m, dz = 0.02 m,
+    initial head -10 m, top Dirichlet h = -0.75 m, free-drainage bottom (unit gradient, bc 3), no-flux
+    sides, dt = 10 s fixed for 360 steps, Picard to 1e-8 m, CG on the 7-point system."""
+
+    def __init__(self, nx=40, ny=40, nz=50, dt=10.0, steps=360, picard_max=50, picard_tol=1e-8):
+        self.nx, self.ny, self.nz, self.dt, self.steps = nx, ny, nz, dt, steps
+        self.soil = abi.soil_params(alpha=3.35, n=2.0, theta_s=0.368, theta_r=0.102, Ks_x=9.22e-5, Ks_y=9.22e-5,
+                                    Ks_z=9.22e-5, Ss=1e-6)
+        self.picard_max, self.picard_tol = picard_max, picard_tol
+        self.h0 = np.full(nx * ny * nz, -10.0)
+
+    def re_params(self, precision=abi.PREC_FP64, guess=abi.GUESS_ZERO, rtol=1e-10):
+        return abi.re_params(soil=self.soil, solver=abi.solver_config(rtol, precision=precision, inner_rtol=1e-5,
+                                                                      max_refine=8),
+                             bc_type=(0, 0, 0, 0, 3, 1), dx=0.1, dy=0.1, dz=0.02, botz=-0.02 * self.nz, dt=self.dt,
+                             adaptive_dt=0, use_full3d=1, htop=-0.75, initial_guess=guess,
+                             picard_max=self.picard_max, picard_tol=self.picard_tol)
+
+    @property
+    def cells(self):
+        return self.nx * self.ny * self.nz
 
 
 class EnsembleCase:

@@ -295,6 +295,8 @@ int frehg_mlp_predict(frehg_ctx_t ctx, const float* x, int n, float* out);
  * whose |d^{n-1}| <= gate starts from h^n + d^{n-1} without the network's correction, and a chunk
  * of cells that all do so skips the network. 0 applies the network to every cell. */
 int frehg_mlp_gate(frehg_ctx_t ctx, double gate);
+/* Cells in the chunks the network ran on in the last learned guess (the rest took h^n + d^{n-1}). */
+int frehg_mlp_inferred_cells(frehg_ctx_t ctx, int64_t* cells);
 /* The network's prediction for a step taken from the current Richards state (features from h,
  * K(h), C(h), theta): the part of the learned guess's increment on top of the previous step's
  * solve increment. nx*ny*nz values, i + j*nx + k*nx*ny. */
