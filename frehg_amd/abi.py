@@ -8,7 +8,7 @@ reference's parameter structs: SoilParams (include/frehg2/re/SoilParams.hpp:13-2
 """
 import ctypes as C
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 OK = 0
 INFO_GUESS_FALLBACK = 1
@@ -20,6 +20,10 @@ ERR_NOT_CONVERGED = -4
 ERR_NONFINITE = -5
 ERR_STATE = -6
 ERR_NO_DEVICE = -7
+
+BC_DISCHARGE = 0
+BC_DEPTH = 1
+BC_CRITICAL = 2
 
 KSP_CG = 0
 KSP_BICGSTAB = 1
@@ -107,7 +111,8 @@ class EnsStats(C.Structure):
 
 class CoupledStats(C.Structure):
     _fields_ = [("sw", SolveStats), ("gw", ReStats), ("gw_substeps", C.c_int32),
-                ("exchange_volume", C.c_double), ("t", C.c_double), ("gw_time", C.c_double)]
+                ("exchange_volume", C.c_double), ("t", C.c_double), ("gw_time", C.c_double),
+                ("boundary_outflow", C.c_double)]
 
 
 def soil_params(**kw):

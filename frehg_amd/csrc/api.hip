@@ -270,6 +270,25 @@ int frehg_swe_set_initial_eta(frehg_ctx_t ctx, const double* eta_interior) {
   });
 }
 
+int frehg_swe_set_boundary_polygons(frehg_ctx_t ctx, int nregions, const int32_t* kind, const double* rate,
+                                    const int32_t* nvert, const double* xy, double x0, double y0) {
+  return guard(ctx, [&] {
+    need_swe(ctx).set_boundary_polygons(nregions, kind, rate, nvert, xy, x0, y0);
+    return FREHG_OK;
+  });
+}
+
+int frehg_swe_apply_boundaries(frehg_ctx_t ctx, double dt, double* outflow) {
+  return guard(ctx, [&] {
+    if (!(dt > 0)) throw frehg::ArgError("frehg_swe_apply_boundaries: dt must be > 0");
+    auto& swe = need_swe(ctx);
+    swe.apply_boundaries_async(dt);
+    const double v = swe.boundary_outflow();
+    if (outflow) *outflow = v;
+    return FREHG_OK;
+  });
+}
+
 int frehg_swe_set_field(frehg_ctx_t ctx, const char* name, const double* src) {
   return guard(ctx, [&] {
     if (!name || !src) throw frehg::ArgError("null argument");
@@ -323,6 +342,9 @@ int frehg_coupled_step(frehg_ctx_t ctx, const frehg_coupling_params* cp, double 
     auto& re = need_re(ctx);
     const double t_new = ctx->t + dt;  // Orchestrator::step, coupled branch (:1322-1336)
     swe.step(rain, evap, &local.sw);
+    // surface boundary regions after the surface step (applySurfacePolygons, Orchestrator.cpp:1240)
+    const bool bc = swe.has_boundaries();
+    if (bc) swe.apply_boundaries_async(dt);
     auto& cpl = need_cpl(ctx);
     cpl.exchange_async(swe, re, *cp, dt);  // its volume is read after the groundwater step
     const double eps = 1.0e-9 * std::max(1.0, t_end);  // gwCatchUpTo (:1243-1252)
@@ -337,6 +359,7 @@ int frehg_coupled_step(frehg_ctx_t ctx, const frehg_coupling_params* cp, double 
     }
     local.gw.solve.iterations = iters;
     local.exchange_volume = cpl.exchanged_volume();
+    local.boundary_outflow = bc ? swe.boundary_outflow() : 0.0;
     ctx->t = t_new;
     local.t = ctx->t;
     local.gw_time = ctx->gw_time;

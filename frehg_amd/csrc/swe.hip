@@ -35,6 +35,25 @@ struct SwK {
   double dx, dy, dt, gravity, manning, min_depth, visc_x, visc_y, hD, hE, wtfh;
 };
 
+// src/bc/Polygon.hpp: even-odd ray casting with a +x ray, bounding-box reject, ring closed.
+bool polygon_contains(const double* xy, int n, double x, double y) {
+  if (n < 3) return false;
+  double xmin = xy[0], xmax = xy[0], ymin = xy[1], ymax = xy[1];
+  for (int v = 1; v < n; ++v) {
+    xmin = std::min(xmin, xy[2 * v]);
+    xmax = std::max(xmax, xy[2 * v]);
+    ymin = std::min(ymin, xy[2 * v + 1]);
+    ymax = std::max(ymax, xy[2 * v + 1]);
+  }
+  if (x < xmin || x > xmax || y < ymin || y > ymax) return false;
+  bool inside = false;
+  for (int i = 0, j = n - 1; i < n; j = i++) {
+    const double xi = xy[2 * i], yi = xy[2 * i + 1], xj = xy[2 * j], yj = xy[2 * j + 1];
+    if ((yi > y) != (yj > y) && x < xj + (y - yj) * (xi - xj) / (yi - yj)) inside = !inside;
+  }
+  return inside;
+}
+
 struct C2 {
   int i, j;
   long long c;
@@ -378,6 +397,52 @@ __global__ __launch_bounds__(NT) void k_swe_geom(SwK P, const double* dept, cons
   }
 }
 
+// PolygonBC::applySurface (src/bc/PolygonBC.cpp:16-78): each interior cell tagged with a region
+// takes its rule (prescribed depth; discharge Q dt spread over the region's cells; critical-depth
+// weir q = sqrt(g) h^1.5 min(dx, dy)), outflow limited by the ponded volume, then updateDepth's
+// interior snap (SweSolver.cpp:219-227) for the cells the rule changed (every other cell was
+// snapped at the end of the step). Per-block partial of the net outflow volume.
+__global__ __launch_bounds__(NT) void k_swe_boundary(SwK P, const int32_t* __restrict__ idx,
+                                                      const SurfaceBcRule* __restrict__ rules, double dt,
+                                                      double* __restrict__ eta, const double* __restrict__ bottom,
+                                                      double* __restrict__ part) {
+  __shared__ double lds[NT / kWave];
+  const C2 t = cell2(P.g);
+  double out = 0.0;
+  if (t.valid) {
+    const int r = idx[t.c];
+    if (r >= 0) {
+      const SurfaceBcRule R = rules[r];
+      const double area = P.dx * P.dy;
+      const double e0 = eta[t.c], b = bottom[t.c];
+      const double depth = fmax(e0 - b, 0.0);
+      double e = e0;
+      if (R.kind == FREHG_BC_DEPTH) {
+        e = b + fmax(R.rate, 0.0);
+        out = (e0 - e) * area;
+      } else if (R.kind == FREHG_BC_DISCHARGE) {
+        if (R.count > 0) {
+          double vol = R.rate * dt / (double)R.count;
+          if (vol > depth * area) vol = depth * area;
+          e = e0 - vol / area;
+          out = vol;
+        }
+      } else {
+        const double q = sqrt(P.gravity) * pow(depth, 1.5) * fmin(P.dx, P.dy);
+        double vol = q * dt;
+        if (vol > depth * area) vol = depth * area;
+        e = e0 - vol / area;
+        out = vol;
+      }
+      const double diff = e - b;
+      if (diff > 0.0 && diff < P.min_depth) e = b;
+      eta[t.c] = e;
+    }
+  }
+  const double s = block_sum<NT>(out, lds);
+  if (threadIdx.x == 0) part[blockIdx.x + gridDim.x * blockIdx.y] = s;
+}
+
 // maxCfl (SweSolver.cpp:954-975).
 __global__ __launch_bounds__(NT) void k_swe_maxcfl(SwK P, const double* dept, const double* uu, const double* vv,
                                                     double* part, unsigned int nblocks) {
@@ -452,7 +517,7 @@ SweState::~SweState() {
     if (ge) (void)hipGraphExecDestroy(ge);
   for (auto& kv : f_) (void)hipFree(kv.second);
   for (void* q : {(void*)D_, (void*)ax_, (void*)ay_, (void*)b_, (void*)fixed_, (void*)red_, (void*)dres_,
-                  (void*)dcount_})
+                  (void*)dcount_, (void*)bcidx_, (void*)bcrules_, (void*)bcpart_})
     if (q) (void)hipFree(q);
   if (hres_) (void)hipHostFree(hres_);
 }
@@ -730,6 +795,78 @@ void SweState::step(double rain, double evap, frehg_solve_stats* st) {
     st->rhs_norm = sr.rhs_norm;
     st->converged = sr.converged ? 1 : 0;
   }
+}
+
+void SweState::set_boundary_polygons(int nreg, const int32_t* kind, const double* rate, const int32_t* nvert,
+                                     const double* xy, double x0, double y0) {
+  if (nreg < 0) throw ArgError("frehg_swe_set_boundary_polygons: nregions must be >= 0");
+  if (nreg > 0 && (!kind || !rate || !nvert || !xy)) throw ArgError("frehg_swe_set_boundary_polygons: null argument");
+  // PolygonIndex::build (src/bc/PolygonIndex.cpp:9-33): cell centres, first match wins; the
+  // per-region cell counts of globalColumnCounts (:45-56)
+  std::vector<SurfaceBcRule> rules((size_t)nreg);
+  std::vector<size_t> off((size_t)nreg + 1, 0);
+  for (int r = 0; r < nreg; ++r) {
+    if (kind[r] != FREHG_BC_DISCHARGE && kind[r] != FREHG_BC_DEPTH && kind[r] != FREHG_BC_CRITICAL)
+      throw ArgError("frehg_swe_set_boundary_polygons: unknown region kind");
+    if (nvert[r] < 0) throw ArgError("frehg_swe_set_boundary_polygons: negative vertex count");
+    rules[(size_t)r].kind = kind[r];
+    rules[(size_t)r].rate = rate[r];
+    rules[(size_t)r].count = 0;
+    off[(size_t)r + 1] = off[(size_t)r] + (size_t)nvert[r];
+  }
+  std::vector<int32_t> idx((size_t)n_, -1);
+  for (int j = 0; j < g_.ny; ++j)
+    for (int i = 0; i < g_.nx; ++i) {
+      const double x = x0 + (i + 0.5) * dx_, y = y0 + (j + 0.5) * dy_;
+      for (int r = 0; r < nreg; ++r)
+        if (polygon_contains(xy + 2 * off[(size_t)r], nvert[r], x, y)) {
+          idx[(size_t)g_.sidx(i, j)] = r;
+          ++rules[(size_t)r].count;
+          break;
+        }
+    }
+  for (void* q : {(void*)bcidx_, (void*)bcrules_})
+    if (q) FREHG_HIP(hipFree(q));
+  bcidx_ = nullptr;
+  bcrules_ = nullptr;
+  nbc_ = nreg;
+  if (nreg == 0) return;
+  FREHG_HIP(hipMalloc(&bcidx_, (size_t)n_ * sizeof(int32_t)));
+  FREHG_HIP(hipMalloc(&bcrules_, (size_t)nreg * sizeof(SurfaceBcRule)));
+  if (!bcpart_) {
+    const unsigned int nb = ceil_div(g_.nx, 64) * ceil_div(g_.ny, 4);
+    FREHG_HIP(hipMalloc(&bcpart_, (size_t)nb * sizeof(double)));
+  }
+  FREHG_HIP(hipMemcpy(bcidx_, idx.data(), (size_t)n_ * sizeof(int32_t), hipMemcpyHostToDevice));
+  FREHG_HIP(hipMemcpy(bcrules_, rules.data(), (size_t)nreg * sizeof(SurfaceBcRule), hipMemcpyHostToDevice));
+}
+
+void SweState::apply_boundaries_async(double dt) {
+  if (nbc_ == 0) {
+    hres_[3] = 0.0;
+    return;
+  }
+  const SwK P = make_swk(g_, dx_, dy_, p_);
+  const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4)), blk(NT);
+  hipLaunchKernelGGL(k_swe_boundary, grid, blk, 0, s_, P, (const int32_t*)bcidx_, (const SurfaceBcRule*)bcrules_, dt,
+                     f_["eta"], (const double*)f_["bottom"], bcpart_);
+  FREHG_LAUNCH_CHECK();
+  // depth and geometry refresh (PolygonBC.cpp:74-77); updateGeometry first keeps Vs as Vsn
+  hipLaunchKernelGGL(k_swe_depth, grid, blk, 0, s_, P, f_["eta"], f_["bottom"], f_["dept"], f_["deptx"], f_["depty"]);
+  FREHG_LAUNCH_CHECK();
+  copy_field(f_["Vs"], f_["Vsn"]);
+  GeoOut go{f_["Vs"], f_["Asx"], f_["Asy"], f_["Asz"], f_["Aszx"], f_["Aszy"], f_["Vsx"], f_["Vsy"]};
+  hipLaunchKernelGGL(k_swe_geom, grid, blk, 0, s_, P, f_["dept"], f_["deptx"], f_["depty"], go);
+  FREHG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kRedThreads), 0, s_, (const double*)bcpart_, grid.x * grid.y, 1,
+                     dres_ + 3);
+  FREHG_LAUNCH_CHECK();
+  FREHG_HIP(hipMemcpyAsync(hres_ + 3, dres_ + 3, sizeof(double), hipMemcpyDeviceToHost, s_));
+}
+
+double SweState::boundary_outflow() {
+  FREHG_HIP(hipStreamSynchronize(s_));  // returns at once when the stream has already drained
+  return hres_[3];
 }
 
 double SweState::max_cfl() {

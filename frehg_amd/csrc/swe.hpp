@@ -11,6 +11,14 @@
 
 namespace frehg {
 
+// One surface boundary region's rule (PolygonBC, src/bc/PolygonRegion.hpp:20-35): kind
+// FREHG_BC_*, rate (Q m^3/s or h m), and the region's cell count.
+struct SurfaceBcRule {
+  int kind;
+  int count;
+  double rate;
+};
+
 class SweState {
  public:
   SweState(int nx, int ny, double dx, double dy, const frehg_swe_params& p, hipStream_t s, Profiler* prof);
@@ -27,6 +35,15 @@ class SweState {
   void get_field(const std::string& name, double* dst);
   void step(double rain, double evap, frehg_solve_stats* st);
   double max_cfl();
+  // PolygonBC regions (src/bc/PolygonBC.hpp:28-46) built by PolygonIndex::build's rule from
+  // polygons; applied after each surface step of a coupled step (and on request).
+  void set_boundary_polygons(int nreg, const int32_t* kind, const double* rate, const int32_t* nvert,
+                             const double* xy, double x0, double y0);
+  bool has_boundaries() const { return nbc_ > 0; }
+  // PolygonBC::applySurface over dt, queued on the stream; boundary_outflow() waits and returns
+  // the net outflow volume [m^3] of the last application.
+  void apply_boundaries_async(double dt);
+  double boundary_outflow();
 
   const Geo& geo() const { return g_; }
   double dx() const { return dx_; }
@@ -53,8 +70,12 @@ class SweState {
   double *D_ = nullptr, *ax_ = nullptr, *ay_ = nullptr, *b_ = nullptr;
   unsigned char* fixed_ = nullptr;
   double* red_ = nullptr;
-  double* dres_ = nullptr;
+  double* dres_ = nullptr;  // [0..1] max CFL, [3] boundary outflow
   double* hres_ = nullptr;
+  int nbc_ = 0;  // surface boundary regions
+  int32_t* bcidx_ = nullptr;
+  SurfaceBcRule* bcrules_ = nullptr;
+  double* bcpart_ = nullptr;
   unsigned int* dcount_ = nullptr;
   std::unique_ptr<Krylov> kry_;
   // captured step graphs (before / after the surface solve) and what they were captured for

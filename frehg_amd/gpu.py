@@ -58,6 +58,9 @@ def load():
         "frehg_swe_set_bathymetry": (C.c_int, [ctx, dp]),
         "frehg_swe_initialize_state": (C.c_int, [ctx, C.c_double]),
         "frehg_swe_set_initial_eta": (C.c_int, [ctx, dp]),
+        "frehg_swe_set_boundary_polygons": (C.c_int, [ctx, C.c_int, P(C.c_int32), dp, P(C.c_int32), dp, C.c_double,
+                                                      C.c_double]),
+        "frehg_swe_apply_boundaries": (C.c_int, [ctx, C.c_double, dp]),
         "frehg_swe_set_field": (C.c_int, [ctx, C.c_char_p, dp]),
         "frehg_swe_get_field": (C.c_int, [ctx, C.c_char_p, dp]),
         "frehg_swe_step": (C.c_int, [ctx, C.c_double, C.c_double, P(abi.SolveStats)]),
@@ -202,6 +205,22 @@ class Context:
         a = np.ascontiguousarray(eta_phys, dtype=np.float64).ravel()
         assert a.size == self.swe_dims[0] * self.swe_dims[1]
         self.check(self.lib.frehg_swe_set_initial_eta(self.h, _ptr(a)))
+
+    def swe_set_boundary_polygons(self, regions, x0=0.0, y0=0.0):
+        """PolygonBC regions: a list of (kind, rate, [(x, y), ...]) with kind abi.BC_*."""
+        kind = np.array([r[0] for r in regions], dtype=np.int32)
+        rate = np.array([float(r[1]) for r in regions], dtype=np.float64)
+        nvert = np.array([len(r[2]) for r in regions], dtype=np.int32)
+        xy = np.ascontiguousarray(np.concatenate([np.asarray(r[2], dtype=np.float64).ravel() for r in regions])
+                                  if regions else np.zeros(1), dtype=np.float64)
+        self.check(self.lib.frehg_swe_set_boundary_polygons(self.h, len(regions), kind.ctypes.data_as(P(C.c_int32)),
+                                                             _ptr(rate), nvert.ctypes.data_as(P(C.c_int32)), _ptr(xy),
+                                                             x0, y0))
+
+    def swe_apply_boundaries(self, dt):
+        v = C.c_double(0.0)
+        self.check(self.lib.frehg_swe_apply_boundaries(self.h, dt, C.byref(v)))
+        return v.value
 
     def swe_nsurf(self):
         nx, ny = self.swe_dims

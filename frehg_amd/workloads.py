@@ -44,7 +44,7 @@ class CoupledCase:
     def __init__(self, nx=512, ny=512, nz=30, dx=1.0, dz=0.05, seed_bed=1, seed_ks=2, slope=0.01,
                  bed_sigma=0.05, bed_corr=20.0, ks_median=1e-5, ks_sigma_ln=1.0, ks_corr=10.0,
                  alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, ss=1e-5, h0=-2.0, manning=0.03,
-                 rain=RAIN_50MM_PER_H, dt=1.0, t_end=1800.0):
+                 rain=RAIN_50MM_PER_H, dt=1.0, t_end=1800.0, outflow=True):
         self.nx, self.ny, self.nz, self.dx, self.dz = nx, ny, nz, dx, dz
         self.dt, self.rain, self.t_end = dt, rain, t_end
         x = (np.arange(nx) + 0.5) * dx
@@ -59,6 +59,15 @@ class CoupledCase:
         self.cp = abi.coupling_params(dx=dx, dy=dx)
         # closed sides and bottom; the exchange is the only top flux (bc z- = 0)
         self.bc = (0, 0, 0, 0, 0, 0)
+        self.outflow = outflow
+
+    def boundary_regions(self):
+        """Outflow at the low-x edge: a critical-depth region (bc_critical) over the first column
+        of surface cells (centres at x = dx/2), as a polygon strip in domain coordinates."""
+        if not self.outflow:
+            return []
+        dx, ly = self.dx, self.ny * self.dx
+        return [(abi.BC_CRITICAL, 0.0, [(-dx, -dx), (dx, -dx), (dx, ly + dx), (-dx, ly + dx)])]
 
     def re_params(self, precision=abi.PREC_FP64, guess=abi.GUESS_ZERO, rtol=1e-10, inner_rtol=1e-5, picard_max=1,
                   picard_tol=1e-8):
@@ -79,6 +88,7 @@ class CoupledCase:
         ctx.swe_init(self.nx, self.ny, self.dx, self.dx, self.swe)
         ctx.swe_set_bathymetry(self.bed_with_offset())
         ctx.swe_initialize_state(-1.0e3)  # dry start: eta clamps to the bed
+        ctx.swe_set_boundary_polygons(self.boundary_regions())
         ctx.re_init(self.nx, self.ny, self.nz, self.re_params(precision, guess, picard_max=picard_max))
         ctx.re_set_soil([self.soil], None, True, self.ks.ravel())
         ctx.re_set_initial(h=self.h0)

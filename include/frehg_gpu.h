@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define FREHG_ABI_VERSION 3
+#define FREHG_ABI_VERSION 4
 
 enum {
   FREHG_OK = 0,
@@ -57,6 +57,9 @@ enum {
                                increment minus the previous one */
   FREHG_GUESS_EXTRAPOLATED = 3 /* x0 = h^n + the previous step's solve increment (h^n on the first step) */
 };
+
+/* Surface boundary region rules: BcKind (src/bc/PolygonRegion.hpp:20-24). */
+enum { FREHG_BC_DISCHARGE = 0, FREHG_BC_DEPTH = 1, FREHG_BC_CRITICAL = 2 };
 
 /* Soil class: include/frehg2/re/SoilParams.hpp:13-25. */
 typedef struct {
@@ -139,6 +142,7 @@ typedef struct {
   int32_t gw_substeps;
   double exchange_volume;  /* sum(q*dt) of this step (+ = net GW->SW seepage) */
   double t, gw_time;
+  double boundary_outflow; /* net volume the surface boundary regions removed this step [m^3] */
 } frehg_coupled_stats;
 
 typedef struct frehg_ctx_s* frehg_ctx_t;
@@ -193,6 +197,16 @@ int frehg_swe_set_field(frehg_ctx_t ctx, const char* name, const double* src);
 int frehg_swe_get_field(frehg_ctx_t ctx, const char* name, double* dst);
 /* SweSolver::advanceStep(rain, evap) (:803-853). */
 int frehg_swe_step(frehg_ctx_t ctx, double rain, double evap, frehg_solve_stats* stats);
+/* PolygonBC(regions, PolygonIndex) (src/bc/PolygonBC.hpp:28-46, src/bc/PolygonIndex.cpp:9-56): nregions
+ * polygons, region r with nvert[r] vertices (x, y pairs in xy, regions concatenated; rings closed),
+ * kind[r] = FREHG_BC_* and rate[r] (discharge Q m^3/s, > 0 out; depth h m; critical: unused). A cell
+ * whose centre (x0 + (i + 1/2) dx, y0 + (j + 1/2) dy) lies inside a polygon (even-odd rule, first
+ * match wins) takes that region's rule. nregions = 0 removes the regions. frehg_coupled_step
+ * applies them after the surface step, before the exchange (Orchestrator.cpp:1240, :1301-1308). */
+int frehg_swe_set_boundary_polygons(frehg_ctx_t ctx, int nregions, const int32_t* kind, const double* rate,
+                                    const int32_t* nvert, const double* xy, double x0, double y0);
+/* PolygonBC::applySurface (src/bc/PolygonBC.cpp:16-78) over dt: *outflow = net volume removed [m^3]. */
+int frehg_swe_apply_boundaries(frehg_ctx_t ctx, double dt, double* outflow);
 /* SweSolver::maxCfl (:954-975). */
 int frehg_swe_max_cfl(frehg_ctx_t ctx, double* out);
 

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <array>
 #include <vector>
 
 namespace frehg_oracle {
@@ -321,6 +322,8 @@ class SweSolver {
   void finalizeInitialState();                                  // :904-924
   void advanceStep(real rain, real evap, SolveStats* st = nullptr);  // :803-853
   real maxCfl() const;                                          // :954-975
+  void updateDepth();     // :207-304 (public for the polygon boundary refresh)
+  void updateGeometry();  // :306-369
   const SweParams& params() const { return p_; }
 
   Grid g;  // nz unused
@@ -334,8 +337,6 @@ class SweSolver {
   long long s(int i, int j) const { return g.sidx(i, j); }
   void ghostsZeroGradient(std::vector<real>& f);
   void updateBottomFaces();
-  void updateDepth();
-  void updateGeometry();
   void enforceSurfBc();
   void momentumSource();
   void shallowwaterRhs();
@@ -348,6 +349,25 @@ class SweSolver {
   void updateVelocity();
   void enforceVeloBc();
   void interpVelocity();
+};
+
+// ---------------------------------------------------------------------------------------------
+// Polygon boundary regions of the surface: src/bc/Polygon.hpp (even-odd membership),
+// src/bc/PolygonIndex.cpp:9-33 (cell-centre index, first match wins), src/bc/PolygonBC.cpp:16-78
+// (applySurface). kind: 0 discharge (rate = Q m^3/s, > 0 out), 1 depth (rate = h m),
+// 2 critical (rate unused).
+struct BcPolygonRegion {
+  int kind = 0;
+  real rate = 0.0;
+  std::vector<std::array<double, 2>> vertices;
+};
+struct PolygonBc {
+  std::vector<BcPolygonRegion> regions;
+  std::vector<int> index;  // per surface storage cell: region or -1
+  std::vector<int> count;  // interior cells per region
+  void build(const SweSolver& swe, double x0, double y0);
+  // Net outflow volume [m^3] of one application over dt; refreshes depth and geometry.
+  real apply(SweSolver& swe, real dt) const;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -364,6 +384,8 @@ real couple_exchange(const CouplingParams& cp, SweSolver& swe, ReSolver& re, rea
 struct CoupledState {
   real t = 0.0, gw_time = 0.0, exchange_volume = 0.0;
   int gw_substeps = 0;
+  const PolygonBc* bc = nullptr;  // surface boundary regions, applied after the SW step
+  real boundary_outflow = 0.0;    // accumulated net outflow volume
 };
 void coupled_step(const CouplingParams& cp, SweSolver& swe, ReSolver& re, CoupledState& cs,
                   real dt, real rain, real evap, real t_end_hint);

@@ -2,6 +2,7 @@
 // Loaded by tests/ and by bench.py's cpu_baseline leg through ctypes. Parameter structs are the
 // ones of the product ABI (include/frehg_gpu.h) so both sides are driven with identical inputs.
 #include <cstring>
+#include <map>
 #include <omp.h>
 #include <exception>
 #include <stdexcept>
@@ -299,6 +300,37 @@ int orc_couple_exchange(void* swe, void* re, const frehg_coupling_params* cp, do
 }
 
 // state[4] = {t, gw_time, exchange_volume, gw_substeps}
+// Surface boundary regions per oracle SweSolver (PolygonBC + PolygonIndex), applied by
+// orc_swe_apply_boundaries and by the coupled step after the SW advance.
+static std::map<const void*, PolygonBc>& bc_regions() {
+  static std::map<const void*, PolygonBc> m;
+  return m;
+}
+int orc_swe_set_boundary_polygons(void* h, int nreg, const int32_t* kind, const double* rate, const int32_t* nvert,
+                                  const double* xy, double x0, double y0) {
+  return guard([&] {
+    auto* s = static_cast<SweSolver*>(h);
+    PolygonBc bc;
+    size_t off = 0;
+    for (int r = 0; r < nreg; ++r) {
+      BcPolygonRegion reg;
+      reg.kind = kind[r];
+      reg.rate = rate[r];
+      for (int v = 0; v < nvert[r]; ++v, ++off) reg.vertices.push_back({xy[2 * off], xy[2 * off + 1]});
+      bc.regions.push_back(reg);
+    }
+    bc.build(*s, x0, y0);
+    if (nreg == 0) bc_regions().erase(h);
+    else bc_regions()[h] = bc;
+  });
+}
+int orc_swe_apply_boundaries(void* h, double dt, double* outflow) {
+  return guard([&] {
+    auto it = bc_regions().find(h);
+    const double v = it == bc_regions().end() ? 0.0 : it->second.apply(*static_cast<SweSolver*>(h), dt);
+    if (outflow) *outflow = v;
+  });
+}
 int orc_coupled_step(void* swe, void* re, const frehg_coupling_params* cp, double* state, double dt,
                      double rain, double evap, double t_end) {
   return guard([&] {
@@ -308,11 +340,15 @@ int orc_coupled_step(void* swe, void* re, const frehg_coupling_params* cp, doubl
     cs.gw_time = state[1];
     cs.exchange_volume = state[2];
     cs.gw_substeps = (int)state[3];
+    cs.boundary_outflow = state[4];
+    auto it = bc_regions().find(swe);
+    cs.bc = it == bc_regions().end() ? nullptr : &it->second;
     coupled_step(c, *static_cast<SweSolver*>(swe), *static_cast<ReSolver*>(re), cs, dt, rain, evap, t_end);
     state[0] = cs.t;
     state[1] = cs.gw_time;
     state[2] = cs.exchange_volume;
     state[3] = cs.gw_substeps;
+    state[4] = cs.boundary_outflow;
   });
 }
 

@@ -469,6 +469,84 @@ real SweSolver::maxCfl() const {  // :954-975
 }
 
 // ---------------------------------------------------------------------------------------------
+// Polygon boundary regions.
+
+// src/bc/Polygon.hpp: even-odd ray casting (+x ray), bounding-box reject, ring closed.
+static bool polygon_contains(const std::vector<std::array<double, 2>>& v, double x, double y) {
+  const size_t n = v.size();
+  if (n < 3) return false;
+  double xmin = v[0][0], xmax = v[0][0], ymin = v[0][1], ymax = v[0][1];
+  for (const auto& p : v) {
+    xmin = std::min(xmin, p[0]);
+    xmax = std::max(xmax, p[0]);
+    ymin = std::min(ymin, p[1]);
+    ymax = std::max(ymax, p[1]);
+  }
+  if (x < xmin || x > xmax || y < ymin || y > ymax) return false;
+  bool inside = false;
+  for (size_t i = 0, j = n - 1; i < n; j = i++) {
+    const double xi = v[i][0], yi = v[i][1], xj = v[j][0], yj = v[j][1];
+    if ((yi > y) != (yj > y)) {
+      const double xc = xj + (y - yj) * (xi - xj) / (yi - yj);
+      if (x < xc) inside = !inside;
+    }
+  }
+  return inside;
+}
+
+void PolygonBc::build(const SweSolver& swe, double x0, double y0) {  // PolygonIndex.cpp:9-33
+  index.assign((size_t)swe.g.nsurf(), -1);
+  count.assign(regions.size(), 0);
+  for (int j = 0; j < swe.g.ny; ++j)
+    for (int i = 0; i < swe.g.nx; ++i) {
+      const double x = x0 + (i + 0.5) * swe.dx, y = y0 + (j + 0.5) * swe.dy;
+      for (size_t p = 0; p < regions.size(); ++p)
+        if (polygon_contains(regions[p].vertices, x, y)) {
+          index[(size_t)swe.g.sidx(i, j)] = (int)p;
+          ++count[p];  // globalColumnCounts (PolygonIndex.cpp:45-56), one rank
+          break;
+        }
+    }
+}
+
+real PolygonBc::apply(SweSolver& swe, real dt) const {  // PolygonBC.cpp:16-78
+  if (regions.empty()) return 0.0;
+  const real area = swe.dx * swe.dy;
+  const real gravity = swe.params().gravity;
+  real net = 0.0;
+  for (int j = 0; j < swe.g.ny; ++j)
+    for (int i = 0; i < swe.g.nx; ++i) {
+      const long long c = swe.g.sidx(i, j);
+      const int p = index[(size_t)c];
+      if (p < 0) continue;
+      const BcPolygonRegion& r = regions[(size_t)p];
+      const real depth = std::max(swe.eta[c] - swe.bottom[c], 0.0);
+      if (r.kind == 1) {  // depth: prescribed ponded depth
+        const real hbc = std::max(r.rate, 0.0);
+        const real old = swe.eta[c];
+        swe.eta[c] = swe.bottom[c] + hbc;
+        net += (old - swe.eta[c]) * area;
+      } else if (r.kind == 0) {  // discharge Q spread over the region's cells, outflow limited
+        const int gc = count[(size_t)p];
+        if (gc <= 0) continue;
+        real vol = r.rate * dt / (real)gc;
+        if (vol > depth * area) vol = depth * area;
+        swe.eta[c] -= vol / area;
+        net += vol;
+      } else {  // critical-depth weir: q = sqrt(g) h^1.5 min(dx, dy), limited
+        const real q = std::sqrt(gravity) * std::pow(depth, 1.5) * std::min(swe.dx, swe.dy);
+        real vol = q * dt;
+        if (vol > depth * area) vol = depth * area;
+        swe.eta[c] -= vol / area;
+        net += vol;
+      }
+    }
+  swe.updateDepth();
+  swe.updateGeometry();
+  return net;
+}
+
+// ---------------------------------------------------------------------------------------------
 // src/coupling/Coupling.cpp:25-164. Soil is class 0 (re.params().soil), as in the reference.
 real couple_exchange(const CouplingParams& cp, SweSolver& swe, ReSolver& re, real dt,
                      std::vector<real>* q_out) {
@@ -530,6 +608,7 @@ void coupled_step(const CouplingParams& cp, SweSolver& swe, ReSolver& re, Couple
                   real dt, real rain, real evap, real t_end_hint) {
   const real t_new = cs.t + dt;  // Orchestrator.cpp:1322-1336
   swe.advanceStep(rain, evap);
+  if (cs.bc) cs.boundary_outflow += cs.bc->apply(swe, dt);  // applySurfacePolygons, :1240, :1301-1308
   cs.exchange_volume += couple_exchange(cp, swe, re, dt, nullptr);
   const real eps = 1.0e-9 * std::max<real>(1.0, t_end_hint);  // gwCatchUpTo :1243-1252
   while (cs.gw_time < t_new - eps) {

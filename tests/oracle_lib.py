@@ -46,6 +46,9 @@ def lib():
         L.orc_swe_set_bathymetry.argtypes = [C.c_void_p, dp]
         L.orc_swe_initialize_state.argtypes = [C.c_void_p, C.c_double]
         L.orc_swe_set_initial_eta.argtypes = [C.c_void_p, dp]
+        L.orc_swe_set_boundary_polygons.argtypes = [C.c_void_p, C.c_int, P(C.c_int32), dp, P(C.c_int32), dp,
+                                                    C.c_double, C.c_double]
+        L.orc_swe_apply_boundaries.argtypes = [C.c_void_p, C.c_double, dp]
         L.orc_swe_get.argtypes = [C.c_void_p, C.c_char_p, dp]
         L.orc_swe_set.argtypes = [C.c_void_p, C.c_char_p, dp]
         L.orc_swe_step.argtypes = [C.c_void_p, C.c_double, C.c_double, P(abi.SolveStats)]
@@ -159,6 +162,16 @@ class OracleSwe:
         """setInitialEtaPhysical + setInitialVelocityConstant(0, 0) + finalizeInitialState."""
         check(lib().orc_swe_set_initial_eta(self.h, ptr(np.ascontiguousarray(eta_phys, dtype=np.float64).ravel())))
 
+    def set_boundary_polygons(self, regions, x0=0.0, y0=0.0):
+        """PolygonBC regions: a list of (kind, rate, [(x, y), ...]) (kind 0 discharge, 1 depth,
+        2 critical)."""
+        check(set_polygons(lib().orc_swe_set_boundary_polygons, self.h, regions, x0, y0))
+
+    def apply_boundaries(self, dt):
+        v = C.c_double(0.0)
+        check(lib().orc_swe_apply_boundaries(self.h, dt, C.byref(v)))
+        return v.value
+
     def get(self, name):
         out = np.empty(self.nsurf, dtype=np.float64)
         check(lib().orc_swe_get(self.h, name.encode(), ptr(out)))
@@ -178,6 +191,19 @@ class OracleSwe:
         return lib().orc_swe_max_cfl(self.h)
 
 
+def set_polygons(fn, handle, regions, x0, y0):
+    """Calls a (handle, nregions, kind, rate, nvert, xy, x0, y0) setter with `regions` =
+    [(kind, rate, [(x, y), ...]), ...] (shared by the oracle and the GPU bindings)."""
+    kind = np.array([r[0] for r in regions], dtype=np.int32)
+    rate = np.array([float(r[1]) for r in regions], dtype=np.float64)
+    nvert = np.array([len(r[2]) for r in regions], dtype=np.int32)
+    xy = np.ascontiguousarray(np.concatenate([np.asarray(r[2], dtype=np.float64).ravel() for r in regions])
+                              if regions else np.zeros(0), dtype=np.float64)
+    rc = fn(handle, len(regions), kind.ctypes.data_as(P(C.c_int32)), rate.ctypes.data_as(dp),
+            nvert.ctypes.data_as(P(C.c_int32)), xy.ctypes.data_as(dp), x0, y0)
+    return rc
+
+
 def couple_exchange(swe, re, cp, dt, want_q=False):
     vol = C.c_double(0.0)
     q = np.empty(swe.nx * swe.ny, dtype=np.float64) if want_q else None
@@ -190,7 +216,7 @@ class OracleCoupled:
 
     def __init__(self, swe, re, cp, t_end):
         self.swe, self.re, self.cp, self.t_end = swe, re, cp, t_end
-        self.state = np.zeros(4, dtype=np.float64)
+        self.state = np.zeros(5, dtype=np.float64)  # t, gw_time, exchange volume, substeps, outflow
 
     def step(self, dt, rain, evap=0.0):
         check(lib().orc_coupled_step(self.swe.h, self.re.h, C.byref(self.cp), ptr(self.state), dt,
