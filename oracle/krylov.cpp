@@ -17,7 +17,7 @@ real det_sum(const std::vector<real>& v) {
   const long long B = 4096;
   const long long nb = (n + B - 1) / B;
   std::vector<real> part((size_t)nb, 0.0);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(nb >= 8)
   for (long long b = 0; b < nb; ++b) {
     real s = 0.0;
     const long long e = std::min(n, (b + 1) * B);
@@ -37,7 +37,7 @@ void StencilOp::resize() {
 
 void StencilOp::apply(const std::vector<real>& x, std::vector<real>& y) const {
   const int nx = g.nx, ny = g.ny, nz = three_d ? g.nz : 1;
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if(omp_wide((long long)nx * ny * nz))
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -62,7 +62,7 @@ real dot_interior(const StencilOp& A, const std::vector<real>& a, const std::vec
                   const std::vector<unsigned char>* mask) {
   const int nx = A.g.nx, ny = A.g.ny, nz = A.three_d ? A.g.nz : 1;
   std::vector<real> part((size_t)nz * ny, 0.0);
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if(omp_wide((long long)nx * ny * nz))
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j) {
       real s = 0.0;
@@ -154,7 +154,7 @@ SolveStats pcg_only(const StencilOp& A, const std::vector<real>& b_in, std::vect
   (void)sz;
 
   auto forInterior = [&](auto&& fn) {
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if(omp_wide((long long)nx * ny * nz))
     for (int k = 0; k < nz; ++k)
       for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {

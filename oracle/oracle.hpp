@@ -164,6 +164,11 @@ struct VG {
 
 // Halo-padded index convention, src/core/Grid.hpp:42-60: one ghost cell on every side,
 // i fastest, then j, then k; k = 0 is the TOP layer; k = -1 and k = nz are ghosts.
+// Loops over fewer cells than this run on one thread: thread start-up and barriers would cost more
+// than the work (and stall badly on a busy host). Results do not depend on it: the parallel loops
+// write one cell each, and the sums use fixed blocks whatever the thread count.
+inline bool omp_wide(long long cells) { return cells >= 32768; }
+
 struct Grid {
   int nx = 1, ny = 1, nz = 1;
   long long plane() const { return (long long)(nx + 2) * (ny + 2); }

@@ -236,6 +236,15 @@ int orc_re_step(void* h, frehg_re_stats* st) {
 }
 double orc_re_dt(void* h) { return static_cast<ReSolver*>(h)->params().dt; }
 
+// Surface boundary regions per oracle SweSolver (PolygonBC + PolygonIndex), applied by
+// orc_swe_apply_boundaries and by the coupled step after the SW advance. An entry lives as long
+// as its solver: orc_swe_destroy removes it, so a later solver at the same address starts with
+// no regions.
+static std::map<const void*, PolygonBc>& bc_regions() {
+  static std::map<const void*, PolygonBc> m;
+  return m;
+}
+
 void* orc_swe_create(int nx, int ny, double dx, double dy, const frehg_swe_params* p) {
   try {
     return new SweSolver(nx, ny, dx, dy, toSwe(*p));
@@ -244,7 +253,10 @@ void* orc_swe_create(int nx, int ny, double dx, double dy, const frehg_swe_param
     return nullptr;
   }
 }
-void orc_swe_destroy(void* h) { delete static_cast<SweSolver*>(h); }
+void orc_swe_destroy(void* h) {
+  bc_regions().erase(h);
+  delete static_cast<SweSolver*>(h);
+}
 int orc_swe_set_bathymetry(void* h, const double* bed) {
   return guard([&] {
     auto* s = static_cast<SweSolver*>(h);
@@ -299,13 +311,7 @@ int orc_couple_exchange(void* swe, void* re, const frehg_coupling_params* cp, do
   });
 }
 
-// state[4] = {t, gw_time, exchange_volume, gw_substeps}
-// Surface boundary regions per oracle SweSolver (PolygonBC + PolygonIndex), applied by
-// orc_swe_apply_boundaries and by the coupled step after the SW advance.
-static std::map<const void*, PolygonBc>& bc_regions() {
-  static std::map<const void*, PolygonBc> m;
-  return m;
-}
+// state[5] = {t, gw_time, exchange_volume, gw_substeps, boundary outflow}
 int orc_swe_set_boundary_polygons(void* h, int nreg, const int32_t* kind, const double* rate, const int32_t* nvert,
                                   const double* xy, double x0, double y0) {
   return guard([&] {

@@ -100,7 +100,7 @@ real ReSolver::soilKs(int i, int j, int k, int dir) const {
 }
 
 void ReSolver::updateDiagnostics() {  // :199-215
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if(omp_wide((long long)g.nx * g.ny * g.nz))
   for (int k = 0; k < g.nz; ++k)
     for (int j = 0; j < g.ny; ++j)
       for (int i = 0; i < g.nx; ++i) {
@@ -117,7 +117,7 @@ void ReSolver::computeKFace() {  // :217-380
   const long long sy = nx + 2, sz = g.plane();
   const int bc2 = p_.bc_type[2];
   const bool full3d = p_.use_full3d != 0;
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if(omp_wide((long long)nx * ny * nz))
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -216,7 +216,7 @@ void ReSolver::matCoeff() {  // :397-470
   const real dx = p_.dx, dy = p_.dy, dtg = p_.dt;
   const long long sy = nx + 2, sz = g.plane();
   const int bc2 = p_.bc_type[2], bc3 = p_.bc_type[3], bc4 = p_.bc_type[4], bc5 = p_.bc_type[5];
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if(omp_wide((long long)nx * ny * nz))
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -260,7 +260,7 @@ void ReSolver::rhs() {  // :472-549 (serial: global == local)
   const long long sy = nx + 2, sz = g.plane();
   const int bc4 = p_.bc_type[4], bc5 = p_.bc_type[5];
   const bool sim_sw = p_.sim_shallowwater != 0;
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if(omp_wide((long long)nx * ny * nz))
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -341,7 +341,7 @@ void ReSolver::groundwaterFlux() {  // :671-743 with darcyFluxZ :24-58
   const long long sy = nx + 2, sz = g.plane();
   const real faceA = dx * dy;
   const int bc4 = p_.bc_type[4], bc5 = p_.bc_type[5];
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if(omp_wide((long long)nx * ny * nz))
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -393,7 +393,7 @@ void ReSolver::updateWaterContent() {  // :762-786
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const real dtg = p_.dt;
   const long long sy = nx + 2, sz = g.plane();
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if(omp_wide((long long)nx * ny * nz))
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -413,7 +413,7 @@ void ReSolver::reallocateWaterContent() {  // :788-837
   const long long sy = nx + 2, sz = g.plane();
   // wch/hwc of a cell depend only on its own h and its own pre-visit wc, so they are evaluated
   // in parallel first; the order-dependent part below is the reference's sequential sweep.
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if(omp_wide((long long)nx * ny * nz))
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -452,7 +452,7 @@ void ReSolver::reallocateWaterContent() {  // :788-837
 }
 
 void ReSolver::clampWaterContent() {  // :839-853
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if(omp_wide((long long)g.nx * g.ny * g.nz))
   for (int k = 0; k < g.nz; ++k)
     for (int j = 0; j < g.ny; ++j)
       for (int i = 0; i < g.nx; ++i) {
@@ -488,7 +488,7 @@ void ReSolver::adaptiveTimeStep() {  // :880-929 (max/min are exact, so order-fr
   real dq_max = 0.0, dt_Comin = 1.0e8;
   p_.dt = dtn;
   const real dtv = p_.dt;
-#pragma omp parallel for collapse(2) schedule(static) reduction(max : dq_max) reduction(min : dt_Comin)
+#pragma omp parallel for collapse(2) schedule(static) reduction(max : dq_max) reduction(min : dt_Comin) if(omp_wide((long long)nx * ny * nz))
   for (int k = 1; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -518,7 +518,7 @@ static bool pmax_is_one(const ReParams& p) { return p.picard_max <= 1; }
 void ReSolver::advanceStep(ReStepStats* st) {  // :931-969
   dt_used = p_.dt;
   const size_t n = h.size();
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(omp_wide((long long)n))
   for (long long c = 0; c < (long long)n; ++c) {  // savePreviousStepFields :190-197
     hn[c] = h[c];
     wcn[c] = wc[c];
@@ -561,7 +561,7 @@ void ReSolver::advanceStep(ReStepStats* st) {  // :931-969
   if (p_.use_corrector) {
     updateWaterContent();
   } else {
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if(omp_wide((long long)g.nx * g.ny * g.nz))
     for (int k = 0; k < g.nz; ++k)
       for (int j = 0; j < g.ny; ++j)
         for (int i = 0; i < g.nx; ++i) wc[g.idx(i, j, k)] = VG::theta(soilAt(i, j, k), h[g.idx(i, j, k)]);

@@ -103,3 +103,20 @@ def test_outflow_volume_accounting_on_a_flowing_surface():
         d = interior2(s.get("dept"), nx, ny)
         np.testing.assert_allclose(d, np.maximum(interior2(s.get("eta") - s.get("bottom"), nx, ny), 0), atol=1e-15)
     assert total > 0.05 * 2e-4 * 40 * nx * (ny - 1)
+
+
+def test_regions_end_with_their_solver():
+    """A solver's regions are dropped when it is destroyed: fresh solvers (which the allocator
+    often places at a freed solver's address) start without regions, so the coupled step and
+    apply_boundaries leave them unchanged."""
+    box = [(-1.0, -1.0), (3.0, -1.0), (3.0, 9.0), (-1.0, 9.0)]
+    for _ in range(8):
+        s = basin()
+        s.set_boundary_polygons([(abi.BC_DEPTH, 0.05, box)])
+        assert s.apply_boundaries(0.5) != 0.0
+        del s
+        fresh = basin()
+        eta = fresh.get("eta").copy()
+        assert fresh.apply_boundaries(0.5) == 0.0
+        np.testing.assert_array_equal(fresh.get("eta"), eta)
+        del fresh
