@@ -519,6 +519,7 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const flo
   const int iblocks = (g3.nx + RB - 1) / RB;
   const long long nchunks = (long long)iblocks * g3.ny * g3.nz;
   float(*xw)[FP] = xs[w];
+  unsigned long long inferred = 0;  // cells of the chunks this wave ran the network on
   for (long long ch = (long long)blockIdx.x * GW + w; ch < nchunks; ch += (long long)gridDim.x * GW) {
     const int row = (int)(ch / iblocks), ib = (int)(ch - (long long)row * iblocks);
     const int j = row % g3.ny, k = row / g3.ny;
@@ -539,7 +540,7 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const flo
       if (i < g3.nx) h_out[c] = base;
       continue;
     }
-    if (lane == 0 && F.inferred) atomicAdd(F.inferred, (unsigned long long)min(RB, g3.nx - ib * RB));
+    inferred += (unsigned long long)min(RB, g3.nx - ib * RB);  // per wave; one atomic per block below
     if (i < g3.nx) {
       cell_features<VG_ONLY>(F, i, j, k, f);
       // standardised features, clamped far outside any training distribution so that the FP16
@@ -599,6 +600,16 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const flo
       const bool ok = isfinite(y);
       if (!ok && F.bad) *F.bad = 1.0;
       h_out[c] = base + ((need && ok) ? y : 0.0);
+    }
+  }
+  if (F.inferred) {
+    __shared__ unsigned long long nw[GW];
+    if (lane == 0) nw[w] = inferred;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = 0;
+      for (int q = 0; q < GW; ++q) t += nw[q];
+      if (t) atomicAdd(F.inferred, t);
     }
   }
 }
