@@ -8,7 +8,7 @@ reference's parameter structs: SoilParams (include/frehg2/re/SoilParams.hpp:13-2
 """
 import ctypes as C
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 OK = 0
 INFO_GUESS_FALLBACK = 1
@@ -113,6 +113,26 @@ class CoupledStats(C.Structure):
     _fields_ = [("sw", SolveStats), ("gw", ReStats), ("gw_substeps", C.c_int32),
                 ("exchange_volume", C.c_double), ("t", C.c_double), ("gw_time", C.c_double),
                 ("boundary_outflow", C.c_double)]
+
+
+class RunSetup(C.Structure):
+    """frehg_run_setup: a configuration as the front end maps it."""
+    _fields_ = [("surface_water", C.c_int32), ("groundwater", C.c_int32), ("solute", C.c_int32),
+                ("coupled", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("botz", C.c_double),
+                ("dt", C.c_double), ("t_end", C.c_double), ("output_interval", C.c_double),
+                ("max_steps", C.c_int64), ("evaporation_rate", C.c_double), ("rain_from_file", C.c_int32),
+                ("rain_samples", C.c_int32), ("rain_rate", C.c_double), ("swe", SweParams), ("re", ReParams),
+                ("coupling", CouplingParams), ("soil_classes", C.c_int32), ("soil_map", C.c_int32),
+                ("recharge", C.c_int32), ("boundary_regions", C.c_int32), ("gw_initial_head", C.c_int32)]
+
+
+class RunStats(C.Structure):
+    """frehg_run_stats: the outcome of frehg_run_config."""
+    _fields_ = [("t", C.c_double), ("steps", C.c_int64), ("gw_substeps", C.c_int64), ("outputs", C.c_int32),
+                ("infos", C.c_int32), ("water_initial", C.c_double), ("water_final", C.c_double),
+                ("rain_volume", C.c_double), ("boundary_outflow", C.c_double), ("exchange_volume", C.c_double),
+                ("wall_seconds", C.c_double)]
 
 
 def soil_params(**kw):

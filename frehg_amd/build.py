@@ -1,7 +1,8 @@
 """Builds libfrehg_gpu.so (the gfx950 backend) in-tree with hipcc.
 
 Usage: python -m frehg_amd.build  (or frehg_amd.build.build()).
-Objects go to frehg_amd/_build/; the shared library to frehg_amd/libfrehg_gpu.so.
+Objects go to frehg_amd/_build/; the shared library to frehg_amd/libfrehg_gpu.so and the command
+line to frehg_amd/frehg_run.
 """
 import concurrent.futures as cf
 import os
@@ -21,6 +22,12 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
          "-I", os.path.join(HERE, "..", "include")]
 
 SOURCES = ["krylov.hip", "re.hip", "swe.hip", "coupling.hip", "mlp.hip", "ensemble.hip", "noalloc.hip", "api.hip"]
+# Host-only C++ above the C ABI (configuration front end), built with g++.
+HOST_SOURCES = ["yaml.cpp", "config.cpp", "driver.cpp"]
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-I", os.path.join(HERE, "..", "include")]
+# The command line, linked against the library (found next to it at run time).
+CLI = os.path.join(HERE, "frehg_run")
 
 # Per-file flags. The MLP kernels keep their MFMA accumulators in ordinary VGPRs (the guess kernel
 # otherwise reads every result back from accumulation registers: 36 extra moves per 16-cell tile).
@@ -36,11 +43,21 @@ def _compile(src):
     return obj, r.stderr
 
 
+def _compile_host(src):
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    cmd = [CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{CXX} failed for {src}:\n{r.stdout}\n{r.stderr}")
+    return obj, r.stderr
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        results = list(ex.map(_compile, srcs))
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        jobs = [ex.submit(_compile, s) for s in srcs] + [ex.submit(_compile_host, s) for s in HOST_SOURCES]
+        results = [j.result() for j in jobs]
     if verbose:
         for _, err in results:
             if err.strip():
@@ -52,6 +69,12 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    cmd = [CXX, *HOST_FLAGS, os.path.join(CSRC, "frehg_run.cpp"), "-o", CLI + ".tmp", "-L", HERE, "-lfrehg_gpu",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"frehg_run link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(CLI + ".tmp", CLI)
     return OUT
 
 

@@ -700,3 +700,12 @@ int frehg_mlp_predict(frehg_ctx_t ctx, const float* x, int n, float* out) {
 }
 
 }  // extern "C"
+
+// Used by the configuration driver (driver.cpp), which runs above the C ABI.
+namespace frehg {
+int set_ctx_error(frehg_ctx_t ctx, int rc, const char* msg) {
+  if (ctx) ctx->err = msg ? msg : "";
+  return rc;
+}
+bool ctx_has_solvers(frehg_ctx_t ctx) { return ctx && (ctx->re || ctx->swe || ctx->ens); }
+}  // namespace frehg

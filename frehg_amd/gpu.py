@@ -95,6 +95,11 @@ def load():
         "frehg_ls_solve_noalloc": (C.c_int, [ctx, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                              P(C.c_void_p), C.c_void_p, C.c_void_p, P(abi.SolverConfig),
                                              P(abi.SolveStats)]),
+        "frehg_config_setup": (C.c_int, [C.c_char_p, P(C.c_char_p), C.c_int, P(abi.RunSetup), C.c_char_p, C.c_int]),
+        "frehg_config_query": (C.c_int, [C.c_char_p, P(C.c_char_p), C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
+        "frehg_run_config": (C.c_int, [ctx, C.c_char_p, P(C.c_char_p), C.c_int, C.c_char_p, P(abi.RunStats)]),
+        "frehg_config_field": (C.c_int64, [C.c_char_p, P(C.c_char_p), C.c_int, C.c_char_p, dp, C.c_int64,
+                                           C.c_char_p, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -111,6 +116,47 @@ def _ptr(a):
         return None
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(dp)
+
+
+def _overrides(overrides):
+    items = [str(o).encode() for o in (overrides or ())]
+    return (C.c_char_p * max(1, len(items)))(*items), len(items)
+
+
+def config_setup(path, overrides=()):
+    """frehg_config_setup: load, gate and map a v2 YAML configuration (no GPU). Raises
+    FrehgError with the front end's message when the configuration is refused."""
+    arr, n = _overrides(overrides)
+    out = abi.RunSetup()
+    err = C.create_string_buffer(4096)
+    rc = load().frehg_config_setup(str(path).encode(), arr, n, C.byref(out), err, len(err))
+    if rc != 0:
+        raise FrehgError(rc, err.value.decode())
+    return out
+
+
+def config_field(path, name, overrides=()):
+    """frehg_config_field: a resolved per-cell input of a configuration as a float64 array."""
+    arr, n = _overrides(overrides)
+    err = C.create_string_buffer(4096)
+    cnt = load().frehg_config_field(str(path).encode(), arr, n, name.encode(), None, 0, err, len(err))
+    if cnt < 0:
+        raise FrehgError(int(cnt), err.value.decode())
+    out = np.zeros(int(cnt))
+    if cnt:
+        load().frehg_config_field(str(path).encode(), arr, n, name.encode(), out.ctypes.data_as(dp), cnt, err, len(err))
+    return out
+
+
+def config_query(path, key, overrides=()):
+    """frehg_config_query: (kind, text) of a dotted key; kind 0 missing/null, 1 scalar,
+    2 sequence, 3 mapping (text = size)."""
+    arr, n = _overrides(overrides)
+    buf = C.create_string_buffer(4096)
+    rc = load().frehg_config_query(str(path).encode(), arr, n, key.encode(), buf, len(buf))
+    if rc < 0:
+        raise FrehgError(rc, buf.value.decode())
+    return rc, buf.value.decode()
 
 
 class Context:
@@ -285,6 +331,15 @@ class Context:
         self.check(self.lib.frehg_ls_solve_noalloc(self.h, ndim, nx, ny, nz, C.c_void_p(int(diag_dev)), arr,
                                                    C.c_void_p(int(b_dev)), C.c_void_p(int(x_dev)), C.byref(cfg),
                                                    C.byref(st)))
+        return st
+
+    # ---- configuration front end -------------------------------------------------------------------
+    def run_config(self, path, overrides=(), out_dir=None):
+        """frehg_run_config: run a v2 YAML configuration on this (empty) context."""
+        arr, n = _overrides(overrides)
+        st = abi.RunStats()
+        self.check(self.lib.frehg_run_config(self.h, str(path).encode(), arr, n,
+                                             None if out_dir is None else str(out_dir).encode(), C.byref(st)))
         return st
 
     # ---- measurement -----------------------------------------------------------------------------------

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define FREHG_ABI_VERSION 4
+#define FREHG_ABI_VERSION 5
 
 enum {
   FREHG_OK = 0,
@@ -344,6 +344,75 @@ int frehg_profile_reset(frehg_ctx_t ctx);
  * at the first solve, or pinned by the environment variable FREHG_SPMV_SHAPE="cpl:kchunk" read at
  * frehg_re_init. No counterpart in the reference (PETSc's MatMult has no launch shape). */
 int frehg_spmv_shape(frehg_ctx_t ctx, int precision, int* cells_per_lane, int* kchunk);
+
+/* ---- Configuration front end: a Frehg2 v2 YAML file run through this library. Replaces
+ * Config::fromFile (src/io/Config.cpp:51-64) + Orchestrator::initialize / run
+ * (src/driver/Orchestrator.cpp:192-304, 1595-1615) behind the frehg2 command line
+ * (src/driver/main.cpp). The same keys, defaults and gates as the reference (schema v2, the
+ * capability gate); options this backend does not implement are refused with
+ * FREHG_ERR_INVALID_ARG and a message naming the key: solute transport, polygon sources, probe
+ * monitors, checkpoints and restarts, formula / polygon initial conditions, HDF5 inputs, wind,
+ * subgrid, diffusive wave. Added keys (all optional, defaults = the reference's behaviour):
+ * groundwater.solver.precision (fp64 | mixed), .inner_rtol, .max_refine; groundwater.picard.
+ * max_iterations (1 = the PCA predictor) and .tolerance; groundwater.initial_guess (zero |
+ * previous | extrapolated). `overrides` are "dotted.key=value" strings (value in YAML syntax)
+ * applied after loading, like the reference's own tests editing the loaded tree. */
+typedef struct {
+  int32_t surface_water, groundwater, solute, coupled;
+  int32_t nx, ny, nz;
+  double dx, dy, dz, botz;
+  double dt, t_end, output_interval;
+  int64_t max_steps;
+  double evaporation_rate;
+  int32_t rain_from_file; /* 1: rain_samples (time, rate) pairs interpolated in time */
+  int32_t rain_samples;
+  double rain_rate;       /* constant rate when rain_from_file = 0 [m/s] */
+  frehg_swe_params swe;   /* offset = -min(bathymetry) */
+  frehg_re_params re;     /* soil = the first soil class */
+  frehg_coupling_params coupling;
+  int32_t soil_classes;
+  int32_t soil_map;       /* 0: uniform, 2: per column, 3: per cell (soil.map.layers) */
+  int32_t recharge;       /* 1: groundwater.recharge sets a top flux field */
+  int32_t boundary_regions;
+  int32_t gw_initial_head; /* 1: the groundwater starts from heads, 0: from water contents */
+} frehg_run_setup;
+
+typedef struct {
+  double t;                /* simulated time reached [s] */
+  int64_t steps;           /* time steps (the groundwater's own steps in a groundwater-only run) */
+  int64_t gw_substeps;
+  int32_t outputs;         /* field snapshots written (the initial one included) */
+  int32_t infos;           /* steps that returned an FREHG_INFO_* code */
+  double water_initial, water_final; /* surface + subsurface water volume [m^3] */
+  double rain_volume;      /* rain added (the surface step skips the last y row, as the reference) */
+  double boundary_outflow; /* net volume the surface boundary regions removed */
+  double exchange_volume;  /* net GW -> SW seepage volume */
+  double wall_seconds;
+} frehg_run_stats;
+
+/* Loads, gates and maps a configuration without a GPU (no context). err receives the message of
+ * a failure (err_len bytes, NUL-terminated). */
+int frehg_config_setup(const char* yaml_path, const char* const* overrides, int n_overrides, frehg_run_setup* out,
+                       char* err, int err_len);
+/* Value at a dotted key of any YAML file (after overrides; no schema check). Returns 0 when the key is missing or
+ * null, 1 for a scalar (its text in value), 2 for a sequence and 3 for a mapping (their sizes in
+ * value), or FREHG_ERR_INVALID_ARG with the parse error in value. */
+int frehg_config_query(const char* yaml_path, const char* const* overrides, int n_overrides, const char* key,
+                       char* value, int value_len);
+/* A resolved per-cell input of a configuration (no GPU): "bathymetry" (nx*ny, plus the datum
+ * offset), "eta0" (nx*ny, physical), "gw_initial" (nx*ny*nz heads or water contents),
+ * "top_flux" (nx*ny), "soil_class" (nx*ny or nx*ny*nz), "rain_t" / "rain_v" (the rainfall
+ * series); x fastest, then y, then z. Copies at most n values into out and returns the count the
+ * field holds (0 when the configuration has none), or FREHG_ERR_INVALID_ARG with the message in err. */
+int64_t frehg_config_field(const char* yaml_path, const char* const* overrides, int n_overrides, const char* name,
+                           double* out, int64_t n, char* err, int err_len);
+/* Runs the configuration on ctx (which must hold no solver yet): set-up, the time loop of
+ * Orchestrator::runLoopSequential (:1518-1548; async coupling runs the same sequence
+ * synchronously), field snapshots and simulation_summary.txt. out_dir (NULL: the directory of
+ * output.filename) receives <variable>_<time>.txt files (HDF5 is not available here) with one
+ * value per interior cell, x fastest then y then z. */
+int frehg_run_config(frehg_ctx_t ctx, const char* yaml_path, const char* const* overrides, int n_overrides,
+                     const char* out_dir, frehg_run_stats* stats);
 
 #ifdef __cplusplus
 }
