@@ -124,7 +124,15 @@ class RunSetup(C.Structure):
                 ("max_steps", C.c_int64), ("evaporation_rate", C.c_double), ("rain_from_file", C.c_int32),
                 ("rain_samples", C.c_int32), ("rain_rate", C.c_double), ("swe", SweParams), ("re", ReParams),
                 ("coupling", CouplingParams), ("soil_classes", C.c_int32), ("soil_map", C.c_int32),
-                ("recharge", C.c_int32), ("boundary_regions", C.c_int32), ("gw_initial_head", C.c_int32)]
+                ("recharge", C.c_int32), ("boundary_regions", C.c_int32), ("gw_initial_head", C.c_int32),
+                ("restart", C.c_int32), ("dt_checkpoint", C.c_double), ("max_checkpoints", C.c_int32)]
+
+
+class CheckpointInfo(C.Structure):
+    """frehg_checkpoint_info: what frehg_checkpoint_load restored."""
+    _fields_ = [("t", C.c_double), ("gw_time", C.c_double), ("gw_dt", C.c_double), ("has_sw", C.c_int32),
+                ("has_gw", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("params_match", C.c_int32)]
 
 
 class RunStats(C.Structure):

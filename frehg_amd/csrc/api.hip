@@ -1,6 +1,7 @@
 // C ABI of the gfx950 backend (include/frehg_gpu.h). Every entry point maps exceptions to the
 // return codes of the header; the reference raises std::runtime_error for the same conditions.
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -695,6 +696,184 @@ int frehg_mlp_increment(frehg_ctx_t ctx, double* dst) {
 int frehg_mlp_predict(frehg_ctx_t ctx, const float* x, int n, float* out) {
   return guard(ctx, [&] {
     need_mlp(ctx).predict_batch(x, n, out);
+    return FREHG_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---- checkpoint / restart ------------------------------------------------------------------
+namespace {
+
+const char* const kSweStateNames[] = {"eta", "etan", "bottom", "bottomXP", "bottomYP", "dept", "deptx", "depty",
+                                      "uu", "vv", "un", "vn", "uy", "vx", "Vs", "Vsn", "Asx", "Asy", "Asz",
+                                      "Aszx", "Aszy", "Vsx", "Vsy", "CDx", "CDy", "Dx", "Dy", "Ex", "Ey",
+                                      "cflx", "cfly", "Fu", "Fv", "Sxp", "Sxm", "Syp", "Sym", "Sct", "Srhs",
+                                      "cfl_active", "wtfx", "wtfy"};
+const char* const kReStateNames[] = {"h", "hn", "wc", "wcn", "wch", "hwc", "ch", "Kx", "Ky", "Kz",
+                                     "qx", "qy", "qz", "Gct", "Grhs"};
+constexpr char kCkptMagic[8] = {'F', 'R', 'H', 'G', 'C', 'K', 'P', '1'};
+constexpr int32_t kCkptVersion = 1;
+
+uint64_t fnv1a(const void* p, size_t n, uint64_t h) {
+  const unsigned char* b = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  return h;
+}
+
+uint64_t params_digest(frehg_ctx_t ctx) {
+  uint64_t h = 1469598103934665603ull;
+  if (ctx->re) {
+    frehg_re_params p = ctx->re->params();
+    p.dt = 0.0;  // the adaptive dt is state, not a parameter
+    h = fnv1a(&p, sizeof p, h);
+  }
+  if (ctx->swe) h = fnv1a(&ctx->swe->params(), sizeof(frehg_swe_params), h);
+  return h;
+}
+
+struct File {
+  FILE* f;
+  explicit File(FILE* ff) : f(ff) {}
+  ~File() {
+    if (f) std::fclose(f);
+  }
+};
+
+void put(FILE* f, const void* p, size_t n, const std::string& path) {
+  if (n && std::fwrite(p, 1, n, f) != n) throw frehg::ArgError("checkpoint: write failed on '" + path + "'");
+}
+void get(FILE* f, void* p, size_t n, const std::string& path) {
+  if (n && std::fread(p, 1, n, f) != n) throw frehg::ArgError("checkpoint: '" + path + "' is truncated");
+}
+
+}  // namespace
+
+extern "C" {
+
+int frehg_checkpoint_save(frehg_ctx_t ctx, const char* path) {
+  return guard(ctx, [&] {
+    if (!path || !*path) throw frehg::ArgError("frehg_checkpoint_save: empty path");
+    if (!ctx->re && !ctx->swe) throw frehg::ArgError("frehg_checkpoint_save: no solver to save");
+    const std::string final_path = path, tmp = final_path + ".tmp";
+    {
+      File f(std::fopen(tmp.c_str(), "wb"));
+      if (!f.f) throw frehg::ArgError("frehg_checkpoint_save: cannot create '" + tmp + "'");
+      const frehg::Geo& g = ctx->re ? ctx->re->geo() : ctx->swe->geo();
+      const int32_t head[6] = {kCkptVersion, ctx->swe ? 1 : 0, ctx->re ? 1 : 0, g.nx, g.ny, ctx->re ? g.nz : 1};
+      const double times[3] = {ctx->t, ctx->gw_time, ctx->re ? ctx->re->dt() : 0.0};
+      const uint64_t digest = params_digest(ctx);
+      put(f.f, kCkptMagic, sizeof kCkptMagic, tmp);
+      put(f.f, head, sizeof head, tmp);
+      put(f.f, times, sizeof times, tmp);
+      put(f.f, &digest, sizeof digest, tmp);
+      const int32_t nfields = (ctx->swe ? (int32_t)(sizeof kSweStateNames / sizeof(char*)) : 0) +
+                              (ctx->re ? (int32_t)(sizeof kReStateNames / sizeof(char*)) : 0);
+      put(f.f, &nfields, sizeof nfields, tmp);
+      auto field = [&](const char* prefix, const char* name, size_t count, auto&& read) {
+        const std::string key = std::string(prefix) + name;
+        const int32_t len = (int32_t)key.size();
+        const int64_t cnt = (int64_t)count;
+        std::vector<double> buf(count);
+        read(name, buf.data());
+        put(f.f, &len, sizeof len, tmp);
+        put(f.f, key.data(), key.size(), tmp);
+        put(f.f, &cnt, sizeof cnt, tmp);
+        put(f.f, buf.data(), count * sizeof(double), tmp);
+      };
+      const size_t n2 = (size_t)(g.nx + 2) * (g.ny + 2);
+      if (ctx->swe)
+        for (const char* nm : kSweStateNames)
+          field("sw_", nm, n2, [&](const char* n, double* d) { ctx->swe->get_field(n, d); });
+      if (ctx->re)
+        for (const char* nm : kReStateNames)
+          field("gw_", nm, n2 * (size_t)(g.nz + 2), [&](const char* n, double* d) { ctx->re->get_field(n, d); });
+      if (std::fflush(f.f) != 0) throw frehg::ArgError("frehg_checkpoint_save: write failed on '" + tmp + "'");
+    }
+    if (std::rename(tmp.c_str(), final_path.c_str()) != 0)
+      throw frehg::ArgError("frehg_checkpoint_save: cannot rename '" + tmp + "' to '" + final_path + "'");
+    return FREHG_OK;
+  });
+}
+
+int frehg_checkpoint_load(frehg_ctx_t ctx, const char* path, frehg_checkpoint_info* info) {
+  return guard(ctx, [&] {
+    if (!path || !*path) throw frehg::ArgError("frehg_checkpoint_load: empty path");
+    if (!ctx->re && !ctx->swe) throw frehg::ArgError("frehg_checkpoint_load: set up the solvers first");
+    const std::string p = path;
+    File f(std::fopen(path, "rb"));
+    if (!f.f) throw frehg::ArgError("frehg_checkpoint_load: cannot open '" + p + "'");
+    char magic[8];
+    int32_t head[6];
+    double times[3];
+    uint64_t digest = 0;
+    int32_t nfields = 0;
+    get(f.f, magic, sizeof magic, p);
+    if (std::memcmp(magic, kCkptMagic, sizeof magic) != 0)
+      throw frehg::ArgError("frehg_checkpoint_load: '" + p + "' is not a checkpoint of this library");
+    get(f.f, head, sizeof head, p);
+    if (head[0] != kCkptVersion) throw frehg::ArgError("frehg_checkpoint_load: unsupported checkpoint version");
+    get(f.f, times, sizeof times, p);
+    get(f.f, &digest, sizeof digest, p);
+    get(f.f, &nfields, sizeof nfields, p);
+    if ((head[1] != 0) != (ctx->swe != nullptr))
+      throw frehg::ArgError(head[1] ? "frehg_checkpoint_load: the checkpoint has surface-water state, the context none"
+                                    : "frehg_checkpoint_load: checkpoint has no SW state");
+    if ((head[2] != 0) != (ctx->re != nullptr))
+      throw frehg::ArgError(head[2] ? "frehg_checkpoint_load: the checkpoint has groundwater state, the context none"
+                                    : "frehg_checkpoint_load: checkpoint has no GW state");
+    const frehg::Geo& g = ctx->re ? ctx->re->geo() : ctx->swe->geo();
+    if (head[3] != g.nx || head[4] != g.ny || head[5] != (ctx->re ? g.nz : 1))
+      throw frehg::ArgError("frehg_checkpoint_load: grid differs from the checkpoint's (" + std::to_string(head[3]) + "x" +
+                            std::to_string(head[4]) + "x" + std::to_string(head[5]) + ")");
+    // read everything first: a bad file leaves the context unchanged
+    std::vector<std::pair<std::string, std::vector<double>>> fields;
+    for (int32_t q = 0; q < nfields; ++q) {
+      int32_t len = 0;
+      int64_t cnt = 0;
+      get(f.f, &len, sizeof len, p);
+      if (len <= 0 || len > 64) throw frehg::ArgError("frehg_checkpoint_load: '" + p + "' is corrupt");
+      std::string key((size_t)len, '\0');
+      get(f.f, &key[0], (size_t)len, p);
+      get(f.f, &cnt, sizeof cnt, p);
+      const size_t n2 = (size_t)(g.nx + 2) * (g.ny + 2);
+      const size_t want = key.rfind("gw_", 0) == 0 ? n2 * (size_t)(g.nz + 2) : n2;
+      if (cnt != (int64_t)want) throw frehg::ArgError("frehg_checkpoint_load: field '" + key + "' size mismatch");
+      std::vector<double> buf((size_t)cnt);
+      get(f.f, buf.data(), buf.size() * sizeof(double), p);
+      fields.emplace_back(std::move(key), std::move(buf));
+    }
+    auto has = [&](const std::string& k) {
+      for (auto& kv : fields)
+        if (kv.first == k) return true;
+      return false;
+    };
+    if (ctx->swe)
+      for (const char* nm : kSweStateNames)
+        if (!has(std::string("sw_") + nm))
+          throw frehg::ArgError(std::string("frehg_checkpoint_load: missing field 'sw_") + nm + "'");
+    if (ctx->re)
+      for (const char* nm : kReStateNames)
+        if (!has(std::string("gw_") + nm))
+          throw frehg::ArgError(std::string("frehg_checkpoint_load: missing field 'gw_") + nm + "'");
+    for (auto& kv : fields) {
+      if (kv.first.rfind("sw_", 0) == 0) ctx->swe->set_field(kv.first.substr(3), kv.second.data());
+      else ctx->re->set_field(kv.first.substr(3), kv.second.data());
+    }
+    ctx->t = times[0];
+    ctx->gw_time = times[1];
+    if (ctx->re) ctx->re->prime_dt(times[2]);
+    if (info) {
+      info->t = times[0];
+      info->gw_time = times[1];
+      info->gw_dt = times[2];
+      info->has_sw = head[1];
+      info->has_gw = head[2];
+      info->nx = head[3];
+      info->ny = head[4];
+      info->nz = head[5];
+      info->params_match = digest == params_digest(ctx) ? 1 : 0;
+    }
     return FREHG_OK;
   });
 }

@@ -348,15 +348,9 @@ void gate_capabilities(const Config& c, bool sw, bool gw, bool solute) {
   if (srcs && srcs->is_seq() && srcs->size() > 0)
     reject(src, "sources", "polygon sources (inflow_rate, rainfall_rate, extraction_well) are not implemented by this backend");
   if (c.has("monitors")) reject(src, "monitors", "probe and line monitors are not implemented by this backend");
-  if (c.get_double("time.dt_checkpoint", 0.0) > 0.0)
-    reject(src, "time.dt_checkpoint", "checkpoint files are not implemented by this backend");
   const yaml::Node* ic = c.find("initial_conditions");
-  if (ic && ic->is_map()) {
-    const yaml::Node* t = ic->get("type");
-    if (t && t->is_scalar() && t->text == "restart")
-      reject(src, "initial_conditions.type", "= restart: restart files are not implemented by this backend");
-    if (ic->get("regions")) reject(src, "initial_conditions.regions", "polygon initial conditions are not implemented by this backend");
-  }
+  if (ic && ic->is_map() && ic->get("regions"))
+    reject(src, "initial_conditions.regions", "polygon initial conditions are not implemented by this backend");
   if (c.get_string("output.io_mode", "serial_gather") != "serial_gather")
     reject(src, "output.io_mode", "only serial_gather output is implemented by this backend");
 }
@@ -890,8 +884,24 @@ RunSetup build_setup(const Config& c) {
     const std::string d = dir_of(resolved);
     s.out_dir = d.empty() ? "." : d;
     s.write_fields = true;
+    const size_t slash = resolved.find_last_of('/'), dot = resolved.find_last_of('.');
+    s.stem = (dot != std::string::npos && (slash == std::string::npos || dot > slash)) ? resolved.substr(0, dot) : resolved;
   }
   s.summary_path = s.out_dir + "/simulation_summary.txt";
+
+  // checkpoints (Orchestrator.cpp:253-254, 1461-1502) and restart (ICConfig.cpp:66-74)
+  s.dt_checkpoint = c.get_double("time.dt_checkpoint", 0.0);
+  s.max_checkpoints = std::max(1, c.get_int("time.max_checkpoints", 2));
+  const yaml::Node* icr = c.find("initial_conditions");
+  const yaml::Node* ict = (icr && icr->is_map()) ? icr->get("type") : nullptr;
+  if (ict && ict->is_scalar() && ict->text == "restart") {
+    s.restart = true;
+    s.restart_file = c.get_string("initial_conditions.file", "");
+    if (s.restart_file.empty()) throw ConfigError("ICConfig: initial_conditions.type=restart requires file");
+    s.restart_file = c.resolve_path(s.restart_file);
+    s.restart_time_given = c.has("initial_conditions.time");
+    s.restart_time = c.get_double("initial_conditions.time", 0.0);
+  }
   return s;
 }
 

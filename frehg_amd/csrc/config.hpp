@@ -87,7 +87,14 @@ struct RunSetup {
   std::vector<double> gw_init;  // initial head or water content per interior cell
   std::vector<std::string> sw_out, gw_out;  // "water_depth", "eta", "velocity_u", "velocity_v"; "head", "moisture", "qx"...
   std::string out_dir = ".", summary_path = "./simulation_summary.txt";
+  std::string stem;  // output.filename without its extension ("" without a filename)
   bool write_fields = false;
+  double dt_checkpoint = 0.0;  // time.dt_checkpoint (0: none)
+  int max_checkpoints = 2;     // time.max_checkpoints: the newest files kept
+  bool restart = false;        // initial_conditions.type = restart
+  std::string restart_file;
+  bool restart_time_given = false;
+  double restart_time = 0.0;
 };
 
 RunSetup build_setup(const Config& cfg);

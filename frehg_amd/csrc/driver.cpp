@@ -69,12 +69,13 @@ void make_dirs(const std::string& dir) {
   }
 }
 
+// timeKey (src/io/Hdf5Writer.cpp:25-33): an integer when t is within 1e-9 of one, else 12
+// significant digits.
 std::string time_tag(double t) {
+  const long long r = std::llround(t);
+  if (std::fabs(t - (double)r) < 1.0e-9) return std::to_string(r);
   char buf[64];
-  if (t == std::floor(t) && std::fabs(t) < 1e15)
-    std::snprintf(buf, sizeof buf, "%.0f", t);
-  else
-    std::snprintf(buf, sizeof buf, "%.10g", t);
+  std::snprintf(buf, sizeof buf, "%.12g", t);
   return buf;
 }
 
@@ -182,15 +183,31 @@ class Runner {
     const auto t0 = std::chrono::steady_clock::now();
     make_dirs(dir_);
     setup();
-    const double water0 = volume();
     const double eps = 1.0e-9 * std::max(1.0, s.t_end);
     auto next_after = [](double interval, double t) {
       if (interval <= 0.0) return std::numeric_limits<double>::max();
       return (std::floor(t / interval + 1.0e-9) + 1.0) * interval;
     };
-    double next_out = s.output_interval > 0.0 ? s.output_interval : s.t_end + 1.0;
-    write_outputs(0.0);
     double t = 0.0, rain_in = 0.0, outflow = 0.0, exch = 0.0;
+    double next_out = s.output_interval > 0.0 ? s.output_interval : s.t_end + 1.0;
+    double next_ckpt = s.dt_checkpoint > 0.0 ? s.dt_checkpoint : s.t_end + 1.0;
+    double water0 = 0.0;
+    if (s.restart) {
+      // loadCheckpointState (:688-747): the state, the clocks and the next groundwater dt; no
+      // snapshot of the starting state
+      frehg_checkpoint_info info;
+      check(ctx_, frehg_checkpoint_load(ctx_, s.restart_file.c_str(), &info), "frehg_checkpoint_load");
+      if (s.restart_time_given && std::fabs(s.restart_time - info.t) > 1.0e-9 * std::max(1.0, std::fabs(info.t)))
+        throw RunFailure{FREHG_ERR_INVALID_ARG, "initial_conditions.time (" + time_tag(s.restart_time) +
+                                                    ") differs from the checkpoint's time (" + time_tag(info.t) + ")"};
+      t = info.t;
+      next_out = next_after(s.output_interval, t);
+      next_ckpt = next_after(s.dt_checkpoint, t);
+      water0 = volume();
+    } else {
+      water0 = volume();
+      write_outputs(0.0);
+    }
     long long steps = 0, subs = 0;
     int infos = 0;
     const int rain_rows = s.ny - 1;  // the surface step adds rain to every row but the last (:1226-1233)
@@ -231,6 +248,10 @@ class Runner {
         write_outputs(t);
         next_out = next_after(s.output_interval, t);
       }
+      if (s.dt_checkpoint > 0.0 && t >= next_ckpt - eps) {
+        write_checkpoint(t);
+        next_ckpt = next_after(s.dt_checkpoint, t);
+      }
     }
     if (last_out_ < t - eps) write_outputs(t);
     const double water1 = volume();
@@ -249,6 +270,24 @@ class Runner {
     out.wall_seconds = wall;
     write_summary(out);
     if (st) *st = out;
+  }
+
+  // writeCheckpointNow + Hdf5Writer::writeCheckpoint (:1461-1502, Hdf5Writer.cpp:400-480):
+  // <stem>.ckpt.<time>.bin, the newest max_checkpoints kept. Like the reference, only a run that
+  // writes fields (output.filename set) writes checkpoints.
+  void write_checkpoint(double t) {
+    if (!s_.write_fields) return;
+    std::string stem = s_.stem;
+    const size_t slash = stem.find_last_of('/');
+    const std::string base = slash == std::string::npos ? stem : stem.substr(slash + 1);
+    stem = dir_ + "/" + (base.empty() ? s_.sim_id : base);
+    const std::string path = stem + ".ckpt." + time_tag(t) + ".bin";
+    check(ctx_, frehg_checkpoint_save(ctx_, path.c_str()), "frehg_checkpoint_save");
+    ckpts_.push_back(path);
+    while ((int)ckpts_.size() > s_.max_checkpoints) {
+      std::remove(ckpts_.front().c_str());
+      ckpts_.erase(ckpts_.begin());
+    }
   }
 
   void write_summary(const frehg_run_stats& r) {
@@ -286,6 +325,7 @@ class Runner {
   std::string dir_;
   int outputs_ = 0;
   double last_out_ = -1.0;
+  std::vector<std::string> ckpts_;
 };
 
 void fill_setup(const RunSetup& s, frehg_run_setup* o) {
@@ -317,6 +357,9 @@ void fill_setup(const RunSetup& s, frehg_run_setup* o) {
   o->recharge = s.top_flux.empty() ? 0 : 1;
   o->boundary_regions = (int32_t)s.regions.size();
   o->gw_initial_head = s.gw_init_head;
+  o->restart = s.restart;
+  o->dt_checkpoint = s.dt_checkpoint;
+  o->max_checkpoints = s.max_checkpoints;
 }
 
 }  // namespace

@@ -98,6 +98,8 @@ def load():
         "frehg_config_setup": (C.c_int, [C.c_char_p, P(C.c_char_p), C.c_int, P(abi.RunSetup), C.c_char_p, C.c_int]),
         "frehg_config_query": (C.c_int, [C.c_char_p, P(C.c_char_p), C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
         "frehg_run_config": (C.c_int, [ctx, C.c_char_p, P(C.c_char_p), C.c_int, C.c_char_p, P(abi.RunStats)]),
+        "frehg_checkpoint_save": (C.c_int, [ctx, C.c_char_p]),
+        "frehg_checkpoint_load": (C.c_int, [ctx, C.c_char_p, P(abi.CheckpointInfo)]),
         "frehg_config_field": (C.c_int64, [C.c_char_p, P(C.c_char_p), C.c_int, C.c_char_p, dp, C.c_int64,
                                            C.c_char_p, C.c_int]),
     }
@@ -332,6 +334,15 @@ class Context:
                                                    C.c_void_p(int(b_dev)), C.c_void_p(int(x_dev)), C.byref(cfg),
                                                    C.byref(st)))
         return st
+
+    # ---- checkpoint / restart --------------------------------------------------------------------------
+    def checkpoint_save(self, path):
+        self.check(self.lib.frehg_checkpoint_save(self.h, str(path).encode()))
+
+    def checkpoint_load(self, path):
+        info = abi.CheckpointInfo()
+        self.check(self.lib.frehg_checkpoint_load(self.h, str(path).encode(), C.byref(info)))
+        return info
 
     # ---- configuration front end -------------------------------------------------------------------
     def run_config(self, path, overrides=(), out_dir=None):

@@ -345,14 +345,36 @@ int frehg_profile_reset(frehg_ctx_t ctx);
  * frehg_re_init. No counterpart in the reference (PETSc's MatMult has no launch shape). */
 int frehg_spmv_shape(frehg_ctx_t ctx, int precision, int* cells_per_lane, int* kchunk);
 
+/* ---- Checkpoint and restart of the device state: Orchestrator::writeCheckpointNow /
+ * loadCheckpointState (src/driver/Orchestrator.cpp:1461-1502, 688-747), Hdf5Writer::
+ * writeCheckpoint (src/io/Hdf5Writer.cpp:400-470). HDF5 is not available here: the file is a
+ * flat binary (magic "FRHGCKP1", version, which solvers, grid, times, the next groundwater dt,
+ * a digest of the solver parameters, then every named field of each solver with its padded
+ * (nx+2)(ny+2)[(nz+2)] values), written to path + ".tmp" and renamed over path. A load needs a
+ * context set up with the same solvers and grid (else FREHG_ERR_INVALID_ARG); it restores the
+ * fields, the coupled-step clocks and the groundwater dt. With the zero or previous-step Krylov
+ * start and the same SpMV launch shape (FREHG_SPMV_SHAPE pins it) the continued run is
+ * bit-identical to an uninterrupted one; the extrapolated and learned starts begin again from
+ * h^n on the first step after a load. */
+typedef struct {
+  double t, gw_time;    /* coupled-step clocks at the save */
+  double gw_dt;         /* groundwater dt of the next step */
+  int32_t has_sw, has_gw, nx, ny, nz;
+  int32_t params_match; /* 1 if the context's solver parameters equal the saved ones */
+} frehg_checkpoint_info;
+
+int frehg_checkpoint_save(frehg_ctx_t ctx, const char* path);
+int frehg_checkpoint_load(frehg_ctx_t ctx, const char* path, frehg_checkpoint_info* info);
+
 /* ---- Configuration front end: a Frehg2 v2 YAML file run through this library. Replaces
  * Config::fromFile (src/io/Config.cpp:51-64) + Orchestrator::initialize / run
  * (src/driver/Orchestrator.cpp:192-304, 1595-1615) behind the frehg2 command line
  * (src/driver/main.cpp). The same keys, defaults and gates as the reference (schema v2, the
  * capability gate); options this backend does not implement are refused with
  * FREHG_ERR_INVALID_ARG and a message naming the key: solute transport, polygon sources, probe
- * monitors, checkpoints and restarts, formula / polygon initial conditions, HDF5 inputs, wind,
- * subgrid, diffusive wave. Added keys (all optional, defaults = the reference's behaviour):
+ * monitors, formula / polygon initial conditions, HDF5 inputs, wind, subgrid, diffusive wave.
+ * Checkpoints (time.dt_checkpoint, time.max_checkpoints) and restarts (initial_conditions.type:
+ * restart, file, time) use frehg_checkpoint_save / _load files. Added keys (all optional, defaults = the reference's behaviour):
  * groundwater.solver.precision (fp64 | mixed), .inner_rtol, .max_refine; groundwater.picard.
  * max_iterations (1 = the PCA predictor) and .tolerance; groundwater.initial_guess (zero |
  * previous | extrapolated). `overrides` are "dotted.key=value" strings (value in YAML syntax)
@@ -375,6 +397,9 @@ typedef struct {
   int32_t recharge;       /* 1: groundwater.recharge sets a top flux field */
   int32_t boundary_regions;
   int32_t gw_initial_head; /* 1: the groundwater starts from heads, 0: from water contents */
+  int32_t restart;         /* 1: initial_conditions.type = restart (state from a checkpoint file) */
+  double dt_checkpoint;    /* 0: no checkpoints */
+  int32_t max_checkpoints;
 } frehg_run_setup;
 
 typedef struct {

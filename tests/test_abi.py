@@ -26,7 +26,9 @@ def test_struct_layouts_match_the_c_header(tmp_path):
     structs = {"frehg_soil_params": abi.SoilParams, "frehg_solver_config": abi.SolverConfig,
                "frehg_re_params": abi.ReParams, "frehg_swe_params": abi.SweParams,
                "frehg_coupling_params": abi.CouplingParams, "frehg_solve_stats": abi.SolveStats,
-               "frehg_re_stats": abi.ReStats, "frehg_coupled_stats": abi.CoupledStats}
+               "frehg_re_stats": abi.ReStats, "frehg_coupled_stats": abi.CoupledStats,
+               "frehg_ens_stats": abi.EnsStats, "frehg_run_setup": abi.RunSetup, "frehg_run_stats": abi.RunStats,
+               "frehg_checkpoint_info": abi.CheckpointInfo}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "frehg_gpu.h"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

@@ -237,9 +237,10 @@ def test_gate_output_variables(tmp_path):
 @pytest.mark.parametrize("edit,key", [
     (lambda t: t.replace("solute: false", "solute: true"), "modules.solute"),
     (lambda t: t + "monitors: {probes: [{i: 0, j: 0}]}\n", "monitors"),
-    (lambda t: t.replace("output_interval: 0}", "output_interval: 0, dt_checkpoint: 5.0}"), "dt_checkpoint"),
     (lambda t: t.replace("{eta: 0.5}", "{eta: {type: formula, formula: 'x'}}"), "formula"),
-    (lambda t: t.replace("initial_conditions: {", "initial_conditions: {type: restart, file: a.h5, "), "restart"),
+    (lambda t: t.replace("{eta: 0.5}", "{eta: {type: polygon, default: 0}}"), "polygon"),
+    (lambda t: t.replace("initial_conditions: {", "initial_conditions: {regions: [], "), "regions"),
+    (lambda t: t.replace("{eta: 0.5}", "{eta: {type: raster, file: e.h5}}"), "HDF5"),
     (lambda t: t.replace("io_mode: serial_gather", "io_mode: parallel"), "io_mode"),
 ])
 def test_gate_refuses_what_this_backend_does_not_implement(tmp_path, edit, key):
@@ -337,3 +338,14 @@ def test_scalar_conversions(tmp_path):
         gpu.config_setup(p, ["groundwater.adaptive_dt=1"])
     with pytest.raises(gpu.FrehgError, match="domain.nx' must be an integer"):
         gpu.config_setup(p, ["domain.nx=one"])
+
+
+def test_checkpoint_and_restart_keys(tmp_path):
+    """time.dt_checkpoint / time.max_checkpoints (Orchestrator.cpp:253-254) and a restart initial
+    condition, which needs a file (src/ic/ICConfig.cpp:66-74)."""
+    p = write(tmp_path, sw_yaml().replace("output_interval: 0}", "output_interval: 0, dt_checkpoint: 5.0, max_checkpoints: 0}"))
+    s = gpu.config_setup(p)
+    assert (s.dt_checkpoint, s.max_checkpoints, s.restart) == (5.0, 1, 0)
+    p = write(tmp_path, sw_yaml().replace("initial_conditions: {", "initial_conditions: {type: restart, file: run.ckpt.5.bin, "))
+    assert gpu.config_setup(p).restart == 1
+    refused(tmp_path, sw_yaml().replace("initial_conditions: {", "initial_conditions: {type: restart, "), "requires file")

@@ -1,0 +1,165 @@
+"""Checkpoint and restart of the device state (Orchestrator::writeCheckpointNow /
+loadCheckpointState, src/driver/Orchestrator.cpp:1461-1502, 688-747; the reference's restart test
+tests/integration/test_restart.cpp checks that a restarted run matches the uninterrupted one). A
+run saved halfway and continued in a fresh context equals, bit for bit, the run continued in the
+context that saved it and the run that never saved; the same through the YAML front end with
+time.dt_checkpoint and a restart initial condition. The SpMV launch shape is pinned so that the
+fresh context does not autotune to another shape (which may change the last bits)."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from frehg_amd import abi, gpu, workloads
+from gpu_common import say
+
+pytestmark = pytest.mark.gpu
+
+SW = ("eta", "dept", "uu", "vv", "Vs", "Asx", "Asy", "cflx", "cfly")
+GW = ("h", "hn", "wc", "wcn", "Kx", "Ky", "Kz", "qx", "qy", "qz")
+
+
+@pytest.fixture
+def pinned_shape():
+    os.environ["FREHG_SPMV_SHAPE"] = "2:8"
+    yield
+    del os.environ["FREHG_SPMV_SHAPE"]
+
+
+def coupled(precision=abi.PREC_FP64, guess=abi.GUESS_ZERO):
+    case = workloads.CoupledCase(nx=24, ny=20, nz=8, rain=5e-4)
+    ctx = gpu.Context(0)
+    case.setup_gpu(ctx, precision, guess)
+    return case, ctx
+
+
+def steps(case, ctx, n):
+    return [ctx.coupled_step(case.cp, case.dt, case.rain, 0.0, case.t_end) for _ in range(n)]
+
+
+def same_state(a, b, label):
+    for f in SW:
+        np.testing.assert_array_equal(a.swe_get(f), b.swe_get(f), err_msg=f"{label}: {f}")
+    for f in GW:
+        np.testing.assert_array_equal(a.re_get(f), b.re_get(f), err_msg=f"{label}: {f}")
+
+
+@pytest.mark.parametrize("precision,guess", [(abi.PREC_FP64, abi.GUESS_ZERO), (abi.PREC_MIXED, abi.GUESS_PREVIOUS)])
+def test_coupled_restart_is_bit_identical(tmp_path, pinned_shape, precision, guess):
+    say(f"checkpoint: coupled 24x20x8, precision {precision}, guess {guess}")
+    ck = tmp_path / "half.bin"
+    case, a = coupled(precision, guess)
+    steps(case, a, 12)
+    a.checkpoint_save(ck)
+    sa = steps(case, a, 12)
+    _, c = coupled(precision, guess)
+    steps(case, c, 24)  # never saved
+    _, b = coupled(precision, guess)
+    info = b.checkpoint_load(ck)
+    assert (info.t, info.gw_time, info.has_sw, info.has_gw, info.nx, info.ny, info.nz, info.params_match) == \
+        (12.0, 12.0, 1, 1, 24, 20, 8, 1)
+    sb = steps(case, b, 12)
+    for x, y in zip(sa, sb):
+        assert (x.t, x.gw_time, x.boundary_outflow, x.exchange_volume, x.gw.solve.iterations) == \
+            (y.t, y.gw_time, y.boundary_outflow, y.exchange_volume, y.gw.solve.iterations)
+    same_state(a, b, "restarted")
+    same_state(a, c, "never saved")
+    assert not os.path.exists(str(ck) + ".tmp")
+
+
+def test_groundwater_adaptive_dt_restart(tmp_path, pinned_shape):
+    """The next dt of the adaptive rule travels with the checkpoint (primeAdaptiveDt)."""
+    soil = abi.soil_params(alpha=1.43, n=1.56, theta_s=0.33, Ks_z=2.89e-6, Ss=1e-5)
+    p = abi.re_params(soil=soil, solver=abi.solver_config(1e-12), bc_type=(0, 0, 0, 0, 0, 1), dz=0.01, dt=1e-4,
+                      htop=0.0)
+
+    def make():
+        ctx = gpu.Context(0)
+        ctx.re_init(1, 1, 100, p)
+        ctx.re_set_initial(wc=np.full(100, 0.033))
+        return ctx
+
+    a = make()
+    for _ in range(40):
+        a.re_step()
+    a.checkpoint_save(tmp_path / "col.bin")
+    da = [a.re_step().dt_used for _ in range(40)]
+    b = make()
+    info = b.checkpoint_load(tmp_path / "col.bin")
+    assert info.gw_dt == b.re_dt() and (info.has_sw, info.has_gw) == (0, 1)
+    db = [b.re_step().dt_used for _ in range(40)]
+    assert da == db
+    for f in GW:
+        np.testing.assert_array_equal(a.re_get(f), b.re_get(f), err_msg=f)
+
+
+def test_load_refusals(tmp_path, pinned_shape):
+    case, a = coupled()
+    steps(case, a, 2)
+    a.checkpoint_save(tmp_path / "c.bin")
+    other = gpu.Context(0)
+    other.re_init(24, 20, 9, case.re_params())
+    other.swe_init(24, 20, 1.0, 1.0, case.swe)
+    with pytest.raises(gpu.FrehgError, match="grid differs") as e:
+        other.checkpoint_load(tmp_path / "c.bin")
+    assert e.value.code == abi.ERR_INVALID_ARG
+    sw_only = gpu.Context(0)
+    sw_only.swe_init(24, 20, 1.0, 1.0, case.swe)
+    with pytest.raises(gpu.FrehgError, match="groundwater state"):
+        sw_only.checkpoint_load(tmp_path / "c.bin")
+    (tmp_path / "junk.bin").write_bytes(b"not a checkpoint at all")
+    with pytest.raises(gpu.FrehgError, match="not a checkpoint"):
+        a.checkpoint_load(tmp_path / "junk.bin")
+    data = (tmp_path / "c.bin").read_bytes()
+    (tmp_path / "short.bin").write_bytes(data[: len(data) // 2])
+    eta = a.swe_get("eta")
+    with pytest.raises(gpu.FrehgError, match="truncated"):
+        a.checkpoint_load(tmp_path / "short.bin")
+    np.testing.assert_array_equal(a.swe_get("eta"), eta)  # a bad file leaves the state alone
+    # other parameters: loaded, and reported
+    p = case.swe
+    p.manning = 0.05
+    d = gpu.Context(0)
+    d.swe_init(24, 20, 1.0, 1.0, p)
+    d.re_init(24, 20, 8, case.re_params())
+    assert d.checkpoint_load(tmp_path / "c.bin").params_match == 0
+
+
+YAML = """\
+schema_version: '2.0'
+simulation: {{id: ckpt, mode: coupled}}
+domain: {{nx: 16, ny: 12, nz: 8, dx: 1.0, dy: 1.0, dz: 0.05, botz: 0.0, bathymetry: {{from_file: true, file: bed.txt}}}}
+time: {{dt: 1.0, t_end: {t_end}, max_steps: 0, output_interval: 10, dt_checkpoint: 10, max_checkpoints: 2}}
+modules: {{surface_water: true, groundwater: true, solute: false}}
+surface_water: {{manning: 0.03, solver: {{rtol: 1.0e-12}}}}
+groundwater: {{solver: {{rtol: 1.0e-12}}, full_3d: true, adaptive_dt: true, dt_max: 2.0, dt_min: 0.01,
+              bc_type_gw: [0, 0, 0, 0, 0, 0]}}
+soil: {{types: [{{id: 0, theta_s: 0.40, theta_r: 0.05, vg: {{alpha: 2.0, n: 1.8}}, k_sat: {{x: 1.0e-5, y: 1.0e-5, z: 1.0e-5}}}}]}}
+initial_conditions: {ic}
+sources: {{surface: {{rainfall: {{from_file: false, rate: 4.0e-4}}}}}}
+output: {{format: hdf5, filename: {out}/run.h5}}
+"""
+
+
+def test_yaml_checkpoints_and_restart(tmp_path, pinned_shape):
+    say("checkpoint: YAML dt_checkpoint + restart")
+    rng = np.random.default_rng(8)
+    np.savetxt(tmp_path / "bed.txt", (0.01 * np.arange(16)[None, :] + 0.02 * rng.standard_normal((12, 16))).ravel())
+    ic = "{surface_water: {eta: -10.0}, groundwater: {wc: 0.2}}"
+    (tmp_path / "full.yaml").write_text(YAML.format(t_end=40.0, ic=ic, out="full"))
+    st = gpu.Context(0).run_config(tmp_path / "full.yaml")
+    assert st.t == 40.0
+    kept = sorted(os.path.basename(f) for f in glob.glob(str(tmp_path / "full" / "run.ckpt.*.bin")))
+    assert kept == ["run.ckpt.30.bin", "run.ckpt.40.bin"]  # max_checkpoints = 2
+    ic = "{type: restart, file: full/run.ckpt.30.bin, time: 30}"
+    (tmp_path / "rest.yaml").write_text(YAML.format(t_end=40.0, ic=ic, out="rest"))
+    st2 = gpu.Context(0).run_config(tmp_path / "rest.yaml")
+    assert (st2.t, st2.steps, st2.outputs) == (40.0, 10, 1)  # no snapshot of the starting state
+    for var in ("water_depth", "eta", "head", "moisture"):
+        np.testing.assert_array_equal(np.loadtxt(tmp_path / "rest" / f"{var}_40.txt"),
+                                      np.loadtxt(tmp_path / "full" / f"{var}_40.txt"), err_msg=var)
+    ic = "{type: restart, file: full/run.ckpt.30.bin, time: 25}"
+    (tmp_path / "bad.yaml").write_text(YAML.format(t_end=40.0, ic=ic, out="bad"))
+    with pytest.raises(gpu.FrehgError, match="differs from the checkpoint"):
+        gpu.Context(0).run_config(tmp_path / "bad.yaml")
