@@ -1,0 +1,93 @@
+"""The learned initial guess inside the coupled step (the held-out evaluation of the learned-guess
+configuration, reduced to a 64x64 surface over 64x64x16 soil): the network is trained on three
+seeded realisations of the workload, then a fourth one runs with the learned start, with the
+extrapolated start (h^n + the previous increment, the method without the network) and with plain
+FP64 PCG. Heads and depths of the learned run match the FP64 oracle of the same Picard
+settings, no step falls back, and the learned start needs fewer Krylov iterations than the
+extrapolated one."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from cases import interior2, interior3
+from frehg_amd import abi, gpu, workloads
+from gpu_common import say
+
+pytestmark = pytest.mark.gpu
+
+N, NZ, STEPS, PICARD = 64, 16, 20, 50
+RAIN = 30 * workloads.RAIN_50MM_PER_H  # ponds within the run, so depths are compared where water stands
+
+
+def case(seed_bed, seed_ks):
+    return workloads.CoupledCase(nx=N, ny=N, nz=NZ, seed_bed=seed_bed, seed_ks=seed_ks, rain=RAIN)
+
+
+def train():
+    ctx = gpu.Context(0)
+    ctx.mlp_init(17)
+    stride, seeds, logged = 4, ((101, 201), (102, 202), (103, 203)), 12
+    ctx.mlp_log(stride, capacity=len(seeds) * logged * ((N * N * NZ + stride - 1) // stride))
+    for sb, sk in seeds:
+        c = case(sb, sk)
+        c.setup_gpu(ctx, abi.PREC_FP64, abi.GUESS_ZERO, picard_max=PICARD)
+        for _ in range(logged):
+            ctx.coupled_step(c.cp, c.dt, c.rain, 0.0, c.t_end)
+    first, last = ctx.mlp_train(800, batch=16384, lr=1e-3, seed=29)
+    say(f"  {ctx.mlp_logged()} pairs, loss {first:.3f} -> {last:.4f}")
+    assert last < 0.2 * first
+    model = ctx.mlp_get_params()
+    ctx.close()
+    return model
+
+
+def run(held, prec, guess, model=None):
+    ctx = gpu.Context(0)
+    if model is not None:
+        ctx.mlp_set_params(*model)
+        ctx.mlp_gate(1e-6)
+    held.setup_gpu(ctx, prec, guess, picard_max=PICARD)
+    iters = fallbacks = picard = 0
+    inferred = 0
+    for _ in range(STEPS):
+        st = ctx.coupled_step(held.cp, held.dt, held.rain, 0.0, held.t_end)
+        assert st.gw.picard_converged == 1
+        iters += st.gw.solve.iterations
+        fallbacks += st.gw.guess_fallback + st.gw.solve.fallback
+        picard += st.gw.picard_iterations
+        if model is not None:
+            inferred += ctx.mlp_inferred_cells()
+    out = (ctx.re_get("h"), ctx.swe_get("dept"), iters, fallbacks, picard, inferred)
+    ctx.close()
+    return out
+
+
+def test_learned_guess_in_the_coupled_step_on_a_held_out_realisation():
+    say("learned guess in the coupled step: 64x64x16, held-out realisation")
+    model = train()
+    held = case(7, 8)
+    h_l, d_l, it_l, fb_l, pic_l, inferred = run(held, abi.PREC_MIXED, abi.GUESS_LEARNED, model)
+    h_e, d_e, it_e, fb_e, pic_e, _ = run(held, abi.PREC_MIXED, abi.GUESS_EXTRAPOLATED)
+    h_p, d_p, it_p, _, pic_p, _ = run(held, abi.PREC_FP64, abi.GUESS_ZERO)
+    say(f"  Krylov iterations over {STEPS} steps: learned {it_l}, extrapolated {it_e}, plain {it_p}; "
+        f"Picard iterates {pic_l} / {pic_e} / {pic_p}; cells inferred {inferred}")
+    s = ol.OracleSwe(N, N, held.dx, held.dx, held.swe)
+    s.set_bathymetry(held.bed_with_offset())
+    s.initialize_state(-1.0e3)
+    s.set_boundary_polygons(held.boundary_regions())
+    r = ol.OracleRe(N, N, NZ, held.re_params(picard_max=PICARD))
+    r.set_soil([held.soil], None, True, held.ks.ravel())
+    r.set_initial(h=held.h0)
+    oc = ol.OracleCoupled(s, r, held.cp, held.t_end)
+    for _ in range(STEPS):
+        oc.step(held.dt, held.rain)
+    ho, do = interior3(r.get("h"), N, N, NZ), interior2(s.get("dept"), N, N)
+    assert (do > 1e-3).sum() > 50, "the held-out run must pond"
+    for label, h, d in (("learned", h_l, d_l), ("extrapolated", h_e, d_e), ("plain", h_p, d_p)):
+        eh = float(np.abs(interior3(h, N, N, NZ) - ho).max())
+        ed = float(np.abs(interior2(d, N, N) - do).max())
+        say(f"  {label}: max |dh| {eh:.2e} m, max |d depth| {ed:.2e} m vs the oracle")
+        assert eh <= 1e-6 and ed <= 1e-6, label
+    assert fb_l == 0 and fb_e == 0
+    assert inferred > 0
+    assert it_l < it_e
