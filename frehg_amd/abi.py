@@ -115,6 +115,19 @@ class CoupledStats(C.Structure):
                 ("boundary_outflow", C.c_double)]
 
 
+class SoluteParams(C.Structure):
+    """frehg_solute_params."""
+    _fields_ = [("c_rain", C.c_double), ("k_decay", C.c_double), ("D", C.c_double), ("cfl_max", C.c_double),
+                ("min_depth", C.c_double), ("start_time", C.c_double), ("muscl", C.c_int32),
+                ("implicit_diffusion", C.c_int32), ("substeps", C.c_int32), ("solver", SolverConfig)]
+
+
+class SoluteStats(C.Structure):
+    """frehg_solute_stats: cumulative solute totals."""
+    _fields_ = [("rain_added", C.c_double), ("decay_loss", C.c_double), ("max_courant", C.c_double),
+                ("substeps", C.c_int64), ("refused_steps", C.c_int32), ("diffusion_iterations", C.c_int32)]
+
+
 class RunSetup(C.Structure):
     """frehg_run_setup: a configuration as the front end maps it."""
     _fields_ = [("surface_water", C.c_int32), ("groundwater", C.c_int32), ("solute", C.c_int32),
@@ -183,6 +196,17 @@ def swe_params(solver=None, **kw):
     """SweParams with the reference defaults (SweSolver.hpp:28-39) and SW solver rtol 1e-12."""
     p = SweParams(gravity=9.81, manning=0.0, min_depth=1.0e-8, visc_x=0.0, visc_y=0.0, hD=0.1,
                   hE=0.0, wtfh=1.0e-8, dt=1.0, offset=0.0)
+    p.solver = solver if solver is not None else solver_config(1e-12)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def solute_params(solver=None, **kw):
+    """SoluteParams with the reference defaults (src/solute/SoluteParams.hpp:18-25) and the diffusion
+    solver of SoluteStepper (rtol 1e-12)."""
+    p = SoluteParams(c_rain=0.0, k_decay=0.0, D=1.0e-9, cfl_max=1.0, min_depth=1.0e-8, start_time=0.0,
+                     muscl=0, implicit_diffusion=1, substeps=1)
     p.solver = solver if solver is not None else solver_config(1e-12)
     for k, v in kw.items():
         setattr(p, k, v)

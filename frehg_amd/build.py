@@ -21,7 +21,8 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", os.path.join(HERE, "..", "include")]
 
-SOURCES = ["krylov.hip", "re.hip", "swe.hip", "coupling.hip", "mlp.hip", "ensemble.hip", "noalloc.hip", "api.hip"]
+SOURCES = ["krylov.hip", "re.hip", "swe.hip", "coupling.hip", "mlp.hip", "ensemble.hip", "solute.hip", "noalloc.hip",
+           "api.hip"]
 # Host-only C++ above the C ABI (configuration front end), built with g++.
 HOST_SOURCES = ["yaml.cpp", "config.cpp", "driver.cpp"]
 CXX = os.environ.get("CXX", "g++")

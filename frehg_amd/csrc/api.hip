@@ -15,6 +15,7 @@
 #include "mlp.hpp"
 #include "noalloc.hpp"
 #include "re.hpp"
+#include "solute.hpp"
 #include "swe.hpp"
 
 struct frehg_ctx_s {
@@ -27,6 +28,7 @@ struct frehg_ctx_s {
   std::unique_ptr<frehg::Coupler> cpl;
   std::unique_ptr<frehg::Mlp> mlp;
   std::unique_ptr<frehg::Ensemble> ens;
+  std::unique_ptr<frehg::Solute> sol;  // refers to re / swe: reset before either changes
   double t = 0.0, gw_time = 0.0;
 };
 
@@ -134,6 +136,7 @@ void frehg_destroy(frehg_ctx_t ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  ctx->sol.reset();
   ctx->ens.reset();
   ctx->mlp.reset();
   ctx->cpl.reset();
@@ -155,6 +158,7 @@ int frehg_synchronize(frehg_ctx_t ctx) {
 int frehg_re_init(frehg_ctx_t ctx, int nx, int ny, int nz, const frehg_re_params* p) {
   return guard(ctx, [&] {
     if (!p) throw frehg::ArgError("frehg_re_init: null params");
+    ctx->sol.reset();  // a solute set up on the old solver goes with it
     ctx->re.reset();
     ctx->re.reset(new frehg::ReState(nx, ny, nz, *p, ctx->stream, &ctx->prof));
     bind_mlp(ctx);
@@ -232,6 +236,7 @@ int frehg_re_prime_dt(frehg_ctx_t ctx, double dt) {
 int frehg_swe_init(frehg_ctx_t ctx, int nx, int ny, double dx, double dy, const frehg_swe_params* p) {
   return guard(ctx, [&] {
     if (!p) throw frehg::ArgError("frehg_swe_init: null params");
+    ctx->sol.reset();
     ctx->swe.reset();
     bind_mlp(ctx);
     ctx->swe.reset(new frehg::SweState(nx, ny, dx, dy, *p, ctx->stream, &ctx->prof));
@@ -347,7 +352,11 @@ int frehg_coupled_step(frehg_ctx_t ctx, const frehg_coupling_params* cp, double 
     const bool bc = swe.has_boundaries();
     if (bc) swe.apply_boundaries_async(dt);
     auto& cpl = need_cpl(ctx);
+    const double t0 = ctx->t;
+    cpl.set_capture_rates(ctx->sol != nullptr);
     cpl.exchange_async(swe, re, *cp, dt);  // its volume is read after the groundwater step
+    // the exchange carries solute with its water (applyCouplingExchange, :1260-1268)
+    if (ctx->sol) ctx->sol->interface_exchange(cpl.rates(), dt, cp->dx * cp->dy);
     const double eps = 1.0e-9 * std::max(1.0, t_end);  // gwCatchUpTo (:1243-1252)
     int fallback = 0, iters = 0, unconverged = 0;
     while (ctx->gw_time < t_new - eps) {
@@ -359,6 +368,7 @@ int frehg_coupled_step(frehg_ctx_t ctx, const frehg_coupling_params* cp, double 
       ++local.gw_substeps;
     }
     local.gw.solve.iterations = iters;
+    if (ctx->sol) ctx->sol->apply(t0, dt, rain);  // applySolute after the flow step (:1333)
     local.exchange_volume = cpl.exchanged_volume();
     local.boundary_outflow = bc ? swe.boundary_outflow() : 0.0;
     ctx->t = t_new;
@@ -696,6 +706,63 @@ int frehg_mlp_increment(frehg_ctx_t ctx, double* dst) {
 int frehg_mlp_predict(frehg_ctx_t ctx, const float* x, int n, float* out) {
   return guard(ctx, [&] {
     need_mlp(ctx).predict_batch(x, n, out);
+    return FREHG_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---- solute ----------------------------------------------------------------------------------
+extern "C" {
+
+int frehg_solute_init(frehg_ctx_t ctx, const frehg_solute_params* p) {
+  return guard(ctx, [&] {
+    if (!p) throw frehg::ArgError("frehg_solute_init: null params");
+    ctx->sol.reset();
+    ctx->sol.reset(new frehg::Solute(*p, ctx->swe.get(), ctx->re.get(), ctx->stream, &ctx->prof));
+    return FREHG_OK;
+  });
+}
+
+int frehg_solute_set_initial(frehg_ctx_t ctx, const double* surface_interior, const double* subsurface_interior) {
+  return guard(ctx, [&] {
+    if (!ctx->sol) throw std::runtime_error("frehg_solute_set_initial: call frehg_solute_init first");
+    ctx->sol->set_initial(surface_interior, subsurface_interior);
+    return FREHG_OK;
+  });
+}
+
+int frehg_solute_set_field(frehg_ctx_t ctx, const char* name, const double* src) {
+  return guard(ctx, [&] {
+    if (!ctx->sol) throw std::runtime_error("frehg_solute_set_field: call frehg_solute_init first");
+    if (!name || !src) throw frehg::ArgError("null argument");
+    ctx->sol->set_field(name, src);
+    return FREHG_OK;
+  });
+}
+
+int frehg_solute_get_field(frehg_ctx_t ctx, const char* name, double* dst) {
+  return guard(ctx, [&] {
+    if (!ctx->sol) throw std::runtime_error("frehg_solute_get_field: call frehg_solute_init first");
+    if (!name || !dst) throw frehg::ArgError("null argument");
+    ctx->sol->get_field(name, dst);
+    return FREHG_OK;
+  });
+}
+
+int frehg_solute_apply(frehg_ctx_t ctx, double t, double dt, double rain) {
+  return guard(ctx, [&] {
+    if (!ctx->sol) throw std::runtime_error("frehg_solute_apply: call frehg_solute_init first");
+    ctx->sol->apply(t, dt, rain);
+    return FREHG_OK;
+  });
+}
+
+int frehg_solute_mass(frehg_ctx_t ctx, double* mass, frehg_solute_stats* totals) {
+  return guard(ctx, [&] {
+    if (!ctx->sol) throw std::runtime_error("frehg_solute_mass: call frehg_solute_init first");
+    if (mass) *mass = ctx->sol->mass();
+    if (totals) *totals = ctx->sol->totals();
     return FREHG_OK;
   });
 }

@@ -137,7 +137,7 @@ void Coupler::exchange_async(SweState& swe, ReState& re, const frehg_coupling_pa
   const unsigned int nb = grid.x * grid.y;
   ensure(nb, (long long)gs.nx * gs.ny);
   hipLaunchKernelGGL(k_exchange, grid, blk, 0, s_, P, swe.d("eta"), swe.d("bottom"), swe.d("dept"), re.d_wc(),
-                     re.d_h(), re.d_hmod(), want_q_ ? qdev_ : nullptr, part_);
+                     re.d_h(), re.d_hmod(), (want_q_ || capture_q_) ? qdev_ : nullptr, part_);
   FREHG_LAUNCH_CHECK();
   // slot 2: volume() uses slots 0 and 1 and may run before this result is read
   hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kRedThreads), 0, s_, (const double*)part_, nb, 1, dres_ + 2);

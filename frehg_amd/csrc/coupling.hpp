@@ -18,6 +18,10 @@ class Coupler {
   double exchanged_volume();
   // Orchestrator::computeWaterVolume (src/driver/Orchestrator.cpp:898-936).
   void volume(SweState* swe, ReState* re, double* surface, double* subsurface);
+  // Keep the per-column rates of every exchange on the device (nx * ny, x fastest; m^3/s,
+  // + = seepage), for the solute exchange that follows the water exchange.
+  void set_capture_rates(bool on) { capture_q_ = on; }
+  const double* rates() const { return qdev_; }
 
  private:
   void ensure(unsigned int nblocks, long long ncol);
@@ -30,6 +34,7 @@ class Coupler {
   double* hres_ = nullptr;
   unsigned int* counter_ = nullptr;
   bool want_q_ = false;  // exchange_async also stores the per-column rates
+  bool capture_q_ = false;
 };
 
 }  // namespace frehg

@@ -43,6 +43,10 @@ class ReState {
   double* d_hn() { return hn_; }
   double* d_wcn() { return wcn_; }
   double* d_kr() { return kr_; }
+  // Darcy fluxes of the last step (call materialize_fluxes() first).
+  double* d_qx() { return qx_; }
+  double* d_qy() { return qy_; }
+  double* d_qz() { return qz_; }
   double* d_ch() { return ch_; }
   // Per-cell "head changed since kr and C were last evaluated" flags for a kernel that changes h
   // between steps (the SW exchange), or null when the next step re-evaluates every cell anyway.

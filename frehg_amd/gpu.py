@@ -98,6 +98,12 @@ def load():
         "frehg_config_setup": (C.c_int, [C.c_char_p, P(C.c_char_p), C.c_int, P(abi.RunSetup), C.c_char_p, C.c_int]),
         "frehg_config_query": (C.c_int, [C.c_char_p, P(C.c_char_p), C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
         "frehg_run_config": (C.c_int, [ctx, C.c_char_p, P(C.c_char_p), C.c_int, C.c_char_p, P(abi.RunStats)]),
+        "frehg_solute_init": (C.c_int, [ctx, P(abi.SoluteParams)]),
+        "frehg_solute_set_initial": (C.c_int, [ctx, dp, dp]),
+        "frehg_solute_set_field": (C.c_int, [ctx, C.c_char_p, dp]),
+        "frehg_solute_get_field": (C.c_int, [ctx, C.c_char_p, dp]),
+        "frehg_solute_apply": (C.c_int, [ctx, C.c_double, C.c_double, C.c_double]),
+        "frehg_solute_mass": (C.c_int, [ctx, dp, P(abi.SoluteStats)]),
         "frehg_checkpoint_save": (C.c_int, [ctx, C.c_char_p]),
         "frehg_checkpoint_load": (C.c_int, [ctx, C.c_char_p, P(abi.CheckpointInfo)]),
         "frehg_config_field": (C.c_int64, [C.c_char_p, P(C.c_char_p), C.c_int, C.c_char_p, dp, C.c_int64,
@@ -334,6 +340,40 @@ class Context:
                                                    C.c_void_p(int(b_dev)), C.c_void_p(int(x_dev)), C.byref(cfg),
                                                    C.byref(st)))
         return st
+
+    # ---- solute --------------------------------------------------------------------------------------
+    def solute_init(self, params):
+        self.check(self.lib.frehg_solute_init(self.h, C.byref(params)))
+
+    def solute_set_initial(self, surface=None, subsurface=None):
+        s = None if surface is None else np.ascontiguousarray(surface, dtype=np.float64).ravel()
+        g = None if subsurface is None else np.ascontiguousarray(subsurface, dtype=np.float64).ravel()
+        self.check(self.lib.frehg_solute_set_initial(self.h, _ptr(s), _ptr(g)))
+
+    def _solute_size(self, name):
+        if name == "surface":
+            nx, ny = self.swe_dims
+            return (nx + 2) * (ny + 2)
+        return self.re_ncell()
+
+    def solute_get(self, name):
+        out = np.empty(self._solute_size(name), dtype=np.float64)
+        self.check(self.lib.frehg_solute_get_field(self.h, name.encode(), _ptr(out)))
+        return out
+
+    def solute_set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        assert a.size == self._solute_size(name)
+        self.check(self.lib.frehg_solute_set_field(self.h, name.encode(), _ptr(a)))
+
+    def solute_apply(self, t, dt, rain=0.0):
+        self.check(self.lib.frehg_solute_apply(self.h, t, dt, rain))
+
+    def solute_mass(self):
+        m = C.c_double()
+        st = abi.SoluteStats()
+        self.check(self.lib.frehg_solute_mass(self.h, C.byref(m), C.byref(st)))
+        return m.value, st
 
     # ---- checkpoint / restart --------------------------------------------------------------------------
     def checkpoint_save(self, path):

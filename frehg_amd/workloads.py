@@ -61,6 +61,16 @@ class CoupledCase:
         self.bc = (0, 0, 0, 0, 0, 0)
         self.outflow = outflow
 
+    def solute_patch(self, side=100.0):
+        """Initial surface concentration: 1 in a side x side m square at the domain centre, 0
+        elsewhere (interior cells, x fastest)."""
+        x = (np.arange(self.nx) + 0.5) * self.dx
+        y = (np.arange(self.ny) + 0.5) * self.dx
+        cx, cy = 0.5 * self.nx * self.dx, 0.5 * self.ny * self.dx
+        inx = np.abs(x - cx) < 0.5 * side
+        iny = np.abs(y - cy) < 0.5 * side
+        return (iny[:, None] & inx[None, :]).astype(np.float64).ravel()
+
     def boundary_regions(self):
         """Outflow at the low-x edge: a critical-depth region (bc_critical) over the first column
         of surface cells (centres at x = dx/2), as a polygon strip in domain coordinates."""

@@ -345,6 +345,51 @@ int frehg_profile_reset(frehg_ctx_t ctx);
  * frehg_re_init. No counterpart in the reference (PETSc's MatMult has no launch shape). */
 int frehg_spmv_shape(frehg_ctx_t ctx, int precision, int* cells_per_lane, int* kchunk);
 
+/* ---- Passive solute: SoluteParams / SoluteStepper / Orchestrator::applySolute
+ * (src/solute/SoluteParams.hpp:16-30, src/solute/SoluteStepper.cpp:24-72,
+ * src/driver/Orchestrator.cpp:938-1056, 1270-1299). One concentration per surface cell and per
+ * soil cell. After each flow step: rain of concentration c_rain mixes into cells wet at the start
+ * of the step, then per substep first-order decay, explicit conservative advection by the
+ * surface velocities and Darcy fluxes (upwind, or minmod MUSCL), refused above cfl_max (the
+ * substep count doubles, up to 4096; beyond that the step leaves the solute unchanged), and
+ * implicit diffusion (I - dt D L) C' = C solved by Jacobi-CG. A coupled step carries solute with
+ * the SW<->GW exchange water before the groundwater catches up. */
+typedef struct {
+  double c_rain;      /* rain concentration */
+  double k_decay;     /* first-order decay rate [1/s] */
+  double D;           /* isotropic diffusion coefficient [m^2/s] */
+  double cfl_max;     /* advective Courant ceiling (default 1) */
+  double min_depth;   /* surface depth below which a cell takes no rain solute [m] */
+  double start_time;  /* no transport in steps that end at or before this time [s] */
+  int32_t muscl;      /* 0: first-order upwind, 1: minmod-limited MUSCL */
+  int32_t implicit_diffusion; /* 0: none, 1: implicit */
+  int32_t substeps;   /* substeps per flow step to start from (>= 1) */
+  frehg_solver_config solver; /* the diffusion solve: FP64 CG (reference rtol 1e-12) */
+} frehg_solute_params;
+
+typedef struct {
+  double rain_added;      /* solute mass the rain brought [concentration * m^3] */
+  double decay_loss;      /* solute mass decay removed */
+  double max_courant;     /* largest Courant number of an accepted substep */
+  int64_t substeps;       /* accepted substeps */
+  int32_t refused_steps;  /* flow steps whose Courant number stayed above cfl_max at 4096 substeps */
+  int32_t diffusion_iterations; /* CG iterations of the diffusion solves */
+} frehg_solute_stats;
+
+/* After frehg_swe_init and/or frehg_re_init: the solute of whichever solvers the context holds
+ * (concentrations start at zero). */
+int frehg_solute_init(frehg_ctx_t ctx, const frehg_solute_params* p);
+/* Interior initial concentrations (x fastest, then y, then z); either may be NULL. */
+int frehg_solute_set_initial(frehg_ctx_t ctx, const double* surface_interior, const double* subsurface_interior);
+/* name "surface" ((nx+2)(ny+2)) or "subsurface" ((nx+2)(ny+2)(nz+2)), halo-padded as the other fields. */
+int frehg_solute_set_field(frehg_ctx_t ctx, const char* name, const double* src);
+int frehg_solute_get_field(frehg_ctx_t ctx, const char* name, double* dst);
+/* applySolute for the flow step [t, t + dt] of a surface-only or groundwater-only driver, after
+ * its flow step (frehg_coupled_step applies it by itself). */
+int frehg_solute_apply(frehg_ctx_t ctx, double t, double dt, double rain);
+/* computeSoluteMass (:852-896) and the cumulative totals. */
+int frehg_solute_mass(frehg_ctx_t ctx, double* mass, frehg_solute_stats* totals);
+
 /* ---- Checkpoint and restart of the device state: Orchestrator::writeCheckpointNow /
  * loadCheckpointState (src/driver/Orchestrator.cpp:1461-1502, 688-747), Hdf5Writer::
  * writeCheckpoint (src/io/Hdf5Writer.cpp:400-470). HDF5 is not available here: the file is a

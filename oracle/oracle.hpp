@@ -395,6 +395,44 @@ struct CoupledState {
 void coupled_step(const CouplingParams& cp, SweSolver& swe, ReSolver& re, CoupledState& cs,
                   real dt, real rain, real evap, real t_end_hint);
 
+// ---------------------------------------------------------------------------------------------
+// Passive solute: src/solute/{SoluteParams,Advection,Diffusion,SourceSink,SoluteStepper}.cpp and
+// Orchestrator::applySolute / applyCouplingSoluteExchange / computeSoluteMass
+// (src/driver/Orchestrator.cpp:852-896, 938-1056, 1270-1299).
+struct SoluteParams {  // SoluteParams.hpp:16-30, plus substeps / start_time (Orchestrator.cpp:759-760)
+  real c_rain = 0.0, k_decay = 0.0, D = 1.0e-9, cfl_max = 1.0, min_depth = 1.0e-8, start_time = 0.0;
+  int muscl = 0, implicit_diffusion = 1, substeps = 1;
+  SolverCfg solver{1e-12, 1e-50, 10000};  // Orchestrator.cpp:767-772
+};
+struct SoluteTotals {
+  real rain_added = 0.0, decay_loss = 0.0, max_courant = 0.0;
+  long long substeps = 0;
+  int refused_steps = 0, diffusion_iterations = 0;
+};
+class SoluteOracle {
+ public:
+  SoluteOracle(const SoluteParams& p, SweSolver* swe, ReSolver* re);
+  std::vector<real> cs, cg;  // surface / subsurface concentration, halo padded
+  SoluteTotals tot;
+  // applyCouplingSoluteExchange with the per-column exchange rates q (nx*ny, m^3/s, + = seepage).
+  void interfaceExchange(const std::vector<real>& q, real dt, real area);
+  void apply(real t, real dt, real rain);  // applySolute for the flow step [t, t + dt]
+  real mass() const;                       // computeSoluteMass
+  // One SoluteStepper::step with rain 0 (decay, advection, diffusion); false if refused.
+  bool step(real dt, real* cfl);
+  const SoluteParams& params() const { return p_; }
+
+ private:
+  SoluteParams p_;
+  SweSolver* swe_;
+  ReSolver* re_;
+  real dx_ = 1.0, dy_ = 1.0, dz_ = 1.0;
+};
+
+// coupled_step with solute: the exchange carries solute, then applySolute after the GW catch-up.
+void coupled_step_solute(const CouplingParams& cp, SweSolver& swe, ReSolver& re, CoupledState& cs,
+                         SoluteOracle* sol, real dt, real rain, real evap, real t_end_hint);
+
 }  // namespace frehg_oracle
 
 #endif  // FREHG_ORACLE_HPP

@@ -358,6 +358,96 @@ int orc_coupled_step(void* swe, void* re, const frehg_coupling_params* cp, doubl
   });
 }
 
+int orc_coupled_step_solute(void* swe, void* re, void* sol, const frehg_coupling_params* cp, double* state,
+                            double dt, double rain, double evap, double t_end) {
+  return guard([&] {
+    CouplingParams c{cp->dx, cp->dy, cp->visc, cp->min_depth};
+    CoupledState cs;
+    cs.t = state[0];
+    cs.gw_time = state[1];
+    cs.exchange_volume = state[2];
+    cs.gw_substeps = (int)state[3];
+    cs.boundary_outflow = state[4];
+    auto it = bc_regions().find(swe);
+    cs.bc = it == bc_regions().end() ? nullptr : &it->second;
+    coupled_step_solute(c, *static_cast<SweSolver*>(swe), *static_cast<ReSolver*>(re), cs,
+                        static_cast<SoluteOracle*>(sol), dt, rain, evap, t_end);
+    state[0] = cs.t;
+    state[1] = cs.gw_time;
+    state[2] = cs.exchange_volume;
+    state[3] = cs.gw_substeps;
+    state[4] = cs.boundary_outflow;
+  });
+}
+
+// ---- solute ----
+void* orc_solute_create(void* swe, void* re, const frehg_solute_params* p) {
+  try {
+    SoluteParams sp;
+    sp.c_rain = p->c_rain;
+    sp.k_decay = p->k_decay;
+    sp.D = p->D;
+    sp.cfl_max = p->cfl_max;
+    sp.min_depth = p->min_depth;
+    sp.start_time = p->start_time;
+    sp.muscl = p->muscl;
+    sp.implicit_diffusion = p->implicit_diffusion;
+    sp.substeps = p->substeps;
+    sp.solver = SolverCfg{p->solver.rtol, p->solver.atol, p->solver.max_it, 0};
+    return new SoluteOracle(sp, static_cast<SweSolver*>(swe), static_cast<ReSolver*>(re));
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+void orc_solute_destroy(void* h) { delete static_cast<SoluteOracle*>(h); }
+int orc_solute_get(void* h, const char* name, double* dst) {
+  return guard([&] {
+    auto* s = static_cast<SoluteOracle*>(h);
+    const std::vector<real>& v = std::string(name) == "surface" ? s->cs : s->cg;
+    if (v.empty()) throw std::runtime_error("solute field absent");
+    std::copy(v.begin(), v.end(), dst);
+  });
+}
+int orc_solute_set(void* h, const char* name, const double* src) {
+  return guard([&] {
+    auto* s = static_cast<SoluteOracle*>(h);
+    std::vector<real>& v = std::string(name) == "surface" ? s->cs : s->cg;
+    if (v.empty()) throw std::runtime_error("solute field absent");
+    std::copy(src, src + v.size(), v.begin());
+  });
+}
+int orc_solute_apply(void* h, double t, double dt, double rain) {
+  return guard([&] { static_cast<SoluteOracle*>(h)->apply(t, dt, rain); });
+}
+// One stepper step with rain 0; out[0] = Courant number, return value FREHG-style (0) with
+// out[1] = 1 if accepted.
+int orc_solute_step(void* h, double dt, double* out) {
+  return guard([&] {
+    real cfl = 0.0;
+    const bool ok = static_cast<SoluteOracle*>(h)->step(dt, &cfl);
+    out[0] = cfl;
+    out[1] = ok ? 1.0 : 0.0;
+  });
+}
+int orc_solute_exchange(void* h, const double* q, int n, double dt, double area) {
+  return guard([&] { static_cast<SoluteOracle*>(h)->interfaceExchange(std::vector<real>(q, q + n), dt, area); });
+}
+int orc_solute_mass(void* h, double* mass, frehg_solute_stats* st) {
+  return guard([&] {
+    auto* s = static_cast<SoluteOracle*>(h);
+    if (mass) *mass = s->mass();
+    if (st) {
+      st->rain_added = s->tot.rain_added;
+      st->decay_loss = s->tot.decay_loss;
+      st->max_courant = s->tot.max_courant;
+      st->substeps = s->tot.substeps;
+      st->refused_steps = s->tot.refused_steps;
+      st->diffusion_iterations = s->tot.diffusion_iterations;
+    }
+  });
+}
+
 // Stencil solve in the reference's owned ordering (see frehg_ls_solve in frehg_gpu.h).
 int orc_ls_solve(int ndim, int nx, int ny, int nz, const double* diag, const double* const* off,
                  const double* b_owned, double* x_owned, const frehg_solver_config* cfg,

@@ -604,19 +604,11 @@ real couple_exchange(const CouplingParams& cp, SweSolver& swe, ReSolver& re, rea
   return total;
 }
 
+// Orchestrator.cpp:1322-1336: the SW advance, the boundary regions (applySurfacePolygons, :1240,
+// :1301-1308), the exchange, the GW catch-up (gwCatchUpTo, :1243-1252); without solute.
 void coupled_step(const CouplingParams& cp, SweSolver& swe, ReSolver& re, CoupledState& cs,
                   real dt, real rain, real evap, real t_end_hint) {
-  const real t_new = cs.t + dt;  // Orchestrator.cpp:1322-1336
-  swe.advanceStep(rain, evap);
-  if (cs.bc) cs.boundary_outflow += cs.bc->apply(swe, dt);  // applySurfacePolygons, :1240, :1301-1308
-  cs.exchange_volume += couple_exchange(cp, swe, re, dt, nullptr);
-  const real eps = 1.0e-9 * std::max<real>(1.0, t_end_hint);  // gwCatchUpTo :1243-1252
-  while (cs.gw_time < t_new - eps) {
-    re.advanceStep();
-    cs.gw_time += re.dt_used;
-    ++cs.gw_substeps;
-  }
-  cs.t = t_new;
+  coupled_step_solute(cp, swe, re, cs, nullptr, dt, rain, evap, t_end_hint);
 }
 
 }  // namespace frehg_oracle

@@ -57,6 +57,17 @@ def lib():
         L.orc_couple_exchange.argtypes = [C.c_void_p, C.c_void_p, P(abi.CouplingParams), C.c_double, dp, dp]
         L.orc_coupled_step.argtypes = [C.c_void_p, C.c_void_p, P(abi.CouplingParams), dp, C.c_double,
                                        C.c_double, C.c_double, C.c_double]
+        L.orc_coupled_step_solute.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.CouplingParams), dp,
+                                              C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_solute_create.restype = C.c_void_p
+        L.orc_solute_create.argtypes = [C.c_void_p, C.c_void_p, P(abi.SoluteParams)]
+        L.orc_solute_destroy.argtypes = [C.c_void_p]
+        L.orc_solute_get.argtypes = [C.c_void_p, C.c_char_p, dp]
+        L.orc_solute_set.argtypes = [C.c_void_p, C.c_char_p, dp]
+        L.orc_solute_apply.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.orc_solute_step.argtypes = [C.c_void_p, C.c_double, dp]
+        L.orc_solute_exchange.argtypes = [C.c_void_p, dp, C.c_int, C.c_double, C.c_double]
+        L.orc_solute_mass.argtypes = [C.c_void_p, dp, P(abi.SoluteStats)]
         L.orc_ls_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dp, P(dp), dp, dp,
                                    P(abi.SolverConfig), P(abi.SolveStats)]
         L.orc_vg_eval.argtypes = [P(abi.SoilParams), C.c_int, dp, dp, dp, dp, dp, dp, dp]
@@ -211,16 +222,72 @@ def couple_exchange(swe, re, cp, dt, want_q=False):
     return (vol.value, q) if want_q else vol.value
 
 
-class OracleCoupled:
-    """Orchestrator::step, coupled branch (src/driver/Orchestrator.cpp:1322-1336)."""
+class OracleSolute:
+    """Oracle passive solute over the given solvers (either may be None): SoluteStepper and
+    Orchestrator::applySolute (src/solute/SoluteStepper.cpp, src/driver/Orchestrator.cpp:938-1056)."""
 
-    def __init__(self, swe, re, cp, t_end):
-        self.swe, self.re, self.cp, self.t_end = swe, re, cp, t_end
+    def __init__(self, swe, re, params):
+        self.swe, self.re, self.params = swe, re, params  # the solvers must outlive this object
+        self.h = lib().orc_solute_create(None if swe is None else swe.h, None if re is None else re.h,
+                                         C.byref(params))
+        if not self.h:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_solute_destroy(self.h)
+            self.h = None
+
+    def _size(self, name):
+        if name == "surface":
+            return (self.swe.nx + 2) * (self.swe.ny + 2)
+        return self.re.ncell
+
+    def get(self, name):
+        out = np.empty(self._size(name), dtype=np.float64)
+        check(lib().orc_solute_get(self.h, name.encode(), ptr(out)))
+        return out
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        assert a.size == self._size(name)
+        check(lib().orc_solute_set(self.h, name.encode(), ptr(a)))
+
+    def apply(self, t, dt, rain=0.0):
+        check(lib().orc_solute_apply(self.h, t, dt, rain))
+
+    def step(self, dt):
+        """One stepper step (decay, advection, diffusion); returns (Courant number, accepted)."""
+        out = np.zeros(2, dtype=np.float64)
+        check(lib().orc_solute_step(self.h, dt, ptr(out)))
+        return float(out[0]), bool(out[1])
+
+    def exchange(self, q, dt, area):
+        qq = np.ascontiguousarray(q, dtype=np.float64).ravel()
+        check(lib().orc_solute_exchange(self.h, ptr(qq), qq.size, dt, area))
+
+    def mass(self):
+        m = C.c_double()
+        st = abi.SoluteStats()
+        check(lib().orc_solute_mass(self.h, C.byref(m), C.byref(st)))
+        return m.value, st
+
+
+class OracleCoupled:
+    """Orchestrator::step, coupled branch (src/driver/Orchestrator.cpp:1322-1336), with the solute
+    after the flow step when one is given."""
+
+    def __init__(self, swe, re, cp, t_end, solute=None):
+        self.swe, self.re, self.cp, self.t_end, self.solute = swe, re, cp, t_end, solute
         self.state = np.zeros(5, dtype=np.float64)  # t, gw_time, exchange volume, substeps, outflow
 
     def step(self, dt, rain, evap=0.0):
-        check(lib().orc_coupled_step(self.swe.h, self.re.h, C.byref(self.cp), ptr(self.state), dt,
-                                     rain, evap, self.t_end))
+        if self.solute is None:
+            check(lib().orc_coupled_step(self.swe.h, self.re.h, C.byref(self.cp), ptr(self.state), dt,
+                                         rain, evap, self.t_end))
+        else:
+            check(lib().orc_coupled_step_solute(self.swe.h, self.re.h, self.solute.h, C.byref(self.cp),
+                                                ptr(self.state), dt, rain, evap, self.t_end))
 
 
 def ls_solve(ndim, nx, ny, nz, diag, offdiag, b, cfg):
