@@ -138,14 +138,15 @@ class RunSetup(C.Structure):
                 ("rain_samples", C.c_int32), ("rain_rate", C.c_double), ("swe", SweParams), ("re", ReParams),
                 ("coupling", CouplingParams), ("soil_classes", C.c_int32), ("soil_map", C.c_int32),
                 ("recharge", C.c_int32), ("boundary_regions", C.c_int32), ("gw_initial_head", C.c_int32),
-                ("restart", C.c_int32), ("dt_checkpoint", C.c_double), ("max_checkpoints", C.c_int32)]
+                ("restart", C.c_int32), ("dt_checkpoint", C.c_double), ("max_checkpoints", C.c_int32),
+                ("solute_params", SoluteParams)]
 
 
 class CheckpointInfo(C.Structure):
     """frehg_checkpoint_info: what frehg_checkpoint_load restored."""
     _fields_ = [("t", C.c_double), ("gw_time", C.c_double), ("gw_dt", C.c_double), ("has_sw", C.c_int32),
                 ("has_gw", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
-                ("params_match", C.c_int32)]
+                ("params_match", C.c_int32), ("has_solute", C.c_int32)]
 
 
 class RunStats(C.Structure):
@@ -153,7 +154,9 @@ class RunStats(C.Structure):
     _fields_ = [("t", C.c_double), ("steps", C.c_int64), ("gw_substeps", C.c_int64), ("outputs", C.c_int32),
                 ("infos", C.c_int32), ("water_initial", C.c_double), ("water_final", C.c_double),
                 ("rain_volume", C.c_double), ("boundary_outflow", C.c_double), ("exchange_volume", C.c_double),
-                ("wall_seconds", C.c_double)]
+                ("wall_seconds", C.c_double), ("solute_initial_mass", C.c_double), ("solute_final_mass", C.c_double),
+                ("solute_added_by_rain", C.c_double), ("solute_decay_loss", C.c_double),
+                ("solute_relative_mass_error", C.c_double)]
 
 
 def soil_params(**kw):

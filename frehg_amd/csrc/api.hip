@@ -834,8 +834,10 @@ int frehg_checkpoint_save(frehg_ctx_t ctx, const char* path) {
       put(f.f, head, sizeof head, tmp);
       put(f.f, times, sizeof times, tmp);
       put(f.f, &digest, sizeof digest, tmp);
+      const bool sol_s = ctx->sol && ctx->swe, sol_g = ctx->sol && ctx->re;
       const int32_t nfields = (ctx->swe ? (int32_t)(sizeof kSweStateNames / sizeof(char*)) : 0) +
-                              (ctx->re ? (int32_t)(sizeof kReStateNames / sizeof(char*)) : 0);
+                              (ctx->re ? (int32_t)(sizeof kReStateNames / sizeof(char*)) : 0) + (sol_s ? 1 : 0) +
+                              (sol_g ? 1 : 0);
       put(f.f, &nfields, sizeof nfields, tmp);
       auto field = [&](const char* prefix, const char* name, size_t count, auto&& read) {
         const std::string key = std::string(prefix) + name;
@@ -855,6 +857,10 @@ int frehg_checkpoint_save(frehg_ctx_t ctx, const char* path) {
       if (ctx->re)
         for (const char* nm : kReStateNames)
           field("gw_", nm, n2 * (size_t)(g.nz + 2), [&](const char* n, double* d) { ctx->re->get_field(n, d); });
+      // solute_sw_conc / solute_gw_conc of writeCheckpointNow (:1461-1502)
+      if (sol_s) field("sl_", "surface", n2, [&](const char* n, double* d) { ctx->sol->get_field(n, d); });
+      if (sol_g)
+        field("sl_", "subsurface", n2 * (size_t)(g.nz + 2), [&](const char* n, double* d) { ctx->sol->get_field(n, d); });
       if (std::fflush(f.f) != 0) throw frehg::ArgError("frehg_checkpoint_save: write failed on '" + tmp + "'");
     }
     if (std::rename(tmp.c_str(), final_path.c_str()) != 0)
@@ -904,7 +910,18 @@ int frehg_checkpoint_load(frehg_ctx_t ctx, const char* path, frehg_checkpoint_in
       get(f.f, &key[0], (size_t)len, p);
       get(f.f, &cnt, sizeof cnt, p);
       const size_t n2 = (size_t)(g.nx + 2) * (g.ny + 2);
-      const size_t want = key.rfind("gw_", 0) == 0 ? n2 * (size_t)(g.nz + 2) : n2;
+      auto known = [&](const char* prefix, const char* const* names, size_t n) {
+        if (key.rfind(prefix, 0) != 0) return false;
+        for (size_t q2 = 0; q2 < n; ++q2)
+          if (key.compare(3, std::string::npos, names[q2]) == 0) return true;
+        return false;
+      };
+      const char* const kSolNames[] = {"surface", "subsurface"};
+      if (!known("sw_", kSweStateNames, sizeof kSweStateNames / sizeof(char*)) &&
+          !known("gw_", kReStateNames, sizeof kReStateNames / sizeof(char*)) && !known("sl_", kSolNames, 2))
+        throw frehg::ArgError("frehg_checkpoint_load: unknown field '" + key + "' in '" + p + "'");
+      const bool three = key.rfind("gw_", 0) == 0 || key == "sl_subsurface";
+      const size_t want = three ? n2 * (size_t)(g.nz + 2) : n2;
       if (cnt != (int64_t)want) throw frehg::ArgError("frehg_checkpoint_load: field '" + key + "' size mismatch");
       std::vector<double> buf((size_t)cnt);
       get(f.f, buf.data(), buf.size() * sizeof(double), p);
@@ -923,10 +940,18 @@ int frehg_checkpoint_load(frehg_ctx_t ctx, const char* path, frehg_checkpoint_in
       for (const char* nm : kReStateNames)
         if (!has(std::string("gw_") + nm))
           throw frehg::ArgError(std::string("frehg_checkpoint_load: missing field 'gw_") + nm + "'");
+    const bool has_solute = has("sl_surface") || has("sl_subsurface");
+    if (ctx->sol) {  // loadCheckpointState (:722-737)
+      if (!has_solute) throw frehg::ArgError("frehg_checkpoint_load: solute set up but the checkpoint has no solute state");
+      if (ctx->swe && !has("sl_surface")) throw frehg::ArgError("frehg_checkpoint_load: missing field 'sl_surface'");
+      if (ctx->re && !has("sl_subsurface")) throw frehg::ArgError("frehg_checkpoint_load: missing field 'sl_subsurface'");
+    }
     for (auto& kv : fields) {
       if (kv.first.rfind("sw_", 0) == 0) ctx->swe->set_field(kv.first.substr(3), kv.second.data());
-      else ctx->re->set_field(kv.first.substr(3), kv.second.data());
+      else if (kv.first.rfind("gw_", 0) == 0) ctx->re->set_field(kv.first.substr(3), kv.second.data());
+      else if (ctx->sol) ctx->sol->set_field(kv.first.substr(3), kv.second.data());
     }
+    if (ctx->sol) ctx->sol->reset_totals();
     ctx->t = times[0];
     ctx->gw_time = times[1];
     if (ctx->re) ctx->re->prime_dt(times[2]);
@@ -940,6 +965,7 @@ int frehg_checkpoint_load(frehg_ctx_t ctx, const char* path, frehg_checkpoint_in
       info->ny = head[4];
       info->nz = head[5];
       info->params_match = digest == params_digest(ctx) ? 1 : 0;
+      info->has_solute = has_solute ? 1 : 0;
     }
     return FREHG_OK;
   });

@@ -342,8 +342,8 @@ void gate_capabilities(const Config& c, bool sw, bool gw, bool solute) {
       }
     }
   }
+  (void)solute;  // the reference's gate takes it and checks nothing with it (:172)
   // ---- not implemented by this backend ----
-  if (solute) reject(src, "modules.solute", "solute transport is not implemented by this backend");
   const yaml::Node* srcs = c.find("sources");
   if (srcs && srcs->is_seq() && srcs->size() > 0)
     reject(src, "sources", "polygon sources (inflow_rate, rainfall_rate, extraction_well) are not implemented by this backend");
@@ -556,8 +556,9 @@ RunSetup build_setup(const Config& c) {
   s.mode = c.get_string("simulation.mode", "coupled");
   s.sw = c.get_bool("modules.surface_water", false);
   s.gw = c.get_bool("modules.groundwater", false);
-  s.solute = c.get_bool("modules.solute", false);
-  gate_capabilities(c, s.sw, s.gw, s.solute);
+  gate_capabilities(c, s.sw, s.gw, c.get_bool("modules.solute", false));
+  // solute.enabled is authoritative: modules.solute alone builds no solute (buildSolute, :749-760)
+  s.solute = c.get_bool("solute.enabled", false);
   if (s.mode == "surface_water" && !s.sw) throw ConfigError("Orchestrator: mode 'surface_water' requires modules.surface_water");
   if (s.mode == "groundwater" && !s.gw) throw ConfigError("Orchestrator: mode 'groundwater' requires modules.groundwater");
   if (!s.sw && !s.gw) throw ConfigError("Orchestrator: neither modules.surface_water nor modules.groundwater is enabled");
@@ -769,6 +770,27 @@ RunSetup build_setup(const Config& c) {
     s.cp.min_depth = c.get_double("surface_water.min_depth", 1.0e-8);
   }
 
+  // solute (SoluteParams::fromConfig, src/solute/SoluteParams.cpp:7-18; buildSolute, :749-783)
+  if (s.solute) {
+    frehg_solute_params& q = s.sol;
+    q.c_rain = c.get_double("solute.c_rain", 0.0);
+    q.k_decay = c.get_double("solute.k_decay", 0.0);
+    q.D = c.get_double("solute.D", 1.0e-9);
+    q.cfl_max = c.get_double("solute.cfl_max", 1.0);
+    q.min_depth = c.get_double("solute.min_depth", 1.0e-8);
+    q.start_time = c.get_double("solute.start_time", 0.0);
+    q.substeps = std::max(1, c.get_int("solute.substeps", 1));
+    // the stepper tests the scheme names for equality: anything but "muscl" advects upwind and
+    // anything but "implicit" does not diffuse (SoluteStepper.cpp, Advection.cpp:12)
+    q.muscl = c.get_string("solute.advection_scheme", "upwind") == "muscl" ? 1 : 0;
+    q.implicit_diffusion = c.get_string("solute.diffusion_scheme", "implicit") == "implicit" ? 1 : 0;
+    q.solver = parse_solver(c, "solute.solver", 1.0e-12, "jacobi");
+    if (q.solver.precision != FREHG_PREC_FP64)
+      reject(src, "solute.solver.precision", "must be fp64: the solute diffusion solve runs in FP64");
+    if (!(q.cfl_max > 0.0)) throw ConfigError("solute.cfl_max must be > 0");
+    if (q.D < 0.0 || q.k_decay < 0.0) throw ConfigError("solute.D and solute.k_decay must be >= 0");
+  }
+
   // boundary polygons (:630-656)
   const yaml::Node* bnds = c.find("boundaries");
   if (bnds && bnds->is_seq()) {
@@ -828,6 +850,14 @@ RunSetup build_setup(const Config& c) {
     s.gw_init_head = head_spatial;
     s.gw_init = head_spatial ? h : wc;
   }
+  if (s.solute) {  // initial_conditions.solute.surface / .subsurface (ICConfig.cpp:126-129, ICApply.cpp:228-249)
+    bool spatial = false;
+    if (s.sw) {
+      std::vector<double> v = ic_field(c, "initial_conditions.solute.surface", "", "", 0.0, s.nx, s.ny, 1, &spatial);
+      s.sol_surface.assign(v.begin(), v.begin() + (long)n2);
+    }
+    if (s.gw) s.sol_subsurface = ic_field(c, "initial_conditions.solute.subsurface", "", "", 0.0, s.nx, s.ny, nz, &spatial);
+  }
 
   // output selection (:1077-1175)
   const yaml::Node* vars = c.find("output.variables");
@@ -837,6 +867,10 @@ RunSetup build_setup(const Config& c) {
   if (!vars || !vars->is_seq() || vars->size() == 0) {
     if (s.sw) s.sw_out = {"water_depth", "eta"};
     if (s.gw) s.gw_out = {"head", "moisture"};
+    if (s.solute) {
+      if (s.sw) s.sw_out.push_back("concentration");
+      if (s.gw) s.gw_out.push_back("concentration");
+    }
   } else {
     for (size_t k = 0; k < vars->size(); ++k) {
       const std::string raw = vars->at(k)->as_string();
@@ -867,7 +901,10 @@ RunSetup build_setup(const Config& c) {
         need(s.gw, "groundwater");
         add(s.gw_out, name);
       } else if (name == "concentration" || name == "conc") {
-        throw ConfigError("output.variables: '" + raw + "' requires modules.solute (or solute.enabled) to be enabled");
+        if (!s.solute)
+          throw ConfigError("output.variables: '" + raw + "' requires modules.solute (or solute.enabled) to be enabled");
+        if (s.sw) add(s.sw_out, "concentration");
+        if (s.gw) add(s.gw_out, "concentration");
       } else {
         throw ConfigError("output.variables: unknown field '" + raw +
                           "'. Supported: water_depth, eta, u, v (surface); head, moisture, qx, qy, qz (subsurface); "

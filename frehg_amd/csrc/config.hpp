@@ -85,7 +85,11 @@ struct RunSetup {
   std::vector<double> eta0;  // physical initial eta per interior cell (surface water)
   bool gw_init_head = false;
   std::vector<double> gw_init;  // initial head or water content per interior cell
-  std::vector<std::string> sw_out, gw_out;  // "water_depth", "eta", "velocity_u", "velocity_v"; "head", "moisture", "qx"...
+  frehg_solute_params sol{};    // the `solute:` block (used when `solute` is set)
+  std::vector<double> sol_surface, sol_subsurface;  // initial concentrations per interior cell
+  // "water_depth", "eta", "velocity_u", "velocity_v", "concentration"; "head", "moisture", "qx"...,
+  // "concentration"
+  std::vector<std::string> sw_out, gw_out;
   std::string out_dir = ".", summary_path = "./simulation_summary.txt";
   std::string stem;  // output.filename without its extension ("" without a filename)
   bool write_fields = false;

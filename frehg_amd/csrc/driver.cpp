@@ -119,6 +119,19 @@ class Runner {
             frehg_re_set_initial(ctx_, s.gw_init_head ? s.gw_init.data() : nullptr,
                                  s.gw_init_head ? nullptr : s.gw_init.data()),
             "frehg_re_set_initial");
+    if (s.solute) {  // buildSolute (:749-783) and the solute initial conditions
+      check(ctx_, frehg_solute_init(ctx_, &s.sol), "frehg_solute_init");
+      check(ctx_,
+            frehg_solute_set_initial(ctx_, s.sol_surface.empty() ? nullptr : s.sol_surface.data(),
+                                     s.sol_subsurface.empty() ? nullptr : s.sol_subsurface.data()),
+            "frehg_solute_set_initial");
+    }
+  }
+
+  double solute_mass() {
+    double m = 0.0;
+    check(ctx_, frehg_solute_mass(ctx_, &m, nullptr), "frehg_solute_mass");
+    return m;
   }
 
   double volume() {
@@ -141,6 +154,7 @@ class Runner {
         std::vector<double> out((size_t)s.nx * s.ny);
         if (v == "velocity_u" || v == "velocity_v")
           check(ctx_, frehg_swe_get_field(ctx_, v == "velocity_u" ? "uu" : "vv", f.data()), "frehg_swe_get_field");
+        if (v == "concentration") check(ctx_, frehg_solute_get_field(ctx_, "surface", f.data()), "frehg_solute_get_field");
         for (int j = 0; j < s.ny; ++j)
           for (int i = 0; i < s.nx; ++i) {
             const size_t c = (size_t)(i + 1) + (size_t)(j + 1) * nsx;
@@ -150,21 +164,26 @@ class Runner {
             else x = f[c];
             out[(size_t)i + (size_t)j * s.nx] = x;
           }
-        write_file(v, t, out);
+        // the surface and subsurface concentrations sit in separate files (two HDF5 groups in
+        // the reference's writer)
+        write_file(v == "concentration" ? "surface_concentration" : v, t, out);
       }
     }
     if (s.gw && !s.gw_out.empty()) {
       std::vector<double> f(nsx * nsy * ((size_t)s.nz + 2));
       for (const std::string& v : s.gw_out) {
         const char* field = v == "head" ? "h" : v == "moisture" ? "wc" : v.c_str();
-        check(ctx_, frehg_re_get_field(ctx_, field, f.data()), "frehg_re_get_field");
+        if (v == "concentration")
+          check(ctx_, frehg_solute_get_field(ctx_, "subsurface", f.data()), "frehg_solute_get_field");
+        else
+          check(ctx_, frehg_re_get_field(ctx_, field, f.data()), "frehg_re_get_field");
         std::vector<double> out((size_t)s.nx * s.ny * s.nz);
         for (int k = 0; k < s.nz; ++k)
           for (int j = 0; j < s.ny; ++j)
             for (int i = 0; i < s.nx; ++i)
               out[(size_t)i + (size_t)j * s.nx + (size_t)k * s.nx * s.ny] =
                   f[(size_t)(i + 1) + (size_t)(j + 1) * nsx + (size_t)(k + 1) * nsx * nsy];
-        write_file(v, t, out);
+        write_file(v == "concentration" ? "subsurface_concentration" : v, t, out);
       }
     }
   }
@@ -208,6 +227,8 @@ class Runner {
       water0 = volume();
       write_outputs(0.0);
     }
+    // runLoopSequential (:1521): the solute mass after the initial conditions or the restart
+    const double sol0 = s.solute ? solute_mass() : 0.0;
     long long steps = 0, subs = 0;
     int infos = 0;
     const int rain_rows = s.ny - 1;  // the surface step adds rain to every row but the last (:1226-1233)
@@ -233,6 +254,7 @@ class Runner {
           outflow += out;
         }
         if (rain != 0.0) rain_in += rain * s.dt * s.dx * s.dy * (double)s.nx * rain_rows;
+        if (s.solute) check(ctx_, frehg_solute_apply(ctx_, t, s.dt, rain), "frehg_solute_apply");
         t = t_new;
       } else {
         // groundwater only: the Richards solver adapts its own dt (:1344-1352)
@@ -240,6 +262,7 @@ class Runner {
         const int rc = frehg_re_step(ctx_, &rs);
         check(ctx_, rc, "frehg_re_step");
         if (rc > 0) ++infos;
+        if (s.solute) check(ctx_, frehg_solute_apply(ctx_, t, rs.dt_used, 0.0), "frehg_solute_apply");
         t += rs.dt_used;
         ++subs;
       }
@@ -268,6 +291,17 @@ class Runner {
     out.boundary_outflow = outflow;
     out.exchange_volume = exch;
     out.wall_seconds = wall;
+    if (s.solute) {
+      frehg_solute_stats ss;
+      double m1 = 0.0;
+      check(ctx_, frehg_solute_mass(ctx_, &m1, &ss), "frehg_solute_mass");
+      out.solute_initial_mass = sol0;
+      out.solute_final_mass = m1;
+      out.solute_added_by_rain = ss.rain_added;
+      out.solute_decay_loss = ss.decay_loss;
+      const double expected = sol0 + ss.rain_added - ss.decay_loss;
+      out.solute_relative_mass_error = std::fabs(m1 - expected) / std::max(std::fabs(sol0 + ss.rain_added), 1.0e-300);
+    }
     write_summary(out);
     if (st) *st = out;
   }
@@ -315,6 +349,13 @@ class Runner {
     std::fprintf(f, "coupling_exchange_volume %.17g\n", r.exchange_volume);
     std::fprintf(f, "groundwater_present %s\n", b(s.gw));
     std::fprintf(f, "surface_water_present %s\n", b(s.sw));
+    if (s.solute) {
+      std::fprintf(f, "solute_initial_mass %.17g\n", r.solute_initial_mass);
+      std::fprintf(f, "solute_final_mass %.17g\n", r.solute_final_mass);
+      std::fprintf(f, "solute_added_by_rain %.17g\n", r.solute_added_by_rain);
+      std::fprintf(f, "solute_decay_loss %.17g\n", r.solute_decay_loss);
+      std::fprintf(f, "solute_relative_mass_error %.17g\n", r.solute_relative_mass_error);
+    }
     std::fprintf(f, "wall_clock_hours %.9g\n", r.wall_seconds / 3600.0);
     std::fclose(f);
   }
@@ -333,6 +374,7 @@ void fill_setup(const RunSetup& s, frehg_run_setup* o) {
   o->surface_water = s.sw;
   o->groundwater = s.gw;
   o->solute = s.solute;
+  o->solute_params = s.sol;
   o->coupled = s.coupled;
   o->nx = s.nx;
   o->ny = s.ny;
@@ -417,6 +459,8 @@ int64_t frehg_config_field(const char* yaml_path, const char* const* overrides, 
     else if (f == "soil_class") v.assign(s.class_idx.begin(), s.class_idx.end());
     else if (f == "rain_t") v = s.rain_t;
     else if (f == "rain_v") v = s.rain_v;
+    else if (f == "solute_surface") v = s.sol_surface;
+    else if (f == "solute_subsurface") v = s.sol_subsurface;
     else throw ConfigError("frehg_config_field: unknown field '" + f + "'");
     if (out && n > 0) std::copy(v.begin(), v.begin() + (long)std::min<int64_t>(n, (int64_t)v.size()), out);
     copy_msg(err, err_len, "");

@@ -256,8 +256,7 @@ Solute::Solute(const frehg_solute_params& p, SweState* swe, ReState* re, hipStre
   if (p.D < 0.0 || p.k_decay < 0.0) throw ArgError("frehg_solute_init: D and k_decay must be >= 0");
   if (p.substeps < 1) throw ArgError("frehg_solute_init: substeps must be >= 1");
   (void)krylov_config(p.solver, "frehg_solute_init");
-  if (p.solver.ksp_type != FREHG_KSP_CG || p.solver.precision != FREHG_PREC_FP64)
-    throw ArgError("frehg_solute_init: the diffusion solve is FP64 CG");
+  if (p.solver.precision != FREHG_PREC_FP64) throw ArgError("frehg_solute_init: the diffusion solve runs in FP64");
   try {
     if (swe) {
       gs_ = swe->geo();

@@ -44,6 +44,7 @@ class Solute {
   double mass();
 
   const frehg_solute_stats& totals() const { return tot_; }
+  void reset_totals() { tot_ = frehg_solute_stats{}; }
   const frehg_solute_params& params() const { return p_; }
 
  private:

@@ -364,7 +364,7 @@ typedef struct {
   int32_t muscl;      /* 0: first-order upwind, 1: minmod-limited MUSCL */
   int32_t implicit_diffusion; /* 0: none, 1: implicit */
   int32_t substeps;   /* substeps per flow step to start from (>= 1) */
-  frehg_solver_config solver; /* the diffusion solve: FP64 CG (reference rtol 1e-12) */
+  frehg_solver_config solver; /* the diffusion solve: FP64 CG or BiCGSTAB, Jacobi (reference: CG, rtol 1e-12) */
 } frehg_solute_params;
 
 typedef struct {
@@ -397,7 +397,10 @@ int frehg_solute_mass(frehg_ctx_t ctx, double* mass, frehg_solute_stats* totals)
  * a digest of the solver parameters, then every named field of each solver with its padded
  * (nx+2)(ny+2)[(nz+2)] values), written to path + ".tmp" and renamed over path. A load needs a
  * context set up with the same solvers and grid (else FREHG_ERR_INVALID_ARG); it restores the
- * fields, the coupled-step clocks and the groundwater dt. With the zero or previous-step Krylov
+ * fields, the coupled-step clocks and the groundwater dt. A context with a solute saves its
+ * concentrations too ("sl_surface", "sl_subsurface") and needs them to load (the solute totals
+ * start again from zero, as loadCheckpointState's, :722-745); a checkpoint's solute is ignored by
+ * a context without one. With the zero or previous-step Krylov
  * start and the same SpMV launch shape (FREHG_SPMV_SHAPE pins it) the continued run is
  * bit-identical to an uninterrupted one; the extrapolated and learned starts begin again from
  * h^n on the first step after a load. */
@@ -406,6 +409,7 @@ typedef struct {
   double gw_dt;         /* groundwater dt of the next step */
   int32_t has_sw, has_gw, nx, ny, nz;
   int32_t params_match; /* 1 if the context's solver parameters equal the saved ones */
+  int32_t has_solute;   /* 1 if the checkpoint holds solute concentrations */
 } frehg_checkpoint_info;
 
 int frehg_checkpoint_save(frehg_ctx_t ctx, const char* path);
@@ -445,6 +449,7 @@ typedef struct {
   int32_t restart;         /* 1: initial_conditions.type = restart (state from a checkpoint file) */
   double dt_checkpoint;    /* 0: no checkpoints */
   int32_t max_checkpoints;
+  frehg_solute_params solute_params; /* the solute: block (when solute = 1, i.e. solute.enabled) */
 } frehg_run_setup;
 
 typedef struct {
@@ -458,6 +463,9 @@ typedef struct {
   double boundary_outflow; /* net volume the surface boundary regions removed */
   double exchange_volume;  /* net GW -> SW seepage volume */
   double wall_seconds;
+  /* the solute ledger of writeSummary (:1666-1675); zero without a solute */
+  double solute_initial_mass, solute_final_mass, solute_added_by_rain, solute_decay_loss;
+  double solute_relative_mass_error; /* |final - (initial + rain - decay)| / |initial + rain| */
 } frehg_run_stats;
 
 /* Loads, gates and maps a configuration without a GPU (no context). err receives the message of
@@ -472,7 +480,7 @@ int frehg_config_query(const char* yaml_path, const char* const* overrides, int 
 /* A resolved per-cell input of a configuration (no GPU): "bathymetry" (nx*ny, plus the datum
  * offset), "eta0" (nx*ny, physical), "gw_initial" (nx*ny*nz heads or water contents),
  * "top_flux" (nx*ny), "soil_class" (nx*ny or nx*ny*nz), "rain_t" / "rain_v" (the rainfall
- * series); x fastest, then y, then z. Copies at most n values into out and returns the count the
+ * series), "solute_surface" (nx*ny) / "solute_subsurface" (nx*ny*nz) initial concentrations; x fastest, then y, then z. Copies at most n values into out and returns the count the
  * field holds (0 when the configuration has none), or FREHG_ERR_INVALID_ARG with the message in err. */
 int64_t frehg_config_field(const char* yaml_path, const char* const* overrides, int n_overrides, const char* name,
                            double* out, int64_t n, char* err, int err_len);

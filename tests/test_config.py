@@ -235,7 +235,6 @@ def test_gate_output_variables(tmp_path):
 
 
 @pytest.mark.parametrize("edit,key", [
-    (lambda t: t.replace("solute: false", "solute: true"), "modules.solute"),
     (lambda t: t + "monitors: {probes: [{i: 0, j: 0}]}\n", "monitors"),
     (lambda t: t.replace("{eta: 0.5}", "{eta: {type: formula, formula: 'x'}}"), "formula"),
     (lambda t: t.replace("{eta: 0.5}", "{eta: {type: polygon, default: 0}}"), "polygon"),
@@ -320,8 +319,8 @@ def test_cli_check_mode(tmp_path):
     exe = os.path.join(os.path.dirname(gpu.LIB_PATH), "frehg_run")
     r = subprocess.run([exe, "--check", B1], capture_output=True, text=True)
     assert r.returncode == 0 and "1x10x30" in r.stdout, r.stderr
-    r = subprocess.run([exe, "--check", "--set", "modules.solute=true", B1], capture_output=True, text=True)
-    assert r.returncode == 1 and "modules.solute" in r.stderr
+    r = subprocess.run([exe, "--check", "--set", "output.io_mode=parallel", B1], capture_output=True, text=True)
+    assert r.returncode == 1 and "io_mode" in r.stderr
 
 
 def test_scalar_conversions(tmp_path):
@@ -349,3 +348,43 @@ def test_checkpoint_and_restart_keys(tmp_path):
     p = write(tmp_path, sw_yaml().replace("initial_conditions: {", "initial_conditions: {type: restart, file: run.ckpt.5.bin, "))
     assert gpu.config_setup(p).restart == 1
     refused(tmp_path, sw_yaml().replace("initial_conditions: {", "initial_conditions: {type: restart, "), "requires file")
+
+
+# ---- the solute block -------------------------------------------------------------------------------
+def test_yaml_solute_defaults_overrides_and_enable_key(tmp_path):
+    """tests/solute/test_yaml_solute.cpp: defaults when keys are omitted, explicit values override
+    them, and no solute without the block; solute.enabled is the key that builds one (modules.solute
+    alone does not, Orchestrator.cpp:749-760)."""
+    base = sw_yaml()
+    s = gpu.config_setup(write(tmp_path, base + "solute: {enabled: true}\n"))
+    p = s.solute_params
+    assert s.solute == 1
+    assert (p.c_rain, p.k_decay, p.D, p.muscl, p.implicit_diffusion, p.cfl_max) == (0.0, 0.0, 1.0e-9, 0, 1, 1.0)
+    assert (p.min_depth, p.start_time, p.substeps) == (1.0e-8, 0.0, 1)
+    assert (p.solver.ksp_type, p.solver.rtol) == (abi.KSP_CG, 1.0e-12)
+    s = gpu.config_setup(write(tmp_path, base + "solute: {enabled: true, c_rain: 12.5, k_decay: 0.02, D: 1.0e-6, "
+                                             "advection_scheme: muscl, diffusion_scheme: implicit, cfl_max: 0.8, "
+                                             "substeps: 4, start_time: 60, solver: {rtol: 1.0e-10}}\n"))
+    p = s.solute_params
+    assert (p.c_rain, p.k_decay, p.D, p.muscl, p.cfl_max, p.substeps, p.start_time) == (12.5, 0.02, 1.0e-6, 1, 0.8, 4, 60.0)
+    assert p.solver.rtol == 1.0e-10
+    s = gpu.config_setup(write(tmp_path, base + "solute: {enabled: true, advection_scheme: tvd, diffusion_scheme: none}\n"))
+    assert (s.solute_params.muscl, s.solute_params.implicit_diffusion) == (0, 0)
+    assert gpu.config_setup(write(tmp_path, base)).solute == 0
+    assert gpu.config_setup(write(tmp_path, base.replace("solute: false", "solute: true"))).solute == 0
+    refused(tmp_path, base + "solute: {enabled: true, solver: {precision: mixed}}\n", "solute.solver.precision")
+
+
+def test_yaml_solute_initial_conditions_and_outputs(tmp_path):
+    """initial_conditions.solute.surface / .subsurface (src/ic/ICConfig.cpp:126-129) as a constant or
+    a file; `concentration` needs the solute (Orchestrator.cpp:1150-1155)."""
+    (tmp_path / "c0").write_text("\n".join(str(v) for v in range(9)))
+    text = sw_yaml().replace("initial_conditions: {surface_water: {eta: 0.5}}",
+                             "initial_conditions: {surface_water: {eta: 0.5}, solute: {surface: {type: file, file: c0}}}")
+    p = write(tmp_path, text + "solute: {enabled: true}\n")
+    np.testing.assert_array_equal(gpu.config_field(p, "solute_surface"), np.arange(9.0))
+    assert gpu.config_field(p, "solute_subsurface").size == 0
+    p = write(tmp_path, sw_yaml().replace("{eta: 0.5}}", "{eta: 0.5}, solute: {surface: 2.5}}") + "solute: {enabled: true}\n")
+    np.testing.assert_array_equal(gpu.config_field(p, "solute_surface"), np.full(9, 2.5))
+    refused(tmp_path, sw_yaml("", "0", "", "[water_depth, concentration]"), "solute.enabled")
+    gpu.config_setup(write(tmp_path, sw_yaml("", "0", "", "[water_depth, conc]") + "solute: {enabled: true}\n"))

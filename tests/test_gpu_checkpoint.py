@@ -68,6 +68,40 @@ def test_coupled_restart_is_bit_identical(tmp_path, pinned_shape, precision, gue
     assert not os.path.exists(str(ck) + ".tmp")
 
 
+def with_solute(ctx, case):
+    ctx.solute_init(abi.solute_params(muscl=1, c_rain=0.7, k_decay=1e-4, D=1e-4))
+    ctx.solute_set_initial(case.solute_patch(side=8.0), None)
+
+
+def test_coupled_restart_with_solute_is_bit_identical(tmp_path, pinned_shape):
+    """The solute concentrations travel with the checkpoint (solute_sw_conc / solute_gw_conc,
+    :722-737): a restarted run's concentrations equal the uninterrupted run's bit for bit. A
+    context with a solute refuses a checkpoint without one; one without a solute ignores them."""
+    say("checkpoint: coupled 24x20x8 with solute")
+    ck, plain = tmp_path / "sol.bin", tmp_path / "plain.bin"
+    case, a = coupled()
+    with_solute(a, case)
+    steps(case, a, 10)
+    a.checkpoint_save(ck)
+    steps(case, a, 10)
+    _, b = coupled()
+    with_solute(b, case)
+    b.solute_set_initial(np.full(case.nx * case.ny, 9.0), None)  # overwritten by the load
+    info = b.checkpoint_load(ck)
+    assert info.has_solute == 1 and b.solute_mass()[1].rain_added == 0.0
+    steps(case, b, 10)
+    for f in ("surface", "subsurface"):
+        np.testing.assert_array_equal(a.solute_get(f), b.solute_get(f), err_msg=f)
+    same_state(a, b, "restarted with solute")
+    _, c = coupled()
+    assert c.checkpoint_load(ck).has_solute == 1  # no solute here: the concentrations are ignored
+    c.checkpoint_save(plain)
+    _, d = coupled()
+    with_solute(d, case)
+    with pytest.raises(gpu.FrehgError, match="no solute state"):
+        d.checkpoint_load(plain)
+
+
 def test_groundwater_adaptive_dt_restart(tmp_path, pinned_shape):
     """The next dt of the adaptive rule travels with the checkpoint (primeAdaptiveDt)."""
     soil = abi.soil_params(alpha=1.43, n=1.56, theta_s=0.33, Ks_z=2.89e-6, Ss=1e-5)

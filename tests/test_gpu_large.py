@@ -1,8 +1,9 @@
-"""The large coupled configuration at its full size, flow only: a 1024x1024 surface at dx = 2 m
-over 1024x1024x64 soil at dz = 0.04 m (67M subsurface cells on one GPU), with the critical-depth
-outflow on the low-x edge. One coupled step against the FP64 oracle (heads and depths), then the
-FP64 water ledger of every step closes to round-off and the water volume moves only by rain,
-outflow and the ledger's own terms."""
+"""The large coupled configuration at its full size: a 1024x1024 surface at dx = 2 m over
+1024x1024x64 soil at dz = 0.04 m (67M subsurface cells on one GPU), with the critical-depth
+outflow on the low-x edge and the solute tracer (concentration 1 in a 100 x 100 m surface patch,
+molecular diffusion 1e-9 m^2/s, minmod MUSCL advection). One coupled step against the FP64
+oracle (heads, depths and both concentrations), then the FP64 water ledger of every step closes
+to round-off."""
 import numpy as np
 import pytest
 
@@ -20,6 +21,10 @@ def test_large_configuration_one_step_against_the_oracle_and_ledger():
     nx, ny, nz = case.nx, case.ny, case.nz
     ctx = gpu.Context(0)
     case.setup_gpu(ctx, abi.PREC_FP64, abi.GUESS_ZERO, picard_max=50)
+    sp = abi.solute_params(muscl=1, D=1e-9)
+    patch = case.solute_patch(side=100.0)
+    ctx.solute_init(sp)
+    ctx.solute_set_initial(patch, None)
     rain = case.rain * 200  # the first steps of the storm at a rate that ponds in one step
     st = ctx.coupled_step(case.cp, case.dt, rain, 0.0, case.t_end)
     assert st.gw.picard_converged == 1
@@ -32,7 +37,9 @@ def test_large_configuration_one_step_against_the_oracle_and_ledger():
     r = ol.OracleRe(nx, ny, nz, case.re_params(picard_max=50))
     r.set_soil([case.soil], None, True, case.ks.ravel())
     r.set_initial(h=case.h0)
-    oc = ol.OracleCoupled(s, r, case.cp, case.t_end)
+    sol = ol.OracleSolute(s, r, sp)
+    sol.set("surface", np.pad(patch.reshape(ny, nx), 1).ravel())
+    oc = ol.OracleCoupled(s, r, case.cp, case.t_end, solute=sol)
     oc.step(case.dt, rain)
     eh = float(np.abs(interior3(h1, nx, ny, nz) - interior3(r.get("h"), nx, ny, nz)).max())
     dd = interior2(s.get("dept"), nx, ny)
@@ -40,7 +47,12 @@ def test_large_configuration_one_step_against_the_oracle_and_ledger():
     say(f"  max |dh| {eh:.2e} m, max |d depth| {ed:.2e} m, ponded cells {(dd > 0).sum()}")
     assert eh <= 1e-9 and ed <= 1e-12
     assert (dd > 0).sum() > 1000
-    del s, r, oc
+    es = float(np.abs(interior2(ctx.solute_get("surface"), nx, ny) - interior2(sol.get("surface"), nx, ny)).max())
+    cg_o = interior3(sol.get("subsurface"), nx, ny, nz)[0]
+    eg = float(np.abs(interior3(ctx.solute_get("subsurface"), nx, ny, nz)[0] - cg_o).max())
+    say(f"  solute: max |dC| surface {es:.2e}, top soil {eg:.2e}; top soil cells reached {(cg_o > 0).sum()}")
+    assert es <= 1e-8 and eg <= 1e-8 and (cg_o > 0).sum() > 100
+    del s, r, sol, oc
     worst = abs(st.gw.mass_residual) / st.gw.mass_before
     for _ in range(3):
         st = ctx.coupled_step(case.cp, case.dt, rain, 0.0, case.t_end)
