@@ -573,6 +573,14 @@ def main():
                "gpu_fp64_max_head_err_vs_oracle_m": parity}
         log(f"[bench]   {cb['elapsed']:.1f} s on {cb['threads']} threads; GPU FP64 vs oracle max |dh| = {parity:.3e} m")
 
+    # the ceiling of the SpMV's access mix on this device: six reads and two writes per cell over
+    # arrays of the SpMV's size, no neighbours (frehg_stream_probe)
+    probe_gbs, probe_ms = gpu.stream_probe(case.nx * case.ny * case.nz, reps=50)
+    stream = {"achieved": round(probe_gbs, 1), "peak": 8000.0, "unit": "GB/s", "frac": round(probe_gbs / 8000.0, 4),
+              "avg_launch_us": round(probe_ms * 1e3, 2), "bytes_per_cell": 64,
+              "spmv_frac_of_probe": round(roof64["achieved"] / probe_gbs, 4) if roof64 else None}
+    log(f"[bench] streaming probe (6 reads + 2 writes per cell): {probe_gbs:.0f} GB/s = {probe_gbs / 80:.1f}% of peak")
+
     value = case.cells * args.steps / fast["elapsed"]
     plain_value = case.cells * args.steps / plain["elapsed"]
     out = {
@@ -640,6 +648,7 @@ def main():
         "roofline_kernel": "FP64 CG SpMV of the plain path (k_spmv_pair<double> at the autotuned shape): "
                            "fused p = z + beta p_prev, 7-point q = A p, partial p.q (64 B per cell)",
         "roofline_fp32_spmv": roof32,
+        "stream_probe_6r2w": stream,
         "roofline_mlp_guess": roof_guess,
         "spmv_shape_cells_per_lane_kchunk": {"plain_fp64": plain["spmv_shape"], "fast_fp32": fast["spmv_shape"]},
         "kernel_ms_per_step_fast": {k: round(v[0] / BREAKDOWN_STEPS, 4) for k, v in fast["kt"].items()},

@@ -22,7 +22,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
          "-I", os.path.join(HERE, "..", "include")]
 
 SOURCES = ["krylov.hip", "re.hip", "swe.hip", "coupling.hip", "mlp.hip", "ensemble.hip", "solute.hip", "noalloc.hip",
-           "api.hip"]
+           "probe.hip", "api.hip"]
 # Host-only C++ above the C ABI (configuration front end), built with g++.
 HOST_SOURCES = ["yaml.cpp", "config.cpp", "driver.cpp"]
 CXX = os.environ.get("CXX", "g++")

@@ -703,6 +703,15 @@ int frehg_mlp_increment(frehg_ctx_t ctx, double* dst) {
   });
 }
 
+int frehg_mlp_start(frehg_ctx_t ctx, double* dst) {
+  return guard(ctx, [&] {
+    if (!dst) throw frehg::ArgError("frehg_mlp_start: null output");
+    need_mlp(ctx);
+    need_re(ctx).learned_start(dst);
+    return FREHG_OK;
+  });
+}
+
 int frehg_mlp_predict(frehg_ctx_t ctx, const float* x, int n, float* out) {
   return guard(ctx, [&] {
     need_mlp(ctx).predict_batch(x, n, out);

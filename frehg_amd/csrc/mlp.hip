@@ -89,6 +89,12 @@ struct FeatSrc {
   double* bad;
   // Cells of the chunks the network ran on (counted per chunk); may be null.
   unsigned long long* inferred;
+  // Gated inference in two passes: masks[ch] = the cells of chunk ch whose correction the network
+  // supplies (k_guess_base writes them, and the base start into the output); null: one pass.
+  const unsigned long long* masks;
+  // The next call's inferred-cell counter, cleared by this call's inference kernel (the counters
+  // alternate, so that no memset sits between launches); may be null.
+  unsigned long long* inferred_next;
 };
 
 __device__ __forceinline__ double base_increment(const FeatSrc& F, long long c) {
@@ -477,8 +483,9 @@ __device__ __forceinline__ f32x4 mfma16(f16x8 a, f16x8 b, f32x4 c) {
 // r = 1 / (1 + 2^z) (z = s times the pre-activation); 0 for large z, 1 for very negative z.
 __device__ __forceinline__ float act_r(float z) { return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z)); }
 
+// Three waves per SIMD for the van Genuchten variant (at most 168 VGPRs, the weights in registers).
 template <bool VG_ONLY>
-__global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const float* norm, FeatSrc F, double* h_out) {
+__global__ __launch_bounds__(64 * GW) __attribute__((amdgpu_waves_per_eu(VG_ONLY ? 3 : 1))) void k_mlp_guess(const float* Q, const float* norm, FeatSrc F, double* h_out) {
   __shared__ __attribute__((aligned(16))) float xs[GW][RB][FP];
   __shared__ float ys[GW][RB];  // network outputs of the chunk's cells
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -517,101 +524,149 @@ __global__ __launch_bounds__(64 * GW) void k_mlp_guess(const float* Q, const flo
   const float ymu = norm[kMlpIn], ysd = norm[kMlpRow + kMlpIn];
   const Geo& g3 = F.g;
   const int iblocks = (g3.nx + RB - 1) / RB;
-  const long long nchunks = (long long)iblocks * g3.ny * g3.nz;
+  const int nchunks = iblocks * g3.ny * g3.nz;  // fits in 32 bits (checked on the host)
+  if (F.inferred_next && blockIdx.x == 0 && threadIdx.x == 0) *F.inferred_next = 0ull;
   float(*xw)[FP] = xs[w];
   unsigned long long inferred = 0;  // cells of the chunks this wave ran the network on
-  for (long long ch = (long long)blockIdx.x * GW + w; ch < nchunks; ch += (long long)gridDim.x * GW) {
-    const int row = (int)(ch / iblocks), ib = (int)(ch - (long long)row * iblocks);
-    const int j = row % g3.ny, k = row / g3.ny;
-    const int i = ib * RB + lane;
-    float f[kMlpIn];
-#pragma unroll
-    for (int t = 0; t < kMlpIn; ++t) f[t] = 0.0f;
-    long long c = 0;
-    double base = 0.0;  // h^n + base increment of this lane's cell
-    bool need = false;  // this lane's cell takes the network's correction
-    if (i < g3.nx) {
-      c = g3.idx(i, j, k);
-      const double inc = base_increment(F, c);
-      base = F.hn[c] + inc;
-      need = !(F.gate > 0.0 && F.inc_a) || fabs(inc) > F.gate;
+  // Wave wid takes chunks wid, wid + nw, wid + 2 nw, ... With masks, lane l first reads the mask
+  // of the l-th of the next 64 of them, and the wave visits only those with a cell to correct.
+  const int nw = (int)gridDim.x * GW, wid = (int)blockIdx.x * GW + w;
+  const int span = F.masks ? 64 : 1;
+  for (int it0 = wid; it0 < nchunks; it0 += nw * span) {
+    unsigned long long lm = 0ull, todo = 1ull;
+    if (F.masks) {
+      const long long cq = (long long)it0 + (long long)lane * nw;
+      lm = cq < nchunks ? F.masks[cq] : 0ull;
+      todo = __ballot(lm != 0ull);
     }
-    if (!__any(need)) {  // the whole chunk takes the extrapolated start (wave-uniform)
-      if (i < g3.nx) h_out[c] = base;
-      continue;
-    }
-    inferred += (unsigned long long)min(RB, g3.nx - ib * RB);  // per wave; one atomic per block below
-    if (i < g3.nx) {
-      cell_features<VG_ONLY>(F, i, j, k, f);
-      // standardised features, clamped far outside any training distribution so that the FP16
-      // operands below stay finite (heads of -1e5 m and beyond occur where the corrector drains a
-      // cell to theta_r)
+    while (todo) {
+      const int l = __ffsll((long long)todo) - 1;
+      todo &= todo - 1ull;
+      const int ch = it0 + l * nw;
+      const unsigned long long mask = F.masks ? __shfl(lm, l) : 0ull;
+      const int row = ch / iblocks, ib = ch - row * iblocks;
+      const int j = row % g3.ny, k = row / g3.ny;
+      const int i = ib * RB + lane;
+      float f[kMlpIn];
 #pragma unroll
-      for (int t = 0; t < kMlpIn; ++t) f[t] = fminf(fmaxf((f[t] - nmu[t]) * nisd[t], -1024.0f), 1024.0f);
-    }
-    wave_lds_fence();
+      for (int t = 0; t < kMlpIn; ++t) f[t] = 0.0f;
+      long long c = 0;
+      double base = 0.0;  // h^n + base increment of this lane's cell
+      bool need = false;  // this lane's cell takes the network's correction
+      if (i < g3.nx) {
+        c = g3.idx(i, j, k);
+        if (F.masks) {  // k_guess_base wrote h^n + base increment here (in place of the increment's inputs)
+          base = h_out[c];
+          need = ((mask >> lane) & 1ull) != 0ull;
+        } else {
+          const double inc = base_increment(F, c);
+          base = F.hn[c] + inc;
+          need = !(F.gate > 0.0 && F.inc_a) || fabs(inc) > F.gate;
+        }
+      }
+      if (!__any(need)) {  // the whole chunk takes the extrapolated start (wave-uniform)
+        if (i < g3.nx) h_out[c] = base;
+        continue;
+      }
+      inferred += (unsigned long long)min(RB, g3.nx - ib * RB);  // per wave; one atomic per block below
+      if (i < g3.nx) {
+        cell_features<VG_ONLY>(F, i, j, k, f);
+        // standardised features, clamped far outside any training distribution so that the FP16
+        // operands below stay finite (heads of -1e5 m and beyond occur where the corrector drains a
+        // cell to theta_r)
 #pragma unroll
-    for (int t = 0; t < kMlpIn; ++t) xw[lane][t] = f[t];
+        for (int t = 0; t < kMlpIn; ++t) f[t] = fminf(fmaxf((f[t] - nmu[t]) * nisd[t], -1024.0f), 1024.0f);
+      }
+      wave_lds_fence();
 #pragma unroll
-    for (int t = kMlpIn; t < FP; ++t) xw[lane][t] = 0.0f;
-    wave_lds_fence();
+      for (int t = 0; t < kMlpIn; ++t) xw[lane][t] = f[t];
+#pragma unroll
+      for (int t = kMlpIn; t < FP; ++t) xw[lane][t] = 0.0f;
+      wave_lds_fence();
 #pragma unroll 1
-    for (int tile = 0; tile < 4; ++tile) {
-      // layer 1: B = X^T, lane supplies features kg..kg+7 of cell 16 tile + ra (zero for kg >= 16)
-      f16x8 xb = pk16x8(0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f);
-      if (g < 2) {
-        const float* xr = &xw[16 * tile + ra][kg];
-        xb = pk16x8(xr[0], xr[1], xr[2], xr[3], xr[4], xr[5], xr[6], xr[7]);
+      for (int tile = 0; tile < 4; ++tile) {
+        // layer 1: B = X^T, lane supplies features kg..kg+7 of cell 16 tile + ra (zero for kg >= 16)
+        f16x8 xb = pk16x8(0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f);
+        if (g < 2) {
+          const float* xr = &xw[16 * tile + ra][kg];
+          xb = pk16x8(xr[0], xr[1], xr[2], xr[3], xr[4], xr[5], xr[6], xr[7]);
+        }
+        float a[4][4];
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+          const f32x4 z = mfma16(a1w[nb], xb, b1r[nb]);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) a[nb][r] = act_r(z[r]);
+        }
+        // layer 2: B[kappa][cell] = r1[pi(kappa)] from this lane's registers
+        f16x8 hb[2];
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+          hb[ks] = pk16x8(a[2 * ks][0], a[2 * ks][1], a[2 * ks][2], a[2 * ks][3], a[2 * ks + 1][0],
+                          a[2 * ks + 1][1], a[2 * ks + 1][2], a[2 * ks + 1][3]);
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+          f32x4 z = b2r[nb];
+#pragma unroll
+          for (int ks = 0; ks < 2; ++ks) z = mfma16(a2w[nb][ks], hb[ks], z);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) a[nb][r] = act_r(z[r]);
+        }
+        // layer 3: row 0 of W3s[pi] times R2^T
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+          hb[ks] = pk16x8(a[2 * ks][0], a[2 * ks][1], a[2 * ks][2], a[2 * ks][3], a[2 * ks + 1][0],
+                          a[2 * ks + 1][1], a[2 * ks + 1][2], a[2 * ks + 1][3]);
+        f32x4 y = b3r;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) y = mfma16(a3w[ks], hb[ks], y);
+        if (lane < 16) ys[w][16 * tile + ra] = y[0];  // D row 0 (g = 0, r = 0) holds y of cell ra
       }
-      float a[4][4];
-#pragma unroll
-      for (int nb = 0; nb < 4; ++nb) {
-        const f32x4 z = mfma16(a1w[nb], xb, b1r[nb]);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) a[nb][r] = act_r(z[r]);
+      wave_lds_fence();
+      if (i < g3.nx) {
+        const double y = (double)(ys[w][lane] * ysd + ymu);
+        const bool ok = isfinite(y);
+        if (!ok && F.bad) *F.bad = 1.0;
+        h_out[c] = base + ((need && ok) ? y : 0.0);
       }
-      // layer 2: B[kappa][cell] = r1[pi(kappa)] from this lane's registers
-      f16x8 hb[2];
-#pragma unroll
-      for (int ks = 0; ks < 2; ++ks)
-        hb[ks] = pk16x8(a[2 * ks][0], a[2 * ks][1], a[2 * ks][2], a[2 * ks][3], a[2 * ks + 1][0],
-                        a[2 * ks + 1][1], a[2 * ks + 1][2], a[2 * ks + 1][3]);
-#pragma unroll
-      for (int nb = 0; nb < 4; ++nb) {
-        f32x4 z = b2r[nb];
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) z = mfma16(a2w[nb][ks], hb[ks], z);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) a[nb][r] = act_r(z[r]);
-      }
-      // layer 3: row 0 of W3s[pi] times R2^T
-#pragma unroll
-      for (int ks = 0; ks < 2; ++ks)
-        hb[ks] = pk16x8(a[2 * ks][0], a[2 * ks][1], a[2 * ks][2], a[2 * ks][3], a[2 * ks + 1][0],
-                        a[2 * ks + 1][1], a[2 * ks + 1][2], a[2 * ks + 1][3]);
-      f32x4 y = b3r;
-#pragma unroll
-      for (int ks = 0; ks < 2; ++ks) y = mfma16(a3w[ks], hb[ks], y);
-      if (lane < 16) ys[w][16 * tile + ra] = y[0];  // D row 0 (g = 0, r = 0) holds y of cell ra
-    }
-    wave_lds_fence();
-    if (i < g3.nx) {
-      const double y = (double)(ys[w][lane] * ysd + ymu);
-      const bool ok = isfinite(y);
-      if (!ok && F.bad) *F.bad = 1.0;
-      h_out[c] = base + ((need && ok) ? y : 0.0);
     }
   }
   if (F.inferred) {
-    __shared__ unsigned long long nw[GW];
-    if (lane == 0) nw[w] = inferred;
+    __shared__ unsigned long long wsum[GW];
+    if (lane == 0) wsum[w] = inferred;
     __syncthreads();
     if (threadIdx.x == 0) {
       unsigned long long t = 0;
-      for (int q = 0; q < GW; ++q) t += nw[q];
+      for (int q = 0; q < GW; ++q) t += wsum[q];
       if (t) atomicAdd(F.inferred, t);
     }
   }
+}
+
+// The gated start's first pass: every interior cell gets the extrapolated start h^n + base
+// increment, and masks[ch] the cells of chunk ch (RB cells along x, one wave) whose |previous
+// increment| exceeds the gate. A light streaming pass at full occupancy, so that the inference's
+// register-heavy waves visit only the chunks with a cell to correct. The increment's inputs may be
+// the output array itself, so the inference kernel takes the base from the output and the cells
+// from the mask.
+__global__ __launch_bounds__(MT) void k_guess_base(FeatSrc F, double* h_out, unsigned long long* masks) {
+  const int lane = threadIdx.x & 63;
+  const Geo& g3 = F.g;
+  const int iblocks = (g3.nx + RB - 1) / RB;
+  const long long ch = (long long)blockIdx.x * (MT / 64) + (threadIdx.x >> 6);
+  if (ch >= (long long)iblocks * g3.ny * g3.nz) return;  // wave-uniform
+  const int row = (int)(ch / iblocks), ib = (int)(ch - (long long)row * iblocks);
+  const int j = row % g3.ny, k = row / g3.ny;
+  const int i = ib * RB + lane;
+  bool need = false;
+  if (i < g3.nx) {
+    const long long c = g3.idx(i, j, k);
+    const double inc = base_increment(F, c);
+    h_out[c] = F.hn[c] + inc;
+    need = fabs(inc) > F.gate;
+  }
+  const unsigned long long m = __ballot(need);
+  if (lane == 0) masks[ch] = m;
 }
 
 // Log (features, h - h^n - base increment) of interior cells q with q % stride == phase into the
@@ -667,6 +722,8 @@ FeatSrc feat_src(ReState& re, const double* depth) {
   F.gate = 0.0;
   F.bad = nullptr;
   F.inferred = nullptr;
+  F.masks = nullptr;
+  F.inferred_next = nullptr;
   return F;
 }
 
@@ -684,8 +741,8 @@ Mlp::Mlp(hipStream_t s, Profiler* prof) : s_(s), prof_(prof) {
   FREHG_HIP(hipMalloc(&part_, (size_t)TRAIN_BLOCKS * (kMlpParams + 1) * sizeof(float)));
   FREHG_HIP(hipMalloc(&stat_, (size_t)kStatBlocks * 2 * kMlpRow * sizeof(double)));
   FREHG_HIP(hipHostMalloc(&hbuf_, (kMlpParams + 1) * sizeof(float), hipHostMallocDefault));
-  FREHG_HIP(hipMalloc(&inferred_, sizeof(unsigned long long)));
-  FREHG_HIP(hipMemsetAsync(inferred_, 0, sizeof(unsigned long long), s_));
+  FREHG_HIP(hipMalloc(&inferred_, 2 * sizeof(unsigned long long)));
+  FREHG_HIP(hipMemsetAsync(inferred_, 0, 2 * sizeof(unsigned long long), s_));
   // The guess kernel's blocks loop over chunks of cells: one round of resident blocks, so that
   // no partial last round idles most of the chip and each block loads the weights once.
   int dev = 0, cus = 0, per_cu = 0;
@@ -698,14 +755,14 @@ Mlp::Mlp(hipStream_t s, Profiler* prof) : s_(s), prof_(prof) {
 
 Mlp::~Mlp() {
   for (void* p : {(void*)params_, (void*)infer_, (void*)m_, (void*)v_, (void*)norm_, (void*)grad_, (void*)part_, (void*)log_,
-                  (void*)scratch_, (void*)stat_, (void*)inferred_})
+                  (void*)scratch_, (void*)stat_, (void*)inferred_, (void*)masks_})
     if (p) (void)hipFree(p);
   if (hbuf_) (void)hipHostFree(hbuf_);
 }
 
 long long Mlp::inferred_cells() {
   unsigned long long n = 0;
-  FREHG_HIP(hipMemcpyAsync(&n, inferred_, sizeof(n), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipMemcpyAsync(&n, inferred_ + last_par_, sizeof(n), hipMemcpyDeviceToHost, s_));
   FREHG_HIP(hipStreamSynchronize(s_));
   return (long long)n;
 }
@@ -838,14 +895,18 @@ void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const d
   F.inc_b = inc_b;
   F.gate = gate_;
   F.bad = bad;
-  F.inferred = inferred_;
-  FREHG_HIP(hipMemsetAsync(inferred_, 0, sizeof(unsigned long long), s_));
+  // The counters alternate between two slots from call to call: this call uses slot `par` (cleared
+  // by the previous call's inference kernel, or at set-up) and its inference kernel clears the other
+  // one for the next call, so no memset sits between the launches.
+  const int par = 1 - last_par_;
+  last_par_ = par;
+  F.inferred = inferred_ + par;
+  F.inferred_next = inferred_ + (1 - par);
   if (hn) F.hn = hn;
   if (wcn) F.wcn = wcn;
-  const long long N = (long long)F.g.nx * F.g.ny * F.g.nz;
   const long long nchunks = (long long)((F.g.nx + RB - 1) / RB) * F.g.ny * F.g.nz;
+  if (nchunks >= (1LL << 31) - (long long)guess_blocks_ * GW) throw ArgError("learned guess: grid too large");
   const unsigned nb = (unsigned)std::min<long long>(guess_blocks_, (nchunks + GW - 1) / GW);
-  (void)N;
   if (infer_stale_) {
     hipLaunchKernelGGL(k_mlp_prep_infer, dim3(1), dim3(MT), 0, s_, (const float*)params_, infer_);
     FREHG_LAUNCH_CHECK();
@@ -853,6 +914,21 @@ void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const d
   }
   hipEvent_t ev0 = prof_->start(FREHG_KT_GW_GUESS);
   hipEvent_t ev1 = prof_->stop();
+  const bool gated = F.gate > 0.0 && F.inc_a != nullptr;
+  if (gated) {  // the base pass marks the cells that need the network; the inference visits only their chunks
+    if (masks_cap_ < nchunks) {
+      if (masks_) FREHG_HIP(hipFree(masks_));
+      masks_ = nullptr;
+      masks_cap_ = 0;
+      FREHG_HIP(hipMalloc(&masks_, (size_t)nchunks * sizeof(unsigned long long)));
+      masks_cap_ = nchunks;
+    }
+    hipExtLaunchKernelGGL(k_guess_base, dim3(ceil_div(nchunks, MT / 64)), dim3(MT), 0, s_, ev0, nullptr, 0, F, h_out,
+                          masks_);
+    FREHG_LAUNCH_CHECK();
+    ev0 = nullptr;
+    F.masks = masks_;
+  }
   if (re.soil_plain_vg())
     hipExtLaunchKernelGGL(k_mlp_guess<true>, dim3(nb), dim3(64 * GW), 0, s_, ev0, ev1, 0, (const float*)infer_,
                           (const float*)norm_, F, h_out);

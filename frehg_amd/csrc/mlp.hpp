@@ -93,7 +93,10 @@ class Mlp {
   bool infer_stale_ = true;
   unsigned guess_blocks_ = 4096;  // guess kernel grid: resident blocks of one round
   double gate_ = 0.0;
-  unsigned long long* inferred_ = nullptr;  // device counter of the last guess
+  unsigned long long* inferred_ = nullptr;  // device counters of inferred cells (two alternating slots)
+  unsigned long long* masks_ = nullptr;      // per chunk: the cells the gated inference corrects
+  long long masks_cap_ = 0;
+  int last_par_ = 1;                         // slot of the last guess's counters
   int stride_ = 0;
   long long cap_ = 0, count_ = 0, log_steps_ = 0;
   int adam_t_ = 0;

@@ -494,6 +494,22 @@ void ReState::learned_increment(double* dst) {
       }
 }
 
+void ReState::learned_start(double* dst) {
+  if (!mlp_ || !mlp_->ready()) throw ArgError("frehg_mlp_start: no trained model");
+  // The next step swaps h and h^n first: its h^n is h_ now, and its previous solve increment is
+  // the post-solve head hp_ minus the last step's starting head, now in hn_.
+  DeviceBuffer<double> tmp((size_t)n_);
+  FREHG_HIP(hipMemsetAsync(tmp.get(), 0, (size_t)n_ * sizeof(double), s_));
+  mlp_->predict_guess(*this, tmp.get(), depth_, inc_valid_ ? hp_ : nullptr, inc_valid_ ? hn_ : nullptr, h_, wc_);
+  std::vector<double> pad((size_t)n_);
+  FREHG_HIP(hipMemcpyAsync(pad.data(), tmp.get(), (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
+  FREHG_HIP(hipStreamSynchronize(s_));
+  for (int k = 0; k < g_.nz; ++k)
+    for (int j = 0; j < g_.ny; ++j)
+      for (int i = 0; i < g_.nx; ++i)
+        dst[(size_t)i + (size_t)j * g_.nx + (size_t)k * g_.nx * g_.ny] = pad[(size_t)g_.idx(i, j, k)];
+}
+
 void ReState::materialize_fluxes() {
   if (!flux_pending_) return;
   const ReK P = make_rek(g_, soil_view(), p_, qtop_);

@@ -64,6 +64,8 @@ class ReState {
   // Learned increment h^{n+1} - h^n the model predicts for a step taken from the current state
   // (features from h, K/C at h, theta), interior cells in i + j nx + k nx ny order.
   void learned_increment(double* dst);
+  // The learned start the next step would take (h^n + d^{n-1} + the gated correction), interior.
+  void learned_start(double* dst);
   // Learned-guess model and the surface depth it reads (either may be null).
   void set_mlp(Mlp* m, const double* surface_depth) {
     mlp_ = m;

@@ -91,6 +91,7 @@ def load():
         "frehg_mlp_predict": (C.c_int, [ctx, P(C.c_float), C.c_int, P(C.c_float)]),
         "frehg_mlp_increment": (C.c_int, [ctx, dp]),
         "frehg_mlp_gate": (C.c_int, [ctx, C.c_double]),
+        "frehg_mlp_start": (C.c_int, [ctx, dp]),
         "frehg_mlp_inferred_cells": (C.c_int, [ctx, P(C.c_int64)]),
         "frehg_ls_solve_noalloc": (C.c_int, [ctx, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                              P(C.c_void_p), C.c_void_p, C.c_void_p, P(abi.SolverConfig),
@@ -104,6 +105,7 @@ def load():
         "frehg_solute_get_field": (C.c_int, [ctx, C.c_char_p, dp]),
         "frehg_solute_apply": (C.c_int, [ctx, C.c_double, C.c_double, C.c_double]),
         "frehg_solute_mass": (C.c_int, [ctx, dp, P(abi.SoluteStats)]),
+        "frehg_stream_probe": (C.c_int, [C.c_int, C.c_int64, C.c_int, dp, dp]),
         "frehg_checkpoint_save": (C.c_int, [ctx, C.c_char_p]),
         "frehg_checkpoint_load": (C.c_int, [ctx, C.c_char_p, P(abi.CheckpointInfo)]),
         "frehg_config_field": (C.c_int64, [C.c_char_p, P(C.c_char_p), C.c_int, C.c_char_p, dp, C.c_int64,
@@ -154,6 +156,15 @@ def config_field(path, name, overrides=()):
     if cnt:
         load().frehg_config_field(str(path).encode(), arr, n, name.encode(), out.ctypes.data_as(dp), cnt, err, len(err))
     return out
+
+
+def stream_probe(cells, reps=50, device=0):
+    """frehg_stream_probe: (GB/s, ms per launch) of the SpMV's six-read / two-write mix."""
+    gbs, ms = C.c_double(), C.c_double()
+    rc = load().frehg_stream_probe(device, int(cells), int(reps), C.byref(gbs), C.byref(ms))
+    if rc != 0:
+        raise FrehgError(rc, "frehg_stream_probe failed")
+    return gbs.value, ms.value
 
 
 def config_query(path, key, overrides=()):
@@ -500,6 +511,12 @@ class Context:
     def mlp_increment(self):
         out = np.zeros(int(np.prod(self.re_dims)))
         self.check(self.lib.frehg_mlp_increment(self.h, out.ctypes.data_as(dp)))
+        return out
+
+    def mlp_start(self):
+        """frehg_mlp_start: the learned start the next step would take (interior, x fastest)."""
+        out = np.zeros(int(np.prod(self.re_dims)))
+        self.check(self.lib.frehg_mlp_start(self.h, out.ctypes.data_as(dp)))
         return out
 
     def spmv_shape(self, precision):

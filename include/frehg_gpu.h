@@ -315,6 +315,11 @@ int frehg_mlp_inferred_cells(frehg_ctx_t ctx, int64_t* cells);
  * K(h), C(h), theta): the part of the learned guess's increment on top of the previous step's
  * solve increment. nx*ny*nz values, i + j*nx + k*nx*ny. */
 int frehg_mlp_increment(frehg_ctx_t ctx, double* dst);
+/* The learned start the next frehg_re_step (or coupled step) would take from the current state:
+ * h^n + d^{n-1} + the network's correction where |d^{n-1}| exceeds the gate, h^n + d^{n-1}
+ * elsewhere (h^n + the correction when no previous increment is known). nx*ny*nz values,
+ * i + j*nx + k*nx*ny. Inspection of the learned start; no counterpart in the reference. */
+int frehg_mlp_start(frehg_ctx_t ctx, double* dst);
 
 /* ---- measurement --------------------------------------------------------------------------- */
 /* Kernel-duration accounting with HIP events on the context's stream. kernel ids below. */
@@ -344,6 +349,13 @@ int frehg_profile_reset(frehg_ctx_t ctx);
  * at the first solve, or pinned by the environment variable FREHG_SPMV_SHAPE="cpl:kchunk" read at
  * frehg_re_init. No counterpart in the reference (PETSc's MatMult has no launch shape). */
 int frehg_spmv_shape(frehg_ctx_t ctx, int precision, int* cells_per_lane, int* kchunk);
+/* Streaming probe on `device`: the CG SpMV's HBM access mix (six FP64 reads and two FP64 writes
+ * per cell, 16-byte accesses) with no neighbours and no reduction, over eight arrays of `cells`
+ * doubles that it allocates and frees; for each of a few grid sizes, `reps` timed launches after
+ * one warm-up. Returns the rate in GB/s of those 64 bytes per cell and the mean launch time of the
+ * fastest grid size. A measurement aid with no
+ * counterpart in the reference: the ceiling the SpMV's access mix streams at on this device. */
+int frehg_stream_probe(int device, int64_t cells, int reps, double* gbytes_per_s, double* ms_per_launch);
 
 /* ---- Passive solute: SoluteParams / SoluteStepper / Orchestrator::applySolute
  * (src/solute/SoluteParams.hpp:16-30, src/solute/SoluteStepper.cpp:24-72,

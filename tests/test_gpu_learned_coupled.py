@@ -23,6 +23,11 @@ def case(seed_bed, seed_ks):
     return workloads.CoupledCase(nx=N, ny=N, nz=NZ, seed_bed=seed_bed, seed_ks=seed_ks, rain=RAIN)
 
 
+@pytest.fixture(scope="module")
+def model():
+    return train()
+
+
 def train():
     ctx = gpu.Context(0)
     ctx.mlp_init(17)
@@ -62,9 +67,8 @@ def run(held, prec, guess, model=None):
     return out
 
 
-def test_learned_guess_in_the_coupled_step_on_a_held_out_realisation():
+def test_learned_guess_in_the_coupled_step_on_a_held_out_realisation(model):
     say("learned guess in the coupled step: 64x64x16, held-out realisation")
-    model = train()
     held = case(7, 8)
     h_l, d_l, it_l, fb_l, pic_l, inferred = run(held, abi.PREC_MIXED, abi.GUESS_LEARNED, model)
     h_e, d_e, it_e, fb_e, pic_e, _ = run(held, abi.PREC_MIXED, abi.GUESS_EXTRAPOLATED)
@@ -91,3 +95,36 @@ def test_learned_guess_in_the_coupled_step_on_a_held_out_realisation():
     assert fb_l == 0 and fb_e == 0
     assert inferred > 0
     assert it_l < it_e
+
+
+def test_gated_start_equals_the_network_where_the_gate_opens(model):
+    """The gated learned start (a base pass that lists the chunks with a cell above the gate, then
+    the network over those chunks only) equals, bit for bit, the network's start where the previous
+    solve increment exceeds the gate and the extrapolated start hn + d elsewhere; the network's
+    start is the same with the gate at 0, and the extrapolated start with an infinite gate."""
+    held = case(7, 8)
+    ctx = gpu.Context(0)
+    ctx.mlp_set_params(*model)
+    held.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_LEARNED, picard_max=PICARD)
+    for _ in range(6):
+        ctx.coupled_step(held.cp, held.dt, held.rain, 0.0, held.t_end)
+    hn = interior3(ctx.re_get("h"), N, N, NZ).ravel()
+    ctx.mlp_gate(0.0)
+    full = ctx.mlp_start()
+    ctx.mlp_gate(1e30)
+    base = ctx.mlp_start()
+    assert ctx.mlp_inferred_cells() == 0
+    gate = 1e-6
+    ctx.mlp_gate(gate)
+    gated = ctx.mlp_start()
+    inferred = ctx.mlp_inferred_cells()
+    inc = np.abs(base - hn)  # |d| up to the rounding of hn + d
+    above = inc > gate * (1 + 1e-6) + 1e-13
+    below = inc < gate * (1 - 1e-6) - 1e-13
+    say(f"  cells above the gate {above.sum()}, below {below.sum()}, in inferred chunks {inferred}")
+    assert 0 < above.sum() < above.size and 0 < inferred < N * N * NZ
+    np.testing.assert_array_equal(gated[above], full[above])
+    np.testing.assert_array_equal(gated[below], base[below])
+    assert np.all((gated == full) | (gated == base))
+    assert np.any(full[above] != base[above])
+    ctx.close()
