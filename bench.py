@@ -180,19 +180,22 @@ def bench_ensemble(args):
     from frehg_amd import abi, gpu, workloads
     log("[bench] generating the 96-box ensemble")
     case = workloads.EnsembleCase(nbox=96, seed=3)
+    case_ex = workloads.EnsembleCase(nbox=96, seed=3, guess=abi.GUESS_EXTRAPOLATED)
     steps, warmup = args.steps, args.warmup
     res = {}
-    for mode in ("graph", "eager"):
+    for mode in ("graph", "eager", "graph_extrapolated"):
+        run_case = case_ex if mode == "graph_extrapolated" else case
+        use_graph = mode != "eager"
         ctx = gpu.Context(0)
-        case.setup_gpu(ctx)
+        run_case.setup_gpu(ctx)
         for _ in range(warmup):
-            ctx.ens_step(use_graph=(mode == "graph"))
+            ctx.ens_step(use_graph=use_graph)
         ctx.synchronize()
         worst = 0.0
         its = refs = extra = 0
         t0 = time.perf_counter()
         for _ in range(steps):
-            st = ctx.ens_step(use_graph=(mode == "graph"))
+            st = ctx.ens_step(use_graph=use_graph)
             worst = max(worst, st.max_rel_mass_residual)
             its = max(its, st.max_iterations)
             refs = max(refs, st.max_refinements)
@@ -227,6 +230,7 @@ def bench_ensemble(args):
                                   "top flux 1e-6 m/s, dt 30 s, FP32-in-FP64 refinement, previous-step guess",
                       "cells": case.cells, "seed": 3},
            "graph": res["graph"], "eager": res["eager"],
+           "graph_extrapolated_start": res["graph_extrapolated"],
            "parity_smallest_box": {"box": b, "dims": case.dims[b], "max_head_err_vs_fp64_oracle_m": err}}
     print(json.dumps(out), flush=True)
 

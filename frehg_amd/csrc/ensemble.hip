@@ -28,6 +28,8 @@ constexpr int NSUM = 6;  // ledger partials per tile: before, storage, boundary,
 
 struct Pools {
   double *h, *hn, *wc, *wcn, *wch, *hwc, *ch, *kr, *Kx, *Ky, *Kz, *qx, *qy, *qz, *D, *ax, *ay, *az, *rhs;
+  // extrapolated start only: the post-solve head of the last step, then the start of the next
+  double* hp = nullptr;
 };
 
 __device__ __forceinline__ int find_box(const unsigned int* t0, int nbox, unsigned int tile) {
@@ -169,19 +171,34 @@ __global__ __launch_bounds__(NT) void kb_assemble(const EnsBox* boxes, const uns
               D32 ? D32 + o : nullptr, ax32 + o, ay32 + o, az32 + o);
 }
 
-// x0 = h^n (previous-step guess) or 0 on interior cells.
-__global__ __launch_bounds__(NT) void kb_guess(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F,
-                                                int previous) {
+// kb_save for the extrapolated start with a previous increment: before h^n is overwritten (it
+// still holds the last step's starting head), the start h^n + (post-solve head - that head) goes
+// to hp, in the single-box arithmetic (k_extrap_interior).
+__global__ __launch_bounds__(NT) void kb_save_extrap(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F) {
   const BoxCell q = box_cell(boxes, t0, nbox);
   if (!q.t.valid) return;
   const long long c = q.off + q.t.c;
-  F.h[c] = previous ? F.hn[c] : 0.0;
+  const double h = F.h[c];
+  F.hp[c] = h + (F.hp[c] - F.hn[c]);
+  F.hn[c] = h;
+  F.wcn[c] = F.wc[c];
+}
+
+// x0 on interior cells: 0 (mode 0), h^n (1), or the extrapolated start kept in hp (2).
+__global__ __launch_bounds__(NT) void kb_guess(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F,
+                                                int mode) {
+  const BoxCell q = box_cell(boxes, t0, nbox);
+  if (!q.t.valid) return;
+  const long long c = q.off + q.t.c;
+  F.h[c] = mode == 2 ? F.hp[c] : (mode == 1 ? F.hn[c] : 0.0);
 }
 
 template <class K, int ZF>
 __global__ __launch_bounds__(NT) void kb_post(const EnsBox* boxes, const unsigned int* t0, int nbox, Pools F) {
   const BoxCellT<K> q = box_cell_as<K, ZF>(boxes, t0, nbox);
-  if (q.t.valid) re_post(q.P, q.t, F.h + q.off, F.kr + q.off, F.ch + q.off, F.wch + q.off);
+  if (!q.t.valid) return;
+  if (F.hp) F.hp[q.off + q.t.c] = F.h[q.off + q.t.c];  // the post-solve head (extrapolated start)
+  re_post(q.P, q.t, F.h + q.off, F.kr + q.off, F.ch + q.off, F.wch + q.off);
 }
 
 // Fluxes at h^{n+1} and the corrector in one pass (re_flux_correct).
@@ -630,9 +647,10 @@ void Ensemble::build(int nbox, const int32_t* dims, const frehg_re_params* param
   kc_ = krylov_config(p0.solver, "frehg_ens_init");
   if (kc_.ksp != FREHG_KSP_CG) throw ArgError("frehg_ens_init: the ensemble solves with CG");
   mixed_ = kc_.precision == 1;
-  if (p0.initial_guess != FREHG_GUESS_ZERO && p0.initial_guess != FREHG_GUESS_PREVIOUS)
+  if (p0.initial_guess != FREHG_GUESS_ZERO && p0.initial_guess != FREHG_GUESS_PREVIOUS &&
+      p0.initial_guess != FREHG_GUESS_EXTRAPOLATED)
     throw ArgError("frehg_ens_init: initial guess must be zero or previous-step");
-  prev_guess_ = p0.initial_guess == FREHG_GUESS_PREVIOUS;
+  guess_ = p0.initial_guess;
   hbox_.resize(nbox);
   tile0_.assign(nbox + 1, 0);
   spmv0_.assign(nbox + 1, 0);
@@ -681,6 +699,7 @@ void Ensemble::build(int nbox, const int32_t* dims, const frehg_re_params* param
   for (double** p : {&F_.h, &F_.hn, &F_.wc, &F_.wcn, &F_.wch, &F_.hwc, &F_.ch, &F_.kr, &F_.Kx, &F_.Ky, &F_.Kz, &F_.qx,
                      &F_.qy, &F_.qz, &F_.D, &F_.ax, &F_.ay, &F_.az, &F_.rhs, &F_.r, &F_.z, &F_.p0, &F_.p1, &F_.q})
     dalloc(p);
+  if (guess_ == FREHG_GUESS_EXTRAPOLATED) dalloc(&F_.hp);
   if (mixed_) {
     dalloc(&F_.xa);
     for (float** p : {&F_.D32, &F_.ax32, &F_.ay32, &F_.az32, &F_.b32, &F_.r32, &F_.z32, &F_.p032, &F_.p132, &F_.q32,
@@ -716,7 +735,7 @@ void Ensemble::release() {
                              F_.Kz,  F_.qx,   F_.qy,  F_.qz,   F_.D,    F_.ax,   F_.ay,   F_.az,   F_.rhs,  F_.r,
                              F_.z,   F_.p0,   F_.p1,  F_.q,    F_.xa,   F_.D32,  F_.ax32, F_.ay32, F_.az32, F_.b32,
                              F_.r32, F_.z32,  F_.p032, F_.p132, F_.q32, F_.d32, F_.sat0, F_.cand, F_.satA, F_.satB,
-                             dbox_,  dtile0_, dspmv0_, ctl_,   part_,  boxsum_, flags_};
+                             dbox_,  dtile0_, dspmv0_, ctl_,   part_,  boxsum_, flags_, F_.hp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto* p : classes_)
@@ -824,6 +843,7 @@ void Ensemble::set_initial(int box, const double* h_int, const double* wc_int) {
   if (box < 0 || box >= nbox_) throw ArgError("frehg_ens_set_initial: box out of range");
   if ((h_int == nullptr) == (wc_int == nullptr))
     throw ArgError("frehg_ens_set_initial: pass exactly one of h / wc");
+  have_inc_ = false;  // the next step starts from h^n (as frehg_re_set_initial)
   const EnsBox& B = hbox_[box];
   const Geo g = B.P.g;
   std::vector<double> pad((size_t)g.ncell(), 0.0);
@@ -873,11 +893,15 @@ void Ensemble::get_field(int box, const std::string& name, double* dst) {
 
 void Ensemble::enq_prepare() {
   const Pools P{F_.h, F_.hn, F_.wc, F_.wcn, F_.wch, F_.hwc, F_.ch, F_.kr, F_.Kx, F_.Ky, F_.Kz, F_.qx, F_.qy, F_.qz,
-                F_.D, F_.ax, F_.ay, F_.az, F_.rhs};
+                F_.D, F_.ax, F_.ay, F_.az, F_.rhs, F_.hp};
   const dim3 grid(ntile_), blk(NT);
   const EnsBox* bx = dbox_;
   const unsigned int* t0 = dtile0_;
-  hipLaunchKernelGGL(kb_save, grid, blk, 0, s_, bx, t0, nbox_, P);
+  const int mode = prep_mode();
+  if (mode == 2)
+    hipLaunchKernelGGL(kb_save_extrap, grid, blk, 0, s_, bx, t0, nbox_, P);
+  else
+    hipLaunchKernelGGL(kb_save, grid, blk, 0, s_, bx, t0, nbox_, P);
   hipLaunchKernelGGL(kb_save_ghosts, grid, blk, 0, s_, bx, t0, nbox_, P);
   with_soil_variant_code(variant_, [&](auto tag) {
     using K = decltype(tag);
@@ -885,13 +909,13 @@ void Ensemble::enq_prepare() {
     hipLaunchKernelGGL((kb_assemble<K, 1>), grid, blk, 0, s_, bx, t0, nbox_, P, mixed_ ? F_.D32 : nullptr, F_.ax32,
                        F_.ay32, F_.az32);
   });
-  hipLaunchKernelGGL(kb_guess, grid, blk, 0, s_, bx, t0, nbox_, P, prev_guess_ ? 1 : 0);
+  hipLaunchKernelGGL(kb_guess, grid, blk, 0, s_, bx, t0, nbox_, P, mode);
   FREHG_LAUNCH_CHECK();
   // Krylov start: FP64 CG init, or the first refinement residual
   if (!mixed_) {
     Sten<double> A{F_.D, F_.ax, F_.ay, F_.az};
     hipLaunchKernelGGL((kb_cg_init<double>), dim3(nspmv_), blk, 0, s_, bx, (const unsigned int*)dspmv0_, nbox_, A, (const double*)F_.rhs, F_.h, F_.r,
-                       F_.z, F_.p0, ctl_, 0, prev_guess_ ? 0 : 1, part_, nspmv_);
+                       F_.z, F_.p0, ctl_, 0, mode == 0 ? 1 : 0, part_, nspmv_);
     hipLaunchKernelGGL(kb_fin_init, dim3(nbox_), blk, 0, s_, (const double*)part_, nspmv_, (const unsigned int*)dspmv0_, ctl_, 0, kc_.rtol,
                        kc_.atol, kc_.max_it, kc_.inner_rtol);
   } else {
@@ -950,7 +974,7 @@ void Ensemble::enq_solve(int inner_iters, int refinements, int inner_first) {
 
 void Ensemble::enq_update(int sweeps) {
   const Pools P{F_.h, F_.hn, F_.wc, F_.wcn, F_.wch, F_.hwc, F_.ch, F_.kr, F_.Kx, F_.Ky, F_.Kz, F_.qx, F_.qy, F_.qz,
-                F_.D, F_.ax, F_.ay, F_.az, F_.rhs};
+                F_.D, F_.ax, F_.ay, F_.az, F_.rhs, F_.hp};
   const dim3 grid(ntile_), blk(NT);
   const EnsBox* bx = dbox_;
   const unsigned int* t0 = dtile0_;
@@ -1037,8 +1061,9 @@ void Ensemble::autotune() {
 }
 
 void Ensemble::run_graph(int which, int budget_a, int budget_b, int budget_c) {
+  const int mode = which == 0 ? prep_mode() : 0;
   if (!graph_[which] || graph_budget_[which] != budget_a || graph_budget2_[which] != budget_b ||
-      graph_budget3_[which] != budget_c) {
+      graph_budget3_[which] != budget_c || graph_mode_[which] != mode) {
     if (graph_[which]) FREHG_HIP(hipGraphExecDestroy(graph_[which]));
     graph_[which] = nullptr;
     hipGraph_t g;
@@ -1062,6 +1087,7 @@ void Ensemble::run_graph(int which, int budget_a, int budget_b, int budget_c) {
     graph_budget_[which] = budget_a;
     graph_budget2_[which] = budget_b;
     graph_budget3_[which] = budget_c;
+    graph_mode_[which] = mode;
   }
   FREHG_HIP(hipGraphLaunch(graph_[which], s_));
 }
@@ -1192,6 +1218,7 @@ void Ensemble::step(int use_graph, frehg_ens_stats* st, frehg_re_stats* per_box)
   }
   FREHG_HIP(hipMemcpyAsync(hboxsum_, boxsum_, (size_t)nbox_ * 8 * sizeof(double), hipMemcpyDeviceToHost, s_));
   FREHG_HIP(hipStreamSynchronize(s_));
+  have_inc_ = guess_ == FREHG_GUESS_EXTRAPOLATED;  // hp holds this step's post-solve head
   // budgets for the next captured step
   if (mixed_) {
     budget_refine_ = std::max(budget_refine_, max_ref);

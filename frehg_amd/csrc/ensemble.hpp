@@ -53,6 +53,7 @@ class Ensemble {
     double *r, *z, *p0, *p1, *q, *xa;
     float *D32, *ax32, *ay32, *az32, *b32, *r32, *z32, *p032, *p132, *q32, *d32;
     unsigned char *sat0, *cand, *satA, *satB;
+    double* hp;  // extrapolated start only (see Pools::hp)
   };
   void build(int nbox, const int32_t* dims, const frehg_re_params* params);
   void release();
@@ -73,7 +74,14 @@ class Ensemble {
   int nbox_;
   hipStream_t s_;
   Profiler* prof_;
-  bool mixed_ = false, prev_guess_ = false;
+  bool mixed_ = false;
+  int guess_ = FREHG_GUESS_ZERO;  // FREHG_GUESS_ZERO, _PREVIOUS or _EXTRAPOLATED (all boxes)
+  bool have_inc_ = false;         // hp holds the last step's post-solve head (extrapolated start)
+  // the start of the next step: 0 zero, 1 h^n, 2 h^n + the last step's solve increment
+  int prep_mode() const {
+    if (guess_ == FREHG_GUESS_ZERO) return 0;
+    return guess_ == FREHG_GUESS_EXTRAPOLATED && have_inc_ ? 2 : 1;
+  }
   KrylovConfig kc_;
   std::vector<int32_t> dims_;
   std::vector<frehg_re_params> params_;
@@ -100,6 +108,7 @@ class Ensemble {
   int graph_budget_[3] = {-1, -1, -1};
   int graph_budget2_[3] = {-1, -1, -1};
   int graph_budget3_[3] = {-1, -1, -1};
+  int graph_mode_[3] = {-1, -1, -1};
   static constexpr int kMaxSweepFlags = 64;
   int variant_ = 0;  // soil variant code of the per-cell kernels (with_soil_variant_code)
 };

@@ -252,7 +252,9 @@ int frehg_ls_solve_noalloc(frehg_ctx_t ctx, int ndim, int nx, int ny, int nz, co
  * (src/re/ReSolver.cpp:931-969); here every kernel of a step covers all boxes in one launch, each
  * box keeps its own Krylov scalars and convergence test, and steps run from captured HIP graphs.
  * Restrictions: fixed dt (adaptive_dt = 0), the same solver configuration in every box,
- * initial guess zero or previous-step, closed lateral sides. ---------------------------------- */
+ * initial guess zero, previous-step or extrapolated (h^n + the last step's solve increment; the
+ * first step after frehg_ens_set_initial starts from h^n), one Picard iterate, closed lateral
+ * sides. --------------------------------------------------------------------------------------- */
 typedef struct {
   int32_t max_iterations;        /* largest Krylov iteration count over the boxes (inner FP32 for mixed) */
   int32_t max_refinements;       /* largest number of FP64 refinement sweeps (mixed) */

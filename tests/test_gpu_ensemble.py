@@ -46,11 +46,12 @@ def single_box_heads(case, b, steps, precision):
     return ctx.re_get("h"), ctx.re_get("wc")
 
 
-@pytest.mark.parametrize("precision", [abi.PREC_FP64, abi.PREC_MIXED])
-def test_ensemble_matches_oracle_per_box(precision):
+@pytest.mark.parametrize("precision,guess", [(abi.PREC_FP64, abi.GUESS_PREVIOUS), (abi.PREC_MIXED, abi.GUESS_PREVIOUS),
+                                             (abi.PREC_MIXED, abi.GUESS_EXTRAPOLATED)])
+def test_ensemble_matches_oracle_per_box(precision, guess):
     steps = 6
-    case = small_case(precision)
-    say(f"ensemble {case.dims} precision {precision}")
+    case = small_case(precision, guess)
+    say(f"ensemble {case.dims} precision {precision} guess {guess}")
     ctx = gpu.Context(0)
     case.setup_gpu(ctx)
     for s in range(steps):
@@ -70,12 +71,14 @@ def test_ensemble_matches_oracle_per_box(precision):
         assert np.abs(wg - ws).max() <= tol, b
 
 
-@pytest.mark.parametrize("precision", [abi.PREC_FP64, abi.PREC_MIXED])
-def test_graph_steps_equal_eager_steps(precision):
+@pytest.mark.parametrize("precision,guess", [(abi.PREC_FP64, abi.GUESS_PREVIOUS), (abi.PREC_MIXED, abi.GUESS_PREVIOUS),
+                                             (abi.PREC_FP64, abi.GUESS_EXTRAPOLATED)])
+def test_graph_steps_equal_eager_steps(precision, guess):
     """FP64: the captured iteration budget plus eager continuation runs the same iterates as the eager
-    loop (bit-identical). Mixed: a refinement whose inner solve outgrows the captured budget is cut
-    and refined again, so the iterates differ but both meet the same tolerance."""
-    case = small_case(precision)
+    loop (bit-identical; with the extrapolated start the first step's graph starts from h^n and is
+    captured again for the later steps). Mixed: a refinement whose inner solve outgrows the captured
+    budget is cut and refined again, so the iterates differ but both meet the same tolerance."""
+    case = small_case(precision, guess)
     out = []
     for use_graph in (False, True):
         ctx = gpu.Context(0)
