@@ -1,12 +1,17 @@
 """Pins the oracle's Richards PCA step (oracle/re_oracle.cpp, restating src/re/ReSolver.cpp)
 with the reference's RE tests (tests/re/test_re_3d.cpp:133-274, test_re_b2_gw.cpp:109-153)
 and a brute-force 1x1x3 column solved by dense linear algebra."""
+import csv
+import os
+
 import numpy as np
 import pytest
 
 import oracle_lib as ol
 from cases import box3d_params, box3d_soil, box3d_wc, interior3
 from frehg_amd import abi
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 NX, NY, NZ = 4, 4, 8
 V = 2.0 * 1.0 * 0.05
@@ -98,7 +103,7 @@ def test_b2gw_adaptive_dt_ramp_and_plateau():
                       dt=1e-4, dt_min=1e-4, dt_max=2.0, co_max=2.0, adaptive_dt=1, htop=0.0)
     r = ol.OracleRe(1, 1, 100, p)
     r.set_initial(wc=np.full(100, 0.033))
-    t, dts = 0.0, []
+    t, dts, snap = 0.0, [], {}
     while t < 46800.0 + 1e-9:
         used = r.dt()
         st = r.step()
@@ -106,6 +111,9 @@ def test_b2gw_adaptive_dt_ramp_and_plateau():
         dts.append(used)
         t += st.dt_used
         assert 1e-4 - 1e-300 <= r.dt() <= 2.0 + 1e-12
+        for ts in (11700, 23400, 46800):
+            if ts not in snap and t >= ts - 1e-6:
+                snap[ts] = r.get("wc").reshape(102, 3, 3)[1:-1, 1, 1].copy()
     ramp = 1e-4 * 1.25 ** np.arange(12)
     np.testing.assert_allclose(dts[:12], ramp, rtol=1e-13)
     assert r.dt() == 2.0
@@ -117,6 +125,21 @@ def test_b2gw_adaptive_dt_ramp_and_plateau():
     assert np.sum(ratio[below] == 1.0) > 0
     k = int(np.argmax(np.array(dts) >= 2.0))
     assert all(d == 2.0 for d in dts[k:])
+    # the Warrick profile (the reference's R-2 comparison, test_re_b2_gw.cpp:275-320, which it
+    # prints without a gate): nine points on the wetting front at three times, the cell nearest
+    # each elevation (kFromElevation). The oracle's front runs deeper than Warrick's (rel-L2 0.30);
+    # the loose bound catches a grossly wrong infiltration rate (a front missing or at the bottom).
+    with open(os.path.join(GOLDEN, "b2_gw", "warrick_water_content_profile.csv")) as fh:
+        rows = list(csv.DictReader(fh))
+    model, warrick = [], []
+    for row in rows:
+        z = float(row["z_m"])
+        kz = int(np.argmin([abs(-0.01 - kk * 0.01 + 0.005 - z) for kk in range(100)]))
+        model.append(snap[int(row["time_s"])][kz])
+        warrick.append(float(row["water_content"]))
+    model, warrick = np.array(model), np.array(warrick)
+    rel = float(np.linalg.norm(model - warrick) / np.linalg.norm(warrick))
+    assert len(rows) == 9 and rel < 0.4, rel
 
 
 def _dt_statistics(r, p, nx, ny, nz, dt_prev):
