@@ -293,3 +293,65 @@ def test_coupled_yaml_with_solute_equals_the_call_by_call_run_and_the_oracle(tmp
     say(f"  max |dC| vs oracle: surface {es:.2e}, soil {eg:.2e}; ledger error {st.solute_relative_mass_error:.3e}")
     assert es <= 1e-8 and eg <= 1e-8
     assert st.solute_added_by_rain == pytest.approx(sol.mass()[1].rain_added, rel=1e-9)
+
+
+def test_surface_only_and_groundwater_only_yaml_runs_with_solute(tmp_path):
+    """The solute in the surface-only branch (applySolute after the surface step, rain of
+    concentration c_rain on the file's initial concentration 15; b1-sw.yaml) and in the groundwater-only branch (after each adaptive
+    groundwater step, with an initial soil concentration; b2-gw.yaml), both switched on by
+    overrides, against the oracle driven with the reference's sequence (Orchestrator::step,
+    :1337-1352)."""
+    say("config front end: b1-sw and b2-gw with solute")
+    sol_keys = ["solute.enabled=true", "solute.c_rain=2.0", "solute.k_decay=1e-5", "solute.D=1e-3",
+                "solute.advection_scheme=muscl", "output.variables=[water_depth, concentration]"]
+    st = gpu.Context(0).run_config(os.path.join(GOLDEN, "b1_sw", "b1-sw.yaml"), ["time.max_steps=200"] + sol_keys,
+                                   out_dir=tmp_path / "b1")
+    assert st.steps == 200 and st.solute_added_by_rain > 0
+    bath, rain, _, offset, p = b1_inputs()
+    sp = gpu.config_setup(os.path.join(GOLDEN, "b1_sw", "b1-sw.yaml"), sol_keys).solute_params
+    orc = ol.OracleSwe(1, 10, 80.0, 80.0, p)
+    orc.set_bathymetry(bath + offset)
+    orc.initialize_state(-2.0)
+    sol = ol.OracleSolute(orc, None, sp)
+    # the file's own initial_conditions.solute.surface (15)
+    c0 = gpu.config_field(os.path.join(GOLDEN, "b1_sw", "b1-sw.yaml"), "solute_surface", sol_keys)
+    assert np.all(c0 == 15.0)
+    sol.set("surface", np.pad(c0.reshape(10, 1), 1).ravel())
+    t = 0.0
+    for _ in range(200):
+        r = interp_series(rain[:, 0], rain[:, 1], t + 5.0)
+        orc.step(r, 0.0)
+        sol.apply(t, 5.0, r)
+        t += 5.0
+    cs = snapshot(tmp_path / "b1", "surface_concentration", 1000)
+    want = interior2(sol.get("surface"), 1, 10).ravel()
+    say(f"  b1: max |dC| {np.abs(cs - want).max():.2e}, C max {want.max():.3f}")
+    assert want.min() < 14.0 and np.abs(cs - want).max() <= 1e-9
+    assert st.solute_added_by_rain == pytest.approx(sol.mass()[1].rain_added, rel=1e-12)
+
+    gw_keys = ["time.max_steps=200", "solute.enabled=true", "solute.D=1e-6", "initial_conditions.solute.subsurface=1.5",
+               "output.variables=[moisture, conc]"]
+    st = gpu.Context(0).run_config(os.path.join(GOLDEN, "b2_gw", "b2-gw.yaml"), gw_keys, out_dir=tmp_path / "b2")
+    setup = gpu.config_setup(os.path.join(GOLDEN, "b2_gw", "b2-gw.yaml"), gw_keys)
+    np.testing.assert_array_equal(gpu.config_field(os.path.join(GOLDEN, "b2_gw", "b2-gw.yaml"), "solute_subsurface",
+                                                   gw_keys), np.full(100, 1.5))
+    soil = abi.soil_params(alpha=1.43, n=1.56, theta_s=0.33, theta_r=0.0, Ks_z=2.89e-6, Ss=1e-5)
+    pr = abi.re_params(soil=soil, solver=abi.solver_config(1e-10), bc_type=(0, 0, 0, 0, 0, 1), dx=1.0, dy=1.0,
+                       dz=0.01, botz=-1.0, dt=1e-4, dt_min=1e-4, dt_max=2.0, co_max=2.0, htop=0.0)
+    r = ol.OracleRe(1, 1, 100, pr)
+    r.set_initial(wc=np.full(100, 0.033))
+    sol = ol.OracleSolute(None, r, setup.solute_params)
+    sol.set("subsurface", np.pad(np.full((100, 1, 1), 1.5), 1).ravel())
+    t = 0.0
+    for _ in range(200):
+        used = r.step().dt_used
+        sol.apply(t, used, 0.0)
+        t += used
+    final = [f for f in glob.glob(str(tmp_path / "b2" / "subsurface_concentration_*.txt"))
+             if not f.endswith("_0.txt")]
+    assert len(final) == 1
+    cg = np.loadtxt(final[0])
+    want = interior3(sol.get("subsurface"), 1, 1, 100).ravel()
+    say(f"  b2: max |dC| {np.abs(cg - want).max():.2e}, mass {st.solute_final_mass:.6e} vs {sol.mass()[0]:.6e}")
+    assert np.abs(cg - want).max() <= 1e-8
+    assert st.solute_final_mass == pytest.approx(sol.mass()[0], rel=1e-9)
