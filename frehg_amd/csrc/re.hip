@@ -17,6 +17,7 @@
 //   finalize   reallocation, clamp, moisture ghosts                                 (:788-878)
 //   dtadapt    flux-imbalance max and Courant min (adaptive dt only)                (:880-929)
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <type_traits>
 #include <cstring>
@@ -235,6 +236,40 @@ __global__ __launch_bounds__(NT) void k_re_post_delta(K P, double* __restrict__ 
   }
 }
 
+// The first test of a solve from the Picard iterate x = h^m, taken before its system is stored:
+// each cell forms its row as k_re_assemble does (the -x/-y/-z couplings of a row are the stored
+// +couplings of the neighbour rows, the same products of the same face conductivities) and
+// r = b - A x in the solvers' order of terms. Per block: sum (r/D)^2 and sum (b/D)^2 over the
+// interior cells to part[0..nblocks) and part[nblocks..2 nblocks); the solvers stop before
+// their first update when ||D^-1 r|| <= max(rtol ||D^-1 b||, atol).
+template <class K, int ZF>
+__global__ __launch_bounds__(NT) void k_re_check(K P, const double* __restrict__ kr, const double* __restrict__ ch,
+                                                  const double* __restrict__ wcn, const double* __restrict__ hn,
+                                                  const double* __restrict__ x, double* __restrict__ part,
+                                                  unsigned int nblocks) {
+  __shared__ double lds[NT / kWave];
+  TileId b;
+  const Cell t = tile_cell<ZF>(P.g, b);
+  double srr = 0.0, sbb = 0.0;
+  if (t.valid) {
+    const ReRow row = re_assemble_row(P, t, kr, ch, wcn, hn);
+    const Geo& g = P.g;
+    const long long c = t.c;
+    double y = row.D * x[c] + row.xp * x[c + 1] + row.xm * x[c - 1] + row.yp * x[c + g.sy] + row.ym * x[c - g.sy];
+    y += row.zp * x[c + g.sz] + row.zm * x[c - g.sz];
+    const double r = row.rhs - y;
+    const double z = r / row.D, bz = row.rhs / row.D;
+    srr = z * z;
+    sbb = bz * bz;
+  }
+  const double a = block_sum<NT>(srr, lds);
+  const double bs = block_sum<NT>(sbb, lds);
+  if (threadIdx.x == 0) {
+    part[b.linear] = a;
+    part[nblocks + b.linear] = bs;
+  }
+}
+
 __global__ __launch_bounds__(NT) void k_re_initial(ReK P, int from_head, double* h, double* wc, double* hn,
                                                     double* wcn, double* wch, double* hwc, double* ch) {
   const Cell t = cell_of(P.g);
@@ -282,6 +317,8 @@ void ReState::build(const frehg_re_params& p) {
   const size_t bytes = (size_t)n_ * sizeof(double);
   const char* lz = std::getenv("FREHG_LAZY_FLUX");
   lazy_flux_ = !(lz && lz[0] == '0');
+  const char* pc = std::getenv("FREHG_PICARD_CHECK");
+  picard_check_ = !(pc && pc[0] == '0');
   for (double** f : {&h_, &hn_, &hp_, &wc_, &wcn_, &wch_, &hwc_, &ch_, &Kx_, &Ky_, &Kz_, &qx_, &qy_, &qz_, &kr_,
                      &D_, &ax_, &ay_, &az_, &rhs_}) {
     FREHG_HIP(hipMalloc(f, bytes));
@@ -350,7 +387,7 @@ void ReState::set_params(const frehg_re_params& p) {
       FREHG_HIP(hipMemsetAsync(*f, 0, b32, s_));
     }
   }
-  materialize_fluxes();
+  flush_lazy();
   const bool first = (dtn_ == 0.0);
   p_ = p;
   kc_fresh_ = false;
@@ -360,7 +397,7 @@ void ReState::set_params(const frehg_re_params& p) {
 void ReState::set_soil(const frehg_soil_params* classes, int ncls, const int32_t* class_idx, int idx3d,
                        const double* ks_cell) {
   if (ncls < 1) throw ArgError("frehg_re_set_soil: need at least one soil class");
-  materialize_fluxes();
+  flush_lazy();
   kc_fresh_ = false;
   host_classes_.clear();
   for (int c = 0; c < ncls; ++c) {
@@ -411,7 +448,7 @@ void ReState::set_soil(const frehg_soil_params* classes, int ncls, const int32_t
 }
 
 void ReState::set_top_flux(const double* q) {
-  materialize_fluxes();
+  flush_lazy();
   std::vector<double> pad((size_t)g_.nsurf(), 0.0);
   for (int j = 0; j < g_.ny; ++j)
     for (int i = 0; i < g_.nx; ++i) pad[(size_t)g_.sidx(i, j)] = q[(size_t)i + (size_t)j * g_.nx];
@@ -440,7 +477,7 @@ double* ReState::field_ptr(const std::string& n) {
 
 void ReState::set_field(const std::string& name, const double* src) {
   double* dst = field_ptr(name);
-  materialize_fluxes();
+  flush_lazy();
   kc_fresh_ = false;
   inc_valid_ = false;
   copy_field_in(g_, g_.nz + 2, dst, src, s_);
@@ -450,6 +487,7 @@ void ReState::set_field(const std::string& name, const double* src) {
 void ReState::get_field(const std::string& name, double* dst) {
   const double* src = field_ptr(name);
   if (src == Kx_ || src == Ky_ || src == Kz_ || src == qx_ || src == qy_ || src == qz_) materialize_fluxes();
+  if (src == D_ || src == rhs_) materialize_operator();
   copy_field_out(g_, g_.nz + 2, dst, src, s_);
   FREHG_HIP(hipStreamSynchronize(s_));
 }
@@ -459,7 +497,7 @@ void ReState::set_initial(const double* h_int, const double* wc_int) {
   if ((h_int == nullptr) == (wc_int == nullptr))
     throw ArgError("frehg_re_set_initial: pass exactly one of h / wc");
   const double* src = h_int ? h_int : wc_int;
-  materialize_fluxes();
+  flush_lazy();
   kc_fresh_ = false;
   std::vector<double> pad((size_t)n_, 0.0);
   FREHG_HIP(hipMemcpy(pad.data(), h_int ? h_ : wc_, (size_t)n_ * sizeof(double), hipMemcpyDeviceToHost));
@@ -510,6 +548,22 @@ void ReState::learned_start(double* dst) {
         dst[(size_t)i + (size_t)j * g_.nx + (size_t)k * g_.nx * g_.ny] = pad[(size_t)g_.idx(i, j, k)];
 }
 
+void ReState::materialize_operator() {
+  if (!op_pending_) return;
+  op_pending_ = false;
+  frehg_re_params q = p_;
+  q.dt = dt_used_;  // the step's; an adaptive step has already set p_.dt to the next one
+  const ReK P = make_rek(g_, soil_view(), q, qtop_);
+  const dim3 grid(ceil_div(g_.nx, 64), ceil_div(g_.ny, 4), g_.nz);
+  with_soil_variant(P, [&](const auto& PK) {
+    using K = std::decay_t<decltype(PK)>;
+    hipLaunchKernelGGL((k_re_assemble<K, 1>), grid, dim3(NT), 0, s_, PK, (const double*)kr_, (const double*)ch_,
+                       (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, nullptr, nullptr, nullptr,
+                       nullptr);
+    FREHG_LAUNCH_CHECK();
+  });
+}
+
 void ReState::materialize_fluxes() {
   if (!flux_pending_) return;
   const ReK P = make_rek(g_, soil_view(), p_, qtop_);
@@ -543,6 +597,7 @@ void ReState::step(frehg_re_stats* st) {
   const bool kc_fresh = kc_fresh_;
   kc_fresh_ = false;  // set again once this step has completed
   flux_pending_ = false;  // this step's fluxes supersede the previous step's
+  op_pending_ = false;    // and its system the previous step's
   // With the Picard loop the learned head is the loop's starting iterate h^0 (below): the system
   // is assembled at it, and K and C at h^n are not needed (the network's features evaluate their
   // own from h^n).
@@ -681,6 +736,34 @@ void ReState::step(frehg_re_stats* st) {
     }
     if (m >= p_.picard_max) break;  // the cap: the step continues from the last iterate
     const bool zero = p_.initial_guess == FREHG_GUESS_ZERO;
+    // A solve from h^m first tests the residual of h^m on the system assembled at h^m and returns
+    // h^m unchanged when the test passes, which ends the loop (below). That test is taken here
+    // on rows formed in registers, without storing the system; only when it fails is the system
+    // stored and solved. The stored system then lags one iterate behind; it is assembled when
+    // read or before its inputs change (op_pending_).
+    if (!zero && picard_check_) {
+      with_soil_variant(P, [&](const auto& PK) {
+        using K = std::decay_t<decltype(PK)>;
+        hipLaunchKernelGGL((k_re_check<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
+                           (const double*)wcn_, (const double*)hn_, (const double*)h_, red_, nbl);
+        FREHG_LAUNCH_CHECK();
+      });
+      reduce_sums(red_, nbl, 2, red2_, dres_ + 11, s_);
+      FREHG_LAUNCH_CHECK();
+      FREHG_HIP(hipMemcpyAsync(hres_ + 11, dres_ + 11, 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
+      FREHG_HIP(hipStreamSynchronize(s_));
+      const double rr = hres_[11], bb = hres_[12];
+      const double ttol = std::fmax(kc.rtol * std::sqrt(bb), kc.atol);
+      if (rr <= ttol * ttol) {
+        local.solve.residual_norm = std::sqrt(rr);
+        local.solve.rhs_norm = std::sqrt(bb);
+        local.picard_iterations = m + 1;
+        local.picard_converged = 1;
+        post_done = true;
+        op_pending_ = true;
+        break;
+      }
+    }
     // The FP32 copy of the operator is refreshed when the coming solve will need inner solves:
     // from x0 = 0, or after a solve that took a refinement. A solve from h^m that converged at
     // its first residual check leaves the previous copy in place (its inner solves, if any,

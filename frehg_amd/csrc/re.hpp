@@ -61,6 +61,8 @@ class ReState {
   Krylov& krylov() { return *kry_; }
   // Stores Kx..qz of the last step if they are pending (see flux_pending_).
   void materialize_fluxes();
+  // Stores the system of the last Picard iterate if it is pending (see op_pending_).
+  void materialize_operator();
   // Learned increment h^{n+1} - h^n the model predicts for a step taken from the current state
   // (features from h, K/C at h, theta), interior cells in i + j nx + k nx ny order.
   void learned_increment(double* dst);
@@ -103,6 +105,18 @@ class ReState {
   unsigned char* kf_ = nullptr;
   bool lazy_flux_ = true;
   bool flux_pending_ = false;
+  // A Picard loop that ended on the residual test of its last iterate h^m taken before assembly
+  // (FREHG_PICARD_CHECK=0 assembles and solves instead) leaves D..rhs at the previous iterate.
+  // Until the next step the system at h^m follows bit for bit from kr_ and ch_ (at h^m), wcn_,
+  // hn_, the soil, the top flux and the parameters with the step's dt, and is stored from them
+  // when read or before anything changes those inputs.
+  bool picard_check_ = true;
+  bool op_pending_ = false;
+  // Both lazy stores, before a change of the state.
+  void flush_lazy() {
+    materialize_fluxes();
+    materialize_operator();
+  }
   // Relative conductivity K/Ks per cell (incl. face-adjacent ghosts).
   double* kr_ = nullptr;
   // Assembled system: D = Gct, ax/ay/az = +x/+y/+z couplings, rhs = Grhs; FP32 copies.

@@ -128,3 +128,31 @@ def test_gated_start_equals_the_network_where_the_gate_opens(model):
     assert np.all((gated == full) | (gated == base))
     assert np.any(full[above] != base[above])
     ctx.close()
+
+
+def test_picard_residual_test_before_assembly_in_the_learned_coupled_step(model, monkeypatch):
+    """The learned start's Picard loop usually ends on the residual test of its second iterate,
+    taken before that iterate's system is stored (the default); with FREHG_PICARD_CHECK=0 the
+    system is stored and solved and the solve returns the iterate unchanged. Heads, depths, the
+    system at the last iterate and every count agree bit for bit over the held-out run."""
+    held = case(7, 8)
+    monkeypatch.setenv("FREHG_SPMV_SHAPE", "2:12")
+    out = []
+    for check in ("0", "1"):
+        monkeypatch.setenv("FREHG_PICARD_CHECK", check)
+        ctx = gpu.Context(0)
+        ctx.mlp_set_params(*model)
+        ctx.mlp_gate(1e-6)
+        held.setup_gpu(ctx, abi.PREC_MIXED, abi.GUESS_LEARNED, picard_max=PICARD)
+        counts = []
+        for _ in range(8):
+            st = ctx.coupled_step(held.cp, held.dt, held.rain, 0.0, held.t_end)
+            counts.append((st.gw.picard_iterations, st.gw.solve.iterations, st.gw.solve.refinements,
+                           st.gw.guess_fallback, st.gw.mass_residual))
+        out.append((counts, ctx.re_get("h"), ctx.swe_get("dept"), ctx.re_get("Gct"), ctx.re_get("Grhs")))
+        ctx.close()
+    (ca, *fa), (cb, *fb) = out
+    say(f"  Picard iterates per step {[c[0] for c in cb]}")
+    assert ca == cb
+    for x, y, name in zip(fa, fb, ("h", "dept", "Gct", "Grhs")):
+        assert np.array_equal(x, y), name

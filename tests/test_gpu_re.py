@@ -196,6 +196,58 @@ def test_picard_outer_loop_matches_oracle(mode):
         compare_re(ctx, orc, rtol=1e-6, atol_h=1e-6, label=f"picard {mode}")
 
 
+@pytest.mark.parametrize("prec,guess,ksp", [(abi.PREC_FP64, abi.GUESS_PREVIOUS, abi.KSP_CG),
+                                            (abi.PREC_MIXED, abi.GUESS_EXTRAPOLATED, abi.KSP_CG),
+                                            (abi.PREC_MIXED, abi.GUESS_PREVIOUS, abi.KSP_BICGSTAB),
+                                            (abi.PREC_FP64, abi.GUESS_EXTRAPOLATED, abi.KSP_BICGSTAB)])
+def test_picard_residual_test_before_assembly_equals_assemble_and_solve(prec, guess, ksp):
+    """A Picard iterate h^m whose residual on the system at h^m already meets the solver's first
+    test ends the loop without the system being stored or solved (the default); with
+    FREHG_PICARD_CHECK=0 the system is stored and the solve returns h^m unchanged. Every field,
+    the system at the last iterate (stored when read) included, and every count of the step must
+    agree bit for bit, also after the dt of an adaptive step has changed and after a parameter
+    change forces the pending system out."""
+    nx, ny, nz = 9, 7, 12
+    say(f"RE Picard residual test before assembly: precision {prec}, guess {guess}, ksp {ksp}")
+    soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5, Ss=1e-5)
+    kw = dict(soil=soil, bc_type=(0, 0, 0, 0, 0, 1), dx=0.5, dy=0.5, dz=0.05, dt=60.0, adaptive_dt=1, dt_max=90.0,
+              dt_min=1.0, use_full3d=1, htop=-0.3, picard_max=40, picard_tol=1e-9)
+    p = abi.re_params(solver=abi.solver_config(1e-13, ksp_type=ksp, precision=prec, inner_rtol=1e-5, max_refine=10),
+                      initial_guess=guess, **kw)
+    k = np.arange(nz)[:, None, None]
+    i = np.arange(nx)[None, None, :]
+    h0 = np.broadcast_to(-1.5 - 0.05 * k - 0.02 * i, (nz, ny, nx)).ravel()
+    ctxs = []
+    os.environ["FREHG_SPMV_SHAPE"] = "2:12"
+    try:
+        for check in ("0", "1"):
+            os.environ["FREHG_PICARD_CHECK"] = check
+            ctxs.append(re_pair(nx, ny, nz, p, h0=h0)[1])
+    finally:
+        del os.environ["FREHG_SPMV_SHAPE"]
+        del os.environ["FREHG_PICARD_CHECK"]
+    a, b = ctxs
+    for s in range(6):
+        sa, sb = a.re_step(), b.re_step()
+        assert sa.picard_converged and sb.picard_converged
+        for fld in ("picard_iterations", "dt_used", "dt_next", "mass_residual", "max_dh"):
+            assert getattr(sa, fld) == getattr(sb, fld), (s, fld)
+        for fld in ("iterations", "refinements", "fallback"):
+            assert getattr(sa.solve, fld) == getattr(sb.solve, fld), (s, fld)
+        if s % 2 == 0:
+            for f in RE_COMPARE:
+                assert np.array_equal(a.re_get(f), b.re_get(f)), f"step {s} field {f}"
+    # a parameter change between steps: the pending system is stored with the last step's dt first
+    p2 = abi.re_params(solver=abi.solver_config(1e-13, ksp_type=ksp, precision=prec, inner_rtol=1e-5, max_refine=10),
+                       initial_guess=guess, **dict(kw, dt=30.0, adaptive_dt=0))
+    for c in (a, b):
+        c.re_set_params(p2)
+    for f in ("Gct", "Grhs"):
+        assert np.array_equal(a.re_get(f), b.re_get(f)), f"after set_params: {f}"
+    for c in (a, b):
+        c.close()
+
+
 def test_heterogeneous_soil_and_top_flux():
     nx, ny, nz = 9, 7, 10
     say("RE heterogeneous soil + per-column top flux")
