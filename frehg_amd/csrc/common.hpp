@@ -160,6 +160,46 @@ __global__ __launch_bounds__(kRedThreads) inline void k_reduce_maxmin(const doub
   }
 }
 
+// Two-stage form of k_reduce_maxmin for long rows: kRedStage blocks each take the max of one
+// chunk of part[0..n) and the min of the same chunk of part[n..2n) (to tmp[0..kRedStage) and
+// tmp[kRedStage..2 kRedStage)), then one block reduces those. Max and min are exact, so the
+// result equals the one-block form bit for bit.
+__global__ __launch_bounds__(256) inline void k_reduce_maxmin_stage(const double* part, unsigned int n,
+                                                                     double* out) {
+  __shared__ double lds[2 * (256 / kWave)];
+  const unsigned int chunk = (n + gridDim.x - 1) / gridDim.x;
+  const unsigned int lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  double a = 0.0, b = 1.0e300;
+  for (unsigned int i = lo + threadIdx.x; i < hi; i += 256) {
+    a = fmax(a, part[i]);
+    b = fmin(b, part[n + i]);
+  }
+  a = wave_max(a);
+  b = wave_min(b);
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  if (lane == 0) {
+    lds[w] = a;
+    lds[256 / kWave + w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = lds[0], l = lds[256 / kWave];
+    for (int i = 1; i < 256 / kWave; ++i) {
+      m = fmax(m, lds[i]);
+      l = fmin(l, lds[256 / kWave + i]);
+    }
+    out[blockIdx.x] = m;
+    out[gridDim.x + blockIdx.x] = l;
+  }
+}
+
+// out[0] = max of part[0..n) (floored at 0), out[1] = min of part[n..2n) (capped at 1e300);
+// tmp holds 2 * kRedStage doubles.
+inline void reduce_maxmin(const double* part, unsigned int n, double* tmp, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_maxmin_stage, dim3(kRedStage), dim3(256), 0, s, part, n, tmp);
+  hipLaunchKernelGGL(k_reduce_maxmin, dim3(1), dim3(kRedThreads), 0, s, (const double*)tmp, kRedStage, out);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Workgroups are dispatched round-robin over the 8 XCDs of the device (linear workgroup id mod 8),
 // and each XCD has its own L2. xcd_remap is a bijection of [0, n) that hands every XCD one

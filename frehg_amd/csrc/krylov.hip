@@ -1153,7 +1153,8 @@ void Krylov::release() {
   for (void* p : {(void*)r_, (void*)z_, (void*)p0_, (void*)p1_, (void*)q_, (void*)xw_, (void*)b32_,
                   (void*)r32_, (void*)z32_, (void*)p032_, (void*)p132_, (void*)q32_, (void*)d32_,
                   (void*)rh_, (void*)v0_, (void*)v1_, (void*)sv_, (void*)tv_, (void*)rh32_, (void*)v032_,
-                  (void*)v132_, (void*)sv32_, (void*)tv32_, (void*)part_, (void*)part_u_, (void*)ctl_})
+                  (void*)v132_, (void*)sv32_, (void*)tv32_, (void*)part_, (void*)part_u_, (void*)ctl_,
+                  (void*)stage_})
     if (p) (void)hipFree(p);
   if (host_ctl_) (void)hipHostFree(host_ctl_);
 }
@@ -1232,6 +1233,11 @@ void Krylov::autotune(const StencilView<T>& A) {
   (void)hipEventDestroy(e1);
 }
 
+Krylov::SweepTarget Krylov::first_sweep_target() {
+  ensure_mixed();
+  return SweepTarget{r32_, z32_, xw_, part_, part_cap_};
+}
+
 void Krylov::ensure_mixed() {
   if (b32_) return;
   if (borrowed_) throw ArgError("Krylov: a solve on a borrowed workspace is FP64 CG only");
@@ -1240,6 +1246,7 @@ void Krylov::ensure_mixed() {
     FREHG_HIP(hipMalloc(p, bytes));
     FREHG_HIP(hipMemsetAsync(*p, 0, bytes, s_));
   }
+  FREHG_HIP(hipMalloc(&stage_, 5 * kRedStage * sizeof(double)));
 }
 
 void Krylov::ensure_bicgstab(bool fp32) {
@@ -1543,9 +1550,12 @@ int Krylov::run(int ksp, const StencilView<T>& A, const T* b, const T* bn, T* x,
 
 SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>* A32, const double* b,
                           const double* bn, double* x, const unsigned char* fixed, bool x0_zero,
-                          bool guard_guess, const KrylovConfig& cfg) {
+                          bool guard_guess, const KrylovConfig& cfg, unsigned int swept_blocks) {
   SolveResult res;
-  if (!tuned_[0]) autotune<double>(A);
+  // (the autotuner's SpMV launches write part_, which holds a caller's sweep partials)
+  if (!tuned_[0] && !swept_blocks) autotune<double>(A);
+  if (swept_blocks && (cfg.precision == 0 || A32 == nullptr))
+    throw ArgError("Krylov: a pre-swept solve is a mixed-precision solve");
   if (cfg.precision == 0 || A32 == nullptr) {
     bool rej = false, conv = false;
     res.iterations = run<double>(cfg.ksp, A, b, bn, x, fixed, x0_zero, guard_guess, cfg.rtol, cfg.atol,
@@ -1615,7 +1625,20 @@ SolveResult Krylov::solve(const StencilView<double>& A, const StencilView<float>
   };
   double* xcur = xw_;
   double* xalt = x;
-  refine(x, nullptr, xw_, true);
+  if (swept_blocks) {
+    if (!fused || x0_zero || swept_blocks > part_cap_)
+      throw ArgError("Krylov: a pre-swept solve needs the paired 3D CG refinement from x0 = x");
+    // one block of partials per tile and layer: summed in two stages
+    hipLaunchKernelGGL(k_reduce_sum_stage, dim3(kRedStage, 5), dim3(256), 0, s_, (const double*)part_, swept_blocks,
+                       stage_);
+    FREHG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fin_refine, dim3(1), dim3(kRedThreads), 0, s_, (const double*)stage_, kRedStage, outer, 1,
+                       cfg.rtol, cfg.atol, guard_guess ? 1 : 0, inner, cfg.inner_rtol, cfg.max_it);
+    FREHG_LAUNCH_CHECK();
+    if (!tuned_[0]) autotune<double>(A);
+  } else {
+    refine(x, nullptr, xw_, true);
+  }
   read_ctl(outer, host_ctl_);
   if (host_ctl_->guess_rejected) {
     res.guess_rejected = true;

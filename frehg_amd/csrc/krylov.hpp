@@ -106,9 +106,21 @@ class Krylov {
   // already eliminated from b. guard_guess: fall back to x0 = 0 when the guess's residual is
   // not below ||D^-1 b||. A32 (may be null) is the FP32 copy of A for precision == 1.
   // cfg.ksp selects CG (symmetric A only) or BiCGSTAB.
+  // swept_blocks > 0: the caller's kernel has already taken the first sweep of the mixed-precision
+  // solve (3D, symmetric, no fixed rows, bn = b, CG as the inner solver, x0 = x) into the buffers
+  // of first_sweep_target(), with swept_blocks blocks of partials.
   SolveResult solve(const StencilView<double>& A, const StencilView<float>* A32, const double* b,
                     const double* bn, double* x, const unsigned char* fixed, bool x0_zero,
-                    bool guard_guess, const KrylovConfig& cfg);
+                    bool guard_guess, const KrylovConfig& cfg, unsigned int swept_blocks = 0);
+  // Where a caller's kernel writes the first sweep of the next mixed-precision solve: r32 =
+  // (float)(b - A x), z32 = r32 / D32, xw = x, and five partials per block at stride nblocks in
+  // part (||D^-1 r||^2, ||D^-1 b||^2, r32.z32, z32.z32, ||D32^-1 r32||^2), nblocks <= cap.
+  struct SweepTarget {
+    float *r32, *z32;
+    double *xw, *part;
+    unsigned int cap;
+  };
+  SweepTarget first_sweep_target();
 
   // FP64 CG split for graph capture: enqueue_cg queues the start and `budget` iterations without
   // any host synchronisation; finish_cg (same budget) waits, continues past the budget if needed
@@ -195,6 +207,7 @@ class Krylov {
   double *rh_ = nullptr, *v0_ = nullptr, *v1_ = nullptr, *sv_ = nullptr, *tv_ = nullptr;
   float *rh32_ = nullptr, *v032_ = nullptr, *v132_ = nullptr, *sv32_ = nullptr, *tv32_ = nullptr;
   double* part_ = nullptr;
+  double* stage_ = nullptr;  // first-stage sums of a caller's sweep partials (5 x kRedStage)
   KrylovCtl* ctl_ = nullptr;  // [0] main / outer, [1] inner
   KrylovCtl* host_ctl_ = nullptr;
   bool borrowed_ = false;  // workspace owned by the caller (no allocation, no autotuning)

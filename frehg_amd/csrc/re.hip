@@ -270,6 +270,58 @@ __global__ __launch_bounds__(NT) void k_re_check(K P, const double* __restrict__
   }
 }
 
+// k_re_assemble with the FP32 copies, fused with the first sweep of the mixed-precision solve
+// from x that follows it (the paired refinement sweep with the start of the inner FP32 CG): from
+// the row in registers, r = b - A x in FP64 (the sweep's order of terms), r32 = (float) r,
+// z32 = r32 / D32, xw = x (the copy that makes one refinement end in x), and per block the five
+// partials of that sweep at stride nblocks: ||D^-1 r||^2, ||D^-1 b||^2, r32.z32, z32.z32 and
+// ||D32^-1 r32||^2.
+template <class K, int ZF>
+__global__ __launch_bounds__(NT) void k_re_assemble_sweep(K P, const double* __restrict__ kr,
+                                                           const double* __restrict__ ch,
+                                                           const double* __restrict__ wcn,
+                                                           const double* __restrict__ hn,
+                                                           const double* __restrict__ x, double* __restrict__ D,
+                                                           double* __restrict__ ax, double* __restrict__ ay,
+                                                           double* __restrict__ az, double* __restrict__ rhs,
+                                                           float* __restrict__ D32, float* __restrict__ ax32,
+                                                           float* __restrict__ ay32, float* __restrict__ az32,
+                                                           double* __restrict__ xw, float* __restrict__ r32,
+                                                           float* __restrict__ z32, double* __restrict__ part,
+                                                           unsigned int nblocks) {
+  __shared__ double lds[NT / kWave];
+  TileId b;
+  const Cell t = tile_cell<ZF>(P.g, b);
+  double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (t.valid) {
+    const ReRow row = re_assemble_row(P, t, kr, ch, wcn, hn);
+    const long long c = t.c;
+    re_store_row(c, row, D, ax, ay, az, rhs, D32, ax32, ay32, az32);
+    const Geo& g = P.g;
+    const double xc = x[c];
+    double y = row.D * xc + row.xp * x[c + 1] + row.xm * x[c - 1] + row.yp * x[c + g.sy] + row.ym * x[c - g.sy];
+    y += row.zp * x[c + g.sz] + row.zm * x[c - g.sz];
+    const double r = row.rhs - y;
+    const double z = r / row.D, bz = row.rhs / row.D;
+    const float rT = (float)r, dd = (float)row.D;
+    const float zT = (float)((double)rT / (double)dd);
+    const double ib = (double)rT / (double)dd;
+    xw[c] = xc;
+    r32[c] = rT;
+    z32[c] = zT;
+    s[0] = z * z;
+    s[1] = bz * bz;
+    s[2] = (double)rT * (double)zT;
+    s[3] = (double)zT * (double)zT;
+    s[4] = ib * ib;
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const double v = block_sum<NT>(s[q], lds);
+    if (threadIdx.x == 0) part[q * nblocks + b.linear] = v;
+  }
+}
+
 __global__ __launch_bounds__(NT) void k_re_initial(ReK P, int from_head, double* h, double* wc, double* hn,
                                                     double* wcn, double* wch, double* hwc, double* ch) {
   const Cell t = cell_of(P.g);
@@ -319,6 +371,8 @@ void ReState::build(const frehg_re_params& p) {
   lazy_flux_ = !(lz && lz[0] == '0');
   const char* pc = std::getenv("FREHG_PICARD_CHECK");
   picard_check_ = !(pc && pc[0] == '0');
+  const char* as = std::getenv("FREHG_ASSEMBLE_SWEEP");
+  assemble_sweep_ = !(as && as[0] == '0');
   for (double** f : {&h_, &hn_, &hp_, &wc_, &wcn_, &wch_, &hwc_, &ch_, &Kx_, &Ky_, &Kz_, &qx_, &qy_, &qz_, &kr_,
                      &D_, &ax_, &ay_, &az_, &rhs_}) {
     FREHG_HIP(hipMalloc(f, bytes));
@@ -661,6 +715,30 @@ void ReState::step(frehg_re_stats* st) {
   // The learned (or extrapolated) head as the Picard loop's h^0: K and C at it (with its head-BC
   // ghosts), the assembly with the time-level-n terms, and the first stopping test against it
   // (kept in hp_).
+  const KrylovConfig kc = krylov_config(p_.solver, "frehg_re_step");
+  // The system at kr_, ch_ (FP32 copies when fp32), for a solve from x = h_ when from_h. A
+  // mixed-precision solve from h_ with fresh FP32 copies and CG inside takes its first sweep in
+  // the assembly kernel (the row in registers instead of re-read); returns the number of blocks
+  // of that sweep's partials for the solve (0: the solve takes the sweep itself).
+  auto assemble = [&](bool fp32, bool from_h) -> unsigned int {
+    const bool sweep = assemble_sweep_ && mixed && fp32 && from_h && kc.ksp == FREHG_KSP_CG;
+    const Krylov::SweepTarget st = sweep ? kry_->first_sweep_target() : Krylov::SweepTarget{};
+    if (sweep && nbl > st.cap) throw ArgError("frehg_re_step: sweep partials exceed the solver's capacity");
+    with_soil_variant(P, [&](const auto& PK) {
+      using K = std::decay_t<decltype(PK)>;
+      if (sweep)
+        hipLaunchKernelGGL((k_re_assemble_sweep<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
+                           (const double*)wcn_, (const double*)hn_, (const double*)h_, D_, ax_, ay_, az_, rhs_,
+                           D32_, ax32_, ay32_, az32_, st.xw, st.r32, st.z32, st.part, nbl);
+      else
+        hipLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
+                           (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, fp32 ? D32_ : nullptr,
+                           ax32_, ay32_, az32_);
+      FREHG_LAUNCH_CHECK();
+    });
+    return sweep ? nbl : 0u;
+  };
+  unsigned int swept = 0;
   if (start_h0) {
     with_soil_variant(P, [&](const auto& PK) {
       using K = std::decay_t<decltype(PK)>;
@@ -668,16 +746,12 @@ void ReState::step(frehg_re_stats* st) {
       hipLaunchKernelGGL((k_re_post_delta<K, 1>), grid, blk, 0, s_, PK, h_, kr_, ch_, wch_, (const double*)hn_, hp_,
                          red_, nbl);
       FREHG_LAUNCH_CHECK();
-      hipLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
-                         (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, mixed ? D32_ : nullptr,
-                         ax32_, ay32_, az32_);
-      FREHG_LAUNCH_CHECK();
     });
+    swept = assemble(mixed, true);
   }
-  const KrylovConfig kc = krylov_config(p_.solver, "frehg_re_step");
   StencilView<double> A{D_, ax_, ay_, az_};
   StencilView<float> A32{D32_, ax32_, ay32_, az32_};
-  SolveResult sr = kry_->solve(A, mixed ? &A32 : nullptr, rhs_, nullptr, h_, nullptr, x0_zero, guard, kc);
+  SolveResult sr = kry_->solve(A, mixed ? &A32 : nullptr, rhs_, nullptr, h_, nullptr, x0_zero, guard, kc, swept);
   // A learned Picard start whose residual did not drop below the zero guess's: the solve above
   // started from zero, but on the operator assembled at the rejected head. The loop restarts as
   // the plain method: h^0 = h^n, K and C at h^n, x0 = 0.
@@ -726,7 +800,7 @@ void ReState::step(frehg_re_stats* st) {
       FREHG_LAUNCH_CHECK();
     });
     post_done = true;
-    hipLaunchKernelGGL(k_reduce_maxmin, dim3(1), dim3(kRedThreads), 0, s_, (const double*)red_, nbl, dres_);
+    reduce_maxmin(red_, nbl, red2_, dres_, s_);
     FREHG_LAUNCH_CHECK();
     FREHG_HIP(hipMemcpyAsync(hres_, dres_, sizeof(double), hipMemcpyDeviceToHost, s_));
     FREHG_HIP(hipStreamSynchronize(s_));
@@ -769,15 +843,9 @@ void ReState::step(frehg_re_stats* st) {
     // its first residual check leaves the previous copy in place (its inner solves, if any,
     // would only serve refinements whose outer residuals use this iterate's FP64 operator).
     const bool fresh32 = mixed && (zero || refined);
-    with_soil_variant(P, [&](const auto& PK) {
-      using K = std::decay_t<decltype(PK)>;
-      hipLaunchKernelGGL((k_re_assemble<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
-                         (const double*)wcn_, (const double*)hn_, D_, ax_, ay_, az_, rhs_, fresh32 ? D32_ : nullptr,
-                         ax32_, ay32_, az32_);
-      FREHG_LAUNCH_CHECK();
-    });
+    const unsigned int swept_m = assemble(fresh32, !zero);
     post_done = false;
-    const SolveResult sm = kry_->solve(A, mixed ? &A32 : nullptr, rhs_, nullptr, h_, nullptr, zero, false, kc);
+    const SolveResult sm = kry_->solve(A, mixed ? &A32 : nullptr, rhs_, nullptr, h_, nullptr, zero, false, kc, swept_m);
     local.solve.iterations += sm.iterations;
     local.solve.refinements += sm.refinements;
     local.solve.residual_norm = sm.residual_norm;
@@ -863,15 +931,14 @@ void ReState::step(frehg_re_stats* st) {
   // ledger: [before, storage_ss, boundary_in, after, adjust] sums, then max |dh|
   reduce_sums(led_, nbl, 5, red2_, dres_ + 2, s_);
   FREHG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_maxmin, dim3(1), dim3(kRedThreads), 0, s_, (const double*)(led_ + 5 * (size_t)nbl), nbl,
-                     dres_ + 7);
+  reduce_maxmin(led_ + 5 * (size_t)nbl, nbl, red2_, dres_ + 7, s_);
   FREHG_LAUNCH_CHECK();
   if (p_.adaptive_dt) {
     materialize_fluxes();
     const unsigned int nb = grid.x * grid.y * grid.z;
     hipLaunchKernelGGL(k_re_dtadapt, grid, blk, 0, s_, P, qx_, qy_, qz_, wc_, red_, nb);
     FREHG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_maxmin, dim3(1), dim3(kRedThreads), 0, s_, (const double*)red_, nb, dres_);
+    reduce_maxmin(red_, nb, red2_, dres_, s_);
     FREHG_LAUNCH_CHECK();
     FREHG_HIP(hipMemcpyAsync(hres_, dres_, 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
     FREHG_HIP(hipStreamSynchronize(s_));

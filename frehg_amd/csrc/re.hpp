@@ -111,6 +111,9 @@ class ReState {
   // hn_, the soil, the top flux and the parameters with the step's dt, and is stored from them
   // when read or before anything changes those inputs.
   bool picard_check_ = true;
+  // The assembly takes the first sweep of the mixed-precision solve after it
+  // (FREHG_ASSEMBLE_SWEEP=0: the solve takes it).
+  bool assemble_sweep_ = true;
   bool op_pending_ = false;
   // Both lazy stores, before a change of the state.
   void flush_lazy() {

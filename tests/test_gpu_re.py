@@ -248,6 +248,45 @@ def test_picard_residual_test_before_assembly_equals_assemble_and_solve(prec, gu
         c.close()
 
 
+@pytest.mark.parametrize("guess", [abi.GUESS_PREVIOUS, abi.GUESS_EXTRAPOLATED])
+def test_assembly_with_the_first_refinement_sweep_matches_separate_kernels(guess):
+    """A mixed-precision Picard solve from a nonzero iterate takes its first refinement sweep
+    (r = b - A x in FP64, its FP32 copy, the start of the inner CG) in the assembly kernel, from
+    the row in registers (default); FREHG_ASSEMBLE_SWEEP=0 assembles and leaves the sweep to the
+    solver. The residual of each cell is the same; only the order of the sweep's partial sums
+    differs, so heads agree to 1e-12 m and every count of every step is the same."""
+    nx, ny, nz = 9, 7, 12
+    say(f"RE assembly + first sweep: guess {guess}")
+    soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5, Ss=1e-5)
+    p = abi.re_params(solver=abi.solver_config(1e-13, precision=abi.PREC_MIXED, inner_rtol=1e-5, max_refine=10),
+                      initial_guess=guess, soil=soil, bc_type=(0, 0, 0, 0, 0, 1), dx=0.5, dy=0.5, dz=0.05,
+                      dt=60.0, adaptive_dt=0, use_full3d=1, htop=-0.3, picard_max=40, picard_tol=1e-9)
+    k = np.arange(nz)[:, None, None]
+    i = np.arange(nx)[None, None, :]
+    h0 = np.broadcast_to(-1.5 - 0.05 * k - 0.02 * i, (nz, ny, nx)).ravel()
+    ctxs = []
+    os.environ["FREHG_SPMV_SHAPE"] = "2:12"
+    try:
+        for fused in ("0", "1"):
+            os.environ["FREHG_ASSEMBLE_SWEEP"] = fused
+            ctxs.append(re_pair(nx, ny, nz, p, h0=h0)[1])
+    finally:
+        del os.environ["FREHG_SPMV_SHAPE"]
+        del os.environ["FREHG_ASSEMBLE_SWEEP"]
+    a, b = ctxs
+    for s in range(5):
+        sa, sb = a.re_step(), b.re_step()
+        assert sa.picard_converged and sb.picard_converged
+        assert (sa.picard_iterations, sa.solve.refinements, sa.solve.fallback) == \
+            (sb.picard_iterations, sb.solve.refinements, sb.solve.fallback), s
+        assert abs(sa.solve.iterations - sb.solve.iterations) <= 1, s
+        assert np.abs(a.re_get("h") - b.re_get("h")).max() <= 1e-12, s
+        assert sa.mass_residual == pytest.approx(sb.mass_residual, abs=1e-12 * sa.mass_before)
+    assert np.abs(a.re_get("Gct") - b.re_get("Gct")).max() <= 1e-12 * np.abs(b.re_get("Gct")).max()
+    for c in (a, b):
+        c.close()
+
+
 def test_heterogeneous_soil_and_top_flux():
     nx, ny, nz = 9, 7, 10
     say("RE heterogeneous soil + per-column top flux")
