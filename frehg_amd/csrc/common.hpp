@@ -251,6 +251,7 @@ struct SoilClass {
   double wcm;    // VanGenuchten::wcm (VanGenuchten.hpp:21-28)
   double capc;   // alpha * n * m * (wcm - theta_r)
   double inv_m, inv_n, inv_alpha;
+  double ss_ts;  // Ss / theta_s
 };
 
 inline SoilClass make_soil_class(double alpha, double n, double theta_s, double theta_r, double Ks_x,
@@ -467,6 +468,7 @@ inline SoilClass make_soil_class(double alpha, double n, double theta_s, double 
   s.wcm = use_mvg ? theta_r + (theta_s - theta_r) * std::pow(1.0 + std::pow(std::fabs(air_entry) * alpha, n), s.m)
                   : theta_s;
   s.capc = alpha * n * s.m * (s.wcm - theta_r);
+  s.ss_ts = Ss / theta_s;
   s.inv_m = 1.0 / s.m;
   s.inv_n = 1.0 / n;
   s.inv_alpha = 1.0 / alpha;

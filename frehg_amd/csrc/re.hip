@@ -270,6 +270,52 @@ __global__ __launch_bounds__(NT) void k_re_check(K P, const double* __restrict__
   }
 }
 
+// k_re_check fused with k_re_flux_correct at the same iterate h (both form the same face
+// conductivities from kr and theta^n): the test's partials to cpart[0..2 nblocks), the flux and
+// corrector outputs and ledger partials as k_re_flux_correct writes them. When the test passes
+// the loop ends at h and these are the step's fluxes and corrector; when it fails the loop goes
+// on (nothing it reads is written here) and the flux kernel runs again at its last iterate.
+template <class K, int ZF>
+__global__ __launch_bounds__(NT) void k_re_check_flux_correct(
+    K P, const double* __restrict__ h, const double* __restrict__ kr, const double* __restrict__ ch,
+    const double* __restrict__ hn, const double* __restrict__ wcn, const double* __restrict__ wch,
+    double* __restrict__ Kx, double* __restrict__ Ky, double* __restrict__ Kz, double* __restrict__ qx,
+    double* __restrict__ qy, double* __restrict__ qz, double* __restrict__ wc, double* __restrict__ hwc,
+    unsigned char* __restrict__ sat0, unsigned char* __restrict__ cand, unsigned char* __restrict__ kf,
+    double* __restrict__ part, double* __restrict__ cpart, unsigned int nblocks, int* __restrict__ flags,
+    int nflags) {
+  __shared__ double lds[NT / kWave];
+  TileId b;
+  const Cell t = tile_cell<ZF>(P.g, b);
+  if (b.linear == 0 && (int)threadIdx.x < nflags) flags[threadIdx.x] = 0;  // reallocation sweep flags
+  double srr = 0.0, sbb = 0.0, l_before = 0.0, l_ss = 0.0, l_bnd = 0.0;
+  if (t.valid) {
+    const ReRow row = re_assemble_row(P, t, kr, ch, wcn, hn);
+    const Geo& g = P.g;
+    const long long c = t.c;
+    double y = row.D * h[c] + row.xp * h[c + 1] + row.xm * h[c - 1] + row.yp * h[c + g.sy] + row.ym * h[c - g.sy];
+    y += row.zp * h[c + g.sz] + row.zm * h[c - g.sz];
+    const double r = row.rhs - y;
+    const double z = r / row.D, bz = row.rhs / row.D;
+    srr = z * z;
+    sbb = bz * bz;
+    re_flux_correct(P, t, h, kr, hn, wcn, wch, Kx, Ky, Kz, qx, qy, qz, wc, hwc, sat0, cand, kf, l_before, l_ss,
+                    l_bnd);
+  }
+  const double a = block_sum<NT>(l_before, lds);
+  const double bs = block_sum<NT>(l_ss, lds);
+  const double d = block_sum<NT>(l_bnd, lds);
+  const double e = block_sum<NT>(srr, lds);
+  const double f = block_sum<NT>(sbb, lds);
+  if (threadIdx.x == 0) {
+    part[b.linear] = a;
+    part[nblocks + b.linear] = bs;
+    part[2 * nblocks + b.linear] = d;
+    cpart[b.linear] = e;
+    cpart[nblocks + b.linear] = f;
+  }
+}
+
 // k_re_assemble with the FP32 copies, fused with the first sweep of the mixed-precision solve
 // from x that follows it (the paired refinement sweep with the start of the inner FP32 CG): from
 // the row in registers, r = b - A x in FP64 (the sweep's order of terms), r32 = (float) r,
@@ -373,6 +419,8 @@ void ReState::build(const frehg_re_params& p) {
   picard_check_ = !(pc && pc[0] == '0');
   const char* as = std::getenv("FREHG_ASSEMBLE_SWEEP");
   assemble_sweep_ = !(as && as[0] == '0');
+  const char* cf = std::getenv("FREHG_CHECK_FLUX");
+  check_flux_ = !(cf && cf[0] == '0');
   for (double** f : {&h_, &hn_, &hp_, &wc_, &wcn_, &wch_, &hwc_, &ch_, &Kx_, &Ky_, &Kz_, &qx_, &qy_, &qz_, &kr_,
                      &D_, &ax_, &ay_, &az_, &rhs_}) {
     FREHG_HIP(hipMalloc(f, bytes));
@@ -791,7 +839,7 @@ void ReState::step(frehg_re_stats* st) {
   // when max |h^{m+1} - h^m| <= picard_tol (h^0 = h^n, or the learned head).
   // The post-solve kernel (head BCs, K, C and theta at the iterate) runs at every iterate with
   // the test fused in; at the last one it is the step's own post-solve evaluation.
-  bool post_done = false;
+  bool post_done = false, flux_done = false;
   for (int m = 1; p_.picard_max > 1; ++m) {
     const double* ref = (m == 1 && from_hn) ? hn_ : hp_;
     with_soil_variant(P, [&](const auto& PK) {
@@ -816,10 +864,20 @@ void ReState::step(frehg_re_stats* st) {
     // stored and solved. The stored system then lags one iterate behind; it is assembled when
     // read or before its inputs change (op_pending_).
     if (!zero && picard_check_) {
+      // With the fluxes and corrector at h^m taken in the same pass (the default): kept when the
+      // test passes.
+      hipEvent_t ev0 = check_flux_ ? prof_->start(FREHG_KT_GW_CORRECT) : nullptr;
+      hipEvent_t ev1 = check_flux_ ? prof_->stop() : nullptr;
       with_soil_variant(P, [&](const auto& PK) {
         using K = std::decay_t<decltype(PK)>;
-        hipLaunchKernelGGL((k_re_check<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
-                           (const double*)wcn_, (const double*)hn_, (const double*)h_, red_, nbl);
+        if (check_flux_)
+          hipExtLaunchKernelGGL((k_re_check_flux_correct<K, 1>), grid, blk, 0, s_, ev0, ev1, 0, PK,
+                                (const double*)h_, (const double*)kr_, (const double*)ch_, (const double*)hn_,
+                                (const double*)wcn_, (const double*)wch_, Kx_, Ky_, Kz_, qx_, qy_, qz_, wc_, hwc_,
+                                sat0_, cand_, lazy_flux_ ? kf_ : nullptr, led_, red_, nbl, dflag_, kSweepFlags);
+        else
+          hipLaunchKernelGGL((k_re_check<K, 1>), grid, blk, 0, s_, PK, (const double*)kr_, (const double*)ch_,
+                             (const double*)wcn_, (const double*)hn_, (const double*)h_, red_, nbl);
         FREHG_LAUNCH_CHECK();
       });
       reduce_sums(red_, nbl, 2, red2_, dres_ + 11, s_);
@@ -834,6 +892,7 @@ void ReState::step(frehg_re_stats* st) {
         local.picard_iterations = m + 1;
         local.picard_converged = 1;
         post_done = true;
+        flux_done = check_flux_;
         op_pending_ = true;
         break;
       }
@@ -870,7 +929,7 @@ void ReState::step(frehg_re_stats* st) {
     hipLaunchKernelGGL(k_copy_interior, grid, blk, 0, s_, g_, (const double*)h_, hp_);
     FREHG_LAUNCH_CHECK();
   }
-  {
+  if (!flux_done) {
     hipEvent_t ev0 = prof_->start(FREHG_KT_GW_CORRECT);
     hipEvent_t ev1 = prof_->stop();
     with_soil_variant(P, [&](const auto& PK) {

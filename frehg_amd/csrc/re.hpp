@@ -114,6 +114,9 @@ class ReState {
   // The assembly takes the first sweep of the mixed-precision solve after it
   // (FREHG_ASSEMBLE_SWEEP=0: the solve takes it).
   bool assemble_sweep_ = true;
+  // The Picard test before assembly shares its pass with the fluxes and corrector at the same
+  // iterate (FREHG_CHECK_FLUX=0: the test alone, the fluxes after the loop).
+  bool check_flux_ = true;
   bool op_pending_ = false;
   // Both lazy stores, before a change of the state.
   void flush_lazy() {

@@ -23,6 +23,11 @@ struct ReKT {
   double dx, dy, dz, dt, htop, hbot, qtop, qbot, co_max;
   int full3d, sim_sw, bc4, bc5, use_corrector;
   const double* qtop_field;  // surface padded or null
+  // Products of the grid spacing and dt that the per-cell code multiplies by instead of dividing
+  // by a spacing at every cell (set by make_rek): the coupling factors of the matrix (dt A / d per
+  // face, with the interior and half-cell top distances in z), dt V / dz of the gravity terms, and
+  // the area-over-distance factors of the Darcy fluxes.
+  double gx, gy, gzi, gzt, gdz, fx, fy, izi, izh;
   __device__ inline bool full() const {
     if constexpr (F3D == 2) return full3d != 0;
     else return F3D == 1;
@@ -54,6 +59,15 @@ __host__ __device__ inline ReKT<CLS, KS, F3D> rek_as(const ReK& P) {
   Q.bc5 = P.bc5;
   Q.use_corrector = P.use_corrector;
   Q.qtop_field = P.qtop_field;
+  Q.gx = P.gx;
+  Q.gy = P.gy;
+  Q.gzi = P.gzi;
+  Q.gzt = P.gzt;
+  Q.gdz = P.gdz;
+  Q.fx = P.fx;
+  Q.fy = P.fy;
+  Q.izi = P.izi;
+  Q.izh = P.izh;
   return Q;
 }
 
@@ -252,19 +266,15 @@ __device__ __forceinline__ ReRow re_assemble_row(const RK& P, const Cell& t, con
   const double Kyp = face_y(P, kr, wcn, c, t.j), Kym = face_y(P, kr, wcn, c - g.sy, t.j - 1);
   const double Kzp = face_z(P, kr, c, t.k), Kzm = face_z(P, kr, c - g.sz, t.k - 1);
   const SoilClass& s = P.soil.at(c);
-  const double dt = P.dt, dzk = P.dz;
-  const double Ax = P.dy * dzk, Ay = P.dx * dzk, V = P.dx * P.dy * dzk;
-  const double Gxp = -Kxp * dt * Ax / P.dx;
-  const double Gxm = -Kxm * dt * Ax / P.dx;
-  const double Gyp = -Kyp * dt * Ay / P.dy;
-  const double Gym = -Kym * dt * Ay / P.dy;
-  const double dzf_p = 0.5 * (dzk + dzk);
-  double Gzp = -Kzp * dt / (dzk * dzf_p);
-  Gzp *= V;
-  const double dzf_m = (t.k == 0) ? 0.5 * dzk : 0.5 * (dzk + dzk);
-  double Gzm = -Kzm * dt / (dzk * dzf_m);
-  Gzm *= V;
-  const double st = ch[c] + s.Ss * wcn[c] / s.theta_s;
+  const double V = P.dx * P.dy * P.dz;
+  // the reference's -K dt A / d per face, with dt A / d formed once per launch (make_rek)
+  const double Gxp = -Kxp * P.gx;
+  const double Gxm = -Kxm * P.gx;
+  const double Gyp = -Kyp * P.gy;
+  const double Gym = -Kym * P.gy;
+  const double Gzp = -Kzp * P.gzi;
+  const double Gzm = -Kzm * (t.k == 0 ? P.gzt : P.gzi);  // half-cell distance to the top face
+  const double st = ch[c] + wcn[c] * s.ss_ts;
   double Gct = st;
   Gct *= V;
   Gct -= (Gxp + Gxm + Gyp + Gym);
@@ -276,20 +286,20 @@ __device__ __forceinline__ ReRow re_assemble_row(const RK& P, const Cell& t, con
   }
   const double hxp = hn[c + 1], hxm = hn[c - 1], hyp = hn[c + g.sy], hym = hn[c - g.sy], hzp = hn[c + g.sz];
   double r = st * hn[c] * V;
-  r -= dt * V * Kzp / dzk;
-  r += dt * V * Kzm / dzk;
+  r -= Kzp * P.gdz;
+  r += Kzm * P.gdz;
   if (t.i == g.nx - 1) r -= Gxp * hxp;
   if (t.i == 0) r -= Gxm * hxm;
   if (t.j == g.ny - 1) r -= Gyp * hyp;
   if (t.j == 0) r -= Gym * hym;
   if (t.k == g.nz - 1) {
     if (P.bc4 == 1) r += Gzp * hzp;
-    if (P.bc4 == 2) r += dt * V * (P.qbot + Kzp) / dzk;
+    if (P.bc4 == 2) r += (P.qbot + Kzp) * P.gdz;
   }
   if (t.k == 0 && !P.sim_sw && P.bc5 == 1) r -= Gzm * P.htop;
   if (t.k == 0 && !P.sim_sw && P.bc5 == 2) {
     const double qcol = P.qtop_field ? P.qtop_field[g.sidx(t.i, t.j)] : P.qtop;
-    r += -dt * V * (qcol + Kzm) / dzk;
+    r -= (qcol + Kzm) * P.gdz;
   }
   ReRow row;
   row.D = Gct;
@@ -350,12 +360,12 @@ __device__ __forceinline__ void re_post(const RK& P, const Cell& t, double* h, d
 template <class RK>
 __device__ inline double flux_x(const RK& P, const double* h, double Kxp, long long c) {
 #pragma clang fp contract(off)
-  return (P.dy * P.dz) * Kxp * (h[c + 1] - h[c]) / P.dx;
+  return P.fx * Kxp * (h[c + 1] - h[c]);
 }
 template <class RK>
 __device__ inline double flux_y(const RK& P, const double* h, double Kyp, long long c) {
 #pragma clang fp contract(off)
-  return (P.dx * P.dz) * Kyp * (h[c + P.g.sy] - h[c]) / P.dy;
+  return P.fy * Kyp * (h[c + P.g.sy] - h[c]);
 }
 template <class RK>
 __device__ inline double flux_z(const RK& P, const double* h, const double* kr, long long c, int k) {
@@ -364,11 +374,11 @@ __device__ inline double flux_z(const RK& P, const double* h, const double* kr, 
   const double faceA = P.dx * P.dy;
   if (k == g.nz - 1 && P.bc4 == 2) return P.qbot * faceA;
   if (k == g.nz - 1 && P.bc4 == 0) return 0.0;
-  const double delta = (k == g.nz - 1) ? 0.5 * P.dz : 0.5 * (P.dz + P.dz);
+  const double idelta = (k == g.nz - 1) ? P.izh : P.izi;  // 1 / distance between the heads
   const double kc = P.soil.Ksz(c) * kr[c];
   const double ks = P.soil.Ksz(c + g.sz) * kr[c + g.sz];
   const double kface = 0.5 * (kc + ks);
-  return faceA * kface * ((h[c + g.sz] - h[c]) / delta - 1.0);
+  return faceA * kface * ((h[c + g.sz] - h[c]) * idelta - 1.0);
 }
 // Top-boundary flux of the column of top cell c (darcyFluxZ's k = -1 face, ReSolver.cpp:24-58).
 template <class RK>
@@ -382,7 +392,7 @@ __device__ inline double flux_top(const RK& P, const Cell& t, const double* h) {
     return qcol * faceA;
   }
   const double kface = (P.bc5 == 0) ? 0.0 : P.soil.Ksz(c);
-  return faceA * kface * ((h[c] - h[c - g.sz]) / (0.5 * P.dz) - 1.0);
+  return faceA * kface * ((h[c] - h[c - g.sz]) * P.izh - 1.0);
 }
 
 // Faces and fluxes of cell t at h^{n+1} (second computeKFace + groundwaterFlux,
@@ -482,12 +492,12 @@ __device__ __forceinline__ void correct_fluxes(const RK& P, const Cell& t, const
   double w;
   const double th = wch[c];
   if (P.use_corrector) {
-    const double coeff = V * (1.0 + s.Ss * (h[c] - hn[c]) / s.theta_s);
+    const double coeff = V * (1.0 + (h[c] - hn[c]) * s.ss_ts);
     const double dqx = P.dt * (qxp - qxm);
     const double dqy = P.dt * (qyp - qym);
     const double dqz = P.dt * (qzp - qzm);
     w = (wcn[c] * V + dqx + dqy + dqz) / coeff;
-    l_ss = -w * V * s.Ss * (h[c] - hn[c]) / s.theta_s;
+    l_ss = -w * V * (h[c] - hn[c]) * s.ss_ts;
   } else {
     w = th;
   }
@@ -685,6 +695,16 @@ inline ReK make_rek(const Geo& g, const SoilView& sv, const frehg_re_params& p, 
   P.bc5 = p.bc_type[5];
   P.use_corrector = p.use_corrector ? 1 : 0;
   P.qtop_field = qtop;
+  const double V = p.dx * p.dy * p.dz;
+  P.gx = p.dt * (p.dy * p.dz) / p.dx;
+  P.gy = p.dt * (p.dx * p.dz) / p.dy;
+  P.gzi = p.dt / (p.dz * p.dz) * V;
+  P.gzt = p.dt / (p.dz * (0.5 * p.dz)) * V;
+  P.gdz = p.dt * V / p.dz;
+  P.fx = (p.dy * p.dz) / p.dx;
+  P.fy = (p.dx * p.dz) / p.dy;
+  P.izi = 1.0 / p.dz;
+  P.izh = 1.0 / (0.5 * p.dz);
   return P;
 }
 

@@ -149,7 +149,16 @@ output: {{format: hdf5, filename: out/run.h5, variables: [water_depth, eta, head
 """
 
 
-def test_coupled_yaml_equals_the_call_by_call_run_and_the_oracle(tmp_path):
+@pytest.fixture
+def pinned_shape():
+    """One SpMV launch shape for every context: the autotuner times the candidates, and two
+    shapes sum a row in different orders (the last bits of a solve may differ)."""
+    os.environ["FREHG_SPMV_SHAPE"] = "2:8"
+    yield
+    del os.environ["FREHG_SPMV_SHAPE"]
+
+
+def test_coupled_yaml_equals_the_call_by_call_run_and_the_oracle(tmp_path, pinned_shape):
     nx, ny, nz = 24, 20, 8
     say("config front end: coupled 24x20x8 with a boundary polygon and a soil map")
     rng = np.random.default_rng(5)
@@ -210,15 +219,6 @@ def gpu_soil(setup):
     """The second soil class of the coupled configuration (the first is setup.re.soil)."""
     return abi.soil_params(theta_s=0.35, theta_r=0.08, alpha=1.5, n=1.6, Ks_x=3.0e-6, Ks_y=3.0e-6, Ks_z=3.0e-6,
                            Ss=1e-5, use_vg=1, use_mvg=0, air_entry=-0.02)
-
-
-@pytest.fixture
-def pinned_shape():
-    """One SpMV launch shape for every context: the autotuner times the candidates, and two
-    shapes sum a row in different orders (the last bits of a solve may differ)."""
-    os.environ["FREHG_SPMV_SHAPE"] = "2:8"
-    yield
-    del os.environ["FREHG_SPMV_SHAPE"]
 
 
 def test_coupled_yaml_with_solute_equals_the_call_by_call_run_and_the_oracle(tmp_path, pinned_shape):

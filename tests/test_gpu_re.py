@@ -196,19 +196,22 @@ def test_picard_outer_loop_matches_oracle(mode):
         compare_re(ctx, orc, rtol=1e-6, atol_h=1e-6, label=f"picard {mode}")
 
 
-@pytest.mark.parametrize("prec,guess,ksp", [(abi.PREC_FP64, abi.GUESS_PREVIOUS, abi.KSP_CG),
-                                            (abi.PREC_MIXED, abi.GUESS_EXTRAPOLATED, abi.KSP_CG),
-                                            (abi.PREC_MIXED, abi.GUESS_PREVIOUS, abi.KSP_BICGSTAB),
-                                            (abi.PREC_FP64, abi.GUESS_EXTRAPOLATED, abi.KSP_BICGSTAB)])
-def test_picard_residual_test_before_assembly_equals_assemble_and_solve(prec, guess, ksp):
+@pytest.mark.parametrize("prec,guess,ksp,lazy", [(abi.PREC_FP64, abi.GUESS_PREVIOUS, abi.KSP_CG, "1"),
+                                                 (abi.PREC_MIXED, abi.GUESS_EXTRAPOLATED, abi.KSP_CG, "1"),
+                                                 (abi.PREC_MIXED, abi.GUESS_EXTRAPOLATED, abi.KSP_CG, "0"),
+                                                 (abi.PREC_MIXED, abi.GUESS_PREVIOUS, abi.KSP_BICGSTAB, "1"),
+                                                 (abi.PREC_FP64, abi.GUESS_EXTRAPOLATED, abi.KSP_BICGSTAB, "1")])
+def test_picard_residual_test_before_assembly_equals_assemble_and_solve(prec, guess, ksp, lazy):
     """A Picard iterate h^m whose residual on the system at h^m already meets the solver's first
-    test ends the loop without the system being stored or solved (the default); with
-    FREHG_PICARD_CHECK=0 the system is stored and the solve returns h^m unchanged. Every field,
-    the system at the last iterate (stored when read) included, and every count of the step must
-    agree bit for bit, also after the dt of an adaptive step has changed and after a parameter
-    change forces the pending system out."""
+    test ends the loop without the system being stored or solved (the default), and the fluxes
+    and corrector taken in the same pass as the test are the step's; with FREHG_PICARD_CHECK=0 the
+    system is stored and the solve returns h^m unchanged, and the fluxes follow in a pass of their
+    own. Every field, the system at the last iterate (stored when read) included, and every count
+    of the step must agree bit for bit, also after the dt of an adaptive step has changed and after
+    a parameter change forces the pending system out; with lazy = "0" (FREHG_LAZY_FLUX=0) the
+    face conductivities and fluxes are stored in the step."""
     nx, ny, nz = 9, 7, 12
-    say(f"RE Picard residual test before assembly: precision {prec}, guess {guess}, ksp {ksp}")
+    say(f"RE Picard residual test before assembly: precision {prec}, guess {guess}, ksp {ksp}, lazy {lazy}")
     soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5, Ss=1e-5)
     kw = dict(soil=soil, bc_type=(0, 0, 0, 0, 0, 1), dx=0.5, dy=0.5, dz=0.05, dt=60.0, adaptive_dt=1, dt_max=90.0,
               dt_min=1.0, use_full3d=1, htop=-0.3, picard_max=40, picard_tol=1e-9)
@@ -219,6 +222,7 @@ def test_picard_residual_test_before_assembly_equals_assemble_and_solve(prec, gu
     h0 = np.broadcast_to(-1.5 - 0.05 * k - 0.02 * i, (nz, ny, nx)).ravel()
     ctxs = []
     os.environ["FREHG_SPMV_SHAPE"] = "2:12"
+    os.environ["FREHG_LAZY_FLUX"] = lazy
     try:
         for check in ("0", "1"):
             os.environ["FREHG_PICARD_CHECK"] = check
@@ -226,6 +230,7 @@ def test_picard_residual_test_before_assembly_equals_assemble_and_solve(prec, gu
     finally:
         del os.environ["FREHG_SPMV_SHAPE"]
         del os.environ["FREHG_PICARD_CHECK"]
+        del os.environ["FREHG_LAZY_FLUX"]
     a, b = ctxs
     for s in range(6):
         sa, sb = a.re_step(), b.re_step()
