@@ -916,19 +916,54 @@ void Mlp::predict_guess(ReState& re, double* h_out, const double* depth, const d
   hipEvent_t ev1 = prof_->stop();
   const bool gated = F.gate > 0.0 && F.inc_a != nullptr;
   if (gated) {  // the base pass marks the cells that need the network; the inference visits only their chunks
-    if (masks_cap_ < nchunks) {
-      if (masks_) FREHG_HIP(hipFree(masks_));
-      masks_ = nullptr;
-      masks_cap_ = 0;
-      FREHG_HIP(hipMalloc(&masks_, (size_t)nchunks * sizeof(unsigned long long)));
-      masks_cap_ = nchunks;
-    }
+    chunk_masks(nchunks);
     hipExtLaunchKernelGGL(k_guess_base, dim3(ceil_div(nchunks, MT / 64)), dim3(MT), 0, s_, ev0, nullptr, 0, F, h_out,
                           masks_);
     FREHG_LAUNCH_CHECK();
     ev0 = nullptr;
     F.masks = masks_;
   }
+  if (re.soil_plain_vg())
+    hipExtLaunchKernelGGL(k_mlp_guess<true>, dim3(nb), dim3(64 * GW), 0, s_, ev0, ev1, 0, (const float*)infer_,
+                          (const float*)norm_, F, h_out);
+  else
+    hipExtLaunchKernelGGL(k_mlp_guess<false>, dim3(nb), dim3(64 * GW), 0, s_, ev0, ev1, 0, (const float*)infer_,
+                          (const float*)norm_, F, h_out);
+  FREHG_LAUNCH_CHECK();
+}
+
+unsigned long long* Mlp::chunk_masks(long long nchunks) {
+  if (masks_cap_ < nchunks) {
+    if (masks_) FREHG_HIP(hipFree(masks_));
+    masks_ = nullptr;
+    masks_cap_ = 0;
+    FREHG_HIP(hipMalloc(&masks_, (size_t)nchunks * sizeof(unsigned long long)));
+    masks_cap_ = nchunks;
+  }
+  return masks_;
+}
+
+void Mlp::infer_masked(ReState& re, double* h_out, const double* depth, double* bad) {
+  if (!ready_) throw ArgError("learned guess requested but the model is not trained or loaded");
+  FeatSrc F = feat_src(re, depth);
+  F.gate = gate_;
+  F.bad = bad;
+  const int par = 1 - last_par_;  // the counter slots alternate as in predict_guess
+  last_par_ = par;
+  F.inferred = inferred_ + par;
+  F.inferred_next = inferred_ + (1 - par);
+  F.masks = masks_;
+  const long long nchunks = (long long)((F.g.nx + RB - 1) / RB) * F.g.ny * F.g.nz;
+  if (!masks_ || masks_cap_ < nchunks) throw ArgError("learned guess: chunk masks not set up");
+  if (nchunks >= (1LL << 31) - (long long)guess_blocks_ * GW) throw ArgError("learned guess: grid too large");
+  const unsigned nb = (unsigned)std::min<long long>(guess_blocks_, (nchunks + GW - 1) / GW);
+  if (infer_stale_) {
+    hipLaunchKernelGGL(k_mlp_prep_infer, dim3(1), dim3(MT), 0, s_, (const float*)params_, infer_);
+    FREHG_LAUNCH_CHECK();
+    infer_stale_ = false;
+  }
+  hipEvent_t ev0 = prof_->start(FREHG_KT_GW_GUESS);
+  hipEvent_t ev1 = prof_->stop();
   if (re.soil_plain_vg())
     hipExtLaunchKernelGGL(k_mlp_guess<true>, dim3(nb), dim3(64 * GW), 0, s_, ev0, ev1, 0, (const float*)infer_,
                           (const float*)norm_, F, h_out);

@@ -66,6 +66,13 @@ class Mlp {
   // increment; the first case sets *bad = 1 (bad may be null).
   void predict_guess(ReState& re, double* h_out, const double* depth, const double* inc_a, const double* inc_b,
                      const double* hn = nullptr, const double* wcn = nullptr, double* bad = nullptr);
+  // The gated start split for a caller that takes the base pass itself: chunk_masks(nchunks) is
+  // the buffer of per-chunk masks (64-cell row chunks along x, then rows, then layers; bit l:
+  // the network corrects cell l of the chunk) that the caller fills while writing h^n + base
+  // increment to h_out; infer_masked then runs the network on the chunks with a set bit, as the
+  // second pass of predict_guess does.
+  unsigned long long* chunk_masks(long long nchunks);
+  void infer_masked(ReState& re, double* h_out, const double* depth, double* bad);
   // Inference gate in metres (0: the network corrects every cell).
   void set_gate(double gate) { gate_ = gate; }
   double gate() const { return gate_; }

@@ -270,6 +270,71 @@ __global__ __launch_bounds__(NT) void k_re_check(K P, const double* __restrict__
   }
 }
 
+// Cell `lane` of 64-cell row chunk ch (chunks numbered along x, then rows j, then layers k: the
+// chunks of the learned start's inference).
+__device__ inline Cell chunk_cell(const Geo& g, long long ch, int lane) {
+  const int iblocks = (g.nx + 63) / 64;
+  const long long row = ch / iblocks;
+  Cell t;
+  t.i = (int)(ch - row * iblocks) * 64 + lane;
+  t.j = (int)(row % g.ny);
+  t.k = (int)(row / g.ny);
+  t.valid = t.i < g.nx;
+  t.c = t.valid ? g.idx(t.i, t.j, t.k) : 0;
+  return t;
+}
+
+// The Picard loop's first iterate h^0 = h^n + (hp - h) (h^n plus the previous step's solve
+// increment: hp holds that step's post-solve head and h its starting head) fused with the
+// post-solve pass at h^0 that follows it (k_re_post_delta's work, without its unused test):
+// head-BC ghosts, K, C and theta at h^0, and hp = h^0 for the first Picard test. One wave per
+// 64-cell row chunk. GATED (the learned start): the cells whose |hp - h| exceeds the gate are
+// marked in masks[chunk]; a chunk with a marked cell gets h^0 only, its post-solve pass waits for
+// the inference (k_re_post_chunks). hp and h are read and written at the thread's own cell only.
+template <class K, bool GATED>
+__global__ __launch_bounds__(NT) void k_re_start_post(K P, const double* __restrict__ hn, double* hp, double* h,
+                                                       double* __restrict__ kr, double* __restrict__ ch,
+                                                       double* __restrict__ wch, double gate,
+                                                       unsigned long long* __restrict__ masks, long long nchunks) {
+  const long long q = (long long)blockIdx.x * (NT / kWave) + (threadIdx.x >> 6);
+  if (q >= nchunks) return;  // wave-uniform
+  const int lane = threadIdx.x & 63;
+  const Cell t = chunk_cell(P.g, q, lane);
+  double hv = 0.0;
+  bool need = false;
+  if (t.valid) {
+    const double inc = hp[t.c] - h[t.c];
+    hv = hn[t.c] + inc;
+    h[t.c] = hv;
+    need = GATED && fabs(inc) > gate;
+  }
+  if (GATED) {
+    const unsigned long long m = __ballot(need);
+    if (lane == 0) masks[q] = m;
+    if (m != 0ull) return;  // wave-uniform
+  }
+  if (t.valid) {
+    hp[t.c] = hv;
+    re_post(P, t, h, kr, ch, wch);
+  }
+}
+
+// The post-solve pass at h^0 (as in k_re_start_post) for the chunks with a marked cell, after the
+// inference has written their heads.
+template <class K>
+__global__ __launch_bounds__(NT) void k_re_post_chunks(K P, const unsigned long long* __restrict__ masks,
+                                                        long long nchunks, double* h, double* __restrict__ kr,
+                                                        double* __restrict__ ch, double* __restrict__ wch,
+                                                        double* __restrict__ hm) {
+  const long long q = (long long)blockIdx.x * (NT / kWave) + (threadIdx.x >> 6);
+  if (q >= nchunks || masks[q] == 0ull) return;  // wave-uniform
+  const Cell t = chunk_cell(P.g, q, threadIdx.x & 63);
+  if (t.valid) {
+    hm[t.c] = h[t.c];
+    re_post(P, t, h, kr, ch, wch);
+  }
+}
+
 // k_re_check fused with k_re_flux_correct at the same iterate h (both form the same face
 // conductivities from kr and theta^n): the test's partials to cpart[0..2 nblocks), the flux and
 // corrector outputs and ledger partials as k_re_flux_correct writes them. When the test passes
@@ -421,6 +486,8 @@ void ReState::build(const frehg_re_params& p) {
   assemble_sweep_ = !(as && as[0] == '0');
   const char* cf = std::getenv("FREHG_CHECK_FLUX");
   check_flux_ = !(cf && cf[0] == '0');
+  const char* fs = std::getenv("FREHG_FUSED_START");
+  fused_start_ = !(fs && fs[0] == '0');
   for (double** f : {&h_, &hn_, &hp_, &wc_, &wcn_, &wch_, &hwc_, &ch_, &Kx_, &Ky_, &Kz_, &qx_, &qy_, &qz_, &kr_,
                      &D_, &ax_, &ay_, &az_, &rhs_}) {
     FREHG_HIP(hipMalloc(f, bytes));
@@ -744,19 +811,51 @@ void ReState::step(frehg_re_stats* st) {
   // Initial guess in h (the solve's x): zero (PETSc default), h^n, h^n + the previous step's
   // solve increment, or h^n + MLP increment.
   bool x0_zero = true, guard = false;
+  bool post0 = false;  // the start kernels took the post-solve pass at h^0
+  const long long nchunks = (long long)ceil_div(g_.nx, 64) * g_.ny * g_.nz;  // 64-cell row chunks
   if (p_.initial_guess == FREHG_GUESS_PREVIOUS || (p_.initial_guess == FREHG_GUESS_EXTRAPOLATED && !have_inc)) {
     hipLaunchKernelGGL(k_copy_interior, grid, blk, 0, s_, g_, hn_, h_);
     FREHG_LAUNCH_CHECK();
     x0_zero = false;
   } else if (p_.initial_guess == FREHG_GUESS_EXTRAPOLATED) {
-    hipLaunchKernelGGL(k_extrap_interior, grid, blk, 0, s_, g_, (const double*)hn_, (const double*)hp_, h_);
-    FREHG_LAUNCH_CHECK();
+    if (extrap_h0 && fused_start_) {  // with the post-solve pass at h^0
+      with_soil_variant(P, [&](const auto& PK) {
+        using K = std::decay_t<decltype(PK)>;
+        hipLaunchKernelGGL((k_re_start_post<K, false>), dim3(ceil_div(nchunks, NT / kWave)), blk, 0, s_, PK,
+                           (const double*)hn_, hp_, h_, kr_, ch_, wch_, 0.0, nullptr, nchunks);
+        FREHG_LAUNCH_CHECK();
+      });
+      post0 = true;
+    } else {
+      hipLaunchKernelGGL(k_extrap_interior, grid, blk, 0, s_, g_, (const double*)hn_, (const double*)hp_, h_);
+      FREHG_LAUNCH_CHECK();
+    }
     x0_zero = false;
   } else if (p_.initial_guess == FREHG_GUESS_LEARNED) {
     if (!mlp_ || !mlp_->ready()) throw ArgError("frehg_re_step: learned guess requested but no trained model loaded");
     FREHG_HIP(hipMemsetAsync(dres_ + 10, 0, sizeof(double), s_));
-    mlp_->predict_guess(*this, h_, depth_, have_inc ? hp_ : nullptr, have_inc ? h_ : nullptr, nullptr, nullptr,
-                        dres_ + 10);
+    if (learned_h0 && have_inc && mlp_->gate() > 0.0 && fused_start_) {
+      // the gated start's base pass with the post-solve pass at h^0 of the chunks the network
+      // leaves alone; the network's chunks get theirs after the inference
+      unsigned long long* masks = mlp_->chunk_masks(nchunks);
+      with_soil_variant(P, [&](const auto& PK) {
+        using K = std::decay_t<decltype(PK)>;
+        hipLaunchKernelGGL((k_re_start_post<K, true>), dim3(ceil_div(nchunks, NT / kWave)), blk, 0, s_, PK,
+                           (const double*)hn_, hp_, h_, kr_, ch_, wch_, mlp_->gate(), masks, nchunks);
+        FREHG_LAUNCH_CHECK();
+      });
+      mlp_->infer_masked(*this, h_, depth_, dres_ + 10);
+      with_soil_variant(P, [&](const auto& PK) {
+        using K = std::decay_t<decltype(PK)>;
+        hipLaunchKernelGGL((k_re_post_chunks<K>), dim3(ceil_div(nchunks, NT / kWave)), blk, 0, s_, PK,
+                           (const unsigned long long*)masks, nchunks, h_, kr_, ch_, wch_, hp_);
+        FREHG_LAUNCH_CHECK();
+      });
+      post0 = true;
+    } else {
+      mlp_->predict_guess(*this, h_, depth_, have_inc ? hp_ : nullptr, have_inc ? h_ : nullptr, nullptr, nullptr,
+                          dres_ + 10);
+    }
     x0_zero = false;
     guard = true;
   }
@@ -788,13 +887,14 @@ void ReState::step(frehg_re_stats* st) {
   };
   unsigned int swept = 0;
   if (start_h0) {
-    with_soil_variant(P, [&](const auto& PK) {
-      using K = std::decay_t<decltype(PK)>;
-      // post-solve pass at h^0, which also keeps h^0 in hp_ (its test output is not used)
-      hipLaunchKernelGGL((k_re_post_delta<K, 1>), grid, blk, 0, s_, PK, h_, kr_, ch_, wch_, (const double*)hn_, hp_,
-                         red_, nbl);
-      FREHG_LAUNCH_CHECK();
-    });
+    if (!post0)
+      with_soil_variant(P, [&](const auto& PK) {
+        using K = std::decay_t<decltype(PK)>;
+        // post-solve pass at h^0, which also keeps h^0 in hp_ (its test output is not used)
+        hipLaunchKernelGGL((k_re_post_delta<K, 1>), grid, blk, 0, s_, PK, h_, kr_, ch_, wch_, (const double*)hn_,
+                           hp_, red_, nbl);
+        FREHG_LAUNCH_CHECK();
+      });
     swept = assemble(mixed, true);
   }
   StencilView<double> A{D_, ax_, ay_, az_};

@@ -117,6 +117,9 @@ class ReState {
   // The Picard test before assembly shares its pass with the fluxes and corrector at the same
   // iterate (FREHG_CHECK_FLUX=0: the test alone, the fluxes after the loop).
   bool check_flux_ = true;
+  // The extrapolated and gated learned starts take the post-solve pass at h^0 in their own
+  // kernels (FREHG_FUSED_START=0: a pass of its own after the start).
+  bool fused_start_ = true;
   bool op_pending_ = false;
   // Both lazy stores, before a change of the state.
   void flush_lazy() {

@@ -130,16 +130,21 @@ def test_gated_start_equals_the_network_where_the_gate_opens(model):
     ctx.close()
 
 
-def test_picard_residual_test_before_assembly_in_the_learned_coupled_step(model, monkeypatch):
-    """The learned start's Picard loop usually ends on the residual test of its second iterate,
-    taken before that iterate's system is stored (the default); with FREHG_PICARD_CHECK=0 the
-    system is stored and solved and the solve returns the iterate unchanged. Heads, depths, the
-    system at the last iterate and every count agree bit for bit over the held-out run."""
+@pytest.mark.parametrize("switch", ["FREHG_PICARD_CHECK", "FREHG_FUSED_START"])
+def test_fused_paths_of_the_learned_coupled_step_equal_the_separate_kernels(model, monkeypatch, switch):
+    """Two fusions of the learned step against the separate kernels, bit for bit over the held-out
+    run (heads, depths, the system at the last iterate and every count):
+    FREHG_PICARD_CHECK: the Picard loop usually ends on the residual test of its second iterate,
+    taken before that iterate's system is stored (with the fluxes and corrector in the same pass);
+    with the switch at 0 the system is stored and solved and the solve returns the iterate
+    unchanged. FREHG_FUSED_START: the gated start's base pass takes the post-solve pass at h^0 of
+    the chunks the network leaves alone, the network's chunks get theirs after the inference; at
+    0 the post-solve pass at h^0 is a pass of its own."""
     held = case(7, 8)
     monkeypatch.setenv("FREHG_SPMV_SHAPE", "2:12")
     out = []
     for check in ("0", "1"):
-        monkeypatch.setenv("FREHG_PICARD_CHECK", check)
+        monkeypatch.setenv(switch, check)
         ctx = gpu.Context(0)
         ctx.mlp_set_params(*model)
         ctx.mlp_gate(1e-6)

@@ -253,6 +253,41 @@ def test_picard_residual_test_before_assembly_equals_assemble_and_solve(prec, gu
         c.close()
 
 
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_MIXED])
+def test_extrapolated_start_with_its_post_solve_pass_equals_separate_kernels(prec):
+    """The extrapolated Picard start h^n + (last post-solve head - last starting head) is formed in
+    the kernel that takes the post-solve pass at it (default); FREHG_FUSED_START=0 forms it and
+    then runs the post-solve pass. Every field and count agrees bit for bit."""
+    nx, ny, nz = 9, 7, 12
+    say(f"RE extrapolated start fused with its post-solve pass: precision {prec}")
+    soil = abi.soil_params(alpha=2.0, n=1.8, theta_s=0.40, theta_r=0.05, Ks_x=1e-5, Ks_y=1e-5, Ks_z=1e-5, Ss=1e-5)
+    p = abi.re_params(solver=abi.solver_config(1e-13, precision=prec, inner_rtol=1e-5, max_refine=10),
+                      initial_guess=abi.GUESS_EXTRAPOLATED, soil=soil, bc_type=(0, 0, 0, 0, 0, 1), dx=0.5, dy=0.5,
+                      dz=0.05, dt=60.0, adaptive_dt=0, use_full3d=1, htop=0.05, picard_max=40, picard_tol=1e-9)
+    k = np.arange(nz)[:, None, None]
+    i = np.arange(nx)[None, None, :]
+    h0 = np.broadcast_to(-0.05 - 0.08 * k - 0.01 * i, (nz, ny, nx)).ravel()
+    ctxs = []
+    os.environ["FREHG_SPMV_SHAPE"] = "2:12"
+    try:
+        for fused in ("0", "1"):
+            os.environ["FREHG_FUSED_START"] = fused
+            ctxs.append(re_pair(nx, ny, nz, p, h0=h0)[1])
+    finally:
+        del os.environ["FREHG_SPMV_SHAPE"]
+        del os.environ["FREHG_FUSED_START"]
+    a, b = ctxs
+    for s in range(6):
+        sa, sb = a.re_step(), b.re_step()
+        assert sa.picard_converged and sb.picard_converged
+        assert (sa.picard_iterations, sa.solve.iterations, sa.solve.refinements, sa.mass_residual) == \
+            (sb.picard_iterations, sb.solve.iterations, sb.solve.refinements, sb.mass_residual), s
+        for f in RE_COMPARE:
+            assert np.array_equal(a.re_get(f), b.re_get(f)), f"step {s} field {f}"
+    for c in (a, b):
+        c.close()
+
+
 @pytest.mark.parametrize("guess", [abi.GUESS_PREVIOUS, abi.GUESS_EXTRAPOLATED])
 def test_assembly_with_the_first_refinement_sweep_matches_separate_kernels(guess):
     """A mixed-precision Picard solve from a nonzero iterate takes its first refinement sweep
