@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # L2<->fabric bytes of one FP64 CG SpMV launch (k_spmv_pair<double>) on the 512x512x30 system
 # per cell, from two rocprofv3 --pmc runs of this script with --copy-probe (FETCH_SIZE x2.000 and
 # WRITE_SIZE x1.000, the scale factors that a 1 GiB read + 1 GiB write elementwise kernel needs).
-SPMV64_TRAFFIC_BYTES_PER_CELL = 561.4e6 / 7864320
+SPMV64_TRAFFIC_BYTES_PER_CELL = 554.7e6 / 7864320
 F16_PEAK_TFLOPS = 2500.0  # dense MFMA, FP16 (the learned-guess inference) and BF16 alike
 
 
