@@ -172,7 +172,9 @@ int frehg_re_set_top_flux(frehg_ctx_t ctx, const double* q_interior);
  * interior values i + j*nx + k*nx*ny; exactly one of h / wc non-NULL. */
 int frehg_re_set_initial(frehg_ctx_t ctx, const double* h_interior, const double* wc_interior);
 /* Direct access to any GwFields array by name (src/re/GwFields.hpp:62-74), halo padded.
- * Names: h hn wc wcn wch hwc ch Kx Ky Kz qx qy qz Gct Grhs (Gct/Grhs after the last assembly). */
+ * Names: h hn wc wcn wch hwc ch Kx Ky Kz qx qy qz Gct Grhs. Gct/Grhs are the system of the
+ * step's last Picard iterate (the last assembly; a loop that ended on the residual test of an
+ * iterate without storing its system forms it when it is read). */
 int frehg_re_set_field(frehg_ctx_t ctx, const char* name, const double* src);
 int frehg_re_get_field(frehg_ctx_t ctx, const char* name, double* dst);
 /* ReSolver::advanceStep (ReSolver.cpp:931-969). */
