@@ -960,22 +960,17 @@ void ReState::step(frehg_re_stats* st) {
     post_done = true;
     reduce_maxmin(red_, nbl, red2_, dres_, s_);
     FREHG_LAUNCH_CHECK();
-    FREHG_HIP(hipMemcpyAsync(hres_, dres_, sizeof(double), hipMemcpyDeviceToHost, s_));
-    FREHG_HIP(hipStreamSynchronize(s_));
-    if (hres_[0] <= p_.picard_tol) {
-      local.picard_converged = 1;
-      break;
-    }
-    if (m >= p_.picard_max) break;  // the cap: the step continues from the last iterate
     const bool zero = p_.initial_guess == FREHG_GUESS_ZERO;
     // A solve from h^m first tests the residual of h^m on the system assembled at h^m and returns
     // h^m unchanged when the test passes, which ends the loop (below). That test is taken here
     // on rows formed in registers, without storing the system; only when it fails is the system
     // stored and solved. The stored system then lags one iterate behind; it is assembled when
-    // read or before its inputs change (op_pending_).
-    if (!zero && picard_check_) {
-      // With the fluxes and corrector at h^m taken in the same pass (the default): kept when the
-      // test passes.
+    // read or before its inputs change (op_pending_). The residual test is queued behind the
+    // Picard test without waiting for its outcome, so one host round trip serves both; with the
+    // fluxes and corrector at h^m taken in the same pass (the default), those are the step's when
+    // either test ends the loop at h^m.
+    const bool spec = !zero && picard_check_ && m < p_.picard_max;
+    if (spec) {
       hipEvent_t ev0 = check_flux_ ? prof_->start(FREHG_KT_GW_CORRECT) : nullptr;
       hipEvent_t ev1 = check_flux_ ? prof_->stop() : nullptr;
       with_soil_variant(P, [&](const auto& PK) {
@@ -992,8 +987,18 @@ void ReState::step(frehg_re_stats* st) {
       });
       reduce_sums(red_, nbl, 2, red2_, dres_ + 11, s_);
       FREHG_LAUNCH_CHECK();
-      FREHG_HIP(hipMemcpyAsync(hres_ + 11, dres_ + 11, 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
-      FREHG_HIP(hipStreamSynchronize(s_));
+    }
+    // [0] the Picard test's max |dh|, [11..12] the residual test's sums (the slots between are
+    // rewritten by the ledger copy at the end of the step)
+    FREHG_HIP(hipMemcpyAsync(hres_, dres_, (spec ? 13 : 1) * sizeof(double), hipMemcpyDeviceToHost, s_));
+    FREHG_HIP(hipStreamSynchronize(s_));
+    if (hres_[0] <= p_.picard_tol) {
+      local.picard_converged = 1;
+      flux_done = spec && check_flux_;
+      break;
+    }
+    if (m >= p_.picard_max) break;  // the cap: the step continues from the last iterate
+    if (spec) {
       const double rr = hres_[11], bb = hres_[12];
       const double ttol = std::fmax(kc.rtol * std::sqrt(bb), kc.atol);
       if (rr <= ttol * ttol) {
