@@ -320,28 +320,18 @@ __global__ __launch_bounds__(NT) void k_re_start_post(K P, const double* __restr
 }
 
 // The post-solve pass at h^0 (as in k_re_start_post) for the chunks with a marked cell, after the
-// inference has written their heads. Each wave looks at kPostChunks consecutive chunks (their
-// masks in one load by the first lanes) and visits the marked ones, so that the unmarked majority
-// costs no wave of its own.
-constexpr int kPostChunks = 8;
+// inference has written their heads.
 template <class K>
 __global__ __launch_bounds__(NT) void k_re_post_chunks(K P, const unsigned long long* __restrict__ masks,
                                                         long long nchunks, double* h, double* __restrict__ kr,
                                                         double* __restrict__ ch, double* __restrict__ wch,
                                                         double* __restrict__ hm) {
-  const int lane = threadIdx.x & 63;
-  const long long q0 = ((long long)blockIdx.x * (NT / kWave) + (threadIdx.x >> 6)) * kPostChunks;
-  if (q0 >= nchunks) return;  // wave-uniform
-  const bool marked = lane < kPostChunks && q0 + lane < nchunks && masks[q0 + lane] != 0ull;
-  unsigned long long todo = __ballot(marked);
-  while (todo) {  // wave-uniform
-    const int l = __ffsll((long long)todo) - 1;
-    todo &= todo - 1ull;
-    const Cell t = chunk_cell(P.g, q0 + l, lane);
-    if (t.valid) {
-      hm[t.c] = h[t.c];
-      re_post(P, t, h, kr, ch, wch);
-    }
+  const long long q = (long long)blockIdx.x * (NT / kWave) + (threadIdx.x >> 6);
+  if (q >= nchunks || masks[q] == 0ull) return;  // wave-uniform
+  const Cell t = chunk_cell(P.g, q, threadIdx.x & 63);
+  if (t.valid) {
+    hm[t.c] = h[t.c];
+    re_post(P, t, h, kr, ch, wch);
   }
 }
 
@@ -857,7 +847,7 @@ void ReState::step(frehg_re_stats* st) {
       mlp_->infer_masked(*this, h_, depth_, dres_ + 10);
       with_soil_variant(P, [&](const auto& PK) {
         using K = std::decay_t<decltype(PK)>;
-        hipLaunchKernelGGL((k_re_post_chunks<K>), dim3(ceil_div(nchunks, (NT / kWave) * kPostChunks)), blk, 0, s_, PK,
+        hipLaunchKernelGGL((k_re_post_chunks<K>), dim3(ceil_div(nchunks, NT / kWave)), blk, 0, s_, PK,
                            (const unsigned long long*)masks, nchunks, h_, kr_, ch_, wch_, hp_);
         FREHG_LAUNCH_CHECK();
       });
