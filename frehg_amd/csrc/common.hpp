@@ -82,6 +82,29 @@ __device__ inline double block_sum(double v, double* lds) {
   return s;
 }
 
+// N sums over the block with one round of LDS traffic; each result equals block_sum's bit for bit
+// (the same wave sums, added in the same order). `lds` holds >= N * blockDim/64 doubles.
+template <int NT, int N>
+__device__ inline void block_sums(double (&v)[N], double* lds) {
+  constexpr int NW = NT / kWave;
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) lds[q * NW + w] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) s += lds[q * NW + i];
+    v[q] = s;
+  }
+}
+
 // Deterministic sum of n partials by one block of NT threads: each thread keeps eight
 // accumulators over a fixed stride pattern (so eight loads are in flight, not one), then a tree.
 template <int NT>

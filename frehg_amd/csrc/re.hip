@@ -96,7 +96,7 @@ __global__ __launch_bounds__(NT) void k_re_flux_correct(K P, const double* __res
                                                          unsigned char* __restrict__ cand,
                                                          unsigned char* __restrict__ kf, double* __restrict__ part,
                                                          unsigned int nblocks, int* __restrict__ flags, int nflags) {
-  __shared__ double lds[NT / kWave];
+  __shared__ double lds3[3 * (NT / kWave)];
   TileId b;
   const Cell t = tile_cell<ZF>(P.g, b);
   if (b.linear == 0 && (int)threadIdx.x < nflags) flags[threadIdx.x] = 0;  // reallocation sweep flags
@@ -104,13 +104,12 @@ __global__ __launch_bounds__(NT) void k_re_flux_correct(K P, const double* __res
   if (t.valid)
     re_flux_correct(P, t, h, kr, hn, wcn, wch, Kx, Ky, Kz, qx, qy, qz, wc, hwc, sat0, cand, kf, l_before, l_ss,
                     l_bnd);
-  const double a = block_sum<NT>(l_before, lds);
-  const double bs = block_sum<NT>(l_ss, lds);
-  const double d = block_sum<NT>(l_bnd, lds);
+  double v[3] = {l_before, l_ss, l_bnd};
+  block_sums<NT, 3>(v, lds3);
   if (threadIdx.x == 0) {
-    part[b.linear] = a;
-    part[nblocks + b.linear] = bs;
-    part[2 * nblocks + b.linear] = d;
+    part[b.linear] = v[0];
+    part[nblocks + b.linear] = v[1];
+    part[2 * nblocks + b.linear] = v[2];
   }
 }
 
@@ -349,7 +348,7 @@ __global__ __launch_bounds__(NT) void k_re_check_flux_correct(
     unsigned char* __restrict__ sat0, unsigned char* __restrict__ cand, unsigned char* __restrict__ kf,
     double* __restrict__ part, double* __restrict__ cpart, unsigned int nblocks, int* __restrict__ flags,
     int nflags) {
-  __shared__ double lds[NT / kWave];
+  __shared__ double lds5[5 * (NT / kWave)];
   TileId b;
   const Cell t = tile_cell<ZF>(P.g, b);
   if (b.linear == 0 && (int)threadIdx.x < nflags) flags[threadIdx.x] = 0;  // reallocation sweep flags
@@ -367,17 +366,14 @@ __global__ __launch_bounds__(NT) void k_re_check_flux_correct(
     re_flux_correct(P, t, h, kr, hn, wcn, wch, Kx, Ky, Kz, qx, qy, qz, wc, hwc, sat0, cand, kf, l_before, l_ss,
                     l_bnd);
   }
-  const double a = block_sum<NT>(l_before, lds);
-  const double bs = block_sum<NT>(l_ss, lds);
-  const double d = block_sum<NT>(l_bnd, lds);
-  const double e = block_sum<NT>(srr, lds);
-  const double f = block_sum<NT>(sbb, lds);
+  double v[5] = {l_before, l_ss, l_bnd, srr, sbb};
+  block_sums<NT, 5>(v, lds5);
   if (threadIdx.x == 0) {
-    part[b.linear] = a;
-    part[nblocks + b.linear] = bs;
-    part[2 * nblocks + b.linear] = d;
-    cpart[b.linear] = e;
-    cpart[nblocks + b.linear] = f;
+    part[b.linear] = v[0];
+    part[nblocks + b.linear] = v[1];
+    part[2 * nblocks + b.linear] = v[2];
+    cpart[b.linear] = v[3];
+    cpart[nblocks + b.linear] = v[4];
   }
 }
 
@@ -400,7 +396,7 @@ __global__ __launch_bounds__(NT) void k_re_assemble_sweep(K P, const double* __r
                                                            double* __restrict__ xw, float* __restrict__ r32,
                                                            float* __restrict__ z32, double* __restrict__ part,
                                                            unsigned int nblocks) {
-  __shared__ double lds[NT / kWave];
+  __shared__ double lds5[5 * (NT / kWave)];
   TileId b;
   const Cell t = tile_cell<ZF>(P.g, b);
   double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -426,10 +422,10 @@ __global__ __launch_bounds__(NT) void k_re_assemble_sweep(K P, const double* __r
     s[3] = (double)zT * (double)zT;
     s[4] = ib * ib;
   }
+  block_sums<NT, 5>(s, lds5);
+  if (threadIdx.x == 0) {
 #pragma unroll
-  for (int q = 0; q < 5; ++q) {
-    const double v = block_sum<NT>(s[q], lds);
-    if (threadIdx.x == 0) part[q * nblocks + b.linear] = v;
+    for (int q = 0; q < 5; ++q) part[q * nblocks + b.linear] = s[q];
   }
 }
 
