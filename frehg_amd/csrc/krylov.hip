@@ -103,7 +103,7 @@ __global__ __launch_bounds__(NT) void k_init(Geo g, int kchunk, StencilView<T> A
                                               const T* bn, T* x, T* r, T* z,
                                               const unsigned char* fixed, int x0_zero,
                                               double* part, unsigned int nblocks) {
-  __shared__ double lds[NT / kWave];
+  __shared__ double lds[3 * (NT / kWave)];
   const Tile t = tile_of(g, kchunk);
   double srz = 0.0, szz = 0.0, sbb = 0.0;
   if (t.valid) {
@@ -126,14 +126,13 @@ __global__ __launch_bounds__(NT) void k_init(Geo g, int kchunk, StencilView<T> A
       if (x0_zero && fixed != nullptr && !fx) x[c] = (T)0;
     }
   }
-  const double a = block_sum<NT>(srz, lds);
-  const double bsum = block_sum<NT>(szz, lds);
-  const double cs = block_sum<NT>(sbb, lds);
+  double v[3] = {srz, szz, sbb};
+  block_sums<NT, 3>(v, lds);
   const unsigned int bid = block_id();
   if (threadIdx.x == 0) {
-    part[bid] = a;
-    part[nblocks + bid] = bsum;
-    part[2 * nblocks + bid] = cs;
+    part[bid] = v[0];
+    part[nblocks + bid] = v[1];
+    part[2 * nblocks + bid] = v[2];
   }
 }
 
@@ -481,7 +480,7 @@ __global__ __launch_bounds__(NT) void k_cg_update(Geo g, int kchunk, const T* __
                                                    double* __restrict__ part_u, KrylovCtl* ctl,
                                                    const double* __restrict__ part_s, unsigned int nb_s, int it,
                                                    int xfresh) {
-  __shared__ double lds[NT / kWave];
+  __shared__ double lds[2 * (NT / kWave)];
   __shared__ int sflag;
   const TileId b = xcd_tile();
   double alpha;
@@ -518,12 +517,12 @@ __global__ __launch_bounds__(NT) void k_cg_update(Geo g, int kchunk, const T* __
       for (int k = k0; k < k1; ++k, c += g.sz) cell(xfresh ? (T)0 : x[c], p[c], r[c], q[c], D[c], x[c], r[c], z[c]);
     }
   }
-  const double a = block_sum<NT>(srz, lds);
-  const double bs = block_sum<NT>(szz, lds);
+  double v[2] = {srz, szz};
+  block_sums<NT, 2>(v, lds);
   if (threadIdx.x == 0) {
     const unsigned int nb = gridDim.x * gridDim.y * gridDim.z;
-    part_u[b.linear] = a;
-    part_u[nb + b.linear] = bs;
+    part_u[b.linear] = v[0];
+    part_u[nb + b.linear] = v[1];
   }
 }
 
@@ -633,7 +632,7 @@ __global__ __launch_bounds__(NT) void k_refine_pair(Geo g, int kchunk, StencilVi
                                                      const float* __restrict__ d, double* __restrict__ xout,
                                                      float* __restrict__ r32, const float* __restrict__ D32,
                                                      float* __restrict__ z32, double* part, int first) {
-  __shared__ double lds[NT / kWave];
+  __shared__ double lds[5 * (NT / kWave)];
   const TileId tb = xcd_tile();
   const int lane = threadIdx.x & 63;
   const int i0 = tb.x * 128, i = i0 + 2 * lane, j = tb.y * TY + (threadIdx.x >> 6);
@@ -719,21 +718,20 @@ __global__ __launch_bounds__(NT) void k_refine_pair(Geo g, int kchunk, StencilVi
       if (more) cur = nxt;
     }
   }
-  const double a = block_sum<NT>(srr, lds);
-  const double bs = block_sum<NT>(sbb, lds);
   const unsigned int bid = block_id(), nbl = gridDim.x * gridDim.y * gridDim.z;
-  if (threadIdx.x == 0) {
-    part[bid] = a;
-    part[nbl + bid] = bs;
-  }
   if (inner) {
-    const double x1 = block_sum<NT>(irz, lds);
-    const double x2 = block_sum<NT>(izz, lds);
-    const double x3 = block_sum<NT>(ibb, lds);
+    double v[5] = {srr, sbb, irz, izz, ibb};
+    block_sums<NT, 5>(v, lds);
     if (threadIdx.x == 0) {
-      part[2 * nbl + bid] = x1;
-      part[3 * nbl + bid] = x2;
-      part[4 * nbl + bid] = x3;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) part[q * nbl + bid] = v[q];
+    }
+  } else {
+    double v[2] = {srr, sbb};
+    block_sums<NT, 2>(v, lds);
+    if (threadIdx.x == 0) {
+      part[bid] = v[0];
+      part[nbl + bid] = v[1];
     }
   }
 }
